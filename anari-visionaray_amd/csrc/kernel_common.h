@@ -1,0 +1,100 @@
+// Device-side scene views and math shared by the gfx950 kernels.
+//
+// The host builds these from the vrt_* plain-old-data structs once per call;
+// they travel to the GPU as kernel arguments (no per-frame allocation, unlike
+// the reference's DevicePointer copies, DeviceArray.h:379-393).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vrt.h"
+
+namespace vrt {
+
+struct f3 {
+  float x, y, z;
+};
+__host__ __device__ inline f3 mk(float a, float b, float c) { return f3{a, b, c}; }
+__host__ __device__ inline f3 mk(float a) { return f3{a, a, a}; }
+__host__ __device__ inline f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__host__ __device__ inline f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__host__ __device__ inline f3 operator-(f3 a) { return {-a.x, -a.y, -a.z}; }
+__host__ __device__ inline f3 operator*(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__host__ __device__ inline f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__host__ __device__ inline float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ inline f3 cross(f3 a, f3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__host__ __device__ inline float length(f3 a) { return sqrtf(dot(a, a)); }
+__host__ __device__ inline f3 normalize(f3 a) { return a * (1.f / length(a)); }
+__host__ __device__ inline float max3(f3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
+__host__ __device__ inline f3 fabs3(f3 a) { return {fabsf(a.x), fabsf(a.y), fabsf(a.z)}; }
+__host__ __device__ inline float comp(f3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+
+// One volume instance, derived from vrt_volume on the host.
+struct KVolume {
+  int field_type;
+  int dims[3];
+  uint32_t bstride_y;  // bricks per z-slab row  * 512 (floats)
+  uint32_t bstride_z;  // bricks per z slab * 512
+  f3 blo, bhi;         // object-space bounds
+  f3 S, T;             // voxel space = S * (p + T)
+  float cell_size;
+  const float *bricks;
+  const void *vdb;
+  const float4 *tf;    // global copy of the table
+  int tf_size;
+  int tf_lds_offset;   // offset of this table in the block's LDS copy
+  float vlo, vhi;
+  float unit_distance;
+  const float *majorants;
+  int gdims[3];
+  float A[9];          // affineInv
+  f3 tinv;             // transInv
+};
+
+struct KLight {
+  int type;
+  int visible;
+  f3 dir;        // directional: unit vector towards the light
+  f3 intensity;  // color * irradiance / intensity
+  f3 v1, e1, e2; // quad
+};
+
+struct KScene {
+  int num_volumes;
+  int num_lights;
+  int tf_lds_total;  // float4 entries of all tables staged in LDS
+  KVolume vol[VRT_MAX_VOLUMES];
+  KLight light[VRT_MAX_LIGHTS];
+};
+
+struct KCamera {
+  f3 pos, U, V, W;
+  float region[4];
+  float shutter[2];
+};
+
+struct KParams {
+  float4 bg;
+  f3 ambient_color;
+  float ambient_radiance;
+  float occlusion_distance;
+  int ambient_samples;
+  int max_bounce;
+  int pixel_samples;
+  float sampling_rate_inv;
+};
+
+struct KFrame {
+  int width, height;
+  int color_type;
+  void *color;
+  float4 *accum;
+  float *depth;
+  double *accum64;
+  unsigned long long *counters;
+};
+
+}  // namespace vrt

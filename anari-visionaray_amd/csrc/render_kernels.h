@@ -1,0 +1,21 @@
+// Launchers of the gfx950 rendering kernels (render_kernels.hip).
+#pragma once
+
+#include "kernel_common.h"
+
+namespace vrt {
+
+hipError_t launch_raycast(const KScene &sc, const KCamera &cam, const KParams &P, const KFrame &F,
+                          hipStream_t s);
+hipError_t launch_pathtrace(const KScene &sc, const KCamera &cam, const KParams &P,
+                            const KFrame &F, int accumID, hipStream_t s);
+hipError_t launch_brickify(const float *lin, const int dims[3], float *bricks, hipStream_t s);
+hipError_t launch_value_ranges(const KVolume &v, float *ranges, hipStream_t s);
+hipError_t launch_majorants(const KVolume &v, const float *ranges, float *maj, hipStream_t s);
+hipError_t launch_frame_reset(const KFrame &F, hipStream_t s);
+hipError_t launch_sample_field(const KVolume &v, int n, const float *pts, float *out,
+                               hipStream_t s);
+hipError_t launch_delta_track(const KScene &sc, int n, const float *rays, uint32_t *rng,
+                              float *tout, hipStream_t s);
+
+}  // namespace vrt

@@ -1,0 +1,874 @@
+// gfx950 kernels of the per-pixel rendering path: camera ray generation,
+// brick-ordered trilinear fetch, TF classification from LDS, emission-
+// absorption ray marching (raycast), majorant-grid DDA + Woodcock delta
+// tracking and the path-tracer state machine (pathtrace), and frame
+// accumulation / tone mapping.
+//
+// Launch shape: 256-thread workgroups cover 16x16 pixel tiles; each 64-lane
+// wave owns an 8x8 sub-tile so the rays of a wave stay spatially coherent
+// (the reference launches 64x1 rows, for_each.h:272-280).  Workgroup ids are
+// remapped so each of the 8 XCDs (one L2 each) renders one contiguous band of
+// tiles and re-reads the same volume bricks from its own L2.
+#include "render_kernels.h"
+
+#include <float.h>
+
+namespace vrt {
+
+// ---------------------------------------------------------------------------
+// RNG: LCG<4> seeded by TEA-4 (common.h:55-102)
+// ---------------------------------------------------------------------------
+__device__ inline uint32_t tea4(uint32_t v0, uint32_t v1) {
+  uint32_t s0 = 0;
+#pragma unroll
+  for (int n = 0; n < 4; n++) {
+    s0 += 0x9e3779b9u;
+    v0 += ((v1 << 4) + 0xa341316cu) ^ (v1 + s0) ^ ((v1 >> 5) + 0xc8013ea4u);
+    v1 += ((v0 << 4) + 0xad90777du) ^ (v0 + s0) ^ ((v0 >> 5) + 0x7e95761eu);
+  }
+  return v0;
+}
+__device__ inline float rnd(uint32_t &s) {
+  s = 1664525u * s + 1013904223u;
+  return float(s & 0x00FFFFFFu) * (1.0f / 16777216.0f);
+}
+
+// ---------------------------------------------------------------------------
+// XCD-aware tile mapping
+// ---------------------------------------------------------------------------
+__device__ inline int xcd_remap(int b, int n) {
+  // hardware dispatches workgroup b to XCD b % 8; hand XCD k the k-th
+  // contiguous share of the logical tile sequence
+  const int q = n >> 3, r = n & 7;
+  const int xcd = b & 7, idx = b >> 3;
+  return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+}
+
+__device__ inline void tile_pixel(int W, int &px, int &py) {
+  const int ntx = (W + 15) >> 4;
+  const int t = xcd_remap(int(blockIdx.x), int(gridDim.x));
+  const int tx = t % ntx, ty = t / ntx;
+  const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
+  px = (tx << 4) + ((wave & 1) << 3) + (lane & 7);
+  py = (ty << 4) + ((wave >> 1) << 3) + (lane >> 3);
+}
+
+// ---------------------------------------------------------------------------
+// Rays / boxes
+// ---------------------------------------------------------------------------
+struct KRay {
+  f3 ori, dir;
+  float tmin, tmax;
+};
+
+struct BoxHit {
+  bool hit;
+  float tnear, tfar;
+};
+
+__device__ inline BoxHit intersect_box(const KRay &r, f3 lo, f3 hi) {
+  f3 inv = mk(1.f / r.dir.x, 1.f / r.dir.y, 1.f / r.dir.z);
+  f3 t1 = (lo - r.ori) * inv;
+  f3 t2 = (hi - r.ori) * inv;
+  float tn = fmaxf(fminf(t1.z, t2.z), fmaxf(fminf(t1.y, t2.y), fminf(t1.x, t2.x)));
+  float tf = fminf(fmaxf(t1.z, t2.z), fminf(fmaxf(t1.y, t2.y), fmaxf(t1.x, t2.x)));
+  return BoxHit{tf >= tn && tf >= 0.f, tn, tf};
+}
+
+// renderer/common.h:14-18
+__device__ inline float epsilon_from(f3 P, f3 dir, float t) {
+  return fmaxf(max3(fabs3(P)), max3(fabs3(dir)) * t) * 0x1.fp-18f;
+}
+
+__device__ inline f3 mulA(const float *A, f3 p) {
+  return mk(A[0] * p.x + A[1] * p.y + A[2] * p.z, A[3] * p.x + A[4] * p.y + A[5] * p.z,
+            A[6] * p.x + A[7] * p.y + A[8] * p.z);
+}
+
+__device__ inline KRay to_local(const KVolume &v, const KRay &r) {
+  KRay l = r;
+  l.ori = mulA(v.A, r.ori + v.tinv);
+  l.dir = mulA(v.A, r.dir);
+  return l;
+}
+
+// ---------------------------------------------------------------------------
+// Field fetch: trilinear over 8^3 bricks, texel-centre convention (p*N-0.5),
+// clamp addressing.  The 8 taps are formed from per-axis partial offsets.
+// ---------------------------------------------------------------------------
+__device__ inline void axis_taps(float p, int n, uint32_t stride_brick, int lshift,
+                                 uint32_t &o0, uint32_t &o1, float &w) {
+  float c = p * float(n) - 0.5f;
+  c = fminf(fmaxf(c, -1.f), float(n));
+  float fl = floorf(c);
+  w = c - fl;
+  int i = int(fl);
+  int i0 = min(max(i, 0), n - 1);
+  int i1 = min(max(i + 1, 0), n - 1);
+  o0 = uint32_t(i0 >> 3) * stride_brick + (uint32_t(i0 & 7) << lshift);
+  o1 = uint32_t(i1 >> 3) * stride_brick + (uint32_t(i1 & 7) << lshift);
+}
+
+__device__ inline float lerpf(float a, float b, float t) { return a + t * (b - a); }
+
+// x^y from IEEE basic operations only (no library transcendental), so the
+// opacity correction powf(1-alpha, dt/unitDistance) of
+// VolumeIntegration.h:416-417 rounds identically on every IEEE-754 machine
+// that evaluates it without contraction: x = m*2^e with m in [sqrt.5, sqrt2),
+// ln m = 2s(1 + s^2/3 + s^4/5 + s^6/7 + s^8/9) with s = (m-1)/(m+1),
+// 2^z = 2^n * exp(f ln2) with a degree-7 Taylor polynomial, |f| <= 1/2.
+// Error <= 1.5 (2 + |y log2 x|) ulp for x in [0, 2^127), y finite.
+__device__ inline float pow_ieee(float x, float y) {
+  if (y == 0.f || x == 1.f) return 1.f;
+  if (!(x > 0.f)) return y > 0.f ? 0.f : __builtin_inff();
+  const uint32_t bits = __float_as_uint(x);
+  int e = int((bits >> 23) & 255u);
+  if (e == 0) return y > 0.f ? 0.f : __builtin_inff();  // subnormal base
+  if (e == 255) return x;                               // inf / nan
+  e -= 127;
+  float m = __uint_as_float((bits & 0x007FFFFFu) | 0x3F800000u);
+  if (m > 1.41421356f) {
+    m = m * 0.5f;
+    e += 1;
+  }
+  const float s = (m - 1.f) / (m + 1.f);
+  const float s2 = s * s;
+  float p = 0.111111111f;
+  p = p * s2 + 0.142857143f;
+  p = p * s2 + 0.2f;
+  p = p * s2 + 0.333333333f;
+  p = p * s2 + 1.f;
+  const float l2 = float(e) + (2.f * s * p) * 1.44269504f;
+  const float z = y * l2;
+  if (z < -150.f) return 0.f;
+  if (z > 128.f) return __builtin_inff();
+  const float n = floorf(z + 0.5f);
+  const float t = (z - n) * 0.693147181f;
+  float q = 1.98412698e-4f;
+  q = q * t + 1.38888889e-3f;
+  q = q * t + 8.33333333e-3f;
+  q = q * t + 4.16666667e-2f;
+  q = q * t + 0.166666667f;
+  q = q * t + 0.5f;
+  q = q * t + 1.f;
+  q = q * t + 1.f;
+  int ni = int(n);
+  if (ni < -126) {  // subnormal result: scale in two steps
+    q = q * 5.87747175e-39f;  // 2^-127
+    ni += 127;
+  }
+  return q * __uint_as_float(uint32_t(ni + 127) << 23);
+}
+
+__device__ inline float sample_structured(const KVolume &v, f3 P) {
+  uint32_t x0, x1, y0, y1, z0, z1;
+  float wx, wy, wz;
+  axis_taps(P.x, v.dims[0], 512u, 0, x0, x1, wx);
+  axis_taps(P.y, v.dims[1], v.bstride_y, 3, y0, y1, wy);
+  axis_taps(P.z, v.dims[2], v.bstride_z, 6, z0, z1, wz);
+  const float *b = v.bricks;
+  const float v000 = b[x0 + y0 + z0], v100 = b[x1 + y0 + z0];
+  const float v010 = b[x0 + y1 + z0], v110 = b[x1 + y1 + z0];
+  const float v001 = b[x0 + y0 + z1], v101 = b[x1 + y0 + z1];
+  const float v011 = b[x0 + y1 + z1], v111 = b[x1 + y1 + z1];
+  float c00 = lerpf(v000, v100, wx), c10 = lerpf(v010, v110, wx);
+  float c01 = lerpf(v001, v101, wx), c11 = lerpf(v011, v111, wx);
+  return lerpf(lerpf(c00, c10, wy), lerpf(c01, c11, wy), wz);
+}
+
+// postClassify (DeviceCopyableObjects.h:615-619): tex1D(linear, clamp) of the
+// TF table, read from the workgroup's LDS copy
+__device__ inline float4 classify(const KVolume &v, const float4 *lds, float val) {
+  float u = (val - v.vlo) / (v.vhi - v.vlo);
+  const int n = v.tf_size;
+  float c = u * float(n) - 0.5f;
+  c = fminf(fmaxf(c, -1.f), float(n));
+  float fl = floorf(c);
+  float w = c - fl;
+  int i = int(fl);
+  int i0 = min(max(i, 0), n - 1), i1 = min(max(i + 1, 0), n - 1);
+  const float4 a = lds[v.tf_lds_offset + i0], b = lds[v.tf_lds_offset + i1];
+  return make_float4(lerpf(a.x, b.x, w), lerpf(a.y, b.y, w), lerpf(a.z, b.z, w),
+                     lerpf(a.w, b.w, w));
+}
+
+__device__ inline void stage_tf(const KScene &sc, float4 *lds) {
+  for (int vi = 0; vi < sc.num_volumes; ++vi) {
+    const KVolume &v = sc.vol[vi];
+    for (int i = threadIdx.x; i < v.tf_size; i += blockDim.x) lds[v.tf_lds_offset + i] = v.tf[i];
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Camera (ANARI perspective; see vrt_camera)
+// ---------------------------------------------------------------------------
+__device__ inline KRay primary_ray(const KCamera &c, float x, float y, float W, float H) {
+  float sx = (x + 0.5f) / W, sy = (y + 0.5f) / H;
+  sx = (1.f - sx) * c.region[0] + sx * c.region[2];
+  sy = (1.f - sy) * c.region[1] + sy * c.region[3];
+  f3 d = c.W + c.U * (2.f * sx - 1.f) + c.V * (2.f * sy - 1.f);
+  KRay r;
+  r.ori = c.pos;
+  r.dir = normalize(d);
+  r.tmin = 0.f;
+  r.tmax = FLT_MAX;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Woodcock / delta tracking through one volume (DeviceCopyableObjects.h:
+// 680-796) with the macrocell DDA of renderer/DDA.h:14-112.
+// ---------------------------------------------------------------------------
+struct VolHit {
+  float t;
+  f3 albedo;
+  bool hit;
+};
+
+// One cell's collision loop.  Returns true on a real collision.
+__device__ inline bool woodcock_cell(const KVolume &v, const float4 *lds, float majorant,
+                                     float invUD, f3 ori, f3 dir, float t0, float t1,
+                                     uint32_t &rs, float &t_hit, f3 &albedo, uint32_t &lookups) {
+  if (majorant <= 0.f) return false;
+  if (!(t0 < t1)) {
+    // the first free-flight step already leaves the cell (-log(1-u) >= 0):
+    // consume the draw, skip the logarithm
+    rs = 1664525u * rs + 1013904223u;
+    return false;
+  }
+  const float scale = 1.f / (majorant * invUD);
+  float t = t0;
+  while (true) {
+    const float u0 = rnd(rs);
+    t -= __logf(1.f - u0) * scale;
+    if (!(t < t1)) return false;  // also leaves on NaN
+    f3 P = ori + dir * t;
+    float val = sample_structured(v, P);
+    ++lookups;
+    float4 s = classify(v, lds, val);
+    float u = rnd(rs);
+    if (s.w >= u * majorant) {
+      t_hit = t;
+      albedo = mk(s.x, s.y, s.z);
+      return true;
+    }
+  }
+}
+
+__device__ inline VolHit delta_track(const KVolume &v, const float4 *lds, KRay ray, uint32_t &rs,
+                                     uint32_t &lookups) {
+  VolHit out;
+  out.hit = false;
+  out.t = FLT_MAX;
+  BoxHit bh = intersect_box(ray, v.blo, v.bhi);
+  if (!bh.hit) return out;
+  ray.tmin = fmaxf(ray.tmin, bh.tnear);
+  ray.tmax = fminf(ray.tmax, bh.tfar);
+  f3 ori = v.S * (ray.ori + v.T);
+  f3 dir = v.S * ray.dir;
+  const float dt_scale = length(dir);
+  dir = normalize(dir);
+  const float tmin = ray.tmin * dt_scale;
+  const float tmax = ray.tmax * dt_scale;
+  const float invUD = 1.f / (v.unit_distance * dt_scale);
+
+  float t_hit = 0.f;
+  f3 albedo = mk(0.f);
+  bool hit = false;
+
+  if (v.gdims[0] == 0) {
+    hit = woodcock_cell(v, lds, 1.f, invUD, ori, dir, tmin, tmax, rs, t_hit, albedo, lookups);
+  } else {
+    // dda3 with the ray moved so that tmin becomes 0 (DDA.h:21-25)
+    const f3 o = ori + dir * tmin;
+    const float tm = tmax - tmin;
+    const f3 rcp = mk(1.f / dir.x, 1.f / dir.y, 1.f / dir.z);
+    // grid bounds = voxel space [0,1]^3
+    const f3 l = (mk(0.f) - o) * rcp;
+    const f3 h = (mk(1.f) - o) * rcp;
+    int cell[3], step[3], stop[3];
+    float dist[3], tnext[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int G = v.gdims[a];
+      const float oa = comp(o, a), da = comp(dir, a);
+      float c = oa * float(G);  // projectOnGrid over [0,1]
+      c = fminf(fmaxf(c, -1.f), float(G));
+      cell[a] = min(max(int(c), 0), G - 1);
+      step[a] = da > 0.f ? 1 : -1;
+      stop[a] = da > 0.f ? G : -1;
+      const float la = comp(l, a), ha = comp(h, a);
+      const float tn = fminf(la, ha), tf = fmaxf(la, ha);
+      if (da == 0.f) {
+        dist[a] = 0.f;
+        tnext[a] = __builtin_inff();
+      } else {
+        dist[a] = fmaxf(0.f, (tf - tn) / float(G));
+        tnext[a] = da > 0.f ? tn + float(cell[a] + 1) * dist[a] : tn + float(G - cell[a]) * dist[a];
+      }
+    }
+    float t0 = 0.f;
+    // a ray visits at most gx+gy+gz cells; the cap only guards degenerate
+    // (non-finite) rays
+    const int max_cells = v.gdims[0] + v.gdims[1] + v.gdims[2] + 3;
+    for (int it = 0; it < max_cells; ++it) {
+      const float tc = fminf(tnext[0], fminf(tnext[1], tnext[2]));
+      const float t1 = fminf(tc, tm);
+      const int li = (cell[2] * v.gdims[1] + cell[1]) * v.gdims[0] + cell[0];
+      const float maj = v.majorants[li];
+      if (woodcock_cell(v, lds, maj, invUD, ori, dir, tmin + t0, tmin + t1, rs, t_hit, albedo,
+                        lookups)) {
+        hit = true;
+        break;
+      }
+      bool out_of_grid = false;
+      if (tnext[0] == tc) {
+        tnext[0] += dist[0];
+        cell[0] += step[0];
+        out_of_grid = cell[0] == stop[0];
+      }
+      if (!out_of_grid && tnext[1] == tc) {
+        tnext[1] += dist[1];
+        cell[1] += step[1];
+        out_of_grid = cell[1] == stop[1];
+      }
+      if (!out_of_grid && tnext[2] == tc) {
+        tnext[2] += dist[2];
+        cell[2] += step[2];
+        out_of_grid = cell[2] == stop[2];
+      }
+      if (out_of_grid) break;
+      t0 = t1;
+    }
+  }
+  if (hit) {
+    out.hit = true;
+    out.t = t_hit / dt_scale;
+    out.albedo = albedo;
+  }
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// Sampling helpers
+// ---------------------------------------------------------------------------
+__device__ inline f3 uniform_sample_sphere(float u1, float u2) {
+  float z = 1.f - 2.f * u1;
+  float r = sqrtf(fmaxf(0.f, 1.f - z * z));
+  float s, c;
+  sincosf(6.28318530717958647692f * u2, &s, &c);
+  return mk(r * c, r * s, z);
+}
+__device__ inline f3 cosine_sample_hemisphere(float u1, float u2) {
+  float sx = 2.f * u1 - 1.f, sy = 2.f * u2 - 1.f;
+  float dx = 0.f, dy = 0.f;
+  if (!(sx == 0.f && sy == 0.f)) {
+    float r, theta;
+    if (fabsf(sx) > fabsf(sy)) {
+      r = sx;
+      theta = 0.78539816339744830962f * (sy / sx);
+    } else {
+      r = sy;
+      theta = 1.57079632679489661923f - 0.78539816339744830962f * (sx / sy);
+    }
+    float s, c;
+    sincosf(theta, &s, &c);
+    dx = r * c;
+    dy = r * s;
+  }
+  return mk(dx, dy, sqrtf(fmaxf(0.f, 1.f - dx * dx - dy * dy)));
+}
+__device__ inline f3 faceforward(f3 n, f3 i, f3 nref) { return dot(i, nref) < 0.f ? -n : n; }
+
+// ---------------------------------------------------------------------------
+// Frame write: Frame::writeSample (DeviceCopyableObjects.h:2776-2781)
+// ---------------------------------------------------------------------------
+__device__ inline uint32_t cvt8(float f) { return uint32_t(255.f * fminf(fmaxf(f, 0.f), 1.f)); }
+__device__ inline float linear_to_srgb(float x) {
+  return x <= 0.0031308f ? 12.92f * x : 1.055f * powf(x, 1.f / 2.4f) - 0.055f;
+}
+
+__device__ inline void write_sample(const KFrame &F, int x, int y, int accumID, float4 c,
+                                    float depth, bool stochastic) {
+  const size_t idx = size_t(y) * F.width + x;
+  if (F.depth && depth <= F.depth[idx]) F.depth[idx] = depth;
+  if (F.accum64) {
+    double *a = F.accum64 + 4 * idx;
+    a[0] += double(c.x);
+    a[1] += double(c.y);
+    a[2] += double(c.z);
+    a[3] += double(c.w);
+  }
+  if (stochastic && F.accum) {
+    // running mean, alpha = 1/(accumID+1) (DeviceCopyableObjects.h:2713-2716)
+    const float alpha = 1.f / float(accumID + 1);
+    const float4 prev = F.accum[idx];
+    c = make_float4((1.f - alpha) * prev.x + alpha * c.x, (1.f - alpha) * prev.y + alpha * c.y,
+                    (1.f - alpha) * prev.z + alpha * c.z, (1.f - alpha) * prev.w + alpha * c.w);
+    F.accum[idx] = c;
+  }
+  // toneMap (DeviceCopyableObjects.h:2720-2739), cvt_uint32 (frame/common.h:31-47)
+  if (F.color_type == VRT_TYPE_FLOAT32_VEC4) {
+    reinterpret_cast<float4 *>(F.color)[idx] = c;
+  } else if (F.color_type == VRT_TYPE_UFIXED8_VEC4) {
+    reinterpret_cast<uint32_t *>(F.color)[idx] =
+        cvt8(c.x) | (cvt8(c.y) << 8) | (cvt8(c.z) << 16) | (cvt8(c.w) << 24);
+  } else if (F.color_type == VRT_TYPE_UFIXED8_RGBA_SRGB) {
+    reinterpret_cast<uint32_t *>(F.color)[idx] = cvt8(linear_to_srgb(c.x)) |
+                                                 (cvt8(linear_to_srgb(c.y)) << 8) |
+                                                 (cvt8(linear_to_srgb(c.z)) << 16) |
+                                                 (cvt8(c.w) << 24);
+  }
+}
+
+__device__ inline void wave_add_counter(unsigned long long *ctr, uint32_t v) {
+  // 64-lane butterfly, then one atomic per wave
+  unsigned long long s = v;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0 && s) atomicAdd(ctr, s);
+}
+
+// ---------------------------------------------------------------------------
+// Raycast: Raycast_impl.cpp:245-322 + rayMarchVolume<0>
+// (VolumeIntegration.h:374-470)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+    k_raycast(KScene sc, KCamera cam, KParams P, KFrame F) {
+  extern __shared__ float4 s_tf[];
+  stage_tf(sc, s_tf);
+  int x, y;
+  tile_pixel(F.width, x, y);
+  const bool active = x < F.width && y < F.height;
+  uint32_t lookups = 0;
+  if (active) {
+    float4 result = P.bg;
+    if (sc.num_volumes > 0) {
+      KRay ray = primary_ray(cam, float(x), float(y), float(F.width), float(F.height));
+      // closest volume box (intersectVolumeBounds, DeviceCopyableObjects.h:1841-1859)
+      int best = -1;
+      float bestT = FLT_MAX;
+      for (int i = 0; i < sc.num_volumes; ++i) {
+        KRay lr = to_local(sc.vol[i], ray);
+        BoxHit bh = intersect_box(lr, sc.vol[i].blo, sc.vol[i].bhi);
+        if (!bh.hit) continue;
+        float t = fmaxf(lr.tmin, bh.tnear);
+        if (t < bestT) {
+          bestT = t;
+          best = i;
+        }
+      }
+      if (best >= 0) {
+        const KVolume &v = sc.vol[best];
+        KRay lr = to_local(v, ray);
+        BoxHit bh = intersect_box(lr, v.blo, v.bhi);
+        const float tmin = fmaxf(lr.tmin, bh.tnear);
+        const float tmax = fminf(lr.tmax, bh.tfar);
+        const f3 ori = v.S * (lr.ori + v.T);
+        const f3 dir = v.S * lr.dir;  // not renormalised (VolumeIntegration.h:395-396)
+        const float dt = v.cell_size * P.sampling_rate_inv;
+        const float expo = dt / v.unit_distance;
+        float cr = 0.f, cg = 0.f, cb = 0.f, alpha = 0.f, trans = 1.f;
+        // steps needed to cross [tmin, tmax] plus slack; a zero or non-finite
+        // step size marches nothing (the reference would not terminate)
+        const float span = (tmax - tmin) / dt;
+        const int max_steps =
+            (dt > 0.f && span < 16777216.f) ? int(fmaxf(span, 0.f) * 1.001f) + 4 : 0;
+        int step = 0;
+        for (float t = tmin; t <= tmax && alpha < 0.99f && step < max_steps; t += dt, ++step) {
+          const f3 p = ori + dir * t;
+          const float val = sample_structured(v, p);
+          ++lookups;
+          const float4 s = classify(v, s_tf, val);
+          const float stepT = pow_ieee(1.f - s.w, expo);
+          const float k = trans * (1.f - stepT);
+          cr += k * s.x;
+          cg += k * s.y;
+          cb += k * s.z;
+          alpha += k;
+          trans *= stepT;
+        }
+        // over(alpha*float4(color,1), bg) (Raycast_impl.cpp:232)
+        const float a = 1.f - alpha;
+        result = make_float4(alpha * cr + a * result.x, alpha * cg + a * result.y,
+                             alpha * cb + a * result.z, alpha + a * result.w);
+      }
+    }
+    write_sample(F, x, y, 0, result, 1e31f, false);
+  }
+  if (F.counters) wave_add_counter(&F.counters[0], lookups);
+}
+
+// ---------------------------------------------------------------------------
+// Path tracer: Pathtrace_impl.cpp:489-641, volume vertices with directional
+// light NEE and the ambient-occlusion term, Russian roulette drawing from the
+// un-advanced pixel RNG (Pathtrace_impl.cpp:594-599).
+// ---------------------------------------------------------------------------
+enum RayType { Radiance = 0, Shadow = 1, AO = 2, Miss = 3 };
+
+__global__ void __launch_bounds__(256)
+    k_pathtrace(KScene sc, KCamera cam, KParams P, KFrame F, int accumID) {
+  extern __shared__ float4 s_tf[];
+  stage_tf(sc, s_tf);
+  int x, y;
+  tile_pixel(F.width, x, y);
+  const bool active = x < F.width && y < F.height;
+  uint32_t lookups = 0;
+  if (active) {
+    const int W = F.width, H = F.height;
+    const uint32_t pixelID = uint32_t(x + W * y);
+    uint32_t rng = tea4(pixelID, uint32_t(accumID));
+    uint32_t srng = rng;  // ScreenSample's copy (Pathtrace_impl.cpp:512-513)
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    float depth = 1e31f;
+    const int spp = P.pixel_samples;
+    for (int sampleID = 0; sampleID < spp; ++sampleID) {
+      float xf = float(x), yf = float(y);
+      const float jx = rnd(srng) - .5f;
+      const float jy = rnd(srng) - .5f;
+      xf += jx;
+      yf += jy;
+      KRay ray = primary_ray(cam, xf, yf, float(W), float(H));
+      (void)rnd(srng);  // shutter time (DeviceCopyableObjects.h:2597)
+      float4 color = P.bg;
+      float sdepth = 1e31f;
+      if (sc.num_volumes > 0) {
+        // ShadeState (Pathtrace_impl.cpp:15-35)
+        f3 baseColor = mk(0.f), shadedColor = mk(0.f), hitPosOff = mk(0.f), sn = mk(0.f);
+        int aoSamples = 0;
+        float aoWeights = 0.f, aoCount = 0.f;
+        f3 bs_dir = mk(0.f), bs_f = mk(0.f);
+        float bs_pdf = 0.f, bs_cosT = 0.f;
+        f3 ls_dir = mk(0.f), ls_int = mk(0.f);
+        float ls_pdf = 0.f, ls_dist = 0.f, ls_dist2 = 0.f;
+        float misWeightNEE = 0.f, vis_light = 1.f, vis_ao = 1.f;
+        int rayType = Radiance;
+        f3 throughput = mk(1.f), intensity = mk(0.f);
+        bool firstHit = false;
+        for (unsigned passID = 0, bounceID = 0;; ++passID) {
+          // intersectAll: volumes only in this scene class
+          VolHit vh;
+          vh.hit = false;
+          vh.t = FLT_MAX;
+          for (int i = 0; i < sc.num_volumes; ++i) {
+            VolHit h = delta_track(sc.vol[i], s_tf, to_local(sc.vol[i], ray), srng, lookups);
+            if (h.hit && h.t < vh.t) vh = h;
+          }
+          const bool hit = vh.hit;
+          if (passID == 0 && bounceID == 0) {
+            firstHit = hit;
+            if (hit) sdepth = vh.t;
+          }
+          // shade (Pathtrace_impl.cpp:93-487)
+          int next = Miss;
+          KRay nray = ray;
+          if (rayType == Radiance) {
+            baseColor = mk(0.f);
+            vis_light = 1.f;
+            vis_ao = 1.f;
+            aoSamples = 0;
+            aoWeights = 0.f;
+            aoCount = 0.f;
+            if (!hit) {
+              shadedColor = mk(0.f);
+            } else {
+              const f3 hitPos = ray.ori + ray.dir * vh.t;
+              const float eps = epsilon_from(hitPos, ray.dir, vh.t);
+              const f3 viewDir = -normalize(ray.dir);
+              f3 gn = mk(0.f);
+              if (P.ambient_samples > 0) {
+                const float u1 = rnd(srng), u2 = rnd(srng);
+                gn = uniform_sample_sphere(u1, u2);
+              }
+              sn = faceforward(gn, viewDir, gn);
+              hitPosOff = hitPos + sn * eps;
+              if (sc.num_lights > 0) {
+                int which = int(rnd(srng) * float(sc.num_lights));
+                if (which == sc.num_lights) which = 0;
+                const KLight &L = sc.light[which];
+                ls_dir = L.dir;
+                ls_int = L.intensity;
+                ls_pdf = 1.f;
+                ls_dist = FLT_MAX;
+                ls_dist2 = 1.f;
+              }
+              const float rcp_pdf = ls_pdf > 0.f ? 1.f / ls_pdf : 0.f;
+              const float rcp_d2 = ls_dist2 > 0.f ? 1.f / ls_dist2 : 0.f;
+              shadedColor = vh.albedo * (ls_int * rcp_d2) * rcp_pdf * rcp_d2;
+              const float u1 = rnd(srng), u2 = rnd(srng);
+              bs_dir = uniform_sample_sphere(u1, u2);
+              bs_f = vh.albedo;
+              bs_pdf = 1.f;
+              bs_cosT = 1.f;
+              misWeightNEE = sc.num_lights > 0 ? 1.f : 0.f;
+              baseColor = vh.albedo;
+              if (ls_pdf >= 0.f) {
+                nray.ori = hitPosOff;
+                nray.dir = normalize(ls_dir);
+                nray.tmin = 0.f;
+                nray.tmax = ls_dist;
+                next = Shadow;
+              }
+            }
+          } else if (rayType == Shadow) {
+            vis_light = hit ? 0.f : 1.f;
+            next = Radiance;
+          } else {  // AO
+            aoSamples++;
+            const float weight = fmaxf(0.f, dot(ray.dir, sn));
+            aoWeights += weight;
+            if (weight > 0.f && hit) aoCount += weight;
+            if (aoSamples < P.ambient_samples) {
+              next = AO;
+            } else {
+              const float aoV = aoWeights > 0.f ? 1.f - aoCount / aoWeights : 0.f;
+              vis_ao *= aoV;
+              next = Radiance;
+            }
+          }
+          // after the shadow pass (or an AO pass with samples left) an AO ray
+          // is traced if ambient samples remain (Pathtrace_impl.cpp:437-455)
+          if ((rayType == Shadow && aoSamples < P.ambient_samples) ||
+              (rayType == AO && next == AO)) {
+            f3 w = sn;
+            f3 v = fabsf(w.x) > fabsf(w.y) ? normalize(mk(-w.z, 0.f, w.x))
+                                           : normalize(mk(0.f, w.z, -w.y));
+            f3 u = cross(v, w);
+            const float u1 = rnd(srng), u2 = rnd(srng);
+            const f3 sp = cosine_sample_hemisphere(u1, u2);
+            nray.ori = hitPosOff;
+            nray.dir = normalize(u * sp.x + v * sp.y + w * sp.z);
+            nray.tmin = 0.f;
+            nray.tmax = P.occlusion_distance;
+            next = AO;
+          } else if (next == Radiance) {
+            nray.ori = hitPosOff;
+            nray.dir = normalize(bs_dir);
+            nray.tmin = 0.f;
+            nray.tmax = 1e31f;
+          }
+          ray = nray;
+          rayType = next;
+          if (rayType == Miss || rayType == Radiance) {
+            const f3 direct = shadedColor * vis_light;
+            const f3 ambient = baseColor * P.ambient_color * P.ambient_radiance * vis_ao;
+            intensity = intensity + throughput * (direct * misWeightNEE);
+            intensity = intensity + throughput * ambient;
+            const float rcp = bs_pdf > 0.f ? 1.f / bs_pdf : 0.f;
+            throughput = throughput * (bs_f * (bs_cosT * rcp));
+            bounceID++;
+          }
+          if (rayType == Miss || int(bounceID) > P.max_bounce) break;
+          if (bounceID > 1) {
+            const float prob = max3(throughput);
+            if (rnd(rng) > prob) break;
+            throughput = throughput * (1.f / prob);
+          }
+        }
+        if (firstHit) color = make_float4(intensity.x, intensity.y, intensity.z, 1.f);
+      }
+      acc.x += color.x;
+      acc.y += color.y;
+      acc.z += color.z;
+      acc.w += color.w;
+      if (sdepth < depth) depth = sdepth;
+    }
+    const float inv = 1.f / float(spp);
+    acc = make_float4(acc.x * inv, acc.y * inv, acc.z * inv, acc.w * inv);
+    write_sample(F, x, y, accumID, acc, depth, true);
+  }
+  if (F.counters) wave_add_counter(&F.counters[0], lookups);
+}
+
+// ---------------------------------------------------------------------------
+// Field preparation
+// ---------------------------------------------------------------------------
+__global__ void k_brickify(const float *__restrict__ lin, int nx, int ny, int nz, int nbx,
+                           int nby, int nbz, float *__restrict__ bricks) {
+  // one thread per brick-ordered output element: coalesced writes
+  const size_t total = size_t(nbx) * nby * nbz * 512;
+  for (size_t o = size_t(blockIdx.x) * blockDim.x + threadIdx.x; o < total;
+       o += size_t(gridDim.x) * blockDim.x) {
+    const size_t b = o >> 9;
+    const int l = int(o & 511);
+    const int lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
+    const int bx = int(b % nbx), by = int((b / nbx) % nby), bz = int(b / (size_t(nbx) * nby));
+    // pad with edge values (clamp addressing)
+    const int x = min(bx * 8 + lx, nx - 1), y = min(by * 8 + ly, ny - 1),
+              z = min(bz * 8 + lz, nz - 1);
+    bricks[o] = lin[(size_t(z) * ny + y) * nx + x];
+  }
+}
+
+// Value range of each macrocell: min/max over every voxel a trilinear sample
+// inside the cell can read (see vrt_build_value_ranges).  One workgroup per
+// macrocell.
+__global__ void __launch_bounds__(256)
+    k_value_ranges(KVolume v, float2 *__restrict__ ranges) {
+  const int G0 = v.gdims[0], G1 = v.gdims[1], G2 = v.gdims[2];
+  const int mc = blockIdx.x;
+  const int m[3] = {mc % G0, (mc / G0) % G1, mc / (G0 * G1)};
+  const int G[3] = {G0, G1, G2};
+  int lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    const double c0 = double(m[a]) * v.dims[a] / G[a] - 0.5;
+    const double c1 = double(m[a] + 1) * v.dims[a] / G[a] - 0.5;
+    lo[a] = max(0, int(floor(c0)));
+    hi[a] = min(v.dims[a] - 1, int(floor(c1)) + 1);
+  }
+  const int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
+  const int n = ex * ey * ez;
+  float vmin = FLT_MAX, vmax = -FLT_MAX;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int x = lo[0] + i % ex, y = lo[1] + (i / ex) % ey, z = lo[2] + i / (ex * ey);
+    const uint32_t o = uint32_t(x >> 3) * 512u + uint32_t(x & 7) +
+                       uint32_t(y >> 3) * v.bstride_y + (uint32_t(y & 7) << 3) +
+                       uint32_t(z >> 3) * v.bstride_z + (uint32_t(z & 7) << 6);
+    const float val = v.bricks[o];
+    vmin = fminf(vmin, val);
+    vmax = fmaxf(vmax, val);
+  }
+  __shared__ float smin[256], smax[256];
+  smin[threadIdx.x] = vmin;
+  smax[threadIdx.x] = vmax;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (int(threadIdx.x) < s) {
+      smin[threadIdx.x] = fminf(smin[threadIdx.x], smin[threadIdx.x + s]);
+      smax[threadIdx.x] = fmaxf(smax[threadIdx.x], smax[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) ranges[mc] = make_float2(smin[0], smax[0]);
+}
+
+// GridAccel::computeMaxOpacities (GridAccel.cpp:69-107) with the TF taps
+// that tex1D can actually read for values in [vmin, vmax]
+__global__ void k_majorants(KVolume v, const float2 *__restrict__ ranges, int nmc,
+                            float *__restrict__ maj) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nmc) return;
+  const float2 r = ranges[i];
+  if (r.y < r.x) {
+    maj[i] = 0.f;
+    return;
+  }
+  const int n = v.tf_size;
+  const float u0 = (r.x - v.vlo) / (v.vhi - v.vlo);
+  const float u1 = (r.y - v.vlo) / (v.vhi - v.vlo);
+  const float c0 = fminf(fmaxf(u0 * float(n) - 0.5f, -1.f), float(n));
+  const float c1 = fminf(fmaxf(u1 * float(n) - 0.5f, -1.f), float(n));
+  const int lo = min(max(int(floorf(c0)), 0), n - 1);
+  const int hi = min(max(int(floorf(c1)) + 1, 0), n - 1);
+  float m = 0.f;
+  for (int k = lo; k <= hi; ++k) m = fmaxf(m, v.tf[k].w);
+  maj[i] = m;
+}
+
+__global__ void k_sample_field(KVolume v, int n, const float *__restrict__ pts,
+                               float *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = sample_structured(v, mk(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
+}
+
+__global__ void __launch_bounds__(256)
+    k_delta_track(KScene sc, int n, const float *__restrict__ rays, uint32_t *__restrict__ rng,
+                  float *__restrict__ tout) {
+  extern __shared__ float4 s_tf[];
+  stage_tf(sc, s_tf);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float *r = rays + 8 * i;
+  KRay ray;
+  ray.ori = mk(r[0], r[1], r[2]);
+  ray.dir = mk(r[3], r[4], r[5]);
+  ray.tmin = r[6];
+  ray.tmax = r[7];
+  uint32_t s = rng[i];
+  uint32_t lookups = 0;
+  VolHit h = delta_track(sc.vol[0], s_tf, to_local(sc.vol[0], ray), s, lookups);
+  rng[i] = s;
+  tout[i] = h.hit ? h.t : __builtin_inff();
+}
+
+__global__ void k_frame_reset(KFrame F) {
+  const size_t n = size_t(F.width) * F.height;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x) {
+    if (F.accum) F.accum[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (F.depth) F.depth[i] = 1e31f;
+    if (F.accum64) {
+      F.accum64[4 * i] = 0.0;
+      F.accum64[4 * i + 1] = 0.0;
+      F.accum64[4 * i + 2] = 0.0;
+      F.accum64[4 * i + 3] = 0.0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline int tiles(const KFrame &F) { return ((F.width + 15) / 16) * ((F.height + 15) / 16); }
+
+hipError_t launch_raycast(const KScene &sc, const KCamera &cam, const KParams &P, const KFrame &F,
+                          hipStream_t s) {
+  const size_t lds = size_t(sc.tf_lds_total) * sizeof(float4);
+  hipLaunchKernelGGL(k_raycast, dim3(tiles(F)), dim3(256), lds, s, sc, cam, P, F);
+  return hipGetLastError();
+}
+
+hipError_t launch_pathtrace(const KScene &sc, const KCamera &cam, const KParams &P,
+                            const KFrame &F, int accumID, hipStream_t s) {
+  const size_t lds = size_t(sc.tf_lds_total) * sizeof(float4);
+  hipLaunchKernelGGL(k_pathtrace, dim3(tiles(F)), dim3(256), lds, s, sc, cam, P, F, accumID);
+  return hipGetLastError();
+}
+
+hipError_t launch_brickify(const float *lin, const int dims[3], float *bricks, hipStream_t s) {
+  const int nbx = (dims[0] + 7) / 8, nby = (dims[1] + 7) / 8, nbz = (dims[2] + 7) / 8;
+  const size_t total = size_t(nbx) * nby * nbz * 512;
+  const size_t blocks = std::min<size_t>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_brickify, dim3(unsigned(blocks)), dim3(256), 0, s, lin, dims[0], dims[1],
+                     dims[2], nbx, nby, nbz, bricks);
+  return hipGetLastError();
+}
+
+hipError_t launch_value_ranges(const KVolume &v, float *ranges, hipStream_t s) {
+  const int nmc = v.gdims[0] * v.gdims[1] * v.gdims[2];
+  hipLaunchKernelGGL(k_value_ranges, dim3(nmc), dim3(256), 0, s, v,
+                     reinterpret_cast<float2 *>(ranges));
+  return hipGetLastError();
+}
+
+hipError_t launch_majorants(const KVolume &v, const float *ranges, float *maj, hipStream_t s) {
+  const int nmc = v.gdims[0] * v.gdims[1] * v.gdims[2];
+  hipLaunchKernelGGL(k_majorants, dim3((nmc + 255) / 256), dim3(256), 0, s, v,
+                     reinterpret_cast<const float2 *>(ranges), nmc, maj);
+  return hipGetLastError();
+}
+
+hipError_t launch_sample_field(const KVolume &v, int n, const float *pts, float *out,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(k_sample_field, dim3((n + 255) / 256), dim3(256), 0, s, v, n, pts, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_delta_track(const KScene &sc, int n, const float *rays, uint32_t *rng,
+                              float *tout, hipStream_t s) {
+  const size_t lds = size_t(sc.tf_lds_total) * sizeof(float4);
+  hipLaunchKernelGGL(k_delta_track, dim3((n + 255) / 256), dim3(256), lds, s, sc, n, rays, rng,
+                     tout);
+  return hipGetLastError();
+}
+
+hipError_t launch_frame_reset(const KFrame &F, hipStream_t s) {
+  const size_t n = size_t(F.width) * F.height;
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_frame_reset, dim3(unsigned(blocks)), dim3(256), 0, s, F);
+  return hipGetLastError();
+}
+
+}  // namespace vrt

@@ -1,0 +1,427 @@
+// C ABI (include/vrt.h): argument validation, host-side derivation of the
+// kernel scene views, and launches on the context stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "render_kernels.h"
+#include "vrt.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_error = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+  return fail(VRT_ERROR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define VRT_HIP(call)                                \
+  do {                                               \
+    hipError_t e_ = (call);                          \
+    if (e_ != hipSuccess) return hip_fail(e_, #call); \
+  } while (0)
+
+}  // namespace
+
+struct vrt_context_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+};
+
+namespace {
+
+using namespace vrt;
+
+int check_ctx(vrt_context ctx) {
+  if (!ctx) return fail(VRT_ERROR_INVALID_ARGUMENT, "null context");
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  return VRT_OK;
+}
+
+int make_volume(const vrt_volume &in, int lds_offset, KVolume &v) {
+  if (in.field_type != VRT_FIELD_STRUCTURED_REGULAR)
+    return fail(VRT_ERROR_UNSUPPORTED, "field type %d not supported by this entry point",
+                in.field_type);
+  for (int a = 0; a < 3; ++a)
+    if (in.dims[a] < 1 || in.dims[a] > 4096)
+      return fail(VRT_ERROR_INVALID_ARGUMENT, "volume dims[%d]=%d out of range", a, in.dims[a]);
+  if (!in.bricks) return fail(VRT_ERROR_INVALID_ARGUMENT, "volume has no voxel bricks");
+  if (!in.tf_rgba || in.tf_size < 1)
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "volume has no transfer function table");
+  if (!(in.unit_distance > 0.f))
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "unit_distance must be > 0");
+  if (in.grid_dims[0] > 0 && (!in.majorants || in.grid_dims[1] < 1 || in.grid_dims[2] < 1))
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "majorant grid incomplete");
+  const size_t nb = size_t((in.dims[0] + 7) / 8) * ((in.dims[1] + 7) / 8) * ((in.dims[2] + 7) / 8);
+  if (nb * 512 > 0xFFFFFFFFull)
+    return fail(VRT_ERROR_UNSUPPORTED, "volume exceeds 32-bit brick addressing");
+  v = KVolume{};
+  v.field_type = in.field_type;
+  for (int a = 0; a < 3; ++a) v.dims[a] = in.dims[a];
+  const uint32_t nbx = uint32_t((in.dims[0] + 7) / 8), nby = uint32_t((in.dims[1] + 7) / 8);
+  v.bstride_y = nbx * 512u;
+  v.bstride_z = nbx * nby * 512u;
+  v.blo = mk(in.bounds_lo[0], in.bounds_lo[1], in.bounds_lo[2]);
+  v.bhi = mk(in.bounds_hi[0], in.bounds_hi[1], in.bounds_hi[2]);
+  // voxelSpaceTransform = (scaling(1/bounds.size()), -bounds.min)
+  // (StructuredRegularField.cpp:37-40)
+  v.S = mk(1.f / (v.bhi.x - v.blo.x), 1.f / (v.bhi.y - v.blo.y), 1.f / (v.bhi.z - v.blo.z));
+  v.T = -v.blo;
+  v.cell_size = in.cell_size;
+  v.bricks = in.bricks;
+  v.vdb = in.nanovdb_grid;
+  v.tf = reinterpret_cast<const float4 *>(in.tf_rgba);
+  v.tf_size = in.tf_size;
+  v.tf_lds_offset = lds_offset;
+  v.vlo = in.value_lo;
+  v.vhi = in.value_hi;
+  v.unit_distance = in.unit_distance;
+  v.majorants = in.majorants;
+  for (int a = 0; a < 3; ++a) v.gdims[a] = in.grid_dims[0] > 0 ? in.grid_dims[a] : 0;
+  for (int i = 0; i < 9; ++i) v.A[i] = in.affine_inv[i];
+  v.tinv = mk(in.trans_inv[0], in.trans_inv[1], in.trans_inv[2]);
+  return VRT_OK;
+}
+
+constexpr int kMaxLdsTf = 4096;  // float4 entries (64 KiB)
+
+int make_scene(const vrt_scene *s, KScene &k) {
+  if (!s) return fail(VRT_ERROR_INVALID_ARGUMENT, "null scene");
+  if (s->num_volumes < 0 || s->num_volumes > VRT_MAX_VOLUMES)
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "num_volumes %d out of range", s->num_volumes);
+  if (s->num_lights < 0 || s->num_lights > VRT_MAX_LIGHTS)
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "num_lights %d out of range", s->num_lights);
+  k = KScene{};
+  k.num_volumes = s->num_volumes;
+  k.num_lights = s->num_lights;
+  int off = 0;
+  for (int i = 0; i < s->num_volumes; ++i) {
+    int r = make_volume(s->volumes[i], off, k.vol[i]);
+    if (r) return r;
+    off += s->volumes[i].tf_size;
+  }
+  if (off > kMaxLdsTf)
+    return fail(VRT_ERROR_UNSUPPORTED, "transfer function tables exceed %d entries", kMaxLdsTf);
+  k.tf_lds_total = off;
+  for (int i = 0; i < s->num_lights; ++i) {
+    const vrt_light &L = s->lights[i];
+    KLight &o = k.light[i];
+    o.type = L.type;
+    o.visible = L.visible;
+    if (L.type == VRT_LIGHT_DIRECTIONAL) {
+      f3 d = mk(L.direction[0], L.direction[1], L.direction[2]);
+      float len = length(d);
+      if (!(len > 0.f)) return fail(VRT_ERROR_INVALID_ARGUMENT, "directional light with zero direction");
+      // Directional.cpp:33 stores -direction: the sample direction points at the light
+      o.dir = -(d * (1.f / len));
+      o.intensity = mk(L.color[0], L.color[1], L.color[2]) * L.intensity;
+    } else {
+      return fail(VRT_ERROR_UNSUPPORTED, "light type %d not supported by this entry point", L.type);
+    }
+  }
+  return VRT_OK;
+}
+
+int make_camera(const vrt_camera *c, KCamera &k) {
+  if (!c) return fail(VRT_ERROR_INVALID_ARGUMENT, "null camera");
+  f3 d = mk(c->direction[0], c->direction[1], c->direction[2]);
+  f3 up = mk(c->up[0], c->up[1], c->up[2]);
+  if (!(length(d) > 0.f) || !(length(up) > 0.f))
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "camera direction/up must be non-zero");
+  d = normalize(d);
+  up = normalize(up);
+  const float th = std::tan(c->fovy * 0.5f);
+  f3 right = cross(d, up);
+  if (!(length(right) > 0.f)) return fail(VRT_ERROR_INVALID_ARGUMENT, "camera up parallel to direction");
+  right = normalize(right);
+  const f3 upv = cross(right, d);
+  k.pos = mk(c->position[0], c->position[1], c->position[2]);
+  k.U = right * (th * c->aspect);
+  k.V = upv * th;
+  k.W = d;
+  for (int i = 0; i < 4; ++i) k.region[i] = c->image_region[i];
+  k.shutter[0] = c->shutter[0];
+  k.shutter[1] = c->shutter[1];
+  return VRT_OK;
+}
+
+int make_params(const vrt_params *p, KParams &k) {
+  if (!p) return fail(VRT_ERROR_INVALID_ARGUMENT, "null params");
+  if (p->pixel_samples < 1 || p->pixel_samples > 256)
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "pixel_samples %d out of [1,256]", p->pixel_samples);
+  if (p->max_bounce < 0 || p->max_bounce > 256)
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "max_bounce %d out of [0,256]", p->max_bounce);
+  if (p->ambient_samples < 0 || p->ambient_samples > 256)
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "ambient_samples %d out of [0,256]", p->ambient_samples);
+  k.bg = make_float4(p->background[0], p->background[1], p->background[2], p->background[3]);
+  k.ambient_color = mk(p->ambient_color[0], p->ambient_color[1], p->ambient_color[2]);
+  k.ambient_radiance = p->ambient_radiance;
+  k.occlusion_distance = p->occlusion_distance;
+  k.ambient_samples = p->ambient_samples;
+  k.max_bounce = p->max_bounce;
+  k.pixel_samples = p->pixel_samples;
+  // Raycast.cpp:27 safe_rcp
+  k.sampling_rate_inv = p->volume_sampling_rate > 0.f ? 1.f / p->volume_sampling_rate : 0.f;
+  return VRT_OK;
+}
+
+int make_frame(const vrt_frame *f, KFrame &k) {
+  if (!f) return fail(VRT_ERROR_INVALID_ARGUMENT, "null frame");
+  if (f->width < 1 || f->height < 1 || f->width > 16384 || f->height > 16384)
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "frame size %dx%d out of range", f->width, f->height);
+  if (f->color_type != VRT_TYPE_FLOAT32_VEC4 && f->color_type != VRT_TYPE_UFIXED8_VEC4 &&
+      f->color_type != VRT_TYPE_UFIXED8_RGBA_SRGB && f->color_type != 0)
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "unsupported color type %d", f->color_type);
+  if (f->color_type != 0 && !f->color)
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "color channel requested without a buffer");
+  k.width = f->width;
+  k.height = f->height;
+  k.color_type = f->color_type;
+  k.color = f->color;
+  k.accum = reinterpret_cast<float4 *>(f->accum);
+  k.depth = f->depth;
+  k.accum64 = f->accum64;
+  k.counters = f->counters;
+  return VRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *vrt_last_error(void) { return g_error.c_str(); }
+
+int vrt_context_create(int device, vrt_context *out) {
+  if (!out) return fail(VRT_ERROR_INVALID_ARGUMENT, "null output");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return fail(VRT_ERROR_NO_DEVICE, "no HIP device available");
+  if (device < 0 || device >= n) return fail(VRT_ERROR_INVALID_ARGUMENT, "device %d of %d", device, n);
+  VRT_HIP(hipSetDevice(device));
+  auto *c = new vrt_context_s;
+  c->device = device;
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&c->t0);
+  if (e == hipSuccess) e = hipEventCreate(&c->t1);
+  if (e != hipSuccess) {
+    delete c;
+    return hip_fail(e, "stream/event creation");
+  }
+  *out = c;
+  return VRT_OK;
+}
+
+int vrt_context_destroy(vrt_context ctx) {
+  if (!ctx) return VRT_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipEventDestroy(ctx->t0);
+  (void)hipEventDestroy(ctx->t1);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return VRT_OK;
+}
+
+int vrt_context_stream(vrt_context ctx, void **s) {
+  if (!ctx || !s) return fail(VRT_ERROR_INVALID_ARGUMENT, "null argument");
+  *s = ctx->stream;
+  return VRT_OK;
+}
+
+int vrt_synchronize(vrt_context ctx) {
+  if (int r = check_ctx(ctx)) return r;
+  VRT_HIP(hipStreamSynchronize(ctx->stream));
+  return VRT_OK;
+}
+
+int vrt_alloc(vrt_context ctx, size_t bytes, void **d) {
+  if (int r = check_ctx(ctx)) return r;
+  if (!d) return fail(VRT_ERROR_INVALID_ARGUMENT, "null output");
+  *d = nullptr;
+  if (bytes == 0) return VRT_OK;
+  hipError_t e = hipMalloc(d, bytes);
+  if (e == hipErrorOutOfMemory) return fail(VRT_ERROR_OUT_OF_MEMORY, "out of device memory (%zu bytes)", bytes);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+  return VRT_OK;
+}
+
+int vrt_free(vrt_context ctx, void *d) {
+  if (int r = check_ctx(ctx)) return r;
+  if (d) VRT_HIP(hipFree(d));
+  return VRT_OK;
+}
+
+int vrt_upload(vrt_context ctx, void *dst, const void *src, size_t bytes) {
+  if (int r = check_ctx(ctx)) return r;
+  if (bytes == 0) return VRT_OK;
+  if (!dst || !src) return fail(VRT_ERROR_INVALID_ARGUMENT, "null buffer");
+  VRT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  VRT_HIP(hipStreamSynchronize(ctx->stream));
+  return VRT_OK;
+}
+
+int vrt_download(vrt_context ctx, void *dst, const void *src, size_t bytes) {
+  if (int r = check_ctx(ctx)) return r;
+  if (bytes == 0) return VRT_OK;
+  if (!dst || !src) return fail(VRT_ERROR_INVALID_ARGUMENT, "null buffer");
+  VRT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  VRT_HIP(hipStreamSynchronize(ctx->stream));
+  return VRT_OK;
+}
+
+int vrt_memset(vrt_context ctx, void *dst, int value, size_t bytes) {
+  if (int r = check_ctx(ctx)) return r;
+  if (bytes == 0) return VRT_OK;
+  if (!dst) return fail(VRT_ERROR_INVALID_ARGUMENT, "null buffer");
+  VRT_HIP(hipMemsetAsync(dst, value, bytes, ctx->stream));
+  return VRT_OK;
+}
+
+int vrt_timer_start(vrt_context ctx) {
+  if (int r = check_ctx(ctx)) return r;
+  VRT_HIP(hipEventRecord(ctx->t0, ctx->stream));
+  return VRT_OK;
+}
+
+int vrt_timer_stop(vrt_context ctx, float *ms) {
+  if (int r = check_ctx(ctx)) return r;
+  VRT_HIP(hipEventRecord(ctx->t1, ctx->stream));
+  VRT_HIP(hipEventSynchronize(ctx->t1));
+  float v = 0.f;
+  VRT_HIP(hipEventElapsedTime(&v, ctx->t0, ctx->t1));
+  if (ms) *ms = v;
+  return VRT_OK;
+}
+
+size_t vrt_structured_brick_bytes(const int dims[3]) {
+  if (!dims || dims[0] < 1 || dims[1] < 1 || dims[2] < 1) return 0;
+  return size_t((dims[0] + 7) / 8) * ((dims[1] + 7) / 8) * ((dims[2] + 7) / 8) * 512 * sizeof(float);
+}
+
+int vrt_structured_brickify(vrt_context ctx, const float *lin, const int dims[3], float *bricks) {
+  if (int r = check_ctx(ctx)) return r;
+  if (!lin || !bricks || !dims) return fail(VRT_ERROR_INVALID_ARGUMENT, "null argument");
+  for (int a = 0; a < 3; ++a)
+    if (dims[a] < 1 || dims[a] > 4096)
+      return fail(VRT_ERROR_INVALID_ARGUMENT, "dims[%d]=%d out of range", a, dims[a]);
+  VRT_HIP(launch_brickify(lin, dims, bricks, ctx->stream));
+  return VRT_OK;
+}
+
+int vrt_build_value_ranges(vrt_context ctx, const vrt_volume *vol, float *ranges) {
+  if (int r = check_ctx(ctx)) return r;
+  if (!vol || !ranges) return fail(VRT_ERROR_INVALID_ARGUMENT, "null argument");
+  KVolume v;
+  vrt_volume tmp = *vol;
+  if (tmp.grid_dims[0] < 1 || tmp.grid_dims[1] < 1 || tmp.grid_dims[2] < 1)
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "grid_dims must be positive");
+  if (!tmp.majorants) tmp.majorants = ranges;  // not read by this kernel
+  if (!tmp.tf_rgba) {
+    static const float dummy[4] = {0, 0, 0, 0};
+    tmp.tf_rgba = dummy;
+    tmp.tf_size = 1;
+  }
+  if (!(tmp.unit_distance > 0.f)) tmp.unit_distance = 1.f;
+  if (int r = make_volume(tmp, 0, v)) return r;
+  VRT_HIP(launch_value_ranges(v, ranges, ctx->stream));
+  return VRT_OK;
+}
+
+int vrt_compute_majorants(vrt_context ctx, const vrt_volume *vol, const float *ranges, float *maj) {
+  if (int r = check_ctx(ctx)) return r;
+  if (!vol || !ranges || !maj) return fail(VRT_ERROR_INVALID_ARGUMENT, "null argument");
+  vrt_volume tmp = *vol;
+  if (tmp.grid_dims[0] < 1 || tmp.grid_dims[1] < 1 || tmp.grid_dims[2] < 1)
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "grid_dims must be positive");
+  if (!tmp.majorants) tmp.majorants = maj;
+  KVolume v;
+  if (int r = make_volume(tmp, 0, v)) return r;
+  VRT_HIP(launch_majorants(v, ranges, maj, ctx->stream));
+  return VRT_OK;
+}
+
+int vrt_sample_field(vrt_context ctx, const vrt_volume *vol, int n, const float *pts,
+                     float *values) {
+  if (int r = check_ctx(ctx)) return r;
+  if (!vol || n < 0 || (n > 0 && (!pts || !values)))
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "bad sample_field arguments");
+  if (n == 0) return VRT_OK;
+  KVolume v;
+  if (int r = make_volume(*vol, 0, v)) return r;
+  VRT_HIP(launch_sample_field(v, n, pts, values, ctx->stream));
+  return VRT_OK;
+}
+
+int vrt_delta_track(vrt_context ctx, const vrt_scene *scene, int n, const float *rays,
+                    uint32_t *rng, float *t_out) {
+  if (int r = check_ctx(ctx)) return r;
+  if (n < 0 || (n > 0 && (!rays || !rng || !t_out)))
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "bad delta_track arguments");
+  KScene sc;
+  if (int r = make_scene(scene, sc)) return r;
+  if (sc.num_volumes < 1) return fail(VRT_ERROR_INVALID_ARGUMENT, "scene has no volume");
+  if (n == 0) return VRT_OK;
+  VRT_HIP(launch_delta_track(sc, n, rays, rng, t_out, ctx->stream));
+  return VRT_OK;
+}
+
+int vrt_frame_reset(vrt_context ctx, const vrt_frame *frame) {
+  if (int r = check_ctx(ctx)) return r;
+  KFrame F;
+  if (int r = make_frame(frame, F)) return r;
+  VRT_HIP(launch_frame_reset(F, ctx->stream));
+  return VRT_OK;
+}
+
+int vrt_raycast(vrt_context ctx, const vrt_scene *scene, const vrt_camera *camera,
+                const vrt_params *params, const vrt_frame *frame, int frame_id) {
+  (void)frame_id;
+  if (int r = check_ctx(ctx)) return r;
+  KScene sc;
+  KCamera cam;
+  KParams P;
+  KFrame F;
+  if (int r = make_scene(scene, sc)) return r;
+  if (int r = make_camera(camera, cam)) return r;
+  if (int r = make_params(params, P)) return r;
+  if (int r = make_frame(frame, F)) return r;
+  VRT_HIP(launch_raycast(sc, cam, P, F, ctx->stream));
+  return VRT_OK;
+}
+
+int vrt_pathtrace(vrt_context ctx, const vrt_scene *scene, const vrt_camera *camera,
+                  const vrt_params *params, const vrt_frame *frame, int accum_id, int frame_id) {
+  (void)frame_id;
+  if (int r = check_ctx(ctx)) return r;
+  if (accum_id < 0) return fail(VRT_ERROR_INVALID_ARGUMENT, "accum_id must be >= 0");
+  KScene sc;
+  KCamera cam;
+  KParams P;
+  KFrame F;
+  if (int r = make_scene(scene, sc)) return r;
+  if (int r = make_camera(camera, cam)) return r;
+  if (int r = make_params(params, P)) return r;
+  if (int r = make_frame(frame, F)) return r;
+  if (!F.accum) return fail(VRT_ERROR_INVALID_ARGUMENT, "pathtrace needs an accumulation buffer");
+  VRT_HIP(launch_pathtrace(sc, cam, P, F, accum_id, ctx->stream));
+  return VRT_OK;
+}
+
+}  // extern "C"

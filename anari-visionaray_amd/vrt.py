@@ -1,0 +1,424 @@
+"""ctypes binding of libvrt.so (include/vrt.h) and of the scene generators.
+
+The binding is thin: structs mirror the C declarations field by field, and
+every call checks the returned status and raises VrtError with the library's
+message.  Loading fails loudly when the HIP library has not been built --
+there is no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(PKG, "_build")
+LIB_PATH = os.path.join(BUILD, "libvrt.so")
+GEN_PATH = os.path.join(BUILD, "libvrt_scenegen.so")
+
+FIELD_STRUCTURED_REGULAR = 0
+FIELD_NANOVDB = 1
+FIELD_UNSTRUCTURED = 2
+LIGHT_DIRECTIONAL = 0
+LIGHT_QUAD = 1
+MAX_VOLUMES = 4
+MAX_LIGHTS = 8
+TYPE_FLOAT32 = 1068
+TYPE_FLOAT32_VEC4 = 1071
+TYPE_UFIXED8_VEC4 = 1039
+TYPE_UFIXED8_RGBA_SRGB = 2003
+
+
+class VrtError(RuntimeError):
+    pass
+
+
+f3 = C.c_float * 3
+f4 = C.c_float * 4
+i3 = C.c_int * 3
+
+
+class Volume(C.Structure):
+    _fields_ = [
+        ("field_type", C.c_int), ("dims", i3), ("bounds_lo", f3), ("bounds_hi", f3),
+        ("cell_size", C.c_float), ("bricks", C.c_void_p), ("nanovdb_grid", C.c_void_p),
+        ("tf_rgba", C.c_void_p), ("tf_size", C.c_int), ("value_lo", C.c_float),
+        ("value_hi", C.c_float), ("unit_distance", C.c_float), ("majorants", C.c_void_p),
+        ("grid_dims", i3), ("affine_inv", C.c_float * 9), ("trans_inv", f3),
+    ]
+
+
+class Light(C.Structure):
+    _fields_ = [
+        ("type", C.c_int), ("visible", C.c_int), ("direction", f3), ("position", f3),
+        ("edge1", f3), ("edge2", f3), ("color", f3), ("intensity", C.c_float),
+    ]
+
+
+class Scene(C.Structure):
+    _fields_ = [
+        ("num_volumes", C.c_int), ("volumes", Volume * MAX_VOLUMES),
+        ("num_lights", C.c_int), ("lights", Light * MAX_LIGHTS),
+    ]
+
+
+class Camera(C.Structure):
+    _fields_ = [
+        ("position", f3), ("direction", f3), ("up", f3), ("fovy", C.c_float),
+        ("aspect", C.c_float), ("image_region", f4), ("shutter", C.c_float * 2),
+    ]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("background", f4), ("ambient_color", f3), ("ambient_radiance", C.c_float),
+        ("occlusion_distance", C.c_float), ("ambient_samples", C.c_int),
+        ("max_bounce", C.c_int), ("pixel_samples", C.c_int),
+        ("volume_sampling_rate", C.c_float),
+    ]
+
+
+class Frame(C.Structure):
+    _fields_ = [
+        ("width", C.c_int), ("height", C.c_int), ("color_type", C.c_int),
+        ("color", C.c_void_p), ("accum", C.c_void_p), ("depth", C.c_void_p),
+        ("accum64", C.c_void_p), ("counters", C.c_void_p),
+    ]
+
+
+_lib = None
+_gen = None
+
+
+def lib():
+    """The loaded libvrt.so; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise VrtError(f"{LIB_PATH} is missing: run python anari-visionaray_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        vp, sz, ip = C.c_void_p, C.c_size_t, C.c_int
+        L.vrt_last_error.restype = C.c_char_p
+        L.vrt_context_create.argtypes = [ip, C.POINTER(vp)]
+        L.vrt_context_destroy.argtypes = [vp]
+        L.vrt_context_stream.argtypes = [vp, C.POINTER(vp)]
+        L.vrt_synchronize.argtypes = [vp]
+        L.vrt_alloc.argtypes = [vp, sz, C.POINTER(vp)]
+        L.vrt_free.argtypes = [vp, vp]
+        L.vrt_upload.argtypes = [vp, vp, vp, sz]
+        L.vrt_download.argtypes = [vp, vp, vp, sz]
+        L.vrt_memset.argtypes = [vp, vp, ip, sz]
+        L.vrt_timer_start.argtypes = [vp]
+        L.vrt_timer_stop.argtypes = [vp, C.POINTER(C.c_float)]
+        L.vrt_structured_brick_bytes.argtypes = [C.POINTER(C.c_int)]
+        L.vrt_structured_brick_bytes.restype = sz
+        L.vrt_structured_brickify.argtypes = [vp, vp, C.POINTER(C.c_int), vp]
+        L.vrt_build_value_ranges.argtypes = [vp, C.POINTER(Volume), vp]
+        L.vrt_compute_majorants.argtypes = [vp, C.POINTER(Volume), vp, vp]
+        L.vrt_frame_reset.argtypes = [vp, C.POINTER(Frame)]
+        L.vrt_sample_field.argtypes = [vp, C.POINTER(Volume), ip, vp, vp]
+        L.vrt_delta_track.argtypes = [vp, C.POINTER(Scene), ip, vp, vp, vp]
+        L.vrt_raycast.argtypes = [vp, C.POINTER(Scene), C.POINTER(Camera), C.POINTER(Params),
+                                  C.POINTER(Frame), ip]
+        L.vrt_pathtrace.argtypes = [vp, C.POINTER(Scene), C.POINTER(Camera), C.POINTER(Params),
+                                    C.POINTER(Frame), ip, ip]
+        _lib = L
+    return _lib
+
+
+def gen():
+    """The loaded scene-generator library."""
+    global _gen
+    if _gen is None:
+        if not os.path.exists(GEN_PATH):
+            raise VrtError(f"{GEN_PATH} is missing: run python anari-visionaray_amd/build.py")
+        G = C.CDLL(GEN_PATH)
+        G.vrtgen_blobs.argtypes = [C.POINTER(C.c_int), C.c_int, C.c_uint64, C.c_void_p, C.c_int]
+        G.vrtgen_fbm.argtypes = [C.POINTER(C.c_int), C.c_int, C.c_double, C.c_double,
+                                 C.c_double, C.c_uint64, C.c_void_p, C.c_int]
+        G.vrtgen_splitmix_uniform.argtypes = [C.c_uint64, C.c_int, C.c_void_p]
+        _gen = G
+    return _gen
+
+
+def check(status):
+    if status != 0:
+        msg = lib().vrt_last_error().decode(errors="replace")
+        raise VrtError(f"vrt status {status}: {msg}")
+    return status
+
+
+def _dims(d):
+    return (C.c_int * 3)(*[int(x) for x in d])
+
+
+def gen_blobs(dims, count=32, seed=0, threads=0):
+    out = np.empty((dims[2], dims[1], dims[0]), np.float32)
+    gen().vrtgen_blobs(_dims(dims), count, seed, out.ctypes.data, threads)
+    return out
+
+
+def gen_fbm(dims, octaves=6, base_freq=4.0, lacunarity=2.0, gain=0.5, seed=0, threads=0):
+    out = np.empty((dims[2], dims[1], dims[0]), np.float32)
+    gen().vrtgen_fbm(_dims(dims), octaves, base_freq, lacunarity, gain, seed, out.ctypes.data,
+                     threads)
+    return out
+
+
+class DeviceBuffer:
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(lib().vrt_alloc(ctx.handle, self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes, (arr.nbytes, self.nbytes)
+        check(lib().vrt_upload(self.ctx.handle, self.ptr, arr.ctypes.data, arr.nbytes))
+
+    def download(self, dtype, shape):
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        check(lib().vrt_download(self.ctx.handle, out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def zero(self):
+        check(lib().vrt_memset(self.ctx.handle, self.ptr, 0, self.nbytes))
+
+    def free(self):
+        if self.ptr:
+            check(lib().vrt_free(self.ctx.handle, self.ptr))
+            self.ptr = None
+
+
+class Context:
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().vrt_context_create(device, C.byref(h)))
+        self.handle = h.value
+        self._bufs = []
+
+    def buffer(self, nbytes):
+        b = DeviceBuffer(self, nbytes)
+        self._bufs.append(b)
+        return b
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        b = self.buffer(max(arr.nbytes, 4))
+        b.upload(arr)
+        return b
+
+    def sync(self):
+        check(lib().vrt_synchronize(self.handle))
+
+    def timer_start(self):
+        check(lib().vrt_timer_start(self.handle))
+
+    def timer_stop(self):
+        ms = C.c_float()
+        check(lib().vrt_timer_stop(self.handle, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        for b in self._bufs:
+            b.free()
+        self._bufs = []
+        if self.handle:
+            check(lib().vrt_context_destroy(self.handle))
+            self.handle = None
+
+
+class DeviceVolume:
+    """A structuredRegular field + transferFunction1D volume resident on the GPU
+    (brick-ordered voxels, TF table, value ranges and majorant grid)."""
+
+    def __init__(self, ctx, voxels, origin, spacing, tf_rgba, value_range, unit_distance,
+                 grid_dims=None, xfm=None):
+        voxels = np.ascontiguousarray(voxels, np.float32)
+        nz, ny, nx = voxels.shape
+        self.dims = (nx, ny, nz)
+        lin = ctx.upload(voxels)
+        nbytes = lib().vrt_structured_brick_bytes(_dims(self.dims))
+        self.bricks = ctx.buffer(nbytes)
+        check(lib().vrt_structured_brickify(ctx.handle, lin.ptr, _dims(self.dims), self.bricks.ptr))
+        ctx.sync()
+        lin.free()
+        tf = np.ascontiguousarray(tf_rgba, np.float32).reshape(-1, 4)
+        self.tf = ctx.upload(tf)
+        v = Volume()
+        v.field_type = FIELD_STRUCTURED_REGULAR
+        v.dims = i3(*self.dims)
+        lo = np.array(origin, np.float32)
+        hi = lo + (np.array(self.dims, np.float32) - np.float32(1)) * np.array(spacing, np.float32)
+        v.bounds_lo = f3(*lo)
+        v.bounds_hi = f3(*hi)
+        v.cell_size = float(np.min(np.array(spacing, np.float32)))
+        v.bricks = self.bricks.ptr
+        v.tf_rgba = self.tf.ptr
+        v.tf_size = tf.shape[0]
+        v.value_lo, v.value_hi = float(value_range[0]), float(value_range[1])
+        v.unit_distance = float(unit_distance)
+        A, tinv = affine_inverse(xfm)
+        v.affine_inv = (C.c_float * 9)(*A.ravel())
+        v.trans_inv = f3(*tinv)
+        self.grid_dims = None
+        if grid_dims is not None:
+            self.grid_dims = tuple(int(g) for g in grid_dims)
+            v.grid_dims = i3(*self.grid_dims)
+            nmc = int(np.prod(self.grid_dims))
+            self.ranges = ctx.buffer(nmc * 8)
+            self.majorants = ctx.buffer(nmc * 4)
+            check(lib().vrt_build_value_ranges(ctx.handle, C.byref(v), self.ranges.ptr))
+            v.majorants = self.majorants.ptr
+            check(lib().vrt_compute_majorants(ctx.handle, C.byref(v), self.ranges.ptr,
+                                              self.majorants.ptr))
+            ctx.sync()
+        self.desc = v
+
+    def majorant_array(self):
+        return self.majorants.download(np.float32, self.grid_dims[::-1])
+
+
+def affine_inverse(xfm):
+    """(affineInv, transInv) of an object->world row-major 3x4 transform."""
+    if xfm is None:
+        return np.eye(3, dtype=np.float32), np.zeros(3, np.float32)
+    m = np.asarray(xfm, np.float64).reshape(3, 4)
+    A = np.linalg.inv(m[:, :3]).astype(np.float32)
+    return A, (-m[:, 3]).astype(np.float32)
+
+
+def make_scene(volumes, lights=()):
+    s = Scene()
+    s.num_volumes = len(volumes)
+    for i, v in enumerate(volumes):
+        s.volumes[i] = v.desc
+    s.num_lights = len(lights)
+    for i, L in enumerate(lights):
+        s.lights[i] = L
+    return s
+
+
+def directional_light(direction, irradiance=1.0, color=(1, 1, 1), visible=True):
+    L = Light()
+    L.type = LIGHT_DIRECTIONAL
+    L.visible = int(visible)
+    L.direction = f3(*direction)
+    L.color = f3(*color)
+    L.intensity = float(irradiance)
+    return L
+
+
+def camera(position, direction, up, fovy, aspect, region=(0, 0, 1, 1), shutter=(0.5, 0.5)):
+    c = Camera()
+    c.position = f3(*position)
+    c.direction = f3(*direction)
+    c.up = f3(*up)
+    c.fovy = float(fovy)
+    c.aspect = float(aspect)
+    c.image_region = f4(*region)
+    c.shutter = (C.c_float * 2)(*shutter)
+    return c
+
+
+def params(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.2,
+           occlusion_distance=1e20, ambient_samples=1, max_bounce=7, pixel_samples=1,
+           volume_sampling_rate=0.5):
+    """Renderer parameters with the ANARI defaults of the reference
+    (Renderer.cpp:42-53, Pathtrace.cpp:19-25, Raycast.cpp:19)."""
+    p = Params()
+    p.background = f4(*background)
+    p.ambient_color = f3(*ambient_color)
+    p.ambient_radiance = ambient_radiance
+    p.occlusion_distance = occlusion_distance
+    p.ambient_samples = ambient_samples
+    p.max_bounce = max_bounce
+    p.pixel_samples = pixel_samples
+    p.volume_sampling_rate = volume_sampling_rate
+    return p
+
+
+class DeviceFrame:
+    def __init__(self, ctx, width, height, color_type=TYPE_FLOAT32_VEC4, depth=False,
+                 accum64=False, counters=True):
+        self.ctx = ctx
+        self.width, self.height = width, height
+        n = width * height
+        bpp = 16 if color_type == TYPE_FLOAT32_VEC4 else 4
+        self.color = ctx.buffer(n * bpp)
+        self.accum = ctx.buffer(n * 16)
+        self.depth = ctx.buffer(n * 4) if depth else None
+        self.accum64 = ctx.buffer(n * 32) if accum64 else None
+        self.counters = ctx.buffer(64) if counters else None
+        f = Frame()
+        f.width, f.height, f.color_type = width, height, color_type
+        f.color = self.color.ptr
+        f.accum = self.accum.ptr
+        f.depth = self.depth.ptr if depth else None
+        f.accum64 = self.accum64.ptr if accum64 else None
+        f.counters = self.counters.ptr if counters else None
+        self.desc = f
+        self.reset()
+
+    def reset(self):
+        check(lib().vrt_frame_reset(self.ctx.handle, C.byref(self.desc)))
+        if self.counters:
+            self.counters.zero()
+
+    def color_array(self):
+        if self.desc.color_type == TYPE_FLOAT32_VEC4:
+            return self.color.download(np.float32, (self.height, self.width, 4))
+        return self.color.download(np.uint32, (self.height, self.width))
+
+    def accum_array(self):
+        return self.accum.download(np.float32, (self.height, self.width, 4))
+
+    def accum64_array(self):
+        return self.accum64.download(np.float64, (self.height, self.width, 4))
+
+    def depth_array(self):
+        return self.depth.download(np.float32, (self.height, self.width))
+
+    def counter_values(self):
+        return self.counters.download(np.uint64, (8,))
+
+
+def sample_field(ctx, volume, points):
+    """Trilinear values of a DeviceVolume at voxel-space points (n, 3)."""
+    pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    n = pts.shape[0]
+    dp = ctx.upload(pts)
+    dv = ctx.buffer(max(4, n * 4))
+    check(lib().vrt_sample_field(ctx.handle, C.byref(volume.desc), n, dp.ptr, dv.ptr))
+    out = dv.download(np.float32, (n,))
+    dp.free()
+    dv.free()
+    return out
+
+
+def delta_track(ctx, scene, rays, rng_states):
+    """Collision distances of world rays (n, 8) through scene volume 0."""
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    n = rays.shape[0]
+    states = np.ascontiguousarray(rng_states, np.uint32)
+    dr = ctx.upload(rays)
+    ds = ctx.upload(states)
+    dt = ctx.buffer(max(4, n * 4))
+    check(lib().vrt_delta_track(ctx.handle, C.byref(scene), n, dr.ptr, ds.ptr, dt.ptr))
+    t = dt.download(np.float32, (n,))
+    s = ds.download(np.uint32, (n,))
+    for b in (dr, ds, dt):
+        b.free()
+    return t, s
+
+
+def raycast(ctx, scene, cam, prm, frame, frame_id=0):
+    check(lib().vrt_raycast(ctx.handle, C.byref(scene), C.byref(cam), C.byref(prm),
+                            C.byref(frame.desc), frame_id))
+
+
+def pathtrace(ctx, scene, cam, prm, frame, accum_id, frame_id=0):
+    check(lib().vrt_pathtrace(ctx.handle, C.byref(scene), C.byref(cam), C.byref(prm),
+                              C.byref(frame.desc), accum_id, frame_id))

@@ -1,0 +1,136 @@
+"""Synthetic workloads of the benchmark configurations, as plain numpy data.
+
+Each builder returns a ``Workload``: the field voxels (x-fastest, shape
+(nz, ny, nx)), the transferFunction1D table and parameters, camera, lights,
+renderer parameters and image size.  Both the GPU binding and the CPU
+oracle consume the same object, so parity runs see bit-identical inputs.
+
+blobs_raycast  -- raycast, 128^3 Gaussian blobs, 256-entry ramp TF, 512^2
+fbm_pathtrace  -- pathtrace, 512^3 fBm, density scale 50, 32^3 majorant grid
+                  (16^3-voxel macrocells), directional light + ambient, 1080p
+"""
+import dataclasses
+import math
+
+import numpy as np
+
+try:
+    from . import vrt as _vrt
+except ImportError:  # loaded by file path
+    import importlib.util as _ilu
+    import os as _os
+    _spec = _ilu.spec_from_file_location(
+        "vrt", _os.path.join(_os.path.dirname(_os.path.abspath(__file__)), "vrt.py"))
+    _vrt = _ilu.module_from_spec(_spec)
+    _spec.loader.exec_module(_vrt)
+
+
+@dataclasses.dataclass
+class Workload:
+    name: str
+    voxels: np.ndarray           # (nz, ny, nx) float32
+    origin: tuple
+    spacing: tuple
+    tf: np.ndarray               # (n, 4) float32 table
+    value_range: tuple
+    unit_distance: float
+    grid_dims: tuple             # majorant grid, None for none
+    camera: dict                 # position, direction, up, fovy, aspect
+    lights: list                 # dicts: type, direction, color, intensity
+    params: dict                 # renderer parameters
+    width: int
+    height: int
+    renderer: str                # "raycast" | "pathtrace"
+    frames: int = 1
+
+    @property
+    def dims(self):
+        nz, ny, nx = self.voxels.shape
+        return (nx, ny, nz)
+
+
+def ramp_tf(n=256):
+    """Piecewise-linear RGBA ramp: rgb blue -> red, alpha 0 -> 1."""
+    t = np.linspace(0.0, 1.0, n, dtype=np.float64)
+    tf = np.stack([t, np.zeros_like(t), 1.0 - t, t], axis=1)
+    return tf.astype(np.float32)
+
+
+def albedo_tf(albedo=0.8, n=256):
+    """Constant albedo, alpha = normalised value (density ramp)."""
+    t = np.linspace(0.0, 1.0, n, dtype=np.float64)
+    tf = np.stack([np.full_like(t, albedo)] * 3 + [t], axis=1)
+    return tf.astype(np.float32)
+
+
+def look_at(position, target=(0.0, 0.0, 0.0)):
+    d = np.asarray(target, np.float64) - np.asarray(position, np.float64)
+    return tuple(float(x) for x in d / np.linalg.norm(d))
+
+
+def blobs_raycast(n=128, width=512, height=512, seed=0, threads=0):
+    """Raycast: structuredRegular n^3 f32 Gaussian blobs (32 blobs), 256-entry
+    ramp TF, perspective fovy 40 deg from distance 2.5, 1 spp, step 0.5 voxel."""
+    vox = _vrt.gen_blobs((n, n, n), 32, seed, threads)
+    sp = 1.0 / (n - 1)
+    pos = (0.0, 0.0, 2.5)
+    return Workload(
+        name=f"raycast blobs {n}^3 {width}x{height}", voxels=vox, origin=(-0.5,) * 3,
+        spacing=(sp,) * 3, tf=ramp_tf(256), value_range=(0.0, 1.0), unit_distance=sp,
+        grid_dims=None,
+        camera=dict(position=pos, direction=look_at(pos), up=(0, 1, 0),
+                    fovy=math.radians(40.0), aspect=width / height),
+        lights=[],
+        params=dict(background=(0, 0, 0, 1), volume_sampling_rate=2.0, pixel_samples=1),
+        width=width, height=height, renderer="raycast")
+
+
+def fbm_pathtrace(n=512, width=1920, height=1080, frames=64, seed=1, brick=16,
+                  density_scale=50.0, albedo=0.8, max_bounce=8, threads=0):
+    """Pathtrace: structuredRegular n^3 f32 fBm (6 octaves, lacunarity 2,
+    gain 0.5) remapped to [0,1]; TF rgb = albedo, alpha = value with
+    unitDistance = 1/density_scale; majorant grid of brick^3-voxel cells;
+    one directional light plus the constant ambient term; isotropic phase."""
+    vox = _vrt.gen_fbm((n, n, n), 6, 4.0, 2.0, 0.5, seed, threads)
+    sp = 1.0 / (n - 1)
+    g = max(1, (n + brick - 1) // brick)
+    pos = (1.5, 1.0, 2.0)
+    return Workload(
+        name=f"pathtrace fbm {n}^3 {width}x{height}", voxels=vox, origin=(-0.5,) * 3,
+        spacing=(sp,) * 3, tf=albedo_tf(albedo, 256), value_range=(0.0, 1.0),
+        unit_distance=1.0 / density_scale, grid_dims=(g, g, g),
+        camera=dict(position=pos, direction=look_at(pos), up=(0, 1, 0),
+                    fovy=math.radians(40.0), aspect=width / height),
+        lights=[dict(type="directional", direction=(-0.4, -1.0, -0.3), color=(1, 1, 1),
+                     intensity=1.0)],
+        params=dict(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.5,
+                    occlusion_distance=1e20, ambient_samples=1, max_bounce=max_bounce,
+                    pixel_samples=1),
+        width=width, height=height, renderer="pathtrace", frames=frames)
+
+
+# ---------------------------------------------------------------------------
+# GPU side
+# ---------------------------------------------------------------------------
+class GpuScene:
+    """The workload resident on one GPU through the vrt C ABI."""
+
+    def __init__(self, ctx, wl: Workload, color_type=None, depth=False, accum64=False):
+        self.wl = wl
+        self.ctx = ctx
+        self.volume = _vrt.DeviceVolume(ctx, wl.voxels, wl.origin, wl.spacing, wl.tf,
+                                        wl.value_range, wl.unit_distance, wl.grid_dims)
+        lights = [_vrt.directional_light(L["direction"], L["intensity"], L["color"])
+                  for L in wl.lights]
+        self.scene = _vrt.make_scene([self.volume], lights)
+        self.camera = _vrt.camera(**wl.camera)
+        p = dict(wl.params)
+        self.params = _vrt.params(**p)
+        self.frame = _vrt.DeviceFrame(ctx, wl.width, wl.height,
+                                      color_type or _vrt.TYPE_FLOAT32_VEC4, depth, accum64)
+
+    def render(self, accum_id=0):
+        if self.wl.renderer == "raycast":
+            _vrt.raycast(self.ctx, self.scene, self.camera, self.params, self.frame, accum_id)
+        else:
+            _vrt.pathtrace(self.ctx, self.scene, self.camera, self.params, self.frame, accum_id)

@@ -1,0 +1,232 @@
+/*
+ * vrt.h -- C ABI of the MI355X (gfx950) per-pixel rendering kernels.
+ *
+ * This is the operator-level boundary under the ANARI device: it replaces
+ *   VisionarayRendererRaycast::renderFrame   (renderer/Raycast_impl.h:14-21)
+ *   VisionarayRendererPathtrace::renderFrame (renderer/Pathtrace_impl.h:15-22)
+ * whose arguments were DevicePointer<DeviceObjectRegistry/RendererState/
+ * dco::Frame/dco::Camera>, uint2 size, SyncContext, worldID, frameID.  Here
+ * every argument is a plain-old-data struct passed by value to the kernel
+ * (no per-frame device allocation), all buffers are device pointers owned by
+ * the caller, and all work is enqueued on the context's stream.
+ *
+ * Conventions
+ *  - Every entry point returns VRT_OK (0) or a negative vrt_status; nothing
+ *    throws across the ABI.  vrt_last_error() gives a message.
+ *  - Buffers passed as device pointers are never freed by the library.
+ *  - A frame is not re-entrant: one render call per frame at a time.
+ */
+#ifndef VRT_H
+#define VRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VRT_API __attribute__((visibility("default")))
+
+typedef enum vrt_status {
+  VRT_OK = 0,
+  VRT_ERROR_INVALID_ARGUMENT = -1,
+  VRT_ERROR_HIP = -2,
+  VRT_ERROR_OUT_OF_MEMORY = -3,
+  VRT_ERROR_UNSUPPORTED = -4,
+  VRT_ERROR_NO_DEVICE = -5
+} vrt_status;
+
+/* ---------------------------------------------------------------------- */
+/* Scene description (plain old data; pointers are device pointers)       */
+/* ---------------------------------------------------------------------- */
+
+#define VRT_FIELD_STRUCTURED_REGULAR 0
+#define VRT_FIELD_NANOVDB 1
+#define VRT_FIELD_UNSTRUCTURED 2
+
+#define VRT_BRICK_LOG2 3 /* 8^3-voxel bricks, 2 KiB of fp32 each */
+
+/* A transferFunction1D volume over one spatial field, placed by one
+ * instance.  Mirrors dco::Volume + dco::SpatialField + dco::GridAccel +
+ * dco::TransferFunction1D (DeviceCopyableObjects.h:331-638) and the
+ * instance's affineInv/transInv (DeviceCopyableObjects.h:1659-1680). */
+typedef struct vrt_volume {
+  int field_type;      /* VRT_FIELD_* */
+  int dims[3];         /* voxels per axis */
+  float bounds_lo[3];  /* object-space bounds: origin ..            */
+  float bounds_hi[3];  /*   origin + (dims-1)*spacing               */
+  float cell_size;     /* min(spacing) (StructuredRegularField.cpp:40) */
+  /* structuredRegular: brick-ordered voxels, see vrt_structured_brickify */
+  const float *bricks;
+  /* nanovdb: device copy of the grid buffer (NanoVDBField.cpp:29-64) */
+  const void *nanovdb_grid;
+  /* transfer function: tf_size float4 entries, value range, unitDistance */
+  const float *tf_rgba;
+  int tf_size;
+  float value_lo, value_hi;
+  float unit_distance;
+  /* majorant grid over voxel space [0,1]^3 (grid_dims[0] == 0: none, the
+     whole box is one segment with majorant 1) */
+  const float *majorants;
+  int grid_dims[3];
+  /* instance: object = affine_inv * (world + trans_inv) */
+  float affine_inv[9]; /* row-major 3x3 */
+  float trans_inv[3];
+} vrt_volume;
+
+#define VRT_LIGHT_DIRECTIONAL 0
+#define VRT_LIGHT_QUAD 1
+
+typedef struct vrt_light {
+  int type;
+  int visible;
+  float direction[3]; /* directional: ANARI "direction" (light travel) */
+  float position[3];  /* quad */
+  float edge1[3];
+  float edge2[3];
+  float color[3];
+  float intensity;    /* directional: irradiance; quad: intensity */
+} vrt_light;
+
+#define VRT_MAX_VOLUMES 4
+#define VRT_MAX_LIGHTS 8
+
+typedef struct vrt_scene {
+  int num_volumes;
+  vrt_volume volumes[VRT_MAX_VOLUMES];
+  int num_lights;
+  vrt_light lights[VRT_MAX_LIGHTS];
+} vrt_scene;
+
+/* ANARI "perspective" camera (camera/Perspective.cpp:14-44,
+ * camera/Camera.cpp:32-40). */
+typedef struct vrt_camera {
+  float position[3];
+  float direction[3];
+  float up[3];
+  float fovy;   /* radians */
+  float aspect;
+  float image_region[4]; /* lo.x lo.y hi.x hi.y */
+  float shutter[2];
+} vrt_camera;
+
+/* RendererState subset (renderer/common.h:55-91) with the ANARI renderer
+ * parameter defaults of Renderer.cpp:42-53, Pathtrace.cpp:19-25,
+ * Raycast.cpp:19. */
+typedef struct vrt_params {
+  float background[4];       /* (0,0,0,1) */
+  float ambient_color[3];    /* (1,1,1) */
+  float ambient_radiance;    /* 0.2 */
+  float occlusion_distance;  /* 1e20 */
+  int ambient_samples;       /* 1 */
+  int max_bounce;            /* 7 */
+  int pixel_samples;         /* 1 */
+  float volume_sampling_rate;/* 0.5 (raycast) */
+} vrt_params;
+
+/* ANARI data types used for frame channels (ANARI 1.0 enum values). */
+#define VRT_TYPE_FLOAT32 1068
+#define VRT_TYPE_FLOAT32_VEC4 1071
+#define VRT_TYPE_UFIXED8_VEC4 1039
+#define VRT_TYPE_UFIXED8_RGBA_SRGB 2003
+
+/* dco::Frame (DeviceCopyableObjects.h:2615-2661) */
+typedef struct vrt_frame {
+  int width, height;
+  int color_type;     /* VRT_TYPE_* */
+  void *color;        /* width*height pixels of color_type */
+  float *accum;       /* float4 running mean (stochastic renderers) */
+  float *depth;       /* optional float depth channel (may be NULL) */
+  double *accum64;    /* optional float4-as-double running sum (FP64
+                         long-run reference accumulation; may be NULL) */
+  unsigned long long *counters; /* optional [0] trilinear lookups,
+                                   [1] paths, [2] collisions */
+} vrt_frame;
+
+/* ---------------------------------------------------------------------- */
+/* Context, memory                                                         */
+/* ---------------------------------------------------------------------- */
+typedef struct vrt_context_s *vrt_context;
+
+VRT_API const char *vrt_last_error(void);
+VRT_API int vrt_context_create(int device, vrt_context *out);
+VRT_API int vrt_context_destroy(vrt_context ctx);
+VRT_API int vrt_context_stream(vrt_context ctx, void **hip_stream);
+VRT_API int vrt_synchronize(vrt_context ctx);
+
+VRT_API int vrt_alloc(vrt_context ctx, size_t bytes, void **dptr);
+VRT_API int vrt_free(vrt_context ctx, void *dptr);
+VRT_API int vrt_upload(vrt_context ctx, void *dst, const void *src, size_t bytes);
+VRT_API int vrt_download(vrt_context ctx, void *dst, const void *src, size_t bytes);
+VRT_API int vrt_memset(vrt_context ctx, void *dst, int value, size_t bytes);
+
+/* Event timing on the context stream: record a start/stop pair around
+ * enqueued work and read the elapsed milliseconds. */
+VRT_API int vrt_timer_start(vrt_context ctx);
+VRT_API int vrt_timer_stop(vrt_context ctx, float *ms);
+
+/* ---------------------------------------------------------------------- */
+/* Spatial field + majorant grid                                           */
+/* ---------------------------------------------------------------------- */
+
+/* Bytes needed for the brick-ordered copy of a dims[0]*dims[1]*dims[2]
+ * field (each axis padded to whole 8-voxel bricks). */
+VRT_API size_t vrt_structured_brick_bytes(const int dims[3]);
+
+/* x-fastest linear fp32 voxels (device) -> brick-ordered copy (device).
+ * Replaces the hipArray texture upload of StructuredRegularField::finalize
+ * (StructuredRegularField.cpp:42-76). */
+VRT_API int vrt_structured_brickify(vrt_context ctx, const float *linear,
+                                    const int dims[3], float *bricks);
+
+/* Per-macrocell value ranges (float2 min/max) of a field over a
+ * grid_dims grid in voxel space, conservative for trilinear sampling.
+ * Replaces StructuredRegularField::buildGrid + rasterizeBox
+ * (StructuredRegularField.cpp:95-127, GridAccel.h:13-67), which the
+ * reference skips on GPU builds. */
+VRT_API int vrt_build_value_ranges(vrt_context ctx, const vrt_volume *vol,
+                                   float *ranges);
+
+/* Majorants (max TF opacity over each macrocell's value range).
+ * Replaces GridAccel::computeMaxOpacities (GridAccel.cpp:69-107). */
+VRT_API int vrt_compute_majorants(vrt_context ctx, const vrt_volume *vol,
+                                  const float *ranges, float *majorants);
+
+/* Trilinear field values at n voxel-space points (device float3 array ->
+ * device float array): the sampleField structuredRegular branch
+ * (DeviceCopyableObjects.h:478-481) on the brick layout. */
+VRT_API int vrt_sample_field(vrt_context ctx, const vrt_volume *vol, int n,
+                             const float *points, float *values);
+
+/* Delta tracking of n world-space rays through scene volume 0:
+ * rays = n x {ori.xyz, dir.xyz, tmin, tmax} (device), rng = n LCG states
+ * (device, advanced in place), t_out = hit distance or +inf (device).
+ * The intersect(Ray, Volume) collision sampler of
+ * DeviceCopyableObjects.h:680-796. */
+VRT_API int vrt_delta_track(vrt_context ctx, const vrt_scene *scene, int n,
+                            const float *rays, uint32_t *rng, float *t_out);
+
+/* ---------------------------------------------------------------------- */
+/* Renderers                                                               */
+/* ---------------------------------------------------------------------- */
+
+/* Clear accum (and accum64, depth) of a frame: the accumulation reset of
+ * Frame::renderFrame (frame/Frame.cpp:287-312). */
+VRT_API int vrt_frame_reset(vrt_context ctx, const vrt_frame *frame);
+
+/* 'raycast' renderer, one frame (renderer/Raycast_impl.cpp:245-322). */
+VRT_API int vrt_raycast(vrt_context ctx, const vrt_scene *scene,
+                        const vrt_camera *camera, const vrt_params *params,
+                        const vrt_frame *frame, int frame_id);
+
+/* 'pathtrace' renderer, one frame at accumulation index accum_id
+ * (renderer/Pathtrace_impl.cpp:489-641). */
+VRT_API int vrt_pathtrace(vrt_context ctx, const vrt_scene *scene,
+                          const vrt_camera *camera, const vrt_params *params,
+                          const vrt_frame *frame, int accum_id, int frame_id);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1156 @@
+/*
+ * ORACLE -- test infrastructure, not product code.  See vrt_oracle.h.
+ *
+ * A CPU restatement of anari-visionaray's volume rendering path written
+ * for checking the HIP kernels; it mirrors the reference CPU device's
+ * execution structure (a std::thread pool of hardware_concurrency() threads
+ * working on 64x64 pixel tiles, for_each.h:55-87, SyncContext.cpp:20-21) so
+ * that it also serves as the timed CPU baseline.
+ */
+#include "vrt_oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <thread>
+#include <vector>
+
+namespace orc {
+
+// ---------------------------------------------------------------------------
+// small vector type, templated on the arithmetic precision
+// ---------------------------------------------------------------------------
+template <class R> struct V3 {
+  R x, y, z;
+  V3() : x(0), y(0), z(0) {}
+  V3(R a) : x(a), y(a), z(a) {}
+  V3(R a, R b, R c) : x(a), y(b), z(c) {}
+  R operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+  R &operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+};
+template <class R> inline V3<R> operator+(V3<R> a, V3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class R> inline V3<R> operator-(V3<R> a, V3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class R> inline V3<R> operator*(V3<R> a, V3<R> b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+template <class R> inline V3<R> operator*(V3<R> a, R s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class R> inline V3<R> operator*(R s, V3<R> a) { return {a.x * s, a.y * s, a.z * s}; }
+template <class R> inline V3<R> operator/(V3<R> a, V3<R> b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
+template <class R> inline V3<R> operator-(V3<R> a) { return {-a.x, -a.y, -a.z}; }
+template <class R> inline R dot(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class R> inline V3<R> cross(V3<R> a, V3<R> b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class R> inline R length(V3<R> a) { return std::sqrt(dot(a, a)); }
+template <class R> inline V3<R> normalize(V3<R> a) { return a * (R(1) / length(a)); }
+template <class R> inline R max_element(V3<R> a) { return std::max(a.x, std::max(a.y, a.z)); }
+template <class R> inline R min_element(V3<R> a) { return std::min(a.x, std::min(a.y, a.z)); }
+template <class R> inline V3<R> vabs(V3<R> a) { return {std::fabs(a.x), std::fabs(a.y), std::fabs(a.z)}; }
+template <class R> inline V3<R> from(const float *p) { return {R(p[0]), R(p[1]), R(p[2])}; }
+
+template <class R> struct V4 { R x, y, z, w; };
+
+// ---------------------------------------------------------------------------
+// A1: LCG<4> with TEA seeding (common.h:55-102)
+// ---------------------------------------------------------------------------
+struct Random {
+  uint32_t state;
+  Random() : state(0) {}
+  Random(uint32_t v0_, uint32_t v1_) {
+    // TEA-4 seed, common.h:70-82
+    uint32_t v0 = v0_, v1 = v1_, s0 = 0;
+    for (int n = 0; n < 4; n++) {
+      s0 += 0x9e3779b9u;
+      v0 += ((v1 << 4) + 0xa341316cu) ^ (v1 + s0) ^ ((v1 >> 5) + 0xc8013ea4u);
+      v1 += ((v0 << 4) + 0xad90777du) ^ (v0 + s0) ^ ((v0 >> 5) + 0x7e95761eu);
+    }
+    state = v0;
+  }
+  // common.h:85-91: state = 1664525*state + 1013904223; (state & 0xFFFFFF)/2^24
+  float operator()() {
+    state = 1664525u * state + 1013904223u;
+    return float(state & 0x00FFFFFFu) / float(0x01000000);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Rays, boxes
+// ---------------------------------------------------------------------------
+template <class R> struct Ray {
+  V3<R> ori, dir;
+  R tmin, tmax;
+};
+
+template <class R> struct BoxHit { bool hit; R tnear, tfar; };
+
+// visionaray intersect(ray, aabb) slab test (parity unpinned; textbook slab
+// method: tnear = max of per-axis minima, tfar = min of per-axis maxima,
+// hit = tfar >= tnear && tfar >= 0)
+template <class R>
+inline BoxHit<R> intersect_box(const Ray<R> &ray, V3<R> lo, V3<R> hi) {
+  V3<R> inv(R(1) / ray.dir.x, R(1) / ray.dir.y, R(1) / ray.dir.z);
+  V3<R> t1 = (lo - ray.ori) * inv;
+  V3<R> t2 = (hi - ray.ori) * inv;
+  R tn = std::max(std::min(t1.z, t2.z), std::max(std::min(t1.y, t2.y), std::min(t1.x, t2.x)));
+  R tf = std::min(std::max(t1.z, t2.z), std::min(std::max(t1.y, t2.y), std::max(t1.x, t2.x)));
+  BoxHit<R> h;
+  h.tnear = tn;
+  h.tfar = tf;
+  h.hit = tf >= tn && tf >= R(0);
+  return h;
+}
+
+// powf of VolumeIntegration.h:416-417.  Parity mode (float) evaluates it
+// from IEEE basic operations only, so that CPU and GPU round identically:
+// x = m*2^e, m in [sqrt.5, sqrt2), ln m = 2s(1 + s^2/3 + s^4/5 + s^6/7 +
+// s^8/9), s = (m-1)/(m+1); 2^z = 2^n exp(f ln2), degree-7 Taylor, |f| <= 1/2.
+// Error <= 1.5 (2 + |y log2 x|) ulp against std::pow, pinned in tests.  Oracle mode
+// (double) calls std::pow.
+inline float pow_ieee(float x, float y) {
+  if (y == 0.f || x == 1.f) return 1.f;
+  if (!(x > 0.f)) return y > 0.f ? 0.f : std::numeric_limits<float>::infinity();
+  uint32_t bits;
+  std::memcpy(&bits, &x, 4);
+  int e = int((bits >> 23) & 255u);
+  if (e == 0) return y > 0.f ? 0.f : std::numeric_limits<float>::infinity();
+  if (e == 255) return x;
+  e -= 127;
+  uint32_t mb = (bits & 0x007FFFFFu) | 0x3F800000u;
+  float m;
+  std::memcpy(&m, &mb, 4);
+  if (m > 1.41421356f) {
+    m = m * 0.5f;
+    e += 1;
+  }
+  const float s = (m - 1.f) / (m + 1.f);
+  const float s2 = s * s;
+  float p = 0.111111111f;
+  p = p * s2 + 0.142857143f;
+  p = p * s2 + 0.2f;
+  p = p * s2 + 0.333333333f;
+  p = p * s2 + 1.f;
+  const float l2 = float(e) + (2.f * s * p) * 1.44269504f;
+  const float z = y * l2;
+  if (z < -150.f) return 0.f;
+  if (z > 128.f) return std::numeric_limits<float>::infinity();
+  const float n = std::floor(z + 0.5f);
+  const float t = (z - n) * 0.693147181f;
+  float q = 1.98412698e-4f;
+  q = q * t + 1.38888889e-3f;
+  q = q * t + 8.33333333e-3f;
+  q = q * t + 4.16666667e-2f;
+  q = q * t + 0.166666667f;
+  q = q * t + 0.5f;
+  q = q * t + 1.f;
+  q = q * t + 1.f;
+  int ni = int(n);
+  if (ni < -126) {
+    q = q * 5.87747175e-39f;
+    ni += 127;
+  }
+  uint32_t sb = uint32_t(ni + 127) << 23;
+  float sc;
+  std::memcpy(&sc, &sb, 4);
+  return q * sc;
+}
+inline float opacity_pow(float x, float y) { return pow_ieee(x, y); }
+inline double opacity_pow(double x, double y) { return std::pow(x, y); }
+
+// renderer/common.h:14-18
+template <class R> inline R epsilon_from(V3<R> P, V3<R> dir, R t) {
+  const R ulpEpsilon = R(0x1.fp-18);
+  R m = std::max(max_element(vabs(P)), max_element(vabs(dir)) * t);
+  return m * ulpEpsilon;
+}
+
+// ---------------------------------------------------------------------------
+// Volume: field + TF (A10, A13)
+// ---------------------------------------------------------------------------
+template <class R> struct Volume {
+  const orc_volume *v;
+  V3<R> blo, bhi;      // StructuredRegularField::bounds, :90-93
+  V3<R> S, T;          // voxelSpaceTransform = (scaling(1/size), -min), :37-40
+  R cellSize;          // min(spacing), :40
+  // instance transform
+  R A[9];              // affineInv
+  V3<R> transInv;      // -translation
+  bool has_grid;
+
+  explicit Volume(const orc_volume *vol) : v(vol) {
+    V3<R> o = from<R>(vol->origin), sp = from<R>(vol->spacing);
+    // bounds computed in float like the reference (float3 math)
+    float bl[3], bh[3];
+    for (int i = 0; i < 3; ++i) {
+      bl[i] = vol->origin[i];
+      bh[i] = vol->origin[i] + (float(vol->dims[i]) - 1.f) * vol->spacing[i];
+    }
+    blo = from<R>(bl);
+    bhi = from<R>(bh);
+    S = V3<R>(R(1) / (bhi.x - blo.x), R(1) / (bhi.y - blo.y), R(1) / (bhi.z - blo.z));
+    T = -blo;
+    cellSize = std::min(sp.x, std::min(sp.y, sp.z));
+    (void)o;
+    // invert the 3x3 part of the instance transform
+    const float *m = vol->xfm;
+    double a = m[0], b = m[1], c = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9], k = m[10];
+    double det = a * (e * k - f * h) - b * (d * k - f * g) + c * (d * h - e * g);
+    double inv[9] = {(e * k - f * h) / det, (c * h - b * k) / det, (b * f - c * e) / det,
+                     (f * g - d * k) / det, (a * k - c * g) / det, (c * d - a * f) / det,
+                     (d * h - e * g) / det, (b * g - a * h) / det, (a * e - b * d) / det};
+    for (int i = 0; i < 9; ++i) A[i] = R(inv[i]);
+    transInv = V3<R>(R(-m[3]), R(-m[7]), R(-m[11]));
+    has_grid = vol->majorants != nullptr && vol->grid_dims[0] > 0;
+  }
+
+  V3<R> mulA(V3<R> p) const {
+    return {A[0] * p.x + A[1] * p.y + A[2] * p.z, A[3] * p.x + A[4] * p.y + A[5] * p.z,
+            A[6] * p.x + A[7] * p.y + A[8] * p.z};
+  }
+  // Instance intersect: DeviceCopyableObjects.h:1772-1814
+  Ray<R> to_local(const Ray<R> &r) const {
+    Ray<R> l = r;
+    l.ori = mulA(r.ori + transInv);
+    l.dir = mulA(r.dir);
+    return l;
+  }
+  // SpatialField::pointToVoxelSpace, DeviceCopyableObjects.h:390-395
+  V3<R> point_to_voxel(V3<R> p) const { return S * (p + T); }
+  V3<R> vector_to_voxel(V3<R> d) const { return S * d; }
+
+  R voxel(int x, int y, int z) const {
+    return R(v->voxels[(size_t(z) * v->dims[1] + y) * v->dims[0] + x]);
+  }
+
+  // tex3D(linear, clamp, normalised coordinates): texel centres at
+  // (i+0.5)/N, i.e. texel coordinate p*N - 0.5 (GridAccel.cpp:103 samples
+  // the TF at (i+.5)/N under the same convention)
+  R sample(V3<R> P) const {
+    int n[3] = {v->dims[0], v->dims[1], v->dims[2]};
+    int i0[3], i1[3];
+    R w[3];
+    for (int a = 0; a < 3; ++a) {
+      R c = P[a] * R(n[a]) - R(0.5);
+      c = std::min(std::max(c, R(-1)), R(n[a]));  // keep the int cast defined
+      R fl = std::floor(c);
+      w[a] = c - fl;
+      int i = int(fl);
+      i0[a] = std::min(std::max(i, 0), n[a] - 1);
+      i1[a] = std::min(std::max(i + 1, 0), n[a] - 1);
+    }
+    auto lerp = [](R a, R b, R t) { return a + t * (b - a); };
+    R c00 = lerp(voxel(i0[0], i0[1], i0[2]), voxel(i1[0], i0[1], i0[2]), w[0]);
+    R c10 = lerp(voxel(i0[0], i1[1], i0[2]), voxel(i1[0], i1[1], i0[2]), w[0]);
+    R c01 = lerp(voxel(i0[0], i0[1], i1[2]), voxel(i1[0], i0[1], i1[2]), w[0]);
+    R c11 = lerp(voxel(i0[0], i1[1], i1[2]), voxel(i1[0], i1[1], i1[2]), w[0]);
+    R c0 = lerp(c00, c10, w[1]);
+    R c1 = lerp(c01, c11, w[1]);
+    return lerp(c0, c1, w[2]);
+  }
+
+  // postClassify, DeviceCopyableObjects.h:615-619: normalise by valueRange,
+  // then tex1D(linear, clamp) over the float4 table
+  V4<R> classify(R val) const {
+    R lo = R(v->value_lo), hi = R(v->value_hi);
+    R u = (val - lo) / (hi - lo);
+    int n = v->tf_size;
+    R c = u * R(n) - R(0.5);
+    c = std::min(std::max(c, R(-1)), R(n));
+    R fl = std::floor(c);
+    R w = c - fl;
+    int i = int(fl);
+    int i0 = std::min(std::max(i, 0), n - 1), i1 = std::min(std::max(i + 1, 0), n - 1);
+    const float *t0 = v->tf_rgba + 4 * i0, *t1 = v->tf_rgba + 4 * i1;
+    V4<R> r;
+    r.x = R(t0[0]) + w * (R(t1[0]) - R(t0[0]));
+    r.y = R(t0[1]) + w * (R(t1[1]) - R(t0[1]));
+    r.z = R(t0[2]) + w * (R(t1[2]) - R(t0[2]));
+    r.w = R(t0[3]) + w * (R(t1[3]) - R(t0[3]));
+    return r;
+  }
+
+  R majorant(int cell) const {
+    // GridAccel maxOpacities, or 1 without a grid (DeviceCopyableObjects.h:724)
+    return has_grid ? R(v->majorants[cell]) : R(1);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// A8: dda3 (renderer/DDA.h:14-112), GridAccel-common.h:10-24
+// ---------------------------------------------------------------------------
+template <class R, class F>
+void dda3(Ray<R> ray, const int dims[3], V3<R> lo, V3<R> hi, F &&func) {
+  const R ray_tmin = ray.tmin;
+  ray.ori = ray.ori + ray.dir * ray.tmin;
+  ray.tmin = R(0);
+  ray.tmax -= ray_tmin;
+
+  V3<R> rcp(R(1) / ray.dir.x, R(1) / ray.dir.y, R(1) / ray.dir.z);
+  V3<R> l = (lo - ray.ori) * rcp;
+  V3<R> h = (hi - ray.ori) * rcp;
+  V3<R> tnear(std::min(l.x, h.x), std::min(l.y, h.y), std::min(l.z, h.z));
+  V3<R> tfar(std::max(l.x, h.x), std::max(l.y, h.y), std::max(l.z, h.z));
+
+  // projectOnGrid, GridAccel-common.h:18-24
+  int cell[3];
+  for (int a = 0; a < 3; ++a) {
+    R v01 = (ray.ori[a] - lo[a]) / (hi[a] - lo[a]);
+    R c = v01 * R(dims[a]);
+    c = std::min(std::max(c, R(-1)), R(dims[a]));
+    int ci = int(c);  // truncation as in vec3i(V01*dims)
+    cell[a] = std::min(std::max(ci, 0), dims[a] - 1);
+  }
+
+  R dist[3], tnext[3];
+  int step[3], stop[3];
+  for (int a = 0; a < 3; ++a) {
+    R d = ray.dir[a];
+    step[a] = d > R(0) ? 1 : -1;
+    stop[a] = d > R(0) ? dims[a] : -1;
+    if (d == R(0)) {
+      // DDA.h:32-40 sets tnear = FLT_MAX on zero components: the axis never
+      // becomes the closest boundary
+      dist[a] = R(0);
+      tnext[a] = std::numeric_limits<R>::infinity();
+      continue;
+    }
+    dist[a] = std::max(R(0), (tfar[a] - tnear[a]) / R(dims[a]));
+    tnext[a] = d > R(0) ? tnear[a] + R(cell[a] + 1) * dist[a]
+                        : tnear[a] + R(dims[a] - cell[a]) * dist[a];
+  }
+
+  R t0 = R(0);
+  while (true) {
+    R tc = std::min(tnext[0], std::min(tnext[1], tnext[2]));
+    R t1 = std::min(tc, ray.tmax);
+    int li = (cell[2] * dims[1] + cell[1]) * dims[0] + cell[0];
+    if (!func(li, ray_tmin + t0, ray_tmin + t1)) return;
+    // DDA.h:88-108: advance every axis whose boundary equals the closest
+    bool out = false;
+    for (int a = 0; a < 3 && !out; ++a) {
+      if (tnext[a] == tc) {
+        tnext[a] += dist[a];
+        cell[a] += step[a];
+        if (cell[a] == stop[a]) out = true;
+      }
+    }
+    if (out) break;
+    t0 = t1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// A7: intersect(Ray, Volume) -- box clip, voxel space, DDA + Woodcock
+// (DeviceCopyableObjects.h:680-796)
+// ---------------------------------------------------------------------------
+template <class R> struct VolHit {
+  bool hit = false;
+  R t = std::numeric_limits<R>::max();
+  V3<R> albedo;
+  R Tr = R(1);
+};
+
+struct Counter { uint64_t lookups = 0; };
+
+template <class R>
+VolHit<R> delta_track(const Volume<R> &vol, Ray<R> ray, Random &rnd, Counter *cnt) {
+  VolHit<R> out;
+  BoxHit<R> bh = intersect_box(ray, vol.blo, vol.bhi);
+  if (!bh.hit) return out;
+
+  ray.tmin = std::max(ray.tmin, bh.tnear);
+  ray.tmax = std::min(ray.tmax, bh.tfar);
+  ray.ori = vol.point_to_voxel(ray.ori);
+  ray.dir = vol.vector_to_voxel(ray.dir);
+  const R dt_scale = length(ray.dir);
+  ray.dir = normalize(ray.dir);
+  ray.tmin = ray.tmin * dt_scale;
+  ray.tmax = ray.tmax * dt_scale;
+  R unitDistance = R(vol.v->unit_distance) * dt_scale;
+  R invUnitDistance = R(1) / unitDistance;
+
+  bool hit = false;
+  R hit_t = ray.tmax;
+  V3<R> albedo;
+
+  auto woodcock = [&](int leafID, R t0, R t1) -> bool {
+    const R majorant = leafID >= 0 ? vol.majorant(leafID) : R(1);
+    R t = t0;
+    while (true) {
+      if (majorant <= R(0)) break;
+      t -= std::log(R(1) - R(rnd())) / (majorant * invUnitDistance);
+      if (t >= t1) break;
+      V3<R> P = ray.ori + ray.dir * t;
+      R v = vol.sample(P);
+      if (cnt) cnt->lookups++;
+      V4<R> s = vol.classify(v);
+      R u = R(rnd());
+      if (s.w >= u * majorant) {
+        albedo = V3<R>(s.x, s.y, s.z);
+        hit = true;
+        hit_t = t;
+        return false;
+      }
+    }
+    return true;
+  };
+
+  if (vol.has_grid) {
+    int dims[3] = {vol.v->grid_dims[0], vol.v->grid_dims[1], vol.v->grid_dims[2]};
+    // gridBounds = voxel-space image of the world bounds = [0,1]^3
+    // (StructuredRegularField.cpp:103-107)
+    dda3(ray, dims, V3<R>(R(0)), V3<R>(R(1)), woodcock);
+  } else {
+    woodcock(-1, ray.tmin, ray.tmax);
+  }
+
+  if (hit) {
+    out.hit = true;
+    out.t = hit_t / dt_scale;
+    out.albedo = albedo;
+    out.Tr = R(0);
+  }
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// Camera (parity unpinned: visionaray thin_lens_camera is not vendored).
+// ANARI perspective camera: position, direction, up, fovy, aspect,
+// imageRegion; pixel (x, y) samples screen point ((x+.5)/W, (y+.5)/H), row 0
+// at the bottom of the image.  Lens radius 0 consumes no random numbers.
+// ---------------------------------------------------------------------------
+template <class R> struct Camera {
+  V3<R> pos, U, V, W;
+  R region[4];
+  R shutter[2];
+  explicit Camera(const orc_camera *c) {
+    V3<R> d = normalize(from<R>(c->dir));
+    V3<R> up = normalize(from<R>(c->up));
+    R th = std::tan(R(c->fovy) * R(0.5));
+    V3<R> right = normalize(cross(d, up));
+    V3<R> upv = cross(right, d);
+    pos = from<R>(c->pos);
+    U = right * (th * R(c->aspect));
+    V = upv * th;
+    W = d;
+    for (int i = 0; i < 4; ++i) region[i] = R(c->region[i]);
+    shutter[0] = R(c->shutter[0]);
+    shutter[1] = R(c->shutter[1]);
+  }
+  Ray<R> primary(R x, R y, R width, R height) const {
+    R sx = (x + R(0.5)) / width, sy = (y + R(0.5)) / height;
+    sx = (R(1) - sx) * region[0] + sx * region[2];
+    sy = (R(1) - sy) * region[1] + sy * region[3];
+    V3<R> d = W + U * (R(2) * sx - R(1)) + V * (R(2) * sy - R(1));
+    Ray<R> r;
+    r.ori = pos;
+    r.dir = normalize(d);
+    r.tmin = R(0);
+    r.tmax = std::numeric_limits<float>::max();
+    return r;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Sampling helpers (parity unpinned; PBRT definitions)
+// ---------------------------------------------------------------------------
+template <class R> inline V3<R> uniform_sample_sphere(R u1, R u2) {
+  R z = R(1) - R(2) * u1;
+  R r = std::sqrt(std::max(R(0), R(1) - z * z));
+  R phi = R(2 * M_PI) * u2;
+  return V3<R>(r * std::cos(phi), r * std::sin(phi), z);
+}
+template <class R> inline V3<R> cosine_sample_hemisphere(R u1, R u2) {
+  // Shirley-Chiu concentric disk, lifted to the hemisphere
+  R sx = R(2) * u1 - R(1), sy = R(2) * u2 - R(1);
+  R dx = 0, dy = 0;
+  if (!(sx == R(0) && sy == R(0))) {
+    R r, theta;
+    if (std::fabs(sx) > std::fabs(sy)) {
+      r = sx;
+      theta = R(M_PI / 4) * (sy / sx);
+    } else {
+      r = sy;
+      theta = R(M_PI / 2) - R(M_PI / 4) * (sx / sy);
+    }
+    dx = r * std::cos(theta);
+    dy = r * std::sin(theta);
+  }
+  R z = std::sqrt(std::max(R(0), R(1) - dx * dx - dy * dy));
+  return V3<R>(dx, dy, z);
+}
+// ONB as in the matte sampler, renderer/common.h:1384-1386
+template <class R> inline void make_onb(V3<R> w, V3<R> &u, V3<R> &v) {
+  v = std::fabs(w.x) > std::fabs(w.y) ? normalize(V3<R>(-w.z, R(0), w.x))
+                                      : normalize(V3<R>(R(0), w.z, -w.y));
+  u = cross(v, w);
+}
+// faceforward(n, i, nref): n flipped into the hemisphere of i
+template <class R> inline V3<R> faceforward(V3<R> n, V3<R> i, V3<R> nref) {
+  return dot(i, nref) < R(0) ? -n : n;
+}
+
+// ---------------------------------------------------------------------------
+// Scene wrapper
+// ---------------------------------------------------------------------------
+template <class R> struct Scene {
+  std::vector<Volume<R>> vols;
+  const orc_scene *s;
+  explicit Scene(const orc_scene *sc) : s(sc) {
+    for (int i = 0; i < sc->num_volumes; ++i) vols.emplace_back(&sc->volumes[i]);
+  }
+  bool empty() const { return s->num_volumes == 0 && s->num_meshes == 0; }
+};
+
+template <class R> struct LightSample {
+  V3<R> intensity, dir;
+  R pdf = R(0), dist = R(0), dist2 = R(0);
+};
+
+// sampleLight (renderer/common.h:1432-1466) for directional lights;
+// visionaray directional_light::sample with angularDiameter 0 returns the
+// stored direction (Directional.cpp:33 stores -direction), pdf 1, intensity
+// color*irradiance, distance FLT_MAX, and draws no random numbers
+// (parity unpinned).
+template <class R> LightSample<R> sample_light(const orc_light &L, V3<R> /*hitPos*/, Random &) {
+  LightSample<R> ls;
+  V3<R> d = normalize(from<R>(L.dir));
+  ls.dir = -d;
+  ls.dist = R(std::numeric_limits<float>::max());
+  ls.intensity = from<R>(L.color) * R(L.intensity);
+  ls.pdf = R(1);
+  ls.dist2 = R(1);  // infinite lights are not attenuated (common.h:1458-1461)
+  return ls;
+}
+
+template <class R> struct HitRec {
+  // surface part reserved for triangle meshes
+  bool surf_hit = false;
+  R surf_t = std::numeric_limits<R>::max();
+  VolHit<R> vol;
+  int vol_inst = -1;
+  bool light_hit = false;
+  R light_t = std::numeric_limits<R>::max();
+  bool light_visible = false;
+  bool hit = false, volumeHit = false, lightHit = false;
+};
+
+template <class R>
+HitRec<R> intersect_all(const Scene<R> &sc, const Ray<R> &ray, Random &rnd, Counter *cnt) {
+  HitRec<R> hr;
+  // intersectSurfaces<1> (no meshes in the volume configurations)
+  // intersectLights: directional lights are never hit (common.h:1566-1603)
+  // sampleFreeFlightDistanceAllVolumes -> intersectVolumes, closest wins
+  for (size_t i = 0; i < sc.vols.size(); ++i) {
+    Ray<R> lr = sc.vols[i].to_local(ray);
+    VolHit<R> vh = delta_track(sc.vols[i], lr, rnd, cnt);
+    if (vh.hit && vh.t < hr.vol.t) {
+      hr.vol = vh;
+      hr.vol_inst = int(i);
+    }
+  }
+  hr.hit = hr.surf_hit || hr.vol.hit || hr.light_hit;
+  hr.lightHit = hr.light_hit && (!hr.surf_hit || hr.light_t <= hr.surf_t) &&
+                (!hr.vol.hit || hr.light_t <= hr.vol.t);
+  hr.volumeHit = hr.vol.hit && (!hr.surf_hit || hr.vol.t < hr.surf_t) &&
+                 (!hr.light_hit || hr.vol.t < hr.light_t);
+  return hr;
+}
+
+// ---------------------------------------------------------------------------
+// A16: path tracer (renderer/Pathtrace_impl.cpp)
+// ---------------------------------------------------------------------------
+enum RayType { Radiance, Shadow, AO, Miss, None };
+
+template <class R> struct ShadeState {
+  V3<R> baseColor, shadedColor, hitPosOff, sn;
+  int aoSamples = 0;
+  R aoWeights = 0, aoCount = 0;
+  // bsdfSample
+  V3<R> bs_dir, bs_f;
+  R bs_pdf = 0, bs_cosT = 0;
+  LightSample<R> ls;
+  R misWeightNEE = 0;
+  RayType nextType = None;
+  Ray<R> next;
+  R vis_light = 1, vis_ao = 1;
+};
+
+template <class R>
+void shade(Random &srnd, const Ray<R> &ray, RayType rayType, const Scene<R> &sc,
+           const orc_params *P, const HitRec<R> &hitRec, ShadeState<R> &st) {
+  st.nextType = Miss;
+  if (rayType == Radiance) {
+    st.baseColor = V3<R>(R(0));
+    st.vis_light = R(1);
+    st.vis_ao = R(1);
+    st.aoSamples = 0;
+    st.aoWeights = R(0);
+    st.aoCount = R(0);
+    if (!hitRec.hit) {
+      st.shadedColor = V3<R>(R(0));
+      st.nextType = Miss;
+      return;
+    }
+    // (light hits only occur with quad lights; not in volume scenes)
+    V3<R> hitPos, gn, sn, color;
+    R eps;
+    V3<R> viewDir = -normalize(ray.dir);
+    if (hitRec.volumeHit) {
+      // Pathtrace_impl.cpp:203-233
+      hitPos = ray.ori + ray.dir * hitRec.vol.t;
+      eps = epsilon_from(hitPos, ray.dir, hitRec.vol.t);
+      gn = V3<R>(R(0));
+      if (P->ambient_samples > 0 && length(gn) < R(1e-3)) {
+        R u1 = R(srnd()), u2 = R(srnd());
+        gn = uniform_sample_sphere(u1, u2);
+      }
+      sn = gn;
+      // identity normal transform for unscaled instances (getNormalTransform)
+      sn = faceforward(sn, viewDir, gn);
+      gn = faceforward(gn, viewDir, gn);
+      color = hitRec.vol.albedo;
+    } else {
+      // surfaces: not present in the volume configurations
+      hitPos = ray.ori + ray.dir * hitRec.surf_t;
+      eps = epsilon_from(hitPos, ray.dir, hitRec.surf_t);
+      sn = gn = V3<R>(R(0), R(0), R(1));
+      color = V3<R>(R(0));
+    }
+    st.sn = sn;
+    st.hitPosOff = hitPos + sn * eps;
+
+    int lightID = -1;
+    const int numLights = sc.s->num_lights;
+    if (numLights > 0) {
+      // uniformSampleOneLight, renderer/common.h:165-169
+      int which = int(srnd() * float(numLights));
+      if (which == numLights) which = 0;
+      lightID = which;
+      st.ls = sample_light<R>(sc.s->lights[lightID], hitPos, srnd);
+    }
+    auto safe_rcp = [](R f) { return f > R(0) ? R(1) / f : R(0); };
+    V3<R> lightIntensity = st.ls.intensity * safe_rcp(st.ls.dist2);
+    if (hitRec.volumeHit) {
+      // Pathtrace_impl.cpp:330-339: no phase factor, no light count
+      st.shadedColor = hitRec.vol.albedo * lightIntensity * safe_rcp(st.ls.pdf) *
+                       safe_rcp(st.ls.dist2);
+      R u1 = R(srnd()), u2 = R(srnd());
+      st.bs_dir = uniform_sample_sphere(u1, u2);
+      st.bs_f = hitRec.vol.albedo;
+      st.bs_pdf = R(1);
+      st.bs_cosT = R(1);
+    }
+    st.misWeightNEE = R(0);
+    if (numLights > 0) st.misWeightNEE = R(1);  // delta light (:377-379)
+    st.baseColor = color;
+  }
+
+  // state machine, Pathtrace_impl.cpp:401-486
+  auto prepShadow = [&]() {
+    st.next.ori = st.hitPosOff;
+    st.next.dir = normalize(st.ls.dir);
+    st.next.tmin = R(0);
+    st.next.tmax = st.ls.dist;
+    st.nextType = Shadow;
+  };
+  auto prepAO = [&]() {
+    V3<R> u, v, w = st.sn;
+    make_onb(w, u, v);
+    R u1 = R(srnd()), u2 = R(srnd());
+    V3<R> sp = cosine_sample_hemisphere(u1, u2);
+    st.next.ori = st.hitPosOff;
+    st.next.dir = normalize(u * sp.x + v * sp.y + w * sp.z);
+    st.next.tmin = R(0);
+    st.next.tmax = R(P->occlusion_distance);
+    st.nextType = AO;
+  };
+  auto prepRadiance = [&]() {
+    st.next.ori = st.hitPosOff;
+    st.next.dir = normalize(st.bs_dir);
+    st.next.tmin = R(0);
+    st.next.tmax = R(1e31f);
+    st.nextType = Radiance;
+  };
+
+  if (rayType == Radiance) {
+    if (st.ls.pdf >= R(0)) { prepShadow(); return; }
+    if (st.aoSamples < P->ambient_samples) { prepAO(); return; }
+    if (st.bs_pdf >= R(0)) { prepRadiance(); return; }
+    return;
+  } else if (rayType == Shadow) {
+    int surfV = hitRec.surf_hit ? 0 : 1;
+    int volV = hitRec.volumeHit ? 0 : 1;
+    st.vis_light = R(surfV * volV) * hitRec.vol.Tr;
+    if (st.aoSamples < P->ambient_samples) { prepAO(); return; }
+    if (st.bs_pdf >= R(0)) { prepRadiance(); return; }
+    return;
+  } else if (rayType == AO) {
+    st.aoSamples++;
+    R weight = std::max(R(0), dot(ray.dir, st.sn));
+    st.aoWeights += weight;
+    if (weight > R(0) && hitRec.hit) st.aoCount += weight;
+    if (st.aoSamples < P->ambient_samples) { prepAO(); return; }
+    R aoV = R(0);
+    if (st.aoWeights > R(0)) aoV = R(1) - st.aoCount / st.aoWeights;
+    st.vis_ao *= aoV;
+    if (st.bs_pdf >= R(0)) { prepRadiance(); return; }
+    return;
+  }
+}
+
+template <class R>
+V4<R> pathtrace_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params *P,
+                      int x, int y, int W, int H, int accumID, Counter *cnt) {
+  // Pathtrace_impl.cpp:510-513: rng seeded by (pixelID, accumID); the
+  // ScreenSample holds a copy, RR draws from the un-advanced original
+  uint32_t pixelID = uint32_t(x + W * y);
+  Random rng(pixelID, uint32_t(accumID));
+  Random srnd = rng;
+  V4<R> accum{0, 0, 0, 0};
+  const int spp = P->pixel_samples;
+  for (int sampleID = 0; sampleID < spp; ++sampleID) {
+    R xf = R(x), yf = R(y);
+    R jx = R(srnd()) - R(0.5);
+    R jy = R(srnd()) - R(0.5);
+    xf += jx;
+    yf += jy;
+    Ray<R> ray = cam.primary(xf, yf, R(W), R(H));
+    (void)srnd();  // shutter time lerp, DeviceCopyableObjects.h:2597
+    V4<R> color{R(P->bg[0]), R(P->bg[1]), R(P->bg[2]), R(P->bg[3])};
+    if (!sc.empty()) {
+      bool firstHit = false, firstLightInvisible = false;
+      ShadeState<R> st;
+      RayType rayType = Radiance;
+      V3<R> throughput(R(1)), intensity(R(0));
+      for (unsigned passID = 0, bounceID = 0;; ++passID) {
+        HitRec<R> hitRec = intersect_all(sc, ray, srnd, cnt);
+        shade(srnd, ray, rayType, sc, P, hitRec, st);
+        if (passID == 0 && bounceID == 0) {
+          firstHit = hitRec.hit;
+          firstLightInvisible = hitRec.lightHit && !hitRec.light_visible;
+        }
+        ray = st.next;
+        rayType = st.nextType;
+        if (rayType == Miss || rayType == Radiance) {
+          V3<R> direct = st.shadedColor * st.vis_light;
+          V3<R> ambient = st.baseColor * from<R>(P->ambient_color) *
+                          R(P->ambient_radiance) * st.vis_ao;
+          intensity = intensity + throughput * (direct * st.misWeightNEE);
+          intensity = intensity + throughput * ambient;
+          R rcp = st.bs_pdf > R(0) ? R(1) / st.bs_pdf : R(0);
+          throughput = throughput * (st.bs_f * (st.bs_cosT * rcp));
+          bounceID++;
+        }
+        if (rayType == Miss || int(bounceID) > P->max_bounce) break;
+        if (bounceID > 1) {  // Russian roulette, :594-599
+          R prob = max_element(throughput);
+          if (R(rng()) > prob) break;
+          throughput = throughput * (R(1) / prob);
+        }
+      }
+      if (firstHit && !firstLightInvisible) color = V4<R>{intensity.x, intensity.y, intensity.z, R(1)};
+    }
+    accum.x += color.x;
+    accum.y += color.y;
+    accum.z += color.z;
+    accum.w += color.w;
+  }
+  R inv = R(1) / R(spp);
+  return V4<R>{accum.x * inv, accum.y * inv, accum.z * inv, accum.w * inv};
+}
+
+// ---------------------------------------------------------------------------
+// A14/A15: raycast (Raycast_impl.cpp, VolumeIntegration.h:374-470)
+// ---------------------------------------------------------------------------
+template <class R>
+V4<R> raycast_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params *P,
+                    int x, int y, int W, int H) {
+  V4<R> bg{R(P->bg[0]), R(P->bg[1]), R(P->bg[2]), R(P->bg[3])};
+  if (sc.empty()) return bg;
+  Ray<R> ray = cam.primary(R(x), R(y), R(W), R(H));
+  // over(float4(0), bg) = bg (no surfaces), Raycast_impl.cpp:170-173
+  V4<R> result = bg;
+  // intersectVolumeBounds: closest volume box (DeviceCopyableObjects.h:695-709)
+  int best = -1;
+  R bestT = std::numeric_limits<R>::max();
+  for (size_t i = 0; i < sc.vols.size(); ++i) {
+    Ray<R> lr = sc.vols[i].to_local(ray);
+    BoxHit<R> bh = intersect_box(lr, sc.vols[i].blo, sc.vols[i].bhi);
+    if (!bh.hit) continue;
+    R t = std::max(lr.tmin, bh.tnear);
+    if (t < bestT) {
+      bestT = t;
+      best = int(i);
+    }
+  }
+  if (best < 0) return result;
+  const Volume<R> &vol = sc.vols[best];
+  Ray<R> lr = vol.to_local(ray);
+  BoxHit<R> bh = intersect_box(lr, vol.blo, vol.bhi);
+  lr.tmin = std::max(lr.tmin, bh.tnear);
+  lr.tmax = std::min(lr.tmax, bh.tfar);
+  V3<R> ori = vol.point_to_voxel(lr.ori);
+  V3<R> dir = vol.vector_to_voxel(lr.dir);  // not renormalised
+  const R samplingRateInv = P->volume_sampling_rate > 0.f ? R(1) / R(P->volume_sampling_rate) : R(0);
+  const R dt = vol.cellSize * samplingRateInv;
+  const R expo = dt / R(vol.v->unit_distance);
+  V3<R> color(R(0));
+  R alpha = R(0), transmittance = R(1);
+  for (R t = lr.tmin; t <= lr.tmax && alpha < R(0.99f); t += dt) {
+    V3<R> p = ori + dir * t;
+    R v = vol.sample(p);
+    V4<R> s = vol.classify(v);
+    R stepT = opacity_pow(R(1) - s.w, expo);
+    color = color + V3<R>(s.x, s.y, s.z) * (transmittance * (R(1) - stepT));
+    alpha += transmittance * (R(1) - stepT);
+    transmittance *= stepT;
+  }
+  // over(alpha*float4(color,1), result) -- Raycast_impl.cpp:232
+  V4<R> A{alpha * color.x, alpha * color.y, alpha * color.z, alpha};
+  R k = R(1) - A.w;
+  return V4<R>{A.x + k * result.x, A.y + k * result.y, A.z + k * result.z, A.w + k * result.w};
+}
+
+// ---------------------------------------------------------------------------
+// thread pool over 64x64 tiles (for_each.h:76-84)
+// ---------------------------------------------------------------------------
+template <class F>
+void parallel_tiles(int W, int H, int threads, F &&fn) {
+  const int T = 64;
+  int tx = (W + T - 1) / T, ty = (H + T - 1) / T;
+  int ntiles = tx * ty;
+  if (threads <= 0) threads = int(std::max(1u, std::thread::hardware_concurrency()));
+  std::atomic<int> next(0);
+  auto work = [&]() {
+    while (true) {
+      int t = next.fetch_add(1);
+      if (t >= ntiles) break;
+      int bx = (t % tx) * T, by = (t / tx) * T;
+      for (int y = by; y < std::min(by + T, H); ++y)
+        for (int x = bx; x < std::min(bx + T, W); ++x) fn(x, y);
+    }
+  };
+  if (threads == 1) {
+    work();
+    return;
+  }
+  std::vector<std::thread> pool;
+  for (int i = 0; i < threads; ++i) pool.emplace_back(work);
+  for (auto &th : pool) th.join();
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+// ===========================================================================
+// C API
+// ===========================================================================
+extern "C" {
+
+uint32_t orc_tea_seed(uint32_t v0, uint32_t v1) { return Random(v0, v1).state; }
+
+void orc_lcg_stream(uint32_t s0, uint32_t s1, int n, float *out) {
+  Random r(s0, s1);
+  for (int i = 0; i < n; ++i) out[i] = r();
+}
+
+int orc_tf_build(const float *colors, int numColors, int nch, const float *opac,
+                 int numOpac, const float uni[4], float *out) {
+  // TransferFunction1D.cpp:74-111
+  size_t nC = colors ? size_t(numColors) : 1;
+  size_t nO = opac ? size_t(numOpac) : 1;
+  size_t tfSize = std::max(nC, nO);
+  if (!out) return int(tfSize);
+  for (size_t i = 0; i < tfSize; ++i) {
+    float colorPos = tfSize > 1 ? (float(i) / float(tfSize - 1)) * float(nC - 1) : 0.f;
+    float colorFrac = colorPos - std::floor(colorPos);
+    float c0[4] = {uni[0], uni[1], uni[2], uni[3]};
+    float c1[4] = {uni[0], uni[1], uni[2], uni[3]};
+    if (colors) {
+      int a = int(std::floor(colorPos)), b = int(std::ceil(colorPos));
+      for (int k = 0; k < nch; ++k) {
+        c0[k] = colors[a * nch + k];
+        c1[k] = colors[b * nch + k];
+      }
+    }
+    float c[4];
+    for (int k = 0; k < 4; ++k) c[k] = c0[k] + colorFrac * (c1[k] - c0[k]);  // lerp_r
+    if (opac) {
+      float alphaPos = tfSize > 1 ? (float(i) / float(tfSize - 1)) * float(nO - 1) : 0.f;
+      float alphaFrac = alphaPos - std::floor(alphaPos);
+      float a0 = opac[int(std::floor(alphaPos))], a1 = opac[int(std::ceil(alphaPos))];
+      c[3] *= a0 + alphaFrac * (a1 - a0);
+    }
+    for (int k = 0; k < 4; ++k) out[4 * i + k] = c[k];
+  }
+  return int(tfSize);
+}
+
+void orc_sample_field(const orc_volume *v, int n, const float *p, float *out) {
+  Volume<float> vol(v);
+  for (int i = 0; i < n; ++i)
+    out[i] = vol.sample(V3<float>(p[3 * i], p[3 * i + 1], p[3 * i + 2]));
+}
+
+void orc_post_classify(const orc_volume *v, int n, const float *vals, float *out) {
+  Volume<float> vol(v);
+  for (int i = 0; i < n; ++i) {
+    V4<float> c = vol.classify(vals[i]);
+    out[4 * i] = c.x;
+    out[4 * i + 1] = c.y;
+    out[4 * i + 2] = c.z;
+    out[4 * i + 3] = c.w;
+  }
+}
+
+// Conservative majorant grid over voxel space [0,1]^3.  For macrocell m on
+// an axis of G cells the samples inside it have texel coordinates
+// c in [m*N/G - .5, (m+1)*N/G - .5] and read voxels floor(c) .. floor(c)+1,
+// clamped; the cell's value range is the min/max over that voxel block.  The
+// TF taps that values in [vmin,vmax] can read are floor(u*n-.5) ..
+// floor(u*n-.5)+1; the majorant is the max table alpha over them (the
+// reference samples the table at texel centres the same way,
+// GridAccel.cpp:69-107).
+void orc_build_majorants(const orc_volume *v, int gx, int gy, int gz, float *out) {
+  int G[3] = {gx, gy, gz};
+  const int *N = v->dims;
+  auto range = [&](int a, int m, int &lo, int &hi) {
+    double c0 = double(m) * N[a] / G[a] - 0.5;
+    double c1 = double(m + 1) * N[a] / G[a] - 0.5;
+    lo = std::max(0, int(std::floor(c0)));
+    hi = std::min(N[a] - 1, int(std::floor(c1)) + 1);
+  };
+  int n = v->tf_size;
+  for (int mz = 0; mz < gz; ++mz)
+    for (int my = 0; my < gy; ++my)
+      for (int mx = 0; mx < gx; ++mx) {
+        int l[3], h[3];
+        range(0, mx, l[0], h[0]);
+        range(1, my, l[1], h[1]);
+        range(2, mz, l[2], h[2]);
+        float vmin = FLT_MAX, vmax = -FLT_MAX;
+        for (int z = l[2]; z <= h[2]; ++z)
+          for (int y = l[1]; y <= h[1]; ++y)
+            for (int x = l[0]; x <= h[0]; ++x) {
+              float val = v->voxels[(size_t(z) * N[1] + y) * N[0] + x];
+              vmin = std::min(vmin, val);
+              vmax = std::max(vmax, val);
+            }
+        float u0 = (vmin - v->value_lo) / (v->value_hi - v->value_lo);
+        float u1 = (vmax - v->value_lo) / (v->value_hi - v->value_lo);
+        float c0 = std::min(std::max(u0 * n - 0.5f, -1.f), float(n));
+        float c1 = std::min(std::max(u1 * n - 0.5f, -1.f), float(n));
+        int lo = std::min(std::max(int(std::floor(c0)), 0), n - 1);
+        int hi = std::min(std::max(int(std::floor(c1)) + 1, 0), n - 1);
+        float m = 0.f;
+        for (int i = lo; i <= hi; ++i) m = std::max(m, v->tf_rgba[4 * i + 3]);
+        out[(size_t(mz) * gy + my) * gx + mx] = m;
+      }
+}
+
+// Restatement of the reference CPU grid build for comparison:
+// StructuredRegularField::buildGrid (:95-127) -> rasterizeBox (GridAccel.h:51-67)
+// -> computeMaxOpacities (GridAccel.cpp:69-107).
+void orc_build_majorants_reference(const orc_volume *v, int gx, int gy, int gz, float *out) {
+  Volume<float> vol(v);
+  int G[3] = {gx, gy, gz};
+  size_t nmc = size_t(gx) * gy * gz;
+  std::vector<float> vmin(nmc, FLT_MAX), vmax(nmc, -FLT_MAX);
+  auto project = [&](V3<float> P, int *c) {
+    for (int a = 0; a < 3; ++a) {
+      float v01 = (P[a] - vol.blo[a]) / (vol.bhi[a] - vol.blo[a]);
+      int ci = int(v01 * float(G[a]));
+      c[a] = std::min(std::max(ci, 0), G[a] - 1);
+    }
+  };
+  for (int z = 0; z < v->dims[2]; ++z)
+    for (int y = 0; y < v->dims[1]; ++y)
+      for (int x = 0; x < v->dims[0]; ++x) {
+        V3<float> P(v->origin[0] + x * v->spacing[0], v->origin[1] + y * v->spacing[1],
+                    v->origin[2] + z * v->spacing[2]);
+        float value = vol.sample(vol.point_to_voxel(P));
+        V3<float> Q(v->origin[0] + (x + 1) * v->spacing[0], v->origin[1] + (y + 1) * v->spacing[1],
+                    v->origin[2] + (z + 1) * v->spacing[2]);
+        int lo[3], hi[3];
+        project(P, lo);
+        project(Q, hi);
+        for (int mz = lo[2]; mz <= hi[2]; ++mz)
+          for (int my = lo[1]; my <= hi[1]; ++my)
+            for (int mx = lo[0]; mx <= hi[0]; ++mx) {
+              size_t id = (size_t(mz) * gy + my) * gx + mx;
+              vmin[id] = std::min(vmin[id], value);
+              vmax[id] = std::max(vmax[id], value);
+            }
+      }
+  int n = v->tf_size;
+  for (size_t id = 0; id < nmc; ++id) {
+    if (vmax[id] < vmin[id]) { out[id] = 0.f; continue; }
+    float lo = (vmin[id] - v->value_lo) / (v->value_hi - v->value_lo);
+    float hi = (vmax[id] - v->value_lo) / (v->value_hi - v->value_lo);
+    int l = std::min(std::max(int(lo * (n - 1)), 0), n - 1);
+    int h = std::min(std::max(int(hi * (n - 1)) + 1, 0), n - 1);
+    float m = 0.f;
+    for (int i = l; i <= h; ++i) m = std::max(m, v->tf_rgba[4 * i + 3]);
+    out[id] = m;
+  }
+}
+
+int orc_dda_cells(const float ori[3], const float dir[3], float tmin, float tmax,
+                  const int dims[3], const float lo[3], const float hi[3], int max_cells,
+                  int *cells, float *t0s, float *t1s) {
+  Ray<float> r;
+  r.ori = from<float>(ori);
+  r.dir = from<float>(dir);
+  r.tmin = tmin;
+  r.tmax = tmax;
+  int n = 0;
+  dda3(r, dims, from<float>(lo), from<float>(hi), [&](int c, float a, float b) {
+    if (n >= max_cells) return false;
+    cells[n] = c;
+    t0s[n] = a;
+    t1s[n] = b;
+    ++n;
+    return true;
+  });
+  return n;
+}
+
+int orc_delta_track(const orc_volume *v, const float ori[3], const float dir[3], float tmin,
+                    float tmax, uint32_t *state, float *t_out, int fp64) {
+  Random r;
+  r.state = *state;
+  int hit;
+  if (fp64) {
+    Volume<double> vol(v);
+    Ray<double> ray{from<double>(ori), from<double>(dir), tmin, tmax};
+    VolHit<double> h = delta_track(vol, vol.to_local(ray), r, nullptr);
+    hit = h.hit;
+    *t_out = float(h.t);
+  } else {
+    Volume<float> vol(v);
+    Ray<float> ray{from<float>(ori), from<float>(dir), tmin, tmax};
+    VolHit<float> h = delta_track(vol, vol.to_local(ray), r, nullptr);
+    hit = h.hit;
+    *t_out = h.t;
+  }
+  *state = r.state;
+  return hit;
+}
+
+void orc_camera_ray(const orc_camera *c, float x, float y, int W, int H, float *ori, float *dir) {
+  Camera<float> cam(c);
+  Ray<float> r = cam.primary(x, y, float(W), float(H));
+  for (int i = 0; i < 3; ++i) {
+    ori[i] = r.ori[i];
+    dir[i] = r.dir[i];
+  }
+}
+
+void orc_raycast(const orc_scene *scene, const orc_camera *c, const orc_params *P, int W,
+                 int H, float *out, int threads) {
+  Scene<float> sc(scene);
+  Camera<float> cam(c);
+  parallel_tiles(W, H, threads, [&](int x, int y) {
+    V4<float> s = raycast_pixel(sc, cam, P, x, y, W, H);
+    size_t i = size_t(y) * W + x;
+    out[4 * i] = s.x;
+    out[4 * i + 1] = s.y;
+    out[4 * i + 2] = s.z;
+    out[4 * i + 3] = s.w;
+  });
+}
+
+}  // extern "C"
+
+namespace orc {
+template <class R>
+static void pt_frame(const orc_scene *scene, const orc_camera *c, const orc_params *P, int W,
+                     int H, int accumID, int stride, int y0, int y1, float *out, int threads,
+                     uint64_t *lookups) {
+  Scene<R> sc(scene);
+  Camera<R> cam(c);
+  std::atomic<uint64_t> total(0);
+  if (y1 <= 0 || y1 > H) y1 = H;
+  int rows = y1 - y0;
+  parallel_tiles(W, rows, threads, [&](int x, int yy) {
+    int y = yy + y0;
+    if (stride > 1 && ((x % stride) != 0 || (y % stride) != 0)) return;
+    Counter cnt;
+    V4<R> s = pathtrace_pixel(sc, cam, P, x, y, W, H, accumID, &cnt);
+    size_t i = stride > 1 ? size_t(y / stride) * ((W + stride - 1) / stride) + x / stride
+                          : size_t(y) * W + x;
+    out[4 * i] = float(s.x);
+    out[4 * i + 1] = float(s.y);
+    out[4 * i + 2] = float(s.z);
+    out[4 * i + 3] = float(s.w);
+    total.fetch_add(cnt.lookups, std::memory_order_relaxed);
+  });
+  if (lookups) *lookups = total.load();
+}
+
+// Running mean of Frame::accumSample (DeviceCopyableObjects.h:2713-2716):
+// FP32 mode reproduces the float recurrence acc = (1-a)*acc + a*c with
+// a = 1/(accumID+1); FP64 mode accumulates the exact double mean.
+template <class R>
+static void pt_accum(const orc_scene *scene, const orc_camera *c, const orc_params *P, int W,
+                     int H, int first, int frames, int stride, double *out, int threads) {
+  Scene<R> sc(scene);
+  Camera<R> cam(c);
+  parallel_tiles(W, H, threads, [&](int x, int y) {
+    if (stride > 1 && ((x % stride) != 0 || (y % stride) != 0)) return;
+    size_t i = stride > 1 ? size_t(y / stride) * ((W + stride - 1) / stride) + x / stride
+                          : size_t(y) * W + x;
+    R acc[4] = {0, 0, 0, 0};
+    for (int f = 0; f < frames; ++f) {
+      int accumID = first + f;
+      V4<R> s = pathtrace_pixel(sc, cam, P, x, y, W, H, accumID, nullptr);
+      R a = R(1) / R(f + 1);
+      R v[4] = {s.x, s.y, s.z, s.w};
+      for (int k = 0; k < 4; ++k) acc[k] = (R(1) - a) * acc[k] + a * v[k];
+    }
+    for (int k = 0; k < 4; ++k) out[4 * i + k] = double(acc[k]);
+  });
+}
+}  // namespace orc
+
+extern "C" {
+
+void orc_pathtrace_frame(const orc_scene *scene, const orc_camera *c, const orc_params *P,
+                         int W, int H, int accumID, int frameID, int fp64, int stride, int y0,
+                         int y1, float *out, int threads, uint64_t *lookups) {
+  (void)frameID;
+  if (fp64)
+    pt_frame<double>(scene, c, P, W, H, accumID, stride, y0, y1, out, threads, lookups);
+  else
+    pt_frame<float>(scene, c, P, W, H, accumID, stride, y0, y1, out, threads, lookups);
+}
+
+void orc_pathtrace_accumulate(const orc_scene *scene, const orc_camera *c, const orc_params *P,
+                              int W, int H, int first, int frames, int fp64, int stride,
+                              double *out, int threads) {
+  if (fp64)
+    pt_accum<double>(scene, c, P, W, H, first, frames, stride, out, threads);
+  else
+    pt_accum<float>(scene, c, P, W, H, first, frames, stride, out, threads);
+}
+
+// frame/common.h:31-47
+static uint32_t cvt1(float f) { return uint32_t(255.f * std::min(std::max(f, 0.f), 1.f)); }
+uint32_t orc_cvt_uint32(const float v[4]) {
+  return (cvt1(v[0]) << 0) | (cvt1(v[1]) << 8) | (cvt1(v[2]) << 16) | (cvt1(v[3]) << 24);
+}
+// linear_to_srgb (parity unpinned: IEC 61966-2-1 piecewise curve)
+static float l2s(float x) {
+  if (x <= 0.0031308f) return 12.92f * x;
+  return 1.055f * std::pow(x, 1.f / 2.4f) - 0.055f;
+}
+uint32_t orc_cvt_uint32_srgb(const float v[4]) {
+  float c[4] = {l2s(v[0]), l2s(v[1]), l2s(v[2]), v[3]};
+  return orc_cvt_uint32(c);
+}
+
+float orc_pow_ieee(float x, float y) { return orc::pow_ieee(x, y); }
+
+}  // extern "C"

@@ -1,0 +1,162 @@
+/*
+ * ORACLE -- test infrastructure, not product code.
+ *
+ * CPU restatement of anari-visionaray's per-pixel volume rendering path.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load this library, and only as the checker / the timed CPU baseline.
+ *
+ * Every routine cites the reference file:line it restates (paths relative to
+ * the reference repository root).  Functions marked "parity unpinned" restate
+ * behaviour that lives in the (absent) visionaray library; they follow the
+ * ANARI specification / textbook definition named next to them.
+ *
+ * Precision: every renderer is instantiated twice, FP32 ("parity mode", the
+ * arithmetic the reference performs) and FP64 ("oracle mode", the converged
+ * reference used for relMSE figures).
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_LIGHT_DIRECTIONAL 0
+#define ORC_LIGHT_QUAD 1
+
+typedef struct orc_light {
+  int type;      /* ORC_LIGHT_* */
+  int visible;
+  float dir[3];  /* directional: ANARI "direction" (direction light travels) */
+  float pos[3];  /* quad: "position" */
+  float e1[3];   /* quad: "edge1" */
+  float e2[3];   /* quad: "edge2" */
+  float color[3];
+  float intensity; /* directional: "irradiance", quad: "intensity" */
+} orc_light;
+
+typedef struct orc_volume {
+  /* structuredRegular field, x-fastest linear float voxels */
+  const float *voxels;
+  int dims[3];
+  float origin[3];
+  float spacing[3];
+  /* transferFunction1D: resampled float4 table (see orc_tf_build) */
+  const float *tf_rgba;
+  int tf_size;
+  float value_lo, value_hi;
+  float unit_distance;
+  /* majorant grid over voxel space [0,1]^3; NULL -> one segment, majorant 1
+     (the reference GPU build's behaviour, DeviceCopyableObjects.h:724,777) */
+  const float *majorants;
+  int grid_dims[3];
+  /* instance transform object->world, row-major 3x4 [R|t] */
+  float xfm[12];
+} orc_volume;
+
+typedef struct orc_trimesh {
+  const float *vertices; /* float3 per vertex */
+  const uint32_t *indices; /* uint3 per triangle; NULL -> soup */
+  int num_triangles;
+  float color[3]; /* matte color */
+  float opacity;
+} orc_trimesh;
+
+typedef struct orc_scene {
+  int num_volumes;
+  const orc_volume *volumes;
+  int num_lights;
+  const orc_light *lights;
+  int num_meshes;
+  const orc_trimesh *meshes;
+} orc_scene;
+
+typedef struct orc_camera {
+  float pos[3], dir[3], up[3];
+  float fovy;   /* radians */
+  float aspect;
+  float region[4]; /* imageRegion: lo.x lo.y hi.x hi.y */
+  float shutter[2];
+} orc_camera;
+
+typedef struct orc_params {
+  float bg[4];
+  float ambient_color[3];
+  float ambient_radiance;
+  float occlusion_distance;
+  int ambient_samples;
+  int max_bounce;
+  int pixel_samples;
+  float volume_sampling_rate; /* raycast only */
+  /* NRC-style truncation for the FP64 reference of truncated estimators:
+     0 = off.  Not a reference feature. */
+  int reserved;
+} orc_params;
+
+/* --- A1: LCG<4> / Random (common.h:55-102) --- */
+uint32_t orc_tea_seed(uint32_t v0, uint32_t v1);
+void orc_lcg_stream(uint32_t seed0, uint32_t seed1, int n, float *out);
+
+/* --- A13: TransferFunction1D table build (TransferFunction1D.cpp:66-111) --- */
+int orc_tf_build(const float *colors, int num_colors, int color_channels,
+                 const float *opacities, int num_opacities,
+                 const float uniform_rgba[4], float *out_rgba);
+
+/* --- A10: tex3D linear/clamp, texel-centre convention --- */
+void orc_sample_field(const orc_volume *vol, int n, const float *p_voxel,
+                      float *out);
+void orc_post_classify(const orc_volume *vol, int n, const float *values,
+                       float *out_rgba);
+
+/* --- A9: majorant grid (conservative), and the reference rasterizer --- */
+void orc_build_majorants(const orc_volume *vol, int gx, int gy, int gz,
+                         float *out);
+void orc_build_majorants_reference(const orc_volume *vol, int gx, int gy,
+                                   int gz, float *out);
+
+/* --- A8: dda3 (renderer/DDA.h:14-112); returns #cells written --- */
+int orc_dda_cells(const float ori[3], const float dir[3], float tmin,
+                  float tmax, const int dims[3], const float lo[3],
+                  const float hi[3], int max_cells, int *cells, float *t0,
+                  float *t1);
+
+/* --- A7: delta tracking along one world ray through one volume.
+   Returns hit (0/1); t_out world distance; consumes rng state in place. --- */
+int orc_delta_track(const orc_volume *vol, const float ori[3],
+                    const float dir[3], float tmin, float tmax,
+                    uint32_t *rng_state, float *t_out, int fp64);
+
+/* --- A14/A15: raycast renderer (Raycast_impl.cpp:245-322) --- */
+void orc_raycast(const orc_scene *scene, const orc_camera *cam,
+                 const orc_params *params, int width, int height,
+                 float *out_rgba, int threads);
+
+/* --- A16: one path-traced frame: per-pixel sample colour (the value
+   Pathtrace_impl.cpp:639 passes to writeSample), over rows [y0,y1) and
+   columns with x % stride == 0 && y % stride == 0 when stride > 1. --- */
+void orc_pathtrace_frame(const orc_scene *scene, const orc_camera *cam,
+                         const orc_params *params, int width, int height,
+                         int accum_id, int frame_id, int fp64, int stride,
+                         int y0, int y1, float *out_rgba, int threads,
+                         uint64_t *lookups);
+
+/* --- A20: Frame::accumSample running mean over frames [0,frames) --- */
+void orc_pathtrace_accumulate(const orc_scene *scene, const orc_camera *cam,
+                              const orc_params *params, int width, int height,
+                              int first_accum_id, int frames, int fp64,
+                              int stride, double *out_rgba, int threads);
+
+/* cvt_uint32 / cvt_uint32_srgb (frame/common.h:31-47) */
+uint32_t orc_cvt_uint32(const float rgba[4]);
+uint32_t orc_cvt_uint32_srgb(const float rgba[4]);
+
+/* powf of the opacity correction from IEEE basic operations */
+float orc_pow_ieee(float x, float y);
+
+/* camera ray for pixel coordinate (x, y) (parity unpinned: ANARI spec) */
+void orc_camera_ray(const orc_camera *cam, float x, float y, int width,
+                    int height, float *ori, float *dir);
+
+#ifdef __cplusplus
+}
+#endif

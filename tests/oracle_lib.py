@@ -1,0 +1,265 @@
+"""ctypes binding of the CPU oracle (oracle/vrt_oracle.h) -- test infrastructure.
+
+Only tests, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this
+module, and only as the checker / the timed CPU restatement.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "_build", "libvrt_oracle.so")
+
+f3 = C.c_float * 3
+
+
+class OrcLight(C.Structure):
+    _fields_ = [("type", C.c_int), ("visible", C.c_int), ("dir", f3), ("pos", f3),
+                ("e1", f3), ("e2", f3), ("color", f3), ("intensity", C.c_float)]
+
+
+class OrcVolume(C.Structure):
+    _fields_ = [("voxels", C.c_void_p), ("dims", C.c_int * 3), ("origin", f3), ("spacing", f3),
+                ("tf_rgba", C.c_void_p), ("tf_size", C.c_int), ("value_lo", C.c_float),
+                ("value_hi", C.c_float), ("unit_distance", C.c_float),
+                ("majorants", C.c_void_p), ("grid_dims", C.c_int * 3),
+                ("xfm", C.c_float * 12)]
+
+
+class OrcTrimesh(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("indices", C.c_void_p), ("num_triangles", C.c_int),
+                ("color", f3), ("opacity", C.c_float)]
+
+
+class OrcScene(C.Structure):
+    _fields_ = [("num_volumes", C.c_int), ("volumes", C.POINTER(OrcVolume)),
+                ("num_lights", C.c_int), ("lights", C.POINTER(OrcLight)),
+                ("num_meshes", C.c_int), ("meshes", C.POINTER(OrcTrimesh))]
+
+
+class OrcCamera(C.Structure):
+    _fields_ = [("pos", f3), ("dir", f3), ("up", f3), ("fovy", C.c_float),
+                ("aspect", C.c_float), ("region", C.c_float * 4), ("shutter", C.c_float * 2)]
+
+
+class OrcParams(C.Structure):
+    _fields_ = [("bg", C.c_float * 4), ("ambient_color", f3), ("ambient_radiance", C.c_float),
+                ("occlusion_distance", C.c_float), ("ambient_samples", C.c_int),
+                ("max_bounce", C.c_int), ("pixel_samples", C.c_int),
+                ("volume_sampling_rate", C.c_float), ("reserved", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run python oracle/build.py")
+        L = C.CDLL(LIB_PATH)
+        vp, ip = C.c_void_p, C.c_int
+        L.orc_tea_seed.restype = C.c_uint32
+        L.orc_tea_seed.argtypes = [C.c_uint32, C.c_uint32]
+        L.orc_lcg_stream.argtypes = [C.c_uint32, C.c_uint32, ip, vp]
+        L.orc_tf_build.argtypes = [vp, ip, ip, vp, ip, vp, vp]
+        L.orc_sample_field.argtypes = [C.POINTER(OrcVolume), ip, vp, vp]
+        L.orc_post_classify.argtypes = [C.POINTER(OrcVolume), ip, vp, vp]
+        L.orc_build_majorants.argtypes = [C.POINTER(OrcVolume), ip, ip, ip, vp]
+        L.orc_build_majorants_reference.argtypes = [C.POINTER(OrcVolume), ip, ip, ip, vp]
+        L.orc_dda_cells.argtypes = [vp, vp, C.c_float, C.c_float, vp, vp, vp, ip, vp, vp, vp]
+        L.orc_delta_track.argtypes = [C.POINTER(OrcVolume), vp, vp, C.c_float, C.c_float,
+                                      C.POINTER(C.c_uint32), C.POINTER(C.c_float), ip]
+        L.orc_raycast.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcCamera),
+                                  C.POINTER(OrcParams), ip, ip, vp, ip]
+        L.orc_pathtrace_frame.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcCamera),
+                                          C.POINTER(OrcParams), ip, ip, ip, ip, ip, ip, ip, ip,
+                                          vp, ip, C.POINTER(C.c_uint64)]
+        L.orc_pathtrace_accumulate.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcCamera),
+                                               C.POINTER(OrcParams), ip, ip, ip, ip, ip, ip,
+                                               vp, ip]
+        L.orc_cvt_uint32.restype = C.c_uint32
+        L.orc_cvt_uint32.argtypes = [vp]
+        L.orc_cvt_uint32_srgb.restype = C.c_uint32
+        L.orc_cvt_uint32_srgb.argtypes = [vp]
+        L.orc_camera_ray.argtypes = [C.POINTER(OrcCamera), C.c_float, C.c_float, ip, ip, vp, vp]
+        _lib = L
+    return _lib
+
+
+def lcg_stream(s0, s1, n):
+    out = np.empty(n, np.float32)
+    lib().orc_lcg_stream(s0, s1, n, out.ctypes.data)
+    return out
+
+
+def tf_build(colors=None, opacities=None, uniform=(1, 1, 1, 1)):
+    uni = np.asarray(uniform, np.float32)
+    cptr, nc, nch = None, 0, 4
+    if colors is not None:
+        colors = np.ascontiguousarray(colors, np.float32)
+        nc, nch = colors.shape
+        cptr = colors.ctypes.data
+    optr, no = None, 0
+    if opacities is not None:
+        opacities = np.ascontiguousarray(opacities, np.float32)
+        no = opacities.shape[0]
+        optr = opacities.ctypes.data
+    n = lib().orc_tf_build(cptr, nc, nch, optr, no, uni.ctypes.data, None)
+    out = np.empty((n, 4), np.float32)
+    lib().orc_tf_build(cptr, nc, nch, optr, no, uni.ctypes.data, out.ctypes.data)
+    return out
+
+
+class OracleVolume:
+    """Keeps numpy arrays alive while the struct points at them."""
+
+    def __init__(self, voxels, origin, spacing, tf, value_range, unit_distance,
+                 majorants=None, grid_dims=None, xfm=None):
+        self.voxels = np.ascontiguousarray(voxels, np.float32)
+        self.tf = np.ascontiguousarray(tf, np.float32).reshape(-1, 4)
+        nz, ny, nx = self.voxels.shape
+        v = OrcVolume()
+        v.voxels = self.voxels.ctypes.data
+        v.dims = (C.c_int * 3)(nx, ny, nz)
+        v.origin = f3(*origin)
+        v.spacing = f3(*spacing)
+        v.tf_rgba = self.tf.ctypes.data
+        v.tf_size = self.tf.shape[0]
+        v.value_lo, v.value_hi = value_range
+        v.unit_distance = unit_distance
+        self.majorants = None
+        if majorants is not None:
+            self.majorants = np.ascontiguousarray(majorants, np.float32)
+            v.majorants = self.majorants.ctypes.data
+            v.grid_dims = (C.c_int * 3)(*grid_dims)
+        x = np.eye(3, 4, dtype=np.float32) if xfm is None else np.asarray(xfm, np.float32)
+        v.xfm = (C.c_float * 12)(*x.ravel())
+        self.s = v
+
+    def build_majorants(self, grid_dims):
+        gx, gy, gz = grid_dims
+        out = np.empty((gz, gy, gx), np.float32)
+        lib().orc_build_majorants(C.byref(self.s), gx, gy, gz, out.ctypes.data)
+        return out
+
+    def attach_majorants(self, grid_dims):
+        m = self.build_majorants(grid_dims)
+        self.majorants = m
+        self.s.majorants = m.ctypes.data
+        self.s.grid_dims = (C.c_int * 3)(*grid_dims)
+        return m
+
+    def sample(self, p_voxel):
+        p = np.ascontiguousarray(p_voxel, np.float32).reshape(-1, 3)
+        out = np.empty(p.shape[0], np.float32)
+        lib().orc_sample_field(C.byref(self.s), p.shape[0], p.ctypes.data, out.ctypes.data)
+        return out
+
+    def classify(self, values):
+        v = np.ascontiguousarray(values, np.float32).ravel()
+        out = np.empty((v.shape[0], 4), np.float32)
+        lib().orc_post_classify(C.byref(self.s), v.shape[0], v.ctypes.data, out.ctypes.data)
+        return out
+
+
+class OracleScene:
+    def __init__(self, volumes, lights=()):
+        self.volumes = list(volumes)
+        self.varr = (OrcVolume * max(1, len(self.volumes)))(*[v.s for v in self.volumes])
+        self.larr = (OrcLight * max(1, len(lights)))()
+        for i, L in enumerate(lights):
+            self.larr[i] = L
+        s = OrcScene()
+        s.num_volumes = len(self.volumes)
+        s.volumes = C.cast(self.varr, C.POINTER(OrcVolume))
+        s.num_lights = len(lights)
+        s.lights = C.cast(self.larr, C.POINTER(OrcLight))
+        s.num_meshes = 0
+        self.s = s
+
+
+def light_directional(direction, intensity=1.0, color=(1, 1, 1), visible=True):
+    L = OrcLight()
+    L.type = 0
+    L.visible = int(visible)
+    L.dir = f3(*direction)
+    L.color = f3(*color)
+    L.intensity = intensity
+    return L
+
+
+def camera(position, direction, up, fovy, aspect, region=(0, 0, 1, 1), shutter=(0.5, 0.5)):
+    c = OrcCamera()
+    c.pos = f3(*position)
+    c.dir = f3(*direction)
+    c.up = f3(*up)
+    c.fovy = fovy
+    c.aspect = aspect
+    c.region = (C.c_float * 4)(*region)
+    c.shutter = (C.c_float * 2)(*shutter)
+    return c
+
+
+def params(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.2,
+           occlusion_distance=1e20, ambient_samples=1, max_bounce=7, pixel_samples=1,
+           volume_sampling_rate=0.5):
+    p = OrcParams()
+    p.bg = (C.c_float * 4)(*background)
+    p.ambient_color = f3(*ambient_color)
+    p.ambient_radiance = ambient_radiance
+    p.occlusion_distance = occlusion_distance
+    p.ambient_samples = ambient_samples
+    p.max_bounce = max_bounce
+    p.pixel_samples = pixel_samples
+    p.volume_sampling_rate = volume_sampling_rate
+    p.reserved = 0
+    return p
+
+
+class OracleWorkload:
+    """A workloads.Workload restated for the oracle."""
+
+    def __init__(self, wl):
+        self.wl = wl
+        self.vol = OracleVolume(wl.voxels, wl.origin, wl.spacing, wl.tf, wl.value_range,
+                                wl.unit_distance)
+        if wl.grid_dims is not None:
+            self.vol.attach_majorants(wl.grid_dims)
+        lights = [light_directional(L["direction"], L["intensity"], L["color"])
+                  for L in wl.lights]
+        self.scene = OracleScene([self.vol], lights)
+        self.cam = camera(**wl.camera)
+        self.params = params(**wl.params)
+
+    def raycast(self, threads=0):
+        out = np.empty((self.wl.height, self.wl.width, 4), np.float32)
+        lib().orc_raycast(C.byref(self.scene.s), C.byref(self.cam), C.byref(self.params),
+                          self.wl.width, self.wl.height, out.ctypes.data, threads)
+        return out
+
+    def pathtrace_frame(self, accum_id=0, fp64=False, stride=1, y0=0, y1=0, threads=0):
+        W, H = self.wl.width, self.wl.height
+        if stride > 1:
+            shape = ((H + stride - 1) // stride, (W + stride - 1) // stride, 4)
+        else:
+            shape = (H, W, 4)
+        out = np.zeros(shape, np.float32)
+        lookups = C.c_uint64(0)
+        lib().orc_pathtrace_frame(C.byref(self.scene.s), C.byref(self.cam), C.byref(self.params),
+                                  W, H, accum_id, 0, int(fp64), stride, y0, y1, out.ctypes.data,
+                                  threads, C.byref(lookups))
+        return out, lookups.value
+
+    def pathtrace_accumulate(self, frames, first_accum_id=0, fp64=False, stride=1, threads=0):
+        W, H = self.wl.width, self.wl.height
+        if stride > 1:
+            shape = ((H + stride - 1) // stride, (W + stride - 1) // stride, 4)
+        else:
+            shape = (H, W, 4)
+        out = np.zeros(shape, np.float64)
+        lib().orc_pathtrace_accumulate(C.byref(self.scene.s), C.byref(self.cam),
+                                       C.byref(self.params), W, H, first_accum_id, frames,
+                                       int(fp64), stride, out.ctypes.data, threads)
+        return out
