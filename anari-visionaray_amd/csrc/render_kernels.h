@@ -7,12 +7,18 @@ namespace vrt {
 
 hipError_t launch_raycast(const KScene &sc, const KCamera &cam, const KParams &P, const KFrame &F,
                           hipStream_t s);
+// persistent path tracer: `blocks` workgroups (a multiple of 8) drain the
+// per-XCD pixel queues; `queues` must be zero, `queues_next` is zeroed for
+// the following launch
 hipError_t launch_pathtrace(const KScene &sc, const KCamera &cam, const KParams &P,
-                            const KFrame &F, int accumID, hipStream_t s);
+                            const KFrame &F, int accumID, uint32_t *queues,
+                            uint32_t *queues_next, int blocks, hipStream_t s);
+int pathtrace_blocks_per_cu(size_t lds_bytes);
 hipError_t launch_brickify(const float *lin, const int dims[3], float *bricks, hipStream_t s);
 hipError_t launch_value_ranges(const KVolume &v, float *ranges, hipStream_t s);
 hipError_t launch_majorants(const KVolume &v, const float *ranges, float *maj, hipStream_t s);
 hipError_t launch_frame_reset(const KFrame &F, hipStream_t s);
+hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 hipError_t launch_sample_field(const KVolume &v, int n, const float *pts, float *out,
                                hipStream_t s);
 hipError_t launch_delta_track(const KScene &sc, int n, const float *rays, uint32_t *rng,

@@ -504,179 +504,303 @@ __global__ void __launch_bounds__(256)
 // Path tracer: Pathtrace_impl.cpp:489-641, volume vertices with directional
 // light NEE and the ambient-occlusion term, Russian roulette drawing from the
 // un-advanced pixel RNG (Pathtrace_impl.cpp:594-599).
+//
+// Persistent kernel with path regeneration.  The reference runs one thread
+// per pixel to completion, so a 64-lane wave lives as long as its longest
+// path.  Here every lane owns one path at a time: each loop iteration traces
+// one ray (radiance, shadow or AO pass) for every active lane, and a lane
+// whose path ended claims the next pixel from its XCD's work queue.  A pixel's
+// random streams depend only on (pixelID, accumID), so the image is
+// identical to the one-thread-per-pixel schedule.
+//
+// Work distribution: the image is cut into 32x32-pixel chunks (16 tiles of
+// 8x8); chunk c belongs to queue c % 8, and workgroup b drains queue b % 8
+// first (the XCD the hardware placed it on), then helps the other queues.
+// A wave claims 64 pixel indices per atomic and hands them to its idle
+// lanes.
 // ---------------------------------------------------------------------------
 enum RayType { Radiance = 0, Shadow = 1, AO = 2, Miss = 3 };
 
+struct PathState {
+  int pixel;  // linear pixel index, -1 = idle
+  uint32_t rng, srng;
+  int sampleID;
+  float4 acc;
+  float depth, sdepth;
+  KRay ray;
+  int rayType;
+  unsigned passID, bounceID;
+  f3 throughput, intensity;
+  // ShadeState (Pathtrace_impl.cpp:15-35)
+  f3 baseColor, shadedColor, hitPosOff, sn, bs_dir, bs_f;
+  float bs_pdf, bs_cosT;
+  int light;  // sampled light, -1 = none yet
+  float misWeightNEE, vis_light, vis_ao;
+  int aoSamples;
+  float aoWeights, aoCount;
+  bool firstHit;
+};
+
+constexpr int kChunkLog2 = 5;  // 32x32-pixel chunks
+constexpr int kQueues = 8;
+
+__device__ inline void chunk_pixel(int q, uint32_t local, int nchx, int &x, int &y) {
+  const uint32_t j = local >> (2 * kChunkLog2);
+  const uint32_t w = local & ((1u << (2 * kChunkLog2)) - 1);
+  const uint32_t c = j * kQueues + uint32_t(q);
+  const uint32_t tile = w >> 6;
+  x = int((c % uint32_t(nchx)) << kChunkLog2) + int((tile & 3) << 3) + int(w & 7);
+  y = int((c / uint32_t(nchx)) << kChunkLog2) + int((tile >> 2) << 3) + int((w >> 3) & 7);
+}
+
+// Start sample `sampleID` of the lane's pixel: jitter, camera ray, shutter
+// draw (Pathtrace_impl.cpp:526-535)
+__device__ inline void begin_sample(PathState &st, const KCamera &cam, const KParams &P, int x,
+                                    int y, int W, int H) {
+  float xf = float(x), yf = float(y);
+  const float jx = rnd(st.srng) - .5f;
+  const float jy = rnd(st.srng) - .5f;
+  xf += jx;
+  yf += jy;
+  st.ray = primary_ray(cam, xf, yf, float(W), float(H));
+  (void)rnd(st.srng);  // shutter time (DeviceCopyableObjects.h:2597)
+  st.sdepth = 1e31f;
+  st.rayType = Radiance;
+  st.passID = 0;
+  st.bounceID = 0;
+  st.throughput = mk(1.f);
+  st.intensity = mk(0.f);
+  st.baseColor = st.shadedColor = st.hitPosOff = st.sn = st.bs_dir = st.bs_f = mk(0.f);
+  st.bs_pdf = st.bs_cosT = 0.f;
+  st.light = -1;
+  st.misWeightNEE = 0.f;
+  st.vis_light = st.vis_ao = 1.f;
+  st.aoSamples = 0;
+  st.aoWeights = st.aoCount = 0.f;
+  st.firstHit = false;
+}
+
+// One pass of the bounce loop (Pathtrace_impl.cpp:558-600): intersect, shade,
+// advance the state machine.  Returns true when the sample's path ended.
+__device__ inline bool path_pass(PathState &st, const KScene &sc, const KParams &P,
+                                 const float4 *s_tf, uint32_t &lookups) {
+  // intersectAll: volumes only in this scene class
+  VolHit vh;
+  vh.hit = false;
+  vh.t = FLT_MAX;
+  for (int i = 0; i < sc.num_volumes; ++i) {
+    VolHit h = delta_track(sc.vol[i], s_tf, to_local(sc.vol[i], st.ray), st.srng, lookups);
+    if (h.hit && h.t < vh.t) vh = h;
+  }
+  const bool hit = vh.hit;
+  if (st.passID == 0 && st.bounceID == 0) {
+    st.firstHit = hit;
+    if (hit) st.sdepth = vh.t;
+  }
+  // shade (Pathtrace_impl.cpp:93-487)
+  int next = Miss;
+  KRay nray = st.ray;
+  const int rayType = st.rayType;
+  if (rayType == Radiance) {
+    st.baseColor = mk(0.f);
+    st.vis_light = 1.f;
+    st.vis_ao = 1.f;
+    st.aoSamples = 0;
+    st.aoWeights = 0.f;
+    st.aoCount = 0.f;
+    if (!hit) {
+      st.shadedColor = mk(0.f);
+    } else {
+      const f3 hitPos = st.ray.ori + st.ray.dir * vh.t;
+      const float eps = epsilon_from(hitPos, st.ray.dir, vh.t);
+      const f3 viewDir = -normalize(st.ray.dir);
+      f3 gn = mk(0.f);
+      if (P.ambient_samples > 0) {
+        const float u1 = rnd(st.srng), u2 = rnd(st.srng);
+        gn = uniform_sample_sphere(u1, u2);
+      }
+      st.sn = faceforward(gn, viewDir, gn);
+      st.hitPosOff = hitPos + st.sn * eps;
+      if (sc.num_lights > 0) {
+        int which = int(rnd(st.srng) * float(sc.num_lights));
+        if (which == sc.num_lights) which = 0;
+        st.light = which;
+      }
+      // directional light sample: pdf 1, dist2 1 (renderer/common.h:1432-1466)
+      f3 ls_int = mk(0.f);
+      float ls_pdf = 0.f, ls_d2 = 0.f;
+      if (st.light >= 0) {
+        ls_int = sc.light[st.light].intensity;
+        ls_pdf = 1.f;
+        ls_d2 = 1.f;
+      }
+      const float rcp_pdf = ls_pdf > 0.f ? 1.f / ls_pdf : 0.f;
+      const float rcp_d2 = ls_d2 > 0.f ? 1.f / ls_d2 : 0.f;
+      st.shadedColor = vh.albedo * (ls_int * rcp_d2) * rcp_pdf * rcp_d2;
+      const float u1 = rnd(st.srng), u2 = rnd(st.srng);
+      st.bs_dir = uniform_sample_sphere(u1, u2);
+      st.bs_f = vh.albedo;
+      st.bs_pdf = 1.f;
+      st.bs_cosT = 1.f;
+      st.misWeightNEE = sc.num_lights > 0 ? 1.f : 0.f;
+      st.baseColor = vh.albedo;
+      // lightSample.pdf >= 0 always holds: shadow ray next (:403-406)
+      nray.ori = st.hitPosOff;
+      nray.dir = normalize(st.light >= 0 ? sc.light[st.light].dir : mk(0.f));
+      nray.tmin = 0.f;
+      nray.tmax = FLT_MAX;
+      next = Shadow;
+    }
+  } else if (rayType == Shadow) {
+    st.vis_light = hit ? 0.f : 1.f;
+    next = Radiance;
+  } else {  // AO
+    st.aoSamples++;
+    const float weight = fmaxf(0.f, dot(st.ray.dir, st.sn));
+    st.aoWeights += weight;
+    if (weight > 0.f && hit) st.aoCount += weight;
+    if (st.aoSamples < P.ambient_samples) {
+      next = AO;
+    } else {
+      const float aoV = st.aoWeights > 0.f ? 1.f - st.aoCount / st.aoWeights : 0.f;
+      st.vis_ao *= aoV;
+      next = Radiance;
+    }
+  }
+  // after the shadow pass (or an AO pass with samples left) an AO ray is
+  // traced while ambient samples remain (Pathtrace_impl.cpp:437-455)
+  if ((rayType == Shadow && st.aoSamples < P.ambient_samples) || (rayType == AO && next == AO)) {
+    const f3 w = st.sn;
+    const f3 v = fabsf(w.x) > fabsf(w.y) ? normalize(mk(-w.z, 0.f, w.x))
+                                         : normalize(mk(0.f, w.z, -w.y));
+    const f3 u = cross(v, w);
+    const float u1 = rnd(st.srng), u2 = rnd(st.srng);
+    const f3 sp = cosine_sample_hemisphere(u1, u2);
+    nray.ori = st.hitPosOff;
+    nray.dir = normalize(u * sp.x + v * sp.y + w * sp.z);
+    nray.tmin = 0.f;
+    nray.tmax = P.occlusion_distance;
+    next = AO;
+  } else if (next == Radiance) {
+    nray.ori = st.hitPosOff;
+    nray.dir = normalize(st.bs_dir);
+    nray.tmin = 0.f;
+    nray.tmax = 1e31f;
+  }
+  st.ray = nray;
+  st.rayType = next;
+  st.passID++;
+  if (next == Miss || next == Radiance) {
+    const f3 direct = st.shadedColor * st.vis_light;
+    const f3 ambient = st.baseColor * P.ambient_color * P.ambient_radiance * st.vis_ao;
+    st.intensity = st.intensity + st.throughput * (direct * st.misWeightNEE);
+    st.intensity = st.intensity + st.throughput * ambient;
+    const float rcp = st.bs_pdf > 0.f ? 1.f / st.bs_pdf : 0.f;
+    st.throughput = st.throughput * (st.bs_f * (st.bs_cosT * rcp));
+    st.bounceID++;
+  }
+  if (next == Miss || int(st.bounceID) > P.max_bounce) return true;
+  if (st.bounceID > 1) {
+    const float prob = max3(st.throughput);
+    if (rnd(st.rng) > prob) return true;
+    st.throughput = st.throughput * (1.f / prob);
+  }
+  return false;
+}
+
 __global__ void __launch_bounds__(256)
-    k_pathtrace(KScene sc, KCamera cam, KParams P, KFrame F, int accumID) {
+    k_pathtrace(KScene sc, KCamera cam, KParams P, KFrame F, int accumID, uint32_t *queues,
+                uint32_t *queues_next) {
   extern __shared__ float4 s_tf[];
   stage_tf(sc, s_tf);
-  int x, y;
-  tile_pixel(F.width, x, y);
-  const bool active = x < F.width && y < F.height;
+  // clear the counters the next launch will use (the current ones were
+  // cleared by the previous launch)
+  if (blockIdx.x == 0 && threadIdx.x < kQueues) queues_next[threadIdx.x] = 0u;
+
+  const int W = F.width, H = F.height;
+  const int nchx = (W + (1 << kChunkLog2) - 1) >> kChunkLog2;
+  const int nchy = (H + (1 << kChunkLog2) - 1) >> kChunkLog2;
+  const uint32_t nch = uint32_t(nchx * nchy);
+  const uint32_t lane = threadIdx.x & 63u;
+  const unsigned long long below = (1ull << lane) - 1ull;
+
+  PathState st;
+  st.pixel = -1;
   uint32_t lookups = 0;
-  if (active) {
-    const int W = F.width, H = F.height;
-    const uint32_t pixelID = uint32_t(x + W * y);
-    uint32_t rng = tea4(pixelID, uint32_t(accumID));
-    uint32_t srng = rng;  // ScreenSample's copy (Pathtrace_impl.cpp:512-513)
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    float depth = 1e31f;
-    const int spp = P.pixel_samples;
-    for (int sampleID = 0; sampleID < spp; ++sampleID) {
-      float xf = float(x), yf = float(y);
-      const float jx = rnd(srng) - .5f;
-      const float jy = rnd(srng) - .5f;
-      xf += jx;
-      yf += jy;
-      KRay ray = primary_ray(cam, xf, yf, float(W), float(H));
-      (void)rnd(srng);  // shutter time (DeviceCopyableObjects.h:2597)
-      float4 color = P.bg;
-      float sdepth = 1e31f;
-      if (sc.num_volumes > 0) {
-        // ShadeState (Pathtrace_impl.cpp:15-35)
-        f3 baseColor = mk(0.f), shadedColor = mk(0.f), hitPosOff = mk(0.f), sn = mk(0.f);
-        int aoSamples = 0;
-        float aoWeights = 0.f, aoCount = 0.f;
-        f3 bs_dir = mk(0.f), bs_f = mk(0.f);
-        float bs_pdf = 0.f, bs_cosT = 0.f;
-        f3 ls_dir = mk(0.f), ls_int = mk(0.f);
-        float ls_pdf = 0.f, ls_dist = 0.f, ls_dist2 = 0.f;
-        float misWeightNEE = 0.f, vis_light = 1.f, vis_ao = 1.f;
-        int rayType = Radiance;
-        f3 throughput = mk(1.f), intensity = mk(0.f);
-        bool firstHit = false;
-        for (unsigned passID = 0, bounceID = 0;; ++passID) {
-          // intersectAll: volumes only in this scene class
-          VolHit vh;
-          vh.hit = false;
-          vh.t = FLT_MAX;
-          for (int i = 0; i < sc.num_volumes; ++i) {
-            VolHit h = delta_track(sc.vol[i], s_tf, to_local(sc.vol[i], ray), srng, lookups);
-            if (h.hit && h.t < vh.t) vh = h;
-          }
-          const bool hit = vh.hit;
-          if (passID == 0 && bounceID == 0) {
-            firstHit = hit;
-            if (hit) sdepth = vh.t;
-          }
-          // shade (Pathtrace_impl.cpp:93-487)
-          int next = Miss;
-          KRay nray = ray;
-          if (rayType == Radiance) {
-            baseColor = mk(0.f);
-            vis_light = 1.f;
-            vis_ao = 1.f;
-            aoSamples = 0;
-            aoWeights = 0.f;
-            aoCount = 0.f;
-            if (!hit) {
-              shadedColor = mk(0.f);
-            } else {
-              const f3 hitPos = ray.ori + ray.dir * vh.t;
-              const float eps = epsilon_from(hitPos, ray.dir, vh.t);
-              const f3 viewDir = -normalize(ray.dir);
-              f3 gn = mk(0.f);
-              if (P.ambient_samples > 0) {
-                const float u1 = rnd(srng), u2 = rnd(srng);
-                gn = uniform_sample_sphere(u1, u2);
-              }
-              sn = faceforward(gn, viewDir, gn);
-              hitPosOff = hitPos + sn * eps;
-              if (sc.num_lights > 0) {
-                int which = int(rnd(srng) * float(sc.num_lights));
-                if (which == sc.num_lights) which = 0;
-                const KLight &L = sc.light[which];
-                ls_dir = L.dir;
-                ls_int = L.intensity;
-                ls_pdf = 1.f;
-                ls_dist = FLT_MAX;
-                ls_dist2 = 1.f;
-              }
-              const float rcp_pdf = ls_pdf > 0.f ? 1.f / ls_pdf : 0.f;
-              const float rcp_d2 = ls_dist2 > 0.f ? 1.f / ls_dist2 : 0.f;
-              shadedColor = vh.albedo * (ls_int * rcp_d2) * rcp_pdf * rcp_d2;
-              const float u1 = rnd(srng), u2 = rnd(srng);
-              bs_dir = uniform_sample_sphere(u1, u2);
-              bs_f = vh.albedo;
-              bs_pdf = 1.f;
-              bs_cosT = 1.f;
-              misWeightNEE = sc.num_lights > 0 ? 1.f : 0.f;
-              baseColor = vh.albedo;
-              if (ls_pdf >= 0.f) {
-                nray.ori = hitPosOff;
-                nray.dir = normalize(ls_dir);
-                nray.tmin = 0.f;
-                nray.tmax = ls_dist;
-                next = Shadow;
-              }
-            }
-          } else if (rayType == Shadow) {
-            vis_light = hit ? 0.f : 1.f;
-            next = Radiance;
-          } else {  // AO
-            aoSamples++;
-            const float weight = fmaxf(0.f, dot(ray.dir, sn));
-            aoWeights += weight;
-            if (weight > 0.f && hit) aoCount += weight;
-            if (aoSamples < P.ambient_samples) {
-              next = AO;
-            } else {
-              const float aoV = aoWeights > 0.f ? 1.f - aoCount / aoWeights : 0.f;
-              vis_ao *= aoV;
-              next = Radiance;
-            }
-          }
-          // after the shadow pass (or an AO pass with samples left) an AO ray
-          // is traced if ambient samples remain (Pathtrace_impl.cpp:437-455)
-          if ((rayType == Shadow && aoSamples < P.ambient_samples) ||
-              (rayType == AO && next == AO)) {
-            f3 w = sn;
-            f3 v = fabsf(w.x) > fabsf(w.y) ? normalize(mk(-w.z, 0.f, w.x))
-                                           : normalize(mk(0.f, w.z, -w.y));
-            f3 u = cross(v, w);
-            const float u1 = rnd(srng), u2 = rnd(srng);
-            const f3 sp = cosine_sample_hemisphere(u1, u2);
-            nray.ori = hitPosOff;
-            nray.dir = normalize(u * sp.x + v * sp.y + w * sp.z);
-            nray.tmin = 0.f;
-            nray.tmax = P.occlusion_distance;
-            next = AO;
-          } else if (next == Radiance) {
-            nray.ori = hitPosOff;
-            nray.dir = normalize(bs_dir);
-            nray.tmin = 0.f;
-            nray.tmax = 1e31f;
-          }
-          ray = nray;
-          rayType = next;
-          if (rayType == Miss || rayType == Radiance) {
-            const f3 direct = shadedColor * vis_light;
-            const f3 ambient = baseColor * P.ambient_color * P.ambient_radiance * vis_ao;
-            intensity = intensity + throughput * (direct * misWeightNEE);
-            intensity = intensity + throughput * ambient;
-            const float rcp = bs_pdf > 0.f ? 1.f / bs_pdf : 0.f;
-            throughput = throughput * (bs_f * (bs_cosT * rcp));
-            bounceID++;
-          }
-          if (rayType == Miss || int(bounceID) > P.max_bounce) break;
-          if (bounceID > 1) {
-            const float prob = max3(throughput);
-            if (rnd(rng) > prob) break;
-            throughput = throughput * (1.f / prob);
-          }
+  int q = int(blockIdx.x) & (kQueues - 1);  // wave-uniform queue
+  int qtries = 0;
+  uint32_t pool_base = 0, pool_left = 0;    // wave-uniform local pool
+  int px = 0, py = 0;
+
+  while (true) {
+    // ---- refill idle lanes --------------------------------------------------
+    unsigned long long idle = __ballot(st.pixel < 0);
+    while (idle && qtries < kQueues) {
+      if (pool_left == 0) {
+        // pixels of queue q: the chunks c = q, q+8, ... times 1024 pixels
+        const uint32_t band = ((nch + kQueues - 1 - uint32_t(q)) / kQueues) << (2 * kChunkLog2);
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&queues[q], 64u);
+        base = __shfl(base, 0, 64);
+        if (base >= band) {
+          q = (q + 1) & (kQueues - 1);
+          ++qtries;
+          continue;
         }
-        if (firstHit) color = make_float4(intensity.x, intensity.y, intensity.z, 1.f);
+        pool_base = base;
+        pool_left = min(64u, band - base);
       }
-      acc.x += color.x;
-      acc.y += color.y;
-      acc.z += color.z;
-      acc.w += color.w;
-      if (sdepth < depth) depth = sdepth;
+      const uint32_t nidle = uint32_t(__popcll(idle));
+      const uint32_t take = min(nidle, pool_left);
+      const uint32_t rank = uint32_t(__popcll(idle & below));
+      if (st.pixel < 0 && rank < take) {
+        chunk_pixel(q, pool_base + rank, nchx, px, py);
+        if (px < W && py < H) {
+          st.pixel = py * W + px;
+          st.rng = tea4(uint32_t(st.pixel), uint32_t(accumID));
+          st.srng = st.rng;  // ScreenSample's copy (Pathtrace_impl.cpp:512-513)
+          st.sampleID = 0;
+          st.acc = make_float4(0.f, 0.f, 0.f, 0.f);
+          st.depth = 1e31f;
+          begin_sample(st, cam, P, px, py, W, H);
+        }
+      }
+      pool_base += take;
+      pool_left -= take;
+      idle = __ballot(st.pixel < 0);
+      if (take == nidle) break;  // every idle lane was offered a pixel
     }
-    const float inv = 1.f / float(spp);
-    acc = make_float4(acc.x * inv, acc.y * inv, acc.z * inv, acc.w * inv);
-    write_sample(F, x, y, accumID, acc, depth, true);
+    if (__ballot(st.pixel >= 0) == 0ull) {
+      // no lane holds a pixel: either every queue is drained, or the last
+      // pool only covered padding pixels outside the image -- claim again
+      if (qtries >= kQueues) break;
+      continue;
+    }
+
+    // ---- one pass for every active lane ---------------------------------------
+    if (st.pixel >= 0) {
+      bool done = sc.num_volumes == 0 || path_pass(st, sc, P, s_tf, lookups);
+      if (done) {
+        float4 color = P.bg;
+        if (st.firstHit)
+          color = make_float4(st.intensity.x, st.intensity.y, st.intensity.z, 1.f);
+        st.acc.x += color.x;
+        st.acc.y += color.y;
+        st.acc.z += color.z;
+        st.acc.w += color.w;
+        if (st.sdepth < st.depth) st.depth = st.sdepth;
+        const int x = st.pixel % W, y = st.pixel / W;
+        if (++st.sampleID < P.pixel_samples) {
+          begin_sample(st, cam, P, x, y, W, H);
+        } else {
+          const float inv = 1.f / float(P.pixel_samples);
+          const float4 a = make_float4(st.acc.x * inv, st.acc.y * inv, st.acc.z * inv,
+                                       st.acc.w * inv);
+          write_sample(F, x, y, accumID, a, st.depth, true);
+          st.pixel = -1;
+        }
+      }
+    }
   }
   if (F.counters) wave_add_counter(&F.counters[0], lookups);
 }
@@ -793,6 +917,13 @@ __global__ void __launch_bounds__(256)
   tout[i] = h.hit ? h.t : __builtin_inff();
 }
 
+__global__ void __launch_bounds__(256)
+    k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n16;
+       i += size_t(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+
 __global__ void k_frame_reset(KFrame F) {
   const size_t n = size_t(F.width) * F.height;
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
@@ -821,10 +952,19 @@ hipError_t launch_raycast(const KScene &sc, const KCamera &cam, const KParams &P
 }
 
 hipError_t launch_pathtrace(const KScene &sc, const KCamera &cam, const KParams &P,
-                            const KFrame &F, int accumID, hipStream_t s) {
+                            const KFrame &F, int accumID, uint32_t *queues,
+                            uint32_t *queues_next, int blocks, hipStream_t s) {
   const size_t lds = size_t(sc.tf_lds_total) * sizeof(float4);
-  hipLaunchKernelGGL(k_pathtrace, dim3(tiles(F)), dim3(256), lds, s, sc, cam, P, F, accumID);
+  hipLaunchKernelGGL(k_pathtrace, dim3(blocks), dim3(256), lds, s, sc, cam, P, F, accumID, queues,
+                     queues_next);
   return hipGetLastError();
+}
+
+int pathtrace_blocks_per_cu(size_t lds_bytes) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pathtrace, 256, lds_bytes) != hipSuccess)
+    return 1;
+  return n > 0 ? n : 1;
 }
 
 hipError_t launch_brickify(const float *lin, const int dims[3], float *bricks, hipStream_t s) {
@@ -861,6 +1001,15 @@ hipError_t launch_delta_track(const KScene &sc, int n, const float *rays, uint32
   const size_t lds = size_t(sc.tf_lds_total) * sizeof(float4);
   hipLaunchKernelGGL(k_delta_track, dim3((n + 255) / 256), dim3(256), lds, s, sc, n, rays, rng,
                      tout);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s) {
+  const size_t n16 = bytes / 16;
+  const size_t blocks = std::min<size_t>((n16 + 255) / 256, 256 * 64);
+  if (n16)
+    hipLaunchKernelGGL(k_copy, dim3(unsigned(blocks)), dim3(256), 0, s,
+                       reinterpret_cast<const uint4 *>(src), reinterpret_cast<uint4 *>(dst), n16);
   return hipGetLastError();
 }
 

@@ -2,6 +2,7 @@
 // kernel scene views, and launches on the context stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -42,6 +43,11 @@ struct vrt_context_s {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t t0 = nullptr, t1 = nullptr;
+  int num_cus = 0;
+  // work-queue counters of the persistent path tracer: two sets of 8, used
+  // alternately; each launch clears the set the next launch uses
+  uint32_t *queues = nullptr;
+  int queue_parity = 0;
 };
 
 namespace {
@@ -220,9 +226,12 @@ int vrt_context_create(int device, vrt_context *out) {
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&c->t0);
   if (e == hipSuccess) e = hipEventCreate(&c->t1);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device);
+  if (e == hipSuccess) e = hipMalloc(&c->queues, 2 * 8 * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMemset(c->queues, 0, 2 * 8 * sizeof(uint32_t));
   if (e != hipSuccess) {
     delete c;
-    return hip_fail(e, "stream/event creation");
+    return hip_fail(e, "context creation");
   }
   *out = c;
   return VRT_OK;
@@ -234,6 +243,7 @@ int vrt_context_destroy(vrt_context ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   (void)hipEventDestroy(ctx->t0);
   (void)hipEventDestroy(ctx->t1);
+  (void)hipFree(ctx->queues);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return VRT_OK;
@@ -291,6 +301,17 @@ int vrt_memset(vrt_context ctx, void *dst, int value, size_t bytes) {
   if (bytes == 0) return VRT_OK;
   if (!dst) return fail(VRT_ERROR_INVALID_ARGUMENT, "null buffer");
   VRT_HIP(hipMemsetAsync(dst, value, bytes, ctx->stream));
+  return VRT_OK;
+}
+
+int vrt_device_copy(vrt_context ctx, void *dst, const void *src, size_t bytes) {
+  if (int r = check_ctx(ctx)) return r;
+  if (bytes == 0) return VRT_OK;
+  if (!dst || !src) return fail(VRT_ERROR_INVALID_ARGUMENT, "null buffer");
+  if ((bytes & 15) || (reinterpret_cast<uintptr_t>(dst) & 15) ||
+      (reinterpret_cast<uintptr_t>(src) & 15))
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "copy needs 16-byte aligned buffers and size");
+  VRT_HIP(launch_copy(dst, src, bytes, ctx->stream));
   return VRT_OK;
 }
 
@@ -420,7 +441,13 @@ int vrt_pathtrace(vrt_context ctx, const vrt_scene *scene, const vrt_camera *cam
   if (int r = make_params(params, P)) return r;
   if (int r = make_frame(frame, F)) return r;
   if (!F.accum) return fail(VRT_ERROR_INVALID_ARGUMENT, "pathtrace needs an accumulation buffer");
-  VRT_HIP(launch_pathtrace(sc, cam, P, F, accum_id, ctx->stream));
+  const size_t lds = size_t(sc.tf_lds_total) * 16;
+  int blocks = std::max(1, ctx->num_cus) * pathtrace_blocks_per_cu(lds);
+  blocks = std::max(8, (blocks + 7) / 8 * 8);  // every XCD queue gets workgroups
+  uint32_t *cur = ctx->queues + 8 * ctx->queue_parity;
+  uint32_t *nxt = ctx->queues + 8 * (1 - ctx->queue_parity);
+  VRT_HIP(launch_pathtrace(sc, cam, P, F, accum_id, cur, nxt, blocks, ctx->stream));
+  ctx->queue_parity ^= 1;
   return VRT_OK;
 }
 

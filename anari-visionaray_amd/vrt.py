@@ -107,6 +107,7 @@ def lib():
         L.vrt_upload.argtypes = [vp, vp, vp, sz]
         L.vrt_download.argtypes = [vp, vp, vp, sz]
         L.vrt_memset.argtypes = [vp, vp, ip, sz]
+        L.vrt_device_copy.argtypes = [vp, vp, vp, sz]
         L.vrt_timer_start.argtypes = [vp]
         L.vrt_timer_stop.argtypes = [vp, C.POINTER(C.c_float)]
         L.vrt_structured_brick_bytes.argtypes = [C.POINTER(C.c_int)]
@@ -212,6 +213,9 @@ class Context:
 
     def sync(self):
         check(lib().vrt_synchronize(self.handle))
+
+    def copy(self, dst, src, nbytes):
+        check(lib().vrt_device_copy(self.handle, dst.ptr, src.ptr, nbytes))
 
     def timer_start(self):
         check(lib().vrt_timer_start(self.handle))

@@ -40,29 +40,47 @@ def _package():
 
 
 def cpu_baseline(wl, target_seconds=12.0):
-    """Time the oracle (FP32, all host cores) on a band of rows of frame 0."""
+    """Time the oracle (FP32, all host cores) on whole frames of the same
+    workload until about target_seconds have passed (at least one frame)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     ow = O.OracleWorkload(wl)
     threads = os.cpu_count() or 1
-    # calibrate on a 16-row band, then size the timed band to ~target_seconds
+    frames = 0
     t0 = time.perf_counter()
-    ow.pathtrace_frame(0, y0=wl.height // 2, y1=wl.height // 2 + 16, threads=threads)
-    dt = max(1e-3, time.perf_counter() - t0)
-    rows = int(min(wl.height, max(16, 16 * target_seconds / dt)))
-    y0 = max(0, wl.height // 2 - rows // 2)
-    t0 = time.perf_counter()
-    ow.pathtrace_frame(0, y0=y0, y1=y0 + rows, threads=threads)
-    dt = time.perf_counter() - t0
-    samples = rows * wl.width
+    while True:
+        ow.pathtrace_frame(frames, threads=threads)
+        frames += 1
+        dt = time.perf_counter() - t0
+        if dt >= target_seconds:
+            break
+    samples = frames * wl.width * wl.height
     return {
         "value": samples / dt / 1e6,
         "unit": "Msamples/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"rows {y0}..{y0 + rows} of frame 0 ({rows}x{wl.width} pixel samples, "
-                  f"{dt:.1f} s), FP32 CPU restatement, {threads} threads, 64x64 tiles",
+        "sample": f"{frames} frames of {wl.width}x{wl.height} x 1 spp ({dt:.1f} s) of the same "
+                  f"workload, FP32 CPU restatement, {threads} threads over 64x64 tiles",
     }
+
+
+def calibrate_copy(vrt, mib, reps):
+    """Copy kernel of known size: each launch reads and writes mib MiB."""
+    ctx = vrt.Context(0)
+    n = mib << 20
+    a, b = ctx.buffer(n), ctx.buffer(n)
+    a.zero()
+    for _ in range(2):
+        ctx.copy(b, a, n)
+    ctx.timer_start()
+    for _ in range(reps):
+        ctx.copy(b, a, n)
+    ms = ctx.timer_stop()
+    print(json.dumps({"kernel": "k_copy", "bytes_read_per_launch": n, "bytes_written_per_launch": n,
+                      "launches": reps, "ms_per_launch": ms / reps,
+                      "GBps_read_plus_write": 2 * n * reps / (ms / 1e3) / 1e9}), flush=True)
+    ctx.close()
 
 
 def main():
@@ -75,6 +93,8 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run the plain copy kernel over MIB MiB (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -85,6 +105,9 @@ def main():
 
     pkg = _package()
     vrt, wls = pkg.vrt, pkg.workloads
+    if args.calibrate_copy:
+        calibrate_copy(vrt, args.calibrate_copy, args.steps)
+        return
     t0 = time.perf_counter()
     wl = wls.fbm_pathtrace(n=args.size, width=args.width, height=args.height,
                            frames=args.steps)

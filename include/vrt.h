@@ -161,6 +161,10 @@ VRT_API int vrt_upload(vrt_context ctx, void *dst, const void *src, size_t bytes
 VRT_API int vrt_download(vrt_context ctx, void *dst, const void *src, size_t bytes);
 VRT_API int vrt_memset(vrt_context ctx, void *dst, int value, size_t bytes);
 
+/* Device-to-device copy by a plain streaming kernel (16 B per lane), also
+ * the calibration kernel for HBM byte counters. */
+VRT_API int vrt_device_copy(vrt_context ctx, void *dst, const void *src, size_t bytes);
+
 /* Event timing on the context stream: record a start/stop pair around
  * enqueued work and read the elapsed milliseconds. */
 VRT_API int vrt_timer_start(vrt_context ctx);
