@@ -1,0 +1,152 @@
+// C ABI of the neural radiance cache (include/vrt.h, vrt_nrc_*).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "abi_internal.h"
+#include "nrc_kernels.h"
+
+struct vrt_nrc_s {
+  vrt_context ctx = nullptr;
+  vrt_nrc_config cfg{};
+  vrt::NrcDevice d{};
+  int max_workgroups = 0;
+  int step = 0;
+};
+
+namespace {
+
+using namespace vrt;
+
+void release(vrt_nrc n) {
+  for (void *p : {(void *)n->d.master, (void *)n->d.adam_m, (void *)n->d.adam_v, (void *)n->d.w,
+                  (void *)n->d.wt, (void *)n->d.bias, (void *)n->d.grad, (void *)n->d.partials,
+                  (void *)n->d.stats})
+    if (p) (void)hipFree(p);
+}
+
+int check_nrc(vrt_nrc n) {
+  if (!n) return fail(VRT_ERROR_INVALID_ARGUMENT, "null cache");
+  return check_ctx(n->ctx);
+}
+
+}  // namespace
+
+extern "C" {
+
+int vrt_nrc_param_count(void) { return kNrcParams; }
+
+int vrt_nrc_create(vrt_context ctx, const vrt_nrc_config *cfg, vrt_nrc *out) {
+  if (int r = check_ctx(ctx)) return r;
+  if (!cfg || !out) return fail(VRT_ERROR_INVALID_ARGUMENT, "null argument");
+  if (cfg->precision != VRT_NRC_FP16 && cfg->precision != VRT_NRC_BF16)
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "precision must be VRT_NRC_FP16 or VRT_NRC_BF16");
+  if (cfg->max_batch < 1 || cfg->max_batch > (1 << 24))
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "max_batch %d out of range", cfg->max_batch);
+  auto *n = new vrt_nrc_s;
+  n->ctx = ctx;
+  n->cfg = *cfg;
+  // one partial gradient per training workgroup (64 samples per chunk)
+  n->max_workgroups = std::max(1, std::min((cfg->max_batch + 63) / 64, 2 * std::max(1, ctx->num_cus)));
+  NrcDevice &d = n->d;
+  d.bf16 = cfg->precision == VRT_NRC_BF16;
+  hipError_t e = hipSuccess;
+  auto A = [&](void **p, size_t bytes) {
+    if (e == hipSuccess) e = hipMalloc(p, bytes);
+  };
+  A((void **)&d.master, sizeof(float) * kNrcParams);
+  A((void **)&d.adam_m, sizeof(float) * kNrcParams);
+  A((void **)&d.adam_v, sizeof(float) * kNrcParams);
+  A((void **)&d.w, sizeof(uint16_t) * kNrcWeights);
+  A((void **)&d.wt, sizeof(uint16_t) * kNrcWeights);
+  A((void **)&d.bias, sizeof(float) * kNrcBiases);
+  A((void **)&d.grad, sizeof(float) * kNrcParams);
+  A((void **)&d.partials, sizeof(float) * kNrcParams * size_t(n->max_workgroups));
+  A((void **)&d.stats, sizeof(float) * 4);
+  if (e == hipSuccess) e = nrc_launch_init(d, cfg->seed, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    release(n);
+    delete n;
+    return hip_fail(e, "cache creation");
+  }
+  *out = n;
+  return VRT_OK;
+}
+
+int vrt_nrc_destroy(vrt_nrc n) {
+  if (!n) return VRT_OK;
+  (void)hipSetDevice(n->ctx->device);
+  (void)hipStreamSynchronize(n->ctx->stream);
+  release(n);
+  delete n;
+  return VRT_OK;
+}
+
+int vrt_nrc_get_params(vrt_nrc n, float *params) {
+  if (int r = check_nrc(n)) return r;
+  if (!params) return fail(VRT_ERROR_INVALID_ARGUMENT, "null output");
+  VRT_HIP(hipMemcpyAsync(params, n->d.master, sizeof(float) * kNrcParams, hipMemcpyDeviceToHost,
+                         n->ctx->stream));
+  VRT_HIP(hipStreamSynchronize(n->ctx->stream));
+  return VRT_OK;
+}
+
+int vrt_nrc_set_params(vrt_nrc n, const float *params) {
+  if (int r = check_nrc(n)) return r;
+  if (!params) return fail(VRT_ERROR_INVALID_ARGUMENT, "null input");
+  VRT_HIP(hipMemcpyAsync(n->d.master, params, sizeof(float) * kNrcParams, hipMemcpyHostToDevice,
+                         n->ctx->stream));
+  VRT_HIP(nrc_launch_refresh(n->d, n->ctx->stream));
+  VRT_HIP(hipStreamSynchronize(n->ctx->stream));
+  return VRT_OK;
+}
+
+int vrt_nrc_infer(vrt_nrc n, int count, const float *raw, float *rgb) {
+  if (int r = check_nrc(n)) return r;
+  if (count < 0 || (count > 0 && (!raw || !rgb)))
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "bad inference arguments");
+  VRT_HIP(nrc_launch_infer(n->d, count, raw, rgb, n->ctx->num_cus, n->ctx->stream));
+  return VRT_OK;
+}
+
+int vrt_nrc_gradients(vrt_nrc n, int count, const float *raw, const float *target, float *grad,
+                      float *loss) {
+  if (int r = check_nrc(n)) return r;
+  if (count < 0 || count > n->cfg.max_batch || (count > 0 && (!raw || !target)))
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "bad gradient arguments (n=%d, max_batch=%d)", count,
+                n->cfg.max_batch);
+  VRT_HIP(nrc_launch_gradients(n->d, count, raw, target, n->max_workgroups, n->ctx->stream));
+  if (grad)
+    VRT_HIP(hipMemcpyAsync(grad, n->d.grad, sizeof(float) * kNrcParams, hipMemcpyDeviceToDevice,
+                           n->ctx->stream));
+  if (loss) {
+    VRT_HIP(hipMemcpyAsync(loss, n->d.stats, sizeof(float), hipMemcpyDeviceToHost, n->ctx->stream));
+    VRT_HIP(hipStreamSynchronize(n->ctx->stream));
+  }
+  return VRT_OK;
+}
+
+int vrt_nrc_train_step(vrt_nrc n, int count, const float *raw, const float *target, float *loss) {
+  if (int r = vrt_nrc_gradients(n, count, raw, target, nullptr, nullptr)) return r;
+  if (count == 0) return VRT_OK;
+  n->step++;
+  VRT_HIP(nrc_launch_adam(n->d, count, n->cfg.learning_rate, n->cfg.beta1, n->cfg.beta2,
+                          n->cfg.epsilon, n->step, n->ctx->stream));
+  if (loss) {
+    VRT_HIP(hipMemcpyAsync(loss, n->d.stats, sizeof(float), hipMemcpyDeviceToHost, n->ctx->stream));
+    VRT_HIP(hipStreamSynchronize(n->ctx->stream));
+  }
+  return VRT_OK;
+}
+
+int vrt_mfma_probe(vrt_context ctx, const float *a, const float *b, float *c, int precision) {
+  if (int r = check_ctx(ctx)) return r;
+  if (!a || !b || !c) return fail(VRT_ERROR_INVALID_ARGUMENT, "null buffer");
+  VRT_HIP(nrc_launch_mfma_probe(a, b, c, precision == VRT_NRC_BF16, ctx->stream));
+  return VRT_OK;
+}
+
+}  // extern "C"

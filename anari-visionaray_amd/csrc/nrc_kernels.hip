@@ -1,0 +1,542 @@
+// Neural radiance cache on gfx950 matrix cores.
+//
+// Every layer is evaluated transposed, Z^T (out x M) = W (out x in) X^T
+// (in x M), with v_mfma_f32_16x16x16 (fp16 or bf16 operands, fp32
+// accumulation).  Lane l of a wave then holds, for a 16x16 block,
+//   A operand: W[l%16][4*(l/16) + j]        (j = 0..3)
+//   B operand: X^T[4*(l/16) + j][l%16]
+//   D result : Z^T[4*(l/16) + j][l%16]
+// so the accumulator of output block b of one layer, after bias + ReLU and
+// rounding to 16 bits, *is* the B operand of input block b of the next
+// layer: activations stay in registers from the encoding to the output.
+//
+// Training keeps the same per-wave forward (16 samples per wave, 64 per
+// workgroup), stores each layer's input X^T and the gradient dZ^T in LDS
+// ([feature][sample], 16-bit), back-propagates dX^T = W^T dZ^T per wave in
+// registers, and forms dW = dZ^T X over the workgroup's 64 samples with one
+// MFMA chain per 16x16 weight tile (tiles dealt round-robin to the 4 waves,
+// accumulated in registers across the workgroup's sample chunks).  Partial
+// gradients per workgroup are summed in a fixed order (deterministic), then
+// Adam updates the fp32 master weights and re-emits the 16-bit copies.
+#include "nrc_kernels.h"
+
+#include <algorithm>
+
+namespace vrt {
+
+typedef _Float16 h4v __attribute__((ext_vector_type(4)));
+typedef short s4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// 16-bit conversions (fp16: IEEE round-to-nearest-even; bf16: RNE on the
+// upper half of the fp32 pattern)
+// ---------------------------------------------------------------------------
+__device__ inline uint16_t to16(float f, int bf16) {
+  if (bf16) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return uint16_t((u >> 16) | 0x40u);  // NaN
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+  }
+  _Float16 h = (_Float16)f;
+  return __builtin_bit_cast(uint16_t, h);
+}
+__device__ inline float from16(uint16_t b, int bf16) {
+  if (bf16) return __uint_as_float(uint32_t(b) << 16);
+  return (float)__builtin_bit_cast(_Float16, b);
+}
+
+struct Frag {  // 4 x 16-bit operand of one lane
+  uint2 v;
+};
+__device__ inline Frag pack4(float a, float b, float c, float d, int bf16) {
+  Frag f;
+  f.v.x = uint32_t(to16(a, bf16)) | (uint32_t(to16(b, bf16)) << 16);
+  f.v.y = uint32_t(to16(c, bf16)) | (uint32_t(to16(d, bf16)) << 16);
+  return f;
+}
+__device__ inline float unpack(const Frag &f, int j, int bf16) {
+  const uint32_t w = j < 2 ? f.v.x : f.v.y;
+  return from16(uint16_t((j & 1) ? (w >> 16) : (w & 0xFFFFu)), bf16);
+}
+
+template <int BF16>
+__device__ inline f4v mfma(const Frag &a, const Frag &b, f4v c) {
+  if constexpr (BF16) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s4v, a.v),
+                                                     __builtin_bit_cast(s4v, b.v), c, 0, 0, 0);
+  } else {
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(h4v, a.v),
+                                                 __builtin_bit_cast(h4v, b.v), c, 0, 0, 0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Input encoding: feature f < 120 is sin/cos(2^k pi x_s) with s = f / 24,
+// k = (f % 24) / 2, sin for even f; f = 120 is the density; 121..127 pad.
+// v_sin/v_cos take revolutions: 2^k pi x = 2 pi * frac(2^(k-1) x), and the
+// scaling by a power of two and the fraction are exact.
+// ---------------------------------------------------------------------------
+__device__ inline float encode(const float *raw, int f) {
+  if (f < 120) {
+    const int s = f / 24, w = f % 24, k = w >> 1;
+    float r = raw[s] * __uint_as_float(uint32_t(126 + k) << 23);  // * 2^(k-1)
+    r = r - floorf(r);
+    return (w & 1) ? __builtin_amdgcn_cosf(r) : __builtin_amdgcn_sinf(r);
+  }
+  return f == 120 ? raw[5] : 0.f;
+}
+
+__device__ inline void load_raw(const float *raw, int n, int q, float r[6]) {
+  if (q < n) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) r[i] = raw[6 * size_t(q) + i];
+  } else {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) r[i] = 0.f;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Forward pass of QB 16-query blocks per wave.  `hin` holds the layer-0 B
+// operands (8 k-blocks); the 3 outputs of query block qb are returned in
+// out[qb] (rows 0..3 of lanes 0..15).  Weights come through `wrow(l, row,
+// col)` -> pointer to 4 consecutive 16-bit weights.
+// ---------------------------------------------------------------------------
+template <int BF16, int QB, class WPtr, class Store>
+__device__ inline void forward(const WPtr &wrow, const float *bias, Frag (&hin)[8][QB],
+                               f4v (&out)[QB], const Store &store_act) {
+  const int lane = threadIdx.x & 63;
+  const int r16 = lane & 15, g = lane >> 4;
+  Frag h[4][QB];
+  // layer 0: 128 -> 64
+  {
+    f4v acc[4][QB];
+#pragma unroll
+    for (int ob = 0; ob < 4; ++ob)
+#pragma unroll
+      for (int qb = 0; qb < QB; ++qb)
+        for (int j = 0; j < 4; ++j) acc[ob][qb][j] = bias[nrc_boff(0) + ob * 16 + 4 * g + j];
+#pragma unroll
+    for (int kb = 0; kb < 8; ++kb)
+#pragma unroll
+      for (int ob = 0; ob < 4; ++ob) {
+        Frag a;
+        a.v = *reinterpret_cast<const uint2 *>(wrow(0, ob * 16 + r16, kb * 16 + 4 * g));
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb) acc[ob][qb] = mfma<BF16>(a, hin[kb][qb], acc[ob][qb]);
+      }
+#pragma unroll
+    for (int ob = 0; ob < 4; ++ob)
+#pragma unroll
+      for (int qb = 0; qb < QB; ++qb) {
+        h[ob][qb] = pack4(fmaxf(acc[ob][qb][0], 0.f), fmaxf(acc[ob][qb][1], 0.f),
+                          fmaxf(acc[ob][qb][2], 0.f), fmaxf(acc[ob][qb][3], 0.f), BF16);
+        store_act(1, ob, qb, h[ob][qb]);
+      }
+  }
+  // hidden layers 1..4: 64 -> 64
+#pragma unroll 1
+  for (int l = 1; l <= 4; ++l) {
+    f4v acc[4][QB];
+#pragma unroll
+    for (int ob = 0; ob < 4; ++ob)
+#pragma unroll
+      for (int qb = 0; qb < QB; ++qb)
+        for (int j = 0; j < 4; ++j) acc[ob][qb][j] = bias[nrc_boff(l) + ob * 16 + 4 * g + j];
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+      for (int ob = 0; ob < 4; ++ob) {
+        Frag a;
+        a.v = *reinterpret_cast<const uint2 *>(wrow(l, ob * 16 + r16, kb * 16 + 4 * g));
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb) acc[ob][qb] = mfma<BF16>(a, h[kb][qb], acc[ob][qb]);
+      }
+#pragma unroll
+    for (int ob = 0; ob < 4; ++ob)
+#pragma unroll
+      for (int qb = 0; qb < QB; ++qb) {
+        h[ob][qb] = pack4(fmaxf(acc[ob][qb][0], 0.f), fmaxf(acc[ob][qb][1], 0.f),
+                          fmaxf(acc[ob][qb][2], 0.f), fmaxf(acc[ob][qb][3], 0.f), BF16);
+        store_act(l + 1, ob, qb, h[ob][qb]);
+      }
+  }
+  // output layer 5: 64 -> 16 (3 used), linear
+#pragma unroll
+  for (int qb = 0; qb < QB; ++qb)
+    for (int j = 0; j < 4; ++j) out[qb][j] = bias[nrc_boff(5) + 4 * g + j];
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) {
+    Frag a;
+    a.v = *reinterpret_cast<const uint2 *>(wrow(5, r16, kb * 16 + 4 * g));
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) out[qb] = mfma<BF16>(a, h[kb][qb], out[qb]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Inference: persistent workgroups, weights staged in LDS with rows padded by
+// 4 halves (row stride 8 B off a multiple of 128 B: the 16 rows one
+// ds_read_b64 touches spread over all banks); 64 queries per wave.
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int lds_stride(int l) { return nrc_in(l) + 4; }
+__host__ __device__ constexpr int lds_woff(int l) {
+  return l == 0 ? 0 : 64 * lds_stride(0) + (l - 1) * 64 * lds_stride(1);
+}
+constexpr int kLdsWeights = 64 * (128 + 4) + 4 * 64 * (64 + 4) + 16 * (64 + 4);
+
+template <int BF16>
+__global__ void __launch_bounds__(256) k_nrc_infer(NrcDevice d, int n, const float *__restrict__ raw,
+                                                   float *__restrict__ out) {
+  __shared__ uint16_t sw[kLdsWeights];
+  __shared__ float sb[kNrcBiases];
+  for (int l = 0; l < kNrcLayers; ++l) {
+    const int rows = nrc_out(l), cols = nrc_in(l);
+    for (int i = threadIdx.x; i < rows * cols; i += blockDim.x)
+      sw[lds_woff(l) + (i / cols) * lds_stride(l) + (i % cols)] = d.w[nrc_woff(l) + i];
+  }
+  for (int i = threadIdx.x; i < kNrcBiases; i += blockDim.x) sb[i] = d.bias[i];
+  __syncthreads();
+
+  auto wrow = [&](int l, int row, int col) -> const uint16_t * {
+    return &sw[lds_woff(l) + row * lds_stride(l) + col];
+  };
+  auto no_store = [](int, int, int, const Frag &) {};
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r16 = lane & 15, g = lane >> 4;
+  constexpr int QB = 4;
+  const int tiles = (n + 63) / 64;  // 64 queries (QB blocks of 16) per wave-tile
+  for (int t = blockIdx.x * 4 + wave; t < tiles; t += gridDim.x * 4) {
+    Frag hin[8][QB];
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+      float r[6];
+      load_raw(raw, n, t * 64 + qb * 16 + r16, r);
+#pragma unroll
+      for (int kb = 0; kb < 8; ++kb) {
+        const int f = kb * 16 + 4 * g;
+        hin[kb][qb] = pack4(encode(r, f), encode(r, f + 1), encode(r, f + 2), encode(r, f + 3), BF16);
+      }
+    }
+    f4v o[QB];
+    forward<BF16, QB>(wrow, sb, hin, o, no_store);
+    if (g == 0) {
+#pragma unroll
+      for (int qb = 0; qb < QB; ++qb) {
+        const int q = t * 64 + qb * 16 + r16;
+        if (q < n) {
+          out[3 * size_t(q)] = o[qb][0];
+          out[3 * size_t(q) + 1] = o[qb][1];
+          out[3 * size_t(q) + 2] = o[qb][2];
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Training gradients (relative L2 loss, Mueller et al. 2021):
+//   L = sum_q sum_c (p - t)^2 / (p^2 + 0.01),  dL/dp = 2 (p - t) / (p^2 + 0.01)
+// with p^2 in the denominator treated as a constant.
+// ---------------------------------------------------------------------------
+constexpr int kTS = 64 + 4;  // LDS row stride of [feature][sample] tiles (halves)
+
+template <int BF16>
+__global__ void __launch_bounds__(256) k_nrc_grad(NrcDevice d, int n, const float *__restrict__ raw,
+                                                  const float *__restrict__ target) {
+  __shared__ uint16_t sx0[128 * kTS];      // layer-0 input X^T [feature][sample]
+  __shared__ uint16_t sx[5][64 * kTS];     // inputs of layers 1..5
+  __shared__ uint16_t sdz[64 * kTS];       // dZ^T of the current layer [out][sample]
+  __shared__ float sdb[kNrcBiases];        // bias-gradient sums of this workgroup
+  __shared__ float sloss[4];
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r16 = lane & 15, g = lane >> 4;
+  for (int i = threadIdx.x; i < kNrcBiases; i += blockDim.x) sdb[i] = 0.f;
+  if (threadIdx.x < 4) sloss[threadIdx.x] = 0.f;
+
+  // dW accumulators: tiles of each layer dealt round-robin to the waves
+  // layer 0: 4x8 tiles (8 per wave), layers 1-4: 4x4 (4 per wave), layer 5:
+  // 1x4 (1 per wave)
+  f4v acc0[8], acc14[4][4], acc5;
+  for (int i = 0; i < 8; ++i) acc0[i] = f4v{0.f, 0.f, 0.f, 0.f};
+  for (int l = 0; l < 4; ++l)
+    for (int i = 0; i < 4; ++i) acc14[l][i] = f4v{0.f, 0.f, 0.f, 0.f};
+  acc5 = f4v{0.f, 0.f, 0.f, 0.f};
+
+  auto wrow = [&](int l, int row, int col) -> const uint16_t * {
+    return &d.w[nrc_woff(l) + row * nrc_in(l) + col];
+  };
+  const int col = wave * 16 + r16;  // this lane's sample column in the chunk
+  auto store_act = [&](int l, int ob, int, const Frag &h) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t w = j < 2 ? h.v.x : h.v.y;
+      sx[l - 1][(ob * 16 + 4 * g + j) * kTS + col] = uint16_t((j & 1) ? (w >> 16) : (w & 0xFFFFu));
+    }
+  };
+
+  const int chunks = (n + 63) / 64;
+  for (int c = blockIdx.x; c < chunks; c += gridDim.x) {
+    const int q = c * 64 + col;
+    const bool valid = q < n;
+    float r[6];
+    load_raw(raw, n, q, r);
+    Frag hin[8][1];
+#pragma unroll
+    for (int kb = 0; kb < 8; ++kb) {
+      const int f = kb * 16 + 4 * g;
+      float e[4];
+      for (int j = 0; j < 4; ++j) e[j] = valid ? encode(r, f + j) : 0.f;
+      hin[kb][0] = pack4(e[0], e[1], e[2], e[3], BF16);
+      for (int j = 0; j < 4; ++j) sx0[(f + j) * kTS + col] = to16(e[j], BF16);
+    }
+    f4v out[1];
+    forward<BF16, 1>(wrow, d.bias, hin, out, store_act);
+
+    // output gradient: rows 0..2 of lanes 0..15 hold the prediction
+    float gz[4] = {0.f, 0.f, 0.f, 0.f};
+    if (g == 0 && valid) {
+      float ls = 0.f;
+      for (int ch = 0; ch < 3; ++ch) {
+        const float p = out[0][ch], t = target[3 * size_t(q) + ch];
+        const float den = p * p + 0.01f;
+        const float diff = p - t;
+        ls += diff * diff / den;
+        gz[ch] = 2.f * diff / den;
+      }
+      atomicAdd(&sloss[wave], ls);
+    }
+    // ---- backward ------------------------------------------------------------
+    // dZ^T of the current layer for this wave's 16 samples, D layout,
+    // [ob][j] -> row ob*16 + 4g + j
+    float dz[4][4];
+    for (int ob = 0; ob < 4; ++ob)
+      for (int j = 0; j < 4; ++j) dz[ob][j] = 0.f;
+    for (int j = 0; j < 4; ++j) dz[0][j] = gz[j];
+    __syncthreads();  // forward activations of all waves are in LDS
+
+#pragma unroll
+    for (int l = 5; l >= 0; --l) {
+      const int nob = nrc_out(l) / 16, nib = nrc_in(l) / 16;
+      // stage dZ^T, accumulate the bias gradient
+      Frag dzf[4];
+      for (int ob = 0; ob < nob; ++ob) {
+        dzf[ob] = pack4(dz[ob][0], dz[ob][1], dz[ob][2], dz[ob][3], BF16);
+        for (int j = 0; j < 4; ++j) {
+          sdz[(ob * 16 + 4 * g + j) * kTS + col] = to16(dz[ob][j], BF16);
+          float s = dz[ob][j];
+          s += __shfl_xor(s, 1, 64);
+          s += __shfl_xor(s, 2, 64);
+          s += __shfl_xor(s, 4, 64);
+          s += __shfl_xor(s, 8, 64);
+          if (r16 == 0) atomicAdd(&sdb[nrc_boff(l) + ob * 16 + 4 * g + j], s);
+        }
+      }
+      __syncthreads();
+      // dW tiles (ob, ib) over the chunk's 64 samples
+      const uint16_t *xin = l == 0 ? sx0 : sx[l - 1];
+      const int ntiles = nob * nib;
+      for (int t = wave, k = 0; t < ntiles; t += 4, ++k) {
+        const int ob = t / nib, ib = t % nib;
+        f4v acc = l == 0 ? acc0[k] : (l == 5 ? acc5 : acc14[l - 1][k]);
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) {
+          Frag a, b;
+          a.v = *reinterpret_cast<const uint2 *>(&sdz[(ob * 16 + r16) * kTS + kb * 16 + 4 * g]);
+          b.v = *reinterpret_cast<const uint2 *>(&xin[(ib * 16 + r16) * kTS + kb * 16 + 4 * g]);
+          acc = mfma<BF16>(a, b, acc);
+        }
+        if (l == 0)
+          acc0[k] = acc;
+        else if (l == 5)
+          acc5 = acc;
+        else
+          acc14[l - 1][k] = acc;
+      }
+      if (l > 0) {
+        // dX^T = W^T dZ^T, then the ReLU mask of layer l-1's output (= X of l)
+        for (int ib = 0; ib < nib; ++ib) {
+          f4v acc = f4v{0.f, 0.f, 0.f, 0.f};
+          for (int kb = 0; kb < nob; ++kb) {
+            Frag a;
+            a.v = *reinterpret_cast<const uint2 *>(
+                &d.wt[nrc_woff(l) + (ib * 16 + r16) * nrc_out(l) + kb * 16 + 4 * g]);
+            acc = mfma<BF16>(a, dzf[kb], acc);
+          }
+          for (int j = 0; j < 4; ++j) {
+            const float x = from16(xin[(ib * 16 + 4 * g + j) * kTS + col], BF16);
+            dz[ib][j] = x > 0.f ? acc[j] : 0.f;
+          }
+        }
+      }
+      __syncthreads();  // sdz is rewritten by the next layer
+    }
+  }
+
+  // ---- write this workgroup's partial gradient ---------------------------------
+  float *part = d.partials + size_t(blockIdx.x) * kNrcParams;
+  auto put = [&](int l, int t, const f4v &acc) {
+    const int nib = nrc_in(l) / 16;
+    const int ob = t / nib, ib = t % nib;
+    for (int j = 0; j < 4; ++j) part[nrc_w(l, ob * 16 + 4 * g + j, ib * 16 + r16)] = acc[j];
+  };
+  for (int t = wave, k = 0; t < 32; t += 4, ++k) put(0, t, acc0[k]);
+  for (int l = 1; l <= 4; ++l)
+    for (int t = wave, k = 0; t < 16; t += 4, ++k) put(l, t, acc14[l - 1][k]);
+  put(5, wave, acc5);
+  __syncthreads();
+  for (int i = threadIdx.x; i < kNrcBiases; i += blockDim.x) part[kNrcWeights + i] = sdb[i];
+  if (threadIdx.x == 0) atomicAdd(&d.stats[0], sloss[0] + sloss[1] + sloss[2] + sloss[3]);
+}
+
+// fixed-order sum of the workgroup partials
+__global__ void k_nrc_reduce(NrcDevice d, int parts) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kNrcParams) return;
+  float s = 0.f;
+  for (int i = 0; i < parts; ++i) s += d.partials[size_t(i) * kNrcParams + p];
+  d.grad[p] = s;
+}
+
+__device__ inline void emit16(const NrcDevice &d, int p, float v) {
+  if (p < kNrcWeights) {
+    int l = 5;
+    while (l > 0 && p < nrc_woff(l)) --l;
+    const int local = p - nrc_woff(l);
+    const int o = local / nrc_in(l), i = local % nrc_in(l);
+    const uint16_t b = to16(v, d.bf16);
+    d.w[p] = b;
+    d.wt[nrc_woff(l) + i * nrc_out(l) + o] = b;
+  } else {
+    d.bias[p - kNrcWeights] = v;
+  }
+}
+
+// Adam on the fp32 master parameters (Kingma & Ba 2015, bias-corrected)
+__global__ void k_nrc_adam(NrcDevice d, float inv_n, float lr, float b1, float b2, float eps,
+                           float c1, float c2) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kNrcParams) return;
+  const float gr = d.grad[p] * inv_n;
+  const float m = b1 * d.adam_m[p] + (1.f - b1) * gr;
+  const float v = b2 * d.adam_v[p] + (1.f - b2) * gr * gr;
+  d.adam_m[p] = m;
+  d.adam_v[p] = v;
+  const float w = d.master[p] - lr * (m * c1) / (sqrtf(v * c2) + eps);
+  d.master[p] = w;
+  emit16(d, p, w);
+}
+
+__global__ void k_nrc_refresh(NrcDevice d) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kNrcParams) return;
+  emit16(d, p, d.master[p]);
+}
+
+// He-uniform weights (bound sqrt(6/fan_in)) from a counter hash, zero pads
+// and biases, zero Adam moments
+__global__ void k_nrc_init(NrcDevice d, uint64_t seed) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kNrcParams) return;
+  float w = 0.f;
+  if (p < kNrcWeights) {
+    int l = 5;
+    while (l > 0 && p < nrc_woff(l)) --l;
+    const int local = p - nrc_woff(l);
+    const int o = local / nrc_in(l), i = local % nrc_in(l);
+    const int fan_in = l == 0 ? kNrcInputs : kNrcWidth;
+    const bool real = i < fan_in && (l < 5 ? o < kNrcWidth : o < 3);
+    if (real) {
+      uint64_t z = seed + 0x9E3779B97F4A7C15ull * uint64_t(p + 1);
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      z ^= z >> 31;
+      const float u = float(z >> 40) * (1.f / 16777216.f);
+      w = (2.f * u - 1.f) * sqrtf(6.f / float(fan_in));
+    }
+  }
+  d.master[p] = w;
+  d.adam_m[p] = 0.f;
+  d.adam_v[p] = 0.f;
+  emit16(d, p, w);
+}
+
+// one-wave MFMA lane-layout probe
+template <int BF16>
+__global__ void k_mfma_probe(const float *a, const float *b, float *c) {
+  const int l = threadIdx.x & 63, r16 = l & 15, g = l >> 4;
+  Frag fa = pack4(a[r16 * 16 + 4 * g], a[r16 * 16 + 4 * g + 1], a[r16 * 16 + 4 * g + 2],
+                  a[r16 * 16 + 4 * g + 3], BF16);
+  Frag fb = pack4(b[(4 * g) * 16 + r16], b[(4 * g + 1) * 16 + r16], b[(4 * g + 2) * 16 + r16],
+                  b[(4 * g + 3) * 16 + r16], BF16);
+  f4v acc = f4v{0.f, 0.f, 0.f, 0.f};
+  acc = mfma<BF16>(fa, fb, acc);
+  for (int j = 0; j < 4; ++j) c[(4 * g + j) * 16 + r16] = acc[j];
+}
+
+// ---------------------------------------------------------------------------
+hipError_t nrc_launch_init(const NrcDevice &d, uint64_t seed, hipStream_t s) {
+  hipLaunchKernelGGL(k_nrc_init, dim3((kNrcParams + 255) / 256), dim3(256), 0, s, d, seed);
+  return hipGetLastError();
+}
+
+hipError_t nrc_launch_refresh(const NrcDevice &d, hipStream_t s) {
+  hipLaunchKernelGGL(k_nrc_refresh, dim3((kNrcParams + 255) / 256), dim3(256), 0, s, d);
+  return hipGetLastError();
+}
+
+hipError_t nrc_launch_infer(const NrcDevice &d, int n, const float *raw, float *out, int num_cus,
+                            hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int tiles = (n + 63) / 64;
+  const int blocks = std::max(1, std::min((tiles + 3) / 4, 2 * std::max(1, num_cus)));
+  if (d.bf16)
+    hipLaunchKernelGGL(k_nrc_infer<1>, dim3(blocks), dim3(256), 0, s, d, n, raw, out);
+  else
+    hipLaunchKernelGGL(k_nrc_infer<0>, dim3(blocks), dim3(256), 0, s, d, n, raw, out);
+  return hipGetLastError();
+}
+
+hipError_t nrc_launch_gradients(const NrcDevice &d, int n, const float *raw, const float *target,
+                                int max_workgroups, hipStream_t s) {
+  hipError_t e = hipMemsetAsync(d.stats, 0, 2 * sizeof(float), s);
+  if (e != hipSuccess) return e;
+  const int chunks = (n + 63) / 64;
+  const int parts = std::max(1, std::min(chunks, max_workgroups));
+  if (n > 0) {
+    if (d.bf16)
+      hipLaunchKernelGGL(k_nrc_grad<1>, dim3(parts), dim3(256), 0, s, d, n, raw, target);
+    else
+      hipLaunchKernelGGL(k_nrc_grad<0>, dim3(parts), dim3(256), 0, s, d, n, raw, target);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  } else {
+    e = hipMemsetAsync(d.partials, 0, sizeof(float) * kNrcParams, s);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_nrc_reduce, dim3((kNrcParams + 255) / 256), dim3(256), 0, s, d, parts);
+  return hipGetLastError();
+}
+
+hipError_t nrc_launch_adam(const NrcDevice &d, int n, float lr, float b1, float b2, float eps,
+                           int step, hipStream_t s) {
+  const float c1 = 1.f / (1.f - powf(b1, float(step)));
+  const float c2 = 1.f / (1.f - powf(b2, float(step)));
+  hipLaunchKernelGGL(k_nrc_adam, dim3((kNrcParams + 255) / 256), dim3(256), 0, s, d,
+                     n > 0 ? 1.f / float(n) : 0.f, lr, b1, b2, eps, c1, c2);
+  return hipGetLastError();
+}
+
+hipError_t nrc_launch_mfma_probe(const float *a, const float *b, float *c, int bf16,
+                                 hipStream_t s) {
+  if (bf16)
+    hipLaunchKernelGGL(k_mfma_probe<1>, dim3(1), dim3(64), 0, s, a, b, c);
+  else
+    hipLaunchKernelGGL(k_mfma_probe<0>, dim3(1), dim3(64), 0, s, a, b, c);
+  return hipGetLastError();
+}
+
+}  // namespace vrt

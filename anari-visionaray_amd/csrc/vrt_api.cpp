@@ -9,14 +9,14 @@
 #include <cstring>
 #include <string>
 
+#include "abi_internal.h"
 #include "render_kernels.h"
 #include "vrt.h"
 
-namespace {
+namespace vrt {
 
-thread_local std::string g_error;
+static thread_local std::string g_error;
 
-int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 int fail(int code, const char *fmt, ...) {
   char buf[512];
   va_list ap;
@@ -31,35 +31,18 @@ int hip_fail(hipError_t e, const char *what) {
   return fail(VRT_ERROR_HIP, "%s: %s", what, hipGetErrorString(e));
 }
 
-#define VRT_HIP(call)                                \
-  do {                                               \
-    hipError_t e_ = (call);                          \
-    if (e_ != hipSuccess) return hip_fail(e_, #call); \
-  } while (0)
-
-}  // namespace
-
-struct vrt_context_s {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t t0 = nullptr, t1 = nullptr;
-  int num_cus = 0;
-  // work-queue counters of the persistent path tracer: two sets of 8, used
-  // alternately; each launch clears the set the next launch uses
-  uint32_t *queues = nullptr;
-  int queue_parity = 0;
-};
-
-namespace {
-
-using namespace vrt;
-
 int check_ctx(vrt_context ctx) {
   if (!ctx) return fail(VRT_ERROR_INVALID_ARGUMENT, "null context");
   hipError_t e = hipSetDevice(ctx->device);
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
   return VRT_OK;
 }
+
+}  // namespace vrt
+
+namespace {
+
+using namespace vrt;
 
 int make_volume(const vrt_volume &in, int lds_offset, KVolume &v) {
   if (in.field_type != VRT_FIELD_STRUCTURED_REGULAR)
@@ -212,7 +195,7 @@ int make_frame(const vrt_frame *f, KFrame &k) {
 
 extern "C" {
 
-const char *vrt_last_error(void) { return g_error.c_str(); }
+const char *vrt_last_error(void) { return vrt::g_error.c_str(); }
 
 int vrt_context_create(int device, vrt_context *out) {
   if (!out) return fail(VRT_ERROR_INVALID_ARGUMENT, "null output");

@@ -85,6 +85,17 @@ class Frame(C.Structure):
     ]
 
 
+class NrcConfig(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int), ("learning_rate", C.c_float), ("beta1", C.c_float),
+        ("beta2", C.c_float), ("epsilon", C.c_float), ("max_batch", C.c_int),
+        ("seed", C.c_uint64),
+    ]
+
+
+NRC_FP16 = 0
+NRC_BF16 = 1
+
 _lib = None
 _gen = None
 
@@ -116,6 +127,15 @@ def lib():
         L.vrt_build_value_ranges.argtypes = [vp, C.POINTER(Volume), vp]
         L.vrt_compute_majorants.argtypes = [vp, C.POINTER(Volume), vp, vp]
         L.vrt_frame_reset.argtypes = [vp, C.POINTER(Frame)]
+        L.vrt_nrc_param_count.restype = ip
+        L.vrt_nrc_create.argtypes = [vp, C.POINTER(NrcConfig), C.POINTER(vp)]
+        L.vrt_nrc_destroy.argtypes = [vp]
+        L.vrt_nrc_get_params.argtypes = [vp, vp]
+        L.vrt_nrc_set_params.argtypes = [vp, vp]
+        L.vrt_nrc_infer.argtypes = [vp, ip, vp, vp]
+        L.vrt_nrc_gradients.argtypes = [vp, ip, vp, vp, vp, C.POINTER(C.c_float)]
+        L.vrt_nrc_train_step.argtypes = [vp, ip, vp, vp, C.POINTER(C.c_float)]
+        L.vrt_mfma_probe.argtypes = [vp, vp, vp, vp, ip]
         L.vrt_sample_field.argtypes = [vp, C.POINTER(Volume), ip, vp, vp]
         L.vrt_delta_track.argtypes = [vp, C.POINTER(Scene), ip, vp, vp, vp]
         L.vrt_raycast.argtypes = [vp, C.POINTER(Scene), C.POINTER(Camera), C.POINTER(Params),
@@ -426,3 +446,73 @@ def raycast(ctx, scene, cam, prm, frame, frame_id=0):
 def pathtrace(ctx, scene, cam, prm, frame, accum_id, frame_id=0):
     check(lib().vrt_pathtrace(ctx.handle, C.byref(scene), C.byref(cam), C.byref(prm),
                               C.byref(frame.desc), accum_id, frame_id))
+
+
+class Nrc:
+    """The neural radiance cache: fp32 master weights + Adam state on the GPU."""
+
+    def __init__(self, ctx, precision=NRC_FP16, learning_rate=1e-2, beta1=0.9, beta2=0.99,
+                 epsilon=1e-8, max_batch=65536, seed=0):
+        self.ctx = ctx
+        cfg = NrcConfig(precision, learning_rate, beta1, beta2, epsilon, max_batch, seed)
+        h = C.c_void_p()
+        check(lib().vrt_nrc_create(ctx.handle, C.byref(cfg), C.byref(h)))
+        self.handle = h.value
+        self.nparams = lib().vrt_nrc_param_count()
+
+    def params(self):
+        out = np.empty(self.nparams, np.float32)
+        check(lib().vrt_nrc_get_params(self.handle, out.ctypes.data))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.nparams
+        check(lib().vrt_nrc_set_params(self.handle, p.ctypes.data))
+
+    def infer(self, raw):
+        raw = np.ascontiguousarray(raw, np.float32).reshape(-1, 6)
+        n = raw.shape[0]
+        dr = self.ctx.upload(raw)
+        do = self.ctx.buffer(max(4, n * 12))
+        check(lib().vrt_nrc_infer(self.handle, n, dr.ptr, do.ptr))
+        out = do.download(np.float32, (n, 3))
+        dr.free()
+        do.free()
+        return out
+
+    def gradients(self, raw, target):
+        raw = np.ascontiguousarray(raw, np.float32).reshape(-1, 6)
+        target = np.ascontiguousarray(target, np.float32).reshape(-1, 3)
+        n = raw.shape[0]
+        dr, dt = self.ctx.upload(raw), self.ctx.upload(target)
+        dg = self.ctx.buffer(self.nparams * 4)
+        loss = C.c_float()
+        check(lib().vrt_nrc_gradients(self.handle, n, dr.ptr, dt.ptr, dg.ptr, C.byref(loss)))
+        g = dg.download(np.float32, (self.nparams,))
+        for b in (dr, dt, dg):
+            b.free()
+        return g, loss.value
+
+    def train_step(self, raw_buf, target_buf, n):
+        """One Adam step on n device-resident samples; returns the loss sum."""
+        loss = C.c_float()
+        check(lib().vrt_nrc_train_step(self.handle, n, raw_buf.ptr, target_buf.ptr,
+                                       C.byref(loss)))
+        return loss.value
+
+    def close(self):
+        if self.handle:
+            check(lib().vrt_nrc_destroy(self.handle))
+            self.handle = None
+
+
+def mfma_probe(ctx, a, b, precision=NRC_FP16):
+    da = ctx.upload(np.ascontiguousarray(a, np.float32))
+    db = ctx.upload(np.ascontiguousarray(b, np.float32))
+    dc = ctx.buffer(16 * 16 * 4)
+    check(lib().vrt_mfma_probe(ctx.handle, da.ptr, db.ptr, dc.ptr, precision))
+    c = dc.download(np.float32, (16, 16))
+    for x in (da, db, dc):
+        x.free()
+    return c

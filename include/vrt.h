@@ -230,6 +230,55 @@ VRT_API int vrt_pathtrace(vrt_context ctx, const vrt_scene *scene,
                           const vrt_camera *camera, const vrt_params *params,
                           const vrt_frame *frame, int accum_id, int frame_id);
 
+/* ---------------------------------------------------------------------- */
+/* Neural radiance cache (no counterpart in the reference)                 */
+/* ---------------------------------------------------------------------- */
+/* A small MLP mapping a scattering vertex (position in [0,1]^3, direction
+ * as (theta/pi, phi/2pi), local density) to outgoing RGB radiance.
+ * Encoding: each of the 5 position/direction scalars x becomes
+ * sin(2^k pi x), cos(2^k pi x), k = 0..11 (120 features), plus the density
+ * (121 inputs).  Network: 5 hidden layers of 64 ReLU units, 3 linear
+ * outputs; fp16 or bf16 weights and activations on the matrix cores with
+ * fp32 accumulation; fp32 master weights trained with Adam on the relative
+ * L2 loss sum (p-t)^2 / (p^2 + 0.01) (p^2 not differentiated).
+ * Parameters are one fp32 vector of vrt_nrc_param_count() values: padded
+ * weights [layer][out][in] (layer 0: 64x128, layers 1-4: 64x64, layer 5:
+ * 16x64) followed by padded biases (5 x 64 + 16). */
+#define VRT_NRC_FP16 0
+#define VRT_NRC_BF16 1
+#define VRT_NRC_RAW_INPUTS 6
+
+typedef struct vrt_nrc_config {
+  int precision;        /* VRT_NRC_FP16 | VRT_NRC_BF16 */
+  float learning_rate;  /* 1e-2 */
+  float beta1, beta2;   /* 0.9, 0.99 */
+  float epsilon;        /* 1e-8 */
+  int max_batch;        /* largest n of one training step */
+  uint64_t seed;        /* weight initialisation */
+} vrt_nrc_config;
+
+typedef struct vrt_nrc_s *vrt_nrc;
+
+VRT_API int vrt_nrc_param_count(void);
+VRT_API int vrt_nrc_create(vrt_context ctx, const vrt_nrc_config *cfg, vrt_nrc *out);
+VRT_API int vrt_nrc_destroy(vrt_nrc nrc);
+/* host fp32 parameter vector in / out (also re-emits the 16-bit copies) */
+VRT_API int vrt_nrc_get_params(vrt_nrc nrc, float *params);
+VRT_API int vrt_nrc_set_params(vrt_nrc nrc, const float *params);
+/* n raw inputs (device, n x 6) -> n RGB predictions (device, n x 3) */
+VRT_API int vrt_nrc_infer(vrt_nrc nrc, int n, const float *raw, float *rgb);
+/* gradient sums over n samples (device inputs/targets n x 3); optional
+ * device output of vrt_nrc_param_count() floats; loss sum to *loss */
+VRT_API int vrt_nrc_gradients(vrt_nrc nrc, int n, const float *raw, const float *target,
+                              float *grad, float *loss);
+/* one Adam step on the mean gradient of n samples; loss sum to *loss */
+VRT_API int vrt_nrc_train_step(vrt_nrc nrc, int n, const float *raw, const float *target,
+                               float *loss);
+/* matrix-core lane-layout check: c (16x16) = a (16x16) b (16x16), device
+ * row-major fp32 in/out, operands rounded to fp16 or bf16 */
+VRT_API int vrt_mfma_probe(vrt_context ctx, const float *a, const float *b, float *c,
+                           int precision);
+
 #ifdef __cplusplus
 }
 #endif

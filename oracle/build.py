@@ -16,7 +16,7 @@ def build():
     lib = os.path.join(OUT, "libvrt_oracle.so")
     cmd = ["g++", "-O3", "-march=x86-64-v3", "-ffp-contract=off", "-std=c++17",
            "-fPIC", "-shared", "-pthread", os.path.join(HERE, "vrt_oracle.cpp"),
-           "-o", lib]
+           os.path.join(HERE, "nrc_oracle.cpp"), "-o", lib]
     print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     return lib

@@ -263,3 +263,47 @@ class OracleWorkload:
                                        C.byref(self.params), W, H, first_accum_id, frames,
                                        int(fp64), stride, out.ctypes.data, threads)
         return out
+
+
+# ---------------------------------------------------------------------------
+# neural radiance cache MLP (oracle/nrc_oracle.cpp)
+# ---------------------------------------------------------------------------
+def _nrc_lib():
+    L = lib()
+    if not getattr(L, "_nrc_ready", False):
+        L.orc_nrc_param_count.restype = C.c_int
+        L.orc_nrc_encode.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_nrc_forward.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+        L.orc_nrc_gradients.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                        C.c_void_p]
+        L.orc_nrc_gradients.restype = C.c_double
+        L._nrc_ready = True
+    return L
+
+
+def nrc_forward(params, raw, precision):
+    L = _nrc_lib()
+    p = np.ascontiguousarray(params, np.float32)
+    r = np.ascontiguousarray(raw, np.float32).reshape(-1, 6)
+    out = np.empty((r.shape[0], 3), np.float64)
+    L.orc_nrc_forward(p.ctypes.data, precision, r.shape[0], r.ctypes.data, out.ctypes.data)
+    return out
+
+
+def nrc_gradients(params, raw, target, precision):
+    L = _nrc_lib()
+    p = np.ascontiguousarray(params, np.float32)
+    r = np.ascontiguousarray(raw, np.float32).reshape(-1, 6)
+    t = np.ascontiguousarray(target, np.float32).reshape(-1, 3)
+    g = np.empty(L.orc_nrc_param_count(), np.float64)
+    loss = L.orc_nrc_gradients(p.ctypes.data, precision, r.shape[0], r.ctypes.data,
+                               t.ctypes.data, g.ctypes.data)
+    return g, loss
+
+
+def nrc_encode(raw):
+    L = _nrc_lib()
+    r = np.ascontiguousarray(raw, np.float32)
+    out = np.empty(128, np.float64)
+    L.orc_nrc_encode(r.ctypes.data, out.ctypes.data)
+    return out
