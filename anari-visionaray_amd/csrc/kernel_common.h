@@ -97,4 +97,22 @@ struct KFrame {
   unsigned long long *counters;
 };
 
+// Per-frame buffers of the radiance-cache path tracer.  Paths that are not
+// training paths end at their second volume vertex and leave a query (raw
+// cache inputs + path throughput); training paths continue and leave one
+// training sample (raw inputs + radiance of the path suffix / throughput).
+struct KNrc {
+  float4 *sample;        // per pixel: sample colour without the cache term
+  int *query_of_pixel;   // per pixel: query index or -1
+  float *q_raw;          // q_cap x 6
+  float *q_T;            // q_cap x 3
+  float *q_out;          // q_cap x 3 cache predictions
+  float *t_raw;          // t_cap x 6
+  float *t_target;       // t_cap x 3
+  uint32_t *ctr;         // [0] queries [1] training claims [2] vertex-2 paths [3] previous [2]
+  int q_cap, t_cap;
+  int train;             // record training paths this frame
+  f3 pos_lo, pos_scale;  // world -> [0,1]^3 cache coordinates
+};
+
 }  // namespace vrt

@@ -6,25 +6,31 @@
 #include <vector>
 
 #include "abi_internal.h"
+#include "nrc_internal.h"
 #include "nrc_kernels.h"
-
-struct vrt_nrc_s {
-  vrt_context ctx = nullptr;
-  vrt_nrc_config cfg{};
-  vrt::NrcDevice d{};
-  int max_workgroups = 0;
-  int step = 0;
-};
 
 namespace {
 
 using namespace vrt;
+
+void release_frame(vrt_nrc n) {
+  for (void *p : {(void *)n->sample, (void *)n->query_of_pixel, (void *)n->q_raw, (void *)n->q_T,
+                  (void *)n->q_out, (void *)n->t_raw, (void *)n->t_target, (void *)n->ctr})
+    if (p) (void)hipFree(p);
+  n->sample = nullptr;
+  n->query_of_pixel = nullptr;
+  n->q_raw = n->q_T = n->q_out = n->t_raw = n->t_target = nullptr;
+  n->ctr = nullptr;
+  n->pixels = 0;
+  n->t_cap = 0;
+}
 
 void release(vrt_nrc n) {
   for (void *p : {(void *)n->d.master, (void *)n->d.adam_m, (void *)n->d.adam_v, (void *)n->d.w,
                   (void *)n->d.wt, (void *)n->d.bias, (void *)n->d.grad, (void *)n->d.partials,
                   (void *)n->d.stats})
     if (p) (void)hipFree(p);
+  release_frame(n);
 }
 
 int check_nrc(vrt_nrc n) {
@@ -33,6 +39,50 @@ int check_nrc(vrt_nrc n) {
 }
 
 }  // namespace
+
+namespace vrt {
+
+int nrc_frame_buffers(vrt_nrc n, size_t pixels, int t_cap) {
+  if (pixels <= n->pixels && t_cap <= n->t_cap && n->ctr) return VRT_OK;
+  VRT_HIP(hipStreamSynchronize(n->ctx->stream));
+  const size_t P = std::max(pixels, n->pixels);
+  const int T = std::max(t_cap, n->t_cap);
+  release_frame(n);
+  hipError_t e = hipSuccess;
+  auto A = [&](void **p, size_t bytes) {
+    if (e == hipSuccess) e = hipMalloc(p, std::max<size_t>(bytes, 16));
+  };
+  A((void **)&n->sample, sizeof(float4) * P);
+  A((void **)&n->query_of_pixel, sizeof(int) * P);
+  A((void **)&n->q_raw, sizeof(float) * 6 * P);
+  A((void **)&n->q_T, sizeof(float) * 3 * P);
+  A((void **)&n->q_out, sizeof(float) * 3 * P);
+  A((void **)&n->t_raw, sizeof(float) * 6 * size_t(T));
+  A((void **)&n->t_target, sizeof(float) * 3 * size_t(T));
+  A((void **)&n->ctr, sizeof(uint32_t) * 4);
+  if (e == hipSuccess) e = hipMemsetAsync(n->ctr, 0, sizeof(uint32_t) * 4, n->ctx->stream);
+  if (e != hipSuccess) {
+    release_frame(n);
+    return e == hipErrorOutOfMemory ? fail(VRT_ERROR_OUT_OF_MEMORY, "cache frame buffers")
+                                    : hip_fail(e, "cache frame buffers");
+  }
+  n->pixels = P;
+  n->t_cap = T;
+  return VRT_OK;
+}
+
+int nrc_train_frame(vrt_nrc n, int steps) {
+  for (int b = 0; b < steps; ++b) {
+    VRT_HIP(nrc_launch_gradients(n->d, n->t_cap, n->t_raw, n->t_target, n->max_workgroups,
+                                 n->ctx->stream, &n->ctr[1], b, steps));
+    n->step++;
+    VRT_HIP(nrc_launch_adam(n->d, n->t_cap, n->cfg.learning_rate, n->cfg.beta1, n->cfg.beta2,
+                            n->cfg.epsilon, n->step, n->ctx->stream, &n->ctr[1], b, steps));
+  }
+  return VRT_OK;
+}
+
+}  // namespace vrt
 
 extern "C" {
 
@@ -139,6 +189,19 @@ int vrt_nrc_train_step(vrt_nrc n, int count, const float *raw, const float *targ
     VRT_HIP(hipMemcpyAsync(loss, n->d.stats, sizeof(float), hipMemcpyDeviceToHost, n->ctx->stream));
     VRT_HIP(hipStreamSynchronize(n->ctx->stream));
   }
+  return VRT_OK;
+}
+
+int vrt_nrc_frame_counts(vrt_nrc n, uint32_t counts[4]) {
+  if (int r = check_nrc(n)) return r;
+  if (!counts) return fail(VRT_ERROR_INVALID_ARGUMENT, "null output");
+  if (!n->ctr) {
+    counts[0] = counts[1] = counts[2] = counts[3] = 0;
+    return VRT_OK;
+  }
+  VRT_HIP(hipMemcpyAsync(counts, n->ctr, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                         n->ctx->stream));
+  VRT_HIP(hipStreamSynchronize(n->ctx->stream));
   return VRT_OK;
 }
 

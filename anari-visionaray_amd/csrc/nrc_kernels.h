@@ -44,14 +44,20 @@ struct NrcDevice {
 };
 
 hipError_t nrc_launch_init(const NrcDevice &d, uint64_t seed, hipStream_t s);
+// n_dev (optional): a sample count written by an earlier kernel; the
+// kernels then use min(n, *n_dev) samples, so no host round trip is needed
 hipError_t nrc_launch_infer(const NrcDevice &d, int n, const float *raw, float *out,
-                            int num_cus, hipStream_t s);
-// gradient sums of the relative-L2 loss over n samples into d.grad (and the
-// loss sum into d.stats[0]); does not update the parameters
+                            int num_cus, hipStream_t s, const uint32_t *n_dev = nullptr);
+// gradient sums of the relative-L2 loss over mini-batch `batch` of
+// `nbatches` into d.grad (and the loss sum into d.stats[0]); does not update
+// the parameters
 hipError_t nrc_launch_gradients(const NrcDevice &d, int n, const float *raw,
-                                const float *target, int max_workgroups, hipStream_t s);
+                                const float *target, int max_workgroups, hipStream_t s,
+                                const uint32_t *n_dev = nullptr, int batch = 0,
+                                int nbatches = 1);
 hipError_t nrc_launch_adam(const NrcDevice &d, int n, float lr, float beta1, float beta2,
-                           float eps, int step, hipStream_t s);
+                           float eps, int step, hipStream_t s, const uint32_t *n_dev = nullptr,
+                           int batch = 0, int nbatches = 1);
 hipError_t nrc_launch_refresh(const NrcDevice &d, hipStream_t s);
 
 // MFMA lane-layout probe: C = A (16x16) * B (16x16) for one wave.

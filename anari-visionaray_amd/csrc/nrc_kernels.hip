@@ -187,9 +187,23 @@ __host__ __device__ constexpr int lds_woff(int l) {
 }
 constexpr int kLdsWeights = 64 * (128 + 4) + 4 * 64 * (64 + 4) + 16 * (64 + 4);
 
+// Sample range [lo, lo + cnt) of mini-batch `batch` of `nbatches` over the
+// first min(n, *n_dev) samples (n_dev: a count produced on the GPU, or null).
+__device__ inline void batch_range(int n, const uint32_t *n_dev, int batch, int nbatches, int &lo,
+                                   int &cnt) {
+  const int total = n_dev ? min(n, int(*n_dev)) : n;
+  lo = int((long long)total * batch / nbatches);
+  cnt = int((long long)total * (batch + 1) / nbatches) - lo;
+}
+
 template <int BF16>
-__global__ void __launch_bounds__(256) k_nrc_infer(NrcDevice d, int n, const float *__restrict__ raw,
+__global__ void __launch_bounds__(256) k_nrc_infer(NrcDevice d, int n, const uint32_t *n_dev,
+                                                   const float *__restrict__ raw,
                                                    float *__restrict__ out) {
+  {
+    int lo;
+    batch_range(n, n_dev, 0, 1, lo, n);
+  }
   __shared__ uint16_t sw[kLdsWeights];
   __shared__ float sb[kNrcBiases];
   for (int l = 0; l < kNrcLayers; ++l) {
@@ -244,8 +258,16 @@ __global__ void __launch_bounds__(256) k_nrc_infer(NrcDevice d, int n, const flo
 constexpr int kTS = 64 + 4;  // LDS row stride of [feature][sample] tiles (halves)
 
 template <int BF16>
-__global__ void __launch_bounds__(256) k_nrc_grad(NrcDevice d, int n, const float *__restrict__ raw,
+__global__ void __launch_bounds__(256) k_nrc_grad(NrcDevice d, int n, const uint32_t *n_dev,
+                                                  int batch, int nbatches,
+                                                  const float *__restrict__ raw,
                                                   const float *__restrict__ target) {
+  {
+    int lo;
+    batch_range(n, n_dev, batch, nbatches, lo, n);
+    raw += 6 * size_t(lo);
+    target += 3 * size_t(lo);
+  }
   __shared__ uint16_t sx0[128 * kTS];      // layer-0 input X^T [feature][sample]
   __shared__ uint16_t sx[5][64 * kTS];     // inputs of layers 1..5
   __shared__ uint16_t sdz[64 * kTS];       // dZ^T of the current layer [out][sample]
@@ -416,11 +438,14 @@ __device__ inline void emit16(const NrcDevice &d, int p, float v) {
 }
 
 // Adam on the fp32 master parameters (Kingma & Ba 2015, bias-corrected)
-__global__ void k_nrc_adam(NrcDevice d, float inv_n, float lr, float b1, float b2, float eps,
-                           float c1, float c2) {
+__global__ void k_nrc_adam(NrcDevice d, int n, const uint32_t *n_dev, int batch, int nbatches,
+                           float lr, float b1, float b2, float eps, float c1, float c2) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= kNrcParams) return;
-  const float gr = d.grad[p] * inv_n;
+  int lo, cnt;
+  batch_range(n, n_dev, batch, nbatches, lo, cnt);
+  if (cnt <= 0) return;  // empty mini-batch: no step
+  const float gr = d.grad[p] * (1.f / float(cnt));
   const float m = b1 * d.adam_m[p] + (1.f - b1) * gr;
   const float v = b2 * d.adam_v[p] + (1.f - b2) * gr * gr;
   d.adam_m[p] = m;
@@ -489,44 +514,45 @@ hipError_t nrc_launch_refresh(const NrcDevice &d, hipStream_t s) {
 }
 
 hipError_t nrc_launch_infer(const NrcDevice &d, int n, const float *raw, float *out, int num_cus,
-                            hipStream_t s) {
+                            hipStream_t s, const uint32_t *n_dev) {
   if (n <= 0) return hipSuccess;
   const int tiles = (n + 63) / 64;
   const int blocks = std::max(1, std::min((tiles + 3) / 4, 2 * std::max(1, num_cus)));
   if (d.bf16)
-    hipLaunchKernelGGL(k_nrc_infer<1>, dim3(blocks), dim3(256), 0, s, d, n, raw, out);
+    hipLaunchKernelGGL(k_nrc_infer<1>, dim3(blocks), dim3(256), 0, s, d, n, n_dev, raw, out);
   else
-    hipLaunchKernelGGL(k_nrc_infer<0>, dim3(blocks), dim3(256), 0, s, d, n, raw, out);
+    hipLaunchKernelGGL(k_nrc_infer<0>, dim3(blocks), dim3(256), 0, s, d, n, n_dev, raw, out);
   return hipGetLastError();
 }
 
 hipError_t nrc_launch_gradients(const NrcDevice &d, int n, const float *raw, const float *target,
-                                int max_workgroups, hipStream_t s) {
+                                int max_workgroups, hipStream_t s, const uint32_t *n_dev,
+                                int batch, int nbatches) {
   hipError_t e = hipMemsetAsync(d.stats, 0, 2 * sizeof(float), s);
   if (e != hipSuccess) return e;
-  const int chunks = (n + 63) / 64;
+  // workgroup count from the largest possible mini-batch; every workgroup
+  // writes its partial (zeros when it got no chunk)
+  const int chunks = (n / std::max(1, nbatches) + 1 + 63) / 64;
   const int parts = std::max(1, std::min(chunks, max_workgroups));
-  if (n > 0) {
-    if (d.bf16)
-      hipLaunchKernelGGL(k_nrc_grad<1>, dim3(parts), dim3(256), 0, s, d, n, raw, target);
-    else
-      hipLaunchKernelGGL(k_nrc_grad<0>, dim3(parts), dim3(256), 0, s, d, n, raw, target);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  } else {
-    e = hipMemsetAsync(d.partials, 0, sizeof(float) * kNrcParams, s);
-    if (e != hipSuccess) return e;
-  }
+  if (d.bf16)
+    hipLaunchKernelGGL(k_nrc_grad<1>, dim3(parts), dim3(256), 0, s, d, n, n_dev, batch, nbatches,
+                       raw, target);
+  else
+    hipLaunchKernelGGL(k_nrc_grad<0>, dim3(parts), dim3(256), 0, s, d, n, n_dev, batch, nbatches,
+                       raw, target);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_nrc_reduce, dim3((kNrcParams + 255) / 256), dim3(256), 0, s, d, parts);
   return hipGetLastError();
 }
 
 hipError_t nrc_launch_adam(const NrcDevice &d, int n, float lr, float b1, float b2, float eps,
-                           int step, hipStream_t s) {
+                           int step, hipStream_t s, const uint32_t *n_dev, int batch,
+                           int nbatches) {
   const float c1 = 1.f / (1.f - powf(b1, float(step)));
   const float c2 = 1.f / (1.f - powf(b2, float(step)));
-  hipLaunchKernelGGL(k_nrc_adam, dim3((kNrcParams + 255) / 256), dim3(256), 0, s, d,
-                     n > 0 ? 1.f / float(n) : 0.f, lr, b1, b2, eps, c1, c2);
+  hipLaunchKernelGGL(k_nrc_adam, dim3((kNrcParams + 255) / 256), dim3(256), 0, s, d, n, n_dev,
+                     batch, nbatches, lr, b1, b2, eps, c1, c2);
   return hipGetLastError();
 }
 

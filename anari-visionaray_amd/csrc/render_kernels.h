@@ -10,10 +10,15 @@ hipError_t launch_raycast(const KScene &sc, const KCamera &cam, const KParams &P
 // persistent path tracer: `blocks` workgroups (a multiple of 8) drain the
 // per-XCD pixel queues; `queues` must be zero, `queues_next` is zeroed for
 // the following launch
+// the following launch.  With `nrc`, paths end in radiance-cache queries at
+// their second volume vertex (see KNrc) and the frame is written by
+// launch_nrc_resolve once the queries are answered.
 hipError_t launch_pathtrace(const KScene &sc, const KCamera &cam, const KParams &P,
                             const KFrame &F, int accumID, uint32_t *queues,
-                            uint32_t *queues_next, int blocks, hipStream_t s);
-int pathtrace_blocks_per_cu(size_t lds_bytes);
+                            uint32_t *queues_next, int blocks, hipStream_t s,
+                            const KNrc *nrc = nullptr);
+hipError_t launch_nrc_resolve(const KFrame &F, const KNrc &N, int accumID, hipStream_t s);
+int pathtrace_blocks_per_cu(size_t lds_bytes, bool nrc = false);
 hipError_t launch_brickify(const float *lin, const int dims[3], float *bricks, hipStream_t s);
 hipError_t launch_value_ranges(const KVolume &v, float *ranges, hipStream_t s);
 hipError_t launch_majorants(const KVolume &v, const float *ranges, float *maj, hipStream_t s);

@@ -223,13 +223,15 @@ __device__ inline KRay primary_ray(const KCamera &c, float x, float y, float W, 
 struct VolHit {
   float t;
   f3 albedo;
+  float extinction;  // classified opacity at the collision (HitRecordVolume::extinction)
   bool hit;
 };
 
-// One cell's collision loop.  Returns true on a real collision.
+// One cell's collision loop.  Returns true on a real collision; `smp` is the
+// classified sample (albedo rgb, opacity) there.
 __device__ inline bool woodcock_cell(const KVolume &v, const float4 *lds, float majorant,
                                      float invUD, f3 ori, f3 dir, float t0, float t1,
-                                     uint32_t &rs, float &t_hit, f3 &albedo, uint32_t &lookups) {
+                                     uint32_t &rs, float &t_hit, float4 &smp, uint32_t &lookups) {
   if (majorant <= 0.f) return false;
   if (!(t0 < t1)) {
     // the first free-flight step already leaves the cell (-log(1-u) >= 0):
@@ -250,7 +252,7 @@ __device__ inline bool woodcock_cell(const KVolume &v, const float4 *lds, float 
     float u = rnd(rs);
     if (s.w >= u * majorant) {
       t_hit = t;
-      albedo = mk(s.x, s.y, s.z);
+      smp = s;
       return true;
     }
   }
@@ -274,11 +276,11 @@ __device__ inline VolHit delta_track(const KVolume &v, const float4 *lds, KRay r
   const float invUD = 1.f / (v.unit_distance * dt_scale);
 
   float t_hit = 0.f;
-  f3 albedo = mk(0.f);
+  float4 smp = make_float4(0.f, 0.f, 0.f, 0.f);
   bool hit = false;
 
   if (v.gdims[0] == 0) {
-    hit = woodcock_cell(v, lds, 1.f, invUD, ori, dir, tmin, tmax, rs, t_hit, albedo, lookups);
+    hit = woodcock_cell(v, lds, 1.f, invUD, ori, dir, tmin, tmax, rs, t_hit, smp, lookups);
   } else {
     // dda3 with the ray moved so that tmin becomes 0 (DDA.h:21-25)
     const f3 o = ori + dir * tmin;
@@ -317,7 +319,7 @@ __device__ inline VolHit delta_track(const KVolume &v, const float4 *lds, KRay r
       const float t1 = fminf(tc, tm);
       const int li = (cell[2] * v.gdims[1] + cell[1]) * v.gdims[0] + cell[0];
       const float maj = v.majorants[li];
-      if (woodcock_cell(v, lds, maj, invUD, ori, dir, tmin + t0, tmin + t1, rs, t_hit, albedo,
+      if (woodcock_cell(v, lds, maj, invUD, ori, dir, tmin + t0, tmin + t1, rs, t_hit, smp,
                         lookups)) {
         hit = true;
         break;
@@ -345,7 +347,8 @@ __device__ inline VolHit delta_track(const KVolume &v, const float4 *lds, KRay r
   if (hit) {
     out.hit = true;
     out.t = t_hit / dt_scale;
-    out.albedo = albedo;
+    out.albedo = mk(smp.x, smp.y, smp.z);
+    out.extinction = smp.w;
   }
   return out;
 }
@@ -539,7 +542,39 @@ struct PathState {
   int aoSamples;
   float aoWeights, aoCount;
   bool firstHit;
+  // radiance cache: query / training slot, training-path flag, radiance and
+  // throughput at the second vertex
+  int qslot, tslot;
+  bool train_sel;
+  f3 I2, T2;
 };
+
+// Wave-aggregated slot claim: every active lane calls it, lanes with `want`
+// receive consecutive indices from one atomic per wave.
+__device__ inline uint32_t wave_claim(uint32_t *ctr, bool want) {
+  const unsigned long long m = __ballot(want);
+  if (m == 0ull) return 0xFFFFFFFFu;
+  const uint32_t lane = __lane_id();
+  const uint32_t rank = uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if (int(lane) == leader) base = atomicAdd(ctr, uint32_t(__popcll(m)));
+  base = __shfl(base, leader, 64);
+  return want ? base + rank : 0xFFFFFFFFu;
+}
+
+// Raw cache inputs of a scattering vertex: position in the cache box,
+// direction as (theta/pi, phi/2pi), classified opacity.
+__device__ inline void cache_inputs(const KNrc &N, f3 pos, f3 dir, float density, float *raw) {
+  const f3 p = (pos - N.pos_lo) * N.pos_scale;
+  raw[0] = fminf(fmaxf(p.x, 0.f), 1.f);
+  raw[1] = fminf(fmaxf(p.y, 0.f), 1.f);
+  raw[2] = fminf(fmaxf(p.z, 0.f), 1.f);
+  const f3 d = normalize(dir);
+  raw[3] = acosf(fminf(fmaxf(d.z, -1.f), 1.f)) * 0.318309886f;
+  raw[4] = (atan2f(d.y, d.x) + 3.14159265f) * 0.159154943f;
+  raw[5] = density;
+}
 
 constexpr int kChunkLog2 = 5;  // 32x32-pixel chunks
 constexpr int kQueues = 8;
@@ -578,12 +613,17 @@ __device__ inline void begin_sample(PathState &st, const KCamera &cam, const KPa
   st.aoSamples = 0;
   st.aoWeights = st.aoCount = 0.f;
   st.firstHit = false;
+  st.qslot = -1;
+  st.tslot = -1;
 }
 
 // One pass of the bounce loop (Pathtrace_impl.cpp:558-600): intersect, shade,
 // advance the state machine.  Returns true when the sample's path ended.
+// With NRC, a path reaching its second volume vertex either becomes a cache
+// query and ends, or -- a training path -- records the vertex and goes on.
+template <bool NRC>
 __device__ inline bool path_pass(PathState &st, const KScene &sc, const KParams &P,
-                                 const float4 *s_tf, uint32_t &lookups) {
+                                 const float4 *s_tf, uint32_t &lookups, const KNrc &N) {
   // intersectAll: volumes only in this scene class
   VolHit vh;
   vh.hit = false;
@@ -596,6 +636,34 @@ __device__ inline bool path_pass(PathState &st, const KScene &sc, const KParams 
   if (st.passID == 0 && st.bounceID == 0) {
     st.firstHit = hit;
     if (hit) st.sdepth = vh.t;
+  }
+  if constexpr (NRC) {
+    const bool v2 = st.rayType == Radiance && hit && st.bounceID == 1;
+    (void)wave_claim(&N.ctr[2], v2);  // vertex-2 paths of this frame
+    const bool want_train = v2 && st.train_sel;
+    const uint32_t tslot = wave_claim(&N.ctr[1], want_train);
+    const bool is_train = want_train && tslot < uint32_t(N.t_cap);
+    const bool want_query = v2 && !is_train;
+    const uint32_t qslot = wave_claim(&N.ctr[0], want_query);
+    if (v2) {
+      float raw[6];
+      cache_inputs(N, st.ray.ori + st.ray.dir * vh.t, st.ray.dir, vh.extinction, raw);
+      if (is_train) {
+        for (int i = 0; i < 6; ++i) N.t_raw[6 * size_t(tslot) + i] = raw[i];
+        st.tslot = int(tslot);
+        st.I2 = st.intensity;
+        st.T2 = st.throughput;
+      } else {
+        if (qslot < uint32_t(N.q_cap)) {
+          for (int i = 0; i < 6; ++i) N.q_raw[6 * size_t(qslot) + i] = raw[i];
+          N.q_T[3 * size_t(qslot)] = st.throughput.x;
+          N.q_T[3 * size_t(qslot) + 1] = st.throughput.y;
+          N.q_T[3 * size_t(qslot) + 2] = st.throughput.z;
+          st.qslot = int(qslot);
+        }
+        return true;  // the cache supplies the radiance from here on
+      }
+    }
   }
   // shade (Pathtrace_impl.cpp:93-487)
   int next = Miss;
@@ -708,14 +776,31 @@ __device__ inline bool path_pass(PathState &st, const KScene &sc, const KParams 
   return false;
 }
 
+// training-path selection: an independent hash of (pixel, accumID), so that
+// the path's own random streams are the ones of the plain path tracer
+__device__ inline float train_hash(uint32_t pixel, uint32_t accumID) {
+  return float(tea4(pixel ^ 0x5bd1e995u, accumID + 0x68e31da4u) & 0x00FFFFFFu) *
+         (1.0f / 16777216.0f);
+}
+
+template <bool NRC>
 __global__ void __launch_bounds__(256)
     k_pathtrace(KScene sc, KCamera cam, KParams P, KFrame F, int accumID, uint32_t *queues,
-                uint32_t *queues_next) {
+                uint32_t *queues_next, KNrc N) {
   extern __shared__ float4 s_tf[];
   stage_tf(sc, s_tf);
   // clear the counters the next launch will use (the current ones were
   // cleared by the previous launch)
   if (blockIdx.x == 0 && threadIdx.x < kQueues) queues_next[threadIdx.x] = 0u;
+  // fraction of paths that become training paths: the per-frame training
+  // budget over the previous frame's count of paths reaching vertex 2
+  float p_train = 0.f;
+  if constexpr (NRC) {
+    if (N.train) {
+      const uint32_t prev = N.ctr[3];
+      p_train = prev == 0u ? 1.f : fminf(1.f, float(N.t_cap) / float(prev));
+    }
+  }
 
   const int W = F.width, H = F.height;
   const int nchx = (W + (1 << kChunkLog2) - 1) >> kChunkLog2;
@@ -762,6 +847,7 @@ __global__ void __launch_bounds__(256)
           st.sampleID = 0;
           st.acc = make_float4(0.f, 0.f, 0.f, 0.f);
           st.depth = 1e31f;
+          st.train_sel = NRC && train_hash(uint32_t(st.pixel), uint32_t(accumID)) < p_train;
           begin_sample(st, cam, P, px, py, W, H);
         }
       }
@@ -779,7 +865,7 @@ __global__ void __launch_bounds__(256)
 
     // ---- one pass for every active lane ---------------------------------------
     if (st.pixel >= 0) {
-      bool done = sc.num_volumes == 0 || path_pass(st, sc, P, s_tf, lookups);
+      bool done = sc.num_volumes == 0 || path_pass<NRC>(st, sc, P, s_tf, lookups, N);
       if (done) {
         float4 color = P.bg;
         if (st.firstHit)
@@ -790,7 +876,22 @@ __global__ void __launch_bounds__(256)
         st.acc.w += color.w;
         if (st.sdepth < st.depth) st.depth = st.sdepth;
         const int x = st.pixel % W, y = st.pixel / W;
-        if (++st.sampleID < P.pixel_samples) {
+        if constexpr (NRC) {
+          if (st.tslot >= 0) {
+            // training target: radiance the path gathered from vertex 2 on,
+            // divided by the throughput it arrived there with
+            const f3 s = st.intensity - st.I2;
+            float *t = N.t_target + 3 * size_t(st.tslot);
+            t[0] = st.T2.x > 0.f ? s.x / st.T2.x : 0.f;
+            t[1] = st.T2.y > 0.f ? s.y / st.T2.y : 0.f;
+            t[2] = st.T2.z > 0.f ? s.z / st.T2.z : 0.f;
+          }
+          // one sample per pixel: the cache term is added by the resolve pass
+          N.sample[st.pixel] = st.acc;
+          N.query_of_pixel[st.pixel] = st.qslot;
+          if (F.depth && st.depth <= F.depth[st.pixel]) F.depth[st.pixel] = st.depth;
+          st.pixel = -1;
+        } else if (++st.sampleID < P.pixel_samples) {
           begin_sample(st, cam, P, x, y, W, H);
         } else {
           const float inv = 1.f / float(P.pixel_samples);
@@ -803,6 +904,33 @@ __global__ void __launch_bounds__(256)
     }
   }
   if (F.counters) wave_add_counter(&F.counters[0], lookups);
+}
+
+// Radiance-cache frame: start (counters) and resolve (cache term + frame
+// accumulation, Frame::writeSample)
+__global__ void k_nrc_frame_begin(KNrc N) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    N.ctr[3] = N.ctr[2];
+    N.ctr[0] = 0u;
+    N.ctr[1] = 0u;
+    N.ctr[2] = 0u;
+  }
+}
+
+__global__ void k_nrc_resolve(KFrame F, KNrc N, int accumID) {
+  const int n = F.width * F.height;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float4 c = N.sample[i];
+    const int q = N.query_of_pixel[i];
+    if (q >= 0) {
+      const float *o = N.q_out + 3 * size_t(q);
+      const float *T = N.q_T + 3 * size_t(q);
+      c.x += T[0] * fmaxf(o[0], 0.f);
+      c.y += T[1] * fmaxf(o[1], 0.f);
+      c.z += T[2] * fmaxf(o[2], 0.f);
+    }
+    write_sample(F, i % F.width, i / F.width, accumID, c, 1e31f, true);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -953,17 +1081,33 @@ hipError_t launch_raycast(const KScene &sc, const KCamera &cam, const KParams &P
 
 hipError_t launch_pathtrace(const KScene &sc, const KCamera &cam, const KParams &P,
                             const KFrame &F, int accumID, uint32_t *queues,
-                            uint32_t *queues_next, int blocks, hipStream_t s) {
+                            uint32_t *queues_next, int blocks, hipStream_t s, const KNrc *nrc) {
   const size_t lds = size_t(sc.tf_lds_total) * sizeof(float4);
-  hipLaunchKernelGGL(k_pathtrace, dim3(blocks), dim3(256), lds, s, sc, cam, P, F, accumID, queues,
-                     queues_next);
+  if (nrc) {
+    hipLaunchKernelGGL(k_nrc_frame_begin, dim3(1), dim3(64), 0, s, *nrc);
+    hipLaunchKernelGGL(k_pathtrace<true>, dim3(blocks), dim3(256), lds, s, sc, cam, P, F, accumID,
+                       queues, queues_next, *nrc);
+  } else {
+    hipLaunchKernelGGL(k_pathtrace<false>, dim3(blocks), dim3(256), lds, s, sc, cam, P, F,
+                       accumID, queues, queues_next, KNrc{});
+  }
   return hipGetLastError();
 }
 
-int pathtrace_blocks_per_cu(size_t lds_bytes) {
+hipError_t launch_nrc_resolve(const KFrame &F, const KNrc &N, int accumID, hipStream_t s) {
+  const int n = F.width * F.height;
+  hipLaunchKernelGGL(k_nrc_resolve, dim3(std::min((n + 255) / 256, 65536)), dim3(256), 0, s, F,
+                     N, accumID);
+  return hipGetLastError();
+}
+
+int pathtrace_blocks_per_cu(size_t lds_bytes, bool nrc) {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pathtrace, 256, lds_bytes) != hipSuccess)
-    return 1;
+  hipError_t e = nrc ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pathtrace<true>, 256,
+                                                                    lds_bytes)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pathtrace<false>, 256,
+                                                                    lds_bytes);
+  if (e != hipSuccess) return 1;
   return n > 0 ? n : 1;
 }
 

@@ -10,6 +10,7 @@
 #include <string>
 
 #include "abi_internal.h"
+#include "nrc_internal.h"
 #include "render_kernels.h"
 #include "vrt.h"
 
@@ -431,6 +432,74 @@ int vrt_pathtrace(vrt_context ctx, const vrt_scene *scene, const vrt_camera *cam
   uint32_t *nxt = ctx->queues + 8 * (1 - ctx->queue_parity);
   VRT_HIP(launch_pathtrace(sc, cam, P, F, accum_id, cur, nxt, blocks, ctx->stream));
   ctx->queue_parity ^= 1;
+  return VRT_OK;
+}
+
+int vrt_pathtrace_nrc(vrt_context ctx, const vrt_scene *scene, const vrt_camera *camera,
+                      const vrt_params *params, const vrt_frame *frame, vrt_nrc nrc,
+                      const vrt_nrc_render *opts, int accum_id, int frame_id) {
+  (void)frame_id;
+  if (int r = check_ctx(ctx)) return r;
+  if (!nrc || !opts) return fail(VRT_ERROR_INVALID_ARGUMENT, "null cache or options");
+  if (nrc->ctx != ctx) return fail(VRT_ERROR_INVALID_ARGUMENT, "cache belongs to another context");
+  if (accum_id < 0) return fail(VRT_ERROR_INVALID_ARGUMENT, "accum_id must be >= 0");
+  if (opts->train_samples < 0 || opts->train_samples > nrc->cfg.max_batch * std::max(1, opts->train_steps))
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "train_samples %d exceeds max_batch x train_steps",
+                opts->train_samples);
+  if (opts->train && opts->train_steps < 1)
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "train_steps must be >= 1 when training");
+  KScene sc;
+  KCamera cam;
+  KParams P;
+  KFrame F;
+  if (int r = make_scene(scene, sc)) return r;
+  if (int r = make_camera(camera, cam)) return r;
+  if (int r = make_params(params, P)) return r;
+  if (int r = make_frame(frame, F)) return r;
+  if (!F.accum) return fail(VRT_ERROR_INVALID_ARGUMENT, "pathtrace needs an accumulation buffer");
+  if (P.pixel_samples != 1)
+    return fail(VRT_ERROR_UNSUPPORTED, "the radiance-cache path tracer takes one sample per pixel");
+  const size_t pixels = size_t(F.width) * F.height;
+  if (int r = nrc_frame_buffers(nrc, pixels, std::max(1, opts->train_samples))) return r;
+  KNrc N{};
+  N.sample = nrc->sample;
+  N.query_of_pixel = nrc->query_of_pixel;
+  N.q_raw = nrc->q_raw;
+  N.q_T = nrc->q_T;
+  N.q_out = nrc->q_out;
+  N.t_raw = nrc->t_raw;
+  N.t_target = nrc->t_target;
+  N.ctr = nrc->ctr;
+  N.q_cap = int(pixels);
+  N.t_cap = opts->train ? std::max(0, opts->train_samples) : 0;
+  N.train = opts->train && opts->train_samples > 0;
+  N.pos_lo = mk(opts->bounds_lo[0], opts->bounds_lo[1], opts->bounds_lo[2]);
+  for (int a = 0; a < 3; ++a)
+    if (!(opts->bounds_hi[a] > opts->bounds_lo[a]))
+      return fail(VRT_ERROR_INVALID_ARGUMENT, "cache bounds must have positive extent");
+  N.pos_scale = mk(1.f / (opts->bounds_hi[0] - opts->bounds_lo[0]),
+                   1.f / (opts->bounds_hi[1] - opts->bounds_lo[1]),
+                   1.f / (opts->bounds_hi[2] - opts->bounds_lo[2]));
+
+  const size_t lds = size_t(sc.tf_lds_total) * 16;
+  int blocks = std::max(1, ctx->num_cus) * pathtrace_blocks_per_cu(lds, true);
+  blocks = std::max(8, (blocks + 7) / 8 * 8);
+  uint32_t *cur = ctx->queues + 8 * ctx->queue_parity;
+  uint32_t *nxt = ctx->queues + 8 * (1 - ctx->queue_parity);
+  VRT_HIP(launch_pathtrace(sc, cam, P, F, accum_id, cur, nxt, blocks, ctx->stream, &N));
+  ctx->queue_parity ^= 1;
+  // answer the queries (count taken on the GPU), add the cache term, write
+  // the frame, then train on this frame's samples
+  VRT_HIP(nrc_launch_infer(nrc->d, N.q_cap, N.q_raw, N.q_out, ctx->num_cus, ctx->stream,
+                           &N.ctr[0]));
+  VRT_HIP(launch_nrc_resolve(F, N, accum_id, ctx->stream));
+  if (N.train) {
+    const int saved = nrc->t_cap;
+    nrc->t_cap = N.t_cap;
+    int r = nrc_train_frame(nrc, opts->train_steps);
+    nrc->t_cap = saved;
+    if (r) return r;
+  }
   return VRT_OK;
 }
 

@@ -93,6 +93,13 @@ class NrcConfig(C.Structure):
     ]
 
 
+class NrcRender(C.Structure):
+    _fields_ = [
+        ("train", C.c_int), ("train_samples", C.c_int), ("train_steps", C.c_int),
+        ("bounds_lo", f3), ("bounds_hi", f3),
+    ]
+
+
 NRC_FP16 = 0
 NRC_BF16 = 1
 
@@ -136,6 +143,10 @@ def lib():
         L.vrt_nrc_gradients.argtypes = [vp, ip, vp, vp, vp, C.POINTER(C.c_float)]
         L.vrt_nrc_train_step.argtypes = [vp, ip, vp, vp, C.POINTER(C.c_float)]
         L.vrt_mfma_probe.argtypes = [vp, vp, vp, vp, ip]
+        L.vrt_pathtrace_nrc.argtypes = [vp, C.POINTER(Scene), C.POINTER(Camera),
+                                        C.POINTER(Params), C.POINTER(Frame), vp,
+                                        C.POINTER(NrcRender), ip, ip]
+        L.vrt_nrc_frame_counts.argtypes = [vp, C.POINTER(C.c_uint32)]
         L.vrt_sample_field.argtypes = [vp, C.POINTER(Volume), ip, vp, vp]
         L.vrt_delta_track.argtypes = [vp, C.POINTER(Scene), ip, vp, vp, vp]
         L.vrt_raycast.argtypes = [vp, C.POINTER(Scene), C.POINTER(Camera), C.POINTER(Params),
@@ -501,10 +512,32 @@ class Nrc:
                                        C.byref(loss)))
         return loss.value
 
+    def frame_counts(self):
+        """[queries, training claims, vertex-2 paths, previous vertex-2 paths]"""
+        out = (C.c_uint32 * 4)()
+        check(lib().vrt_nrc_frame_counts(self.handle, out))
+        return list(out)
+
     def close(self):
         if self.handle:
             check(lib().vrt_nrc_destroy(self.handle))
             self.handle = None
+
+
+def nrc_render_options(bounds_lo, bounds_hi, train=True, train_samples=65536, train_steps=4):
+    o = NrcRender()
+    o.train = int(train)
+    o.train_samples = int(train_samples)
+    o.train_steps = int(train_steps)
+    o.bounds_lo = f3(*bounds_lo)
+    o.bounds_hi = f3(*bounds_hi)
+    return o
+
+
+def pathtrace_nrc(ctx, scene, cam, prm, frame, nrc, opts, accum_id, frame_id=0):
+    check(lib().vrt_pathtrace_nrc(ctx.handle, C.byref(scene), C.byref(cam), C.byref(prm),
+                                  C.byref(frame.desc), nrc.handle, C.byref(opts), accum_id,
+                                  frame_id))
 
 
 def mfma_probe(ctx, a, b, precision=NRC_FP16):

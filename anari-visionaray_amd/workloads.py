@@ -134,3 +134,14 @@ class GpuScene:
             _vrt.raycast(self.ctx, self.scene, self.camera, self.params, self.frame, accum_id)
         else:
             _vrt.pathtrace(self.ctx, self.scene, self.camera, self.params, self.frame, accum_id)
+
+    def nrc_options(self, train=True, train_samples=65536, train_steps=4):
+        """Cache options whose [0,1]^3 box is the volume's bounds."""
+        wl = self.wl
+        lo = np.array(wl.origin, np.float32)
+        hi = lo + (np.array(wl.dims, np.float32) - 1) * np.array(wl.spacing, np.float32)
+        return _vrt.nrc_render_options(lo, hi, train, train_samples, train_steps)
+
+    def render_nrc(self, nrc, opts, accum_id=0):
+        _vrt.pathtrace_nrc(self.ctx, self.scene, self.camera, self.params, self.frame, nrc, opts,
+                           accum_id)

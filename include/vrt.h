@@ -274,6 +274,30 @@ VRT_API int vrt_nrc_gradients(vrt_nrc nrc, int n, const float *raw, const float 
 /* one Adam step on the mean gradient of n samples; loss sum to *loss */
 VRT_API int vrt_nrc_train_step(vrt_nrc nrc, int n, const float *raw, const float *target,
                                float *loss);
+/* 'pathtrace' frame with the radiance cache.  Paths end at their second
+ * volume vertex in a cache query (the cache supplies the radiance leaving
+ * that vertex towards the path); a fraction of paths, adapted on the GPU to
+ * give `train_samples` per frame, are traced to full length instead and
+ * leave one training sample (vertex-2 inputs, suffix radiance / throughput).
+ * After the frame is resolved the cache takes `train_steps` Adam steps on
+ * those samples.  Requires params->pixel_samples == 1.  The cache inputs
+ * map world positions in [bounds_lo, bounds_hi] onto [0,1]^3. */
+typedef struct vrt_nrc_render {
+  int train;          /* 1: record training paths and train after the frame */
+  int train_samples;  /* training samples per frame (65536) */
+  int train_steps;    /* Adam steps (mini-batches) per frame */
+  float bounds_lo[3];
+  float bounds_hi[3];
+} vrt_nrc_render;
+
+VRT_API int vrt_pathtrace_nrc(vrt_context ctx, const vrt_scene *scene,
+                              const vrt_camera *camera, const vrt_params *params,
+                              const vrt_frame *frame, vrt_nrc nrc,
+                              const vrt_nrc_render *opts, int accum_id, int frame_id);
+/* counts of the last cache frame: [0] queries, [1] training-slot claims,
+ * [2] paths reaching vertex 2, [3] the previous frame's [2] (synchronises) */
+VRT_API int vrt_nrc_frame_counts(vrt_nrc nrc, uint32_t counts[4]);
+
 /* matrix-core lane-layout check: c (16x16) = a (16x16) b (16x16), device
  * row-major fp32 in/out, operands rounded to fp16 or bf16 */
 VRT_API int vrt_mfma_probe(vrt_context ctx, const float *a, const float *b, float *c,
