@@ -85,6 +85,7 @@ struct KParams {
   int max_bounce;
   int pixel_samples;
   float sampling_rate_inv;
+  uint32_t rng_offset;
 };
 
 struct KFrame {

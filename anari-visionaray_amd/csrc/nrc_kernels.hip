@@ -414,13 +414,19 @@ __global__ void __launch_bounds__(256) k_nrc_grad(NrcDevice d, int n, const uint
   if (threadIdx.x == 0) atomicAdd(&d.stats[0], sloss[0] + sloss[1] + sloss[2] + sloss[3]);
 }
 
-// fixed-order sum of the workgroup partials
-__global__ void k_nrc_reduce(NrcDevice d, int parts) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= kNrcParams) return;
+// fixed-order sum of the workgroup partials: a workgroup owns 64 parameters;
+// its 4 waves sum interleaved quarters of the partials (64 consecutive floats
+// per load), then combine in LDS in a fixed order
+__global__ void __launch_bounds__(256) k_nrc_reduce(NrcDevice d, int parts) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int p = blockIdx.x * 64 + lane;
   float s = 0.f;
-  for (int i = 0; i < parts; ++i) s += d.partials[size_t(i) * kNrcParams + p];
-  d.grad[p] = s;
+  if (p < kNrcParams)
+    for (int i = wave; i < parts; i += 4) s += d.partials[size_t(i) * kNrcParams + p];
+  __shared__ float sh[4][64];
+  sh[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0 && p < kNrcParams) d.grad[p] = (sh[0][lane] + sh[1][lane]) + (sh[2][lane] + sh[3][lane]);
 }
 
 __device__ inline void emit16(const NrcDevice &d, int p, float v) {
@@ -542,7 +548,7 @@ hipError_t nrc_launch_gradients(const NrcDevice &d, int n, const float *raw, con
                        raw, target);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_nrc_reduce, dim3((kNrcParams + 255) / 256), dim3(256), 0, s, d, parts);
+  hipLaunchKernelGGL(k_nrc_reduce, dim3((kNrcParams + 63) / 64), dim3(256), 0, s, d, parts);
   return hipGetLastError();
 }
 

@@ -18,6 +18,9 @@ hipError_t launch_pathtrace(const KScene &sc, const KCamera &cam, const KParams 
                             uint32_t *queues_next, int blocks, hipStream_t s,
                             const KNrc *nrc = nullptr);
 hipError_t launch_nrc_resolve(const KFrame &F, const KNrc &N, int accumID, hipStream_t s);
+// adds sum (x - r)^2 / (r^2 + eps) over n pixels' RGB to *out
+hipError_t launch_relmse(const float4 *x, const double *ref_sum, double ref_scale, int n,
+                         float eps, double *out, hipStream_t s);
 int pathtrace_blocks_per_cu(size_t lds_bytes, bool nrc = false);
 hipError_t launch_brickify(const float *lin, const int dims[3], float *bricks, hipStream_t s);
 hipError_t launch_value_ranges(const KVolume &v, float *ranges, hipStream_t s);

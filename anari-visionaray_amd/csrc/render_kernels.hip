@@ -842,12 +842,13 @@ __global__ void __launch_bounds__(256)
         chunk_pixel(q, pool_base + rank, nchx, px, py);
         if (px < W && py < H) {
           st.pixel = py * W + px;
-          st.rng = tea4(uint32_t(st.pixel), uint32_t(accumID));
+          st.rng = tea4(uint32_t(st.pixel), uint32_t(accumID) + P.rng_offset);
           st.srng = st.rng;  // ScreenSample's copy (Pathtrace_impl.cpp:512-513)
           st.sampleID = 0;
           st.acc = make_float4(0.f, 0.f, 0.f, 0.f);
           st.depth = 1e31f;
-          st.train_sel = NRC && train_hash(uint32_t(st.pixel), uint32_t(accumID)) < p_train;
+          st.train_sel = NRC && train_hash(uint32_t(st.pixel), uint32_t(accumID) + P.rng_offset) <
+                                    p_train;
           begin_sample(st, cam, P, px, py, W, H);
         }
       }
@@ -908,6 +909,38 @@ __global__ void __launch_bounds__(256)
 
 // Radiance-cache frame: start (counters) and resolve (cache term + frame
 // accumulation, Frame::writeSample)
+// relMSE = mean over pixels and RGB of (x - r)^2 / (r^2 + eps), x the float4
+// running mean, r = ref_sum * ref_scale (FP64 sums); one double atomic per
+// workgroup
+__global__ void __launch_bounds__(256)
+    k_relmse(const float4 *__restrict__ x, const double *__restrict__ ref_sum, double ref_scale,
+             int n, float eps, double *out) {
+  double s = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float4 v = x[i];
+    const double *r = ref_sum + 4 * size_t(i);
+    const double r0 = r[0] * ref_scale, r1 = r[1] * ref_scale, r2 = r[2] * ref_scale;
+    s += (double(v.x) - r0) * (double(v.x) - r0) / (r0 * r0 + eps);
+    s += (double(v.y) - r1) * (double(v.y) - r1) / (r1 * r1 + eps);
+    s += (double(v.z) - r2) * (double(v.z) - r2) / (r2 * r2 + eps);
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (int(threadIdx.x) < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(out, sh[0]);
+}
+
+hipError_t launch_relmse(const float4 *x, const double *ref_sum, double ref_scale, int n,
+                         float eps, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_relmse, dim3(std::max(1, std::min((n + 255) / 256, 1024))), dim3(256), 0,
+                     s, x, ref_sum, ref_scale, n, eps, out);
+  return hipGetLastError();
+}
+
 __global__ void k_nrc_frame_begin(KNrc N) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     N.ctr[3] = N.ctr[2];

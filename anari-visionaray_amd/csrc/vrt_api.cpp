@@ -169,6 +169,7 @@ int make_params(const vrt_params *p, KParams &k) {
   k.pixel_samples = p->pixel_samples;
   // Raycast.cpp:27 safe_rcp
   k.sampling_rate_inv = p->volume_sampling_rate > 0.f ? 1.f / p->volume_sampling_rate : 0.f;
+  k.rng_offset = p->rng_offset;
   return VRT_OK;
 }
 
@@ -384,6 +385,26 @@ int vrt_delta_track(vrt_context ctx, const vrt_scene *scene, int n, const float 
   if (sc.num_volumes < 1) return fail(VRT_ERROR_INVALID_ARGUMENT, "scene has no volume");
   if (n == 0) return VRT_OK;
   VRT_HIP(launch_delta_track(sc, n, rays, rng, t_out, ctx->stream));
+  return VRT_OK;
+}
+
+int vrt_relmse(vrt_context ctx, const float *accum, const double *ref_sum, double ref_scale,
+               int num_pixels, float eps, double *out) {
+  if (int r = check_ctx(ctx)) return r;
+  if (!accum || !ref_sum || !out || num_pixels < 1)
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "bad relmse arguments");
+  double *d = nullptr;
+  VRT_HIP(hipMalloc(&d, sizeof(double)));
+  hipError_t e = hipMemsetAsync(d, 0, sizeof(double), ctx->stream);
+  if (e == hipSuccess)
+    e = launch_relmse(reinterpret_cast<const float4 *>(accum), ref_sum, ref_scale, num_pixels,
+                      eps, d, ctx->stream);
+  double s = 0.0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&s, d, sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return hip_fail(e, "relmse");
+  *out = s / (3.0 * double(num_pixels));
   return VRT_OK;
 }
 

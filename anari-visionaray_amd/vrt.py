@@ -73,7 +73,7 @@ class Params(C.Structure):
         ("background", f4), ("ambient_color", f3), ("ambient_radiance", C.c_float),
         ("occlusion_distance", C.c_float), ("ambient_samples", C.c_int),
         ("max_bounce", C.c_int), ("pixel_samples", C.c_int),
-        ("volume_sampling_rate", C.c_float),
+        ("volume_sampling_rate", C.c_float), ("rng_offset", C.c_uint32),
     ]
 
 
@@ -147,6 +147,8 @@ def lib():
                                         C.POINTER(Params), C.POINTER(Frame), vp,
                                         C.POINTER(NrcRender), ip, ip]
         L.vrt_nrc_frame_counts.argtypes = [vp, C.POINTER(C.c_uint32)]
+        L.vrt_relmse.argtypes = [vp, vp, vp, C.c_double, ip, C.c_float,
+                                 C.POINTER(C.c_double)]
         L.vrt_sample_field.argtypes = [vp, C.POINTER(Volume), ip, vp, vp]
         L.vrt_delta_track.argtypes = [vp, C.POINTER(Scene), ip, vp, vp, vp]
         L.vrt_raycast.argtypes = [vp, C.POINTER(Scene), C.POINTER(Camera), C.POINTER(Params),
@@ -360,10 +362,11 @@ def camera(position, direction, up, fovy, aspect, region=(0, 0, 1, 1), shutter=(
 
 def params(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.2,
            occlusion_distance=1e20, ambient_samples=1, max_bounce=7, pixel_samples=1,
-           volume_sampling_rate=0.5):
+           volume_sampling_rate=0.5, rng_offset=0):
     """Renderer parameters with the ANARI defaults of the reference
     (Renderer.cpp:42-53, Pathtrace.cpp:19-25, Raycast.cpp:19)."""
     p = Params()
+    p.rng_offset = rng_offset
     p.background = f4(*background)
     p.ambient_color = f3(*ambient_color)
     p.ambient_radiance = ambient_radiance
@@ -532,6 +535,14 @@ def nrc_render_options(bounds_lo, bounds_hi, train=True, train_samples=65536, tr
     o.bounds_lo = f3(*bounds_lo)
     o.bounds_hi = f3(*bounds_hi)
     return o
+
+
+def relmse(ctx, frame, ref_frame, ref_frames, eps=1e-2):
+    """relMSE of frame's running mean against ref_frame's FP64 sums / ref_frames."""
+    out = C.c_double()
+    check(lib().vrt_relmse(ctx.handle, frame.accum.ptr, ref_frame.accum64.ptr,
+                           1.0 / ref_frames, frame.width * frame.height, eps, C.byref(out)))
+    return out.value
 
 
 def pathtrace_nrc(ctx, scene, cam, prm, frame, nrc, opts, accum_id, frame_id=0):

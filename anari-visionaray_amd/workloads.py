@@ -115,16 +115,18 @@ def fbm_pathtrace(n=512, width=1920, height=1080, frames=64, seed=1, brick=16,
 class GpuScene:
     """The workload resident on one GPU through the vrt C ABI."""
 
-    def __init__(self, ctx, wl: Workload, color_type=None, depth=False, accum64=False):
+    def __init__(self, ctx, wl: Workload, color_type=None, depth=False, accum64=False,
+                 volume=None, params=None):
         self.wl = wl
         self.ctx = ctx
-        self.volume = _vrt.DeviceVolume(ctx, wl.voxels, wl.origin, wl.spacing, wl.tf,
-                                        wl.value_range, wl.unit_distance, wl.grid_dims)
+        self.volume = volume or _vrt.DeviceVolume(ctx, wl.voxels, wl.origin, wl.spacing, wl.tf,
+                                                  wl.value_range, wl.unit_distance, wl.grid_dims)
         lights = [_vrt.directional_light(L["direction"], L["intensity"], L["color"])
                   for L in wl.lights]
         self.scene = _vrt.make_scene([self.volume], lights)
         self.camera = _vrt.camera(**wl.camera)
         p = dict(wl.params)
+        p.update(params or {})
         self.params = _vrt.params(**p)
         self.frame = _vrt.DeviceFrame(ctx, wl.width, wl.height,
                                       color_type or _vrt.TYPE_FLOAT32_VEC4, depth, accum64)

@@ -65,6 +65,82 @@ def cpu_baseline(wl, target_seconds=12.0):
     }
 
 
+def nrc_vs_plain(pkg, ctx, wl, volume, plain_ms, ref_frames=2048, cap=1024, warmup=64,
+                 target=1e-2, oracle_stride=24, oracle_spp=4096):
+    """Wall time to relMSE <= target of the plain path tracer and of the
+    radiance-cache path tracer, both measured against an FP64-accumulated
+    plain reference of ref_frames spp; the reference's own relMSE against
+    the CPU FP64 oracle is measured on every oracle_stride-th pixel."""
+    vrt, wls = pkg.vrt, pkg.workloads
+    ref = wls.GpuScene(ctx, wl, accum64=True, volume=volume)
+    ref.frame.reset()
+    for a in range(ref_frames):
+        ref.render(a)
+    ctx.sync()
+
+    # plain: pixel streams independent of the reference's (rng_offset)
+    gs = wls.GpuScene(ctx, wl, volume=volume, params={"rng_offset": 1 << 20})
+    gs.frame.reset()
+    frames_plain, e = 0, float("inf")
+    while e > target and frames_plain < cap:
+        gs.render(frames_plain)
+        frames_plain += 1
+        e = vrt.relmse(ctx, gs.frame, ref.frame, ref_frames)
+    plain_reached = e <= target
+
+    # radiance cache: train from scratch for `warmup` frames, then accumulate
+    nrc = vrt.Nrc(ctx, max_batch=16384, seed=1)
+    warm = wls.GpuScene(ctx, wl, volume=volume, params={"rng_offset": 2 << 20})
+    opts = warm.nrc_options(train=True, train_samples=65536, train_steps=4)
+    ctx.timer_start()
+    for a in range(warmup):
+        warm.render_nrc(nrc, opts, a)
+    warm_ms = ctx.timer_stop()
+    nrc_ms = warm_ms / warmup
+    gn = wls.GpuScene(ctx, wl, volume=volume, params={"rng_offset": 3 << 20})
+    gn.frame.reset()
+    frames_nrc, e_nrc = 0, float("inf")
+    while e_nrc > target and frames_nrc < cap:
+        gn.render_nrc(nrc, opts, frames_nrc)
+        frames_nrc += 1
+        e_nrc = vrt.relmse(ctx, gn.frame, ref.frame, ref_frames)
+    counts = nrc.frame_counts()
+    nrc.close()
+    nrc_reached = e_nrc <= target
+
+    t_plain = frames_plain * plain_ms
+    t_nrc = (warmup + frames_nrc) * nrc_ms
+    out = {
+        "target_relmse": target,
+        "relmse_eps": 1e-2,
+        "reference": f"plain GPU path tracer, {ref_frames} spp, FP64 accumulation",
+        "plain": {"frames": frames_plain, "ms_per_frame": round(plain_ms, 4),
+                  "ms_to_target": round(t_plain, 2) if plain_reached else None,
+                  "relmse": e},
+        "nrc": {"warmup_frames": warmup, "frames": frames_nrc, "ms_per_frame": round(nrc_ms, 4),
+                "ms_to_target": round(t_nrc, 2) if nrc_reached else None, "relmse": e_nrc,
+                "queries_last_frame": counts[0], "training_samples_last_frame": min(counts[1], 65536),
+                "precision": "fp16 weights/activations, fp32 accumulation, fp32 Adam"},
+        "speedup": round(t_plain / t_nrc, 3) if plain_reached and nrc_reached else None,
+    }
+    # the reference against the CPU FP64 oracle on a pixel subset
+    if oracle_stride:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        import numpy as np
+        t0 = time.perf_counter()
+        o = O.OracleWorkload(wl).pathtrace_accumulate(oracle_spp, first_accum_id=50000, fp64=True,
+                                                      stride=oracle_stride,
+                                                      threads=os.cpu_count() or 1)
+        g = (ref.frame.accum64_array() / ref_frames)[::oracle_stride, ::oracle_stride]
+        o3, g3 = o[..., :3], g[..., :3]
+        out["reference_vs_cpu_fp64_oracle"] = {
+            "relmse": float(np.mean((g3 - o3) ** 2 / (o3 ** 2 + 1e-2))),
+            "pixels": int(o.shape[0] * o.shape[1]), "oracle_spp": oracle_spp,
+            "seconds": round(time.perf_counter() - t0, 1)}
+    return out
+
+
 def calibrate_copy(vrt, mib, reps):
     """Copy kernel of known size: each launch reads and writes mib MiB."""
     ctx = vrt.Context(0)
@@ -92,6 +168,8 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-nrc", action="store_true",
+                    help="skip the time-to-relMSE comparison with the radiance cache")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (counter calibration)")
@@ -147,6 +225,11 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(wl, args.cpu_seconds)
+    nrc = None
+    if not args.no_nrc:
+        nrc = nrc_vs_plain(pkg, ctx, wl, gs.volume, ms_per_step,
+                           oracle_stride=0 if args.no_cpu else 24)
+        print(f"# time to relMSE 1e-2: {json.dumps(nrc)}", file=sys.stderr, flush=True)
 
     out = {
         "metric": "Msamples/s at 1080p; time-to-relMSE 1e-2, NRC vs plain pathtrace",
@@ -182,6 +265,7 @@ def main():
             "bytes_per_lookup": BYTES_PER_LOOKUP,
         },
         "cpu_baseline": cpu,
+        "time_to_relmse": nrc,
     }
     ctx.close()
     print(json.dumps(out), flush=True)

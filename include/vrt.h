@@ -123,6 +123,10 @@ typedef struct vrt_params {
   int max_bounce;            /* 7 */
   int pixel_samples;         /* 1 */
   float volume_sampling_rate;/* 0.5 (raycast) */
+  /* extension: pixel streams are seeded with (pixelID, accumID + rng_offset);
+     0 reproduces the reference (Pathtrace_impl.cpp:512), other values give
+     accumulations that are independent of it */
+  uint32_t rng_offset;
 } vrt_params;
 
 /* ANARI data types used for frame channels (ANARI 1.0 enum values). */
@@ -214,6 +218,12 @@ VRT_API int vrt_delta_track(vrt_context ctx, const vrt_scene *scene, int n,
 /* ---------------------------------------------------------------------- */
 /* Renderers                                                               */
 /* ---------------------------------------------------------------------- */
+
+/* relMSE of a frame's float4 running mean against an FP64 reference given
+ * as per-pixel sums (double4) times ref_scale: the mean over pixels and RGB
+ * of (x - r)^2 / (r^2 + eps).  Synchronises; result in *out. */
+VRT_API int vrt_relmse(vrt_context ctx, const float *accum, const double *ref_sum,
+                       double ref_scale, int num_pixels, float eps, double *out);
 
 /* Clear accum (and accum64, depth) of a frame: the accumulation reset of
  * Frame::renderFrame (frame/Frame.cpp:287-312). */

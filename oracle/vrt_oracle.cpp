@@ -705,7 +705,7 @@ V4<R> pathtrace_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params
   // Pathtrace_impl.cpp:510-513: rng seeded by (pixelID, accumID); the
   // ScreenSample holds a copy, RR draws from the un-advanced original
   uint32_t pixelID = uint32_t(x + W * y);
-  Random rng(pixelID, uint32_t(accumID));
+  Random rng(pixelID, uint32_t(accumID) + P->rng_offset);
   Random srnd = rng;
   V4<R> accum{0, 0, 0, 0};
   const int spp = P->pixel_samples;

@@ -88,9 +88,9 @@ typedef struct orc_params {
   int max_bounce;
   int pixel_samples;
   float volume_sampling_rate; /* raycast only */
-  /* NRC-style truncation for the FP64 reference of truncated estimators:
-     0 = off.  Not a reference feature. */
-  int reserved;
+  /* pixel streams seeded with (pixelID, accumID + rng_offset); 0 is the
+     reference's seeding */
+  uint32_t rng_offset;
 } orc_params;
 
 /* --- A1: LCG<4> / Random (common.h:55-102) --- */

@@ -47,7 +47,7 @@ class OrcParams(C.Structure):
     _fields_ = [("bg", C.c_float * 4), ("ambient_color", f3), ("ambient_radiance", C.c_float),
                 ("occlusion_distance", C.c_float), ("ambient_samples", C.c_int),
                 ("max_bounce", C.c_int), ("pixel_samples", C.c_int),
-                ("volume_sampling_rate", C.c_float), ("reserved", C.c_int)]
+                ("volume_sampling_rate", C.c_float), ("rng_offset", C.c_uint32)]
 
 
 _lib = None
@@ -204,7 +204,7 @@ def camera(position, direction, up, fovy, aspect, region=(0, 0, 1, 1), shutter=(
 
 def params(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.2,
            occlusion_distance=1e20, ambient_samples=1, max_bounce=7, pixel_samples=1,
-           volume_sampling_rate=0.5):
+           volume_sampling_rate=0.5, rng_offset=0):
     p = OrcParams()
     p.bg = (C.c_float * 4)(*background)
     p.ambient_color = f3(*ambient_color)
@@ -214,7 +214,7 @@ def params(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.
     p.max_bounce = max_bounce
     p.pixel_samples = pixel_samples
     p.volume_sampling_rate = volume_sampling_rate
-    p.reserved = 0
+    p.rng_offset = rng_offset
     return p
 
 
