@@ -50,6 +50,9 @@ struct KVolume {
   float unit_distance;
   const float *majorants;
   int gdims[3];
+  const uint8_t *mbytes;  // majorant multiples of *mstep (optional)
+  const float *mstep;
+  int maj_lds_offset;     // byte offset of this grid in the workgroup's LDS, -1: global
   float A[9];          // affineInv
   f3 tinv;             // transInv
 };
@@ -65,7 +68,8 @@ struct KLight {
 struct KScene {
   int num_volumes;
   int num_lights;
-  int tf_lds_total;  // float4 entries of all tables staged in LDS
+  int tf_lds_total;   // float4 entries of all tables staged in LDS
+  int maj_lds_bytes;  // majorant bytes staged in LDS after the tables
   KVolume vol[VRT_MAX_VOLUMES];
   KLight light[VRT_MAX_LIGHTS];
 };

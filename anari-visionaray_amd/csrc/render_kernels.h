@@ -24,7 +24,10 @@ hipError_t launch_relmse(const float4 *x, const double *ref_sum, double ref_scal
 int pathtrace_blocks_per_cu(size_t lds_bytes, bool nrc = false);
 hipError_t launch_brickify(const float *lin, const int dims[3], float *bricks, hipStream_t s);
 hipError_t launch_value_ranges(const KVolume &v, float *ranges, hipStream_t s);
-hipError_t launch_majorants(const KVolume &v, const float *ranges, float *maj, hipStream_t s);
+hipError_t launch_majorants(const KVolume &v, const float *ranges, float *maj, uint8_t *bytes,
+                            float *step, hipStream_t s);
+// LDS bytes a render kernel needs for the scene's tables and majorant grids
+size_t scene_lds_size(const KScene &sc);
 hipError_t launch_frame_reset(const KFrame &F, hipStream_t s);
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 hipError_t launch_sample_field(const KVolume &v, int n, const float *pts, float *out,

@@ -13,7 +13,23 @@
 
 #include <float.h>
 
+#ifndef VRT_NATIVE_TRIG
+#define VRT_NATIVE_TRIG 0
+#endif
+#ifndef VRT_FAST_RCP
+#define VRT_FAST_RCP 0
+#endif
+
 namespace vrt {
+
+// reciprocal of a ray direction component (slab tests, DDA setup)
+__device__ inline float rcp_dir(float x) {
+#if VRT_FAST_RCP
+  return x == 0.f ? copysignf(__builtin_inff(), x) : __builtin_amdgcn_rcpf(x);
+#else
+  return 1.f / x;
+#endif
+}
 
 // ---------------------------------------------------------------------------
 // RNG: LCG<4> seeded by TEA-4 (common.h:55-102)
@@ -67,7 +83,7 @@ struct BoxHit {
 };
 
 __device__ inline BoxHit intersect_box(const KRay &r, f3 lo, f3 hi) {
-  f3 inv = mk(1.f / r.dir.x, 1.f / r.dir.y, 1.f / r.dir.z);
+  f3 inv = mk(rcp_dir(r.dir.x), rcp_dir(r.dir.y), rcp_dir(r.dir.z));
   f3 t1 = (lo - r.ori) * inv;
   f3 t2 = (hi - r.ori) * inv;
   float tn = fmaxf(fminf(t1.z, t2.z), fmaxf(fminf(t1.y, t2.y), fminf(t1.x, t2.x)));
@@ -192,10 +208,35 @@ __device__ inline float4 classify(const KVolume &v, const float4 *lds, float val
                      lerpf(a.w, b.w, w));
 }
 
+// Workgroup LDS: [TF tables, float4 each][majorant steps, one float4][majorant
+// bytes of the grids that fit]
+__host__ __device__ inline size_t scene_lds_bytes(const KScene &sc) {
+  return size_t(sc.tf_lds_total) * 16 + 16 + ((size_t(sc.maj_lds_bytes) + 15) & ~size_t(15));
+}
+__device__ inline float *lds_steps(const KScene &sc, float4 *lds) {
+  return reinterpret_cast<float *>(lds + sc.tf_lds_total);
+}
+__device__ inline const uint8_t *lds_majorants(const KScene &sc, const float4 *lds, int vi) {
+  const KVolume &v = sc.vol[vi];
+  if (v.maj_lds_offset < 0) return nullptr;
+  return reinterpret_cast<const uint8_t *>(lds + sc.tf_lds_total + 1) + v.maj_lds_offset;
+}
+
 __device__ inline void stage_tf(const KScene &sc, float4 *lds) {
+  float *steps = lds_steps(sc, lds);
+  uint8_t *bytes = reinterpret_cast<uint8_t *>(lds + sc.tf_lds_total + 1);
   for (int vi = 0; vi < sc.num_volumes; ++vi) {
     const KVolume &v = sc.vol[vi];
     for (int i = threadIdx.x; i < v.tf_size; i += blockDim.x) lds[v.tf_lds_offset + i] = v.tf[i];
+    if (v.maj_lds_offset >= 0) {
+      const int n = v.gdims[0] * v.gdims[1] * v.gdims[2];
+      uint8_t *dst = bytes + v.maj_lds_offset;
+      const int n16 = n >> 4;
+      for (int i = threadIdx.x; i < n16; i += blockDim.x)
+        reinterpret_cast<uint4 *>(dst)[i] = reinterpret_cast<const uint4 *>(v.mbytes)[i];
+      for (int i = (n16 << 4) + threadIdx.x; i < n; i += blockDim.x) dst[i] = v.mbytes[i];
+    }
+    if (threadIdx.x == 0) steps[vi] = v.mstep ? *v.mstep : 0.f;
   }
   __syncthreads();
 }
@@ -258,8 +299,10 @@ __device__ inline bool woodcock_cell(const KVolume &v, const float4 *lds, float 
   }
 }
 
-__device__ inline VolHit delta_track(const KVolume &v, const float4 *lds, KRay ray, uint32_t &rs,
-                                     uint32_t &lookups) {
+// `mb`/`mstep`: the volume's majorant bytes in LDS and their step (mb null:
+// the float majorants in global memory)
+__device__ inline VolHit delta_track(const KVolume &v, const float4 *lds, const uint8_t *mb,
+                                     float mstep, KRay ray, uint32_t &rs, uint32_t &lookups) {
   VolHit out;
   out.hit = false;
   out.t = FLT_MAX;
@@ -285,7 +328,7 @@ __device__ inline VolHit delta_track(const KVolume &v, const float4 *lds, KRay r
     // dda3 with the ray moved so that tmin becomes 0 (DDA.h:21-25)
     const f3 o = ori + dir * tmin;
     const float tm = tmax - tmin;
-    const f3 rcp = mk(1.f / dir.x, 1.f / dir.y, 1.f / dir.z);
+    const f3 rcp = mk(rcp_dir(dir.x), rcp_dir(dir.y), rcp_dir(dir.z));
     // grid bounds = voxel space [0,1]^3
     const f3 l = (mk(0.f) - o) * rcp;
     const f3 h = (mk(1.f) - o) * rcp;
@@ -318,7 +361,7 @@ __device__ inline VolHit delta_track(const KVolume &v, const float4 *lds, KRay r
       const float tc = fminf(tnext[0], fminf(tnext[1], tnext[2]));
       const float t1 = fminf(tc, tm);
       const int li = (cell[2] * v.gdims[1] + cell[1]) * v.gdims[0] + cell[0];
-      const float maj = v.majorants[li];
+      const float maj = mb ? float(mb[li]) * mstep : v.majorants[li];
       if (woodcock_cell(v, lds, maj, invUD, ori, dir, tmin + t0, tmin + t1, rs, t_hit, smp,
                         lookups)) {
         hit = true;
@@ -356,27 +399,39 @@ __device__ inline VolHit delta_track(const KVolume &v, const float4 *lds, KRay r
 // ---------------------------------------------------------------------------
 // Sampling helpers
 // ---------------------------------------------------------------------------
+// sin/cos of an angle given in revolutions (turns).  The hardware sine and
+// cosine take revolutions, so the sampled angles are formed in turns.
+__device__ inline void sincos_turns(float t, float &s, float &c) {
+#if VRT_NATIVE_TRIG
+  s = __builtin_amdgcn_sinf(t);
+  c = __builtin_amdgcn_cosf(t);
+#else
+  sincosf(6.28318530717958647692f * t, &s, &c);
+#endif
+}
+
 __device__ inline f3 uniform_sample_sphere(float u1, float u2) {
   float z = 1.f - 2.f * u1;
   float r = sqrtf(fmaxf(0.f, 1.f - z * z));
   float s, c;
-  sincosf(6.28318530717958647692f * u2, &s, &c);
+  sincos_turns(u2, s, c);  // phi = 2 pi u2
   return mk(r * c, r * s, z);
 }
 __device__ inline f3 cosine_sample_hemisphere(float u1, float u2) {
   float sx = 2.f * u1 - 1.f, sy = 2.f * u2 - 1.f;
   float dx = 0.f, dy = 0.f;
   if (!(sx == 0.f && sy == 0.f)) {
-    float r, theta;
+    // Shirley-Chiu: theta = pi/4 (sy/sx) or pi/2 - pi/4 (sx/sy), in turns
+    float r, turns;
     if (fabsf(sx) > fabsf(sy)) {
       r = sx;
-      theta = 0.78539816339744830962f * (sy / sx);
+      turns = 0.125f * (sy / sx);
     } else {
       r = sy;
-      theta = 1.57079632679489661923f - 0.78539816339744830962f * (sx / sy);
+      turns = 0.25f - 0.125f * (sx / sy);
     }
     float s, c;
-    sincosf(theta, &s, &c);
+    sincos_turns(turns, s, c);
     dx = r * c;
     dy = r * s;
   }
@@ -629,7 +684,9 @@ __device__ inline bool path_pass(PathState &st, const KScene &sc, const KParams 
   vh.hit = false;
   vh.t = FLT_MAX;
   for (int i = 0; i < sc.num_volumes; ++i) {
-    VolHit h = delta_track(sc.vol[i], s_tf, to_local(sc.vol[i], st.ray), st.srng, lookups);
+    VolHit h = delta_track(sc.vol[i], s_tf, lds_majorants(sc, s_tf, i),
+                           lds_steps(sc, const_cast<float4 *>(s_tf))[i], to_local(sc.vol[i], st.ray),
+                           st.srng, lookups);
     if (h.hit && h.t < vh.t) vh = h;
   }
   const bool hit = vh.hit;
@@ -1030,25 +1087,47 @@ __global__ void __launch_bounds__(256)
 
 // GridAccel::computeMaxOpacities (GridAccel.cpp:69-107) with the TF taps
 // that tex1D can actually read for values in [vmin, vmax]
-__global__ void k_majorants(KVolume v, const float2 *__restrict__ ranges, int nmc,
-                            float *__restrict__ maj) {
+// The stored majorant is rounded up to a multiple q*step, q in 0..255, with
+// step = (max TF opacity)/255 rounded up, so the grid fits in LDS as bytes
+// (a larger majorant leaves delta tracking unbiased).
+__global__ void __launch_bounds__(256)
+    k_majorants(KVolume v, const float2 *__restrict__ ranges, int nmc, float *__restrict__ maj,
+                uint8_t *__restrict__ bytes, float *__restrict__ step_out) {
+  __shared__ float smax[256];
+  float t = 0.f;
+  for (int k = threadIdx.x; k < v.tf_size; k += blockDim.x) t = fmaxf(t, v.tf[k].w);
+  smax[threadIdx.x] = t;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (int(threadIdx.x) < s) smax[threadIdx.x] = fmaxf(smax[threadIdx.x], smax[threadIdx.x + s]);
+    __syncthreads();
+  }
+  const float tf_max = smax[0];
+  float step = tf_max / 255.f;
+  if (step * 255.f < tf_max) step = __uint_as_float(__float_as_uint(step) + 1u);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && step_out) *step_out = step;
+
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nmc) return;
   const float2 r = ranges[i];
-  if (r.y < r.x) {
-    maj[i] = 0.f;
-    return;
-  }
-  const int n = v.tf_size;
-  const float u0 = (r.x - v.vlo) / (v.vhi - v.vlo);
-  const float u1 = (r.y - v.vlo) / (v.vhi - v.vlo);
-  const float c0 = fminf(fmaxf(u0 * float(n) - 0.5f, -1.f), float(n));
-  const float c1 = fminf(fmaxf(u1 * float(n) - 0.5f, -1.f), float(n));
-  const int lo = min(max(int(floorf(c0)), 0), n - 1);
-  const int hi = min(max(int(floorf(c1)) + 1, 0), n - 1);
   float m = 0.f;
-  for (int k = lo; k <= hi; ++k) m = fmaxf(m, v.tf[k].w);
-  maj[i] = m;
+  if (!(r.y < r.x)) {
+    const int n = v.tf_size;
+    const float u0 = (r.x - v.vlo) / (v.vhi - v.vlo);
+    const float u1 = (r.y - v.vlo) / (v.vhi - v.vlo);
+    const float c0 = fminf(fmaxf(u0 * float(n) - 0.5f, -1.f), float(n));
+    const float c1 = fminf(fmaxf(u1 * float(n) - 0.5f, -1.f), float(n));
+    const int lo = min(max(int(floorf(c0)), 0), n - 1);
+    const int hi = min(max(int(floorf(c1)) + 1, 0), n - 1);
+    for (int k = lo; k <= hi; ++k) m = fmaxf(m, v.tf[k].w);
+  }
+  int q = 0;
+  if (m > 0.f) {
+    q = int(fminf(255.f, ceilf(m / step)));
+    while (q < 255 && float(q) * step < m) ++q;
+  }
+  maj[i] = float(q) * step;
+  if (bytes) bytes[i] = uint8_t(q);
 }
 
 __global__ void k_sample_field(KVolume v, int n, const float *__restrict__ pts,
@@ -1073,7 +1152,8 @@ __global__ void __launch_bounds__(256)
   ray.tmax = r[7];
   uint32_t s = rng[i];
   uint32_t lookups = 0;
-  VolHit h = delta_track(sc.vol[0], s_tf, to_local(sc.vol[0], ray), s, lookups);
+  VolHit h = delta_track(sc.vol[0], s_tf, lds_majorants(sc, s_tf, 0), lds_steps(sc, s_tf)[0],
+                         to_local(sc.vol[0], ray), s, lookups);
   rng[i] = s;
   tout[i] = h.hit ? h.t : __builtin_inff();
 }
@@ -1107,7 +1187,7 @@ static inline int tiles(const KFrame &F) { return ((F.width + 15) / 16) * ((F.he
 
 hipError_t launch_raycast(const KScene &sc, const KCamera &cam, const KParams &P, const KFrame &F,
                           hipStream_t s) {
-  const size_t lds = size_t(sc.tf_lds_total) * sizeof(float4);
+  const size_t lds = scene_lds_bytes(sc);
   hipLaunchKernelGGL(k_raycast, dim3(tiles(F)), dim3(256), lds, s, sc, cam, P, F);
   return hipGetLastError();
 }
@@ -1115,7 +1195,7 @@ hipError_t launch_raycast(const KScene &sc, const KCamera &cam, const KParams &P
 hipError_t launch_pathtrace(const KScene &sc, const KCamera &cam, const KParams &P,
                             const KFrame &F, int accumID, uint32_t *queues,
                             uint32_t *queues_next, int blocks, hipStream_t s, const KNrc *nrc) {
-  const size_t lds = size_t(sc.tf_lds_total) * sizeof(float4);
+  const size_t lds = scene_lds_bytes(sc);
   if (nrc) {
     hipLaunchKernelGGL(k_nrc_frame_begin, dim3(1), dim3(64), 0, s, *nrc);
     hipLaunchKernelGGL(k_pathtrace<true>, dim3(blocks), dim3(256), lds, s, sc, cam, P, F, accumID,
@@ -1160,10 +1240,13 @@ hipError_t launch_value_ranges(const KVolume &v, float *ranges, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_majorants(const KVolume &v, const float *ranges, float *maj, hipStream_t s) {
+size_t scene_lds_size(const KScene &sc) { return scene_lds_bytes(sc); }
+
+hipError_t launch_majorants(const KVolume &v, const float *ranges, float *maj, uint8_t *bytes,
+                            float *step, hipStream_t s) {
   const int nmc = v.gdims[0] * v.gdims[1] * v.gdims[2];
   hipLaunchKernelGGL(k_majorants, dim3((nmc + 255) / 256), dim3(256), 0, s, v,
-                     reinterpret_cast<const float2 *>(ranges), nmc, maj);
+                     reinterpret_cast<const float2 *>(ranges), nmc, maj, bytes, step);
   return hipGetLastError();
 }
 
@@ -1175,7 +1258,7 @@ hipError_t launch_sample_field(const KVolume &v, int n, const float *pts, float 
 
 hipError_t launch_delta_track(const KScene &sc, int n, const float *rays, uint32_t *rng,
                               float *tout, hipStream_t s) {
-  const size_t lds = size_t(sc.tf_lds_total) * sizeof(float4);
+  const size_t lds = scene_lds_bytes(sc);
   hipLaunchKernelGGL(k_delta_track, dim3((n + 255) / 256), dim3(256), lds, s, sc, n, rays, rng,
                      tout);
   return hipGetLastError();

@@ -85,12 +85,16 @@ int make_volume(const vrt_volume &in, int lds_offset, KVolume &v) {
   v.unit_distance = in.unit_distance;
   v.majorants = in.majorants;
   for (int a = 0; a < 3; ++a) v.gdims[a] = in.grid_dims[0] > 0 ? in.grid_dims[a] : 0;
+  v.mbytes = in.majorant_bytes;
+  v.mstep = in.majorant_step;
+  v.maj_lds_offset = -1;
   for (int i = 0; i < 9; ++i) v.A[i] = in.affine_inv[i];
   v.tinv = mk(in.trans_inv[0], in.trans_inv[1], in.trans_inv[2]);
   return VRT_OK;
 }
 
-constexpr int kMaxLdsTf = 4096;  // float4 entries (64 KiB)
+constexpr int kMaxLdsTf = 4096;           // float4 entries (64 KiB)
+constexpr int kMaxLdsMajorants = 40960;   // bytes: a 32^3 grid (16^3-voxel cells at 512^3)
 
 int make_scene(const vrt_scene *s, KScene &k) {
   if (!s) return fail(VRT_ERROR_INVALID_ARGUMENT, "null scene");
@@ -110,6 +114,18 @@ int make_scene(const vrt_scene *s, KScene &k) {
   if (off > kMaxLdsTf)
     return fail(VRT_ERROR_UNSUPPORTED, "transfer function tables exceed %d entries", kMaxLdsTf);
   k.tf_lds_total = off;
+  // majorant grids with byte copies go to LDS while they fit; the others are
+  // read as floats from global memory
+  int mbytes = 0;
+  for (int i = 0; i < s->num_volumes; ++i) {
+    KVolume &v = k.vol[i];
+    const int cells = v.gdims[0] * v.gdims[1] * v.gdims[2];
+    if (cells > 0 && v.mbytes && v.mstep && mbytes + cells <= kMaxLdsMajorants) {
+      v.maj_lds_offset = mbytes;
+      mbytes += (cells + 15) & ~15;
+    }
+  }
+  k.maj_lds_bytes = mbytes;
   for (int i = 0; i < s->num_lights; ++i) {
     const vrt_light &L = s->lights[i];
     KLight &o = k.light[i];
@@ -350,7 +366,8 @@ int vrt_build_value_ranges(vrt_context ctx, const vrt_volume *vol, float *ranges
   return VRT_OK;
 }
 
-int vrt_compute_majorants(vrt_context ctx, const vrt_volume *vol, const float *ranges, float *maj) {
+int vrt_compute_majorants(vrt_context ctx, const vrt_volume *vol, const float *ranges, float *maj,
+                          uint8_t *bytes, float *step) {
   if (int r = check_ctx(ctx)) return r;
   if (!vol || !ranges || !maj) return fail(VRT_ERROR_INVALID_ARGUMENT, "null argument");
   vrt_volume tmp = *vol;
@@ -359,7 +376,7 @@ int vrt_compute_majorants(vrt_context ctx, const vrt_volume *vol, const float *r
   if (!tmp.majorants) tmp.majorants = maj;
   KVolume v;
   if (int r = make_volume(tmp, 0, v)) return r;
-  VRT_HIP(launch_majorants(v, ranges, maj, ctx->stream));
+  VRT_HIP(launch_majorants(v, ranges, maj, bytes, step, ctx->stream));
   return VRT_OK;
 }
 
@@ -428,6 +445,9 @@ int vrt_raycast(vrt_context ctx, const vrt_scene *scene, const vrt_camera *camer
   if (int r = make_camera(camera, cam)) return r;
   if (int r = make_params(params, P)) return r;
   if (int r = make_frame(frame, F)) return r;
+  // the fixed-step marcher reads no majorants
+  sc.maj_lds_bytes = 0;
+  for (int i = 0; i < sc.num_volumes; ++i) sc.vol[i].maj_lds_offset = -1;
   VRT_HIP(launch_raycast(sc, cam, P, F, ctx->stream));
   return VRT_OK;
 }
@@ -446,7 +466,7 @@ int vrt_pathtrace(vrt_context ctx, const vrt_scene *scene, const vrt_camera *cam
   if (int r = make_params(params, P)) return r;
   if (int r = make_frame(frame, F)) return r;
   if (!F.accum) return fail(VRT_ERROR_INVALID_ARGUMENT, "pathtrace needs an accumulation buffer");
-  const size_t lds = size_t(sc.tf_lds_total) * 16;
+  const size_t lds = scene_lds_size(sc);
   int blocks = std::max(1, ctx->num_cus) * pathtrace_blocks_per_cu(lds);
   blocks = std::max(8, (blocks + 7) / 8 * 8);  // every XCD queue gets workgroups
   uint32_t *cur = ctx->queues + 8 * ctx->queue_parity;
@@ -502,7 +522,7 @@ int vrt_pathtrace_nrc(vrt_context ctx, const vrt_scene *scene, const vrt_camera 
                    1.f / (opts->bounds_hi[1] - opts->bounds_lo[1]),
                    1.f / (opts->bounds_hi[2] - opts->bounds_lo[2]));
 
-  const size_t lds = size_t(sc.tf_lds_total) * 16;
+  const size_t lds = scene_lds_size(sc);
   int blocks = std::max(1, ctx->num_cus) * pathtrace_blocks_per_cu(lds, true);
   blocks = std::max(8, (blocks + 7) / 8 * 8);
   uint32_t *cur = ctx->queues + 8 * ctx->queue_parity;

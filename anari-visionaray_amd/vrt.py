@@ -43,7 +43,8 @@ class Volume(C.Structure):
         ("cell_size", C.c_float), ("bricks", C.c_void_p), ("nanovdb_grid", C.c_void_p),
         ("tf_rgba", C.c_void_p), ("tf_size", C.c_int), ("value_lo", C.c_float),
         ("value_hi", C.c_float), ("unit_distance", C.c_float), ("majorants", C.c_void_p),
-        ("grid_dims", i3), ("affine_inv", C.c_float * 9), ("trans_inv", f3),
+        ("grid_dims", i3), ("majorant_bytes", C.c_void_p), ("majorant_step", C.c_void_p),
+        ("affine_inv", C.c_float * 9), ("trans_inv", f3),
     ]
 
 
@@ -132,7 +133,7 @@ def lib():
         L.vrt_structured_brick_bytes.restype = sz
         L.vrt_structured_brickify.argtypes = [vp, vp, C.POINTER(C.c_int), vp]
         L.vrt_build_value_ranges.argtypes = [vp, C.POINTER(Volume), vp]
-        L.vrt_compute_majorants.argtypes = [vp, C.POINTER(Volume), vp, vp]
+        L.vrt_compute_majorants.argtypes = [vp, C.POINTER(Volume), vp, vp, vp, vp]
         L.vrt_frame_reset.argtypes = [vp, C.POINTER(Frame)]
         L.vrt_nrc_param_count.restype = ip
         L.vrt_nrc_create.argtypes = [vp, C.POINTER(NrcConfig), C.POINTER(vp)]
@@ -307,10 +308,15 @@ class DeviceVolume:
             nmc = int(np.prod(self.grid_dims))
             self.ranges = ctx.buffer(nmc * 8)
             self.majorants = ctx.buffer(nmc * 4)
+            self.majorant_bytes = ctx.buffer(nmc)
+            self.majorant_step = ctx.buffer(4)
             check(lib().vrt_build_value_ranges(ctx.handle, C.byref(v), self.ranges.ptr))
             v.majorants = self.majorants.ptr
+            v.majorant_bytes = self.majorant_bytes.ptr
+            v.majorant_step = self.majorant_step.ptr
             check(lib().vrt_compute_majorants(ctx.handle, C.byref(v), self.ranges.ptr,
-                                              self.majorants.ptr))
+                                              self.majorants.ptr, self.majorant_bytes.ptr,
+                                              self.majorant_step.ptr))
             ctx.sync()
         self.desc = v
 

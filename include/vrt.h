@@ -67,9 +67,14 @@ typedef struct vrt_volume {
   float value_lo, value_hi;
   float unit_distance;
   /* majorant grid over voxel space [0,1]^3 (grid_dims[0] == 0: none, the
-     whole box is one segment with majorant 1) */
+     whole box is one segment with majorant 1).  Majorants are multiples
+     q * step of step = (max TF opacity)/255 rounded up, q in 0..255;
+     majorant_bytes holds q per cell and *majorant_step the step (device;
+     optional -- without them the kernels read the float values) */
   const float *majorants;
   int grid_dims[3];
+  const uint8_t *majorant_bytes;
+  const float *majorant_step;
   /* instance: object = affine_inv * (world + trans_inv) */
   float affine_inv[9]; /* row-major 3x3 */
   float trans_inv[3];
@@ -196,10 +201,13 @@ VRT_API int vrt_structured_brickify(vrt_context ctx, const float *linear,
 VRT_API int vrt_build_value_ranges(vrt_context ctx, const vrt_volume *vol,
                                    float *ranges);
 
-/* Majorants (max TF opacity over each macrocell's value range).
- * Replaces GridAccel::computeMaxOpacities (GridAccel.cpp:69-107). */
+/* Majorants (max TF opacity over each macrocell's value range, rounded up
+ * to a multiple of the 8-bit step, see vrt_volume).  Replaces
+ * GridAccel::computeMaxOpacities (GridAccel.cpp:69-107).  `bytes` (one per
+ * cell) and `step` (one float) may be NULL. */
 VRT_API int vrt_compute_majorants(vrt_context ctx, const vrt_volume *vol,
-                                  const float *ranges, float *majorants);
+                                  const float *ranges, float *majorants,
+                                  uint8_t *bytes, float *step);
 
 /* Trilinear field values at n voxel-space points (device float3 array ->
  * device float array): the sampleField structuredRegular branch

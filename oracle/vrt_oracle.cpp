@@ -490,6 +490,24 @@ template <class R> inline V3<R> faceforward(V3<R> n, V3<R> i, V3<R> nref) {
   return dot(i, nref) < R(0) ? -n : n;
 }
 
+// Majorants are stored as 8-bit multiples of step = (max TF opacity)/255,
+// rounded up (a rounded-up majorant keeps delta tracking unbiased); the
+// kernels keep the bytes in LDS.  Same float operations as the GPU build.
+inline float majorant_step(const orc_volume *v) {
+  float tf_max = 0.f;
+  for (int i = 0; i < v->tf_size; ++i) tf_max = std::max(tf_max, v->tf_rgba[4 * i + 3]);
+  float step = tf_max / 255.f;
+  if (step * 255.f < tf_max) step = std::nextafter(step, std::numeric_limits<float>::infinity());
+  return step;
+}
+
+inline float quantize_majorant(float m, float step) {
+  if (!(m > 0.f)) return 0.f;
+  int q = int(std::min(255.f, std::ceil(m / step)));
+  while (q < 255 && float(q) * step < m) ++q;
+  return float(q) * step;
+}
+
 // ---------------------------------------------------------------------------
 // Scene wrapper
 // ---------------------------------------------------------------------------
@@ -946,7 +964,7 @@ void orc_build_majorants(const orc_volume *v, int gx, int gy, int gz, float *out
         int hi = std::min(std::max(int(std::floor(c1)) + 1, 0), n - 1);
         float m = 0.f;
         for (int i = lo; i <= hi; ++i) m = std::max(m, v->tf_rgba[4 * i + 3]);
-        out[(size_t(mz) * gy + my) * gx + mx] = m;
+        out[(size_t(mz) * gy + my) * gx + mx] = quantize_majorant(m, majorant_step(v));
       }
 }
 
