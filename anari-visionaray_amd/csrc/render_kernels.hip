@@ -268,132 +268,180 @@ struct VolHit {
   bool hit;
 };
 
-// One cell's collision loop.  Returns true on a real collision; `smp` is the
-// classified sample (albedo rgb, opacity) there.
-__device__ inline bool woodcock_cell(const KVolume &v, const float4 *lds, float majorant,
-                                     float invUD, f3 ori, f3 dir, float t0, float t1,
-                                     uint32_t &rs, float &t_hit, float4 &smp, uint32_t &lookups) {
-  if (majorant <= 0.f) return false;
-  if (!(t0 < t1)) {
-    // the first free-flight step already leaves the cell (-log(1-u) >= 0):
-    // consume the draw, skip the logarithm
-    rs = 1664525u * rs + 1013904223u;
-    return false;
-  }
-  const float scale = 1.f / (majorant * invUD);
-  float t = t0;
-  while (true) {
-    const float u0 = rnd(rs);
-    t -= __logf(1.f - u0) * scale;
-    if (!(t < t1)) return false;  // also leaves on NaN
-    f3 P = ori + dir * t;
-    float val = sample_structured(v, P);
-    ++lookups;
-    float4 s = classify(v, lds, val);
-    float u = rnd(rs);
-    if (s.w >= u * majorant) {
-      t_hit = t;
-      smp = s;
-      return true;
-    }
-  }
-}
+// Resumable delta tracking through one volume.  The walk is split into
+// trav_begin (box clip, voxel space, DDA set-up) and trav_step (one step of a
+// flat loop over "set up the current cell" and "one free-flight step in
+// it"), so that a persistent wave can hand lanes whose ray ended new rays
+// while the other lanes keep walking.  Per cell the draws are those of the
+// reference's callback: none for a zero majorant, one for an empty [t0, t1)
+// range (dda3 keeps calling it with empty ranges past tmax), else one per
+// free-flight step plus one per tentative collision.
+struct Trav {
+  f3 ori, dir;      // voxel space, unit direction
+  float tmin;       // voxel-space start; the DDA runs on t - tmin
+  float tm;         // grid: tmax - tmin; no grid: tmax
+  float invUD, dt_scale;
+  int cell[3];
+  float dist[3], tnext[3];
+  float t0, t1;     // current cell's range relative to tmin
+  float t, tend;    // free-flight position and the cell's end (absolute)
+  float maj, scale;
+  int cells_left;   // < 0: no majorant grid, one range with majorant 1
+  bool in_cell;
+};
 
-// `mb`/`mstep`: the volume's majorant bytes in LDS and their step (mb null:
-// the float majorants in global memory)
-__device__ inline VolHit delta_track(const KVolume &v, const float4 *lds, const uint8_t *mb,
-                                     float mstep, KRay ray, uint32_t &rs, uint32_t &lookups) {
-  VolHit out;
-  out.hit = false;
-  out.t = FLT_MAX;
+// false: the ray misses the volume's box
+__device__ inline bool trav_begin(const KVolume &v, KRay ray, Trav &T) {
   BoxHit bh = intersect_box(ray, v.blo, v.bhi);
-  if (!bh.hit) return out;
+  if (!bh.hit) return false;
   ray.tmin = fmaxf(ray.tmin, bh.tnear);
   ray.tmax = fminf(ray.tmax, bh.tfar);
-  f3 ori = v.S * (ray.ori + v.T);
+  const f3 ori = v.S * (ray.ori + v.T);
   f3 dir = v.S * ray.dir;
-  const float dt_scale = length(dir);
+  T.dt_scale = length(dir);
   dir = normalize(dir);
-  const float tmin = ray.tmin * dt_scale;
-  const float tmax = ray.tmax * dt_scale;
-  const float invUD = 1.f / (v.unit_distance * dt_scale);
+  T.ori = ori;
+  T.dir = dir;
+  T.tmin = ray.tmin * T.dt_scale;
+  const float tmax = ray.tmax * T.dt_scale;
+  T.invUD = 1.f / (v.unit_distance * T.dt_scale);
+  T.t0 = 0.f;
+  T.in_cell = false;
+  if (v.gdims[0] == 0) {
+    T.tm = tmax;
+    T.cells_left = -1;
+    return true;
+  }
+  // dda3 with the ray moved so that tmin becomes 0 (DDA.h:21-25)
+  const f3 o = ori + dir * T.tmin;
+  T.tm = tmax - T.tmin;
+  const f3 rcp = mk(rcp_dir(dir.x), rcp_dir(dir.y), rcp_dir(dir.z));
+  // grid bounds = voxel space [0,1]^3
+  const f3 l = (mk(0.f) - o) * rcp;
+  const f3 h = (mk(1.f) - o) * rcp;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int G = v.gdims[a];
+    const float oa = comp(o, a), da = comp(dir, a);
+    float c = oa * float(G);  // projectOnGrid over [0,1]
+    c = fminf(fmaxf(c, -1.f), float(G));
+    T.cell[a] = min(max(int(c), 0), G - 1);
+    const float la = comp(l, a), ha = comp(h, a);
+    const float tn = fminf(la, ha), tf = fmaxf(la, ha);
+    if (da == 0.f) {
+      T.dist[a] = 0.f;
+      T.tnext[a] = __builtin_inff();
+    } else {
+      T.dist[a] = fmaxf(0.f, (tf - tn) / float(G));
+      T.tnext[a] = da > 0.f ? tn + float(T.cell[a] + 1) * T.dist[a]
+                            : tn + float(G - T.cell[a]) * T.dist[a];
+    }
+  }
+  // a ray visits at most gx+gy+gz cells; the cap only guards degenerate
+  // (non-finite) rays
+  T.cells_left = v.gdims[0] + v.gdims[1] + v.gdims[2] + 3;
+  return true;
+}
 
+// One step of the walk.  Returns true when it ended: on a collision (`hit`,
+// voxel-space `t_hit`, classified sample `smp`) or on leaving the grid.
+// `mb`/`mstep`: the volume's majorant bytes in LDS and their step (mb null:
+// the float majorants in global memory).
+__device__ inline bool trav_step(const KVolume &v, const float4 *lds, const uint8_t *mb,
+                                 float mstep, Trav &T, uint32_t &rs, uint32_t &lookups,
+                                 bool &hit, float &t_hit, float4 &smp) {
+  bool advance = false;
+  if (!T.in_cell) {
+    float a0, a1, maj;
+    if (T.cells_left < 0) {
+      a0 = T.tmin;
+      a1 = T.tm;
+      maj = 1.f;
+    } else {
+      const float tc = fminf(T.tnext[0], fminf(T.tnext[1], T.tnext[2]));
+      T.t1 = fminf(tc, T.tm);
+      a0 = T.tmin + T.t0;
+      a1 = T.tmin + T.t1;
+      const int li = (T.cell[2] * v.gdims[1] + T.cell[1]) * v.gdims[0] + T.cell[0];
+      maj = mb ? float(mb[li]) * mstep : v.majorants[li];
+    }
+    if (maj > 0.f) {
+      if (a0 < a1) {
+        T.maj = maj;
+        T.scale = 1.f / (maj * T.invUD);
+        T.t = a0;
+        T.tend = a1;
+        T.in_cell = true;
+      } else {
+        rs = 1664525u * rs + 1013904223u;  // the step from an empty range
+        advance = true;
+      }
+    } else {
+      advance = true;
+    }
+  }
+  if (T.in_cell) {
+    T.t -= __logf(1.f - rnd(rs)) * T.scale;
+    if (!(T.t < T.tend)) {  // also leaves on NaN
+      T.in_cell = false;
+      advance = true;
+    } else {
+      const float val = sample_structured(v, T.ori + T.dir * T.t);
+      ++lookups;
+      const float4 s = classify(v, lds, val);
+      if (s.w >= rnd(rs) * T.maj) {
+        hit = true;
+        t_hit = T.t;
+        smp = s;
+        return true;
+      }
+    }
+  }
+  if (advance) {
+    if (T.cells_left < 0) return true;
+    const float tc = fminf(T.tnext[0], fminf(T.tnext[1], T.tnext[2]));
+    bool out_of_grid = false;
+    if (T.tnext[0] == tc) {
+      T.tnext[0] += T.dist[0];
+      T.cell[0] += T.dir.x > 0.f ? 1 : -1;
+      out_of_grid = T.cell[0] == (T.dir.x > 0.f ? v.gdims[0] : -1);
+    }
+    if (!out_of_grid && T.tnext[1] == tc) {
+      T.tnext[1] += T.dist[1];
+      T.cell[1] += T.dir.y > 0.f ? 1 : -1;
+      out_of_grid = T.cell[1] == (T.dir.y > 0.f ? v.gdims[1] : -1);
+    }
+    if (!out_of_grid && T.tnext[2] == tc) {
+      T.tnext[2] += T.dist[2];
+      T.cell[2] += T.dir.z > 0.f ? 1 : -1;
+      out_of_grid = T.cell[2] == (T.dir.z > 0.f ? v.gdims[2] : -1);
+    }
+    if (out_of_grid || --T.cells_left == 0) return true;
+    T.t0 = T.t1;
+  }
+  return false;
+}
+
+__device__ inline VolHit make_hit(bool hit, float t_hit, float4 smp, float dt_scale) {
+  VolHit out;
+  out.hit = hit;
+  out.t = hit ? t_hit / dt_scale : FLT_MAX;
+  out.albedo = mk(smp.x, smp.y, smp.z);
+  out.extinction = smp.w;
+  return out;
+}
+
+// intersect(Ray, Volume) (DeviceCopyableObjects.h:680-796) in one call
+__device__ inline VolHit delta_track(const KVolume &v, const float4 *lds, const uint8_t *mb,
+                                     float mstep, KRay ray, uint32_t &rs, uint32_t &lookups) {
+  Trav T;
+  bool hit = false;
   float t_hit = 0.f;
   float4 smp = make_float4(0.f, 0.f, 0.f, 0.f);
-  bool hit = false;
-
-  if (v.gdims[0] == 0) {
-    hit = woodcock_cell(v, lds, 1.f, invUD, ori, dir, tmin, tmax, rs, t_hit, smp, lookups);
-  } else {
-    // dda3 with the ray moved so that tmin becomes 0 (DDA.h:21-25)
-    const f3 o = ori + dir * tmin;
-    const float tm = tmax - tmin;
-    const f3 rcp = mk(rcp_dir(dir.x), rcp_dir(dir.y), rcp_dir(dir.z));
-    // grid bounds = voxel space [0,1]^3
-    const f3 l = (mk(0.f) - o) * rcp;
-    const f3 h = (mk(1.f) - o) * rcp;
-    int cell[3], step[3], stop[3];
-    float dist[3], tnext[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const int G = v.gdims[a];
-      const float oa = comp(o, a), da = comp(dir, a);
-      float c = oa * float(G);  // projectOnGrid over [0,1]
-      c = fminf(fmaxf(c, -1.f), float(G));
-      cell[a] = min(max(int(c), 0), G - 1);
-      step[a] = da > 0.f ? 1 : -1;
-      stop[a] = da > 0.f ? G : -1;
-      const float la = comp(l, a), ha = comp(h, a);
-      const float tn = fminf(la, ha), tf = fmaxf(la, ha);
-      if (da == 0.f) {
-        dist[a] = 0.f;
-        tnext[a] = __builtin_inff();
-      } else {
-        dist[a] = fmaxf(0.f, (tf - tn) / float(G));
-        tnext[a] = da > 0.f ? tn + float(cell[a] + 1) * dist[a] : tn + float(G - cell[a]) * dist[a];
-      }
-    }
-    float t0 = 0.f;
-    // a ray visits at most gx+gy+gz cells; the cap only guards degenerate
-    // (non-finite) rays
-    const int max_cells = v.gdims[0] + v.gdims[1] + v.gdims[2] + 3;
-    for (int it = 0; it < max_cells; ++it) {
-      const float tc = fminf(tnext[0], fminf(tnext[1], tnext[2]));
-      const float t1 = fminf(tc, tm);
-      const int li = (cell[2] * v.gdims[1] + cell[1]) * v.gdims[0] + cell[0];
-      const float maj = mb ? float(mb[li]) * mstep : v.majorants[li];
-      if (woodcock_cell(v, lds, maj, invUD, ori, dir, tmin + t0, tmin + t1, rs, t_hit, smp,
-                        lookups)) {
-        hit = true;
-        break;
-      }
-      bool out_of_grid = false;
-      if (tnext[0] == tc) {
-        tnext[0] += dist[0];
-        cell[0] += step[0];
-        out_of_grid = cell[0] == stop[0];
-      }
-      if (!out_of_grid && tnext[1] == tc) {
-        tnext[1] += dist[1];
-        cell[1] += step[1];
-        out_of_grid = cell[1] == stop[1];
-      }
-      if (!out_of_grid && tnext[2] == tc) {
-        tnext[2] += dist[2];
-        cell[2] += step[2];
-        out_of_grid = cell[2] == stop[2];
-      }
-      if (out_of_grid) break;
-      t0 = t1;
-    }
+  if (!trav_begin(v, ray, T)) return make_hit(false, 0.f, smp, 1.f);
+  while (!trav_step(v, lds, mb, mstep, T, rs, lookups, hit, t_hit, smp)) {
   }
-  if (hit) {
-    out.hit = true;
-    out.t = t_hit / dt_scale;
-    out.albedo = mk(smp.x, smp.y, smp.z);
-    out.extinction = smp.w;
-  }
-  return out;
+  return make_hit(hit, t_hit, smp, T.dt_scale);
 }
 
 // ---------------------------------------------------------------------------
@@ -585,18 +633,24 @@ struct PathState {
   int sampleID;
   float4 acc;
   float depth, sdepth;
-  KRay ray;
+  // current ray; tmin is 0 for every ray of a path, and shadow / AO rays
+  // start at the vertex's offset position hitPosOff, which the continuation
+  // ray of the bounce starts from as well
+  f3 ori, dir;
+  float tmax;
   int rayType;
-  unsigned passID, bounceID;
+  int bounceID;
+  bool first;  // the sample's first pass
+  bool firstHit;
   f3 throughput, intensity;
-  // ShadeState (Pathtrace_impl.cpp:15-35)
-  f3 baseColor, shadedColor, hitPosOff, sn, bs_dir, bs_f;
-  float bs_pdf, bs_cosT;
-  int light;  // sampled light, -1 = none yet
-  float misWeightNEE, vis_light, vis_ao;
+  // ShadeState (Pathtrace_impl.cpp:15-35) of a volume vertex.  Its BSDF
+  // sample is the isotropic phase function: f = baseColor = albedo,
+  // pdf = cosTheta = 1, so the throughput update is a product with albedo.
+  f3 albedo;
+  f3 direct;  // shadedColor * lightVisibility
+  f3 sn, bs_dir;
   int aoSamples;
   float aoWeights, aoCount;
-  bool firstHit;
   // radiance cache: query / training slot, training-path flag, radiance and
   // throughput at the second vertex
   int qslot, tslot;
@@ -652,45 +706,36 @@ __device__ inline void begin_sample(PathState &st, const KCamera &cam, const KPa
   const float jy = rnd(st.srng) - .5f;
   xf += jx;
   yf += jy;
-  st.ray = primary_ray(cam, xf, yf, float(W), float(H));
+  const KRay r = primary_ray(cam, xf, yf, float(W), float(H));
+  st.ori = r.ori;
+  st.dir = r.dir;
+  st.tmax = r.tmax;
   (void)rnd(st.srng);  // shutter time (DeviceCopyableObjects.h:2597)
   st.sdepth = 1e31f;
   st.rayType = Radiance;
-  st.passID = 0;
   st.bounceID = 0;
+  st.first = true;
+  st.firstHit = false;
   st.throughput = mk(1.f);
   st.intensity = mk(0.f);
-  st.baseColor = st.shadedColor = st.hitPosOff = st.sn = st.bs_dir = st.bs_f = mk(0.f);
-  st.bs_pdf = st.bs_cosT = 0.f;
-  st.light = -1;
-  st.misWeightNEE = 0.f;
-  st.vis_light = st.vis_ao = 1.f;
-  st.aoSamples = 0;
-  st.aoWeights = st.aoCount = 0.f;
-  st.firstHit = false;
   st.qslot = -1;
   st.tslot = -1;
 }
 
-// One pass of the bounce loop (Pathtrace_impl.cpp:558-600): intersect, shade,
-// advance the state machine.  Returns true when the sample's path ended.
+// One pass of the bounce loop (Pathtrace_impl.cpp:558-600) after the ray's
+// intersection `vh` (closest over the volumes): shade, advance the state
+// machine, set up the next ray.  Returns true when the sample's path ended.
 // With NRC, a path reaching its second volume vertex either becomes a cache
 // query and ends, or -- a training path -- records the vertex and goes on.
 template <bool NRC>
-__device__ inline bool path_pass(PathState &st, const KScene &sc, const KParams &P,
-                                 const float4 *s_tf, uint32_t &lookups, const KNrc &N) {
-  // intersectAll: volumes only in this scene class
-  VolHit vh;
-  vh.hit = false;
-  vh.t = FLT_MAX;
-  for (int i = 0; i < sc.num_volumes; ++i) {
-    VolHit h = delta_track(sc.vol[i], s_tf, lds_majorants(sc, s_tf, i),
-                           lds_steps(sc, const_cast<float4 *>(s_tf))[i], to_local(sc.vol[i], st.ray),
-                           st.srng, lookups);
-    if (h.hit && h.t < vh.t) vh = h;
-  }
+__device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams &P,
+                                  const KNrc &N, const VolHit &vh) {
+  KRay ray;
+  ray.ori = st.ori;
+  ray.dir = st.dir;
   const bool hit = vh.hit;
-  if (st.passID == 0 && st.bounceID == 0) {
+  if (st.first) {
+    st.first = false;
     st.firstHit = hit;
     if (hit) st.sdepth = vh.t;
   }
@@ -704,7 +749,7 @@ __device__ inline bool path_pass(PathState &st, const KScene &sc, const KParams 
     const uint32_t qslot = wave_claim(&N.ctr[0], want_query);
     if (v2) {
       float raw[6];
-      cache_inputs(N, st.ray.ori + st.ray.dir * vh.t, st.ray.dir, vh.extinction, raw);
+      cache_inputs(N, ray.ori + ray.dir * vh.t, ray.dir, vh.extinction, raw);
       if (is_train) {
         for (int i = 0; i < 6; ++i) N.t_raw[6 * size_t(tslot) + i] = raw[i];
         st.tslot = int(tslot);
@@ -723,113 +768,84 @@ __device__ inline bool path_pass(PathState &st, const KScene &sc, const KParams 
     }
   }
   // shade (Pathtrace_impl.cpp:93-487)
-  int next = Miss;
-  KRay nray = st.ray;
   const int rayType = st.rayType;
   if (rayType == Radiance) {
-    st.baseColor = mk(0.f);
-    st.vis_light = 1.f;
-    st.vis_ao = 1.f;
+    // a radiance miss ends the path; its shade state adds nothing
+    if (!hit) return true;
+    const f3 hitPos = ray.ori + ray.dir * vh.t;
+    const float eps = epsilon_from(hitPos, ray.dir, vh.t);
+    const f3 viewDir = -normalize(ray.dir);
+    f3 gn = mk(0.f);
+    if (P.ambient_samples > 0) {
+      const float u1 = rnd(st.srng), u2 = rnd(st.srng);
+      gn = uniform_sample_sphere(u1, u2);
+    }
+    st.sn = faceforward(gn, viewDir, gn);
+    f3 ls_int = mk(0.f), ls_dir = mk(0.f);
+    if (sc.num_lights > 0) {
+      int which = int(rnd(st.srng) * float(sc.num_lights));
+      if (which == sc.num_lights) which = 0;
+      // directional light sample: pdf 1, dist2 1 (renderer/common.h:1432-1466),
+      // shadedColor = albedo * E / dist2 / pdf / dist2
+      ls_int = sc.light[which].intensity;
+      ls_dir = sc.light[which].dir;
+    }
+    st.direct = vh.albedo * ls_int;
+    const float u1 = rnd(st.srng), u2 = rnd(st.srng);
+    st.bs_dir = uniform_sample_sphere(u1, u2);
+    st.albedo = vh.albedo;
     st.aoSamples = 0;
     st.aoWeights = 0.f;
     st.aoCount = 0.f;
-    if (!hit) {
-      st.shadedColor = mk(0.f);
-    } else {
-      const f3 hitPos = st.ray.ori + st.ray.dir * vh.t;
-      const float eps = epsilon_from(hitPos, st.ray.dir, vh.t);
-      const f3 viewDir = -normalize(st.ray.dir);
-      f3 gn = mk(0.f);
-      if (P.ambient_samples > 0) {
-        const float u1 = rnd(st.srng), u2 = rnd(st.srng);
-        gn = uniform_sample_sphere(u1, u2);
-      }
-      st.sn = faceforward(gn, viewDir, gn);
-      st.hitPosOff = hitPos + st.sn * eps;
-      if (sc.num_lights > 0) {
-        int which = int(rnd(st.srng) * float(sc.num_lights));
-        if (which == sc.num_lights) which = 0;
-        st.light = which;
-      }
-      // directional light sample: pdf 1, dist2 1 (renderer/common.h:1432-1466)
-      f3 ls_int = mk(0.f);
-      float ls_pdf = 0.f, ls_d2 = 0.f;
-      if (st.light >= 0) {
-        ls_int = sc.light[st.light].intensity;
-        ls_pdf = 1.f;
-        ls_d2 = 1.f;
-      }
-      const float rcp_pdf = ls_pdf > 0.f ? 1.f / ls_pdf : 0.f;
-      const float rcp_d2 = ls_d2 > 0.f ? 1.f / ls_d2 : 0.f;
-      st.shadedColor = vh.albedo * (ls_int * rcp_d2) * rcp_pdf * rcp_d2;
-      const float u1 = rnd(st.srng), u2 = rnd(st.srng);
-      st.bs_dir = uniform_sample_sphere(u1, u2);
-      st.bs_f = vh.albedo;
-      st.bs_pdf = 1.f;
-      st.bs_cosT = 1.f;
-      st.misWeightNEE = sc.num_lights > 0 ? 1.f : 0.f;
-      st.baseColor = vh.albedo;
-      // lightSample.pdf >= 0 always holds: shadow ray next (:403-406)
-      nray.ori = st.hitPosOff;
-      nray.dir = normalize(st.light >= 0 ? sc.light[st.light].dir : mk(0.f));
-      nray.tmin = 0.f;
-      nray.tmax = FLT_MAX;
-      next = Shadow;
-    }
-  } else if (rayType == Shadow) {
-    st.vis_light = hit ? 0.f : 1.f;
-    next = Radiance;
+    // lightSample.pdf >= 0 always holds: shadow ray next (:403-406)
+    st.ori = hitPos + st.sn * eps;  // hitPosOff
+    st.dir = normalize(ls_dir);
+    st.tmax = FLT_MAX;
+    st.rayType = Shadow;
+    return false;
+  }
+  float vis_ao = 1.f;
+  if (rayType == Shadow) {
+    st.direct = st.direct * (hit ? 0.f : 1.f);
   } else {  // AO
     st.aoSamples++;
-    const float weight = fmaxf(0.f, dot(st.ray.dir, st.sn));
+    const float weight = fmaxf(0.f, dot(ray.dir, st.sn));
     st.aoWeights += weight;
     if (weight > 0.f && hit) st.aoCount += weight;
-    if (st.aoSamples < P.ambient_samples) {
-      next = AO;
-    } else {
-      const float aoV = st.aoWeights > 0.f ? 1.f - st.aoCount / st.aoWeights : 0.f;
-      st.vis_ao *= aoV;
-      next = Radiance;
-    }
+    if (st.aoSamples >= P.ambient_samples)
+      vis_ao = st.aoWeights > 0.f ? 1.f - st.aoCount / st.aoWeights : 0.f;
   }
   // after the shadow pass (or an AO pass with samples left) an AO ray is
-  // traced while ambient samples remain (Pathtrace_impl.cpp:437-455)
-  if ((rayType == Shadow && st.aoSamples < P.ambient_samples) || (rayType == AO && next == AO)) {
+  // traced while ambient samples remain (Pathtrace_impl.cpp:437-455); the
+  // origin stays hitPosOff
+  if (st.aoSamples < P.ambient_samples) {
     const f3 w = st.sn;
     const f3 v = fabsf(w.x) > fabsf(w.y) ? normalize(mk(-w.z, 0.f, w.x))
                                          : normalize(mk(0.f, w.z, -w.y));
     const f3 u = cross(v, w);
     const float u1 = rnd(st.srng), u2 = rnd(st.srng);
     const f3 sp = cosine_sample_hemisphere(u1, u2);
-    nray.ori = st.hitPosOff;
-    nray.dir = normalize(u * sp.x + v * sp.y + w * sp.z);
-    nray.tmin = 0.f;
-    nray.tmax = P.occlusion_distance;
-    next = AO;
-  } else if (next == Radiance) {
-    nray.ori = st.hitPosOff;
-    nray.dir = normalize(st.bs_dir);
-    nray.tmin = 0.f;
-    nray.tmax = 1e31f;
+    st.dir = normalize(u * sp.x + v * sp.y + w * sp.z);
+    st.tmax = P.occlusion_distance;
+    st.rayType = AO;
+    return false;
   }
-  st.ray = nray;
-  st.rayType = next;
-  st.passID++;
-  if (next == Miss || next == Radiance) {
-    const f3 direct = st.shadedColor * st.vis_light;
-    const f3 ambient = st.baseColor * P.ambient_color * P.ambient_radiance * st.vis_ao;
-    st.intensity = st.intensity + st.throughput * (direct * st.misWeightNEE);
-    st.intensity = st.intensity + st.throughput * ambient;
-    const float rcp = st.bs_pdf > 0.f ? 1.f / st.bs_pdf : 0.f;
-    st.throughput = st.throughput * (st.bs_f * (st.bs_cosT * rcp));
-    st.bounceID++;
-  }
-  if (next == Miss || int(st.bounceID) > P.max_bounce) return true;
+  // the bounce ends: accumulate, continue along the phase-function sample
+  const float misWeightNEE = sc.num_lights > 0 ? 1.f : 0.f;
+  const f3 ambient = st.albedo * P.ambient_color * P.ambient_radiance * vis_ao;
+  st.intensity = st.intensity + st.throughput * (st.direct * misWeightNEE);
+  st.intensity = st.intensity + st.throughput * ambient;
+  st.throughput = st.throughput * st.albedo;
+  st.bounceID++;
+  if (st.bounceID > P.max_bounce) return true;
   if (st.bounceID > 1) {
     const float prob = max3(st.throughput);
     if (rnd(st.rng) > prob) return true;
     st.throughput = st.throughput * (1.f / prob);
   }
+  st.dir = normalize(st.bs_dir);
+  st.tmax = 1e31f;
+  st.rayType = Radiance;
   return false;
 }
 
@@ -865,6 +881,7 @@ __global__ void __launch_bounds__(256)
   const uint32_t nch = uint32_t(nchx * nchy);
   const uint32_t lane = threadIdx.x & 63u;
   const unsigned long long below = (1ull << lane) - 1ull;
+  const float *steps = lds_steps(sc, s_tf);
 
   PathState st;
   st.pixel = -1;
@@ -875,7 +892,7 @@ __global__ void __launch_bounds__(256)
   int px = 0, py = 0;
 
   while (true) {
-    // ---- refill idle lanes --------------------------------------------------
+    // ---- refill idle lanes ----------------------------------------------------
     unsigned long long idle = __ballot(st.pixel < 0);
     while (idle && qtries < kQueues) {
       if (pool_left == 0) {
@@ -895,19 +912,17 @@ __global__ void __launch_bounds__(256)
       const uint32_t nidle = uint32_t(__popcll(idle));
       const uint32_t take = min(nidle, pool_left);
       const uint32_t rank = uint32_t(__popcll(idle & below));
-      if (st.pixel < 0 && rank < take) {
-        chunk_pixel(q, pool_base + rank, nchx, px, py);
-        if (px < W && py < H) {
-          st.pixel = py * W + px;
-          st.rng = tea4(uint32_t(st.pixel), uint32_t(accumID) + P.rng_offset);
-          st.srng = st.rng;  // ScreenSample's copy (Pathtrace_impl.cpp:512-513)
-          st.sampleID = 0;
-          st.acc = make_float4(0.f, 0.f, 0.f, 0.f);
-          st.depth = 1e31f;
-          st.train_sel = NRC && train_hash(uint32_t(st.pixel), uint32_t(accumID) + P.rng_offset) <
-                                    p_train;
-          begin_sample(st, cam, P, px, py, W, H);
-        }
+      chunk_pixel(q, pool_base + rank, nchx, px, py);
+      if (st.pixel < 0 && rank < take && px < W && py < H) {
+        st.pixel = py * W + px;
+        st.rng = tea4(uint32_t(st.pixel), uint32_t(accumID) + P.rng_offset);
+        st.srng = st.rng;  // ScreenSample's copy (Pathtrace_impl.cpp:512-513)
+        st.sampleID = 0;
+        st.acc = make_float4(0.f, 0.f, 0.f, 0.f);
+        st.depth = 1e31f;
+        st.train_sel =
+            NRC && train_hash(uint32_t(st.pixel), uint32_t(accumID) + P.rng_offset) < p_train;
+        begin_sample(st, cam, P, px, py, W, H);
       }
       pool_base += take;
       pool_left -= take;
@@ -923,11 +938,23 @@ __global__ void __launch_bounds__(256)
 
     // ---- one pass for every active lane ---------------------------------------
     if (st.pixel >= 0) {
-      bool done = sc.num_volumes == 0 || path_pass<NRC>(st, sc, P, s_tf, lookups, N);
-      if (done) {
+      // intersectAll: the closest collision over the volumes
+      KRay ray;
+      ray.ori = st.ori;
+      ray.dir = st.dir;
+      ray.tmin = 0.f;
+      ray.tmax = st.tmax;
+      VolHit vh;
+      vh.hit = false;
+      vh.t = FLT_MAX;
+      for (int i = 0; i < sc.num_volumes; ++i) {
+        const VolHit h = delta_track(sc.vol[i], s_tf, lds_majorants(sc, s_tf, i), steps[i],
+                                     to_local(sc.vol[i], ray), st.srng, lookups);
+        if (h.hit && h.t < vh.t) vh = h;
+      }
+      if (path_shade<NRC>(st, sc, P, N, vh)) {
         float4 color = P.bg;
-        if (st.firstHit)
-          color = make_float4(st.intensity.x, st.intensity.y, st.intensity.z, 1.f);
+        if (st.firstHit) color = make_float4(st.intensity.x, st.intensity.y, st.intensity.z, 1.f);
         st.acc.x += color.x;
         st.acc.y += color.y;
         st.acc.z += color.z;
@@ -944,7 +971,7 @@ __global__ void __launch_bounds__(256)
             t[1] = st.T2.y > 0.f ? s.y / st.T2.y : 0.f;
             t[2] = st.T2.z > 0.f ? s.z / st.T2.z : 0.f;
           }
-          // one sample per pixel: the cache term is added by the resolve pass
+          // one sample per pixel: the resolve pass adds the cache term
           N.sample[st.pixel] = st.acc;
           N.query_of_pixel[st.pixel] = st.qslot;
           if (F.depth && st.depth <= F.depth[st.pixel]) F.depth[st.pixel] = st.depth;
@@ -1184,6 +1211,7 @@ __global__ void k_frame_reset(KFrame F) {
 // launchers
 // ---------------------------------------------------------------------------
 static inline int tiles(const KFrame &F) { return ((F.width + 15) / 16) * ((F.height + 15) / 16); }
+
 
 hipError_t launch_raycast(const KScene &sc, const KCamera &cam, const KParams &P, const KFrame &F,
                           hipStream_t s) {
