@@ -90,6 +90,8 @@ struct KParams {
   int pixel_samples;
   float sampling_rate_inv;
   uint32_t rng_offset;
+  int num_clip_planes;
+  float4 clip[VRT_MAX_CLIP_PLANES];
 };
 
 struct KFrame {

@@ -257,6 +257,22 @@ __device__ inline KRay primary_ray(const KCamera &c, float x, float y, float W, 
   return r;
 }
 
+// clipRay (renderer/common.h:1469-1482): every plane (N, D) keeps N.x <= D
+__device__ inline void clip_ray(KRay &r, const KParams &P) {
+  for (int i = 0; i < P.num_clip_planes; ++i) {
+    const float4 c = P.clip[i];
+    const f3 n = mk(c.x, c.y, c.z);
+    const float s = dot(n, r.dir);
+    if (s != 0.f) {
+      const float t = (c.w - dot(n, r.ori)) / s;
+      if (s < 0.f)
+        r.tmin = fmaxf(r.tmin, t);
+      else
+        r.tmax = fminf(r.tmax, t);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Woodcock / delta tracking through one volume (DeviceCopyableObjects.h:
 // 680-796) with the macrocell DDA of renderer/DDA.h:14-112.
@@ -552,13 +568,15 @@ __global__ void __launch_bounds__(256)
     float4 result = P.bg;
     if (sc.num_volumes > 0) {
       KRay ray = primary_ray(cam, float(x), float(y), float(F.width), float(F.height));
-      // closest volume box (intersectVolumeBounds, DeviceCopyableObjects.h:1841-1859)
+      clip_ray(ray, P);
+      // closest volume box (intersectVolumeBounds, DeviceCopyableObjects.h:1841-1859;
+      // a box hit counts when tfar >= tmin, :695-700)
       int best = -1;
       float bestT = FLT_MAX;
       for (int i = 0; i < sc.num_volumes; ++i) {
         KRay lr = to_local(sc.vol[i], ray);
         BoxHit bh = intersect_box(lr, sc.vol[i].blo, sc.vol[i].bhi);
-        if (!bh.hit) continue;
+        if (!bh.hit || !(bh.tfar >= lr.tmin)) continue;
         float t = fmaxf(lr.tmin, bh.tnear);
         if (t < bestT) {
           bestT = t;
@@ -944,6 +962,7 @@ __global__ void __launch_bounds__(256)
       ray.dir = st.dir;
       ray.tmin = 0.f;
       ray.tmax = st.tmax;
+      clip_ray(ray, P);  // every pass (Pathtrace_impl.cpp:559)
       VolHit vh;
       vh.hit = false;
       vh.t = FLT_MAX;

@@ -186,6 +186,15 @@ int make_params(const vrt_params *p, KParams &k) {
   // Raycast.cpp:27 safe_rcp
   k.sampling_rate_inv = p->volume_sampling_rate > 0.f ? 1.f / p->volume_sampling_rate : 0.f;
   k.rng_offset = p->rng_offset;
+  if (p->num_clip_planes < 0 || p->num_clip_planes > VRT_MAX_CLIP_PLANES)
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "num_clip_planes %d out of [0,%d]", p->num_clip_planes,
+                VRT_MAX_CLIP_PLANES);
+  k.num_clip_planes = p->num_clip_planes;
+  for (int i = 0; i < VRT_MAX_CLIP_PLANES; ++i) {
+    const float *c = p->clip_planes[i];
+    k.clip[i] = i < p->num_clip_planes ? make_float4(c[0], c[1], c[2], c[3])
+                                       : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
   return VRT_OK;
 }
 

@@ -75,6 +75,7 @@ class Params(C.Structure):
         ("occlusion_distance", C.c_float), ("ambient_samples", C.c_int),
         ("max_bounce", C.c_int), ("pixel_samples", C.c_int),
         ("volume_sampling_rate", C.c_float), ("rng_offset", C.c_uint32),
+        ("num_clip_planes", C.c_int), ("clip_planes", (C.c_float * 4) * 8),
     ]
 
 
@@ -368,11 +369,17 @@ def camera(position, direction, up, fovy, aspect, region=(0, 0, 1, 1), shutter=(
 
 def params(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.2,
            occlusion_distance=1e20, ambient_samples=1, max_bounce=7, pixel_samples=1,
-           volume_sampling_rate=0.5, rng_offset=0):
+           volume_sampling_rate=0.5, rng_offset=0, clip_planes=()):
     """Renderer parameters with the ANARI defaults of the reference
-    (Renderer.cpp:42-53, Pathtrace.cpp:19-25, Raycast.cpp:19)."""
+    (Renderer.cpp:42-53, Pathtrace.cpp:19-25, Raycast.cpp:19).  `clip_planes`:
+    up to 8 (nx, ny, nz, d); rays keep n.x <= d."""
     p = Params()
     p.rng_offset = rng_offset
+    if len(clip_planes) > 8:
+        raise ValueError("at most 8 clip planes")
+    p.num_clip_planes = len(clip_planes)
+    for i, c in enumerate(clip_planes):
+        p.clip_planes[i] = (C.c_float * 4)(*c)
     p.background = f4(*background)
     p.ambient_color = f3(*ambient_color)
     p.ambient_radiance = ambient_radiance

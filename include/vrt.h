@@ -96,6 +96,7 @@ typedef struct vrt_light {
 
 #define VRT_MAX_VOLUMES 4
 #define VRT_MAX_LIGHTS 8
+#define VRT_MAX_CLIP_PLANES 8
 
 typedef struct vrt_scene {
   int num_volumes;
@@ -132,6 +133,11 @@ typedef struct vrt_params {
      0 reproduces the reference (Pathtrace_impl.cpp:512), other values give
      accumulations that are independent of it */
   uint32_t rng_offset;
+  /* clip planes (rendererState.clipPlanes, Renderer.cpp:44-66): each ray keeps
+     the half-space N.x <= D of every plane (N, D) = clip_planes[i]
+     (clipRay, renderer/common.h:1469-1482) */
+  int num_clip_planes;
+  float clip_planes[VRT_MAX_CLIP_PLANES][4];
 } vrt_params;
 
 /* ANARI data types used for frame channels (ANARI 1.0 enum values). */

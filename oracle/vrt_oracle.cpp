@@ -575,6 +575,22 @@ HitRec<R> intersect_all(const Scene<R> &sc, const Ray<R> &ray, Random &rnd, Coun
   return hr;
 }
 
+// A4: clipRay (renderer/common.h:1469-1482)
+template <class R> void clip_ray(Ray<R> &ray, const orc_params *P) {
+  for (int i = 0; i < P->num_clip_planes; ++i) {
+    const float *c = P->clip_planes[i];
+    const V3<R> n{R(c[0]), R(c[1]), R(c[2])};
+    const R s = dot(n, ray.dir);
+    if (s != R(0)) {
+      const R t = (R(c[3]) - dot(n, ray.ori)) / s;
+      if (s < R(0))
+        ray.tmin = std::max(ray.tmin, t);
+      else
+        ray.tmax = std::min(ray.tmax, t);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // A16: path tracer (renderer/Pathtrace_impl.cpp)
 // ---------------------------------------------------------------------------
@@ -742,6 +758,7 @@ V4<R> pathtrace_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params
       RayType rayType = Radiance;
       V3<R> throughput(R(1)), intensity(R(0));
       for (unsigned passID = 0, bounceID = 0;; ++passID) {
+        clip_ray(ray, P);  // Pathtrace_impl.cpp:559
         HitRec<R> hitRec = intersect_all(sc, ray, srnd, cnt);
         shade(srnd, ray, rayType, sc, P, hitRec, st);
         if (passID == 0 && bounceID == 0) {
@@ -787,6 +804,7 @@ V4<R> raycast_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params *
   V4<R> bg{R(P->bg[0]), R(P->bg[1]), R(P->bg[2]), R(P->bg[3])};
   if (sc.empty()) return bg;
   Ray<R> ray = cam.primary(R(x), R(y), R(W), R(H));
+  clip_ray(ray, P);  // Raycast_impl.cpp:288
   // over(float4(0), bg) = bg (no surfaces), Raycast_impl.cpp:170-173
   V4<R> result = bg;
   // intersectVolumeBounds: closest volume box (DeviceCopyableObjects.h:695-709)
@@ -795,7 +813,7 @@ V4<R> raycast_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params *
   for (size_t i = 0; i < sc.vols.size(); ++i) {
     Ray<R> lr = sc.vols[i].to_local(ray);
     BoxHit<R> bh = intersect_box(lr, sc.vols[i].blo, sc.vols[i].bhi);
-    if (!bh.hit) continue;
+    if (!bh.hit || !(bh.tfar >= lr.tmin)) continue;  // :695-700
     R t = std::max(lr.tmin, bh.tnear);
     if (t < bestT) {
       bestT = t;

@@ -91,6 +91,9 @@ typedef struct orc_params {
   /* pixel streams seeded with (pixelID, accumID + rng_offset); 0 is the
      reference's seeding */
   uint32_t rng_offset;
+  /* clip planes (N, D): each ray keeps N.x <= D (renderer/common.h:1469-1482) */
+  int num_clip_planes;
+  float clip_planes[8][4];
 } orc_params;
 
 /* --- A1: LCG<4> / Random (common.h:55-102) --- */

@@ -47,7 +47,8 @@ class OrcParams(C.Structure):
     _fields_ = [("bg", C.c_float * 4), ("ambient_color", f3), ("ambient_radiance", C.c_float),
                 ("occlusion_distance", C.c_float), ("ambient_samples", C.c_int),
                 ("max_bounce", C.c_int), ("pixel_samples", C.c_int),
-                ("volume_sampling_rate", C.c_float), ("rng_offset", C.c_uint32)]
+                ("volume_sampling_rate", C.c_float), ("rng_offset", C.c_uint32),
+                ("num_clip_planes", C.c_int), ("clip_planes", (C.c_float * 4) * 8)]
 
 
 _lib = None
@@ -204,8 +205,11 @@ def camera(position, direction, up, fovy, aspect, region=(0, 0, 1, 1), shutter=(
 
 def params(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.2,
            occlusion_distance=1e20, ambient_samples=1, max_bounce=7, pixel_samples=1,
-           volume_sampling_rate=0.5, rng_offset=0):
+           volume_sampling_rate=0.5, rng_offset=0, clip_planes=()):
     p = OrcParams()
+    p.num_clip_planes = len(clip_planes)
+    for i, c in enumerate(clip_planes):
+        p.clip_planes[i] = (C.c_float * 4)(*c)
     p.bg = (C.c_float * 4)(*background)
     p.ambient_color = f3(*ambient_color)
     p.ambient_radiance = ambient_radiance

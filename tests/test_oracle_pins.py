@@ -336,3 +336,56 @@ def test_fp32_and_fp64_agree_statistically(pkg, oracle):
     b = ow.pathtrace_accumulate(64, fp64=True)
     err = np.mean((a[..., :3] - b[..., :3]) ** 2 / (b[..., :3] ** 2 + 1e-2))
     assert err <= 1e-3
+
+
+def test_clip_planes(pkg, oracle):
+    """clipRay (renderer/common.h:1469-1482) keeps n.x <= d on every ray.  A
+    plane that keeps the whole volume changes neither renderer's image, one
+    that keeps none of it leaves the background, and the plane z <= 0 through a
+    constant medium starts the march on the plane (closed form)."""
+    import dataclasses
+    wls = pkg.workloads
+    for wl in (wls.blobs_raycast(n=16, width=16, height=12),
+               wls.fbm_pathtrace(n=16, width=16, height=12, frames=1, brick=8)):
+        def render(planes):
+            w = dataclasses.replace(wl, params={**wl.params, "clip_planes": planes})
+            ow = oracle.OracleWorkload(w)
+            return ow.raycast() if wl.renderer == "raycast" else ow.pathtrace_frame(3)[0]
+        base = render(())
+        np.testing.assert_array_equal(render([(0, 0, 1, 10.0), (1, 0, 0, 9.0)]), base)
+        bg = np.broadcast_to(np.asarray(wl.params["background"], np.float32), base.shape)
+        np.testing.assert_array_equal(render([(0, 0, 1, -10.0)]), bg)
+        assert not np.array_equal(base, bg)
+
+    a, rgb, ud = 0.05, (0.9, 0.5, 0.1), 0.02
+    n = 9
+    vox = np.full((n, n, n), 0.5, np.float32)
+    tf = np.array([list(rgb) + [a]] * 4, np.float32)
+    vol = oracle.OracleVolume(vox, (-0.5,) * 3, (1 / (n - 1),) * 3, tf, (0, 1), ud)
+    scene = oracle.OracleScene([vol])
+    W = H = 8
+    cam = oracle.camera((0.1, 0.2, 3.0), (0, 0, -1), (0, 1, 0), math.radians(10), 1.0)
+    bg = (0.1, 0.2, 0.3, 1.0)
+    prm = oracle.params(background=bg, volume_sampling_rate=2.0, clip_planes=[(0, 0, 1, 0.0)])
+    out = np.empty((H, W, 4), np.float32)
+    oracle.lib().orc_raycast(C.byref(scene.s), C.byref(cam), C.byref(prm), W, H,
+                             out.ctypes.data, 1)
+    dt = np.float32(1 / (n - 1)) * np.float32(1 / 2.0)
+    for y in range(H):
+        for x in range(W):
+            o = np.zeros(3, np.float32)
+            dd = np.zeros(3, np.float32)
+            oracle.lib().orc_camera_ray(C.byref(cam), x, y, W, H, o.ctypes.data, dd.ctypes.data)
+            inv = 1 / dd.astype(np.float64)
+            ta, tb = (-0.5 - o) * inv, (0.5 - o) * inv
+            tf_ = np.maximum(ta, tb).min()
+            # the plane z = 0, reached at t = -o.z / d.z (the ray runs towards -z)
+            t, k = np.float32(-o[2]) / dd[2], 0
+            while t <= np.float32(tf_):
+                k += 1
+                t = np.float32(t + dt)
+            step_t = (1 - a) ** (float(dt) / ud)
+            k = min(k, math.ceil(math.log(0.01) / math.log(step_t)))
+            alpha = 1 - step_t ** k
+            exp = np.append(alpha * np.array(rgb) * alpha, alpha) + (1 - alpha) * np.array(bg)
+            np.testing.assert_allclose(out[y, x], exp, atol=2e-5)
