@@ -172,6 +172,14 @@ def gen():
         G.vrtgen_fbm.argtypes = [C.POINTER(C.c_int), C.c_int, C.c_double, C.c_double,
                                  C.c_double, C.c_uint64, C.c_void_p, C.c_int]
         G.vrtgen_splitmix_uniform.argtypes = [C.c_uint64, C.c_int, C.c_void_p]
+        G.vrtgen_nanovdb_from_dense.argtypes = [C.POINTER(C.c_int), C.c_void_p, C.c_float,
+                                                C.c_double, C.POINTER(C.c_double),
+                                                C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]
+        G.vrtgen_nanovdb_cloud.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                           C.c_uint64, C.c_double, C.c_int,
+                                           C.POINTER(C.c_void_p), C.POINTER(C.c_uint64),
+                                           C.POINTER(C.c_double)]
+        G.vrtgen_free.argtypes = [C.c_void_p]
         _gen = G
     return _gen
 
@@ -198,6 +206,44 @@ def gen_fbm(dims, octaves=6, base_freq=4.0, lacunarity=2.0, gain=0.5, seed=0, th
     gen().vrtgen_fbm(_dims(dims), octaves, base_freq, lacunarity, gain, seed, out.ctypes.data,
                      threads)
     return out
+
+
+def aligned_bytes(n, align=64):
+    """A uint8 array of n bytes whose data pointer is `align`-byte aligned
+    (NanoVDB buffers need 32-byte alignment)."""
+    raw = np.empty(int(n) + align, np.uint8)
+    off = (-raw.ctypes.data) % align
+    return raw[off:off + int(n)]
+
+
+def _take_gen_buffer(ptr, nbytes):
+    out = aligned_bytes(nbytes.value)
+    C.memmove(out.ctypes.data, ptr.value, nbytes.value)
+    gen().vrtgen_free(ptr)
+    return out
+
+
+def gen_nanovdb_from_dense(values, background=0.0, voxel=1.0, trans=(0.0, 0.0, 0.0)):
+    """NanoVDB float grid of a dense (z, y, x) array: voxels != background are
+    active; world = voxel * index + trans."""
+    v = np.ascontiguousarray(values, np.float32)
+    ptr, n = C.c_void_p(), C.c_uint64()
+    dims = (v.shape[2], v.shape[1], v.shape[0])
+    if gen().vrtgen_nanovdb_from_dense(_dims(dims), v.ctypes.data, background, voxel,
+                                       (C.c_double * 3)(*trans), C.byref(ptr), C.byref(n)):
+        raise VrtError("nanovdb writer failed")
+    return _take_gen_buffer(ptr, n)
+
+
+def gen_nanovdb_cloud(n=1024, octaves=6, base_freq=4.0, lacunarity=2.0, gain=0.5, seed=3,
+                      active_fraction=0.05, threads=0):
+    """Sparse fBm cloud as a NanoVDB float grid over index space [0, n)^3
+    (identity map).  Returns (buffer, threshold)."""
+    ptr, nb, tau = C.c_void_p(), C.c_uint64(), C.c_double()
+    if gen().vrtgen_nanovdb_cloud(n, octaves, base_freq, lacunarity, gain, seed, active_fraction,
+                                  threads, C.byref(ptr), C.byref(nb), C.byref(tau)):
+        raise VrtError("nanovdb cloud generator failed")
+    return _take_gen_buffer(ptr, nb), tau.value
 
 
 class DeviceBuffer:

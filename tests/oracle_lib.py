@@ -10,6 +10,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(ROOT, "oracle", "_build", "libvrt_oracle.so")
+REF_NVDB_PATH = os.path.join(ROOT, "oracle", "_ref", "libnvdb_ref.so")
 
 f3 = C.c_float * 3
 
@@ -85,8 +86,52 @@ def lib():
         L.orc_cvt_uint32_srgb.restype = C.c_uint32
         L.orc_cvt_uint32_srgb.argtypes = [vp]
         L.orc_camera_ray.argtypes = [C.POINTER(OrcCamera), C.c_float, C.c_float, ip, ip, vp, vp]
+        L.orc_nvdb_get.argtypes = [vp, ip, vp, vp]
+        L.orc_nvdb_sample.argtypes = [vp, ip, vp, ip, vp]
+        L.orc_nvdb_world_bbox.argtypes = [vp, vp]
         _lib = L
     return _lib
+
+
+_ref_nvdb = None
+
+
+def ref_nvdb():
+    """The reference's vendored NanoVDB compiled into oracle/_ref (None when it
+    has not been built: the reference tree is absent)."""
+    global _ref_nvdb
+    if _ref_nvdb is None and os.path.exists(REF_NVDB_PATH):
+        R = C.CDLL(REF_NVDB_PATH)
+        vp, ip = C.c_void_p, C.c_int
+        R.ref_nvdb_check.argtypes = [vp, C.c_char_p, ip]
+        R.ref_nvdb_get.argtypes = [vp, ip, vp, vp]
+        R.ref_nvdb_sample.argtypes = [vp, ip, vp, ip, vp]
+        R.ref_nvdb_world_bbox.argtypes = [vp, vp]
+        R.ref_nvdb_make_tiled.argtypes = [C.c_uint64, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]
+        R.ref_nvdb_free.argtypes = [vp]
+        _ref_nvdb = R
+    return _ref_nvdb
+
+
+def nvdb_get(buf, ijk):
+    ijk = np.ascontiguousarray(ijk, np.int32)
+    out = np.empty(len(ijk), np.float32)
+    assert lib().orc_nvdb_get(buf.ctypes.data, len(ijk), ijk.ctypes.data, out.ctypes.data) == 0
+    return out
+
+
+def nvdb_sample(buf, xyz, order=1):
+    xyz = np.ascontiguousarray(xyz, np.float32)
+    out = np.empty(len(xyz), np.float32)
+    assert lib().orc_nvdb_sample(buf.ctypes.data, len(xyz), xyz.ctypes.data, order,
+                                 out.ctypes.data) == 0
+    return out
+
+
+def nvdb_world_bbox(buf):
+    out = np.empty(6, np.float64)
+    assert lib().orc_nvdb_world_bbox(buf.ctypes.data, out.ctypes.data) == 0
+    return out
 
 
 def lcg_stream(s0, s1, n):
