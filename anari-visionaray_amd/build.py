@@ -29,7 +29,8 @@ HIP_FLAGS = [
     "-I", INCLUDE, "-I", CSRC,
 ]
 
-HIP_SOURCES = ["render_kernels.hip", "nrc_kernels.hip", "vrt_api.cpp", "nrc_api.cpp"]
+HIP_SOURCES = ["render_kernels.hip", "nrc_kernels.hip", "vrt_api.cpp", "nrc_api.cpp",
+               "nanovdb_field.cpp"]
 
 
 def _run(cmd):

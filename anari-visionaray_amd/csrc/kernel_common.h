@@ -42,7 +42,6 @@ struct KVolume {
   f3 S, T;             // voxel space = S * (p + T)
   float cell_size;
   const float *bricks;
-  const void *vdb;
   const float4 *tf;    // global copy of the table
   int tf_size;
   int tf_lds_offset;   // offset of this table in the block's LDS copy
@@ -55,6 +54,18 @@ struct KVolume {
   int maj_lds_offset;     // byte offset of this grid in the workgroup's LDS, -1: global
   float A[9];          // affineInv
   f3 tinv;             // transInv
+  f3 glo, ghi;         // majorant grid bounds in voxel space
+  // nanovdb: top table over 128^3 regions (>= 0 block index, < 0: -1 -
+  // index into nv_consts), blocks of 16^3 slot entries (>= 0 leaf index
+  // into `bricks`, else a constant), top origin (in 128-voxel units) / dims,
+  // world -> index map, filter
+  const int32_t *nv_top;
+  const int32_t *nv_blocks;
+  const float *nv_consts;
+  const float *nv_slot_ranges;
+  int nv_lo[3], nv_n[3];
+  float nv_inv[9], nv_vec[3];
+  int filter;
 };
 
 struct KLight {

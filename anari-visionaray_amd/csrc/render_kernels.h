@@ -24,6 +24,13 @@ hipError_t launch_relmse(const float4 *x, const double *ref_sum, double ref_scal
 int pathtrace_blocks_per_cu(size_t lds_bytes, bool nrc = false);
 hipError_t launch_brickify(const float *lin, const int dims[3], float *bricks, hipStream_t s);
 hipError_t launch_value_ranges(const KVolume &v, float *ranges, hipStream_t s);
+// nanovdb: min/max (float2) of each block slot's values
+hipError_t launch_nanovdb_slot_ranges(const int32_t *slots, size_t nslots, const float *leaves,
+                                      const float *consts, float *slot_ranges, hipStream_t s);
+// nanovdb: cell value ranges from the cells' voxel boxes (6 ints per cell:
+// lo xyz, hi xyz inclusive) over the slot ranges
+hipError_t launch_nanovdb_value_ranges(const KVolume &v, const int32_t *cell_voxels,
+                                       float *ranges, hipStream_t s);
 hipError_t launch_majorants(const KVolume &v, const float *ranges, float *maj, uint8_t *bytes,
                             float *step, hipStream_t s);
 // LDS bytes a render kernel needs for the scene's tables and majorant grids

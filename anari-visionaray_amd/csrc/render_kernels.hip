@@ -192,6 +192,60 @@ __device__ inline float sample_structured(const KVolume &v, f3 P) {
   return lerpf(lerpf(c00, c10, wy), lerpf(c01, c11, wy), wz);
 }
 
+// ---------------------------------------------------------------------------
+// NanoVDB field fetch (sampleField, DeviceCopyableObjects.h:554-569) on the
+// two-level layout of nanovdb_field.cpp: NanoGrid<float>::getValue is a top
+// read (128^3 region: block or constant), a block read (8^3 slot: leaf or
+// constant) and, in a leaf, one value read.
+// ---------------------------------------------------------------------------
+// entry of the 8^3 slot (sx, sy, sz) (voxel index >> 3): >= 0 leaf index,
+// else -1 - constant index
+__device__ inline int nvdb_slot(const KVolume &v, int sx, int sy, int sz) {
+  const int tx = (sx >> 4) - v.nv_lo[0], ty = (sy >> 4) - v.nv_lo[1], tz = (sz >> 4) - v.nv_lo[2];
+  if (uint32_t(tx) >= uint32_t(v.nv_n[0]) || uint32_t(ty) >= uint32_t(v.nv_n[1]) ||
+      uint32_t(tz) >= uint32_t(v.nv_n[2]))
+    return -1;  // background
+  const int t = v.nv_top[(size_t(tz) * size_t(v.nv_n[1]) + size_t(ty)) * size_t(v.nv_n[0]) + tx];
+  if (t < 0) return t;
+  return v.nv_blocks[size_t(t) * 4096u + uint32_t((((sz & 15) << 4) | (sy & 15)) << 4 | (sx & 15))];
+}
+
+__device__ inline float nvdb_value(const KVolume &v, int i, int j, int k) {
+  const int e = nvdb_slot(v, i >> 3, j >> 3, k >> 3);
+  return e >= 0 ? v.bricks[size_t(e) * 512u + uint32_t(((i & 7) << 6) | ((j & 7) << 3) | (k & 7))]
+                : v.nv_consts[-1 - e];
+}
+
+// createSampler<1|0>(acc)(worldToIndexF(P)): the map is an fmaf chain over
+// P - translation (Map::applyInverseMapF); the trilinear sampler lerps over
+// z, then y, then x with lerp(a, b, w) = a + w (b - a) and weights
+// idx - floor(idx) (math/SampleFromVoxels.h:174-290, 324-430); the nearest
+// sampler reads the voxel at rint(idx)
+__device__ inline float sample_nanovdb(const KVolume &v, f3 P) {
+  const float x = P.x - v.nv_vec[0], y = P.y - v.nv_vec[1], z = P.z - v.nv_vec[2];
+  float idx[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    idx[a] = __builtin_fmaf(x, v.nv_inv[3 * a],
+                            __builtin_fmaf(y, v.nv_inv[3 * a + 1], z * v.nv_inv[3 * a + 2]));
+  if (v.filter == VRT_FILTER_NEAREST)
+    return nvdb_value(v, int(__builtin_rintf(idx[0])), int(__builtin_rintf(idx[1])),
+                      int(__builtin_rintf(idx[2])));
+  const float fx = floorf(idx[0]), fy = floorf(idx[1]), fz = floorf(idx[2]);
+  const float wx = idx[0] - fx, wy = idx[1] - fy, wz = idx[2] - fz;
+  const int i = int(fx), j = int(fy), k = int(fz);
+  const float v00 = lerpf(nvdb_value(v, i, j, k), nvdb_value(v, i, j, k + 1), wz);
+  const float v01 = lerpf(nvdb_value(v, i, j + 1, k), nvdb_value(v, i, j + 1, k + 1), wz);
+  const float v10 = lerpf(nvdb_value(v, i + 1, j, k), nvdb_value(v, i + 1, j, k + 1), wz);
+  const float v11 = lerpf(nvdb_value(v, i + 1, j + 1, k), nvdb_value(v, i + 1, j + 1, k + 1), wz);
+  return lerpf(lerpf(v00, v01, wy), lerpf(v10, v11, wy), wx);
+}
+
+// sampleField: voxel-space point -> field value
+__device__ inline float sample_field(const KVolume &v, f3 P) {
+  return v.field_type == VRT_FIELD_NANOVDB ? sample_nanovdb(v, P) : sample_structured(v, P);
+}
+
 // postClassify (DeviceCopyableObjects.h:615-619): tex1D(linear, clamp) of the
 // TF table, read from the workgroup's LDS copy
 __device__ inline float4 classify(const KVolume &v, const float4 *lds, float val) {
@@ -332,14 +386,16 @@ __device__ inline bool trav_begin(const KVolume &v, KRay ray, Trav &T) {
   const f3 o = ori + dir * T.tmin;
   T.tm = tmax - T.tmin;
   const f3 rcp = mk(rcp_dir(dir.x), rcp_dir(dir.y), rcp_dir(dir.z));
-  // grid bounds = voxel space [0,1]^3
-  const f3 l = (mk(0.f) - o) * rcp;
-  const f3 h = (mk(1.f) - o) * rcp;
+  // grid bounds in voxel space: [0,1]^3 for structured fields, the world
+  // bounds for nanovdb (identity voxel transform)
+  const f3 l = (v.glo - o) * rcp;
+  const f3 h = (v.ghi - o) * rcp;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const int G = v.gdims[a];
     const float oa = comp(o, a), da = comp(dir, a);
-    float c = oa * float(G);  // projectOnGrid over [0,1]
+    const float ga = comp(v.glo, a);
+    float c = (oa - ga) / (comp(v.ghi, a) - ga) * float(G);  // projectOnGrid
     c = fminf(fmaxf(c, -1.f), float(G));
     T.cell[a] = min(max(int(c), 0), G - 1);
     const float la = comp(l, a), ha = comp(h, a);
@@ -402,7 +458,7 @@ __device__ inline bool trav_step(const KVolume &v, const float4 *lds, const uint
       T.in_cell = false;
       advance = true;
     } else {
-      const float val = sample_structured(v, T.ori + T.dir * T.t);
+      const float val = sample_field(v, T.ori + T.dir * T.t);
       ++lookups;
       const float4 s = classify(v, lds, val);
       if (s.w >= rnd(rs) * T.maj) {
@@ -602,7 +658,7 @@ __global__ void __launch_bounds__(256)
         int step = 0;
         for (float t = tmin; t <= tmax && alpha < 0.99f && step < max_steps; t += dt, ++step) {
           const f3 p = ori + dir * t;
-          const float val = sample_structured(v, p);
+          const float val = sample_field(v, p);
           ++lookups;
           const float4 s = classify(v, s_tf, val);
           const float stepT = pow_ieee(1.f - s.w, expo);
@@ -1131,6 +1187,64 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x == 0) ranges[mc] = make_float2(smin[0], smax[0]);
 }
 
+// nanovdb: value range of every block slot (a leaf's 512 values, or its
+// constant)
+__global__ void __launch_bounds__(256)
+    k_nvdb_slot_ranges(const int32_t *__restrict__ table, size_t nslots,
+                       const float *__restrict__ leaves, const float *__restrict__ consts,
+                       float2 *__restrict__ out) {
+  const size_t s = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= nslots) return;
+  const int e = table[s];
+  if (e < 0) {
+    const float c = consts[-1 - e];
+    out[s] = make_float2(c, c);
+    return;
+  }
+  const float4 *l = reinterpret_cast<const float4 *>(leaves + size_t(e) * 512u);
+  float mn = FLT_MAX, mx = -FLT_MAX;
+  for (int i = 0; i < 128; ++i) {
+    const float4 q = l[i];
+    mn = fminf(mn, fminf(fminf(q.x, q.y), fminf(q.z, q.w)));
+    mx = fmaxf(mx, fmaxf(fmaxf(q.x, q.y), fmaxf(q.z, q.w)));
+  }
+  out[s] = make_float2(mn, mx);
+}
+
+// nanovdb: a macrocell's value range = union of the ranges of the 8^3 slots
+// its voxel box touches (constant slots: their constant)
+__global__ void __launch_bounds__(256)
+    k_nvdb_value_ranges(KVolume v, int ncells, const int32_t *__restrict__ cell_voxels,
+                        float2 *__restrict__ ranges) {
+  const int c = int(blockIdx.x * blockDim.x + threadIdx.x);
+  if (c >= ncells) return;
+  const int32_t *b = cell_voxels + 6 * size_t(c);
+  const float2 *slot_ranges = reinterpret_cast<const float2 *>(v.nv_slot_ranges);
+  float mn = FLT_MAX, mx = -FLT_MAX;
+  for (int sz = b[2] >> 3; sz <= (b[5] >> 3); ++sz)
+    for (int sy = b[1] >> 3; sy <= (b[4] >> 3); ++sy)
+      for (int sx = b[0] >> 3; sx <= (b[3] >> 3); ++sx) {
+        const int tx = (sx >> 4) - v.nv_lo[0], ty = (sy >> 4) - v.nv_lo[1],
+                  tz = (sz >> 4) - v.nv_lo[2];
+        float lo, hi;
+        int t = -1;
+        if (uint32_t(tx) < uint32_t(v.nv_n[0]) && uint32_t(ty) < uint32_t(v.nv_n[1]) &&
+            uint32_t(tz) < uint32_t(v.nv_n[2]))
+          t = v.nv_top[(size_t(tz) * v.nv_n[1] + ty) * v.nv_n[0] + tx];
+        if (t < 0) {
+          lo = hi = v.nv_consts[-1 - t];
+        } else {
+          const float2 r =
+              slot_ranges[size_t(t) * 4096u + uint32_t((((sz & 15) << 4) | (sy & 15)) << 4 | (sx & 15))];
+          lo = r.x;
+          hi = r.y;
+        }
+        mn = fminf(mn, lo);
+        mx = fmaxf(mx, hi);
+      }
+  ranges[c] = make_float2(mn, mx);
+}
+
 // GridAccel::computeMaxOpacities (GridAccel.cpp:69-107) with the TF taps
 // that tex1D can actually read for values in [vmin, vmax]
 // The stored majorant is rounded up to a multiple q*step, q in 0..255, with
@@ -1180,7 +1294,7 @@ __global__ void k_sample_field(KVolume v, int n, const float *__restrict__ pts,
                                float *__restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  out[i] = sample_structured(v, mk(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
+  out[i] = sample_field(v, mk(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
 }
 
 __global__ void __launch_bounds__(256)
@@ -1288,6 +1402,23 @@ hipError_t launch_value_ranges(const KVolume &v, float *ranges, hipStream_t s) {
 }
 
 size_t scene_lds_size(const KScene &sc) { return scene_lds_bytes(sc); }
+
+hipError_t launch_nanovdb_slot_ranges(const int32_t *table, size_t nslots, const float *leaves,
+                                      const float *consts, float *slot_ranges, hipStream_t s) {
+  if (nslots == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_nvdb_slot_ranges, dim3(unsigned((nslots + 255) / 256)), dim3(256), 0, s,
+                     table, nslots, leaves, consts, reinterpret_cast<float2 *>(slot_ranges));
+  return hipGetLastError();
+}
+
+hipError_t launch_nanovdb_value_ranges(const KVolume &v, const int32_t *cell_voxels,
+                                       float *ranges, hipStream_t s) {
+  const int n = v.gdims[0] * v.gdims[1] * v.gdims[2];
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_nvdb_value_ranges, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, v, n,
+                     cell_voxels, reinterpret_cast<float2 *>(ranges));
+  return hipGetLastError();
+}
 
 hipError_t launch_majorants(const KVolume &v, const float *ranges, float *maj, uint8_t *bytes,
                             float *step, hipStream_t s) {

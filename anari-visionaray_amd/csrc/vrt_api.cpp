@@ -8,8 +8,10 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "abi_internal.h"
+#include "nanovdb_internal.h"
 #include "nrc_internal.h"
 #include "render_kernels.h"
 #include "vrt.h"
@@ -45,7 +47,64 @@ namespace {
 
 using namespace vrt;
 
+// shared by both field types: transfer function, majorants, instance
+int volume_common(const vrt_volume &in, int lds_offset, KVolume &v) {
+  if (!in.tf_rgba || in.tf_size < 1)
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "volume has no transfer function table");
+  if (!(in.unit_distance > 0.f))
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "unit_distance must be > 0");
+  if (in.grid_dims[0] > 0 && (!in.majorants || in.grid_dims[1] < 1 || in.grid_dims[2] < 1))
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "majorant grid incomplete");
+  if (in.filter != VRT_FILTER_LINEAR && in.filter != VRT_FILTER_NEAREST)
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "unknown filter %d", in.filter);
+  v.tf = reinterpret_cast<const float4 *>(in.tf_rgba);
+  v.tf_size = in.tf_size;
+  v.tf_lds_offset = lds_offset;
+  v.vlo = in.value_lo;
+  v.vhi = in.value_hi;
+  v.unit_distance = in.unit_distance;
+  v.majorants = in.majorants;
+  for (int a = 0; a < 3; ++a) v.gdims[a] = in.grid_dims[0] > 0 ? in.grid_dims[a] : 0;
+  v.mbytes = in.majorant_bytes;
+  v.mstep = in.majorant_step;
+  v.maj_lds_offset = -1;
+  for (int i = 0; i < 9; ++i) v.A[i] = in.affine_inv[i];
+  v.tinv = mk(in.trans_inv[0], in.trans_inv[1], in.trans_inv[2]);
+  v.filter = in.filter;
+  return VRT_OK;
+}
+
+// NanoVDBField (NanoVDBField.cpp:30-65): bounds = the grid's world bbox as
+// float, identity voxel-space transform, cell size 1; the majorant grid
+// spans the bounds
+int make_nanovdb_volume(const vrt_volume &in, int lds_offset, KVolume &v) {
+  const vrt_nanovdb_s *g = in.nanovdb;
+  if (!g) return fail(VRT_ERROR_INVALID_ARGUMENT, "nanovdb volume has no grid");
+  v = KVolume{};
+  v.field_type = VRT_FIELD_NANOVDB;
+  v.blo = mk(g->blo[0], g->blo[1], g->blo[2]);
+  v.bhi = mk(g->bhi[0], g->bhi[1], g->bhi[2]);
+  v.S = mk(1.f);
+  v.T = mk(0.f);
+  v.glo = v.blo;
+  v.ghi = v.bhi;
+  v.cell_size = 1.f;
+  v.bricks = g->leaves;
+  v.nv_top = g->top;
+  v.nv_blocks = g->blocks;
+  v.nv_consts = g->consts;
+  v.nv_slot_ranges = g->slot_ranges;
+  for (int a = 0; a < 3; ++a) {
+    v.nv_lo[a] = g->lo[a];
+    v.nv_n[a] = g->n[a];
+    v.nv_vec[a] = g->vec_f[a];
+  }
+  for (int i = 0; i < 9; ++i) v.nv_inv[i] = g->inv_f[i];
+  return volume_common(in, lds_offset, v);
+}
+
 int make_volume(const vrt_volume &in, int lds_offset, KVolume &v) {
+  if (in.field_type == VRT_FIELD_NANOVDB) return make_nanovdb_volume(in, lds_offset, v);
   if (in.field_type != VRT_FIELD_STRUCTURED_REGULAR)
     return fail(VRT_ERROR_UNSUPPORTED, "field type %d not supported by this entry point",
                 in.field_type);
@@ -53,12 +112,6 @@ int make_volume(const vrt_volume &in, int lds_offset, KVolume &v) {
     if (in.dims[a] < 1 || in.dims[a] > 4096)
       return fail(VRT_ERROR_INVALID_ARGUMENT, "volume dims[%d]=%d out of range", a, in.dims[a]);
   if (!in.bricks) return fail(VRT_ERROR_INVALID_ARGUMENT, "volume has no voxel bricks");
-  if (!in.tf_rgba || in.tf_size < 1)
-    return fail(VRT_ERROR_INVALID_ARGUMENT, "volume has no transfer function table");
-  if (!(in.unit_distance > 0.f))
-    return fail(VRT_ERROR_INVALID_ARGUMENT, "unit_distance must be > 0");
-  if (in.grid_dims[0] > 0 && (!in.majorants || in.grid_dims[1] < 1 || in.grid_dims[2] < 1))
-    return fail(VRT_ERROR_INVALID_ARGUMENT, "majorant grid incomplete");
   const size_t nb = size_t((in.dims[0] + 7) / 8) * ((in.dims[1] + 7) / 8) * ((in.dims[2] + 7) / 8);
   if (nb * 512 > 0xFFFFFFFFull)
     return fail(VRT_ERROR_UNSUPPORTED, "volume exceeds 32-bit brick addressing");
@@ -76,21 +129,9 @@ int make_volume(const vrt_volume &in, int lds_offset, KVolume &v) {
   v.T = -v.blo;
   v.cell_size = in.cell_size;
   v.bricks = in.bricks;
-  v.vdb = in.nanovdb_grid;
-  v.tf = reinterpret_cast<const float4 *>(in.tf_rgba);
-  v.tf_size = in.tf_size;
-  v.tf_lds_offset = lds_offset;
-  v.vlo = in.value_lo;
-  v.vhi = in.value_hi;
-  v.unit_distance = in.unit_distance;
-  v.majorants = in.majorants;
-  for (int a = 0; a < 3; ++a) v.gdims[a] = in.grid_dims[0] > 0 ? in.grid_dims[a] : 0;
-  v.mbytes = in.majorant_bytes;
-  v.mstep = in.majorant_step;
-  v.maj_lds_offset = -1;
-  for (int i = 0; i < 9; ++i) v.A[i] = in.affine_inv[i];
-  v.tinv = mk(in.trans_inv[0], in.trans_inv[1], in.trans_inv[2]);
-  return VRT_OK;
+  v.glo = mk(0.f);
+  v.ghi = mk(1.f);
+  return volume_common(in, lds_offset, v);
 }
 
 constexpr int kMaxLdsTf = 4096;           // float4 entries (64 KiB)
@@ -371,6 +412,29 @@ int vrt_build_value_ranges(vrt_context ctx, const vrt_volume *vol, float *ranges
   }
   if (!(tmp.unit_distance > 0.f)) tmp.unit_distance = 1.f;
   if (int r = make_volume(tmp, 0, v)) return r;
+  if (v.field_type == VRT_FIELD_NANOVDB) {
+    // each cell's voxel box on the host (double), then a union of slot ranges
+    const int *G = tmp.grid_dims;
+    const size_t n = size_t(G[0]) * G[1] * G[2];
+    std::vector<int32_t> boxes(6 * n);
+    const float glo[3] = {v.glo.x, v.glo.y, v.glo.z}, ghi[3] = {v.ghi.x, v.ghi.y, v.ghi.z};
+    for (int z = 0; z < G[2]; ++z)
+      for (int y = 0; y < G[1]; ++y)
+        for (int x = 0; x < G[0]; ++x)
+          nanovdb_cell_voxels(*tmp.nanovdb, glo, ghi, G, x, y, z,
+                              &boxes[6 * ((size_t(z) * G[1] + y) * G[0] + x)],
+                              &boxes[6 * ((size_t(z) * G[1] + y) * G[0] + x) + 3]);
+    int32_t *d = nullptr;
+    VRT_HIP(hipMalloc(&d, boxes.size() * 4));
+    hipError_t e = hipMemcpyAsync(d, boxes.data(), boxes.size() * 4, hipMemcpyHostToDevice,
+                                  ctx->stream);
+    if (e == hipSuccess)
+      e = launch_nanovdb_value_ranges(v, d, ranges, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    VRT_HIP(e);
+    return VRT_OK;
+  }
   VRT_HIP(launch_value_ranges(v, ranges, ctx->stream));
   return VRT_OK;
 }

@@ -40,7 +40,8 @@ i3 = C.c_int * 3
 class Volume(C.Structure):
     _fields_ = [
         ("field_type", C.c_int), ("dims", i3), ("bounds_lo", f3), ("bounds_hi", f3),
-        ("cell_size", C.c_float), ("bricks", C.c_void_p), ("nanovdb_grid", C.c_void_p),
+        ("cell_size", C.c_float), ("bricks", C.c_void_p), ("nanovdb", C.c_void_p),
+        ("filter", C.c_int),
         ("tf_rgba", C.c_void_p), ("tf_size", C.c_int), ("value_lo", C.c_float),
         ("value_hi", C.c_float), ("unit_distance", C.c_float), ("majorants", C.c_void_p),
         ("grid_dims", i3), ("majorant_bytes", C.c_void_p), ("majorant_step", C.c_void_p),
@@ -152,6 +153,9 @@ def lib():
         L.vrt_relmse.argtypes = [vp, vp, vp, C.c_double, ip, C.c_float,
                                  C.POINTER(C.c_double)]
         L.vrt_sample_field.argtypes = [vp, C.POINTER(Volume), ip, vp, vp]
+        L.vrt_nanovdb_create.argtypes = [vp, vp, C.c_uint64, C.POINTER(C.c_void_p)]
+        L.vrt_nanovdb_get_info.argtypes = [vp, C.c_void_p]
+        L.vrt_nanovdb_destroy.argtypes = [vp]
         L.vrt_delta_track.argtypes = [vp, C.POINTER(Scene), ip, vp, vp, vp]
         L.vrt_raycast.argtypes = [vp, C.POINTER(Scene), C.POINTER(Camera), C.POINTER(Params),
                                   C.POINTER(Frame), ip]
@@ -315,9 +319,57 @@ class Context:
             self.handle = None
 
 
+class NanoVDBInfo(C.Structure):
+    _fields_ = [("bounds_lo", f3), ("bounds_hi", f3), ("background", C.c_float),
+                ("top_origin", i3), ("top_dims", i3), ("blocks", C.c_uint64),
+                ("leaves", C.c_uint64),
+                ("constants", C.c_uint64), ("active_voxels", C.c_uint64),
+                ("device_bytes", C.c_uint64)]
+
+
+class DeviceNanoVDB:
+    """A NanoVDB float grid converted to the GPU slot-table layout."""
+
+    def __init__(self, ctx, grid_buffer):
+        buf = np.ascontiguousarray(grid_buffer, np.uint8)
+        h = C.c_void_p()
+        check(lib().vrt_nanovdb_create(ctx.handle, buf.ctypes.data, buf.nbytes, C.byref(h)))
+        self.handle = h.value
+        self.info = NanoVDBInfo()
+        check(lib().vrt_nanovdb_get_info(self.handle, C.byref(self.info)))
+
+    def close(self):
+        if self.handle:
+            lib().vrt_nanovdb_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class DeviceVolume:
-    """A structuredRegular field + transferFunction1D volume resident on the GPU
-    (brick-ordered voxels, TF table, value ranges and majorant grid)."""
+    """A structuredRegular or nanovdb field + transferFunction1D volume resident
+    on the GPU (field data, TF table, value ranges and majorant grid)."""
+
+    @classmethod
+    def from_nanovdb(cls, ctx, grid, tf_rgba, value_range, unit_distance, grid_dims=None,
+                     xfm=None, filter="linear"):
+        """`grid`: a NanoVDB buffer (uint8 array) or a DeviceNanoVDB."""
+        self = cls.__new__(cls)
+        self.grid = grid if isinstance(grid, DeviceNanoVDB) else DeviceNanoVDB(ctx, grid)
+        v = Volume()
+        v.field_type = FIELD_NANOVDB
+        v.nanovdb = self.grid.handle
+        v.filter = {"linear": 0, "nearest": 1}[filter]
+        v.bounds_lo = self.grid.info.bounds_lo
+        v.bounds_hi = self.grid.info.bounds_hi
+        v.cell_size = 1.0
+        self.dims = None
+        self._finish(ctx, v, tf_rgba, value_range, unit_distance, grid_dims, xfm)
+        return self
 
     def __init__(self, ctx, voxels, origin, spacing, tf_rgba, value_range, unit_distance,
                  grid_dims=None, xfm=None):
@@ -330,8 +382,6 @@ class DeviceVolume:
         check(lib().vrt_structured_brickify(ctx.handle, lin.ptr, _dims(self.dims), self.bricks.ptr))
         ctx.sync()
         lin.free()
-        tf = np.ascontiguousarray(tf_rgba, np.float32).reshape(-1, 4)
-        self.tf = ctx.upload(tf)
         v = Volume()
         v.field_type = FIELD_STRUCTURED_REGULAR
         v.dims = i3(*self.dims)
@@ -341,6 +391,11 @@ class DeviceVolume:
         v.bounds_hi = f3(*hi)
         v.cell_size = float(np.min(np.array(spacing, np.float32)))
         v.bricks = self.bricks.ptr
+        self._finish(ctx, v, tf_rgba, value_range, unit_distance, grid_dims, xfm)
+
+    def _finish(self, ctx, v, tf_rgba, value_range, unit_distance, grid_dims, xfm):
+        tf = np.ascontiguousarray(tf_rgba, np.float32).reshape(-1, 4)
+        self.tf = ctx.upload(tf)
         v.tf_rgba = self.tf.ptr
         v.tf_size = tf.shape[0]
         v.value_lo, v.value_hi = float(value_range[0]), float(value_range[1])

@@ -42,6 +42,8 @@ class Workload:
     height: int
     renderer: str                # "raycast" | "pathtrace"
     frames: int = 1
+    nanovdb: np.ndarray = None   # NanoVDB grid buffer: a nanovdb field instead of voxels
+    filter: str = "linear"       # nanovdb sampler
 
     @property
     def dims(self):
@@ -109,6 +111,49 @@ def fbm_pathtrace(n=512, width=1920, height=1080, frames=64, seed=1, brick=16,
         width=width, height=height, renderer="pathtrace", frames=frames)
 
 
+def cloud_nanovdb_pathtrace(n=1024, width=1920, height=1080, frames=256, seed=3,
+                            active_fraction=0.05, albedo=0.8, max_bounce=8, brick=32,
+                            unit_distance=None, threads=0):
+    """Sparse fBm cloud as a 'nanovdb' field (1024^3 index space, identity
+    map, 8^3 leaves, ~5% active), path traced with a directional light and
+    ambient term; majorant cells of `brick` voxels over the grid bounds."""
+    buf, _ = _vrt.gen_nanovdb_cloud(n=n, seed=seed, active_fraction=active_fraction,
+                                    threads=threads)
+    ud = unit_distance if unit_distance is not None else 10.0 * n / 1024.0
+    c = n / 2.0
+    d = np.array([1.5, 1.0, 2.0]) / math.sqrt(7.25)
+    pos = tuple(float(x) for x in c + 1.25 * n * d)  # 1.25 n from the centre
+    bb = _grid_bounds(buf)
+    g = tuple(max(1, int(math.ceil((bb[3 + a] - bb[a]) / brick))) for a in range(3))
+    return Workload(
+        name=f"pathtrace nanovdb cloud {n}^3 {width}x{height}", voxels=None, origin=None,
+        spacing=None, tf=albedo_tf(albedo, 256), value_range=(0.0, 1.0), unit_distance=ud,
+        grid_dims=g,
+        camera=dict(position=pos, direction=look_at(pos, (c, c, c)), up=(0, 1, 0),
+                    fovy=math.radians(40.0), aspect=width / height),
+        lights=[dict(type="directional", direction=(-0.4, -1.0, -0.3), color=(1, 1, 1),
+                     intensity=1.0)],
+        params=dict(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.5,
+                    occlusion_distance=1e20, ambient_samples=1, max_bounce=max_bounce,
+                    pixel_samples=1),
+        width=width, height=height, renderer="pathtrace", frames=frames, nanovdb=buf)
+
+
+def _grid_bounds(buf):
+    """GridData::mWorldBBox of a NanoVDB buffer as float (lo xyz, hi xyz)."""
+    return np.frombuffer(buf[560:608].tobytes(), np.float64).astype(np.float32)
+
+
+def volume_bounds(wl):
+    """Object-space bounds of the workload's volume."""
+    if wl.nanovdb is not None:
+        bb = _grid_bounds(wl.nanovdb)
+        return bb[:3], bb[3:]
+    lo = np.array(wl.origin, np.float32)
+    hi = lo + (np.array(wl.dims, np.float32) - 1) * np.array(wl.spacing, np.float32)
+    return lo, hi
+
+
 # ---------------------------------------------------------------------------
 # GPU side
 # ---------------------------------------------------------------------------
@@ -119,6 +164,10 @@ class GpuScene:
                  volume=None, params=None):
         self.wl = wl
         self.ctx = ctx
+        if volume is None and wl.nanovdb is not None:
+            volume = _vrt.DeviceVolume.from_nanovdb(ctx, wl.nanovdb, wl.tf, wl.value_range,
+                                                    wl.unit_distance, wl.grid_dims,
+                                                    filter=wl.filter)
         self.volume = volume or _vrt.DeviceVolume(ctx, wl.voxels, wl.origin, wl.spacing, wl.tf,
                                                   wl.value_range, wl.unit_distance, wl.grid_dims)
         lights = [_vrt.directional_light(L["direction"], L["intensity"], L["color"])
@@ -139,9 +188,7 @@ class GpuScene:
 
     def nrc_options(self, train=True, train_samples=65536, train_steps=4):
         """Cache options whose [0,1]^3 box is the volume's bounds."""
-        wl = self.wl
-        lo = np.array(wl.origin, np.float32)
-        hi = lo + (np.array(wl.dims, np.float32) - 1) * np.array(wl.spacing, np.float32)
+        lo, hi = volume_bounds(self.wl)
         return _vrt.nrc_render_options(lo, hi, train, train_samples, train_steps)
 
     def render_nrc(self, nrc, opts, accum_id=0):

@@ -45,6 +45,9 @@ typedef enum vrt_status {
 #define VRT_FIELD_NANOVDB 1
 #define VRT_FIELD_UNSTRUCTURED 2
 
+#define VRT_FILTER_LINEAR 0
+#define VRT_FILTER_NEAREST 1
+
 #define VRT_BRICK_LOG2 3 /* 8^3-voxel bricks, 2 KiB of fp32 each */
 
 /* A transferFunction1D volume over one spatial field, placed by one
@@ -59,8 +62,11 @@ typedef struct vrt_volume {
   float cell_size;     /* min(spacing) (StructuredRegularField.cpp:40) */
   /* structuredRegular: brick-ordered voxels, see vrt_structured_brickify */
   const float *bricks;
-  /* nanovdb: device copy of the grid buffer (NanoVDBField.cpp:29-64) */
-  const void *nanovdb_grid;
+  /* nanovdb: the converted grid (vrt_nanovdb_create, replacing the device
+     copy of the grid buffer, NanoVDBField.cpp:29-64); bounds and cell size
+     then come from the grid and dims / bricks are unused */
+  struct vrt_nanovdb_s *nanovdb;
+  int filter;          /* VRT_FILTER_*: nanovdb "filter" (NanoVDBField.cpp:27) */
   /* transfer function: tf_size float4 entries, value range, unitDistance */
   const float *tf_rgba;
   int tf_size;
@@ -198,6 +204,35 @@ VRT_API size_t vrt_structured_brick_bytes(const int dims[3]);
  * (StructuredRegularField.cpp:42-76). */
 VRT_API int vrt_structured_brickify(vrt_context ctx, const float *linear,
                                     const int dims[3], float *bricks);
+
+/* NanoVDB float grid (NanoVDBField.cpp:25-65).  The host buffer of one
+ * float grid in the NanoVDB 32.x format (the "data" array of a "nanovdb"
+ * spatial field) is converted once into the GPU layout: a dense top table of
+ * 128^3-voxel regions over the grid's bounds (+1 region apron), each a
+ * constant (background or tile value) or a block of 16^3 slot entries (leaf
+ * index or lower-node tile constant), and the leaf values in NanoVDB's
+ * in-leaf order.  Lookups inside the top table reproduce
+ * NanoGrid<float>::getValue exactly; outside it they read the background.
+ * The sampler is the one NanoVDBField uses: createSampler<0|1> at
+ * worldToIndexF(P) (DeviceCopyableObjects.h:554-569). */
+typedef struct vrt_nanovdb_s *vrt_nanovdb;
+
+typedef struct vrt_nanovdb_info {
+  float bounds_lo[3], bounds_hi[3]; /* GridData::mWorldBBox, as float */
+  float background;
+  int top_origin[3];                /* index of top entry (0,0,0) / 128 */
+  int top_dims[3];                  /* top table dims */
+  uint64_t blocks;                  /* lower nodes inside the bounds */
+  uint64_t leaves;                  /* leaves copied to the device */
+  uint64_t constants;               /* distinct constants (background first) */
+  uint64_t active_voxels;           /* TreeData::mVoxelCount */
+  uint64_t device_bytes;            /* tables + leaves + constants + slot ranges */
+} vrt_nanovdb_info;
+
+VRT_API int vrt_nanovdb_create(vrt_context ctx, const void *grid_buffer, uint64_t bytes,
+                               vrt_nanovdb *out);
+VRT_API int vrt_nanovdb_get_info(vrt_nanovdb grid, vrt_nanovdb_info *info);
+VRT_API int vrt_nanovdb_destroy(vrt_nanovdb grid);
 
 /* Per-macrocell value ranges (float2 min/max) of a field over a
  * grid_dims grid in voxel space, conservative for trilinear sampling.

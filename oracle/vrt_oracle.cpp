@@ -8,6 +8,7 @@
  * that it also serves as the timed CPU baseline.
  */
 #include "vrt_oracle.h"
+#include "nvdb_oracle.h"
 
 #include <algorithm>
 #include <atomic>
@@ -17,6 +18,8 @@
 #include <limits>
 #include <thread>
 #include <vector>
+#include <map>
+#include <array>
 
 namespace orc {
 
@@ -169,6 +172,9 @@ template <class R> inline R epsilon_from(V3<R> P, V3<R> dir, R t) {
 // ---------------------------------------------------------------------------
 template <class R> struct Volume {
   const orc_volume *v;
+  NanoVDBGrid nv;      // nanovdb field
+  bool is_nvdb = false;
+  V3<R> glo, gho;      // majorant grid bounds in voxel space
   V3<R> blo, bhi;      // StructuredRegularField::bounds, :90-93
   V3<R> S, T;          // voxelSpaceTransform = (scaling(1/size), -min), :37-40
   R cellSize;          // min(spacing), :40
@@ -201,6 +207,25 @@ template <class R> struct Volume {
     for (int i = 0; i < 9; ++i) A[i] = R(inv[i]);
     transInv = V3<R>(R(-m[3]), R(-m[7]), R(-m[11]));
     has_grid = vol->majorants != nullptr && vol->grid_dims[0] > 0;
+    glo = V3<R>(R(0));
+    gho = V3<R>(R(1));
+    if (vol->field_type == 1) {
+      // NanoVDBField::finalize (NanoVDBField.cpp:43-60): float world bbox,
+      // identity voxel-space transform, cell size 1, grid over the bounds
+      is_nvdb = nv.open(vol->nanovdb);
+      float bl[3], bh[3];
+      for (int i = 0; i < 3; ++i) {
+        bl[i] = float(nv.wlo[i]);
+        bh[i] = float(nv.whi[i]);
+      }
+      blo = from<R>(bl);
+      bhi = from<R>(bh);
+      S = V3<R>(R(1));
+      T = V3<R>(R(0));
+      cellSize = R(1);
+      glo = blo;
+      gho = bhi;
+    }
   }
 
   V3<R> mulA(V3<R> p) const {
@@ -226,6 +251,10 @@ template <class R> struct Volume {
   // (i+0.5)/N, i.e. texel coordinate p*N - 0.5 (GridAccel.cpp:103 samples
   // the TF at (i+.5)/N under the same convention)
   R sample(V3<R> P) const {
+    if (is_nvdb) {
+      const R p[3] = {P.x, P.y, P.z};
+      return nv.sample(p, v->filter == 1 ? 0 : 1);
+    }
     int n[3] = {v->dims[0], v->dims[1], v->dims[2]};
     int i0[3], i1[3];
     R w[3];
@@ -399,7 +428,7 @@ VolHit<R> delta_track(const Volume<R> &vol, Ray<R> ray, Random &rnd, Counter *cn
     int dims[3] = {vol.v->grid_dims[0], vol.v->grid_dims[1], vol.v->grid_dims[2]};
     // gridBounds = voxel-space image of the world bounds = [0,1]^3
     // (StructuredRegularField.cpp:103-107)
-    dda3(ray, dims, V3<R>(R(0)), V3<R>(R(1)), woodcock);
+    dda3(ray, dims, vol.glo, vol.gho, woodcock);
   } else {
     woodcock(-1, ray.tmin, ray.tmax);
   }
@@ -883,6 +912,91 @@ using namespace orc;
 // ===========================================================================
 // C API
 // ===========================================================================
+namespace orc {
+
+// max TF opacity over the taps tex1D reads for values in [vmin, vmax],
+// quantized like the GPU's 8-bit majorants
+inline float tf_majorant(const orc_volume *v, float vmin, float vmax) {
+  const int n = v->tf_size;
+  float u0 = (vmin - v->value_lo) / (v->value_hi - v->value_lo);
+  float u1 = (vmax - v->value_lo) / (v->value_hi - v->value_lo);
+  float c0 = std::min(std::max(u0 * n - 0.5f, -1.f), float(n));
+  float c1 = std::min(std::max(u1 * n - 0.5f, -1.f), float(n));
+  int lo = std::min(std::max(int(std::floor(c0)), 0), n - 1);
+  int hi = std::min(std::max(int(std::floor(c1)) + 1, 0), n - 1);
+  float m = 0.f;
+  for (int i = lo; i <= hi; ++i) m = std::max(m, v->tf_rgba[4 * i + 3]);
+  return quantize_majorant(m, majorant_step(v));
+}
+
+// nanovdb majorants: a cell of the grid over the float world bounds reads
+// voxels in the box from the index images of its 8 corners (double map),
+// floor, one voxel below and two above; the value range is the union over
+// the 8^3 slots (leaf-aligned blocks) that box touches
+void build_majorants_nanovdb(const orc_volume *v, const int G[3], float *out) {
+  NanoVDBGrid g;
+  if (!g.open(v->nanovdb)) return;
+  float glo[3], ghi[3];
+  for (int a = 0; a < 3; ++a) {
+    glo[a] = float(g.wlo[a]);
+    ghi[a] = float(g.whi[a]);
+  }
+  std::map<std::array<int, 3>, std::pair<float, float>> slot_range;
+  auto range_of = [&](int sx, int sy, int sz) {
+    auto key = std::array<int, 3>{sx, sy, sz};
+    auto it = slot_range.find(key);
+    if (it != slot_range.end()) return it->second;
+    float mn = FLT_MAX, mx = -FLT_MAX;
+    for (int i = 0; i < 8; ++i)
+      for (int j = 0; j < 8; ++j)
+        for (int k = 0; k < 8; ++k) {
+          const float val = g.get(sx * 8 + i, sy * 8 + j, sz * 8 + k);
+          mn = std::min(mn, val);
+          mx = std::max(mx, val);
+        }
+    slot_range.emplace(key, std::make_pair(mn, mx));
+    return std::make_pair(mn, mx);
+  };
+  for (int cz = 0; cz < G[2]; ++cz)
+    for (int cy = 0; cy < G[1]; ++cy)
+      for (int cx = 0; cx < G[0]; ++cx) {
+        const int c[3] = {cx, cy, cz};
+        double wmin[3], wmax[3];
+        for (int a = 0; a < 3; ++a) {
+          const double w = (double(ghi[a]) - double(glo[a])) / double(G[a]);
+          wmin[a] = double(glo[a]) + double(c[a]) * w;
+          wmax[a] = double(glo[a]) + double(c[a] + 1) * w;
+        }
+        double imin[3] = {INFINITY, INFINITY, INFINITY}, imax[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int k = 0; k < 8; ++k) {
+          const double p[3] = {(k & 1 ? wmax[0] : wmin[0]) - g.vec_d[0],
+                               (k & 2 ? wmax[1] : wmin[1]) - g.vec_d[1],
+                               (k & 4 ? wmax[2] : wmin[2]) - g.vec_d[2]};
+          for (int a = 0; a < 3; ++a) {
+            const double i = g.inv_d[3 * a] * p[0] + g.inv_d[3 * a + 1] * p[1] + g.inv_d[3 * a + 2] * p[2];
+            imin[a] = std::min(imin[a], i);
+            imax[a] = std::max(imax[a], i);
+          }
+        }
+        int s0[3], s1[3];
+        for (int a = 0; a < 3; ++a) {
+          s0[a] = (int(std::floor(imin[a])) - 1) >> 3;
+          s1[a] = (int(std::floor(imax[a])) + 2) >> 3;
+        }
+        float mn = FLT_MAX, mx = -FLT_MAX;
+        for (int sz = s0[2]; sz <= s1[2]; ++sz)
+          for (int sy = s0[1]; sy <= s1[1]; ++sy)
+            for (int sx = s0[0]; sx <= s1[0]; ++sx) {
+              const auto r = range_of(sx, sy, sz);
+              mn = std::min(mn, r.first);
+              mx = std::max(mx, r.second);
+            }
+        out[(size_t(cz) * G[1] + cy) * G[0] + cx] = tf_majorant(v, mn, mx);
+      }
+}
+
+}  // namespace orc
+
 extern "C" {
 
 uint32_t orc_tea_seed(uint32_t v0, uint32_t v1) { return Random(v0, v1).state; }
@@ -951,6 +1065,10 @@ void orc_post_classify(const orc_volume *v, int n, const float *vals, float *out
 // GridAccel.cpp:69-107).
 void orc_build_majorants(const orc_volume *v, int gx, int gy, int gz, float *out) {
   int G[3] = {gx, gy, gz};
+  if (v->field_type == 1) {
+    orc::build_majorants_nanovdb(v, G, out);
+    return;
+  }
   const int *N = v->dims;
   auto range = [&](int a, int m, int &lo, int &hi) {
     double c0 = double(m) * N[a] / G[a] - 0.5;

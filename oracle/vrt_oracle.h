@@ -52,6 +52,12 @@ typedef struct orc_volume {
   int grid_dims[3];
   /* instance transform object->world, row-major 3x4 [R|t] */
   float xfm[12];
+  /* field type: 0 structuredRegular (above), 1 nanovdb: `nanovdb` is a
+     NanoVDB float grid buffer (NanoVDBField.cpp), filter 0 linear,
+     1 nearest; bounds, voxel space and cell size then come from the grid */
+  int field_type;
+  const void *nanovdb;
+  int filter;
 } orc_volume;
 
 typedef struct orc_trimesh {

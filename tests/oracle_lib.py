@@ -25,7 +25,8 @@ class OrcVolume(C.Structure):
                 ("tf_rgba", C.c_void_p), ("tf_size", C.c_int), ("value_lo", C.c_float),
                 ("value_hi", C.c_float), ("unit_distance", C.c_float),
                 ("majorants", C.c_void_p), ("grid_dims", C.c_int * 3),
-                ("xfm", C.c_float * 12)]
+                ("xfm", C.c_float * 12), ("field_type", C.c_int), ("nanovdb", C.c_void_p),
+                ("filter", C.c_int)]
 
 
 class OrcTrimesh(C.Structure):
@@ -184,6 +185,26 @@ class OracleVolume:
         v.xfm = (C.c_float * 12)(*x.ravel())
         self.s = v
 
+    @classmethod
+    def from_nanovdb(cls, buf, tf, value_range, unit_distance, filter="linear", xfm=None):
+        self = cls.__new__(cls)
+        self.voxels = None
+        self.buf = buf
+        self.tf = np.ascontiguousarray(tf, np.float32).reshape(-1, 4)
+        v = OrcVolume()
+        v.field_type = 1
+        v.nanovdb = buf.ctypes.data
+        v.filter = {"linear": 0, "nearest": 1}[filter]
+        v.tf_rgba = self.tf.ctypes.data
+        v.tf_size = self.tf.shape[0]
+        v.value_lo, v.value_hi = value_range
+        v.unit_distance = unit_distance
+        self.majorants = None
+        x = np.eye(3, 4, dtype=np.float32) if xfm is None else np.asarray(xfm, np.float32)
+        v.xfm = (C.c_float * 12)(*x.ravel())
+        self.s = v
+        return self
+
     def build_majorants(self, grid_dims):
         gx, gy, gz = grid_dims
         out = np.empty((gz, gy, gx), np.float32)
@@ -272,8 +293,12 @@ class OracleWorkload:
 
     def __init__(self, wl):
         self.wl = wl
-        self.vol = OracleVolume(wl.voxels, wl.origin, wl.spacing, wl.tf, wl.value_range,
-                                wl.unit_distance)
+        if wl.nanovdb is not None:
+            self.vol = OracleVolume.from_nanovdb(wl.nanovdb, wl.tf, wl.value_range,
+                                                 wl.unit_distance, wl.filter)
+        else:
+            self.vol = OracleVolume(wl.voxels, wl.origin, wl.spacing, wl.tf, wl.value_range,
+                                    wl.unit_distance)
         if wl.grid_dims is not None:
             self.vol.attach_majorants(wl.grid_dims)
         lights = [light_directional(L["direction"], L["intensity"], L["color"])
