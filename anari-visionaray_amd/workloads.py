@@ -119,7 +119,7 @@ def cloud_nanovdb_pathtrace(n=1024, width=1920, height=1080, frames=256, seed=3,
     ambient term; majorant cells of `brick` voxels over the grid bounds."""
     buf, _ = _vrt.gen_nanovdb_cloud(n=n, seed=seed, active_fraction=active_fraction,
                                     threads=threads)
-    ud = unit_distance if unit_distance is not None else 10.0 * n / 1024.0
+    ud = unit_distance if unit_distance is not None else 2.0 * n / 1024.0
     c = n / 2.0
     d = np.array([1.5, 1.0, 2.0]) / math.sqrt(7.25)
     pos = tuple(float(x) for x in c + 1.25 * n * d)  # 1.25 n from the centre

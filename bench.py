@@ -65,14 +65,23 @@ def cpu_baseline(wl, target_seconds=12.0):
     }
 
 
-def nrc_vs_plain(pkg, ctx, wl, volume, plain_ms, ref_frames=2048, cap=1024, warmup=64,
-                 target=1e-2, oracle_stride=24, oracle_spp=4096):
+def nrc_vs_plain(pkg, ctx, wl, volume=None, ref_frames=2048, cap=1024, warmup=4, train_steps=8,
+                 target=1e-2, oracle_stride=24, oracle_spp=4096, timed_frames=32):
     """Wall time to relMSE <= target of the plain path tracer and of the
-    radiance-cache path tracer, both measured against an FP64-accumulated
-    plain reference of ref_frames spp; the reference's own relMSE against
-    the CPU FP64 oracle is measured on every oracle_stride-th pixel."""
+    radiance-cache path tracer (its training warm-up included), both measured
+    against an FP64-accumulated plain reference of ref_frames spp; frame
+    times on HIP events; the reference's own relMSE against the CPU FP64
+    oracle is measured on every oracle_stride-th pixel."""
     vrt, wls = pkg.vrt, pkg.workloads
     ref = wls.GpuScene(ctx, wl, accum64=True, volume=volume)
+    volume = ref.volume
+    for a in range(4):
+        ref.render(a)
+    ctx.sync()
+    ctx.timer_start()
+    for a in range(timed_frames):
+        ref.render(4 + a)
+    plain_ms = ctx.timer_stop() / timed_frames
     ref.frame.reset()
     for a in range(ref_frames):
         ref.render(a)
@@ -89,28 +98,38 @@ def nrc_vs_plain(pkg, ctx, wl, volume, plain_ms, ref_frames=2048, cap=1024, warm
     plain_reached = e <= target
 
     # radiance cache: train from scratch for `warmup` frames, then accumulate
-    nrc = vrt.Nrc(ctx, max_batch=16384, seed=1)
+    # until the target; the run is then repeated from scratch (same seeds,
+    # same images) without the relMSE checks and timed on HIP events
     warm = wls.GpuScene(ctx, wl, volume=volume, params={"rng_offset": 2 << 20})
-    opts = warm.nrc_options(train=True, train_samples=65536, train_steps=4)
-    ctx.timer_start()
-    for a in range(warmup):
-        warm.render_nrc(nrc, opts, a)
-    warm_ms = ctx.timer_stop()
-    nrc_ms = warm_ms / warmup
     gn = wls.GpuScene(ctx, wl, volume=volume, params={"rng_offset": 3 << 20})
-    gn.frame.reset()
-    frames_nrc, e_nrc = 0, float("inf")
-    while e_nrc > target and frames_nrc < cap:
-        gn.render_nrc(nrc, opts, frames_nrc)
-        frames_nrc += 1
-        e_nrc = vrt.relmse(ctx, gn.frame, ref.frame, ref_frames)
-    counts = nrc.frame_counts()
-    nrc.close()
+    opts = warm.nrc_options(train=True, train_samples=65536, train_steps=train_steps)
+
+    def nrc_run(frames=None):
+        nrc = vrt.Nrc(ctx, max_batch=16384, seed=1)
+        ctx.sync()
+        ctx.timer_start()
+        for a in range(warmup):
+            warm.render_nrc(nrc, opts, a)
+        gn.frame.reset()
+        n, e = 0, float("inf")
+        while (n < frames) if frames is not None else (e > target and n < cap):
+            gn.render_nrc(nrc, opts, n)
+            n += 1
+            if frames is None:
+                e = vrt.relmse(ctx, gn.frame, ref.frame, ref_frames)
+        ms = ctx.timer_stop()
+        counts = nrc.frame_counts()
+        nrc.close()
+        return n, e, ms, counts
+
+    frames_nrc, e_nrc, _, counts = nrc_run()
     nrc_reached = e_nrc <= target
+    _, _, t_nrc, _ = nrc_run(frames_nrc)
+    nrc_ms = t_nrc / (warmup + frames_nrc)
 
     t_plain = frames_plain * plain_ms
-    t_nrc = (warmup + frames_nrc) * nrc_ms
     out = {
+        "workload": wl.name,
         "target_relmse": target,
         "relmse_eps": 1e-2,
         "reference": f"plain GPU path tracer, {ref_frames} spp, FP64 accumulation",
@@ -171,6 +190,8 @@ def main():
     ap.add_argument("--no-nrc", action="store_true",
                     help="skip the time-to-relMSE comparison with the radiance cache")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--ttr-size", type=int, default=1024,
+                    help="index-space edge of the nanovdb cloud of the time-to-relMSE run")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -227,8 +248,13 @@ def main():
         cpu = cpu_baseline(wl, args.cpu_seconds)
     nrc = None
     if not args.no_nrc:
-        nrc = nrc_vs_plain(pkg, ctx, wl, gs.volume, ms_per_step,
-                           oracle_stride=0 if args.no_cpu else 24)
+        # radiance cache vs plain on the nanovdb cloud: 1024^3 index space,
+        # ~5% active, 8^3 leaves
+        t0 = time.perf_counter()
+        wl3 = wls.cloud_nanovdb_pathtrace(n=args.ttr_size, width=args.width, height=args.height)
+        print(f"# generated {wl3.name} in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+              flush=True)
+        nrc = nrc_vs_plain(pkg, ctx, wl3, oracle_stride=0 if args.no_cpu else 24)
         print(f"# time to relMSE 1e-2: {json.dumps(nrc)}", file=sys.stderr, flush=True)
 
     out = {
