@@ -1,0 +1,17 @@
+// The triangle BVH behind vrt_bvh (bvh_build.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+struct vrt_bvh_s {
+  int device = 0;
+  // device: 4-wide nodes (8 float4 each: child boxes lo/hi x, y, z as four
+  // lanes, child entries, leaf counts) and triangles (3 float4: v1 + prim id
+  // bits, e1, e2) in leaf order
+  float4 *nodes = nullptr;
+  float4 *tris = nullptr;
+  uint32_t num_nodes = 0, num_tris = 0, depth = 0;
+  float lo[3], hi[3];
+  double build_ms = 0.0;
+};

@@ -73,7 +73,15 @@ struct KLight {
   int visible;
   f3 dir;        // directional: unit vector towards the light
   f3 intensity;  // color * irradiance / intensity
-  f3 v1, e1, e2; // quad
+  f3 v1, e1, e2; // quad: position, edges
+  f3 nrm;        // quad: normalize(e1 x (e1 + e2))
+  float a1, a2;  // quad: areas of its triangles {v1, e1, e1+e2}, {v1, e1+e2, e2}
+};
+
+struct KSurface {
+  const float4 *nodes;  // 4-wide BVH nodes, 8 float4 each
+  const float4 *tris;   // {v1, prim id bits}, e1, e2 per triangle
+  f3 color;             // matte
 };
 
 struct KScene {
@@ -81,8 +89,10 @@ struct KScene {
   int num_lights;
   int tf_lds_total;   // float4 entries of all tables staged in LDS
   int maj_lds_bytes;  // majorant bytes staged in LDS after the tables
+  int num_surfaces;
   KVolume vol[VRT_MAX_VOLUMES];
   KLight light[VRT_MAX_LIGHTS];
+  KSurface surf[VRT_MAX_SURFACES];
 };
 
 struct KCamera {

@@ -21,7 +21,7 @@ hipError_t launch_nrc_resolve(const KFrame &F, const KNrc &N, int accumID, hipSt
 // adds sum (x - r)^2 / (r^2 + eps) over n pixels' RGB to *out
 hipError_t launch_relmse(const float4 *x, const double *ref_sum, double ref_scale, int n,
                          float eps, double *out, hipStream_t s);
-int pathtrace_blocks_per_cu(size_t lds_bytes, bool nrc = false);
+int pathtrace_blocks_per_cu(const KScene &sc, size_t lds_bytes, bool nrc = false);
 hipError_t launch_brickify(const float *lin, const int dims[3], float *bricks, hipStream_t s);
 hipError_t launch_value_ranges(const KVolume &v, float *ranges, hipStream_t s);
 // nanovdb: min/max (float2) of each block slot's values
@@ -39,6 +39,8 @@ hipError_t launch_frame_reset(const KFrame &F, hipStream_t s);
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 hipError_t launch_sample_field(const KVolume &v, int n, const float *pts, float *out,
                                hipStream_t s);
+hipError_t launch_intersect_surfaces(const KScene &sc, int n, const float *rays, int any,
+                                     float *t_out, int *prim_out, hipStream_t s);
 hipError_t launch_delta_track(const KScene &sc, int n, const float *rays, uint32_t *rng,
                               float *tout, hipStream_t s);
 

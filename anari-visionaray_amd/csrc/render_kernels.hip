@@ -328,6 +328,142 @@ __device__ inline void clip_ray(KRay &r, const KParams &P) {
 }
 
 // ---------------------------------------------------------------------------
+// Triangles and the 4-wide BVH (bvh_build.cpp), quad lights
+// ---------------------------------------------------------------------------
+// basic_triangle {v1, e1, e2} (visionaray, not vendored): Moller-Trumbore
+// with inclusive edges; t is returned unchecked
+__device__ inline bool tri_test(const KRay &r, f3 v1, f3 e1, f3 e2, float &t, float &u,
+                                float &v) {
+  const f3 s1 = cross(r.dir, e2);
+  const float div = dot(s1, e1);
+  if (div == 0.f) return false;
+  const float inv = 1.f / div;
+  const f3 d = r.ori - v1;
+  const float b1 = dot(d, s1) * inv;
+  if (b1 < 0.f || b1 > 1.f) return false;
+  const f3 s2 = cross(d, e1);
+  const float b2 = dot(r.dir, s2) * inv;
+  if (b2 < 0.f || b1 + b2 > 1.f) return false;
+  t = dot(e2, s2) * inv;
+  u = b1;
+  v = b2;
+  return true;
+}
+
+constexpr int kBvhStack = 64;
+
+// Closest hit in [tmin, tbest] (tbest in: the far end), or any hit when
+// `any`; ties go to the lower triangle index, so the result does not depend
+// on the traversal order.  Per-lane stack of (node, entry distance); boxes
+// are tested with slab distances from finite reciprocals (a zero direction
+// component becomes +-1e20), which can only add box hits, never drop one.
+__device__ inline bool bvh_hit(const KSurface &S, const KRay &r, bool any, float &tbest,
+                               int &prim, uint32_t &rec) {
+  f3 inv;
+  {
+    const float dx = fabsf(r.dir.x) < 1e-20f ? copysignf(1e-20f, r.dir.x) : r.dir.x;
+    const float dy = fabsf(r.dir.y) < 1e-20f ? copysignf(1e-20f, r.dir.y) : r.dir.y;
+    const float dz = fabsf(r.dir.z) < 1e-20f ? copysignf(1e-20f, r.dir.z) : r.dir.z;
+    inv = mk(1.f / dx, 1.f / dy, 1.f / dz);
+  }
+  int stack[kBvhStack];
+  float stack_t[kBvhStack];
+  int sp = 0;
+  int node = 0;
+  bool found = false;
+  while (true) {
+    const float4 *n = S.nodes + size_t(node) * 8u;
+    const float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5];
+    const float4 chf = n[6], cnf = n[7];
+    const float lo_x[4] = {lx.x, lx.y, lx.z, lx.w}, hi_x[4] = {hx.x, hx.y, hx.z, hx.w};
+    const float lo_y[4] = {ly.x, ly.y, ly.z, ly.w}, hi_y[4] = {hy.x, hy.y, hy.z, hy.w};
+    const float lo_z[4] = {lz.x, lz.y, lz.z, lz.w}, hi_z[4] = {hz.x, hz.y, hz.z, hz.w};
+    const int child[4] = {__float_as_int(chf.x), __float_as_int(chf.y), __float_as_int(chf.z),
+                          __float_as_int(chf.w)};
+    const uint32_t count[4] = {__float_as_uint(cnf.x), __float_as_uint(cnf.y),
+                               __float_as_uint(cnf.z), __float_as_uint(cnf.w)};
+    int near_node[4];
+    float near_t[4];
+    int nn = 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if (child[c] < 0) continue;
+      const float tx0 = (lo_x[c] - r.ori.x) * inv.x, tx1 = (hi_x[c] - r.ori.x) * inv.x;
+      const float ty0 = (lo_y[c] - r.ori.y) * inv.y, ty1 = (hi_y[c] - r.ori.y) * inv.y;
+      const float tz0 = (lo_z[c] - r.ori.z) * inv.z, tz1 = (hi_z[c] - r.ori.z) * inv.z;
+      const float tn = fmaxf(r.tmin, fmaxf(fminf(tx0, tx1), fmaxf(fminf(ty0, ty1), fminf(tz0, tz1))));
+      const float tf = fminf(tbest, fminf(fmaxf(tx0, tx1), fminf(fmaxf(ty0, ty1), fmaxf(tz0, tz1))));
+      if (!(tn <= tf * 1.0000002f + 1e-30f)) continue;
+      if (count[c] == 0u) {
+        near_node[nn] = child[c];
+        near_t[nn] = tn;
+        ++nn;
+        continue;
+      }
+      for (uint32_t k = 0; k < count[c]; ++k) {
+        const float4 *q = S.tris + 3 * (size_t(child[c]) + k);
+        const float4 a = q[0], b = q[1], d = q[2];
+        float t, u, v;
+        if (!tri_test(r, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), mk(d.x, d.y, d.z), t, u, v))
+          continue;
+        const int p = __float_as_int(a.w);
+        if (t < r.tmin || t > tbest || (found && t == tbest && p > prim)) continue;
+        found = true;
+        tbest = t;
+        prim = p;
+        rec = uint32_t(child[c]) + k;
+        if (any) return true;
+      }
+    }
+    // push inner children far to near, so the nearest is visited next
+    for (int i = 1; i < nn; ++i)
+      for (int j = i; j > 0 && near_t[j] > near_t[j - 1]; --j) {
+        const float tt = near_t[j];
+        near_t[j] = near_t[j - 1];
+        near_t[j - 1] = tt;
+        const int tn = near_node[j];
+        near_node[j] = near_node[j - 1];
+        near_node[j - 1] = tn;
+      }
+    for (int i = 0; i < nn; ++i)
+      if (sp < kBvhStack) {
+        stack[sp] = near_node[i];
+        stack_t[sp] = near_t[i];
+        ++sp;
+      }
+    // next node: skip those entered beyond the current closest hit
+    node = -1;
+    while (sp > 0) {
+      --sp;
+      if (stack_t[sp] <= tbest * 1.0000002f + 1e-30f) {
+        node = stack[sp];
+        break;
+      }
+    }
+    if (node < 0) return found;
+  }
+}
+
+// intersect(Ray, Quad) (DeviceCopyableObjects.h:2209-2230): the closer of
+// the two triangles, t > 0, no [tmin, tmax] test (intersectLights)
+__device__ inline bool quad_hit(const KLight &L, const KRay &r, float &t) {
+  const f3 d = L.e1 + L.e2;
+  float t1, t2, u, v;
+  const bool h1 = tri_test(r, L.v1, L.e1, d, t1, u, v) && t1 > 0.f;
+  const bool h2 = tri_test(r, L.v1, d, L.e2, t2, u, v) && t2 > 0.f;
+  if (!h1 && !h2) return false;
+  t = h1 ? (h2 && t2 < t1 ? t2 : t1) : t2;
+  return true;
+}
+
+// power_heuristic (visionaray, parity unpinned): f^2 / (f^2 + g^2), 0 when
+// both vanish
+__device__ inline float power_heuristic(float f, float g) {
+  const float d = f * f + g * g;
+  return d > 0.f ? (f * f) / d : 0.f;
+}
+
+// ---------------------------------------------------------------------------
 // Woodcock / delta tracking through one volume (DeviceCopyableObjects.h:
 // 680-796) with the macrocell DDA of renderer/DDA.h:14-112.
 // ---------------------------------------------------------------------------
@@ -717,11 +853,12 @@ struct PathState {
   bool first;  // the sample's first pass
   bool firstHit;
   f3 throughput, intensity;
-  // ShadeState (Pathtrace_impl.cpp:15-35) of a volume vertex.  Its BSDF
-  // sample is the isotropic phase function: f = baseColor = albedo,
-  // pdf = cosTheta = 1, so the throughput update is a product with albedo.
-  f3 albedo;
-  f3 direct;  // shadedColor * lightVisibility
+  // ShadeState (Pathtrace_impl.cpp:15-35) of the current vertex
+  f3 albedo;       // throughput factor of the BSDF sample: f * (cosTheta / pdf)
+  f3 base;         // baseColor (ambient term)
+  f3 direct;       // shadedColor * lightVisibility
+  float bs_pdf;    // BSDF sample pdf (MIS at a light hit)
+  float misw;      // misWeightNEE
   f3 sn, bs_dir;
   int aoSamples;
   float aoWeights, aoCount;
@@ -792,29 +929,173 @@ __device__ inline void begin_sample(PathState &st, const KCamera &cam, const KPa
   st.firstHit = false;
   st.throughput = mk(1.f);
   st.intensity = mk(0.f);
+  st.bs_pdf = 0.f;
   st.qslot = -1;
   st.tslot = -1;
 }
 
+// intersectAll (renderer/common.h:1606-1620) of one pass: surfaces (closest
+// hit; any hit for shadow / AO rays), quad lights (invisible ones skipped for
+// camera rays), volumes (delta tracking, closest); a light wins ties with a
+// surface or volume, a volume must be strictly closer than a surface.
+enum HitKind { kNone = 0, kVolume = 1, kSurface = 2, kLight = 3 };
+
+struct PassHit {
+  int kind;
+  bool surf_hit;     // any surface hit in [tmin, tmax], whichever kind wins
+  float t;
+  f3 albedo;         // volume: classified color
+  float extinction;  // volume: classified opacity
+  int id;            // surface or light index
+  uint32_t rec;      // surface: triangle record
+};
+
+// AREA: the scene has surfaces or quad lights (else the kernel is built
+// without the BVH and quad code, which would only add registers)
+template <bool AREA>
+__device__ inline PassHit intersect_all(const KScene &sc, const KRay &ray, bool shadow,
+                                        int bounceID, const float4 *s_tf, const float *steps,
+                                        uint32_t &srng, uint32_t &lookups) {
+  bool sh = false;
+  float st = ray.tmax;
+  int sid = -1;
+  uint32_t srec = 0;
+  for (int i = 0; AREA && i < sc.num_surfaces; ++i) {
+    float t = st;
+    int prim = -1;
+    uint32_t rec = 0;
+    if (bvh_hit(sc.surf[i], ray, shadow, t, prim, rec)) {
+      sh = true;
+      st = t;
+      sid = i;
+      srec = rec;
+    }
+  }
+  bool lh = false;
+  float lt = FLT_MAX;
+  int lid = -1;
+  for (int i = 0; AREA && i < sc.num_lights; ++i) {
+    const KLight &L = sc.light[i];
+    if (L.type != VRT_LIGHT_QUAD || (bounceID == 0 && !L.visible)) continue;
+    float t;
+    if (quad_hit(L, ray, t) && t < lt) {
+      lh = true;
+      lt = t;
+      lid = i;
+    }
+  }
+  VolHit vh;
+  vh.hit = false;
+  vh.t = FLT_MAX;
+  for (int i = 0; i < sc.num_volumes; ++i) {
+    VolHit h = delta_track(sc.vol[i], s_tf, lds_majorants(sc, s_tf, i), steps[i],
+                           to_local(sc.vol[i], ray), srng, lookups);
+    if (h.hit && h.t < vh.t) vh = h;
+  }
+  PassHit out;
+  out.kind = kNone;
+  out.surf_hit = sh;
+  out.t = FLT_MAX;
+  out.id = -1;
+  out.rec = 0;
+  if (lh && (!sh || lt <= st) && (!vh.hit || lt <= vh.t)) {
+    out.kind = kLight;
+    out.t = lt;
+    out.id = lid;
+  } else if (vh.hit && (!sh || vh.t < st) && (!lh || vh.t < lt)) {
+    out.kind = kVolume;
+    out.t = vh.t;
+    out.albedo = vh.albedo;
+    out.extinction = vh.extinction;
+  } else if (sh) {
+    out.kind = kSurface;
+    out.t = st;
+    out.id = sid;
+    out.rec = srec;
+  } else if (vh.hit || lh) {
+    // a hit that is neither (e.g. a volume behind a light): hitRec.hit holds
+    out.kind = vh.hit ? kVolume : kLight;
+    out.t = vh.hit ? vh.t : lt;
+    out.albedo = vh.albedo;
+    out.extinction = vh.extinction;
+    out.id = lid;
+  }
+  return out;
+}
+
+// sampleLight (renderer/common.h:1432-1466).  Directional: the stored
+// direction, pdf 1, distance FLT_MAX, no draws.  Quad (area_light<Quad>, not
+// vendored: parity unpinned): one draw picks a triangle
+// (DeviceCopyableObjects.h:2258-2270), two place a uniform point on it;
+// pdf = 1 / (A |cos theta_light|), so that with the 1/dist^2 of sampleLight
+// the NEE estimate is the area-sampling estimator.
+struct LightSmp {
+  f3 intensity, dir;
+  float pdf, dist, dist2;
+  int type;
+};
+
+template <bool AREA>
+__device__ inline LightSmp sample_light(const KLight &L, f3 hitPos, uint32_t &rs) {
+  LightSmp ls;
+  ls.type = L.type;
+  ls.intensity = L.intensity;
+  if (AREA && L.type == VRT_LIGHT_QUAD) {
+    const float r = rnd(rs);
+    const f3 d = L.e1 + L.e2;
+    const bool first = L.a1 / (L.a1 + L.a2) < r;
+    const f3 te1 = first ? L.e1 : d, te2 = first ? d : L.e2;
+    const float u1 = rnd(rs), u2 = rnd(rs);
+    const float su = sqrtf(u1);
+    const f3 v2 = L.v1 + te1, v3 = L.v1 + te2;
+    const f3 pos = L.v1 * (1.f - su) + v2 * (su * (1.f - u2)) + v3 * (su * u2);
+    ls.dir = pos - hitPos;
+    ls.dist = length(ls.dir);
+    const float cosl = fabsf(dot(L.nrm, ls.dir * (1.f / ls.dist)));
+    ls.pdf = cosl > 0.f ? 1.f / ((L.a1 + L.a2) * cosl) : 0.f;
+    ls.dist2 = ls.dist * ls.dist;
+  } else {
+    ls.dir = L.dir;
+    ls.dist = FLT_MAX;
+    ls.pdf = 1.f;
+    ls.dist2 = 1.f;
+  }
+  return ls;
+}
+
+__device__ inline float safe_rcp(float f) { return f > 0.f ? 1.f / f : 0.f; }
+
+// Russian roulette, run after every pass that continues the path once
+// bounceID > 1 -- shadow and AO passes included -- drawing from the
+// un-advanced pixel RNG (Pathtrace_impl.cpp:594-599).  True: the path ends.
+__device__ inline bool roulette(PathState &st) {
+  if (st.bounceID > 1) {
+    const float prob = max3(st.throughput);
+    if (rnd(st.rng) > prob) return true;
+    st.throughput = st.throughput * (1.f / prob);
+  }
+  return false;
+}
+
 // One pass of the bounce loop (Pathtrace_impl.cpp:558-600) after the ray's
-// intersection `vh` (closest over the volumes): shade, advance the state
-// machine, set up the next ray.  Returns true when the sample's path ended.
-// With NRC, a path reaching its second volume vertex either becomes a cache
-// query and ends, or -- a training path -- records the vertex and goes on.
-template <bool NRC>
+// intersection `h`: shade, advance the state machine, set up the next ray.
+// Returns true when the sample's path ended.  With NRC, a path reaching its
+// second volume vertex either becomes a cache query and ends, or -- a
+// training path -- records the vertex and goes on.
+template <bool NRC, bool AREA>
 __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams &P,
-                                  const KNrc &N, const VolHit &vh) {
+                                  const KNrc &N, const PassHit &h) {
   KRay ray;
   ray.ori = st.ori;
   ray.dir = st.dir;
-  const bool hit = vh.hit;
+  const bool hit = h.kind != kNone;
   if (st.first) {
     st.first = false;
     st.firstHit = hit;
-    if (hit) st.sdepth = vh.t;
+    if (h.kind == kVolume || h.kind == kSurface) st.sdepth = h.t;
   }
   if constexpr (NRC) {
-    const bool v2 = st.rayType == Radiance && hit && st.bounceID == 1;
+    const bool v2 = st.rayType == Radiance && h.kind == kVolume && st.bounceID == 1;
     (void)wave_claim(&N.ctr[2], v2);  // vertex-2 paths of this frame
     const bool want_train = v2 && st.train_sel;
     const uint32_t tslot = wave_claim(&N.ctr[1], want_train);
@@ -823,7 +1104,7 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
     const uint32_t qslot = wave_claim(&N.ctr[0], want_query);
     if (v2) {
       float raw[6];
-      cache_inputs(N, ray.ori + ray.dir * vh.t, ray.dir, vh.extinction, raw);
+      cache_inputs(N, ray.ori + ray.dir * h.t, ray.dir, h.extinction, raw);
       if (is_train) {
         for (int i = 0; i < 6; ++i) N.t_raw[6 * size_t(tslot) + i] = raw[i];
         st.tslot = int(tslot);
@@ -846,41 +1127,113 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
   if (rayType == Radiance) {
     // a radiance miss ends the path; its shade state adds nothing
     if (!hit) return true;
-    const f3 hitPos = ray.ori + ray.dir * vh.t;
-    const float eps = epsilon_from(hitPos, ray.dir, vh.t);
+    const int nl = sc.num_lights;
+    if (AREA && h.kind == kLight) {
+      // the quad's intensity, MIS-weighted against the BSDF sample after the
+      // first bounce (Pathtrace_impl.cpp:139-183); the path ends
+      KLight L = sc.light[0];
+      for (int i = 1; i < nl; ++i)
+        if (i == h.id) L = sc.light[i];
+      f3 shaded = L.intensity;
+      if (st.bounceID > 0) {
+        const f3 hitPos = ray.ori + ray.dir * h.t;
+        const float A = L.a1 + L.a2;
+        const float ld = length(hitPos - ray.ori);
+        const f3 Ld = normalize(hitPos - ray.ori);
+        const float LdotNl = fabsf(dot(-Ld, L.nrm));
+        const float solidAngle = (LdotNl * A) / (ld * ld);
+        const float lightPDF = 1.f / solidAngle;
+        shaded = shaded * power_heuristic(st.bs_pdf, lightPDF / float(nl));
+      }
+      const f3 ambient = mk(0.f) * P.ambient_color * P.ambient_radiance * 1.f;
+      st.intensity = st.intensity + st.throughput * ((shaded * 1.f) * 1.f);
+      st.intensity = st.intensity + st.throughput * ambient;
+      return true;
+    }
     const f3 viewDir = -normalize(ray.dir);
-    f3 gn = mk(0.f);
-    if (P.ambient_samples > 0) {
+    const f3 hitPos = ray.ori + ray.dir * h.t;
+    const float eps = epsilon_from(hitPos, ray.dir, h.t);
+    f3 gn = mk(0.f), color;
+    if (h.kind == kVolume) {
+      if (P.ambient_samples > 0) {
+        const float u1 = rnd(st.srng), u2 = rnd(st.srng);
+        gn = uniform_sample_sphere(u1, u2);
+      }
+      st.sn = faceforward(gn, viewDir, gn);
+      color = h.albedo;
+    } else if (AREA) {
+      // triangle: Ng = normalize(e1 x e2), no vertex normals (getNormals,
+      // common.h:233-264), matte color
+      KSurface S = sc.surf[0];
+      for (int i = 1; i < sc.num_surfaces; ++i)
+        if (i == h.id) S = sc.surf[i];
+      const float4 e1 = S.tris[3 * size_t(h.rec) + 1], e2 = S.tris[3 * size_t(h.rec) + 2];
+      gn = normalize(cross(mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z)));
+      st.sn = faceforward(gn, viewDir, gn);
+      gn = faceforward(gn, viewDir, gn);
+      color = S.color;
+    }
+    LightSmp ls;
+    ls.intensity = mk(0.f);
+    ls.dir = mk(0.f);
+    ls.pdf = ls.dist = ls.dist2 = 0.f;
+    ls.type = -1;
+    if (nl > 0) {
+      int which = int(rnd(st.srng) * float(nl));
+      if (which == nl) which = 0;
+      KLight L = sc.light[0];
+      for (int i = 1; i < nl; ++i)
+        if (i == which) L = sc.light[i];
+      ls = sample_light<AREA>(L, hitPos, st.srng);
+    }
+    const f3 lightIntensity = ls.intensity * safe_rcp(ls.dist2);
+    if (h.kind == kVolume) {
+      // Pathtrace_impl.cpp:330-339: no phase factor, no light count;
+      // isotropic phase sample: f = albedo, pdf = cosTheta = 1
+      st.direct = h.albedo * lightIntensity * safe_rcp(ls.pdf) * safe_rcp(ls.dist2);
       const float u1 = rnd(st.srng), u2 = rnd(st.srng);
-      gn = uniform_sample_sphere(u1, u2);
+      st.bs_dir = uniform_sample_sphere(u1, u2);
+      st.bs_pdf = 1.f;
+      st.albedo = h.albedo * (1.f * 1.f);
+    } else if (AREA) {
+      // matte (common.h:1033-1048, 1371-1413): f = color / pi, cosine
+      // hemisphere around faceforward(Ns, viewDir, Ng), pdf = cos / pi
+      const float inv_pi = 0.318309886183790671538f;
+      const f3 f = color * inv_pi;
+      const f3 lightDir = normalize(ls.dir);
+      const float NdotL = fmaxf(0.f, dot(st.sn, lightDir));
+      st.direct = f * lightIntensity * NdotL * safe_rcp(ls.pdf) * float(nl);
+      const f3 w = faceforward(st.sn, viewDir, gn);
+      const f3 v = fabsf(w.x) > fabsf(w.y) ? normalize(mk(-w.z, 0.f, w.x))
+                                           : normalize(mk(0.f, w.z, -w.y));
+      const f3 u = cross(v, w);
+      const float u1 = rnd(st.srng), u2 = rnd(st.srng);
+      const f3 sp = cosine_sample_hemisphere(u1, u2);
+      st.bs_dir = normalize(u * sp.x + v * sp.y + w * sp.z);
+      st.bs_pdf = fmaxf(0.f, dot(st.sn, st.bs_dir)) * inv_pi;
+      const float cosT = fmaxf(0.f, dot(st.sn, st.bs_dir));
+      st.albedo = f * (cosT * safe_rcp(st.bs_pdf));
     }
-    st.sn = faceforward(gn, viewDir, gn);
-    f3 ls_int = mk(0.f), ls_dir = mk(0.f);
-    if (sc.num_lights > 0) {
-      int which = int(rnd(st.srng) * float(sc.num_lights));
-      if (which == sc.num_lights) which = 0;
-      // directional light sample: pdf 1, dist2 1 (renderer/common.h:1432-1466),
-      // shadedColor = albedo * E / dist2 / pdf / dist2
-      ls_int = sc.light[which].intensity;
-      ls_dir = sc.light[which].dir;
-    }
-    st.direct = vh.albedo * ls_int;
-    const float u1 = rnd(st.srng), u2 = rnd(st.srng);
-    st.bs_dir = uniform_sample_sphere(u1, u2);
-    st.albedo = vh.albedo;
+    // misWeightNEE (Pathtrace_impl.cpp:370-380)
+    st.misw = nl > 0 ? (ls.type == VRT_LIGHT_QUAD ? power_heuristic(ls.pdf / float(nl), st.bs_pdf)
+                                                 : 1.f)
+                     : 0.f;
+    st.base = color;
     st.aoSamples = 0;
     st.aoWeights = 0.f;
     st.aoCount = 0.f;
     // lightSample.pdf >= 0 always holds: shadow ray next (:403-406)
     st.ori = hitPos + st.sn * eps;  // hitPosOff
-    st.dir = normalize(ls_dir);
-    st.tmax = FLT_MAX;
+    st.dir = normalize(ls.dir);
+    st.tmax = nl > 0 ? ls.dist : 0.f;
     st.rayType = Shadow;
-    return false;
+    return roulette(st);
   }
   float vis_ao = 1.f;
   if (rayType == Shadow) {
-    st.direct = st.direct * (hit ? 0.f : 1.f);
+    // visibility = surfV * volV * Tr (Pathtrace_impl.cpp:439-444): blocked
+    // by any surface hit or by a volume collision that wins the pass
+    st.direct = st.direct * ((h.surf_hit || h.kind == kVolume) ? 0.f : 1.f);
   } else {  // AO
     st.aoSamples++;
     const float weight = fmaxf(0.f, dot(ray.dir, st.sn));
@@ -902,21 +1255,16 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
     st.dir = normalize(u * sp.x + v * sp.y + w * sp.z);
     st.tmax = P.occlusion_distance;
     st.rayType = AO;
-    return false;
+    return roulette(st);
   }
-  // the bounce ends: accumulate, continue along the phase-function sample
-  const float misWeightNEE = sc.num_lights > 0 ? 1.f : 0.f;
-  const f3 ambient = st.albedo * P.ambient_color * P.ambient_radiance * vis_ao;
-  st.intensity = st.intensity + st.throughput * (st.direct * misWeightNEE);
+  // the bounce ends: accumulate, continue along the BSDF sample
+  const f3 ambient = st.base * P.ambient_color * P.ambient_radiance * vis_ao;
+  st.intensity = st.intensity + st.throughput * (st.direct * st.misw);
   st.intensity = st.intensity + st.throughput * ambient;
   st.throughput = st.throughput * st.albedo;
   st.bounceID++;
   if (st.bounceID > P.max_bounce) return true;
-  if (st.bounceID > 1) {
-    const float prob = max3(st.throughput);
-    if (rnd(st.rng) > prob) return true;
-    st.throughput = st.throughput * (1.f / prob);
-  }
+  if (roulette(st)) return true;
   st.dir = normalize(st.bs_dir);
   st.tmax = 1e31f;
   st.rayType = Radiance;
@@ -930,7 +1278,7 @@ __device__ inline float train_hash(uint32_t pixel, uint32_t accumID) {
          (1.0f / 16777216.0f);
 }
 
-template <bool NRC>
+template <bool NRC, bool AREA>
 __global__ void __launch_bounds__(256)
     k_pathtrace(KScene sc, KCamera cam, KParams P, KFrame F, int accumID, uint32_t *queues,
                 uint32_t *queues_next, KNrc N) {
@@ -1012,22 +1360,15 @@ __global__ void __launch_bounds__(256)
 
     // ---- one pass for every active lane ---------------------------------------
     if (st.pixel >= 0) {
-      // intersectAll: the closest collision over the volumes
       KRay ray;
       ray.ori = st.ori;
       ray.dir = st.dir;
       ray.tmin = 0.f;
       ray.tmax = st.tmax;
       clip_ray(ray, P);  // every pass (Pathtrace_impl.cpp:559)
-      VolHit vh;
-      vh.hit = false;
-      vh.t = FLT_MAX;
-      for (int i = 0; i < sc.num_volumes; ++i) {
-        const VolHit h = delta_track(sc.vol[i], s_tf, lds_majorants(sc, s_tf, i), steps[i],
-                                     to_local(sc.vol[i], ray), st.srng, lookups);
-        if (h.hit && h.t < vh.t) vh = h;
-      }
-      if (path_shade<NRC>(st, sc, P, N, vh)) {
+      const PassHit hit = intersect_all<AREA>(sc, ray, st.rayType != Radiance, st.bounceID, s_tf,
+                                              steps, st.srng, lookups);
+      if (path_shade<NRC, AREA>(st, sc, P, N, hit)) {
         float4 color = P.bg;
         if (st.firstHit) color = make_float4(st.intensity.x, st.intensity.y, st.intensity.z, 1.f);
         st.acc.x += color.x;
@@ -1325,6 +1666,36 @@ __global__ void __launch_bounds__(256)
     dst[i] = src[i];
 }
 
+// closest / any surface hit of n rays {ori, dir, tmin, tmax}: t (inf on a
+// miss) and the triangle's index in its mesh (-1)
+__global__ void __launch_bounds__(256)
+    k_intersect_surfaces(KScene sc, int n, const float *__restrict__ rays, int any,
+                         float *__restrict__ t_out, int *__restrict__ prim_out) {
+  const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= n) return;
+  const float *r = rays + 8 * size_t(i);
+  KRay ray;
+  ray.ori = mk(r[0], r[1], r[2]);
+  ray.dir = mk(r[3], r[4], r[5]);
+  ray.tmin = r[6];
+  ray.tmax = r[7];
+  bool hit = false;
+  float best = ray.tmax;
+  int prim = -1;
+  for (int s = 0; s < sc.num_surfaces; ++s) {
+    float t = best;
+    int p = -1;
+    uint32_t rec;
+    if (bvh_hit(sc.surf[s], ray, any != 0, t, p, rec)) {
+      hit = true;
+      best = t;
+      prim = p;
+    }
+  }
+  t_out[i] = hit ? best : __builtin_inff();
+  prim_out[i] = hit ? prim : -1;
+}
+
 __global__ void k_frame_reset(KFrame F) {
   const size_t n = size_t(F.width) * F.height;
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
@@ -1353,17 +1724,33 @@ hipError_t launch_raycast(const KScene &sc, const KCamera &cam, const KParams &P
   return hipGetLastError();
 }
 
+// scenes with surfaces or quad lights take the kernel with the BVH code
+static bool scene_has_area(const KScene &sc) {
+  if (sc.num_surfaces > 0) return true;
+  for (int i = 0; i < sc.num_lights; ++i)
+    if (sc.light[i].type == VRT_LIGHT_QUAD) return true;
+  return false;
+}
+
 hipError_t launch_pathtrace(const KScene &sc, const KCamera &cam, const KParams &P,
                             const KFrame &F, int accumID, uint32_t *queues,
                             uint32_t *queues_next, int blocks, hipStream_t s, const KNrc *nrc) {
   const size_t lds = scene_lds_bytes(sc);
+  const bool area = scene_has_area(sc);
   if (nrc) {
     hipLaunchKernelGGL(k_nrc_frame_begin, dim3(1), dim3(64), 0, s, *nrc);
-    hipLaunchKernelGGL(k_pathtrace<true>, dim3(blocks), dim3(256), lds, s, sc, cam, P, F, accumID,
-                       queues, queues_next, *nrc);
-  } else {
-    hipLaunchKernelGGL(k_pathtrace<false>, dim3(blocks), dim3(256), lds, s, sc, cam, P, F,
+    if (area)
+      hipLaunchKernelGGL((k_pathtrace<true, true>), dim3(blocks), dim3(256), lds, s, sc, cam, P, F,
+                         accumID, queues, queues_next, *nrc);
+    else
+      hipLaunchKernelGGL((k_pathtrace<true, false>), dim3(blocks), dim3(256), lds, s, sc, cam, P,
+                         F, accumID, queues, queues_next, *nrc);
+  } else if (area) {
+    hipLaunchKernelGGL((k_pathtrace<false, true>), dim3(blocks), dim3(256), lds, s, sc, cam, P, F,
                        accumID, queues, queues_next, KNrc{});
+  } else {
+    hipLaunchKernelGGL((k_pathtrace<false, false>), dim3(blocks), dim3(256), lds, s, sc, cam, P,
+                       F, accumID, queues, queues_next, KNrc{});
   }
   return hipGetLastError();
 }
@@ -1375,12 +1762,16 @@ hipError_t launch_nrc_resolve(const KFrame &F, const KNrc &N, int accumID, hipSt
   return hipGetLastError();
 }
 
-int pathtrace_blocks_per_cu(size_t lds_bytes, bool nrc) {
+int pathtrace_blocks_per_cu(const KScene &sc, size_t lds_bytes, bool nrc) {
   int n = 0;
-  hipError_t e = nrc ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pathtrace<true>, 256,
-                                                                    lds_bytes)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pathtrace<false>, 256,
-                                                                    lds_bytes);
+  const bool area = scene_has_area(sc);
+  hipError_t e;
+  if (nrc)
+    e = area ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pathtrace<true, true>, 256, lds_bytes)
+             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pathtrace<true, false>, 256, lds_bytes);
+  else
+    e = area ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pathtrace<false, true>, 256, lds_bytes)
+             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pathtrace<false, false>, 256, lds_bytes);
   if (e != hipSuccess) return 1;
   return n > 0 ? n : 1;
 }
@@ -1431,6 +1822,14 @@ hipError_t launch_majorants(const KVolume &v, const float *ranges, float *maj, u
 hipError_t launch_sample_field(const KVolume &v, int n, const float *pts, float *out,
                                hipStream_t s) {
   hipLaunchKernelGGL(k_sample_field, dim3((n + 255) / 256), dim3(256), 0, s, v, n, pts, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_intersect_surfaces(const KScene &sc, int n, const float *rays, int any,
+                                     float *t_out, int *prim_out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_intersect_surfaces, dim3((n + 255) / 256), dim3(256), 0, s, sc, n, rays, any,
+                     t_out, prim_out);
   return hipGetLastError();
 }
 

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <array>
 #include <map>
+#include <unordered_map>
 #include <thread>
 #include <vector>
 
@@ -531,5 +532,69 @@ __attribute__((visibility("default"))) int vrtgen_nanovdb_cloud(
 }
 
 __attribute__((visibility("default"))) void vrtgen_free(void *p) { std::free(p); }
+
+// Icosphere of 20 * 4^level triangles (unit icosahedron, each edge split at
+// its midpoint `level` times, vertices projected to the sphere), displaced
+// radially: r = radius * (1 + amp * f) with f the fBm (4 octaves, lacunarity
+// 2, gain 0.5) of gradient noise at the unit-sphere position * freq + 7.
+// out_vertices: (10 * 4^level + 2) float3; out_indices: 20 * 4^level uint3.
+__attribute__((visibility("default"))) int vrtgen_icosphere(int level, double radius, double amp,
+                                                            double freq, uint64_t seed,
+                                                            float *out_vertices,
+                                                            uint32_t *out_indices) {
+  if (level < 0 || level > 10) return -1;
+  const double t = (1.0 + std::sqrt(5.0)) / 2.0;
+  std::vector<double> v = {-1, t, 0, 1, t, 0, -1, -t, 0, 1, -t, 0, 0, -1, t, 0, 1, t,
+                           0, -1, -t, 0, 1, -t, t, 0, -1, t, 0, 1, -t, 0, -1, -t, 0, 1};
+  for (size_t i = 0; i < v.size(); i += 3) {
+    const double n = std::sqrt(v[i] * v[i] + v[i + 1] * v[i + 1] + v[i + 2] * v[i + 2]);
+    v[i] /= n;
+    v[i + 1] /= n;
+    v[i + 2] /= n;
+  }
+  std::vector<uint32_t> f = {0, 11, 5, 0, 5, 1,  0, 1, 7,   0, 7, 10, 0, 10, 11, 1, 5, 9,  5, 11, 4,
+                             11, 10, 2, 10, 7, 6, 7, 1, 8, 3, 9, 4, 3, 4, 2,   3, 2, 6,  3, 6, 8,
+                             3, 8, 9, 4, 9, 5,  2, 4, 11, 6, 2, 10, 8, 6, 7,  9, 8, 1};
+  for (int l = 0; l < level; ++l) {
+    std::unordered_map<uint64_t, uint32_t> mid;
+    mid.reserve(f.size());
+    auto midpoint = [&](uint32_t a, uint32_t b) {
+      const uint64_t key = a < b ? (uint64_t(a) << 32 | b) : (uint64_t(b) << 32 | a);
+      auto it = mid.find(key);
+      if (it != mid.end()) return it->second;
+      double m[3];
+      for (int k = 0; k < 3; ++k) m[k] = 0.5 * (v[3 * size_t(a) + k] + v[3 * size_t(b) + k]);
+      const double n = std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+      const uint32_t id = uint32_t(v.size() / 3);
+      for (int k = 0; k < 3; ++k) v.push_back(m[k] / n);
+      mid.emplace(key, id);
+      return id;
+    };
+    std::vector<uint32_t> g;
+    g.reserve(4 * f.size());
+    for (size_t i = 0; i < f.size(); i += 3) {
+      const uint32_t a = f[i], b = f[i + 1], c = f[i + 2];
+      const uint32_t ab = midpoint(a, b), bc = midpoint(b, c), ca = midpoint(c, a);
+      const uint32_t tri[12] = {a, ab, ca, b, bc, ab, c, ca, bc, ab, bc, ca};
+      g.insert(g.end(), tri, tri + 12);
+    }
+    f.swap(g);
+  }
+  Noise noise(seed);
+  const size_t nv = v.size() / 3;
+  for (size_t i = 0; i < nv; ++i) {
+    const double *p = &v[3 * i];
+    double fbm = 0.0, fr = freq, a = 1.0;
+    for (int o = 0; o < 4; ++o) {
+      fbm += a * noise(p[0] * fr + 7.0, p[1] * fr + 7.0, p[2] * fr + 7.0);
+      fr *= 2.0;
+      a *= 0.5;
+    }
+    const double r = radius * (1.0 + amp * fbm);
+    for (int k = 0; k < 3; ++k) out_vertices[3 * i + k] = float(p[k] * r);
+  }
+  std::memcpy(out_indices, f.data(), f.size() * sizeof(uint32_t));
+  return 0;
+}
 
 }  // extern "C"

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "abi_internal.h"
+#include "bvh_internal.h"
 #include "nanovdb_internal.h"
 #include "nrc_internal.h"
 #include "render_kernels.h"
@@ -179,9 +180,32 @@ int make_scene(const vrt_scene *s, KScene &k) {
       // Directional.cpp:33 stores -direction: the sample direction points at the light
       o.dir = -(d * (1.f / len));
       o.intensity = mk(L.color[0], L.color[1], L.color[2]) * L.intensity;
+    } else if (L.type == VRT_LIGHT_QUAD) {
+      // QuadLight::finalize (Quad.cpp:30-52), tessellated as in
+      // DeviceCopyableObjects.h:2191-2196
+      o.v1 = mk(L.position[0], L.position[1], L.position[2]);
+      o.e1 = mk(L.edge1[0], L.edge1[1], L.edge1[2]);
+      o.e2 = mk(L.edge2[0], L.edge2[1], L.edge2[2]);
+      if (!(length(o.e1) > 0.f) || !(length(o.e2) > 0.f))
+        return fail(VRT_ERROR_INVALID_ARGUMENT, "quad light has zero-length edges");
+      const f3 d = o.e1 + o.e2;
+      o.nrm = normalize(cross(o.e1, d));
+      o.a1 = 0.5f * length(cross(o.e1, d));
+      o.a2 = 0.5f * length(cross(d, o.e2));
+      o.intensity = mk(L.color[0], L.color[1], L.color[2]) * std::max(0.f, L.intensity);
     } else {
       return fail(VRT_ERROR_UNSUPPORTED, "light type %d not supported by this entry point", L.type);
     }
+  }
+  if (s->num_surfaces < 0 || s->num_surfaces > VRT_MAX_SURFACES)
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "num_surfaces %d out of range", s->num_surfaces);
+  k.num_surfaces = s->num_surfaces;
+  for (int i = 0; i < s->num_surfaces; ++i) {
+    const vrt_surface &S = s->surfaces[i];
+    if (!S.bvh) return fail(VRT_ERROR_INVALID_ARGUMENT, "surface %d has no bvh", i);
+    k.surf[i].nodes = S.bvh->nodes;
+    k.surf[i].tris = S.bvh->tris;
+    k.surf[i].color = mk(S.color[0], S.color[1], S.color[2]);
   }
   return VRT_OK;
 }
@@ -465,6 +489,18 @@ int vrt_sample_field(vrt_context ctx, const vrt_volume *vol, int n, const float 
   return VRT_OK;
 }
 
+int vrt_intersect_surfaces(vrt_context ctx, const vrt_scene *scene, int n, const float *rays,
+                           int any, float *t_out, int *prim_out) {
+  if (int r = check_ctx(ctx)) return r;
+  if (n < 0 || (n > 0 && (!rays || !t_out || !prim_out)))
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "bad intersect_surfaces arguments");
+  KScene sc;
+  if (int r = make_scene(scene, sc)) return r;
+  if (n == 0) return VRT_OK;
+  VRT_HIP(launch_intersect_surfaces(sc, n, rays, any, t_out, prim_out, ctx->stream));
+  return VRT_OK;
+}
+
 int vrt_delta_track(vrt_context ctx, const vrt_scene *scene, int n, const float *rays,
                     uint32_t *rng, float *t_out) {
   if (int r = check_ctx(ctx)) return r;
@@ -540,7 +576,7 @@ int vrt_pathtrace(vrt_context ctx, const vrt_scene *scene, const vrt_camera *cam
   if (int r = make_frame(frame, F)) return r;
   if (!F.accum) return fail(VRT_ERROR_INVALID_ARGUMENT, "pathtrace needs an accumulation buffer");
   const size_t lds = scene_lds_size(sc);
-  int blocks = std::max(1, ctx->num_cus) * pathtrace_blocks_per_cu(lds);
+  int blocks = std::max(1, ctx->num_cus) * pathtrace_blocks_per_cu(sc, lds);
   blocks = std::max(8, (blocks + 7) / 8 * 8);  // every XCD queue gets workgroups
   uint32_t *cur = ctx->queues + 8 * ctx->queue_parity;
   uint32_t *nxt = ctx->queues + 8 * (1 - ctx->queue_parity);
@@ -596,7 +632,7 @@ int vrt_pathtrace_nrc(vrt_context ctx, const vrt_scene *scene, const vrt_camera 
                    1.f / (opts->bounds_hi[2] - opts->bounds_lo[2]));
 
   const size_t lds = scene_lds_size(sc);
-  int blocks = std::max(1, ctx->num_cus) * pathtrace_blocks_per_cu(lds, true);
+  int blocks = std::max(1, ctx->num_cus) * pathtrace_blocks_per_cu(sc, lds, true);
   blocks = std::max(8, (blocks + 7) / 8 * 8);
   uint32_t *cur = ctx->queues + 8 * ctx->queue_parity;
   uint32_t *nxt = ctx->queues + 8 * (1 - ctx->queue_parity);

@@ -56,11 +56,25 @@ class Light(C.Structure):
     ]
 
 
+class Surface(C.Structure):
+    _fields_ = [("bvh", C.c_void_p), ("color", f3)]
+
+
+MAX_SURFACES = 4
+
+
 class Scene(C.Structure):
     _fields_ = [
         ("num_volumes", C.c_int), ("volumes", Volume * MAX_VOLUMES),
         ("num_lights", C.c_int), ("lights", Light * MAX_LIGHTS),
+        ("num_surfaces", C.c_int), ("surfaces", Surface * MAX_SURFACES),
     ]
+
+
+class BvhInfo(C.Structure):
+    _fields_ = [("num_nodes", C.c_uint32), ("num_triangles", C.c_uint32), ("depth", C.c_uint32),
+                ("bounds_lo", f3), ("bounds_hi", f3), ("build_ms", C.c_double),
+                ("device_bytes", C.c_uint64)]
 
 
 class Camera(C.Structure):
@@ -156,6 +170,10 @@ def lib():
         L.vrt_nanovdb_create.argtypes = [vp, vp, C.c_uint64, C.POINTER(C.c_void_p)]
         L.vrt_nanovdb_get_info.argtypes = [vp, C.c_void_p]
         L.vrt_nanovdb_destroy.argtypes = [vp]
+        L.vrt_intersect_surfaces.argtypes = [vp, C.POINTER(Scene), ip, vp, ip, vp, vp]
+        L.vrt_bvh_build.argtypes = [vp, vp, C.c_int64, vp, C.c_int64, C.POINTER(C.c_void_p)]
+        L.vrt_bvh_get_info.argtypes = [vp, C.c_void_p]
+        L.vrt_bvh_destroy.argtypes = [vp]
         L.vrt_delta_track.argtypes = [vp, C.POINTER(Scene), ip, vp, vp, vp]
         L.vrt_raycast.argtypes = [vp, C.POINTER(Scene), C.POINTER(Camera), C.POINTER(Params),
                                   C.POINTER(Frame), ip]
@@ -184,6 +202,8 @@ def gen():
                                            C.POINTER(C.c_void_p), C.POINTER(C.c_uint64),
                                            C.POINTER(C.c_double)]
         G.vrtgen_free.argtypes = [C.c_void_p]
+        G.vrtgen_icosphere.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64,
+                                       C.c_void_p, C.c_void_p]
         _gen = G
     return _gen
 
@@ -248,6 +268,18 @@ def gen_nanovdb_cloud(n=1024, octaves=6, base_freq=4.0, lacunarity=2.0, gain=0.5
                                   threads, C.byref(ptr), C.byref(nb), C.byref(tau)):
         raise VrtError("nanovdb cloud generator failed")
     return _take_gen_buffer(ptr, nb), tau.value
+
+
+def gen_icosphere(level=8, radius=0.5, amp=0.08, freq=3.0, seed=2):
+    """Noise-displaced icosphere: (vertices (n, 3) float32, indices (m, 3) uint32),
+    m = 20 * 4^level."""
+    nt = 20 * 4 ** level
+    nv = 10 * 4 ** level + 2
+    v = np.empty((nv, 3), np.float32)
+    i = np.empty((nt, 3), np.uint32)
+    if gen().vrtgen_icosphere(level, radius, amp, freq, seed, v.ctypes.data, i.ctypes.data):
+        raise VrtError("icosphere generator failed")
+    return v, i
 
 
 class DeviceBuffer:
@@ -435,7 +467,8 @@ def affine_inverse(xfm):
     return A, (-m[:, 3]).astype(np.float32)
 
 
-def make_scene(volumes, lights=()):
+def make_scene(volumes, lights=(), surfaces=()):
+    """`surfaces`: (DeviceMesh, matte color) pairs."""
     s = Scene()
     s.num_volumes = len(volumes)
     for i, v in enumerate(volumes):
@@ -443,7 +476,50 @@ def make_scene(volumes, lights=()):
     s.num_lights = len(lights)
     for i, L in enumerate(lights):
         s.lights[i] = L
+    s.num_surfaces = len(surfaces)
+    for i, (mesh, color) in enumerate(surfaces):
+        s.surfaces[i].bvh = mesh.handle
+        s.surfaces[i].color = f3(*color)
     return s
+
+
+class DeviceMesh:
+    """A triangle mesh's BVH (binned SAH, 4-wide) resident on the GPU."""
+
+    def __init__(self, ctx, vertices, indices=None):
+        v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        ip, nt = None, len(v) // 3
+        if indices is not None:
+            idx = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
+            ip, nt = idx.ctypes.data, len(idx)
+        h = C.c_void_p()
+        check(lib().vrt_bvh_build(ctx.handle, v.ctypes.data, len(v), ip, nt, C.byref(h)))
+        self.handle = h.value
+        self.info = BvhInfo()
+        check(lib().vrt_bvh_get_info(self.handle, C.byref(self.info)))
+
+    def close(self):
+        if self.handle:
+            lib().vrt_bvh_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def quad_light(position, edge1, edge2, intensity=1.0, color=(1, 1, 1), visible=True):
+    L = Light()
+    L.type = LIGHT_QUAD
+    L.visible = int(visible)
+    L.position = f3(*position)
+    L.edge1 = f3(*edge1)
+    L.edge2 = f3(*edge2)
+    L.color = f3(*color)
+    L.intensity = float(intensity)
+    return L
 
 
 def directional_light(direction, irradiance=1.0, color=(1, 1, 1), visible=True):
@@ -548,6 +624,21 @@ def sample_field(ctx, volume, points):
     dp.free()
     dv.free()
     return out
+
+
+def intersect_surfaces(ctx, scene, rays, any_hit=False):
+    """Surface hits of rays (n, 8): {ori, dir, tmin, tmax} -> (t, prim)."""
+    r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    n = len(r)
+    dr = ctx.upload(r)
+    dt, dp = ctx.buffer(max(4, 4 * n)), ctx.buffer(max(4, 4 * n))
+    check(lib().vrt_intersect_surfaces(ctx.handle, C.byref(scene), n, dr.ptr, int(any_hit),
+                                       dt.ptr, dp.ptr))
+    t = dt.download(np.float32, (n,))
+    p = dp.download(np.int32, (n,))
+    for b in (dr, dt, dp):
+        b.free()
+    return t, p
 
 
 def delta_track(ctx, scene, rays, rng_states):

@@ -44,6 +44,7 @@ class Workload:
     frames: int = 1
     nanovdb: np.ndarray = None   # NanoVDB grid buffer: a nanovdb field instead of voxels
     filter: str = "linear"       # nanovdb sampler
+    meshes: list = None          # dicts: vertices (n,3), indices (m,3) or None, color
 
     @property
     def dims(self):
@@ -139,6 +140,32 @@ def cloud_nanovdb_pathtrace(n=1024, width=1920, height=1080, frames=256, seed=3,
         width=width, height=height, renderer="pathtrace", frames=frames, nanovdb=buf)
 
 
+def mixed_pathtrace(level=8, vol_n=256, width=1920, height=1080, frames=128, max_bounce=6,
+                    seed=1, threads=0):
+    """Mixed scene: a noise-displaced icosphere of 20 * 4^level triangles
+    (level 8: 1,310,720), matte 0.7, enclosing a vol_n^3 Gaussian-blob volume
+    (32 blobs, seed 1, ramp TF) and one quad light; camera and light inside
+    the sphere; <= 6 bounces, 1 spp per frame, `frames` frames."""
+    v, i = _vrt.gen_icosphere(level=level, radius=0.5, amp=0.08, freq=3.0, seed=2)
+    vox = _vrt.gen_blobs((vol_n, vol_n, vol_n), count=32, seed=seed, threads=threads)
+    sp = 0.5 / (vol_n - 1)
+    pos = (0.08, 0.12, 0.36)
+    g = max(1, (vol_n + 15) // 16)
+    return Workload(
+        name=f"pathtrace mixed icosphere {len(i)} tris + blobs {vol_n}^3 {width}x{height}",
+        voxels=vox, origin=(-0.25,) * 3, spacing=(sp,) * 3, tf=ramp_tf(256),
+        value_range=(0.0, 1.0), unit_distance=0.05, grid_dims=(g, g, g),
+        camera=dict(position=pos, direction=look_at(pos), up=(0, 1, 0),
+                    fovy=math.radians(60.0), aspect=width / height),
+        lights=[dict(type="quad", position=(-0.08, 0.36, -0.08), edge1=(0.16, 0, 0),
+                     edge2=(0, 0, 0.16), color=(1, 1, 1), intensity=12.0, visible=True)],
+        params=dict(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.0,
+                    occlusion_distance=1e20, ambient_samples=0, max_bounce=max_bounce,
+                    pixel_samples=1),
+        width=width, height=height, renderer="pathtrace", frames=frames,
+        meshes=[dict(vertices=v, indices=i, color=(0.7, 0.7, 0.7))])
+
+
 def _grid_bounds(buf):
     """GridData::mWorldBBox of a NanoVDB buffer as float (lo xyz, hi xyz)."""
     return np.frombuffer(buf[560:608].tobytes(), np.float64).astype(np.float32)
@@ -157,6 +184,13 @@ def volume_bounds(wl):
 # ---------------------------------------------------------------------------
 # GPU side
 # ---------------------------------------------------------------------------
+def light_desc(L):
+    if L.get("type", "directional") == "quad":
+        return _vrt.quad_light(L["position"], L["edge1"], L["edge2"], L["intensity"], L["color"],
+                               L.get("visible", True))
+    return _vrt.directional_light(L["direction"], L["intensity"], L["color"],
+                                  L.get("visible", True))
+
 class GpuScene:
     """The workload resident on one GPU through the vrt C ABI."""
 
@@ -168,11 +202,14 @@ class GpuScene:
             volume = _vrt.DeviceVolume.from_nanovdb(ctx, wl.nanovdb, wl.tf, wl.value_range,
                                                     wl.unit_distance, wl.grid_dims,
                                                     filter=wl.filter)
-        self.volume = volume or _vrt.DeviceVolume(ctx, wl.voxels, wl.origin, wl.spacing, wl.tf,
-                                                  wl.value_range, wl.unit_distance, wl.grid_dims)
-        lights = [_vrt.directional_light(L["direction"], L["intensity"], L["color"])
-                  for L in wl.lights]
-        self.scene = _vrt.make_scene([self.volume], lights)
+        if volume is None and wl.voxels is not None:
+            volume = _vrt.DeviceVolume(ctx, wl.voxels, wl.origin, wl.spacing, wl.tf,
+                                       wl.value_range, wl.unit_distance, wl.grid_dims)
+        self.volume = volume
+        lights = [light_desc(L) for L in wl.lights]
+        self.meshes = [(_vrt.DeviceMesh(ctx, m["vertices"], m.get("indices")), m["color"])
+                       for m in (wl.meshes or [])]
+        self.scene = _vrt.make_scene([volume] if volume is not None else [], lights, self.meshes)
         self.camera = _vrt.camera(**wl.camera)
         p = dict(wl.params)
         p.update(params or {})

@@ -104,11 +104,26 @@ typedef struct vrt_light {
 #define VRT_MAX_LIGHTS 8
 #define VRT_MAX_CLIP_PLANES 8
 
+/* Triangle BVH (vrt_bvh_build) of one mesh; see below. */
+typedef struct vrt_bvh_s *vrt_bvh;
+
+/* A triangle surface with a matte material (Triangle.cpp + Matte.cpp:
+ * color; the reference's other material and attribute paths are not part of
+ * this build), identity instance transform. */
+typedef struct vrt_surface {
+  vrt_bvh bvh;
+  float color[3];
+} vrt_surface;
+
+#define VRT_MAX_SURFACES 4
+
 typedef struct vrt_scene {
   int num_volumes;
   vrt_volume volumes[VRT_MAX_VOLUMES];
   int num_lights;
   vrt_light lights[VRT_MAX_LIGHTS];
+  int num_surfaces;
+  vrt_surface surfaces[VRT_MAX_SURFACES];
 } vrt_scene;
 
 /* ANARI "perspective" camera (camera/Perspective.cpp:14-44,
@@ -234,6 +249,23 @@ VRT_API int vrt_nanovdb_create(vrt_context ctx, const void *grid_buffer, uint64_
 VRT_API int vrt_nanovdb_get_info(vrt_nanovdb grid, vrt_nanovdb_info *info);
 VRT_API int vrt_nanovdb_destroy(vrt_nanovdb grid);
 
+/* Triangle BVH of a mesh given as host arrays: num_vertices float3 positions
+ * and num_triangles uint3 indices (NULL: a triangle soup of 3 vertices per
+ * triangle).  Built on the host by binned SAH, collapsed to 4-wide nodes and
+ * uploaded; replaces the reference's per-geometry BVH build and upload
+ * (DeviceBVH.h:153-255, 320; Triangle.cpp). */
+typedef struct vrt_bvh_info {
+  uint32_t num_nodes, num_triangles, depth;
+  float bounds_lo[3], bounds_hi[3];
+  double build_ms;
+  uint64_t device_bytes;
+} vrt_bvh_info;
+
+VRT_API int vrt_bvh_build(vrt_context ctx, const float *vertices, int64_t num_vertices,
+                          const uint32_t *indices, int64_t num_triangles, vrt_bvh *out);
+VRT_API int vrt_bvh_get_info(vrt_bvh bvh, vrt_bvh_info *info);
+VRT_API int vrt_bvh_destroy(vrt_bvh bvh);
+
 /* Per-macrocell value ranges (float2 min/max) of a field over a
  * grid_dims grid in voxel space, conservative for trilinear sampling.
  * Replaces StructuredRegularField::buildGrid + rasterizeBox
@@ -261,6 +293,13 @@ VRT_API int vrt_sample_field(vrt_context ctx, const vrt_volume *vol, int n,
  * (device, advanced in place), t_out = hit distance or +inf (device).
  * The intersect(Ray, Volume) collision sampler of
  * DeviceCopyableObjects.h:680-796. */
+/* Closest (any = 0) or any (any = 1) surface hit of n rays {ori xyz, dir
+ * xyz, tmin, tmax} (device float[8] each): t (+inf on a miss) and the
+ * triangle's index in its mesh (-1).  The surface part of intersectAll
+ * (renderer/common.h:1606-1620), exposed for tests and tools. */
+VRT_API int vrt_intersect_surfaces(vrt_context ctx, const vrt_scene *scene, int n,
+                                   const float *rays, int any, float *t_out, int *prim_out);
+
 VRT_API int vrt_delta_track(vrt_context ctx, const vrt_scene *scene, int n,
                             const float *rays, uint32_t *rng, float *t_out);
 

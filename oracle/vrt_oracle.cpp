@@ -540,11 +540,195 @@ inline float quantize_majorant(float m, float step) {
 // ---------------------------------------------------------------------------
 // Scene wrapper
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// A6: triangles (visionaray basic_triangle {v1, e1, e2}, not vendored: parity
+// unpinned).  Moller-Trumbore with inclusive edges; a hit needs
+// tmin <= t <= tmax.  Closest-hit ties go to the lower triangle index so the
+// result does not depend on the acceleration structure.
+// ---------------------------------------------------------------------------
+template <class R> struct Tri {
+  V3<R> v1, e1, e2;
+};
+
+template <class R> struct TriHit {
+  bool hit = false;
+  R t = R(0), u = R(0), v = R(0);
+};
+
+template <class R> inline TriHit<R> intersect_tri(const Ray<R> &ray, const Tri<R> &tri) {
+  TriHit<R> h;
+  const V3<R> s1 = cross(ray.dir, tri.e2);
+  const R div = dot(s1, tri.e1);
+  if (div == R(0)) return h;
+  const R inv = R(1) / div;
+  const V3<R> d = ray.ori - tri.v1;
+  const R b1 = dot(d, s1) * inv;
+  if (b1 < R(0) || b1 > R(1)) return h;
+  const V3<R> s2 = cross(d, tri.e1);
+  const R b2 = dot(ray.dir, s2) * inv;
+  if (b2 < R(0) || b1 + b2 > R(1)) return h;
+  h.t = dot(tri.e2, s2) * inv;
+  h.u = b1;
+  h.v = b2;
+  h.hit = true;
+  return h;
+}
+
+// A5 (surfaces): a binary BVH over one mesh, median split of the centroids
+// along the longest axis, leaves of <= 4 triangles -- a builder unrelated to
+// the GPU's, so equal closest hits pin the GPU traversal.
+template <class R> struct Mesh {
+  std::vector<Tri<R>> tris;  // in input order (prim id = index)
+  V3<R> color;
+  struct Node {
+    V3<R> lo, hi;
+    int left = -1, right = -1, first = 0, count = 0;
+  };
+  std::vector<Node> nodes;
+  std::vector<int> order;
+
+  explicit Mesh(const orc_trimesh *m) {
+    color = from<R>(m->color);
+    tris.resize(size_t(m->num_triangles));
+    for (int i = 0; i < m->num_triangles; ++i) {
+      uint32_t idx[3] = {uint32_t(3 * i), uint32_t(3 * i + 1), uint32_t(3 * i + 2)};
+      if (m->indices)
+        for (int k = 0; k < 3; ++k) idx[k] = m->indices[3 * i + k];
+      // basic_triangle edges formed in float from the vertex positions
+      float v[3][3];
+      for (int k = 0; k < 3; ++k)
+        for (int a = 0; a < 3; ++a) v[k][a] = m->vertices[3 * idx[k] + a];
+      float e1[3], e2[3];
+      for (int a = 0; a < 3; ++a) {
+        e1[a] = v[1][a] - v[0][a];
+        e2[a] = v[2][a] - v[0][a];
+      }
+      tris[i] = Tri<R>{from<R>(v[0]), from<R>(e1), from<R>(e2)};
+    }
+    order.resize(tris.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = int(i);
+    if (!tris.empty()) build(0, int(tris.size()));
+  }
+  void bounds(int first, int count, V3<R> &lo, V3<R> &hi) const {
+    lo = V3<R>(std::numeric_limits<R>::max());
+    hi = V3<R>(-std::numeric_limits<R>::max());
+    for (int i = first; i < first + count; ++i) {
+      const Tri<R> &t = tris[order[i]];
+      const V3<R> p[3] = {t.v1, t.v1 + t.e1, t.v1 + t.e2};
+      for (int k = 0; k < 3; ++k)
+        for (int a = 0; a < 3; ++a) {
+          lo[a] = std::min(lo[a], p[k][a]);
+          hi[a] = std::max(hi[a], p[k][a]);
+        }
+    }
+  }
+  int build(int first, int count) {
+    const int id = int(nodes.size());
+    nodes.emplace_back();
+    V3<R> lo, hi;
+    bounds(first, count, lo, hi);
+    nodes[id].lo = lo;
+    nodes[id].hi = hi;
+    if (count <= 4) {
+      nodes[id].first = first;
+      nodes[id].count = count;
+      return id;
+    }
+    const V3<R> ext = hi - lo;
+    const int axis = ext.x >= ext.y && ext.x >= ext.z ? 0 : (ext.y >= ext.z ? 1 : 2);
+    auto centre = [&](int i) {
+      const Tri<R> &t = tris[i];
+      return t.v1[axis] * R(3) + t.e1[axis] + t.e2[axis];
+    };
+    const int mid = first + count / 2;
+    std::nth_element(order.begin() + first, order.begin() + mid, order.begin() + first + count,
+                     [&](int a, int b) { return centre(a) < centre(b); });
+    const int l = build(first, mid - first);
+    const int r = build(mid, first + count - mid);
+    nodes[id].left = l;
+    nodes[id].right = r;
+    return id;
+  }
+  static bool box_hit(const Ray<R> &ray, V3<R> lo, V3<R> hi, R tmin, R tmax) {
+    for (int a = 0; a < 3; ++a) {
+      const R inv = R(1) / ray.dir[a];
+      R t0 = (lo[a] - ray.ori[a]) * inv, t1 = (hi[a] - ray.ori[a]) * inv;
+      if (t0 > t1) std::swap(t0, t1);
+      if (!(t0 <= t1)) {  // NaN: the origin lies on a slab plane of a zero component
+        if (ray.ori[a] < lo[a] || ray.ori[a] > hi[a]) return false;
+        continue;
+      }
+      tmin = std::max(tmin, t0);
+      tmax = std::min(tmax, t1);
+    }
+    // slack for rounding of the box itself
+    return tmin <= tmax * R(1.0000002) + R(1e-30);
+  }
+  // closest hit in [tmin, tmax] (any hit when `any`); prim = triangle index
+  bool intersect(const Ray<R> &ray, bool any, R &t_best, int &prim, R &u, R &v) const {
+    if (nodes.empty()) return false;
+    bool found = false;
+    t_best = ray.tmax;
+    int stack[128], sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+      const Node &n = nodes[stack[--sp]];
+      if (!box_hit(ray, n.lo, n.hi, ray.tmin, t_best)) continue;
+      if (n.count) {
+        for (int i = n.first; i < n.first + n.count; ++i) {
+          const int p = order[i];
+          const TriHit<R> h = intersect_tri(ray, tris[p]);
+          if (!h.hit || h.t < ray.tmin || h.t > t_best) continue;
+          if (found && h.t == t_best && p > prim) continue;
+          found = true;
+          t_best = h.t;
+          prim = p;
+          u = h.u;
+          v = h.v;
+          if (any) return true;
+        }
+      } else {
+        stack[sp++] = n.left;
+        stack[sp++] = n.right;
+      }
+    }
+    return found;
+  }
+};
+
+// A17: quad light (DeviceCopyableObjects.h:2187-2273): two triangles
+// {v1, e1, e1+e2} and {v1, e1+e2, e2}
+template <class R> struct QuadLight {
+  Tri<R> t1, t2;
+  V3<R> intensity;  // color * intensity (area_light::intensity, parity unpinned)
+  R A1 = R(0), A2 = R(0);
+  explicit QuadLight(const orc_light &L) {
+    const V3<R> v1 = from<R>(L.pos), e1 = from<R>(L.e1), e2 = from<R>(L.e2);
+    t1 = Tri<R>{v1, e1, e1 + e2};
+    t2 = Tri<R>{v1, e1 + e2, e2};
+    intensity = from<R>(L.color) * R(L.intensity);
+    A1 = R(0.5) * length(cross(t1.e1, t1.e2));
+    A2 = R(0.5) * length(cross(t2.e1, t2.e2));
+  }
+  V3<R> normal() const { return normalize(cross(t1.e1, t1.e2)); }
+  // intersect(Ray, Quad): the closer of the two triangles, t > 0
+  TriHit<R> intersect(const Ray<R> &ray) const {
+    TriHit<R> h1 = intersect_tri(ray, t1), h2 = intersect_tri(ray, t2), r;
+    if (h1.hit && h1.t > R(0)) r = h1;
+    if (h2.hit && h2.t > R(0) && (!r.hit || h2.t < r.t)) r = h2;
+    return r;
+  }
+};
+
 template <class R> struct Scene {
   std::vector<Volume<R>> vols;
+  std::vector<Mesh<R>> meshes;
+  std::vector<QuadLight<R>> quads;  // per light (unused entries for other types)
   const orc_scene *s;
   explicit Scene(const orc_scene *sc) : s(sc) {
     for (int i = 0; i < sc->num_volumes; ++i) vols.emplace_back(&sc->volumes[i]);
+    for (int i = 0; i < sc->num_meshes; ++i) meshes.emplace_back(&sc->meshes[i]);
+    for (int i = 0; i < sc->num_lights; ++i) quads.emplace_back(sc->lights[i]);
   }
   bool empty() const { return s->num_volumes == 0 && s->num_meshes == 0; }
 };
@@ -559,8 +743,30 @@ template <class R> struct LightSample {
 // stored direction (Directional.cpp:33 stores -direction), pdf 1, intensity
 // color*irradiance, distance FLT_MAX, and draws no random numbers
 // (parity unpinned).
-template <class R> LightSample<R> sample_light(const orc_light &L, V3<R> /*hitPos*/, Random &) {
+template <class R>
+LightSample<R> sample_light(const orc_light &L, const QuadLight<R> &Q, V3<R> hitPos, Random &rnd) {
   LightSample<R> ls;
+  if (L.type == ORC_LIGHT_QUAD) {
+    // area_light<float, Quad>::sample (visionaray, not vendored: parity
+    // unpinned): sample_surface(Quad) picks a triangle with one draw
+    // (DeviceCopyableObjects.h:2258-2270), then a uniform point with two
+    // (sqrt parametrisation).  pdf = 1 / (A |cos theta_light|): with the
+    // 1/dist^2 of sampleLight (common.h:1455-1463) the NEE estimate
+    // f Le cos / (dist^2 pdf) is then the area-sampling estimator.
+    const R r = R(rnd());
+    const Tri<R> &t = (Q.A1 / (Q.A1 + Q.A2) < r) ? Q.t1 : Q.t2;
+    const R u1 = R(rnd()), u2 = R(rnd());
+    const R su = std::sqrt(u1);
+    const V3<R> v2 = t.v1 + t.e1, v3 = t.v1 + t.e2;
+    const V3<R> pos = t.v1 * (R(1) - su) + v2 * (su * (R(1) - u2)) + v3 * (su * u2);
+    ls.dir = pos - hitPos;
+    ls.dist = length(ls.dir);
+    const R cosl = std::fabs(dot(Q.normal(), ls.dir * (R(1) / ls.dist)));
+    ls.pdf = cosl > R(0) ? R(1) / ((Q.A1 + Q.A2) * cosl) : R(0);
+    ls.intensity = Q.intensity;
+    ls.dist2 = ls.dist * ls.dist;
+    return ls;
+  }
   V3<R> d = normalize(from<R>(L.dir));
   ls.dir = -d;
   ls.dist = R(std::numeric_limits<float>::max());
@@ -571,9 +777,11 @@ template <class R> LightSample<R> sample_light(const orc_light &L, V3<R> /*hitPo
 }
 
 template <class R> struct HitRec {
-  // surface part reserved for triangle meshes
   bool surf_hit = false;
   R surf_t = std::numeric_limits<R>::max();
+  int surf_mesh = -1, surf_prim = -1;
+  R surf_u = R(0), surf_v = R(0);
+  int light_id = -1;
   VolHit<R> vol;
   int vol_inst = -1;
   bool light_hit = false;
@@ -583,10 +791,38 @@ template <class R> struct HitRec {
 };
 
 template <class R>
-HitRec<R> intersect_all(const Scene<R> &sc, const Ray<R> &ray, Random &rnd, Counter *cnt) {
+HitRec<R> intersect_all(const Scene<R> &sc, const Ray<R> &ray, Random &rnd, Counter *cnt,
+                        unsigned bounceID = 0, bool shadow = false) {
   HitRec<R> hr;
-  // intersectSurfaces<1> (no meshes in the volume configurations)
-  // intersectLights: directional lights are never hit (common.h:1566-1603)
+  // intersectSurfaces<1>: closest hit, any hit for shadow / AO rays
+  for (size_t m = 0; m < sc.meshes.size(); ++m) {
+    R t, u, v;
+    int prim = -1;
+    Ray<R> r = ray;
+    if (hr.surf_hit) r.tmax = hr.surf_t;
+    if (sc.meshes[m].intersect(r, shadow, t, prim, u, v)) {
+      hr.surf_hit = true;
+      hr.surf_t = t;
+      hr.surf_mesh = int(m);
+      hr.surf_prim = prim;
+      hr.surf_u = u;
+      hr.surf_v = v;
+    }
+  }
+  // intersectLights (common.h:1566-1603): quads, skipping invisible lights
+  // for camera rays; directional lights are never hit
+  for (int i = 0; i < sc.s->num_lights; ++i) {
+    const orc_light &L = sc.s->lights[i];
+    if (bounceID == 0 && !L.visible) continue;
+    if (L.type != ORC_LIGHT_QUAD) continue;
+    const TriHit<R> h = sc.quads[i].intersect(ray);
+    if (h.hit && h.t < hr.light_t) {
+      hr.light_hit = true;
+      hr.light_t = h.t;
+      hr.light_visible = L.visible != 0;
+      hr.light_id = i;
+    }
+  }
   // sampleFreeFlightDistanceAllVolumes -> intersectVolumes, closest wins
   for (size_t i = 0; i < sc.vols.size(); ++i) {
     Ray<R> lr = sc.vols[i].to_local(ray);
@@ -639,9 +875,17 @@ template <class R> struct ShadeState {
   R vis_light = 1, vis_ao = 1;
 };
 
+// power_heuristic (visionaray, parity unpinned): f^2 / (f^2 + g^2), 0 when
+// both vanish
+template <class R> inline R power_heuristic(R f, R g) {
+  const R d = f * f + g * g;
+  return d > R(0) ? (f * f) / d : R(0);
+}
+
 template <class R>
 void shade(Random &srnd, const Ray<R> &ray, RayType rayType, const Scene<R> &sc,
-           const orc_params *P, const HitRec<R> &hitRec, ShadeState<R> &st) {
+           const orc_params *P, const HitRec<R> &hitRec, ShadeState<R> &st,
+           unsigned bounceID = 0) {
   st.nextType = Miss;
   if (rayType == Radiance) {
     st.baseColor = V3<R>(R(0));
@@ -655,7 +899,26 @@ void shade(Random &srnd, const Ray<R> &ray, RayType rayType, const Scene<R> &sc,
       st.nextType = Miss;
       return;
     }
-    // (light hits only occur with quad lights; not in volume scenes)
+    const int numLights = sc.s->num_lights;
+    if (hitRec.lightHit) {
+      // Pathtrace_impl.cpp:139-183: the quad's intensity, MIS-weighted
+      // against the BSDF sample after the first bounce
+      const QuadLight<R> &Q = sc.quads[hitRec.light_id];
+      const V3<R> hitPos = ray.ori + ray.dir * hitRec.light_t;
+      st.shadedColor = Q.intensity;
+      st.misWeightNEE = R(1);
+      if (bounceID > 0) {
+        const R A = Q.A1 + Q.A2;
+        const R ld = length(hitPos - ray.ori);
+        const V3<R> L = normalize(hitPos - ray.ori);
+        const R LdotNl = std::fabs(dot(-L, Q.normal()));
+        const R solidAngle = (LdotNl * A) / (ld * ld);
+        const R lightPDF = R(1) / solidAngle;
+        st.shadedColor = st.shadedColor * power_heuristic(st.bs_pdf, lightPDF / R(numLights));
+      }
+      st.nextType = Miss;
+      return;
+    }
     V3<R> hitPos, gn, sn, color;
     R eps;
     V3<R> viewDir = -normalize(ray.dir);
@@ -674,26 +937,49 @@ void shade(Random &srnd, const Ray<R> &ray, RayType rayType, const Scene<R> &sc,
       gn = faceforward(gn, viewDir, gn);
       color = hitRec.vol.albedo;
     } else {
-      // surfaces: not present in the volume configurations
+      // triangle surface (Pathtrace_impl.cpp:234-274): Ng = normalize(e1 x
+      // e2), no vertex normals (getNormals, common.h:233-264), matte color
+      const Mesh<R> &M = sc.meshes[hitRec.surf_mesh];
+      const Tri<R> &T = M.tris[hitRec.surf_prim];
       hitPos = ray.ori + ray.dir * hitRec.surf_t;
       eps = epsilon_from(hitPos, ray.dir, hitRec.surf_t);
-      sn = gn = V3<R>(R(0), R(0), R(1));
-      color = V3<R>(R(0));
+      gn = normalize(cross(T.e1, T.e2));
+      sn = gn;
+      sn = faceforward(sn, viewDir, gn);
+      gn = faceforward(gn, viewDir, gn);
+      color = M.color;
     }
     st.sn = sn;
     st.hitPosOff = hitPos + sn * eps;
 
     int lightID = -1;
-    const int numLights = sc.s->num_lights;
     if (numLights > 0) {
       // uniformSampleOneLight, renderer/common.h:165-169
       int which = int(srnd() * float(numLights));
       if (which == numLights) which = 0;
       lightID = which;
-      st.ls = sample_light<R>(sc.s->lights[lightID], hitPos, srnd);
+      st.ls = sample_light<R>(sc.s->lights[lightID], sc.quads[lightID], hitPos, srnd);
     }
     auto safe_rcp = [](R f) { return f > R(0) ? R(1) / f : R(0); };
     V3<R> lightIntensity = st.ls.intensity * safe_rcp(st.ls.dist2);
+    if (!hitRec.volumeHit) {
+      // matte (common.h:1033-1048, 1371-1413): f = color / pi, cosine
+      // hemisphere around faceforward(Ns, viewDir, Ng), pdf = cos / pi
+      const R inv_pi = R(0.318309886183790671538);
+      const V3<R> f = color * inv_pi;
+      const V3<R> lightDir = normalize(st.ls.dir);
+      const R NdotL = std::max(R(0), dot(sn, lightDir));
+      st.shadedColor = f * lightIntensity * NdotL * safe_rcp(st.ls.pdf) * R(numLights);
+      const V3<R> w = faceforward(sn, viewDir, gn);
+      V3<R> u, v;
+      make_onb(w, u, v);
+      const R u1 = R(srnd()), u2 = R(srnd());
+      const V3<R> sp = cosine_sample_hemisphere(u1, u2);
+      st.bs_dir = normalize(u * sp.x + v * sp.y + w * sp.z);
+      st.bs_pdf = std::max(R(0), dot(sn, st.bs_dir)) * inv_pi;
+      st.bs_cosT = std::max(R(0), dot(sn, st.bs_dir));
+      st.bs_f = f;
+    }
     if (hitRec.volumeHit) {
       // Pathtrace_impl.cpp:330-339: no phase factor, no light count
       st.shadedColor = hitRec.vol.albedo * lightIntensity * safe_rcp(st.ls.pdf) *
@@ -705,7 +991,14 @@ void shade(Random &srnd, const Ray<R> &ray, RayType rayType, const Scene<R> &sc,
       st.bs_cosT = R(1);
     }
     st.misWeightNEE = R(0);
-    if (numLights > 0) st.misWeightNEE = R(1);  // delta light (:377-379)
+    if (numLights > 0) {
+      // MIS against the BSDF sample for area lights, 1 for delta lights
+      // (:370-380)
+      if (sc.s->lights[lightID].type == ORC_LIGHT_QUAD)
+        st.misWeightNEE = power_heuristic(st.ls.pdf / R(numLights), st.bs_pdf);
+      else
+        st.misWeightNEE = R(1);
+    }
     st.baseColor = color;
   }
 
@@ -788,8 +1081,9 @@ V4<R> pathtrace_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params
       V3<R> throughput(R(1)), intensity(R(0));
       for (unsigned passID = 0, bounceID = 0;; ++passID) {
         clip_ray(ray, P);  // Pathtrace_impl.cpp:559
-        HitRec<R> hitRec = intersect_all(sc, ray, srnd, cnt);
-        shade(srnd, ray, rayType, sc, P, hitRec, st);
+        const bool shadow = rayType == Shadow || rayType == AO;
+        HitRec<R> hitRec = intersect_all(sc, ray, srnd, cnt, bounceID, shadow);
+        shade(srnd, ray, rayType, sc, P, hitRec, st, bounceID);
         if (passID == 0 && bounceID == 0) {
           firstHit = hitRec.hit;
           firstLightInvisible = hitRec.lightHit && !hitRec.light_visible;
@@ -1306,5 +1600,59 @@ uint32_t orc_cvt_uint32_srgb(const float v[4]) {
 }
 
 float orc_pow_ieee(float x, float y) { return orc::pow_ieee(x, y); }
+
+}  // extern "C"
+
+extern "C" {
+
+void orc_mesh_intersect(const orc_trimesh *m, int n, const float *rays, int any, int brute,
+                        float *t_out, int *prim_out) {
+  orc::Mesh<float> mesh(m);
+  for (int i = 0; i < n; ++i) {
+    const float *r = rays + 8 * size_t(i);
+    orc::Ray<float> ray;
+    ray.ori = orc::V3<float>(r[0], r[1], r[2]);
+    ray.dir = orc::V3<float>(r[3], r[4], r[5]);
+    ray.tmin = r[6];
+    ray.tmax = r[7];
+    float t = std::numeric_limits<float>::infinity(), u, v;
+    int prim = -1;
+    bool hit = false;
+    if (brute) {
+      float best = ray.tmax;
+      for (size_t k = 0; k < mesh.tris.size(); ++k) {
+        const orc::TriHit<float> h = orc::intersect_tri(ray, mesh.tris[k]);
+        if (!h.hit || h.t < ray.tmin || h.t > best) continue;
+        if (hit && h.t == best && int(k) > prim) continue;
+        hit = true;
+        best = h.t;
+        prim = int(k);
+        if (any) break;
+      }
+      if (hit) t = best;
+    } else {
+      float tb;
+      hit = mesh.intersect(ray, any != 0, tb, prim, u, v);
+      if (hit) t = tb;
+    }
+    t_out[i] = hit ? t : std::numeric_limits<float>::infinity();
+    prim_out[i] = hit ? prim : -1;
+  }
+}
+
+void orc_quad_light_sample(const orc_light *L, const float ref[3], int n, uint32_t seed,
+                           float *pos_out, float *pdf_out) {
+  orc::QuadLight<float> Q(*L);
+  const orc::V3<float> p(ref[0], ref[1], ref[2]);
+  for (int i = 0; i < n; ++i) {
+    orc::Random rnd(seed, uint32_t(i));
+    const orc::LightSample<float> ls = orc::sample_light<float>(*L, Q, p, rnd);
+    const orc::V3<float> pos = p + ls.dir;
+    pos_out[3 * i] = pos.x;
+    pos_out[3 * i + 1] = pos.y;
+    pos_out[3 * i + 2] = pos.z;
+    pdf_out[i] = ls.pdf;
+  }
+}
 
 }  // extern "C"

@@ -163,6 +163,15 @@ uint32_t orc_cvt_uint32_srgb(const float rgba[4]);
 float orc_pow_ieee(float x, float y);
 
 /* camera ray for pixel coordinate (x, y) (parity unpinned: ANARI spec) */
+/* --- A5/A6: closest (any=0) or any (any=1) hit of n rays {ori, dir, tmin,
+   tmax} against a mesh, through the oracle's BVH or by brute force; t = inf
+   and prim = -1 on a miss --- */
+void orc_mesh_intersect(const orc_trimesh *m, int n, const float *rays, int any, int brute,
+                        float *t_out, int *prim_out);
+/* --- A17: n quad-light samples seen from ref (RNG seeded (seed, i)):
+   positions and pdfs --- */
+void orc_quad_light_sample(const orc_light *L, const float ref[3], int n, uint32_t seed,
+                           float *pos_out, float *pdf_out);
 void orc_camera_ray(const orc_camera *cam, float x, float y, int width,
                     int height, float *ori, float *dir);
 

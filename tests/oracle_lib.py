@@ -87,6 +87,8 @@ def lib():
         L.orc_cvt_uint32_srgb.restype = C.c_uint32
         L.orc_cvt_uint32_srgb.argtypes = [vp]
         L.orc_camera_ray.argtypes = [C.POINTER(OrcCamera), C.c_float, C.c_float, ip, ip, vp, vp]
+        L.orc_mesh_intersect.argtypes = [C.POINTER(OrcTrimesh), ip, vp, ip, ip, vp, vp]
+        L.orc_quad_light_sample.argtypes = [C.POINTER(OrcLight), vp, ip, C.c_uint32, vp, vp]
         L.orc_nvdb_get.argtypes = [vp, ip, vp, vp]
         L.orc_nvdb_sample.argtypes = [vp, ip, vp, ip, vp]
         L.orc_nvdb_world_bbox.argtypes = [vp, vp]
@@ -231,10 +233,27 @@ class OracleVolume:
         return out
 
 
+class OracleMesh:
+    """A matte triangle mesh (float3 vertices, uint3 indices or a soup)."""
+
+    def __init__(self, vertices, indices=None, color=(0.7, 0.7, 0.7)):
+        self.v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        self.i = None if indices is None else np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
+        m = OrcTrimesh()
+        m.vertices = self.v.ctypes.data
+        m.indices = None if self.i is None else self.i.ctypes.data
+        m.num_triangles = len(self.v) // 3 if self.i is None else len(self.i)
+        m.color = f3(*color)
+        m.opacity = 1.0
+        self.s = m
+
+
 class OracleScene:
-    def __init__(self, volumes, lights=()):
+    def __init__(self, volumes, lights=(), meshes=()):
         self.volumes = list(volumes)
+        self.meshes = list(meshes)
         self.varr = (OrcVolume * max(1, len(self.volumes)))(*[v.s for v in self.volumes])
+        self.marr = (OrcTrimesh * max(1, len(self.meshes)))(*[m.s for m in self.meshes])
         self.larr = (OrcLight * max(1, len(lights)))()
         for i, L in enumerate(lights):
             self.larr[i] = L
@@ -243,8 +262,21 @@ class OracleScene:
         s.volumes = C.cast(self.varr, C.POINTER(OrcVolume))
         s.num_lights = len(lights)
         s.lights = C.cast(self.larr, C.POINTER(OrcLight))
-        s.num_meshes = 0
+        s.num_meshes = len(self.meshes)
+        s.meshes = C.cast(self.marr, C.POINTER(OrcTrimesh))
         self.s = s
+
+
+def light_quad(position, edge1, edge2, intensity=1.0, color=(1, 1, 1), visible=True):
+    L = OrcLight()
+    L.type = 1
+    L.visible = int(visible)
+    L.pos = f3(*position)
+    L.e1 = f3(*edge1)
+    L.e2 = f3(*edge2)
+    L.color = f3(*color)
+    L.intensity = max(0.0, intensity)
+    return L
 
 
 def light_directional(direction, intensity=1.0, color=(1, 1, 1), visible=True):
@@ -293,17 +325,26 @@ class OracleWorkload:
 
     def __init__(self, wl):
         self.wl = wl
+        self.vol = None
         if wl.nanovdb is not None:
             self.vol = OracleVolume.from_nanovdb(wl.nanovdb, wl.tf, wl.value_range,
                                                  wl.unit_distance, wl.filter)
-        else:
+        elif wl.voxels is not None:
             self.vol = OracleVolume(wl.voxels, wl.origin, wl.spacing, wl.tf, wl.value_range,
                                     wl.unit_distance)
-        if wl.grid_dims is not None:
+        if self.vol is not None and wl.grid_dims is not None:
             self.vol.attach_majorants(wl.grid_dims)
-        lights = [light_directional(L["direction"], L["intensity"], L["color"])
-                  for L in wl.lights]
-        self.scene = OracleScene([self.vol], lights)
+        lights = []
+        for L in wl.lights:
+            if L.get("type", "directional") == "quad":
+                lights.append(light_quad(L["position"], L["edge1"], L["edge2"], L["intensity"],
+                                         L["color"], L.get("visible", True)))
+            else:
+                lights.append(light_directional(L["direction"], L["intensity"], L["color"],
+                                                L.get("visible", True)))
+        self.meshes = [OracleMesh(m["vertices"], m.get("indices"), m["color"])
+                       for m in (wl.meshes or [])]
+        self.scene = OracleScene([self.vol] if self.vol is not None else [], lights, self.meshes)
         self.cam = camera(**wl.camera)
         self.params = params(**wl.params)
 
@@ -381,3 +422,21 @@ def nrc_encode(raw):
     out = np.empty(128, np.float64)
     L.orc_nrc_encode(r.ctypes.data, out.ctypes.data)
     return out
+
+
+def mesh_intersect(mesh, rays, any_hit=False, brute=False):
+    r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    t = np.empty(len(r), np.float32)
+    p = np.empty(len(r), np.int32)
+    lib().orc_mesh_intersect(C.byref(mesh.s), len(r), r.ctypes.data, int(any_hit), int(brute),
+                             t.ctypes.data, p.ctypes.data)
+    return t, p
+
+
+def quad_light_sample(light, ref, n, seed=1):
+    ref = np.ascontiguousarray(ref, np.float32)
+    pos = np.empty((n, 3), np.float32)
+    pdf = np.empty(n, np.float32)
+    lib().orc_quad_light_sample(C.byref(light), ref.ctypes.data, n, seed, pos.ctypes.data,
+                                pdf.ctypes.data)
+    return pos, pdf
