@@ -93,8 +93,9 @@ def test_nanovdb_pathtrace_parity(pkg, oracle, gpu_ctx):
         gpu_ctx.sync()
         g = gs.frame.color_array()
         o = (np.float32(1) / np.float32(accum_id + 1)) * ow.pathtrace_frame(accum_id)[0]
-        close = np.all(np.abs(g - o) <= 1e-4 * np.maximum(1, np.abs(o)), axis=-1)
-        assert close.mean() >= 0.9, close.mean()
+        agree = oracle.stream_agreement(g, o)
+        print("nanovdb stream agreement", accum_id, agree)
+        assert agree >= oracle.STREAM_AGREEMENT_MIN, agree
     frames = 32
     gs.frame.reset()
     for a in range(frames):

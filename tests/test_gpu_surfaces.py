@@ -62,8 +62,9 @@ def test_mixed_pathtrace_parity(pkg, oracle, gpu_ctx):
         gpu_ctx.sync()
         g = gs.frame.color_array()
         o = (np.float32(1) / np.float32(accum_id + 1)) * ow.pathtrace_frame(accum_id)[0]
-        close = np.all(np.abs(g - o) <= 1e-4 * np.maximum(1, np.abs(o)), axis=-1)
-        assert close.mean() >= 0.9, close.mean()
+        agree = oracle.stream_agreement(g, o)
+        print("mixed stream agreement", accum_id, agree)
+        assert agree >= oracle.STREAM_AGREEMENT_MIN, agree
     frames = 32
     gs.frame.reset()
     for a in range(frames):
@@ -91,5 +92,6 @@ def test_quad_light_on_volume(pkg, oracle, gpu_ctx):
     gpu_ctx.sync()
     g = gs.frame.color_array()
     o = ow.pathtrace_frame(0)[0]
-    close = np.all(np.abs(g - o) <= 1e-4 * np.maximum(1, np.abs(o)), axis=-1)
-    assert close.mean() >= 0.9, close.mean()
+    agree = oracle.stream_agreement(g, o)
+    print("quad light stream agreement", agree)
+    assert agree >= oracle.STREAM_AGREEMENT_MIN, agree

@@ -108,6 +108,47 @@ def test_matte_closed_forms(oracle):
     np.testing.assert_allclose(out[..., :3], np.broadcast_to(exp, out[..., :3].shape), rtol=2e-6)
 
 
+def _rect_form_factor(x1, x2, z1, z2, h):
+    """Form factor from a differential area at the origin (normal +y) to the
+    rectangle [x1, x2] x [z1, z2] in the plane y = h: the corner-rectangle
+    formula (Howell's catalogue, differential element to a parallel
+    rectangle) combined by inclusion-exclusion; it is odd in each side."""
+    def corner(X, Z):
+        a, b = math.sqrt(1 + X * X), math.sqrt(1 + Z * Z)
+        return (X / a * math.atan(Z / a) + Z / b * math.atan(X / b)) / (2 * math.pi)
+    return (corner(x2 / h, z2 / h) - corner(x1 / h, z2 / h) - corner(x2 / h, z1 / h)
+            + corner(x1 / h, z1 / h))
+
+
+def test_ambient_occlusion_form_factor(oracle):
+    """AO rays leave a matte point in a cosine-weighted hemisphere, so the
+    chance that one hits a parallel rectangle above it is that rectangle's
+    form factor F (0.162 here; a uniform hemisphere would give 0.126).  The
+    mean AO-lit value is color * ambient * (1 - F) within 4 binomial sigma."""
+    c = np.array([0.6, 0.5, 0.4])
+    h, x1, x2, z1, z2 = 0.5, 0.25, 1.25, -0.5, 0.5
+    plane = oracle.OracleMesh(np.array([[-5, 0, -5], [5, 0, -5], [5, 0, 5], [-5, 0, 5]], np.float32),
+                              np.array([[0, 2, 1], [0, 3, 2]], np.uint32), c)
+    occ = oracle.OracleMesh(np.array([[x1, h, z1], [x2, h, z1], [x2, h, z2], [x1, h, z2]], np.float32),
+                            np.array([[0, 1, 2], [0, 2, 3]], np.uint32), (0.7, 0.7, 0.7))
+    scene = oracle.OracleScene([], [], [plane, occ])
+    # a near-zero field of view: every camera ray lands within 1e-3 of the origin
+    cam = oracle.camera((0, 1, 0), (0, -1, 0), (0, 0, -1), 1e-3, 1.0)
+    prm = oracle.params(ambient_radiance=1.0, ambient_samples=1, max_bounce=0)
+    W = H = 32
+    frames = 40
+    out = np.zeros((H, W, 4), np.float64)
+    oracle.lib().orc_pathtrace_accumulate(oracle.C.byref(scene.s), oracle.C.byref(cam),
+                                          oracle.C.byref(prm), W, H, 0, frames, 0, 1,
+                                          out.ctypes.data, 4)
+    F = _rect_form_factor(x1, x2, z1, z2, h)
+    n = W * H * frames
+    vis = out[..., :3].mean(axis=(0, 1)) / c
+    sigma = math.sqrt(F * (1 - F) / n)
+    print("AO visibility", vis, "expected", 1 - F)
+    assert np.all(np.abs(vis - (1 - F)) <= 4 * sigma), (vis, 1 - F, sigma)
+
+
 def test_visible_quad_light_closed_form(oracle):
     """Camera rays that hit a visible quad light return its intensity (no MIS
     at the first vertex); an invisible one leaves the background."""
