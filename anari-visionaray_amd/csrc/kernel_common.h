@@ -66,6 +66,16 @@ struct KVolume {
   int nv_lo[3], nv_n[3];
   float nv_inv[9], nv_vec[3];
   int filter;
+  // unstructured: point-location grid (list start per grid cell, dims
+  // product + 1 entries; tetrahedron copies of four float4 {x, y, z, value}
+  // in list order), its origin, cells per unit length and dims; per
+  // tetrahedron {bbox min, value min}, {bbox max, value max}
+  const uint32_t *ug_start;
+  const float4 *ug_recs;
+  const float4 *ug_tets;
+  uint32_t ug_num_tets;
+  f3 ug_lo, ug_inv;
+  int ug_dims[3];
 };
 
 struct KLight {

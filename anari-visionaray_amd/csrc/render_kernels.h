@@ -31,6 +31,9 @@ hipError_t launch_nanovdb_slot_ranges(const int32_t *slots, size_t nslots, const
 // lo xyz, hi xyz inclusive) over the slot ranges
 hipError_t launch_nanovdb_value_ranges(const KVolume &v, const int32_t *cell_voxels,
                                        float *ranges, hipStream_t s);
+// unstructured: cell value ranges rasterized from the tetrahedra's bounding
+// boxes and value ranges (cells without one keep the empty range)
+hipError_t launch_tet_value_ranges(const KVolume &v, float *ranges, hipStream_t s);
 hipError_t launch_majorants(const KVolume &v, const float *ranges, float *maj, uint8_t *bytes,
                             float *step, hipStream_t s);
 // LDS bytes a render kernel needs for the scene's tables and majorant grids

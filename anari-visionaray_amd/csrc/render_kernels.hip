@@ -241,9 +241,63 @@ __device__ inline float sample_nanovdb(const KVolume &v, f3 P) {
   return lerpf(lerpf(v00, v01, wy), lerpf(v10, v11, wy), wx);
 }
 
-// sampleField: voxel-space point -> field value
-__device__ inline float sample_field(const KVolume &v, f3 P) {
-  return v.field_type == VRT_FIELD_NANOVDB ? sample_nanovdb(v, P) : sample_structured(v, P);
+// ---------------------------------------------------------------------------
+// Unstructured field fetch (sampleField, DeviceCopyableObjects.h:482-527).
+// ---------------------------------------------------------------------------
+// intersectTet (Plane.h:56-96) with its arithmetic and test order: vertices
+// relative to P, the plane through three of them (N = cross(b-a, c-a),
+// d = dot(a, N)), the ratio eval(0)/eval(opposite vertex) as the barycentric
+// weight; each weight must be >= 0 and the first <= 1.  With no data at the
+// first vertex the value is the cell value, kept in the second vertex's w
+// (DeviceCopyableObjects.h:150-153).
+__device__ inline float plane_ratio(f3 a, f3 b, f3 c, f3 opp) {
+  const f3 n = cross(b - a, c - a);
+  const float d = dot(a, n);
+  return (dot(mk(0.f), n) - d) / (dot(opp, n) - d);
+}
+
+__device__ inline bool tet_value(f3 P, float4 A, float4 B, float4 C, float4 D, float &value) {
+  const f3 va = mk(A.x, A.y, A.z) - P, vb = mk(B.x, B.y, B.z) - P;
+  const f3 vc = mk(C.x, C.y, C.z) - P, vd = mk(D.x, D.y, D.z) - P;
+  const float fa = plane_ratio(vb, vd, vc, va);
+  if (fa < 0.f || fa > 1.f) return false;
+  const float fb = plane_ratio(va, vc, vd, vb);
+  if (fb < 0.f) return false;
+  const float fc = plane_ratio(va, vd, vb, vc);
+  if (fc < 0.f) return false;
+  const float fd = plane_ratio(va, vb, vc, vd);
+  if (fd < 0.f) return false;
+  value = __builtin_isnan(A.w) ? B.w : fa * A.w + fb * B.w + fc * C.w + fd * D.w;
+  return true;
+}
+
+// Points inside the field bounds, faces included (the element BVH's root box
+// test is inclusive), map to the grid cell of P (float arithmetic the list
+// construction allows for, the top face clamped into the last cell); its
+// tetrahedra are tested in list order until one contains P.
+__device__ inline bool sample_tets(const KVolume &v, f3 P, float &value) {
+  if (!(P.x >= v.blo.x && P.x <= v.bhi.x && P.y >= v.blo.y && P.y <= v.bhi.y &&
+        P.z >= v.blo.z && P.z <= v.bhi.z))
+    return false;  // also on NaN
+  // the grid origin is the bounds' lower corner, so q >= 0
+  const uint32_t ix = min(uint32_t((P.x - v.ug_lo.x) * v.ug_inv.x), uint32_t(v.ug_dims[0] - 1));
+  const uint32_t iy = min(uint32_t((P.y - v.ug_lo.y) * v.ug_inv.y), uint32_t(v.ug_dims[1] - 1));
+  const uint32_t iz = min(uint32_t((P.z - v.ug_lo.z) * v.ug_inv.z), uint32_t(v.ug_dims[2] - 1));
+  const uint32_t c = (iz * uint32_t(v.ug_dims[1]) + iy) * uint32_t(v.ug_dims[0]) + ix;
+  const uint32_t b = v.ug_start[c], e = v.ug_start[c + 1];
+  for (uint32_t i = b; i < e; ++i) {
+    const float4 *r = v.ug_recs + size_t(i) * 4u;
+    if (tet_value(P, r[0], r[1], r[2], r[3], value)) return true;
+  }
+  return false;
+}
+
+// sampleField: voxel-space point -> field value; false where the field has
+// no value (outside every cell of an unstructured field)
+__device__ inline bool sample_field(const KVolume &v, f3 P, float &value) {
+  if (v.field_type == VRT_FIELD_UNSTRUCTURED) return sample_tets(v, P, value);
+  value = v.field_type == VRT_FIELD_NANOVDB ? sample_nanovdb(v, P) : sample_structured(v, P);
+  return true;
 }
 
 // postClassify (DeviceCopyableObjects.h:615-619): tex1D(linear, clamp) of the
@@ -594,14 +648,18 @@ __device__ inline bool trav_step(const KVolume &v, const float4 *lds, const uint
       T.in_cell = false;
       advance = true;
     } else {
-      const float val = sample_field(v, T.ori + T.dir * T.t);
+      float val;
       ++lookups;
-      const float4 s = classify(v, lds, val);
-      if (s.w >= rnd(rs) * T.maj) {
-        hit = true;
-        t_hit = T.t;
-        smp = s;
-        return true;
+      // no acceptance draw where the field has no value
+      // (DeviceCopyableObjects.h:739-753)
+      if (sample_field(v, T.ori + T.dir * T.t, val)) {
+        const float4 s = classify(v, lds, val);
+        if (s.w >= rnd(rs) * T.maj) {
+          hit = true;
+          t_hit = T.t;
+          smp = s;
+          return true;
+        }
       }
     }
   }
@@ -794,8 +852,9 @@ __global__ void __launch_bounds__(256)
         int step = 0;
         for (float t = tmin; t <= tmax && alpha < 0.99f && step < max_steps; t += dt, ++step) {
           const f3 p = ori + dir * t;
-          const float val = sample_field(v, p);
+          float val;
           ++lookups;
+          if (!sample_field(v, p, val)) continue;  // VolumeIntegration.h:412
           const float4 s = classify(v, s_tf, val);
           const float stepT = pow_ieee(1.f - s.w, expo);
           const float k = trans * (1.f - stepT);
@@ -1528,6 +1587,61 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x == 0) ranges[mc] = make_float2(smin[0], smax[0]);
 }
 
+// Atomic min / max of floats through their order-preserving integer images:
+// non-negative floats order like their int bits, negative ones inversely to
+// their unsigned bits.  -0 is folded into +0 first; NaN is ignored.
+__device__ inline void atomic_min_float(float *a, float v) {
+  if (v != v) return;
+  v += 0.f;
+  if (v >= 0.f)
+    atomicMin(reinterpret_cast<int *>(a), __float_as_int(v));
+  else
+    atomicMax(reinterpret_cast<unsigned int *>(a), __float_as_uint(v));
+}
+__device__ inline void atomic_max_float(float *a, float v) {
+  if (v != v) return;
+  v += 0.f;
+  if (v >= 0.f)
+    atomicMax(reinterpret_cast<int *>(a), __float_as_int(v));
+  else
+    atomicMin(reinterpret_cast<unsigned int *>(a), __float_as_uint(v));
+}
+
+__global__ void __launch_bounds__(256) k_empty_ranges(float2 *__restrict__ r, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i] = make_float2(FLT_MAX, -FLT_MAX);
+}
+
+// unstructured: rasterizeBox of every tetrahedron's bounding box with its
+// value range (UnstructuredField.cpp:387-399, GridAccel.h:51-67), the cells
+// from projectOnGrid (GridAccel-common.h:18-24) over the grid bounds.  The
+// reference's updateMC is a plain read-modify-write (a race when run in
+// parallel); atomics make the result independent of the schedule.
+__device__ inline int project_on_grid(float p, float lo, float hi, int G) {
+  const float c = fminf(fmaxf((p - lo) / (hi - lo) * float(G), -1.f), float(G));
+  return min(max(int(c), 0), G - 1);
+}
+
+__global__ void __launch_bounds__(256) k_tet_value_ranges(KVolume v, float2 *__restrict__ ranges) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= v.ug_num_tets) return;
+  const float4 lo = v.ug_tets[2 * size_t(t)], hi = v.ug_tets[2 * size_t(t) + 1];
+  const int G0 = v.gdims[0], G1 = v.gdims[1], G2 = v.gdims[2];
+  const int x0 = project_on_grid(lo.x, v.glo.x, v.ghi.x, G0);
+  const int x1 = project_on_grid(hi.x, v.glo.x, v.ghi.x, G0);
+  const int y0 = project_on_grid(lo.y, v.glo.y, v.ghi.y, G1);
+  const int y1 = project_on_grid(hi.y, v.glo.y, v.ghi.y, G1);
+  const int z0 = project_on_grid(lo.z, v.glo.z, v.ghi.z, G2);
+  const int z1 = project_on_grid(hi.z, v.glo.z, v.ghi.z, G2);
+  for (int z = z0; z <= z1; ++z)
+    for (int y = y0; y <= y1; ++y)
+      for (int x = x0; x <= x1; ++x) {
+        float2 *r = ranges + (size_t(z) * G1 + y) * G0 + x;
+        atomic_min_float(&r->x, lo.w);
+        atomic_max_float(&r->y, hi.w);
+      }
+}
+
 // nanovdb: value range of every block slot (a leaf's 512 values, or its
 // constant)
 __global__ void __launch_bounds__(256)
@@ -1635,7 +1749,10 @@ __global__ void k_sample_field(KVolume v, int n, const float *__restrict__ pts,
                                float *__restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  out[i] = sample_field(v, mk(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
+  float val;
+  out[i] = sample_field(v, mk(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]), val)
+               ? val
+               : __builtin_nanf("");
 }
 
 __global__ void __launch_bounds__(256)
@@ -1808,6 +1925,17 @@ hipError_t launch_nanovdb_value_ranges(const KVolume &v, const int32_t *cell_vox
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_nvdb_value_ranges, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, v, n,
                      cell_voxels, reinterpret_cast<float2 *>(ranges));
+  return hipGetLastError();
+}
+
+hipError_t launch_tet_value_ranges(const KVolume &v, float *ranges, hipStream_t s) {
+  const int n = v.gdims[0] * v.gdims[1] * v.gdims[2];
+  if (n <= 0) return hipSuccess;
+  float2 *r = reinterpret_cast<float2 *>(ranges);
+  hipLaunchKernelGGL(k_empty_ranges, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, r, n);
+  if (v.ug_num_tets)
+    hipLaunchKernelGGL(k_tet_value_ranges, dim3((v.ug_num_tets + 255u) / 256u), dim3(256), 0, s,
+                       v, r);
   return hipGetLastError();
 }
 

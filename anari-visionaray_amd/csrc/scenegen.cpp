@@ -597,4 +597,85 @@ __attribute__((visibility("default"))) int vrtgen_icosphere(int level, double ra
   return 0;
 }
 
+
+// Tetrahedral mesh of a jittered lattice: (n+1)^3 vertices over [-0.5, 0.5]^3,
+// vertex (i, j, k) at -0.5 + (i + d)/n per axis with d ~ U[-jitter, jitter]
+// for interior lattice lines and d = 0 on the boundary planes (so the mesh
+// fills the cube), three draws per vertex in x-fastest vertex order.  Each
+// of the n^3 cubes is split into the 6 tetrahedra around its main diagonal
+// (c0, c0+e_p0, c0+e_p0+e_p1, c0+(1,1,1) for the 6 axis permutations p),
+// which matches across shared faces.  Vertex values: `blob_count` Gaussian
+// blobs as vrtgen_blobs (seed + 1), at the jittered positions.  Outputs:
+// positions float3, values float, index uint32 x 4 per tetrahedron (cube
+// order x-fastest, permutations in lexicographic order).  Returns -1 if a
+// tetrahedron's volume falls below 1% of the undistorted one (jitter too
+// large), else 0.
+__attribute__((visibility("default"))) int vrtgen_tet_lattice(int n, double jitter,
+                                                               int blob_count, uint64_t seed,
+                                                               float *positions, float *values,
+                                                               uint32_t *index, int threads) {
+  if (n < 1 || jitter < 0.0 || jitter >= 0.5) return -1;
+  const int m = n + 1;
+  const size_t nv = size_t(m) * m * m;
+  SplitMix64 r(seed);
+  for (size_t v = 0; v < nv; ++v) {
+    const int c[3] = {int(v % m), int((v / m) % m), int(v / (size_t(m) * m))};
+    for (int a = 0; a < 3; ++a) {
+      const double d = (2.0 * r.uniform() - 1.0) * jitter;
+      const double q = (c[a] == 0 || c[a] == n) ? double(c[a]) : double(c[a]) + d;
+      positions[3 * v + a] = float(-0.5 + q / n);
+    }
+  }
+  SplitMix64 rb(seed + 1);
+  std::vector<double> cx(blob_count), cy(blob_count), cz(blob_count), s2(blob_count),
+      amp(blob_count);
+  for (int i = 0; i < blob_count; ++i) {
+    cx[i] = rb.uniform() - 0.5;
+    cy[i] = rb.uniform() - 0.5;
+    cz[i] = rb.uniform() - 0.5;
+    const double sigma = 0.03 + 0.09 * rb.uniform();
+    s2[i] = 1.0 / (2.0 * sigma * sigma);
+    amp[i] = 0.2 + 0.8 * rb.uniform();
+  }
+  parallel_slices(m, threads, [&](int z) {
+    for (size_t v = size_t(z) * m * m; v < size_t(z + 1) * m * m; ++v) {
+      const double px = positions[3 * v], py = positions[3 * v + 1], pz = positions[3 * v + 2];
+      double s = 0.0;
+      for (int i = 0; i < blob_count; ++i) {
+        const double dx = px - cx[i], dy = py - cy[i], dz = pz - cz[i];
+        s += amp[i] * std::exp(-(dx * dx + dy * dy + dz * dz) * s2[i]);
+      }
+      values[v] = float(std::min(std::max(s, 0.0), 1.0));
+    }
+  });
+  static const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  std::atomic<int> bad{0};
+  const double vmin = 0.01 / (6.0 * double(n) * n * n);
+  parallel_slices(n, threads, [&](int k) {
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        const size_t cube = (size_t(k) * n + j) * n + i;
+        for (int t = 0; t < 6; ++t) {
+          int c[3] = {i, j, k};
+          uint32_t *out = index + 4 * (6 * cube + t);
+          out[0] = uint32_t((size_t(c[2]) * m + c[1]) * m + c[0]);
+          for (int s = 0; s < 2; ++s) {
+            ++c[perm[t][s]];
+            out[s + 1] = uint32_t((size_t(c[2]) * m + c[1]) * m + c[0]);
+          }
+          out[3] = uint32_t((size_t(k + 1) * m + (j + 1)) * m + (i + 1));
+          double e[3][3];
+          for (int s = 0; s < 3; ++s)
+            for (int a = 0; a < 3; ++a)
+              e[s][a] = double(positions[3 * out[s + 1] + a]) - double(positions[3 * out[0] + a]);
+          const double det = e[0][0] * (e[1][1] * e[2][2] - e[1][2] * e[2][1]) -
+                             e[0][1] * (e[1][0] * e[2][2] - e[1][2] * e[2][0]) +
+                             e[0][2] * (e[1][0] * e[2][1] - e[1][1] * e[2][0]);
+          if (std::fabs(det) / 6.0 < vmin) bad.fetch_add(1);
+        }
+      }
+  });
+  return bad.load() ? -1 : 0;
+}
+
 }  // extern "C"

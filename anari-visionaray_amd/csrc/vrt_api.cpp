@@ -15,6 +15,7 @@
 #include "nanovdb_internal.h"
 #include "nrc_internal.h"
 #include "render_kernels.h"
+#include "unstructured_internal.h"
 #include "vrt.h"
 
 namespace vrt {
@@ -104,8 +105,34 @@ int make_nanovdb_volume(const vrt_volume &in, int lds_offset, KVolume &v) {
   return volume_common(in, lds_offset, v);
 }
 
+// UnstructuredField (UnstructuredField.cpp:280-285): identity voxel-space
+// transform, bounds = union of the cells' boxes, cell size = the running
+// average diagonal; the majorant grid spans the bounds
+int make_unstructured_volume(const vrt_volume &in, int lds_offset, KVolume &v) {
+  const vrt_unstructured_s *f = in.unstructured;
+  if (!f) return fail(VRT_ERROR_INVALID_ARGUMENT, "unstructured volume has no field");
+  v = KVolume{};
+  v.field_type = VRT_FIELD_UNSTRUCTURED;
+  v.blo = mk(f->blo[0], f->blo[1], f->blo[2]);
+  v.bhi = mk(f->bhi[0], f->bhi[1], f->bhi[2]);
+  v.S = mk(1.f);
+  v.T = mk(0.f);
+  v.glo = v.blo;
+  v.ghi = v.bhi;
+  v.cell_size = f->cell_size;
+  v.ug_start = f->start;
+  v.ug_recs = f->recs;
+  v.ug_tets = f->tets;
+  v.ug_num_tets = uint32_t(f->num_tets);
+  v.ug_lo = mk(f->lo[0], f->lo[1], f->lo[2]);
+  v.ug_inv = mk(f->inv_cell[0], f->inv_cell[1], f->inv_cell[2]);
+  for (int a = 0; a < 3; ++a) v.ug_dims[a] = f->dims[a];
+  return volume_common(in, lds_offset, v);
+}
+
 int make_volume(const vrt_volume &in, int lds_offset, KVolume &v) {
   if (in.field_type == VRT_FIELD_NANOVDB) return make_nanovdb_volume(in, lds_offset, v);
+  if (in.field_type == VRT_FIELD_UNSTRUCTURED) return make_unstructured_volume(in, lds_offset, v);
   if (in.field_type != VRT_FIELD_STRUCTURED_REGULAR)
     return fail(VRT_ERROR_UNSUPPORTED, "field type %d not supported by this entry point",
                 in.field_type);
@@ -457,6 +484,10 @@ int vrt_build_value_ranges(vrt_context ctx, const vrt_volume *vol, float *ranges
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     (void)hipFree(d);
     VRT_HIP(e);
+    return VRT_OK;
+  }
+  if (v.field_type == VRT_FIELD_UNSTRUCTURED) {
+    VRT_HIP(launch_tet_value_ranges(v, ranges, ctx->stream));
     return VRT_OK;
   }
   VRT_HIP(launch_value_ranges(v, ranges, ctx->stream));

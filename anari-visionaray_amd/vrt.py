@@ -41,7 +41,7 @@ class Volume(C.Structure):
     _fields_ = [
         ("field_type", C.c_int), ("dims", i3), ("bounds_lo", f3), ("bounds_hi", f3),
         ("cell_size", C.c_float), ("bricks", C.c_void_p), ("nanovdb", C.c_void_p),
-        ("filter", C.c_int),
+        ("filter", C.c_int), ("unstructured", C.c_void_p),
         ("tf_rgba", C.c_void_p), ("tf_size", C.c_int), ("value_lo", C.c_float),
         ("value_hi", C.c_float), ("unit_distance", C.c_float), ("majorants", C.c_void_p),
         ("grid_dims", i3), ("majorant_bytes", C.c_void_p), ("majorant_step", C.c_void_p),
@@ -170,6 +170,10 @@ def lib():
         L.vrt_nanovdb_create.argtypes = [vp, vp, C.c_uint64, C.POINTER(C.c_void_p)]
         L.vrt_nanovdb_get_info.argtypes = [vp, C.c_void_p]
         L.vrt_nanovdb_destroy.argtypes = [vp]
+        L.vrt_unstructured_create.argtypes = [vp, vp, vp, C.c_int64, vp, C.c_int64, vp, vp, vp,
+                                              C.c_int64, C.POINTER(C.c_void_p)]
+        L.vrt_unstructured_get_info.argtypes = [vp, C.c_void_p]
+        L.vrt_unstructured_destroy.argtypes = [vp]
         L.vrt_intersect_surfaces.argtypes = [vp, C.POINTER(Scene), ip, vp, ip, vp, vp]
         L.vrt_bvh_build.argtypes = [vp, vp, C.c_int64, vp, C.c_int64, C.POINTER(C.c_void_p)]
         L.vrt_bvh_get_info.argtypes = [vp, C.c_void_p]
@@ -204,6 +208,8 @@ def gen():
         G.vrtgen_free.argtypes = [C.c_void_p]
         G.vrtgen_icosphere.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64,
                                        C.c_void_p, C.c_void_p]
+        G.vrtgen_tet_lattice.argtypes = [C.c_int, C.c_double, C.c_int, C.c_uint64, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_int]
         _gen = G
     return _gen
 
@@ -280,6 +286,32 @@ def gen_icosphere(level=8, radius=0.5, amp=0.08, freq=3.0, seed=2):
     if gen().vrtgen_icosphere(level, radius, amp, freq, seed, v.ctypes.data, i.ctypes.data):
         raise VrtError("icosphere generator failed")
     return v, i
+
+
+def splitmix_uniform(seed, n):
+    """n uniform doubles in [0, 1) from SplitMix64(seed)."""
+    out = np.empty(n, np.float64)
+    gen().vrtgen_splitmix_uniform(seed, n, out.ctypes.data)
+    return out
+
+
+def gen_tet_lattice(n=70, jitter=0.15, blob_count=32, seed=5, threads=0):
+    """Jittered lattice of n^3 cubes over [-0.5, 0.5]^3, 6 tetrahedra per cube,
+    Gaussian-blob vertex values: (positions (V, 3) float32, values (V,) float32,
+    index (T, 4) uint32), V = (n+1)^3, T = 6 n^3."""
+    nv, nt = (n + 1) ** 3, 6 * n ** 3
+    pos = np.empty((nv, 3), np.float32)
+    val = np.empty(nv, np.float32)
+    idx = np.empty((nt, 4), np.uint32)
+    if gen().vrtgen_tet_lattice(n, jitter, blob_count, seed, pos.ctypes.data, val.ctypes.data,
+                                idx.ctypes.data, threads):
+        raise VrtError("tetrahedral lattice generator failed (degenerate cells)")
+    return pos, val, idx
+
+
+def tet_cells(num_tets):
+    """cell.index / cell.type arrays of a tetrahedra-only index buffer."""
+    return (np.arange(num_tets, dtype=np.uint32) * 4, np.full(num_tets, 10, np.uint8))
 
 
 class DeviceBuffer:
@@ -382,9 +414,69 @@ class DeviceNanoVDB:
             pass
 
 
+class UnstructuredInfo(C.Structure):
+    _fields_ = [("bounds_lo", f3), ("bounds_hi", f3), ("cell_size", C.c_float),
+                ("num_cells", C.c_uint64), ("skipped_cells", C.c_uint64), ("grid_dims", i3),
+                ("list_entries", C.c_uint64), ("max_list", C.c_uint32),
+                ("mean_list", C.c_double), ("expected_tests", C.c_double),
+                ("build_ms", C.c_double), ("device_bytes", C.c_uint64)]
+
+
+class DeviceUnstructured:
+    """An unstructured (tetrahedral) field in the GPU point-location layout,
+    from the field's ANARI arrays."""
+
+    def __init__(self, ctx, positions, index, cell_index=None, cell_type=None,
+                 vertex_data=None, cell_data=None):
+        pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        idx = np.ascontiguousarray(index, np.uint32).ravel()
+        if cell_index is None:
+            cell_index, cell_type = tet_cells(len(idx) // 4)
+        ci = np.ascontiguousarray(cell_index, np.uint32)
+        ct = np.ascontiguousarray(cell_type, np.uint8)
+        vd = None if vertex_data is None else np.ascontiguousarray(vertex_data, np.float32)
+        cd = None if cell_data is None else np.ascontiguousarray(cell_data, np.float32)
+        h = C.c_void_p()
+        check(lib().vrt_unstructured_create(
+            ctx.handle, pos.ctypes.data, None if vd is None else vd.ctypes.data, len(pos),
+            idx.ctypes.data, len(idx), ci.ctypes.data, ct.ctypes.data,
+            None if cd is None else cd.ctypes.data, len(ci), C.byref(h)))
+        self.handle = h.value
+        self.info = UnstructuredInfo()
+        check(lib().vrt_unstructured_get_info(self.handle, C.byref(self.info)))
+
+    def close(self):
+        if self.handle:
+            lib().vrt_unstructured_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class DeviceVolume:
-    """A structuredRegular or nanovdb field + transferFunction1D volume resident
-    on the GPU (field data, TF table, value ranges and majorant grid)."""
+    """A structuredRegular, nanovdb or unstructured field + transferFunction1D
+    volume resident on the GPU (field data, TF table, value ranges and
+    majorant grid)."""
+
+    @classmethod
+    def from_unstructured(cls, ctx, field, tf_rgba, value_range, unit_distance, grid_dims=None,
+                          xfm=None):
+        """`field`: a DeviceUnstructured."""
+        self = cls.__new__(cls)
+        self.field = field
+        v = Volume()
+        v.field_type = FIELD_UNSTRUCTURED
+        v.unstructured = field.handle
+        v.bounds_lo = field.info.bounds_lo
+        v.bounds_hi = field.info.bounds_hi
+        v.cell_size = field.info.cell_size
+        self.dims = None
+        self._finish(ctx, v, tf_rgba, value_range, unit_distance, grid_dims, xfm)
+        return self
 
     @classmethod
     def from_nanovdb(cls, ctx, grid, tf_rgba, value_range, unit_distance, grid_dims=None,

@@ -8,6 +8,9 @@ oracle consume the same object, so parity runs see bit-identical inputs.
 blobs_raycast  -- raycast, 128^3 Gaussian blobs, 256-entry ramp TF, 512^2
 fbm_pathtrace  -- pathtrace, 512^3 fBm, density scale 50, 32^3 majorant grid
                   (16^3-voxel macrocells), directional light + ambient, 1080p
+cloud_nanovdb_pathtrace -- pathtrace, 1024^3 sparse fBm cloud as a NanoVDB grid
+mixed_pathtrace -- pathtrace, 1.3 M-triangle icosphere + blob volume + quad light
+tets_pathtrace -- pathtrace, 2 M tetrahedra (jittered 70^3 lattice), 4K
 """
 import dataclasses
 import math
@@ -45,6 +48,8 @@ class Workload:
     nanovdb: np.ndarray = None   # NanoVDB grid buffer: a nanovdb field instead of voxels
     filter: str = "linear"       # nanovdb sampler
     meshes: list = None          # dicts: vertices (n,3), indices (m,3) or None, color
+    tets: dict = None            # unstructured field: positions (V,3), index (T,4),
+                                 # vertex_data (V,) [, cell_index, cell_type, cell_data]
 
     @property
     def dims(self):
@@ -166,6 +171,55 @@ def mixed_pathtrace(level=8, vol_n=256, width=1920, height=1080, frames=128, max
         meshes=[dict(vertices=v, indices=i, color=(0.7, 0.7, 0.7))])
 
 
+def _orbit_camera(seed, radius, height, width, aspect_h, fovy_deg=40.0):
+    """Camera on a circle of `radius` around the y axis at `height`, at a
+    seeded angle, looking at the origin."""
+    u = _vrt.splitmix_uniform(seed, 1)[0]
+    a = 2.0 * math.pi * u
+    pos = (radius * math.cos(a), height, radius * math.sin(a))
+    return dict(position=pos, direction=look_at(pos), up=(0, 1, 0),
+                fovy=math.radians(fovy_deg), aspect=width / aspect_h)
+
+
+def tets_pathtrace(n=70, width=3840, height=2160, frames=64, jitter=0.15, seed=5,
+                   density_scale=40.0, albedo=0.8, max_bounce=8, grid=64, camera_seed=11,
+                   threads=0):
+    """Pathtrace an unstructured field: a jittered lattice of n^3 cubes over
+    [-0.5, 0.5]^3 split into 6 n^3 tetrahedra (n = 70: 2,058,000) with
+    Gaussian-blob vertex values (32 blobs); TF rgb = albedo, alpha = value,
+    unitDistance = 1/density_scale; the reference's 64^3 majorant grid over
+    the bounds; directional light + ambient; seeded orbit camera."""
+    pos, val, idx = _vrt.gen_tet_lattice(n, jitter, 32, seed, threads)
+    return Workload(
+        name=f"pathtrace tets {len(idx)} {width}x{height}", voxels=None, origin=None,
+        spacing=None, tf=albedo_tf(albedo, 256), value_range=(0.0, 1.0),
+        unit_distance=1.0 / density_scale, grid_dims=(grid, grid, grid),
+        camera=_orbit_camera(camera_seed, 1.9, 0.7, width, height),
+        lights=[dict(type="directional", direction=(-0.4, -1.0, -0.3), color=(1, 1, 1),
+                     intensity=1.0)],
+        params=dict(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.5,
+                    occlusion_distance=1e20, ambient_samples=1, max_bounce=max_bounce,
+                    pixel_samples=1),
+        width=width, height=height, renderer="pathtrace", frames=frames,
+        tets=dict(positions=pos, index=idx, vertex_data=val))
+
+
+def tets_raycast(n=12, width=128, height=96, jitter=0.15, seed=5, threads=0):
+    """Raycast the tetrahedral lattice field (ramp TF, step = cell size / 2)."""
+    pos, val, idx = _vrt.gen_tet_lattice(n, jitter, 32, seed, threads)
+    p = (0.0, 0.0, 2.5)
+    return Workload(
+        name=f"raycast tets {len(idx)} {width}x{height}", voxels=None, origin=None,
+        spacing=None, tf=ramp_tf(256), value_range=(0.0, 1.0), unit_distance=1.0 / n,
+        grid_dims=None,
+        camera=dict(position=p, direction=look_at(p), up=(0, 1, 0),
+                    fovy=math.radians(40.0), aspect=width / height),
+        lights=[],
+        params=dict(background=(0, 0, 0, 1), volume_sampling_rate=2.0, pixel_samples=1),
+        width=width, height=height, renderer="raycast",
+        tets=dict(positions=pos, index=idx, vertex_data=val))
+
+
 def _grid_bounds(buf):
     """GridData::mWorldBBox of a NanoVDB buffer as float (lo xyz, hi xyz)."""
     return np.frombuffer(buf[560:608].tobytes(), np.float64).astype(np.float32)
@@ -176,6 +230,9 @@ def volume_bounds(wl):
     if wl.nanovdb is not None:
         bb = _grid_bounds(wl.nanovdb)
         return bb[:3], bb[3:]
+    if wl.tets is not None:
+        p = wl.tets["positions"][wl.tets["index"].ravel()]
+        return p.min(axis=0), p.max(axis=0)
     lo = np.array(wl.origin, np.float32)
     hi = lo + (np.array(wl.dims, np.float32) - 1) * np.array(wl.spacing, np.float32)
     return lo, hi
@@ -202,6 +259,13 @@ class GpuScene:
             volume = _vrt.DeviceVolume.from_nanovdb(ctx, wl.nanovdb, wl.tf, wl.value_range,
                                                     wl.unit_distance, wl.grid_dims,
                                                     filter=wl.filter)
+        if volume is None and wl.tets is not None:
+            t = wl.tets
+            field = _vrt.DeviceUnstructured(ctx, t["positions"], t["index"], t.get("cell_index"),
+                                            t.get("cell_type"), t.get("vertex_data"),
+                                            t.get("cell_data"))
+            volume = _vrt.DeviceVolume.from_unstructured(ctx, field, wl.tf, wl.value_range,
+                                                         wl.unit_distance, wl.grid_dims)
         if volume is None and wl.voxels is not None:
             volume = _vrt.DeviceVolume(ctx, wl.voxels, wl.origin, wl.spacing, wl.tf,
                                        wl.value_range, wl.unit_distance, wl.grid_dims)

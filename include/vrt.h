@@ -67,6 +67,9 @@ typedef struct vrt_volume {
      then come from the grid and dims / bricks are unused */
   struct vrt_nanovdb_s *nanovdb;
   int filter;          /* VRT_FILTER_*: nanovdb "filter" (NanoVDBField.cpp:27) */
+  /* unstructured: the tetrahedral field (vrt_unstructured_create); bounds
+     and cell size then come from the field, voxel space = world space */
+  struct vrt_unstructured_s *unstructured;
   /* transfer function: tf_size float4 entries, value range, unitDistance */
   const float *tf_rgba;
   int tf_size;
@@ -266,6 +269,55 @@ VRT_API int vrt_bvh_build(vrt_context ctx, const float *vertices, int64_t num_ve
 VRT_API int vrt_bvh_get_info(vrt_bvh bvh, vrt_bvh_info *info);
 VRT_API int vrt_bvh_destroy(vrt_bvh bvh);
 
+/* Unstructured (tetrahedral) field: UnstructuredField::commitParameters /
+ * finalize (UnstructuredField.cpp:17-198).  The host arrays are the field's
+ * ANARI parameters: "vertex.position" (float3 x num_vertices), "vertex.data"
+ * (float x num_vertices, or NULL), "index" (uint32 x num_indices),
+ * "cell.index" (uint32 x num_cells: first index of each cell), "cell.type"
+ * (uint8 x num_cells: 10 = VTK tetrahedron, 0 = its barney code) and
+ * "cell.data" (float x num_cells, or NULL).  vertex.data or cell.data must be
+ * given.  A cell's value at a point inside it is the barycentric blend of its
+ * vertex data (intersectTet, Plane.h:56-96); when its first vertex has no
+ * data (NaN, or no vertex.data) it is the cell's cell.data value (0 without
+ * one) (DeviceCopyableObjects.h:128-164).  Cells with fewer than 4 indices
+ * or a type/count mismatch are skipped, as the reference skips them; cells
+ * of other types (pyramid, wedge, hexahedron) are rejected with
+ * VRT_ERROR_UNSUPPORTED, as are indices outside the vertex array.
+ *
+ * Point location replaces the element BVH's any-hit query
+ * (DeviceCopyableObjects.h:482-527) with a uniform grid over the bounds
+ * (about one grid cell per tetrahedron) whose cells list a copy of every
+ * tetrahedron overlapping them, most-covering first: a lookup tests the
+ * listed tetrahedra in order and takes the first that contains the point.
+ * Bounds = the union of the cells' bounding boxes (the BVH root,
+ * UnstructuredField.cpp:295-332); cell_size = the reference's running
+ * "avgCellDiagonal", which divides each cell's diagonal by the number of
+ * cells kept so far (UnstructuredField.cpp:120-170, 281). */
+typedef struct vrt_unstructured_s *vrt_unstructured;
+
+typedef struct vrt_unstructured_info {
+  float bounds_lo[3], bounds_hi[3];
+  float cell_size;
+  uint64_t num_cells;       /* tetrahedra kept */
+  uint64_t skipped_cells;   /* cells skipped as invalid */
+  int grid_dims[3];         /* point-location grid */
+  uint64_t list_entries;    /* tetrahedron copies over all grid cells */
+  uint32_t max_list;
+  double mean_list;
+  double expected_tests;    /* estimated tetrahedron tests per lookup */
+  double build_ms;          /* host build of the grid */
+  uint64_t device_bytes;
+} vrt_unstructured_info;
+
+VRT_API int vrt_unstructured_create(vrt_context ctx, const float *vertex_position,
+                                    const float *vertex_data, int64_t num_vertices,
+                                    const uint32_t *index, int64_t num_indices,
+                                    const uint32_t *cell_index, const uint8_t *cell_type,
+                                    const float *cell_data, int64_t num_cells,
+                                    vrt_unstructured *out);
+VRT_API int vrt_unstructured_get_info(vrt_unstructured field, vrt_unstructured_info *info);
+VRT_API int vrt_unstructured_destroy(vrt_unstructured field);
+
 /* Per-macrocell value ranges (float2 min/max) of a field over a
  * grid_dims grid in voxel space, conservative for trilinear sampling.
  * Replaces StructuredRegularField::buildGrid + rasterizeBox
@@ -282,9 +334,11 @@ VRT_API int vrt_compute_majorants(vrt_context ctx, const vrt_volume *vol,
                                   const float *ranges, float *majorants,
                                   uint8_t *bytes, float *step);
 
-/* Trilinear field values at n voxel-space points (device float3 array ->
- * device float array): the sampleField structuredRegular branch
- * (DeviceCopyableObjects.h:478-481) on the brick layout. */
+/* Field values at n voxel-space points (device float3 array -> device
+ * float array): sampleField (DeviceCopyableObjects.h:474-570) -- trilinear
+ * on the brick layout, the NanoVDB sampler, or the containing tetrahedron's
+ * value; a point outside every tetrahedron (sampleField returns false) gives
+ * NaN. */
 VRT_API int vrt_sample_field(vrt_context ctx, const vrt_volume *vol, int n,
                              const float *points, float *values);
 

@@ -168,12 +168,184 @@ template <class R> inline R epsilon_from(V3<R> P, V3<R> dir, R t) {
 }
 
 // ---------------------------------------------------------------------------
+// A12: unstructured field of tetrahedra (UnstructuredField.cpp:29-198,
+// sampleField DeviceCopyableObjects.h:482-527, intersectTet Plane.h:56-96)
+// ---------------------------------------------------------------------------
+// intersectTet: vertices relative to P; makePlane(a, b, c) has N = cross(b-a,
+// c-a), d = dot(a, N), eval(v) = dot(v, N) - d; the weight of a vertex is
+// eval(0)/eval(vertex) of the plane through the other three; the tests are
+// fa in [0,1], then fb, fc, fd >= 0 (the reference checks "fa > 1" again
+// where "fb > 1" etc. would be expected, Plane.h:81-89)
+template <class R> inline R tet_plane_ratio(V3<R> a, V3<R> b, V3<R> c, V3<R> opp) {
+  const V3<R> n = cross(b - a, c - a);
+  const R d = dot(a, n);
+  return (dot(V3<R>(R(0)), n) - d) / (dot(opp, n) - d);
+}
+
+template <class R>
+inline bool intersect_tet(R &value, V3<R> P, V3<R> a, R wa, V3<R> b, R wb, V3<R> c, R wc,
+                          V3<R> d, R wd) {
+  const V3<R> va = a - P, vb = b - P, vc = c - P, vd = d - P;
+  const R fa = tet_plane_ratio(vb, vd, vc, va);
+  if (fa < R(0) || fa > R(1)) return false;
+  const R fb = tet_plane_ratio(va, vc, vd, vb);
+  if (fb < R(0) || fa > R(1)) return false;
+  const R fc = tet_plane_ratio(va, vd, vb, vc);
+  if (fc < R(0) || fa > R(1)) return false;
+  const R fd = tet_plane_ratio(va, vb, vc, vd);
+  if (fd < R(0) || fa > R(1)) return false;
+  value = fa * wa + fb * wb + fc * wc + fd * wd;
+  return true;
+}
+
+struct TetField {
+  struct Cell {
+    uint32_t v[4];
+    float cell_value;
+    float lo[3], hi[3];
+    float vmin, vmax;
+  };
+  const float *pos = nullptr;
+  std::vector<float> w;  // vertex values, NaN without vertex data
+  std::vector<Cell> cells;
+  float blo[3] = {0, 0, 0}, bhi[3] = {0, 0, 0};
+  float cell_size = 0.f;
+  int64_t skipped = 0;
+  // the oracle's own point-location aid: a uniform grid over the bounds
+  // listing, in ascending cell id, every cell whose box meets the grid cell
+  int G = 1;
+  std::vector<uint32_t> gstart, gids;
+
+  // finalize (UnstructuredField.cpp:60-170): cells with fewer than 4
+  // indices or a type/count mismatch are skipped; only tetrahedra are
+  // restated (false if another kind of cell remains)
+  bool open(const orc_volume *v) {
+    pos = v->vertex_position;
+    w.resize(size_t(v->num_vertices));
+    for (int64_t i = 0; i < v->num_vertices; ++i)
+      w[i] = v->vertex_data ? v->vertex_data[i] : std::numeric_limits<float>::quiet_NaN();
+    for (int a = 0; a < 3; ++a) {
+      blo[a] = FLT_MAX;
+      bhi[a] = -FLT_MAX;
+    }
+    float avg = 0.f;
+    for (int64_t c = 0; c < v->num_cells; ++c) {
+      const uint64_t first = v->cell_index[c];
+      const uint64_t last = c < v->num_cells - 1 ? uint64_t(v->cell_index[c + 1])
+                                                 : uint64_t(v->num_indices);
+      const uint64_t ic = last - first;
+      const uint8_t t = v->cell_type[c];
+      const bool tet = t == 0 || t == 10, hex = t == 1 || t == 12, wedge = t == 2 || t == 13,
+                 pyr = t == 3 || t == 14;
+      if (ic < 4 || (tet && ic != 4) || (pyr && ic != 5) || (wedge && ic != 6) ||
+          (hex && ic != 8)) {
+        ++skipped;
+        continue;
+      }
+      if (ic != 4) return false;
+      Cell cell;
+      cell.cell_value = v->cell_data ? v->cell_data[c] : 0.f;
+      for (int a = 0; a < 3; ++a) {
+        cell.lo[a] = FLT_MAX;
+        cell.hi[a] = -FLT_MAX;
+      }
+      cell.vmin = FLT_MAX;
+      cell.vmax = -FLT_MAX;
+      for (int k = 0; k < 4; ++k) {
+        cell.v[k] = v->index[first + k];
+        for (int a = 0; a < 3; ++a) {
+          cell.lo[a] = std::min(cell.lo[a], pos[3 * size_t(cell.v[k]) + a]);
+          cell.hi[a] = std::max(cell.hi[a], pos[3 * size_t(cell.v[k]) + a]);
+        }
+        // buildGrid's value range (UnstructuredField.cpp:387-399)
+        const float wk = w[cell.v[k]];
+        const float val = std::isnan(wk) ? cell.cell_value : wk;
+        cell.vmin = std::min(cell.vmin, val);
+        cell.vmax = std::max(cell.vmax, val);
+      }
+      cells.push_back(cell);
+      for (int a = 0; a < 3; ++a) {
+        blo[a] = std::min(blo[a], cell.lo[a]);
+        bhi[a] = std::max(bhi[a], cell.hi[a]);
+      }
+      // avgCellDiagonal += length(max - min) / m_elements.size(), the
+      // size counting the cells kept so far (UnstructuredField.cpp:170)
+      const float dx = cell.hi[0] - cell.lo[0], dy = cell.hi[1] - cell.lo[1],
+                  dz = cell.hi[2] - cell.lo[2];
+      avg += std::sqrt(dx * dx + dy * dy + dz * dz) / float(cells.size());
+    }
+    cell_size = avg;
+    G = std::max(1, int(std::cbrt(double(cells.size()) / 4.0)));
+    const size_t ng = size_t(G) * G * G;
+    std::vector<std::vector<uint32_t>> lists(ng);
+    for (size_t id = 0; id < cells.size(); ++id) {
+      int c0[3], c1[3];
+      for (int a = 0; a < 3; ++a) {
+        c0[a] = grid_coord(a, double(cells[id].lo[a]));
+        c1[a] = grid_coord(a, double(cells[id].hi[a]));
+      }
+      for (int z = c0[2]; z <= c1[2]; ++z)
+        for (int y = c0[1]; y <= c1[1]; ++y)
+          for (int x = c0[0]; x <= c1[0]; ++x)
+            lists[(size_t(z) * G + y) * G + x].push_back(uint32_t(id));
+    }
+    gstart.assign(ng + 1, 0);
+    for (size_t i = 0; i < ng; ++i) gstart[i + 1] = gstart[i] + uint32_t(lists[i].size());
+    gids.reserve(gstart[ng]);
+    for (auto &l : lists) gids.insert(gids.end(), l.begin(), l.end());
+    return !cells.empty();
+  }
+
+  // monotone in x, so a box [lo, hi] containing x covers x's grid cell
+  int grid_coord(int a, double x) const {
+    const double q = (x - double(blo[a])) / (double(bhi[a]) - double(blo[a])) * G;
+    return std::min(std::max(int(std::floor(std::max(q, -1.0))), 0), G - 1);
+  }
+
+  template <class R> bool test_cell(size_t id, V3<R> P, R &value) const {
+    const Cell &c = cells[id];
+    V3<R> p[4];
+    R wv[4];
+    for (int k = 0; k < 4; ++k) {
+      p[k] = from<R>(pos + 3 * size_t(c.v[k]));
+      wv[k] = R(w[c.v[k]]);
+    }
+    if (!intersect_tet(value, P, p[0], wv[0], p[1], wv[1], p[2], wv[2], p[3], wv[3]))
+      return false;
+    // no vertex data: the cell value (DeviceCopyableObjects.h:150-153)
+    if (std::isnan(w[c.v[0]])) value = R(c.cell_value);
+    return true;
+  }
+
+  // sampleField: the first cell (in id order) containing P, found through
+  // the grid; false outside every cell
+  template <class R> bool sample(V3<R> P, R &value) const {
+    for (int a = 0; a < 3; ++a)
+      if (!(P[a] >= R(blo[a]) && P[a] <= R(bhi[a]))) return false;
+    const int x = grid_coord(0, double(P.x)), y = grid_coord(1, double(P.y)),
+              z = grid_coord(2, double(P.z));
+    const size_t g = (size_t(z) * G + y) * G + x;
+    for (uint32_t i = gstart[g]; i < gstart[g + 1]; ++i)
+      if (test_cell(gids[i], P, value)) return true;
+    return false;
+  }
+
+  template <class R> bool sample_brute(V3<R> P, R &value) const {
+    for (size_t id = 0; id < cells.size(); ++id)
+      if (test_cell(id, P, value)) return true;
+    return false;
+  }
+};
+
+// ---------------------------------------------------------------------------
 // Volume: field + TF (A10, A13)
 // ---------------------------------------------------------------------------
 template <class R> struct Volume {
   const orc_volume *v;
   NanoVDBGrid nv;      // nanovdb field
   bool is_nvdb = false;
+  TetField tets;       // unstructured field
+  bool is_tets = false;
   V3<R> glo, gho;      // majorant grid bounds in voxel space
   V3<R> blo, bhi;      // StructuredRegularField::bounds, :90-93
   V3<R> S, T;          // voxelSpaceTransform = (scaling(1/size), -min), :37-40
@@ -226,6 +398,19 @@ template <class R> struct Volume {
       glo = blo;
       gho = bhi;
     }
+    if (vol->field_type == 2) {
+      // UnstructuredField::finalize (UnstructuredField.cpp:280-285): identity
+      // voxel-space transform, bounds of the cells, the running cell size,
+      // grid over the bounds
+      is_tets = tets.open(vol);
+      blo = from<R>(tets.blo);
+      bhi = from<R>(tets.bhi);
+      S = V3<R>(R(1));
+      T = V3<R>(R(0));
+      cellSize = R(tets.cell_size);
+      glo = blo;
+      gho = bhi;
+    }
   }
 
   V3<R> mulA(V3<R> p) const {
@@ -250,7 +435,15 @@ template <class R> struct Volume {
   // tex3D(linear, clamp, normalised coordinates): texel centres at
   // (i+0.5)/N, i.e. texel coordinate p*N - 0.5 (GridAccel.cpp:103 samples
   // the TF at (i+.5)/N under the same convention)
-  R sample(V3<R> P) const {
+  // sampleField (DeviceCopyableObjects.h:474-570): false where the field has
+  // no value (a point outside every cell of an unstructured field)
+  bool sample(V3<R> P, R &value) const {
+    if (v->field_type == 2) return is_tets && tets.sample(P, value);
+    value = sample_regular(P);
+    return true;
+  }
+
+  R sample_regular(V3<R> P) const {
     if (is_nvdb) {
       const R p[3] = {P.x, P.y, P.z};
       return nv.sample(p, v->filter == 1 ? 0 : 1);
@@ -410,15 +603,18 @@ VolHit<R> delta_track(const Volume<R> &vol, Ray<R> ray, Random &rnd, Counter *cn
       t -= std::log(R(1) - R(rnd())) / (majorant * invUnitDistance);
       if (t >= t1) break;
       V3<R> P = ray.ori + ray.dir * t;
-      R v = vol.sample(P);
+      R v;
       if (cnt) cnt->lookups++;
-      V4<R> s = vol.classify(v);
-      R u = R(rnd());
-      if (s.w >= u * majorant) {
-        albedo = V3<R>(s.x, s.y, s.z);
-        hit = true;
-        hit_t = t;
-        return false;
+      // the acceptance draw only where the field has a value (:739-753)
+      if (vol.sample(P, v)) {
+        V4<R> s = vol.classify(v);
+        R u = R(rnd());
+        if (s.w >= u * majorant) {
+          albedo = V3<R>(s.x, s.y, s.z);
+          hit = true;
+          hit_t = t;
+          return false;
+        }
       }
     }
     return true;
@@ -1158,7 +1354,8 @@ V4<R> raycast_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params *
   R alpha = R(0), transmittance = R(1);
   for (R t = lr.tmin; t <= lr.tmax && alpha < R(0.99f); t += dt) {
     V3<R> p = ori + dir * t;
-    R v = vol.sample(p);
+    R v;
+    if (!vol.sample(p, v)) continue;  // VolumeIntegration.h:412
     V4<R> s = vol.classify(v);
     R stepT = opacity_pow(R(1) - s.w, expo);
     color = color + V3<R>(s.x, s.y, s.z) * (transmittance * (R(1) - stepT));
@@ -1289,6 +1486,41 @@ void build_majorants_nanovdb(const orc_volume *v, const int G[3], float *out) {
       }
 }
 
+// unstructured majorants: buildGrid (UnstructuredField.cpp:376-399) ->
+// rasterizeBox (GridAccel.h:51-67): every cell's box, with its value range,
+// into the grid cells from projectOnGrid (GridAccel-common.h:18-24; the
+// float image is clamped to [-1, G] before the int conversion, which leaves
+// in-range results as they are); cells no box reaches get 0
+void build_majorants_unstructured(const orc_volume *v, const int G[3], float *out) {
+  TetField f;
+  const size_t n = size_t(G[0]) * G[1] * G[2];
+  if (!f.open(v)) {
+    std::fill(out, out + n, 0.f);
+    return;
+  }
+  std::vector<float> mn(n, FLT_MAX), mx(n, -FLT_MAX);
+  auto project = [&](float p, int a) {
+    const float c = std::min(std::max((p - f.blo[a]) / (f.bhi[a] - f.blo[a]) * float(G[a]), -1.f),
+                             float(G[a]));
+    return std::min(std::max(int(c), 0), G[a] - 1);
+  };
+  for (const auto &c : f.cells) {
+    int c0[3], c1[3];
+    for (int a = 0; a < 3; ++a) {
+      c0[a] = project(c.lo[a], a);
+      c1[a] = project(c.hi[a], a);
+    }
+    for (int z = c0[2]; z <= c1[2]; ++z)
+      for (int y = c0[1]; y <= c1[1]; ++y)
+        for (int x = c0[0]; x <= c1[0]; ++x) {
+          const size_t i = (size_t(z) * G[1] + y) * G[0] + x;
+          mn[i] = std::min(mn[i], c.vmin);
+          mx[i] = std::max(mx[i], c.vmax);
+        }
+  }
+  for (size_t i = 0; i < n; ++i) out[i] = mx[i] < mn[i] ? 0.f : tf_majorant(v, mn[i], mx[i]);
+}
+
 }  // namespace orc
 
 extern "C" {
@@ -1334,8 +1566,37 @@ int orc_tf_build(const float *colors, int numColors, int nch, const float *opac,
 
 void orc_sample_field(const orc_volume *v, int n, const float *p, float *out) {
   Volume<float> vol(v);
-  for (int i = 0; i < n; ++i)
-    out[i] = vol.sample(V3<float>(p[3 * i], p[3 * i + 1], p[3 * i + 2]));
+  for (int i = 0; i < n; ++i) {
+    float val;
+    out[i] = vol.sample(V3<float>(p[3 * i], p[3 * i + 1], p[3 * i + 2]), val)
+                 ? val
+                 : std::numeric_limits<float>::quiet_NaN();
+  }
+}
+
+void orc_sample_field_brute(const orc_volume *v, int n, const float *p, float *out) {
+  orc::TetField f;
+  const bool ok = v->field_type == 2 && f.open(v);
+  for (int i = 0; i < n; ++i) {
+    float val;
+    out[i] = ok && f.sample_brute(V3<float>(p[3 * i], p[3 * i + 1], p[3 * i + 2]), val)
+                 ? val
+                 : std::numeric_limits<float>::quiet_NaN();
+  }
+}
+
+int orc_unstructured_info(const orc_volume *v, float bounds[6], float *cell_size, int64_t *kept,
+                          int64_t *skipped) {
+  orc::TetField f;
+  if (v->field_type != 2 || !f.open(v)) return -1;
+  for (int a = 0; a < 3; ++a) {
+    bounds[a] = f.blo[a];
+    bounds[3 + a] = f.bhi[a];
+  }
+  *cell_size = f.cell_size;
+  *kept = int64_t(f.cells.size());
+  *skipped = f.skipped;
+  return 0;
 }
 
 void orc_post_classify(const orc_volume *v, int n, const float *vals, float *out) {
@@ -1361,6 +1622,10 @@ void orc_build_majorants(const orc_volume *v, int gx, int gy, int gz, float *out
   int G[3] = {gx, gy, gz};
   if (v->field_type == 1) {
     orc::build_majorants_nanovdb(v, G, out);
+    return;
+  }
+  if (v->field_type == 2) {
+    orc::build_majorants_unstructured(v, G, out);
     return;
   }
   const int *N = v->dims;
@@ -1418,7 +1683,7 @@ void orc_build_majorants_reference(const orc_volume *v, int gx, int gy, int gz, 
       for (int x = 0; x < v->dims[0]; ++x) {
         V3<float> P(v->origin[0] + x * v->spacing[0], v->origin[1] + y * v->spacing[1],
                     v->origin[2] + z * v->spacing[2]);
-        float value = vol.sample(vol.point_to_voxel(P));
+        float value = vol.sample_regular(vol.point_to_voxel(P));
         V3<float> Q(v->origin[0] + (x + 1) * v->spacing[0], v->origin[1] + (y + 1) * v->spacing[1],
                     v->origin[2] + (z + 1) * v->spacing[2]);
         int lo[3], hi[3];

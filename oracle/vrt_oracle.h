@@ -58,6 +58,20 @@ typedef struct orc_volume {
   int field_type;
   const void *nanovdb;
   int filter;
+  /* field type 2, unstructured: the ANARI arrays of UnstructuredField
+     (UnstructuredField.cpp:17-27): float3 positions, per-vertex values
+     (NULL: none), indices, first index per cell, VTK/barney cell types,
+     per-cell values (NULL: none).  Only tetrahedral cells are restated;
+     bounds, voxel space (identity) and cell size come from the cells. */
+  const float *vertex_position;
+  const float *vertex_data;
+  int64_t num_vertices;
+  const uint32_t *index;
+  int64_t num_indices;
+  const uint32_t *cell_index;
+  const uint8_t *cell_type;
+  const float *cell_data;
+  int64_t num_cells;
 } orc_volume;
 
 typedef struct orc_trimesh {
@@ -111,11 +125,21 @@ int orc_tf_build(const float *colors, int num_colors, int color_channels,
                  const float *opacities, int num_opacities,
                  const float uniform_rgba[4], float *out_rgba);
 
-/* --- A10: tex3D linear/clamp, texel-centre convention --- */
+/* --- A10/A11/A12: sampleField at voxel-space points (NaN where the field
+   has no value) --- */
 void orc_sample_field(const orc_volume *vol, int n, const float *p_voxel,
                       float *out);
 void orc_post_classify(const orc_volume *vol, int n, const float *values,
                        float *out_rgba);
+
+/* --- A12: unstructured field.  Info: bounds (lo xyz, hi xyz), cell size
+   (the running avgCellDiagonal, UnstructuredField.cpp:120-170), cells kept /
+   skipped; returns 0, or -1 if the field has non-tetrahedral cells.  The
+   brute-force sampler tests every cell in id order (the pin of the oracle's
+   own point-location grid); NaN where no cell contains the point. --- */
+int orc_unstructured_info(const orc_volume *vol, float bounds[6], float *cell_size,
+                          int64_t *kept, int64_t *skipped);
+void orc_sample_field_brute(const orc_volume *vol, int n, const float *p, float *out);
 
 /* --- A9: majorant grid (conservative), and the reference rasterizer --- */
 void orc_build_majorants(const orc_volume *vol, int gx, int gy, int gz,

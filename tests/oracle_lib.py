@@ -26,7 +26,11 @@ class OrcVolume(C.Structure):
                 ("value_hi", C.c_float), ("unit_distance", C.c_float),
                 ("majorants", C.c_void_p), ("grid_dims", C.c_int * 3),
                 ("xfm", C.c_float * 12), ("field_type", C.c_int), ("nanovdb", C.c_void_p),
-                ("filter", C.c_int)]
+                ("filter", C.c_int), ("vertex_position", C.c_void_p),
+                ("vertex_data", C.c_void_p), ("num_vertices", C.c_int64),
+                ("index", C.c_void_p), ("num_indices", C.c_int64),
+                ("cell_index", C.c_void_p), ("cell_type", C.c_void_p),
+                ("cell_data", C.c_void_p), ("num_cells", C.c_int64)]
 
 
 class OrcTrimesh(C.Structure):
@@ -89,6 +93,8 @@ def lib():
         L.orc_camera_ray.argtypes = [C.POINTER(OrcCamera), C.c_float, C.c_float, ip, ip, vp, vp]
         L.orc_mesh_intersect.argtypes = [C.POINTER(OrcTrimesh), ip, vp, ip, ip, vp, vp]
         L.orc_quad_light_sample.argtypes = [C.POINTER(OrcLight), vp, ip, C.c_uint32, vp, vp]
+        L.orc_unstructured_info.argtypes = [C.POINTER(OrcVolume), vp, vp, vp, vp]
+        L.orc_sample_field_brute.argtypes = [C.POINTER(OrcVolume), ip, vp, vp]
         L.orc_nvdb_get.argtypes = [vp, ip, vp, vp]
         L.orc_nvdb_sample.argtypes = [vp, ip, vp, ip, vp]
         L.orc_nvdb_world_bbox.argtypes = [vp, vp]
@@ -206,6 +212,63 @@ class OracleVolume:
         v.xfm = (C.c_float * 12)(*x.ravel())
         self.s = v
         return self
+
+    @classmethod
+    def from_unstructured(cls, positions, index, tf, value_range, unit_distance,
+                          vertex_data=None, cell_index=None, cell_type=None, cell_data=None,
+                          xfm=None):
+        """An unstructured field from its ANARI arrays (tetrahedra only when
+        cell_index / cell_type are omitted)."""
+        self = cls.__new__(cls)
+        self.voxels = None
+        self.pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        self.idx = np.ascontiguousarray(index, np.uint32).ravel()
+        if cell_index is None:
+            cell_index = np.arange(len(self.idx) // 4, dtype=np.uint32) * 4
+            cell_type = np.full(len(self.idx) // 4, 10, np.uint8)
+        self.ci = np.ascontiguousarray(cell_index, np.uint32)
+        self.ct = np.ascontiguousarray(cell_type, np.uint8)
+        self.vd = None if vertex_data is None else np.ascontiguousarray(vertex_data, np.float32)
+        self.cd = None if cell_data is None else np.ascontiguousarray(cell_data, np.float32)
+        self.tf = np.ascontiguousarray(tf, np.float32).reshape(-1, 4)
+        v = OrcVolume()
+        v.field_type = 2
+        v.vertex_position = self.pos.ctypes.data
+        v.vertex_data = None if self.vd is None else self.vd.ctypes.data
+        v.num_vertices = len(self.pos)
+        v.index = self.idx.ctypes.data
+        v.num_indices = len(self.idx)
+        v.cell_index = self.ci.ctypes.data
+        v.cell_type = self.ct.ctypes.data
+        v.cell_data = None if self.cd is None else self.cd.ctypes.data
+        v.num_cells = len(self.ci)
+        v.tf_rgba = self.tf.ctypes.data
+        v.tf_size = self.tf.shape[0]
+        v.value_lo, v.value_hi = value_range
+        v.unit_distance = unit_distance
+        self.majorants = None
+        x = np.eye(3, 4, dtype=np.float32) if xfm is None else np.asarray(xfm, np.float32)
+        v.xfm = (C.c_float * 12)(*x.ravel())
+        self.s = v
+        return self
+
+    def unstructured_info(self):
+        """(bounds (6,), cell_size, cells kept, cells skipped), or None when the
+        field holds non-tetrahedral cells."""
+        b = np.empty(6, np.float32)
+        cs = C.c_float()
+        kept, skipped = C.c_int64(), C.c_int64()
+        if lib().orc_unstructured_info(C.byref(self.s), b.ctypes.data, C.byref(cs),
+                                       C.byref(kept), C.byref(skipped)):
+            return None
+        return b, cs.value, kept.value, skipped.value
+
+    def sample_brute(self, p_voxel):
+        p = np.ascontiguousarray(p_voxel, np.float32).reshape(-1, 3)
+        out = np.empty(p.shape[0], np.float32)
+        lib().orc_sample_field_brute(C.byref(self.s), p.shape[0], p.ctypes.data,
+                                     out.ctypes.data)
+        return out
 
     def build_majorants(self, grid_dims):
         gx, gy, gz = grid_dims
@@ -329,6 +392,11 @@ class OracleWorkload:
         if wl.nanovdb is not None:
             self.vol = OracleVolume.from_nanovdb(wl.nanovdb, wl.tf, wl.value_range,
                                                  wl.unit_distance, wl.filter)
+        elif getattr(wl, "tets", None) is not None:
+            t = wl.tets
+            self.vol = OracleVolume.from_unstructured(
+                t["positions"], t["index"], wl.tf, wl.value_range, wl.unit_distance,
+                t.get("vertex_data"), t.get("cell_index"), t.get("cell_type"), t.get("cell_data"))
         elif wl.voxels is not None:
             self.vol = OracleVolume(wl.voxels, wl.origin, wl.spacing, wl.tf, wl.value_range,
                                     wl.unit_distance)
