@@ -293,10 +293,13 @@ __device__ inline bool sample_tets(const KVolume &v, f3 P, float &value) {
 }
 
 // sampleField: voxel-space point -> field value; false where the field has
-// no value (outside every cell of an unstructured field)
+// no value (outside every cell of an unstructured field).  ANYF = false
+// compiles the structuredRegular fetch alone (scenes without other fields).
+template <bool ANYF = true>
 __device__ inline bool sample_field(const KVolume &v, f3 P, float &value) {
-  if (v.field_type == VRT_FIELD_UNSTRUCTURED) return sample_tets(v, P, value);
-  value = v.field_type == VRT_FIELD_NANOVDB ? sample_nanovdb(v, P) : sample_structured(v, P);
+  if (ANYF && v.field_type == VRT_FIELD_UNSTRUCTURED) return sample_tets(v, P, value);
+  value = (ANYF && v.field_type == VRT_FIELD_NANOVDB) ? sample_nanovdb(v, P)
+                                                      : sample_structured(v, P);
   return true;
 }
 
@@ -609,6 +612,7 @@ __device__ inline bool trav_begin(const KVolume &v, KRay ray, Trav &T) {
 // voxel-space `t_hit`, classified sample `smp`) or on leaving the grid.
 // `mb`/`mstep`: the volume's majorant bytes in LDS and their step (mb null:
 // the float majorants in global memory).
+template <bool ANYF = true>
 __device__ inline bool trav_step(const KVolume &v, const float4 *lds, const uint8_t *mb,
                                  float mstep, Trav &T, uint32_t &rs, uint32_t &lookups,
                                  bool &hit, float &t_hit, float4 &smp) {
@@ -652,7 +656,7 @@ __device__ inline bool trav_step(const KVolume &v, const float4 *lds, const uint
       ++lookups;
       // no acceptance draw where the field has no value
       // (DeviceCopyableObjects.h:739-753)
-      if (sample_field(v, T.ori + T.dir * T.t, val)) {
+      if (sample_field<ANYF>(v, T.ori + T.dir * T.t, val)) {
         const float4 s = classify(v, lds, val);
         if (s.w >= rnd(rs) * T.maj) {
           hit = true;
@@ -698,6 +702,7 @@ __device__ inline VolHit make_hit(bool hit, float t_hit, float4 smp, float dt_sc
 }
 
 // intersect(Ray, Volume) (DeviceCopyableObjects.h:680-796) in one call
+template <bool ANYF = true>
 __device__ inline VolHit delta_track(const KVolume &v, const float4 *lds, const uint8_t *mb,
                                      float mstep, KRay ray, uint32_t &rs, uint32_t &lookups) {
   Trav T;
@@ -705,7 +710,7 @@ __device__ inline VolHit delta_track(const KVolume &v, const float4 *lds, const 
   float t_hit = 0.f;
   float4 smp = make_float4(0.f, 0.f, 0.f, 0.f);
   if (!trav_begin(v, ray, T)) return make_hit(false, 0.f, smp, 1.f);
-  while (!trav_step(v, lds, mb, mstep, T, rs, lookups, hit, t_hit, smp)) {
+  while (!trav_step<ANYF>(v, lds, mb, mstep, T, rs, lookups, hit, t_hit, smp)) {
   }
   return make_hit(hit, t_hit, smp, T.dt_scale);
 }
@@ -1010,8 +1015,9 @@ struct PassHit {
 };
 
 // AREA: the scene has surfaces or quad lights (else the kernel is built
-// without the BVH and quad code, which would only add registers)
-template <bool AREA>
+// without the BVH and quad code, which would only add registers); ANYF: it
+// has fields other than structuredRegular
+template <bool AREA, bool ANYF>
 __device__ inline PassHit intersect_all(const KScene &sc, const KRay &ray, bool shadow,
                                         int bounceID, const float4 *s_tf, const float *steps,
                                         uint32_t &srng, uint32_t &lookups) {
@@ -1047,8 +1053,8 @@ __device__ inline PassHit intersect_all(const KScene &sc, const KRay &ray, bool 
   vh.hit = false;
   vh.t = FLT_MAX;
   for (int i = 0; i < sc.num_volumes; ++i) {
-    VolHit h = delta_track(sc.vol[i], s_tf, lds_majorants(sc, s_tf, i), steps[i],
-                           to_local(sc.vol[i], ray), srng, lookups);
+    VolHit h = delta_track<ANYF>(sc.vol[i], s_tf, lds_majorants(sc, s_tf, i), steps[i],
+                                 to_local(sc.vol[i], ray), srng, lookups);
     if (h.hit && h.t < vh.t) vh = h;
   }
   PassHit out;
@@ -1337,7 +1343,7 @@ __device__ inline float train_hash(uint32_t pixel, uint32_t accumID) {
          (1.0f / 16777216.0f);
 }
 
-template <bool NRC, bool AREA>
+template <bool NRC, bool AREA, bool ANYF>
 __global__ void __launch_bounds__(256)
     k_pathtrace(KScene sc, KCamera cam, KParams P, KFrame F, int accumID, uint32_t *queues,
                 uint32_t *queues_next, KNrc N) {
@@ -1425,8 +1431,8 @@ __global__ void __launch_bounds__(256)
       ray.tmin = 0.f;
       ray.tmax = st.tmax;
       clip_ray(ray, P);  // every pass (Pathtrace_impl.cpp:559)
-      const PassHit hit = intersect_all<AREA>(sc, ray, st.rayType != Radiance, st.bounceID, s_tf,
-                                              steps, st.srng, lookups);
+      const PassHit hit = intersect_all<AREA, ANYF>(sc, ray, st.rayType != Radiance, st.bounceID,
+                                                    s_tf, steps, st.srng, lookups);
       if (path_shade<NRC, AREA>(st, sc, P, N, hit)) {
         float4 color = P.bg;
         if (st.firstHit) color = make_float4(st.intensity.x, st.intensity.y, st.intensity.z, 1.f);
@@ -1849,26 +1855,33 @@ static bool scene_has_area(const KScene &sc) {
   return false;
 }
 
+// scenes with a nanovdb or unstructured field take the kernel with the
+// field-type dispatch
+static bool scene_has_other_fields(const KScene &sc) {
+  for (int i = 0; i < sc.num_volumes; ++i)
+    if (sc.vol[i].field_type != VRT_FIELD_STRUCTURED_REGULAR) return true;
+  return false;
+}
+
+using PathtraceKernel = void (*)(KScene, KCamera, KParams, KFrame, int, uint32_t *, uint32_t *,
+                                 KNrc);
+
+static PathtraceKernel pathtrace_kernel(const KScene &sc, bool nrc) {
+  static const PathtraceKernel table[8] = {
+      k_pathtrace<false, false, false>, k_pathtrace<false, false, true>,
+      k_pathtrace<false, true, false>,  k_pathtrace<false, true, true>,
+      k_pathtrace<true, false, false>,  k_pathtrace<true, false, true>,
+      k_pathtrace<true, true, false>,   k_pathtrace<true, true, true>};
+  return table[(nrc ? 4 : 0) + (scene_has_area(sc) ? 2 : 0) + (scene_has_other_fields(sc) ? 1 : 0)];
+}
+
 hipError_t launch_pathtrace(const KScene &sc, const KCamera &cam, const KParams &P,
                             const KFrame &F, int accumID, uint32_t *queues,
                             uint32_t *queues_next, int blocks, hipStream_t s, const KNrc *nrc) {
   const size_t lds = scene_lds_bytes(sc);
-  const bool area = scene_has_area(sc);
-  if (nrc) {
-    hipLaunchKernelGGL(k_nrc_frame_begin, dim3(1), dim3(64), 0, s, *nrc);
-    if (area)
-      hipLaunchKernelGGL((k_pathtrace<true, true>), dim3(blocks), dim3(256), lds, s, sc, cam, P, F,
-                         accumID, queues, queues_next, *nrc);
-    else
-      hipLaunchKernelGGL((k_pathtrace<true, false>), dim3(blocks), dim3(256), lds, s, sc, cam, P,
-                         F, accumID, queues, queues_next, *nrc);
-  } else if (area) {
-    hipLaunchKernelGGL((k_pathtrace<false, true>), dim3(blocks), dim3(256), lds, s, sc, cam, P, F,
-                       accumID, queues, queues_next, KNrc{});
-  } else {
-    hipLaunchKernelGGL((k_pathtrace<false, false>), dim3(blocks), dim3(256), lds, s, sc, cam, P,
-                       F, accumID, queues, queues_next, KNrc{});
-  }
+  if (nrc) hipLaunchKernelGGL(k_nrc_frame_begin, dim3(1), dim3(64), 0, s, *nrc);
+  hipLaunchKernelGGL(pathtrace_kernel(sc, nrc != nullptr), dim3(blocks), dim3(256), lds, s, sc,
+                     cam, P, F, accumID, queues, queues_next, nrc ? *nrc : KNrc{});
   return hipGetLastError();
 }
 
@@ -1881,14 +1894,8 @@ hipError_t launch_nrc_resolve(const KFrame &F, const KNrc &N, int accumID, hipSt
 
 int pathtrace_blocks_per_cu(const KScene &sc, size_t lds_bytes, bool nrc) {
   int n = 0;
-  const bool area = scene_has_area(sc);
-  hipError_t e;
-  if (nrc)
-    e = area ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pathtrace<true, true>, 256, lds_bytes)
-             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pathtrace<true, false>, 256, lds_bytes);
-  else
-    e = area ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pathtrace<false, true>, 256, lds_bytes)
-             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pathtrace<false, false>, 256, lds_bytes);
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &n, pathtrace_kernel(sc, nrc), 256, lds_bytes);
   if (e != hipSuccess) return 1;
   return n > 0 ? n : 1;
 }

@@ -65,13 +65,18 @@ def cpu_baseline(wl, target_seconds=12.0):
     }
 
 
+PRECISION_NOTE = {"fp16": "fp16 weights/activations, fp32 accumulation, fp32 Adam",
+                  "bf16": "bf16 weights/activations, fp32 accumulation, fp32 Adam"}
+
+
 def nrc_vs_plain(pkg, ctx, wl, volume=None, ref_frames=2048, cap=1024, warmup=4, train_steps=8,
-                 target=1e-2, oracle_stride=24, oracle_spp=4096, timed_frames=32):
+                 target=1e-2, oracle_stride=24, oracle_spp=4096, timed_frames=32,
+                 precisions=("fp16",)):
     """Wall time to relMSE <= target of the plain path tracer and of the
-    radiance-cache path tracer (its training warm-up included), both measured
-    against an FP64-accumulated plain reference of ref_frames spp; frame
-    times on HIP events; the reference's own relMSE against the CPU FP64
-    oracle is measured on every oracle_stride-th pixel."""
+    radiance-cache path tracer (its training warm-up included) for each cache
+    precision, all measured against an FP64-accumulated plain reference of
+    ref_frames spp; frame times on HIP events; the reference's own relMSE
+    against the CPU FP64 oracle is measured on every oracle_stride-th pixel."""
     vrt, wls = pkg.vrt, pkg.workloads
     ref = wls.GpuScene(ctx, wl, accum64=True, volume=volume)
     volume = ref.volume
@@ -104,8 +109,8 @@ def nrc_vs_plain(pkg, ctx, wl, volume=None, ref_frames=2048, cap=1024, warmup=4,
     gn = wls.GpuScene(ctx, wl, volume=volume, params={"rng_offset": 3 << 20})
     opts = warm.nrc_options(train=True, train_samples=65536, train_steps=train_steps)
 
-    def nrc_run(frames=None):
-        nrc = vrt.Nrc(ctx, max_batch=16384, seed=1)
+    def nrc_run(precision, frames=None):
+        nrc = vrt.Nrc(ctx, precision=precision, max_batch=16384, seed=1)
         ctx.sync()
         ctx.timer_start()
         for a in range(warmup):
@@ -122,11 +127,6 @@ def nrc_vs_plain(pkg, ctx, wl, volume=None, ref_frames=2048, cap=1024, warmup=4,
         nrc.close()
         return n, e, ms, counts
 
-    frames_nrc, e_nrc, _, counts = nrc_run()
-    nrc_reached = e_nrc <= target
-    _, _, t_nrc, _ = nrc_run(frames_nrc)
-    nrc_ms = t_nrc / (warmup + frames_nrc)
-
     t_plain = frames_plain * plain_ms
     out = {
         "workload": wl.name,
@@ -136,12 +136,22 @@ def nrc_vs_plain(pkg, ctx, wl, volume=None, ref_frames=2048, cap=1024, warmup=4,
         "plain": {"frames": frames_plain, "ms_per_frame": round(plain_ms, 4),
                   "ms_to_target": round(t_plain, 2) if plain_reached else None,
                   "relmse": e},
-        "nrc": {"warmup_frames": warmup, "frames": frames_nrc, "ms_per_frame": round(nrc_ms, 4),
-                "ms_to_target": round(t_nrc, 2) if nrc_reached else None, "relmse": e_nrc,
-                "queries_last_frame": counts[0], "training_samples_last_frame": min(counts[1], 65536),
-                "precision": "fp16 weights/activations, fp32 accumulation, fp32 Adam"},
-        "speedup": round(t_plain / t_nrc, 3) if plain_reached and nrc_reached else None,
     }
+    for i, prec in enumerate(precisions):
+        code = {"fp16": vrt.NRC_FP16, "bf16": vrt.NRC_BF16}[prec]
+        frames_nrc, e_nrc, _, counts = nrc_run(code)
+        nrc_reached = e_nrc <= target
+        _, _, t_nrc, _ = nrc_run(code, frames_nrc)
+        nrc_ms = t_nrc / (warmup + frames_nrc)
+        key = "nrc" if i == 0 else f"nrc_{prec}"
+        out[key] = {"warmup_frames": warmup, "frames": frames_nrc,
+                    "ms_per_frame": round(nrc_ms, 4),
+                    "ms_to_target": round(t_nrc, 2) if nrc_reached else None, "relmse": e_nrc,
+                    "queries_last_frame": counts[0],
+                    "training_samples_last_frame": min(counts[1], 65536),
+                    "precision": PRECISION_NOTE[prec]}
+        out["speedup" if i == 0 else f"speedup_{prec}"] = (
+            round(t_plain / t_nrc, 3) if plain_reached and nrc_reached else None)
     # the reference against the CPU FP64 oracle on a pixel subset
     if oracle_stride:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -158,6 +168,30 @@ def nrc_vs_plain(pkg, ctx, wl, volume=None, ref_frames=2048, cap=1024, warmup=4,
             "pixels": int(o.shape[0] * o.shape[1]), "oracle_spp": oracle_spp,
             "seconds": round(time.perf_counter() - t0, 1)}
     return out
+
+
+def mixed_frame_time(pkg, ctx, width, height, frames=32, warmup=4):
+    """Plain path-tracer frame time on the mixed surface + volume scene:
+    1,310,720-triangle icosphere (BVH built on the host), 256^3 blob
+    volume, quad light, <= 6 bounces, 1 spp per frame."""
+    wls = pkg.workloads
+    t0 = time.perf_counter()
+    wl = wls.mixed_pathtrace(level=8, vol_n=256, width=width, height=height, frames=frames)
+    gs = wls.GpuScene(ctx, wl)
+    setup = time.perf_counter() - t0
+    info = gs.meshes[0][0].info
+    gs.frame.reset()
+    for a in range(warmup):
+        gs.render(a)
+    ctx.sync()
+    ctx.timer_start()
+    for a in range(frames):
+        gs.render(warmup + a)
+    ms = ctx.timer_stop() / frames
+    return {"workload": wl.name, "ms_per_frame": round(ms, 4),
+            "Msamples_per_s": round(width * height / (ms / 1e3) / 1e6, 2),
+            "triangles": int(info.num_triangles), "bvh_nodes": int(info.num_nodes),
+            "bvh_build_ms": round(info.build_ms, 1), "setup_s": round(setup, 1)}
 
 
 def calibrate_copy(vrt, mib, reps):
@@ -192,6 +226,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--ttr-size", type=int, default=1024,
                     help="index-space edge of the nanovdb cloud of the time-to-relMSE run")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the frame time of the mixed triangle + volume scene")
+    ap.add_argument("--c4", action="store_true",
+                    help="also run time-to-relMSE on the 2M-tetrahedron 4K scene, plain vs "
+                         "radiance cache in fp16 and bf16")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -257,6 +296,27 @@ def main():
         nrc = nrc_vs_plain(pkg, ctx, wl3, oracle_stride=0 if args.no_cpu else 24)
         print(f"# time to relMSE 1e-2: {json.dumps(nrc)}", file=sys.stderr, flush=True)
 
+    other = {}
+    if not args.no_mixed:
+        other["mixed_surfaces_volume"] = mixed_frame_time(pkg, ctx, args.width, args.height)
+        print(f"# mixed scene: {json.dumps(other['mixed_surfaces_volume'])}", file=sys.stderr,
+              flush=True)
+    if args.c4:
+        t0 = time.perf_counter()
+        wl4 = wls.tets_pathtrace()
+        gs4 = wls.GpuScene(ctx, wl4)
+        inf = gs4.volume.field.info
+        setup = time.perf_counter() - t0
+        c4 = nrc_vs_plain(pkg, ctx, wl4, volume=gs4.volume, oracle_stride=0,
+                          precisions=("fp16", "bf16"))
+        c4["field"] = {"tetrahedra": int(inf.num_cells), "grid_dims": list(inf.grid_dims),
+                       "mean_list": round(inf.mean_list, 2),
+                       "expected_tests": round(inf.expected_tests, 3),
+                       "build_ms": round(inf.build_ms, 1),
+                       "device_MB": round(inf.device_bytes / 1e6, 1), "setup_s": round(setup, 1)}
+        other["tets_time_to_relmse"] = c4
+        print(f"# tets: {json.dumps(c4)}", file=sys.stderr, flush=True)
+
     out = {
         "metric": "Msamples/s at 1080p; time-to-relMSE 1e-2, NRC vs plain pathtrace",
         "value": round(value, 3),
@@ -292,6 +352,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "time_to_relmse": nrc,
+        "other_configs": other or None,
     }
     ctx.close()
     print(json.dumps(out), flush=True)
