@@ -123,6 +123,7 @@ struct KParams {
   uint32_t rng_offset;
   int num_clip_planes;
   float4 clip[VRT_MAX_CLIP_PLANES];
+  float phase_g;
 };
 
 struct KFrame {

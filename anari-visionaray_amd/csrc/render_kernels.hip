@@ -407,15 +407,38 @@ __device__ inline bool tri_test(const KRay &r, f3 v1, f3 e1, f3 e2, float &t, fl
   return true;
 }
 
-constexpr int kBvhStack = 64;
+constexpr int kBvhStack = 64;     // entries per lane
+constexpr int kBvhLdsStack = 16;  // of which the first (top of a shallow stack) live in LDS
+constexpr int kBvhLanes = 256;    // lanes of the workgroups that traverse (LDS stride)
+
+// LDS bytes of the traversal stacks of one 256-lane workgroup
+__host__ __device__ constexpr size_t bvh_lds_stack_bytes() {
+  return size_t(kBvhLdsStack) * kBvhLanes * sizeof(int2);
+}
+
+// compare-exchange of (node, entry distance) pairs by distance
+__device__ inline void cswap(float &ta, int &na, float &tb, int &nb) {
+  if (tb < ta) {
+    const float t = ta;
+    ta = tb;
+    tb = t;
+    const int n = na;
+    na = nb;
+    nb = n;
+  }
+}
 
 // Closest hit in [tmin, tbest] (tbest in: the far end), or any hit when
 // `any`; ties go to the lower triangle index, so the result does not depend
-// on the traversal order.  Per-lane stack of (node, entry distance); boxes
-// are tested with slab distances from finite reciprocals (a zero direction
-// component becomes +-1e20), which can only add box hits, never drop one.
+// on the traversal order.  Boxes are tested with slab distances from finite
+// reciprocals (a zero direction component becomes +-1e20), which can only
+// add box hits, never drop one.  The traversal descends into the nearest
+// child box that was hit and stacks the others (node, entry distance); the
+// stack's first kBvhLdsStack entries live in LDS (`lst`: this lane's slot,
+// entries kBvhLanes apart, so the lanes of a wave hit distinct banks), the
+// rest in private memory, which only deep traversals reach.
 __device__ inline bool bvh_hit(const KSurface &S, const KRay &r, bool any, float &tbest,
-                               int &prim, uint32_t &rec) {
+                               int &prim, uint32_t &rec, int2 *lst) {
   f3 inv;
   {
     const float dx = fabsf(r.dir.x) < 1e-20f ? copysignf(1e-20f, r.dir.x) : r.dir.x;
@@ -423,11 +446,11 @@ __device__ inline bool bvh_hit(const KSurface &S, const KRay &r, bool any, float
     const float dz = fabsf(r.dir.z) < 1e-20f ? copysignf(1e-20f, r.dir.z) : r.dir.z;
     inv = mk(1.f / dx, 1.f / dy, 1.f / dz);
   }
-  int stack[kBvhStack];
-  float stack_t[kBvhStack];
+  int2 deep[kBvhStack - kBvhLdsStack];
   int sp = 0;
   int node = 0;
   bool found = false;
+  const float inf = __builtin_inff();
   while (true) {
     const float4 *n = S.nodes + size_t(node) * 8u;
     const float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5];
@@ -439,11 +462,13 @@ __device__ inline bool bvh_hit(const KSurface &S, const KRay &r, bool any, float
                           __float_as_int(chf.w)};
     const uint32_t count[4] = {__float_as_uint(cnf.x), __float_as_uint(cnf.y),
                                __float_as_uint(cnf.z), __float_as_uint(cnf.w)};
-    int near_node[4];
-    float near_t[4];
-    int nn = 0;
+    // entry distance of every inner child that was hit, +inf for the rest
+    float ct[4];
+    int cn[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
+      ct[c] = inf;
+      cn[c] = child[c];
       if (child[c] < 0) continue;
       const float tx0 = (lo_x[c] - r.ori.x) * inv.x, tx1 = (hi_x[c] - r.ori.x) * inv.x;
       const float ty0 = (lo_y[c] - r.ori.y) * inv.y, ty1 = (hi_y[c] - r.ori.y) * inv.y;
@@ -452,9 +477,7 @@ __device__ inline bool bvh_hit(const KSurface &S, const KRay &r, bool any, float
       const float tf = fminf(tbest, fminf(fmaxf(tx0, tx1), fminf(fmaxf(ty0, ty1), fmaxf(tz0, tz1))));
       if (!(tn <= tf * 1.0000002f + 1e-30f)) continue;
       if (count[c] == 0u) {
-        near_node[nn] = child[c];
-        near_t[nn] = tn;
-        ++nn;
+        ct[c] = tn;
         continue;
       }
       for (uint32_t k = 0; k < count[c]; ++k) {
@@ -472,28 +495,37 @@ __device__ inline bool bvh_hit(const KSurface &S, const KRay &r, bool any, float
         if (any) return true;
       }
     }
-    // push inner children far to near, so the nearest is visited next
-    for (int i = 1; i < nn; ++i)
-      for (int j = i; j > 0 && near_t[j] > near_t[j - 1]; --j) {
-        const float tt = near_t[j];
-        near_t[j] = near_t[j - 1];
-        near_t[j - 1] = tt;
-        const int tn = near_node[j];
-        near_node[j] = near_node[j - 1];
-        near_node[j - 1] = tn;
-      }
-    for (int i = 0; i < nn; ++i)
-      if (sp < kBvhStack) {
-        stack[sp] = near_node[i];
-        stack_t[sp] = near_t[i];
-        ++sp;
-      }
+    // ascending by distance (a 5-comparator network, all in registers)
+    cswap(ct[0], cn[0], ct[1], cn[1]);
+    cswap(ct[2], cn[2], ct[3], cn[3]);
+    cswap(ct[0], cn[0], ct[2], cn[2]);
+    cswap(ct[1], cn[1], ct[3], cn[3]);
+    cswap(ct[1], cn[1], ct[2], cn[2]);
+    // stack the farther hits, far first, so the nearer ones are popped first
+#pragma unroll
+    for (int c = 3; c >= 1; --c) {
+      if (!(ct[c] < inf) || sp >= kBvhStack) continue;
+      const int2 e = make_int2(cn[c], __float_as_int(ct[c]));
+      if (sp < kBvhLdsStack)
+        lst[sp * kBvhLanes] = e;
+      else
+        deep[sp - kBvhLdsStack] = e;
+      ++sp;
+    }
+    // descend into the nearest hit unless a closer triangle was found since
+    // (ct[0] = +inf: no inner child was hit; the limit itself may be +inf)
+    const float lim = tbest * 1.0000002f + 1e-30f;
+    if (ct[0] < inf && ct[0] <= lim) {
+      node = cn[0];
+      continue;
+    }
     // next node: skip those entered beyond the current closest hit
     node = -1;
     while (sp > 0) {
       --sp;
-      if (stack_t[sp] <= tbest * 1.0000002f + 1e-30f) {
-        node = stack[sp];
+      const int2 e = sp < kBvhLdsStack ? lst[sp * kBvhLanes] : deep[sp - kBvhLdsStack];
+      if (__int_as_float(e.y) <= tbest * 1.0000002f + 1e-30f) {
+        node = e.x;
         break;
       }
     }
@@ -758,6 +790,32 @@ __device__ inline f3 cosine_sample_hemisphere(float u1, float u2) {
 }
 __device__ inline f3 faceforward(f3 n, f3 i, f3 nref) { return dot(i, nref) < 0.f ? -n : n; }
 
+// Henyey-Greenstein phase function (an extension: the reference's phase is
+// isotropic).  hg_4pi = 4 pi p_HG(cos theta) = (1 - g^2) / (1 + g^2 - 2 g
+// cos theta)^(3/2), exactly 1 at g = 0; cos theta is taken between the ray
+// direction and the scattered (or light) direction, so g > 0 scatters
+// forward.
+__device__ inline float hg_4pi(float g, float cosT) {
+  const float d = 1.f + g * g - 2.f * g * cosT;
+  return (1.f - g * g) / (d * sqrtf(d));
+}
+// Direction with density p_HG around unit d: cos theta from the inverse CDF
+// (1 + g^2 - ((1 - g^2) / (1 - g + 2 g u1))^2) / (2 g), phi = 2 pi u2, in
+// the orthonormal basis of Duff et al. (2017) around d.  g != 0.
+__device__ inline f3 hg_sample(float g, f3 d, float u1, float u2) {
+  const float s = (1.f - g * g) / (1.f - g + 2.f * g * u1);
+  const float cosT = fminf(fmaxf((1.f + g * g - s * s) / (2.f * g), -1.f), 1.f);
+  const float sinT = sqrtf(fmaxf(0.f, 1.f - cosT * cosT));
+  float sp, cp;
+  sincos_turns(u2, sp, cp);
+  const float sign = copysignf(1.f, d.z);
+  const float a = -1.f / (sign + d.z);
+  const float b = d.x * d.y * a;
+  const f3 t1 = mk(1.f + sign * d.x * d.x * a, sign * b, -sign * d.x);
+  const f3 t2 = mk(b, sign + d.y * d.y * a, -d.y);
+  return t1 * (sinT * cp) + t2 * (sinT * sp) + d * cosT;
+}
+
 // ---------------------------------------------------------------------------
 // Frame write: Frame::writeSample (DeviceCopyableObjects.h:2776-2781)
 // ---------------------------------------------------------------------------
@@ -1020,7 +1078,7 @@ struct PassHit {
 template <bool AREA, bool ANYF>
 __device__ inline PassHit intersect_all(const KScene &sc, const KRay &ray, bool shadow,
                                         int bounceID, const float4 *s_tf, const float *steps,
-                                        uint32_t &srng, uint32_t &lookups) {
+                                        uint32_t &srng, uint32_t &lookups, int2 *bstack) {
   bool sh = false;
   float st = ray.tmax;
   int sid = -1;
@@ -1029,7 +1087,7 @@ __device__ inline PassHit intersect_all(const KScene &sc, const KRay &ray, bool 
     float t = st;
     int prim = -1;
     uint32_t rec = 0;
-    if (bvh_hit(sc.surf[i], ray, shadow, t, prim, rec)) {
+    if (bvh_hit(sc.surf[i], ray, shadow, t, prim, rec, bstack)) {
       sh = true;
       st = t;
       sid = i;
@@ -1254,11 +1312,20 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
     const f3 lightIntensity = ls.intensity * safe_rcp(ls.dist2);
     if (h.kind == kVolume) {
       // Pathtrace_impl.cpp:330-339: no phase factor, no light count;
-      // isotropic phase sample: f = albedo, pdf = cosTheta = 1
+      // isotropic phase sample: f = albedo, pdf = cosTheta = 1.  With an
+      // anisotropic phase (g != 0) NEE carries 4 pi p_HG towards the light
+      // and the sample is drawn from p_HG, whose f/pdf is again the albedo.
       st.direct = h.albedo * lightIntensity * safe_rcp(ls.pdf) * safe_rcp(ls.dist2);
       const float u1 = rnd(st.srng), u2 = rnd(st.srng);
-      st.bs_dir = uniform_sample_sphere(u1, u2);
-      st.bs_pdf = 1.f;
+      if (P.phase_g == 0.f) {
+        st.bs_dir = uniform_sample_sphere(u1, u2);
+        st.bs_pdf = 1.f;
+      } else {
+        const f3 d = normalize(ray.dir);
+        if (nl > 0) st.direct = st.direct * hg_4pi(P.phase_g, dot(d, normalize(ls.dir)));
+        st.bs_dir = hg_sample(P.phase_g, d, u1, u2);
+        st.bs_pdf = hg_4pi(P.phase_g, dot(d, st.bs_dir));
+      }
       st.albedo = h.albedo * (1.f * 1.f);
     } else if (AREA) {
       // matte (common.h:1033-1048, 1371-1413): f = color / pi, cosine
@@ -1369,6 +1436,11 @@ __global__ void __launch_bounds__(256)
   const uint32_t lane = threadIdx.x & 63u;
   const unsigned long long below = (1ull << lane) - 1ull;
   const float *steps = lds_steps(sc, s_tf);
+  // this lane's BVH traversal stack, after the tables and majorants
+  int2 *bstack = AREA ? reinterpret_cast<int2 *>(reinterpret_cast<char *>(s_tf) +
+                                                 scene_lds_bytes(sc)) +
+                            threadIdx.x
+                      : nullptr;
 
   PathState st;
   st.pixel = -1;
@@ -1432,7 +1504,7 @@ __global__ void __launch_bounds__(256)
       ray.tmax = st.tmax;
       clip_ray(ray, P);  // every pass (Pathtrace_impl.cpp:559)
       const PassHit hit = intersect_all<AREA, ANYF>(sc, ray, st.rayType != Radiance, st.bounceID,
-                                                    s_tf, steps, st.srng, lookups);
+                                                    s_tf, steps, st.srng, lookups, bstack);
       if (path_shade<NRC, AREA>(st, sc, P, N, hit)) {
         float4 color = P.bg;
         if (st.firstHit) color = make_float4(st.intensity.x, st.intensity.y, st.intensity.z, 1.f);
@@ -1794,6 +1866,7 @@ __global__ void __launch_bounds__(256)
 __global__ void __launch_bounds__(256)
     k_intersect_surfaces(KScene sc, int n, const float *__restrict__ rays, int any,
                          float *__restrict__ t_out, int *__restrict__ prim_out) {
+  extern __shared__ int2 s_bstack[];
   const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
   if (i >= n) return;
   const float *r = rays + 8 * size_t(i);
@@ -1809,7 +1882,7 @@ __global__ void __launch_bounds__(256)
     float t = best;
     int p = -1;
     uint32_t rec;
-    if (bvh_hit(sc.surf[s], ray, any != 0, t, p, rec)) {
+    if (bvh_hit(sc.surf[s], ray, any != 0, t, p, rec, s_bstack + threadIdx.x)) {
       hit = true;
       best = t;
       prim = p;
@@ -1875,10 +1948,14 @@ static PathtraceKernel pathtrace_kernel(const KScene &sc, bool nrc) {
   return table[(nrc ? 4 : 0) + (scene_has_area(sc) ? 2 : 0) + (scene_has_other_fields(sc) ? 1 : 0)];
 }
 
+size_t pathtrace_lds_size(const KScene &sc) {
+  return scene_lds_bytes(sc) + (scene_has_area(sc) ? bvh_lds_stack_bytes() : 0);
+}
+
 hipError_t launch_pathtrace(const KScene &sc, const KCamera &cam, const KParams &P,
                             const KFrame &F, int accumID, uint32_t *queues,
                             uint32_t *queues_next, int blocks, hipStream_t s, const KNrc *nrc) {
-  const size_t lds = scene_lds_bytes(sc);
+  const size_t lds = pathtrace_lds_size(sc);
   if (nrc) hipLaunchKernelGGL(k_nrc_frame_begin, dim3(1), dim3(64), 0, s, *nrc);
   hipLaunchKernelGGL(pathtrace_kernel(sc, nrc != nullptr), dim3(blocks), dim3(256), lds, s, sc,
                      cam, P, F, accumID, queues, queues_next, nrc ? *nrc : KNrc{});
@@ -1963,8 +2040,8 @@ hipError_t launch_sample_field(const KVolume &v, int n, const float *pts, float 
 hipError_t launch_intersect_surfaces(const KScene &sc, int n, const float *rays, int any,
                                      float *t_out, int *prim_out, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_intersect_surfaces, dim3((n + 255) / 256), dim3(256), 0, s, sc, n, rays, any,
-                     t_out, prim_out);
+  hipLaunchKernelGGL(k_intersect_surfaces, dim3((n + 255) / 256), dim3(kBvhLanes),
+                     bvh_lds_stack_bytes(), s, sc, n, rays, any, t_out, prim_out);
   return hipGetLastError();
 }
 

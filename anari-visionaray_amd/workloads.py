@@ -94,11 +94,12 @@ def blobs_raycast(n=128, width=512, height=512, seed=0, threads=0):
 
 
 def fbm_pathtrace(n=512, width=1920, height=1080, frames=64, seed=1, brick=16,
-                  density_scale=50.0, albedo=0.8, max_bounce=8, threads=0):
+                  density_scale=50.0, albedo=0.8, max_bounce=8, phase_g=0.0, threads=0):
     """Pathtrace: structuredRegular n^3 f32 fBm (6 octaves, lacunarity 2,
     gain 0.5) remapped to [0,1]; TF rgb = albedo, alpha = value with
     unitDistance = 1/density_scale; majorant grid of brick^3-voxel cells;
-    one directional light plus the constant ambient term; isotropic phase."""
+    one directional light plus the constant ambient term; Henyey-Greenstein
+    phase of anisotropy phase_g (0: the reference's isotropic phase)."""
     vox = _vrt.gen_fbm((n, n, n), 6, 4.0, 2.0, 0.5, seed, threads)
     sp = 1.0 / (n - 1)
     g = max(1, (n + brick - 1) // brick)
@@ -113,13 +114,13 @@ def fbm_pathtrace(n=512, width=1920, height=1080, frames=64, seed=1, brick=16,
                      intensity=1.0)],
         params=dict(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.5,
                     occlusion_distance=1e20, ambient_samples=1, max_bounce=max_bounce,
-                    pixel_samples=1),
+                    pixel_samples=1, phase_g=phase_g),
         width=width, height=height, renderer="pathtrace", frames=frames)
 
 
 def cloud_nanovdb_pathtrace(n=1024, width=1920, height=1080, frames=256, seed=3,
                             active_fraction=0.05, albedo=0.8, max_bounce=8, brick=32,
-                            unit_distance=None, threads=0):
+                            unit_distance=None, phase_g=0.0, threads=0):
     """Sparse fBm cloud as a 'nanovdb' field (1024^3 index space, identity
     map, 8^3 leaves, ~5% active), path traced with a directional light and
     ambient term; majorant cells of `brick` voxels over the grid bounds."""
@@ -141,7 +142,7 @@ def cloud_nanovdb_pathtrace(n=1024, width=1920, height=1080, frames=256, seed=3,
                      intensity=1.0)],
         params=dict(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.5,
                     occlusion_distance=1e20, ambient_samples=1, max_bounce=max_bounce,
-                    pixel_samples=1),
+                    pixel_samples=1, phase_g=phase_g),
         width=width, height=height, renderer="pathtrace", frames=frames, nanovdb=buf)
 
 

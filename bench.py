@@ -31,6 +31,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_LOOKUP = 32
+# Henyey-Greenstein anisotropy of the fBm path-tracing workload (and of the
+# cloud whose time to relMSE is measured), as the workload is specified
+PHASE_G = 0.3
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_pathtrace_fbm512_1080p.json")
 
 
@@ -248,7 +251,7 @@ def main():
         return
     t0 = time.perf_counter()
     wl = wls.fbm_pathtrace(n=args.size, width=args.width, height=args.height,
-                           frames=args.steps)
+                           frames=args.steps, phase_g=PHASE_G)
     print(f"# generated {wl.name} in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
           flush=True)
 
@@ -290,7 +293,8 @@ def main():
         # radiance cache vs plain on the nanovdb cloud: 1024^3 index space,
         # ~5% active, 8^3 leaves
         t0 = time.perf_counter()
-        wl3 = wls.cloud_nanovdb_pathtrace(n=args.ttr_size, width=args.width, height=args.height)
+        wl3 = wls.cloud_nanovdb_pathtrace(n=args.ttr_size, width=args.width, height=args.height,
+                                          phase_g=PHASE_G)
         print(f"# generated {wl3.name} in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
               flush=True)
         nrc = nrc_vs_plain(pkg, ctx, wl3, oracle_stride=0 if args.no_cpu else 24)
@@ -332,8 +336,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"pathtrace fBm {args.size}^3 f32, density x50, 32^3 majorants, "
-                        f"directional + ambient, <=8 bounces, {args.width}x{args.height}, "
-                        f"1 spp/frame",
+                        f"albedo 0.8, Henyey-Greenstein g={PHASE_G}, directional + ambient, "
+                        f"<=8 bounces, {args.width}x{args.height}, 1 spp/frame",
             "renderer": "pathtrace",
             "image": [args.width, args.height],
             "volume": [args.size] * 3,

@@ -162,6 +162,12 @@ typedef struct vrt_params {
      (clipRay, renderer/common.h:1469-1482) */
   int num_clip_planes;
   float clip_planes[VRT_MAX_CLIP_PLANES][4];
+  /* extension: Henyey-Greenstein anisotropy g in (-1, 1) of the volume phase
+     function.  0 is the reference's isotropic phase (uniform sphere sample,
+     NEE without phase factor, Pathtrace_impl.cpp:330-339) exactly; other
+     values sample HG around the ray direction with the same two random
+     numbers and weight volume NEE by 4 pi p_HG, which is 1 at g = 0. */
+  float phase_g;
 } vrt_params;
 
 /* ANARI data types used for frame channels (ANARI 1.0 enum values). */

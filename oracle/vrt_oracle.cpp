@@ -685,6 +685,28 @@ template <class R> inline V3<R> uniform_sample_sphere(R u1, R u2) {
   R phi = R(2 * M_PI) * u2;
   return V3<R>(r * std::cos(phi), r * std::sin(phi), z);
 }
+// Henyey-Greenstein phase function (an extension, no reference counterpart;
+// g = 0 is the isotropic phase above): hg_4pi = 4 pi p_HG(cos theta) =
+// (1 - g^2) / (1 + g^2 - 2 g cos theta)^(3/2); samples by the inverse CDF
+// cos theta = (1 + g^2 - ((1 - g^2) / (1 - g + 2 g u1))^2) / (2 g) around
+// the unit direction d, phi = 2 pi u2, in the orthonormal basis of Duff et
+// al., "Building an Orthonormal Basis, Revisited" (2017).  g != 0.
+template <class R> inline R hg_4pi(R g, R cosT) {
+  const R d = R(1) + g * g - R(2) * g * cosT;
+  return (R(1) - g * g) / (d * std::sqrt(d));
+}
+template <class R> inline V3<R> hg_sample(R g, V3<R> d, R u1, R u2) {
+  const R s = (R(1) - g * g) / (R(1) - g + R(2) * g * u1);
+  const R cosT = std::min(std::max((R(1) + g * g - s * s) / (R(2) * g), R(-1)), R(1));
+  const R sinT = std::sqrt(std::max(R(0), R(1) - cosT * cosT));
+  const R phi = R(2 * M_PI) * u2;
+  const R sign = std::copysign(R(1), d.z);
+  const R a = R(-1) / (sign + d.z);
+  const R b = d.x * d.y * a;
+  const V3<R> t1(R(1) + sign * d.x * d.x * a, sign * b, -sign * d.x);
+  const V3<R> t2(b, sign + d.y * d.y * a, -d.y);
+  return t1 * (sinT * std::cos(phi)) + t2 * (sinT * std::sin(phi)) + d * cosT;
+}
 template <class R> inline V3<R> cosine_sample_hemisphere(R u1, R u2) {
   // Shirley-Chiu concentric disk, lifted to the hemisphere
   R sx = R(2) * u1 - R(1), sy = R(2) * u2 - R(1);
@@ -1064,6 +1086,7 @@ template <class R> struct ShadeState {
   // bsdfSample
   V3<R> bs_dir, bs_f;
   R bs_pdf = 0, bs_cosT = 0;
+  V3<R> bs_w;  // path weight f cos / pdf of the sample
   LightSample<R> ls;
   R misWeightNEE = 0;
   RayType nextType = None;
@@ -1175,16 +1198,28 @@ void shade(Random &srnd, const Ray<R> &ray, RayType rayType, const Scene<R> &sc,
       st.bs_pdf = std::max(R(0), dot(sn, st.bs_dir)) * inv_pi;
       st.bs_cosT = std::max(R(0), dot(sn, st.bs_dir));
       st.bs_f = f;
+      st.bs_w = f * (st.bs_cosT * safe_rcp(st.bs_pdf));
     }
     if (hitRec.volumeHit) {
       // Pathtrace_impl.cpp:330-339: no phase factor, no light count
       st.shadedColor = hitRec.vol.albedo * lightIntensity * safe_rcp(st.ls.pdf) *
                        safe_rcp(st.ls.dist2);
       R u1 = R(srnd()), u2 = R(srnd());
-      st.bs_dir = uniform_sample_sphere(u1, u2);
+      if (P->phase_g == 0.f) {
+        st.bs_dir = uniform_sample_sphere(u1, u2);
+        st.bs_pdf = R(1);
+      } else {
+        // anisotropic phase (an extension): NEE carries 4 pi p_HG towards
+        // the light, the sample is drawn from p_HG and weighs the albedo
+        const R g = R(P->phase_g);
+        const V3<R> d = normalize(ray.dir);
+        if (numLights > 0) st.shadedColor = st.shadedColor * hg_4pi(g, dot(d, normalize(st.ls.dir)));
+        st.bs_dir = hg_sample(g, d, u1, u2);
+        st.bs_pdf = hg_4pi(g, dot(d, st.bs_dir));
+      }
       st.bs_f = hitRec.vol.albedo;
-      st.bs_pdf = R(1);
       st.bs_cosT = R(1);
+      st.bs_w = hitRec.vol.albedo * (R(1) * R(1));
     }
     st.misWeightNEE = R(0);
     if (numLights > 0) {
@@ -1292,8 +1327,7 @@ V4<R> pathtrace_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params
                           R(P->ambient_radiance) * st.vis_ao;
           intensity = intensity + throughput * (direct * st.misWeightNEE);
           intensity = intensity + throughput * ambient;
-          R rcp = st.bs_pdf > R(0) ? R(1) / st.bs_pdf : R(0);
-          throughput = throughput * (st.bs_f * (st.bs_cosT * rcp));
+          throughput = throughput * st.bs_w;
           bounceID++;
         }
         if (rayType == Miss || int(bounceID) > P->max_bounce) break;
@@ -1904,6 +1938,20 @@ void orc_mesh_intersect(const orc_trimesh *m, int n, const float *rays, int any,
     prim_out[i] = hit ? prim : -1;
   }
 }
+
+void orc_hg_sample(float g, const float d[3], int n, uint32_t seed, float *out) {
+  const V3<float> dir = normalize(V3<float>(d[0], d[1], d[2]));
+  for (int i = 0; i < n; ++i) {
+    Random r(seed, uint32_t(i));
+    const float u1 = r(), u2 = r();
+    const V3<float> w = hg_sample(g, dir, u1, u2);
+    out[3 * i] = w.x;
+    out[3 * i + 1] = w.y;
+    out[3 * i + 2] = w.z;
+  }
+}
+
+double orc_hg_4pi(double g, double cos_theta) { return hg_4pi(g, cos_theta); }
 
 void orc_quad_light_sample(const orc_light *L, const float ref[3], int n, uint32_t seed,
                            float *pos_out, float *pdf_out) {

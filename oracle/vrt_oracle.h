@@ -114,6 +114,9 @@ typedef struct orc_params {
   /* clip planes (N, D): each ray keeps N.x <= D (renderer/common.h:1469-1482) */
   int num_clip_planes;
   float clip_planes[8][4];
+  /* Henyey-Greenstein anisotropy of the volume phase function (an
+     extension; 0 = the reference's isotropic phase) */
+  float phase_g;
 } orc_params;
 
 /* --- A1: LCG<4> / Random (common.h:55-102) --- */
@@ -196,6 +199,10 @@ void orc_mesh_intersect(const orc_trimesh *m, int n, const float *rays, int any,
    positions and pdfs --- */
 void orc_quad_light_sample(const orc_light *L, const float ref[3], int n, uint32_t seed,
                            float *pos_out, float *pdf_out);
+/* --- Henyey-Greenstein phase (extension): n directions sampled around d
+   (RNG seeded (seed, i)), and 4 pi p_HG(cos theta) --- */
+void orc_hg_sample(float g, const float d[3], int n, uint32_t seed, float *out);
+double orc_hg_4pi(double g, double cos_theta);
 void orc_camera_ray(const orc_camera *cam, float x, float y, int width,
                     int height, float *ori, float *dir);
 

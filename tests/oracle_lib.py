@@ -54,7 +54,8 @@ class OrcParams(C.Structure):
                 ("occlusion_distance", C.c_float), ("ambient_samples", C.c_int),
                 ("max_bounce", C.c_int), ("pixel_samples", C.c_int),
                 ("volume_sampling_rate", C.c_float), ("rng_offset", C.c_uint32),
-                ("num_clip_planes", C.c_int), ("clip_planes", (C.c_float * 4) * 8)]
+                ("num_clip_planes", C.c_int), ("clip_planes", (C.c_float * 4) * 8),
+                ("phase_g", C.c_float)]
 
 
 _lib = None
@@ -95,6 +96,9 @@ def lib():
         L.orc_quad_light_sample.argtypes = [C.POINTER(OrcLight), vp, ip, C.c_uint32, vp, vp]
         L.orc_unstructured_info.argtypes = [C.POINTER(OrcVolume), vp, vp, vp, vp]
         L.orc_sample_field_brute.argtypes = [C.POINTER(OrcVolume), ip, vp, vp]
+        L.orc_hg_sample.argtypes = [C.c_float, vp, ip, C.c_uint32, vp]
+        L.orc_hg_4pi.restype = C.c_double
+        L.orc_hg_4pi.argtypes = [C.c_double, C.c_double]
         L.orc_nvdb_get.argtypes = [vp, ip, vp, vp]
         L.orc_nvdb_sample.argtypes = [vp, ip, vp, ip, vp]
         L.orc_nvdb_world_bbox.argtypes = [vp, vp]
@@ -366,8 +370,9 @@ def camera(position, direction, up, fovy, aspect, region=(0, 0, 1, 1), shutter=(
 
 def params(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.2,
            occlusion_distance=1e20, ambient_samples=1, max_bounce=7, pixel_samples=1,
-           volume_sampling_rate=0.5, rng_offset=0, clip_planes=()):
+           volume_sampling_rate=0.5, rng_offset=0, clip_planes=(), phase_g=0.0):
     p = OrcParams()
+    p.phase_g = phase_g
     p.num_clip_planes = len(clip_planes)
     for i, c in enumerate(clip_planes):
         p.clip_planes[i] = (C.c_float * 4)(*c)

@@ -204,6 +204,35 @@ def test_pathtrace_mean_parity(pkg, oracle, gpu_ctx):
     assert np.max(np.abs(g[..., :3] - g64[..., :3]) / np.maximum(1e-3, np.abs(g64[..., :3]))) < 1e-4
 
 
+def test_pathtrace_anisotropic_phase(pkg, oracle, gpu_ctx):
+    """Henyey-Greenstein g = 0.3 (the extension the fBm workload uses):
+    stream-aligned with the oracle and within relMSE 1e-3 of its FP32 and FP64
+    means; the image differs from the isotropic one."""
+    import dataclasses
+    wl = _pt_workload(pkg)
+    wl = dataclasses.replace(wl, params={**wl.params, "phase_g": 0.3})
+    gs = pkg.workloads.GpuScene(gpu_ctx, wl)
+    ow = oracle.OracleWorkload(wl)
+    gs.frame.reset()
+    gs.render(0)
+    gpu_ctx.sync()
+    agree = oracle.stream_agreement(gs.frame.color_array(), ow.pathtrace_frame(0)[0])
+    print("HG stream agreement", agree)
+    assert agree >= oracle.STREAM_AGREEMENT_MIN, agree
+    frames = 64
+    gs.frame.reset()
+    for a in range(frames):
+        gs.render(a)
+    gpu_ctx.sync()
+    g = gs.frame.accum_array()
+    e32 = relmse(g, ow.pathtrace_accumulate(frames))
+    e64 = relmse(g, ow.pathtrace_accumulate(frames, fp64=True))
+    print("HG relMSE vs oracle fp32", e32, "fp64", e64)
+    assert e32 <= 1e-3 and e64 <= 1e-3
+    iso = oracle.OracleWorkload(_pt_workload(pkg)).pathtrace_accumulate(frames)
+    assert relmse(g, iso) > 10 * max(e32, 1e-6)
+
+
 def test_pathtrace_counts_lookups(pkg, oracle, gpu_ctx):
     wl = _pt_workload(pkg)
     gs = pkg.workloads.GpuScene(gpu_ctx, wl)
