@@ -29,16 +29,11 @@ typedef short s4v __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
-// 16-bit conversions (fp16: IEEE round-to-nearest-even; bf16: RNE on the
-// upper half of the fp32 pattern)
+// 16-bit conversions, both IEEE round-to-nearest-even in hardware (gfx950
+// converts fp32 to bf16 with v_cvt_pk_bf16_f32)
 // ---------------------------------------------------------------------------
 __device__ inline uint16_t to16(float f, int bf16) {
-  if (bf16) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return uint16_t((u >> 16) | 0x40u);  // NaN
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return uint16_t(u >> 16);
-  }
+  if (bf16) return __builtin_bit_cast(uint16_t, (__bf16)f);
   _Float16 h = (_Float16)f;
   return __builtin_bit_cast(uint16_t, h);
 }

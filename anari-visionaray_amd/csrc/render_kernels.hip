@@ -305,18 +305,33 @@ __device__ inline bool sample_field(const KVolume &v, f3 P, float &value) {
 
 // postClassify (DeviceCopyableObjects.h:615-619): tex1D(linear, clamp) of the
 // TF table, read from the workgroup's LDS copy
-__device__ inline float4 classify(const KVolume &v, const float4 *lds, float val) {
+// the two table entries and the weight tex1D(linear, clamp) reads for val
+__device__ inline void tf_taps(const KVolume &v, float val, int &i0, int &i1, float &w) {
   float u = (val - v.vlo) / (v.vhi - v.vlo);
   const int n = v.tf_size;
   float c = u * float(n) - 0.5f;
   c = fminf(fmaxf(c, -1.f), float(n));
   float fl = floorf(c);
-  float w = c - fl;
+  w = c - fl;
   int i = int(fl);
-  int i0 = min(max(i, 0), n - 1), i1 = min(max(i + 1, 0), n - 1);
-  const float4 a = lds[v.tf_lds_offset + i0], b = lds[v.tf_lds_offset + i1];
+  i0 = v.tf_lds_offset + min(max(i, 0), n - 1);
+  i1 = v.tf_lds_offset + min(max(i + 1, 0), n - 1);
+}
+__device__ inline float4 classify(const KVolume &v, const float4 *lds, float val) {
+  int i0, i1;
+  float w;
+  tf_taps(v, val, i0, i1, w);
+  const float4 a = lds[i0], b = lds[i1];
   return make_float4(lerpf(a.x, b.x, w), lerpf(a.y, b.y, w), lerpf(a.z, b.z, w),
                      lerpf(a.w, b.w, w));
+}
+// opacity alone (the component delta tracking tests), as classify computes it
+__device__ inline float classify_alpha(const KVolume &v, const float4 *lds, float val) {
+  int i0, i1;
+  float w;
+  tf_taps(v, val, i0, i1, w);
+  const float *t = reinterpret_cast<const float *>(lds);
+  return lerpf(t[4 * i0 + 3], t[4 * i1 + 3], w);
 }
 
 // Workgroup LDS: [TF tables, float4 each][majorant steps, one float4][majorant
@@ -689,11 +704,11 @@ __device__ inline bool trav_step(const KVolume &v, const float4 *lds, const uint
       // no acceptance draw where the field has no value
       // (DeviceCopyableObjects.h:739-753)
       if (sample_field<ANYF>(v, T.ori + T.dir * T.t, val)) {
-        const float4 s = classify(v, lds, val);
-        if (s.w >= rnd(rs) * T.maj) {
+        // colour only for an accepted collision
+        if (classify_alpha(v, lds, val) >= rnd(rs) * T.maj) {
           hit = true;
           t_hit = T.t;
-          smp = s;
+          smp = classify(v, lds, val);
           return true;
         }
       }

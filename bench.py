@@ -197,6 +197,46 @@ def mixed_frame_time(pkg, ctx, width, height, frames=32, warmup=4):
             "bvh_build_ms": round(info.build_ms, 1), "setup_s": round(setup, 1)}
 
 
+def raycast_frame_time(pkg, ctx, frames=32, warmup=4, cpu=True):
+    """The ray caster on the 128^3 blob volume at 512x512 (the reference's
+    CPU-runnable configuration): GPU frame time on HIP events, trilinear
+    lookups per frame (exact: the march is deterministic) and, with `cpu`,
+    the CPU restatement timed on this host's cores for the same frame."""
+    wls = pkg.workloads
+    wl = wls.blobs_raycast(n=128, width=512, height=512)
+    gs = wls.GpuScene(ctx, wl)
+    for a in range(warmup):
+        gs.render(a)
+    ctx.sync()
+    gs.frame.counters.zero()
+    ctx.sync()
+    ctx.timer_start()
+    for a in range(frames):
+        gs.render(a)
+    ms = ctx.timer_stop() / frames
+    lookups = int(gs.frame.counter_values()[0]) / frames
+    out = {"workload": wl.name, "ms_per_frame": round(ms, 4),
+           "Msamples_per_s": round(wl.width * wl.height / (ms / 1e3) / 1e6, 1),
+           "lookups_per_frame": int(lookups),
+           "achieved_GBps": round(lookups * BYTES_PER_LOOKUP / (ms / 1e3) / 1e9, 1),
+           "note": "the 8 MiB volume lives in L2/MALL"}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        threads = os.cpu_count() or 1
+        ow = O.OracleWorkload(wl)
+        ow.raycast(threads=threads)
+        t0 = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            ow.raycast(threads=threads)
+        cms = (time.perf_counter() - t0) / reps * 1e3
+        out["cpu_restatement"] = {"ms_per_frame": round(cms, 1), "threads": threads,
+                                  "Msamples_per_s": round(wl.width * wl.height / (cms / 1e3) / 1e6,
+                                                          2)}
+    return out
+
+
 def calibrate_copy(vrt, mib, reps):
     """Copy kernel of known size: each launch reads and writes mib MiB."""
     ctx = vrt.Context(0)
@@ -300,7 +340,8 @@ def main():
         nrc = nrc_vs_plain(pkg, ctx, wl3, oracle_stride=0 if args.no_cpu else 24)
         print(f"# time to relMSE 1e-2: {json.dumps(nrc)}", file=sys.stderr, flush=True)
 
-    other = {}
+    other = {"raycast_blobs": raycast_frame_time(pkg, ctx, cpu=not args.no_cpu)}
+    print(f"# raycast: {json.dumps(other['raycast_blobs'])}", file=sys.stderr, flush=True)
     if not args.no_mixed:
         other["mixed_surfaces_volume"] = mixed_frame_time(pkg, ctx, args.width, args.height)
         print(f"# mixed scene: {json.dumps(other['mixed_surfaces_volume'])}", file=sys.stderr,
