@@ -1,6 +1,7 @@
 """Build the native libraries in-tree.
 
 libvrt.so          gfx950 HIP kernels + the C ABI of include/vrt.h
+libanari_vrt.so    the ANARI C API front end of include/anari_vrt.h, over libvrt.so
 libvrt_scenegen.so host-only seeded scene generators (inputs for tests/bench)
 
 Run as ``python anari-visionaray_amd/build.py``; every call recompiles.
@@ -53,6 +54,16 @@ def build_vrt():
     return lib
 
 
+def build_anari():
+    os.makedirs(OUT, exist_ok=True)
+    obj = os.path.join(OUT, "anari_device.o")
+    _run([HIPCC, "-x", "hip", "-c", os.path.join(CSRC, "anari_device.cpp"), "-o", obj] + HIP_FLAGS)
+    lib = os.path.join(OUT, "libanari_vrt.so")
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", lib, obj, "-L", OUT, "-lvrt",
+          "-Wl,-rpath,$ORIGIN"])
+    return lib
+
+
 def build_scenegen():
     os.makedirs(OUT, exist_ok=True)
     lib = os.path.join(OUT, "libvrt_scenegen.so")
@@ -64,6 +75,7 @@ def build_scenegen():
 def build_all():
     build_scenegen()
     build_vrt()
+    build_anari()
 
 
 if __name__ == "__main__":

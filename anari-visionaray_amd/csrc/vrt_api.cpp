@@ -497,6 +497,43 @@ int vrt_build_value_ranges(vrt_context ctx, const vrt_volume *vol, float *ranges
   return VRT_OK;
 }
 
+int vrt_tf_build(const float *colors, int num_colors, int channels, const float *opac,
+                 int num_opac, const float uni[4], float *out) {
+  if (!uni || (colors && (num_colors < 1 || (channels != 3 && channels != 4))) ||
+      (opac && num_opac < 1))
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "bad transfer function arrays");
+  const size_t nc = colors ? size_t(num_colors) : 1, no = opac ? size_t(num_opac) : 1;
+  const size_t n = std::max(nc, no);
+  if (n > 0x7FFFFFFF) return fail(VRT_ERROR_UNSUPPORTED, "transfer function too large");
+  if (!out) return int(n);
+  // position of entry i on an array of m samples, and its lerp weight
+  auto pos = [n](size_t i, size_t m, size_t &lo, size_t &hi, float &w) {
+    const float p = n > 1 ? (float(i) / float(n - 1)) * float(m - 1) : 0.f;
+    lo = size_t(std::floor(p));
+    hi = size_t(std::ceil(p));
+    w = p - std::floor(p);
+  };
+  for (size_t i = 0; i < n; ++i) {
+    float a[4] = {uni[0], uni[1], uni[2], uni[3]}, b[4] = {uni[0], uni[1], uni[2], uni[3]};
+    size_t lo, hi;
+    float w;
+    pos(i, nc, lo, hi, w);
+    if (colors)
+      for (int k = 0; k < channels; ++k) {
+        a[k] = colors[lo * size_t(channels) + size_t(k)];
+        b[k] = colors[hi * size_t(channels) + size_t(k)];
+      }
+    float c[4];
+    for (int k = 0; k < 4; ++k) c[k] = a[k] + w * (b[k] - a[k]);
+    if (opac) {
+      pos(i, no, lo, hi, w);
+      c[3] *= opac[lo] + w * (opac[hi] - opac[lo]);
+    }
+    for (int k = 0; k < 4; ++k) out[4 * i + size_t(k)] = c[k];
+  }
+  return int(n);
+}
+
 int vrt_compute_majorants(vrt_context ctx, const vrt_volume *vol, const float *ranges, float *maj,
                           uint8_t *bytes, float *step) {
   if (int r = check_ctx(ctx)) return r;

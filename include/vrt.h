@@ -332,6 +332,19 @@ VRT_API int vrt_unstructured_destroy(vrt_unstructured field);
 VRT_API int vrt_build_value_ranges(vrt_context ctx, const vrt_volume *vol,
                                    float *ranges);
 
+/* The transferFunction1D table (TransferFunction1D::finalize,
+ * TransferFunction1D.cpp:66-111), on the host: max(num_colors,
+ * num_opacities) float4 entries (1 when neither array is given); entry i
+ * lerps the color array (color_channels 3 or 4; NULL: uniform_rgba) at
+ * i/(n-1)*(num_colors-1), with alpha uniform_rgba[3] for 3-channel colors,
+ * and multiplies alpha by the opacity array lerped the same way.
+ * uniform_rgba = {color.xyz, opacity * color.w} of the volume's uniform
+ * "color"/"opacity" parameters.  Returns the entry count and writes the
+ * entries when out_rgba is not NULL; a negative status on bad arguments. */
+VRT_API int vrt_tf_build(const float *colors, int num_colors, int color_channels,
+                         const float *opacities, int num_opacities,
+                         const float uniform_rgba[4], float *out_rgba);
+
 /* Majorants (max TF opacity over each macrocell's value range, rounded up
  * to a multiple of the 8-bit step, see vrt_volume).  Replaces
  * GridAccel::computeMaxOpacities (GridAccel.cpp:69-107).  `bytes` (one per

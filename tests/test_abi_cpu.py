@@ -9,6 +9,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -85,3 +86,60 @@ def test_splitmix64_published_values(pkg):
     vrt.gen().vrtgen_splitmix_uniform(0, 3, u.ctypes.data)
     raw = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
     np.testing.assert_array_equal(u, [(r >> 11) * 2.0 ** -53 for r in raw])
+
+
+ANARI_LIB = os.path.join(ROOT, "anari-visionaray_amd", "_build", "libanari_vrt.so")
+
+
+def test_anari_exports_every_declared_symbol(pkg):
+    lib = C.CDLL(ANARI_LIB)
+    names = _declared("anari_vrt.h", "ANARI_VRT_API")
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_tf_build_matches_oracle(pkg, oracle, case):
+    """The product's transferFunction1D table equals the oracle's
+    restatement bit for bit (colors 3/4 channels or uniform, opacities
+    or uniform, different lengths)."""
+    rng = np.random.default_rng(case)
+    nch = 3 if case % 2 else 4
+    nc = [1, 7, 256, 0, 33, 2][case]
+    no = [1, 256, 7, 64, 0, 300][case]
+    colors = rng.random((nc, nch)).astype(np.float32) if nc else None
+    opac = rng.random(no).astype(np.float32) if no else None
+    uni = np.array([0.2, 0.4, 0.6, 0.9], np.float32)
+    L = pkg.vrt.lib()
+    L.vrt_tf_build.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                               C.c_void_p]
+    cp = None if colors is None else colors.ctypes.data
+    op = None if opac is None else opac.ctypes.data
+    n = L.vrt_tf_build(cp, nc, nch, op, no, uni.ctypes.data, None)
+    assert n == max(nc or 1, no or 1)
+    out = np.empty((n, 4), np.float32)
+    assert L.vrt_tf_build(cp, nc, nch, op, no, uni.ctypes.data, out.ctypes.data) == n
+    ref = oracle.tf_build(colors, opac, uni)
+    np.testing.assert_array_equal(out, ref)
+    if nc:  # colors must have 3 or 4 channels
+        assert L.vrt_tf_build(cp, nc, 2, op, no, uni.ctypes.data, None) < 0
+
+
+def test_anari_device_without_gpu_reports_and_returns_null(pkg):
+    """No usable GPU: the direct constructor reports a fatal error through the
+    status callback and returns NULL (on a machine with a GPU the device is
+    created, released, and the check is skipped)."""
+    lib = C.CDLL(ANARI_LIB)
+    seen = []
+    CB = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                     C.c_char_p)
+    cb = CB(lambda ud, dev, src, st, sev, code, msg: seen.append((sev, code, msg)))
+    lib.anariNewVisionarayDevice.restype = C.c_void_p
+    lib.anariNewVisionarayDevice.argtypes = [CB, C.c_void_p]
+    dev = lib.anariNewVisionarayDevice(cb, None)
+    if dev:
+        lib.anariRelease.argtypes = [C.c_void_p, C.c_void_p]
+        lib.anariRelease(dev, dev)
+        pytest.skip("a GPU is present")
+    assert seen and seen[0][0] == 0 and seen[0][1] == 5  # FATAL_ERROR, UNSUPPORTED_DEVICE
