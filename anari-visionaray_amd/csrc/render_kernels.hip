@@ -884,6 +884,13 @@ __device__ inline void wave_add_counter(unsigned long long *ctr, uint32_t v) {
 // Raycast: Raycast_impl.cpp:245-322 + rayMarchVolume<0>
 // (VolumeIntegration.h:374-470)
 // ---------------------------------------------------------------------------
+// The march fetches kRaySteps steps ahead: their positions are known before
+// any of them is composited, so their voxel loads are in flight together;
+// compositing then runs step by step with the reference's tests (t <= tmax,
+// alpha < 0.99), and fetches past the end are discarded.
+constexpr int kRaySteps = 4;
+
+template <bool ANYF>
 __global__ void __launch_bounds__(256)
     k_raycast(KScene sc, KCamera cam, KParams P, KFrame F) {
   extern __shared__ float4 s_tf[];
@@ -928,19 +935,36 @@ __global__ void __launch_bounds__(256)
         const int max_steps =
             (dt > 0.f && span < 16777216.f) ? int(fmaxf(span, 0.f) * 1.001f) + 4 : 0;
         int step = 0;
-        for (float t = tmin; t <= tmax && alpha < 0.99f && step < max_steps; t += dt, ++step) {
-          const f3 p = ori + dir * t;
-          float val;
-          ++lookups;
-          if (!sample_field(v, p, val)) continue;  // VolumeIntegration.h:412
-          const float4 s = classify(v, s_tf, val);
-          const float stepT = pow_ieee(1.f - s.w, expo);
-          const float k = trans * (1.f - stepT);
-          cr += k * s.x;
-          cg += k * s.y;
-          cb += k * s.z;
-          alpha += k;
-          trans *= stepT;
+        float t = tmin;
+        bool marching = true;
+        while (marching) {
+          float val[kRaySteps];
+          bool in[kRaySteps], has[kRaySteps];
+#pragma unroll
+          for (int b = 0; b < kRaySteps; ++b) {
+            // the same t sequence as for (t = tmin; ...; t += dt)
+            in[b] = t <= tmax && step + b < max_steps;
+            has[b] = in[b] && sample_field<ANYF>(v, ori + dir * t, val[b]);
+            t += dt;
+          }
+#pragma unroll
+          for (int b = 0; b < kRaySteps; ++b) {
+            if (!in[b] || !(alpha < 0.99f)) {
+              marching = false;
+              break;
+            }
+            ++lookups;
+            if (!has[b]) continue;  // VolumeIntegration.h:412
+            const float4 s = classify(v, s_tf, val[b]);
+            const float stepT = pow_ieee(1.f - s.w, expo);
+            const float k = trans * (1.f - stepT);
+            cr += k * s.x;
+            cg += k * s.y;
+            cb += k * s.z;
+            alpha += k;
+            trans *= stepT;
+          }
+          step += kRaySteps;
         }
         // over(alpha*float4(color,1), bg) (Raycast_impl.cpp:232)
         const float a = 1.f - alpha;
@@ -1928,10 +1952,15 @@ __global__ void k_frame_reset(KFrame F) {
 static inline int tiles(const KFrame &F) { return ((F.width + 15) / 16) * ((F.height + 15) / 16); }
 
 
+static bool scene_has_other_fields(const KScene &sc);
+
 hipError_t launch_raycast(const KScene &sc, const KCamera &cam, const KParams &P, const KFrame &F,
                           hipStream_t s) {
   const size_t lds = scene_lds_bytes(sc);
-  hipLaunchKernelGGL(k_raycast, dim3(tiles(F)), dim3(256), lds, s, sc, cam, P, F);
+  if (scene_has_other_fields(sc))
+    hipLaunchKernelGGL(k_raycast<true>, dim3(tiles(F)), dim3(256), lds, s, sc, cam, P, F);
+  else
+    hipLaunchKernelGGL(k_raycast<false>, dim3(tiles(F)), dim3(256), lds, s, sc, cam, P, F);
   return hipGetLastError();
 }
 

@@ -197,13 +197,15 @@ def mixed_frame_time(pkg, ctx, width, height, frames=32, warmup=4):
             "bvh_build_ms": round(info.build_ms, 1), "setup_s": round(setup, 1)}
 
 
-def raycast_frame_time(pkg, ctx, frames=32, warmup=4, cpu=True):
-    """The ray caster on the 128^3 blob volume at 512x512 (the reference's
-    CPU-runnable configuration): GPU frame time on HIP events, trilinear
-    lookups per frame (exact: the march is deterministic) and, with `cpu`,
-    the CPU restatement timed on this host's cores for the same frame."""
+def raycast_frame_time(pkg, ctx, n=128, width=512, height=512, frames=32, warmup=4, cpu=True,
+                       note="the 8 MiB volume lives in L2/MALL"):
+    """The ray caster on an n^3 blob volume (n = 128 at 512x512 is the
+    reference's CPU-runnable configuration): GPU frame time on HIP events,
+    trilinear lookups per frame (exact: the march is deterministic) and the
+    algorithmic fetch rate; with `cpu`, the CPU restatement timed on this
+    host's cores for the same frame."""
     wls = pkg.workloads
-    wl = wls.blobs_raycast(n=128, width=512, height=512)
+    wl = wls.blobs_raycast(n=n, width=width, height=height)
     gs = wls.GpuScene(ctx, wl)
     for a in range(warmup):
         gs.render(a)
@@ -219,7 +221,8 @@ def raycast_frame_time(pkg, ctx, frames=32, warmup=4, cpu=True):
            "Msamples_per_s": round(wl.width * wl.height / (ms / 1e3) / 1e6, 1),
            "lookups_per_frame": int(lookups),
            "achieved_GBps": round(lookups * BYTES_PER_LOOKUP / (ms / 1e3) / 1e9, 1),
-           "note": "the 8 MiB volume lives in L2/MALL"}
+           "frac_of_hbm_peak": round(lookups * BYTES_PER_LOOKUP / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+           "note": note}
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as O
@@ -342,6 +345,12 @@ def main():
 
     other = {"raycast_blobs": raycast_frame_time(pkg, ctx, cpu=not args.no_cpu)}
     print(f"# raycast: {json.dumps(other['raycast_blobs'])}", file=sys.stderr, flush=True)
+    other["raycast_blobs512_1080p"] = raycast_frame_time(
+        pkg, ctx, n=512, width=1920, height=1080, frames=8, warmup=2, cpu=False,
+        note="512 MiB volume, larger than the 256 MiB MALL; algorithmic bytes count 32 B per "
+             "lookup, cache hits included")
+    print(f"# raycast 512: {json.dumps(other['raycast_blobs512_1080p'])}", file=sys.stderr,
+          flush=True)
     if not args.no_mixed:
         other["mixed_surfaces_volume"] = mixed_frame_time(pkg, ctx, args.width, args.height)
         print(f"# mixed scene: {json.dumps(other['mixed_surfaces_volume'])}", file=sys.stderr,
