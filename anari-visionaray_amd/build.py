@@ -31,7 +31,7 @@ HIP_FLAGS = [
 ]
 
 HIP_SOURCES = ["render_kernels.hip", "nrc_kernels.hip", "vrt_api.cpp", "nrc_api.cpp",
-               "nanovdb_field.cpp", "bvh_build.cpp", "unstructured_field.cpp"]
+               "nanovdb_field.cpp", "bvh_build.cpp", "bvh_gpu.hip", "unstructured_field.cpp"]
 
 
 def _run(cmd):

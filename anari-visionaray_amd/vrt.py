@@ -177,6 +177,7 @@ def lib():
         L.vrt_unstructured_destroy.argtypes = [vp]
         L.vrt_intersect_surfaces.argtypes = [vp, C.POINTER(Scene), ip, vp, ip, vp, vp]
         L.vrt_bvh_build.argtypes = [vp, vp, C.c_int64, vp, C.c_int64, C.POINTER(C.c_void_p)]
+        L.vrt_bvh_build_gpu.argtypes = [vp, vp, C.c_int64, vp, C.c_int64, C.POINTER(C.c_void_p)]
         L.vrt_bvh_get_info.argtypes = [vp, C.c_void_p]
         L.vrt_bvh_destroy.argtypes = [vp]
         L.vrt_delta_track.argtypes = [vp, C.POINTER(Scene), ip, vp, vp, vp]
@@ -577,16 +578,32 @@ def make_scene(volumes, lights=(), surfaces=()):
 
 
 class DeviceMesh:
-    """A triangle mesh's BVH (binned SAH, 4-wide) resident on the GPU."""
+    """A triangle mesh's 4-wide BVH resident on the GPU, built on the host
+    by binned SAH (builder "sah") or on the GPU as a linear BVH ("lbvh")."""
 
-    def __init__(self, ctx, vertices, indices=None):
+    def __init__(self, ctx, vertices, indices=None, builder="sah"):
         v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
-        ip, nt = None, len(v) // 3
+        ip, nt, idx = None, len(v) // 3, None
         if indices is not None:
             idx = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
             ip, nt = idx.ctypes.data, len(idx)
         h = C.c_void_p()
-        check(lib().vrt_bvh_build(ctx.handle, v.ctypes.data, len(v), ip, nt, C.byref(h)))
+        if builder == "lbvh":
+            dv = ctx.upload(v)
+            di = ctx.upload(idx) if idx is not None else None
+            ctx.sync()
+            try:
+                check(lib().vrt_bvh_build_gpu(ctx.handle, dv.ptr, len(v),
+                                              di.ptr if di is not None else None, nt,
+                                              C.byref(h)))
+            finally:
+                dv.free()
+                if di is not None:
+                    di.free()
+        elif builder == "sah":
+            check(lib().vrt_bvh_build(ctx.handle, v.ctypes.data, len(v), ip, nt, C.byref(h)))
+        else:
+            raise ValueError(f"unknown BVH builder {builder!r}")
         self.handle = h.value
         self.info = BvhInfo()
         check(lib().vrt_bvh_get_info(self.handle, C.byref(self.info)))

@@ -253,7 +253,7 @@ class GpuScene:
     """The workload resident on one GPU through the vrt C ABI."""
 
     def __init__(self, ctx, wl: Workload, color_type=None, depth=False, accum64=False,
-                 volume=None, params=None):
+                 volume=None, params=None, bvh_builder="sah"):
         self.wl = wl
         self.ctx = ctx
         if volume is None and wl.nanovdb is not None:
@@ -272,8 +272,8 @@ class GpuScene:
                                        wl.value_range, wl.unit_distance, wl.grid_dims)
         self.volume = volume
         lights = [light_desc(L) for L in wl.lights]
-        self.meshes = [(_vrt.DeviceMesh(ctx, m["vertices"], m.get("indices")), m["color"])
-                       for m in (wl.meshes or [])]
+        self.meshes = [(_vrt.DeviceMesh(ctx, m["vertices"], m.get("indices"), bvh_builder),
+                        m["color"]) for m in (wl.meshes or [])]
         self.scene = _vrt.make_scene([volume] if volume is not None else [], lights, self.meshes)
         self.camera = _vrt.camera(**wl.camera)
         p = dict(wl.params)

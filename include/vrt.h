@@ -272,6 +272,14 @@ typedef struct vrt_bvh_info {
 
 VRT_API int vrt_bvh_build(vrt_context ctx, const float *vertices, int64_t num_vertices,
                           const uint32_t *indices, int64_t num_triangles, vrt_bvh *out);
+/* The same BVH built on the GPU from device arrays (vertices: float3,
+ * indices: uint3 or NULL for a soup), for meshes rebuilt at every commit:
+ * a linear BVH (Morton order, Karras 2012) collapsed into the same 4-wide
+ * layout, leaves of <= 4 triangles.  Hits are identical to the host build's
+ * (closest hit, ties to the lower triangle index); the tree differs.
+ * Replaces the reference's host build + upload (DeviceBVH.h:153-255, 320). */
+VRT_API int vrt_bvh_build_gpu(vrt_context ctx, const float *vertices, int64_t num_vertices,
+                              const uint32_t *indices, int64_t num_triangles, vrt_bvh *out);
 VRT_API int vrt_bvh_get_info(vrt_bvh bvh, vrt_bvh_info *info);
 VRT_API int vrt_bvh_destroy(vrt_bvh bvh);
 

@@ -21,8 +21,9 @@ def _rays(rng, n, lo=-1.5, hi=1.5):
     return r
 
 
+@pytest.mark.parametrize("builder", ["sah", "lbvh"])
 @pytest.mark.parametrize("kind", ["soup", "icosphere"])
-def test_bvh_hits_bit_exact(pkg, oracle, gpu_ctx, kind):
+def test_bvh_hits_bit_exact(pkg, oracle, gpu_ctx, kind, builder):
     vrt = pkg.vrt
     rng = np.random.default_rng(6)
     if kind == "soup":
@@ -31,7 +32,9 @@ def test_bvh_hits_bit_exact(pkg, oracle, gpu_ctx, kind):
         idx = None
     else:
         v, idx = vrt.gen_icosphere(level=5, radius=0.8)
-    mesh = vrt.DeviceMesh(gpu_ctx, v, idx)
+    mesh = vrt.DeviceMesh(gpu_ctx, v, idx, builder)
+    print(builder, kind, "nodes", mesh.info.num_nodes, "depth", mesh.info.depth,
+          "build ms", round(mesh.info.build_ms, 2))
     scene = vrt.make_scene([], [], [(mesh, (0.7, 0.7, 0.7))])
     om = oracle.OracleMesh(v, idx)
     r = _rays(rng, 20000)
@@ -95,3 +98,44 @@ def test_quad_light_on_volume(pkg, oracle, gpu_ctx):
     agree = oracle.stream_agreement(g, o)
     print("quad light stream agreement", agree)
     assert agree >= oracle.STREAM_AGREEMENT_MIN, agree
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 64])
+def test_lbvh_small_and_duplicate_triangles(pkg, oracle, gpu_ctx, n):
+    """Tiny meshes (a lone leaf, roots of <= 4 triangles) and repeated
+    triangles (equal Morton codes, split by index) give the brute-force hits."""
+    vrt = pkg.vrt
+    rng = np.random.default_rng(n)
+    base = rng.uniform(-0.6, 0.6, ((n + 1) // 2, 3, 3)).astype(np.float32)
+    tris = np.concatenate([base, base])[:n]
+    v = tris.reshape(-1, 3)
+    mesh = vrt.DeviceMesh(gpu_ctx, v, None, "lbvh")
+    assert mesh.info.num_triangles == n and mesh.info.num_nodes >= 1
+    scene = vrt.make_scene([], [], [(mesh, (0.7, 0.7, 0.7))])
+    r = _rays(rng, 4000, -1.0, 1.0)
+    t, p = vrt.intersect_surfaces(gpu_ctx, scene, r, False)
+    to, po = oracle.mesh_intersect(oracle.OracleMesh(v), r, brute=True)
+    np.testing.assert_array_equal(p, po)
+    np.testing.assert_array_equal(t, to)
+
+
+def test_lbvh_rejects_bad_indices(pkg, gpu_ctx):
+    vrt = pkg.vrt
+    v = np.zeros((4, 3), np.float32)
+    with pytest.raises(vrt.VrtError, match="out of range"):
+        vrt.DeviceMesh(gpu_ctx, v, np.array([[0, 1, 2], [1, 2, 7]], np.uint32), "lbvh")
+
+
+def test_mixed_scene_lbvh_same_image(pkg, gpu_ctx):
+    """Hits do not depend on the tree: the GPU-built BVH renders the image of
+    the host-built one."""
+    wl = _mixed(pkg)
+    imgs = []
+    for builder in ("sah", "lbvh"):
+        gs = pkg.workloads.GpuScene(gpu_ctx, wl, bvh_builder=builder)
+        gs.frame.reset()
+        for a in range(4):
+            gs.render(a)
+        gpu_ctx.sync()
+        imgs.append(gs.frame.color_array())
+    np.testing.assert_array_equal(imgs[0], imgs[1])
