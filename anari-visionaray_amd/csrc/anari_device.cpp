@@ -555,9 +555,41 @@ vrt_bvh build_geometry(Device *d, Geometry *g) {
     return nullptr;
   }
   const int64_t ntri = idx ? int64_t(idx->count()) : int64_t(pos->count() / 3);
-  if (vrt_bvh_build(d->ctx, reinterpret_cast<const float *>(pos->data()), int64_t(pos->count()),
-                    idx ? reinterpret_cast<const uint32_t *>(idx->data()) : nullptr, ntri,
-                    &g->bvh) != VRT_OK) {
+  // extension "bvhBuilder": "sah" (default: the reference's host binned-SAH
+  // build, Triangle.cpp:200-202) or "lbvh" (built on the GPU from uploaded
+  // copies of the arrays, for geometry re-committed every frame)
+  const std::string builder = g->get_s("bvhBuilder", "sah");
+  int rc = VRT_OK;
+  if (builder == "lbvh") {
+    const size_t pbytes = pos->count() * 12, ibytes = idx ? idx->count() * 12 : 0;
+    void *dp = nullptr, *di = nullptr;
+    hipError_t e = hipMalloc(&dp, pbytes);
+    if (e == hipSuccess && idx) e = hipMalloc(&di, ibytes);
+    if (e == hipSuccess) e = hipMemcpyAsync(dp, pos->data(), pbytes, hipMemcpyHostToDevice, d->stream);
+    if (e == hipSuccess && idx)
+      e = hipMemcpyAsync(di, idx->data(), ibytes, hipMemcpyHostToDevice, d->stream);
+    if (e == hipSuccess) {
+      rc = vrt_bvh_build_gpu(d->ctx, static_cast<const float *>(dp), int64_t(pos->count()),
+                             static_cast<const uint32_t *>(di), ntri, &g->bvh);
+    } else {
+      d->report(g, ANARI_SEVERITY_ERROR, ANARI_STATUS_OUT_OF_MEMORY, "BVH upload: %s",
+                hipGetErrorString(e));
+      g->bvh = nullptr;
+    }
+    (void)hipStreamSynchronize(d->stream);
+    (void)hipFree(dp);
+    (void)hipFree(di);
+    if (e != hipSuccess) return nullptr;
+  } else {
+    if (builder != "sah")
+      d->report(g, ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
+                "unknown bvhBuilder '%s': using 'sah'", builder.c_str());
+    rc = vrt_bvh_build(d->ctx, reinterpret_cast<const float *>(pos->data()),
+                       int64_t(pos->count()),
+                       idx ? reinterpret_cast<const uint32_t *>(idx->data()) : nullptr, ntri,
+                       &g->bvh);
+  }
+  if (rc != VRT_OK) {
     d->vrt_error(g, "triangle BVH");
     g->bvh = nullptr;
   }
@@ -1026,7 +1058,28 @@ int anariGetProperty(ANARIDevice dh, ANARIObject oh, const char *name, ANARIData
                      void *mem, uint64_t size, ANARIWaitMask mask) {
   Device *d = as_device(dh);
   Object *o = as_obj(oh);
-  if (!d || !o || !name || !mem || o->kind != ANARI_FRAME) return 0;
+  if (!d || !o || !name || !mem) return 0;
+  if (o->kind == ANARI_GEOMETRY) {
+    // extension: the triangle BVH of the last build ("bvhBuildTime" in
+    // seconds, "bvhNodes"); the build itself happens when a frame uses the
+    // geometry after its commit
+    d->flush();
+    const vrt_bvh b = static_cast<Geometry *>(o)->bvh;
+    vrt_bvh_info info{};
+    if (!b || vrt_bvh_get_info(b, &info) != VRT_OK) return 0;
+    if (!std::strcmp(name, "bvhBuildTime") && t == ANARI_FLOAT32 && size >= 4) {
+      const float sec = float(info.build_ms * 1e-3);
+      std::memcpy(mem, &sec, 4);
+      return 1;
+    }
+    if (!std::strcmp(name, "bvhNodes") && (t == ANARI_INT32 || t == ANARI_UINT32) && size >= 4) {
+      const uint32_t n = info.num_nodes;
+      std::memcpy(mem, &n, 4);
+      return 1;
+    }
+    return 0;
+  }
+  if (o->kind != ANARI_FRAME) return 0;
   Frame *fr = static_cast<Frame *>(o);
   if (mask == ANARI_WAIT) (void)hipStreamSynchronize(d->stream);
   if (!std::strcmp(name, "duration") && t == ANARI_FLOAT32 && size >= 4) {

@@ -173,14 +173,15 @@ def nrc_vs_plain(pkg, ctx, wl, volume=None, ref_frames=2048, cap=1024, warmup=4,
     return out
 
 
-def mixed_frame_time(pkg, ctx, width, height, frames=32, warmup=4):
+def mixed_frame_time(pkg, ctx, width, height, frames=32, warmup=4, builder="sah"):
     """Plain path-tracer frame time on the mixed surface + volume scene:
-    1,310,720-triangle icosphere (BVH built on the host), 256^3 blob
-    volume, quad light, <= 6 bounces, 1 spp per frame."""
+    1,310,720-triangle icosphere, 256^3 blob volume, quad light, <= 6
+    bounces, 1 spp per frame; the mesh BVH built on the host by binned SAH
+    ("sah") or on the GPU as a linear BVH ("lbvh")."""
     wls = pkg.workloads
     t0 = time.perf_counter()
     wl = wls.mixed_pathtrace(level=8, vol_n=256, width=width, height=height, frames=frames)
-    gs = wls.GpuScene(ctx, wl)
+    gs = wls.GpuScene(ctx, wl, bvh_builder=builder)
     setup = time.perf_counter() - t0
     info = gs.meshes[0][0].info
     gs.frame.reset()
@@ -191,10 +192,11 @@ def mixed_frame_time(pkg, ctx, width, height, frames=32, warmup=4):
     for a in range(frames):
         gs.render(warmup + a)
     ms = ctx.timer_stop() / frames
-    return {"workload": wl.name, "ms_per_frame": round(ms, 4),
+    return {"workload": wl.name, "bvh_builder": builder, "ms_per_frame": round(ms, 4),
             "Msamples_per_s": round(width * height / (ms / 1e3) / 1e6, 2),
             "triangles": int(info.num_triangles), "bvh_nodes": int(info.num_nodes),
-            "bvh_build_ms": round(info.build_ms, 1), "setup_s": round(setup, 1)}
+            "bvh_depth": int(info.depth), "bvh_build_ms": round(info.build_ms, 1),
+            "setup_s": round(setup, 1)}
 
 
 def raycast_frame_time(pkg, ctx, n=128, width=512, height=512, frames=32, warmup=4, cpu=True,
@@ -355,6 +357,10 @@ def main():
         other["mixed_surfaces_volume"] = mixed_frame_time(pkg, ctx, args.width, args.height)
         print(f"# mixed scene: {json.dumps(other['mixed_surfaces_volume'])}", file=sys.stderr,
               flush=True)
+        other["mixed_surfaces_volume_lbvh"] = mixed_frame_time(pkg, ctx, args.width,
+                                                               args.height, builder="lbvh")
+        print(f"# mixed scene, GPU-built BVH: "
+              f"{json.dumps(other['mixed_surfaces_volume_lbvh'])}", file=sys.stderr, flush=True)
     if args.c4:
         t0 = time.perf_counter()
         wl4 = wls.tets_pathtrace()

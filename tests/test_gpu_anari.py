@@ -32,7 +32,7 @@ def _tf_table(pkg, colors, opacity):
     return out
 
 
-def _anari_scene(a, wl, xfm=None):
+def _anari_scene(a, wl, xfm=None, bvh_builder=None):
     """World, camera, renderer and frame of a workloads.Workload; returns
     (frame, objects by name)."""
     objs = {}
@@ -88,6 +88,9 @@ def _anari_scene(a, wl, xfm=None):
     for m in wl.meshes or []:
         g = a.new(A.GEOMETRY, "triangle")
         a.set(g, "vertex.position", A.ARRAY1D, a.array1d(m["vertices"], A.FLOAT32_VEC3))
+        if bvh_builder is not None:
+            a.set(g, "bvhBuilder", A.STRING, bvh_builder)
+        objs.setdefault("geometries", []).append(g)
         if m.get("indices") is not None:
             a.set(g, "primitive.index", A.ARRAY1D, a.array1d(m["indices"], A.UINT32_VEC3))
         mat = a.new(A.MATERIAL, "matte")
@@ -275,5 +278,34 @@ def test_color_channel_types_and_errors(pkg, gpu_ctx):
         n = len(a.messages)
         a.render(fr)
         assert any("amr" in msg for _, _, msg in a.messages[n:])
+    finally:
+        a.close()
+
+
+def test_geometry_gpu_bvh_builder(pkg, gpu_ctx):
+    """Extension parameter bvhBuilder = "lbvh" builds the triangle BVH on the
+    GPU: the same images as the host SAH build, build time and node count as
+    geometry properties, and a switch back to "sah" on re-commit."""
+    wl = _same_tf(pkg, CASES["mixed surfaces"](pkg.workloads))
+    ref = _vrt_frames(pkg, gpu_ctx, wl, 2)
+    a = A.Anari()
+    try:
+        fr, objs = _anari_scene(a, wl, bvh_builder="lbvh")
+        g = objs["geometries"][0]
+        for k in range(2):
+            a.render(fr)
+            np.testing.assert_array_equal(a.map(fr)[0], ref[k])
+        t_gpu, n_gpu = a.prop_f(g, "bvhBuildTime"), a.prop_i(g, "bvhNodes")
+        assert t_gpu is not None and t_gpu > 0 and n_gpu > 0
+        a.set(g, "bvhBuilder", A.STRING, "sah")
+        a.commit(g)
+        a.render(fr)
+        np.testing.assert_array_equal(a.map(fr)[0], ref[0])  # accumulation restarted
+        assert a.prop_i(fr, "numSamples") == 1
+        assert a.prop_i(g, "bvhNodes") > 0
+        print("lbvh", t_gpu * 1e3, "ms", n_gpu, "nodes; sah", a.prop_f(g, "bvhBuildTime") * 1e3,
+              "ms", a.prop_i(g, "bvhNodes"), "nodes")
+        errors = [m for m in a.messages if m[0] <= 1]
+        assert not errors, errors
     finally:
         a.close()
