@@ -54,6 +54,7 @@ struct KVolume {
   int maj_lds_offset;     // byte offset of this grid in the workgroup's LDS, -1: global
   float A[9];          // affineInv
   f3 tinv;             // transInv
+  uint32_t obj_id, inst_id;
   f3 glo, ghi;         // majorant grid bounds in voxel space
   // nanovdb: top table over 128^3 regions (>= 0 block index, < 0: -1 -
   // index into nv_consts), blocks of 16^3 slot entries (>= 0 leaf index
@@ -72,6 +73,7 @@ struct KVolume {
   // tetrahedron {bbox min, value min}, {bbox max, value max}
   const uint32_t *ug_start;
   const float4 *ug_recs;
+  const uint32_t *ug_cell;  // per list entry: primitive id (ANARI cell index)
   const float4 *ug_tets;
   uint32_t ug_num_tets;
   f3 ug_lo, ug_inv;
@@ -92,6 +94,7 @@ struct KSurface {
   const float4 *nodes;  // 4-wide BVH nodes, 8 float4 each
   const float4 *tris;   // {v1, prim id bits}, e1, e2 per triangle
   f3 color;             // matte
+  uint32_t obj_id, inst_id;
 };
 
 struct KScene {
@@ -134,6 +137,9 @@ struct KFrame {
   float *depth;
   double *accum64;
   unsigned long long *counters;
+  float *normal;  // g-buffer (float3 per pixel), optional
+  float *albedo;
+  uint32_t *prim_id, *obj_id, *inst_id;
 };
 
 // Per-frame buffers of the radiance-cache path tracer.  Paths that are not

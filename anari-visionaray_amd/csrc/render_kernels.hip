@@ -275,7 +275,7 @@ __device__ inline bool tet_value(f3 P, float4 A, float4 B, float4 C, float4 D, f
 // test is inclusive), map to the grid cell of P (float arithmetic the list
 // construction allows for, the top face clamped into the last cell); its
 // tetrahedra are tested in list order until one contains P.
-__device__ inline bool sample_tets(const KVolume &v, f3 P, float &value) {
+__device__ inline bool sample_tets(const KVolume &v, f3 P, float &value, uint32_t &rec) {
   if (!(P.x >= v.blo.x && P.x <= v.bhi.x && P.y >= v.blo.y && P.y <= v.bhi.y &&
         P.z >= v.blo.z && P.z <= v.bhi.z))
     return false;  // also on NaN
@@ -287,7 +287,10 @@ __device__ inline bool sample_tets(const KVolume &v, f3 P, float &value) {
   const uint32_t b = v.ug_start[c], e = v.ug_start[c + 1];
   for (uint32_t i = b; i < e; ++i) {
     const float4 *r = v.ug_recs + size_t(i) * 4u;
-    if (tet_value(P, r[0], r[1], r[2], r[3], value)) return true;
+    if (tet_value(P, r[0], r[1], r[2], r[3], value)) {
+      rec = i;
+      return true;
+    }
   }
   return false;
 }
@@ -295,12 +298,23 @@ __device__ inline bool sample_tets(const KVolume &v, f3 P, float &value) {
 // sampleField: voxel-space point -> field value; false where the field has
 // no value (outside every cell of an unstructured field).  ANYF = false
 // compiles the structuredRegular fetch alone (scenes without other fields).
+// `rec`: the list entry of the containing tetrahedron (kNoRec for the other
+// fields, whose primitive id is 0, DeviceCopyableObjects.h:478-481, 556-567).
+constexpr uint32_t kNoRec = 0xFFFFFFFFu;
+
 template <bool ANYF = true>
-__device__ inline bool sample_field(const KVolume &v, f3 P, float &value) {
-  if (ANYF && v.field_type == VRT_FIELD_UNSTRUCTURED) return sample_tets(v, P, value);
+__device__ inline bool sample_field(const KVolume &v, f3 P, float &value, uint32_t &rec) {
+  rec = kNoRec;
+  if (ANYF && v.field_type == VRT_FIELD_UNSTRUCTURED) return sample_tets(v, P, value, rec);
   value = (ANYF && v.field_type == VRT_FIELD_NANOVDB) ? sample_nanovdb(v, P)
                                                       : sample_structured(v, P);
   return true;
+}
+
+template <bool ANYF = true>
+__device__ inline bool sample_field(const KVolume &v, f3 P, float &value) {
+  uint32_t rec;
+  return sample_field<ANYF>(v, P, value, rec);
 }
 
 // postClassify (DeviceCopyableObjects.h:615-619): tex1D(linear, clamp) of the
@@ -575,6 +589,7 @@ struct VolHit {
   float t;
   f3 albedo;
   float extinction;  // classified opacity at the collision (HitRecordVolume::extinction)
+  uint32_t rec;      // sample_field's record of the collision
   bool hit;
 };
 
@@ -662,7 +677,7 @@ __device__ inline bool trav_begin(const KVolume &v, KRay ray, Trav &T) {
 template <bool ANYF = true>
 __device__ inline bool trav_step(const KVolume &v, const float4 *lds, const uint8_t *mb,
                                  float mstep, Trav &T, uint32_t &rs, uint32_t &lookups,
-                                 bool &hit, float &t_hit, float4 &smp) {
+                                 bool &hit, float &t_hit, float4 &smp, uint32_t &rec_hit) {
   bool advance = false;
   if (!T.in_cell) {
     float a0, a1, maj;
@@ -700,15 +715,17 @@ __device__ inline bool trav_step(const KVolume &v, const float4 *lds, const uint
       advance = true;
     } else {
       float val;
+      uint32_t rec;
       ++lookups;
       // no acceptance draw where the field has no value
       // (DeviceCopyableObjects.h:739-753)
-      if (sample_field<ANYF>(v, T.ori + T.dir * T.t, val)) {
+      if (sample_field<ANYF>(v, T.ori + T.dir * T.t, val, rec)) {
         // colour only for an accepted collision
         if (classify_alpha(v, lds, val) >= rnd(rs) * T.maj) {
           hit = true;
           t_hit = T.t;
           smp = classify(v, lds, val);
+          rec_hit = rec;
           return true;
         }
       }
@@ -739,9 +756,11 @@ __device__ inline bool trav_step(const KVolume &v, const float4 *lds, const uint
   return false;
 }
 
-__device__ inline VolHit make_hit(bool hit, float t_hit, float4 smp, float dt_scale) {
+__device__ inline VolHit make_hit(bool hit, float t_hit, float4 smp, float dt_scale,
+                                  uint32_t rec) {
   VolHit out;
   out.hit = hit;
+  out.rec = rec;
   out.t = hit ? t_hit / dt_scale : FLT_MAX;
   out.albedo = mk(smp.x, smp.y, smp.z);
   out.extinction = smp.w;
@@ -756,10 +775,11 @@ __device__ inline VolHit delta_track(const KVolume &v, const float4 *lds, const 
   bool hit = false;
   float t_hit = 0.f;
   float4 smp = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (!trav_begin(v, ray, T)) return make_hit(false, 0.f, smp, 1.f);
-  while (!trav_step<ANYF>(v, lds, mb, mstep, T, rs, lookups, hit, t_hit, smp)) {
+  uint32_t rec = kNoRec;
+  if (!trav_begin(v, ray, T)) return make_hit(false, 0.f, smp, 1.f, rec);
+  while (!trav_step<ANYF>(v, lds, mb, mstep, T, rs, lookups, hit, t_hit, smp, rec)) {
   }
-  return make_hit(hit, t_hit, smp, T.dt_scale);
+  return make_hit(hit, t_hit, smp, T.dt_scale, rec);
 }
 
 // ---------------------------------------------------------------------------
@@ -890,6 +910,41 @@ __device__ inline void wave_add_counter(unsigned long long *ctr, uint32_t v) {
 // alpha < 0.99), and fetches past the end are discarded.
 constexpr int kRaySteps = 4;
 
+// Frame::fillGBuffer (DeviceCopyableObjects.h:2745-2774): the channels other
+// than depth take the values of the pixel's closest sample (first of equal
+// depths, Pathtrace_impl.cpp:616) when the frame has a depth channel and that
+// depth is <= the stored one.  The path tracer writes them at a sample's
+// first hit when it is closer than the pixel's earlier samples and passes
+// the depth test, which leaves the same values: the last write is the
+// closest sample, and it passes exactly when the reference's test does.
+__device__ inline bool gbuf_wanted(const KFrame &F) {
+  return F.depth && (F.normal || F.albedo || F.prim_id || F.obj_id || F.inst_id);
+}
+
+__device__ inline void gbuf_write(const KFrame &F, size_t i, f3 n, f3 a, uint32_t prim,
+                                  uint32_t obj, uint32_t inst) {
+  if (F.normal) {
+    F.normal[3 * i] = n.x;
+    F.normal[3 * i + 1] = n.y;
+    F.normal[3 * i + 2] = n.z;
+  }
+  if (F.albedo) {
+    F.albedo[3 * i] = a.x;
+    F.albedo[3 * i + 1] = a.y;
+    F.albedo[3 * i + 2] = a.z;
+  }
+  if (F.prim_id) F.prim_id[i] = prim;
+  if (F.obj_id) F.obj_id[i] = obj;
+  if (F.inst_id) F.inst_id[i] = inst;
+}
+
+// a pixel none of whose samples hit a volume or surface: the default
+// PixelSample (frame/common.h:15-26: ids ~0u; normal and albedo, left
+// unset there, written as 0) at depth 1e31
+__device__ inline void gbuf_miss(const KFrame &F, size_t i) {
+  if (1e31f <= F.depth[i]) gbuf_write(F, i, mk(0.f), mk(0.f), ~0u, ~0u, ~0u);
+}
+
 template <bool ANYF>
 __global__ void __launch_bounds__(256)
     k_raycast(KScene sc, KCamera cam, KParams P, KFrame F) {
@@ -901,6 +956,7 @@ __global__ void __launch_bounds__(256)
   uint32_t lookups = 0;
   if (active) {
     float4 result = P.bg;
+    int hit_vol = -1;
     if (sc.num_volumes > 0) {
       KRay ray = primary_ray(cam, float(x), float(y), float(F.width), float(F.height));
       clip_ray(ray, P);
@@ -918,6 +974,7 @@ __global__ void __launch_bounds__(256)
           best = i;
         }
       }
+      hit_vol = best;
       if (best >= 0) {
         const KVolume &v = sc.vol[best];
         KRay lr = to_local(v, ray);
@@ -971,6 +1028,19 @@ __global__ void __launch_bounds__(256)
         result = make_float4(alpha * cr + a * result.x, alpha * cg + a * result.y,
                              alpha * cb + a * result.z, alpha + a * result.w);
       }
+    }
+    if (gbuf_wanted(F)) {
+      // a volume box hit sets the object and instance ids, normal and
+      // albedo 0; depth and primitive id keep their defaults
+      // (Raycast_impl.cpp:18-20, 230-237)
+      uint32_t obj = ~0u, inst = ~0u;
+      for (int i = 0; i < sc.num_volumes; ++i)
+        if (i == hit_vol) {
+          obj = sc.vol[i].obj_id;
+          inst = sc.vol[i].inst_id;
+        }
+      const size_t i = size_t(y) * F.width + x;
+      if (1e31f <= F.depth[i]) gbuf_write(F, i, mk(0.f), mk(0.f), ~0u, obj, inst);
     }
     write_sample(F, x, y, 0, result, 1e31f, false);
   }
@@ -1107,8 +1177,8 @@ struct PassHit {
   float t;
   f3 albedo;         // volume: classified color
   float extinction;  // volume: classified opacity
-  int id;            // surface or light index
-  uint32_t rec;      // surface: triangle record
+  int id;            // surface, light or volume index
+  uint32_t rec;      // surface: triangle record; volume: sample_field's record
 };
 
 // AREA: the scene has surfaces or quad lights (else the kernel is built
@@ -1149,10 +1219,14 @@ __device__ inline PassHit intersect_all(const KScene &sc, const KRay &ray, bool 
   VolHit vh;
   vh.hit = false;
   vh.t = FLT_MAX;
+  int vid = -1;
   for (int i = 0; i < sc.num_volumes; ++i) {
     VolHit h = delta_track<ANYF>(sc.vol[i], s_tf, lds_majorants(sc, s_tf, i), steps[i],
                                  to_local(sc.vol[i], ray), srng, lookups);
-    if (h.hit && h.t < vh.t) vh = h;
+    if (h.hit && h.t < vh.t) {
+      vh = h;
+      vid = i;
+    }
   }
   PassHit out;
   out.kind = kNone;
@@ -1169,6 +1243,8 @@ __device__ inline PassHit intersect_all(const KScene &sc, const KRay &ray, bool 
     out.t = vh.t;
     out.albedo = vh.albedo;
     out.extinction = vh.extinction;
+    out.id = vid;
+    out.rec = vh.rec;
   } else if (sh) {
     out.kind = kSurface;
     out.t = st;
@@ -1180,7 +1256,8 @@ __device__ inline PassHit intersect_all(const KScene &sc, const KRay &ray, bool 
     out.t = vh.hit ? vh.t : lt;
     out.albedo = vh.albedo;
     out.extinction = vh.extinction;
-    out.id = lid;
+    out.id = vh.hit ? vid : lid;
+    out.rec = vh.hit ? vh.rec : 0u;
   }
   return out;
 }
@@ -1246,7 +1323,7 @@ __device__ inline bool roulette(PathState &st) {
 // training path -- records the vertex and goes on.
 template <bool NRC, bool AREA>
 __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams &P,
-                                  const KNrc &N, const PassHit &h) {
+                                  const KNrc &N, const KFrame &F, const PassHit &h) {
   KRay ray;
   ray.ori = st.ori;
   ray.dir = st.dir;
@@ -1316,24 +1393,49 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
     const f3 hitPos = ray.ori + ray.dir * h.t;
     const float eps = epsilon_from(hitPos, ray.dir, h.t);
     f3 gn = mk(0.f), color;
+    const bool gbuf = st.bounceID == 0 && gbuf_wanted(F) && h.t < st.depth &&
+                      h.t <= F.depth[st.pixel];
     if (h.kind == kVolume) {
       if (P.ambient_samples > 0) {
         const float u1 = rnd(st.srng), u2 = rnd(st.srng);
         gn = uniform_sample_sphere(u1, u2);
+        // normal transform transpose(affineInv) (Pathtrace_impl.cpp:217-219,
+        // Instance.cpp:96-98), as mat3 * vec3 = col0 x + col1 y + col2 z
+        float A[9];
+        for (int k = 0; k < 9; ++k) A[k] = sc.vol[0].A[k];
+        for (int i = 1; i < sc.num_volumes; ++i)
+          if (i == h.id)
+            for (int k = 0; k < 9; ++k) A[k] = sc.vol[i].A[k];
+        gn = mk(A[0] * gn.x + A[3] * gn.y + A[6] * gn.z, A[1] * gn.x + A[4] * gn.y + A[7] * gn.z,
+                A[2] * gn.x + A[5] * gn.y + A[8] * gn.z);
       }
       st.sn = faceforward(gn, viewDir, gn);
       color = h.albedo;
+      if (gbuf) {
+        uint32_t obj = 0, inst = 0, prim = 0;
+        for (int i = 0; i < sc.num_volumes; ++i)
+          if (i == h.id) {
+            obj = sc.vol[i].obj_id;
+            inst = sc.vol[i].inst_id;
+            if (h.rec != kNoRec) prim = sc.vol[i].ug_cell[h.rec];
+          }
+        gbuf_write(F, size_t(st.pixel), faceforward(gn, viewDir, gn), color, prim, obj, inst);
+      }
     } else if (AREA) {
       // triangle: Ng = normalize(e1 x e2), no vertex normals (getNormals,
-      // common.h:233-264), matte color
+      // common.h:233-264), normalised again after the (identity) normal
+      // transform (Pathtrace_impl.cpp:243-247); matte color
       KSurface S = sc.surf[0];
       for (int i = 1; i < sc.num_surfaces; ++i)
         if (i == h.id) S = sc.surf[i];
+      const float4 v1 = S.tris[3 * size_t(h.rec)];
       const float4 e1 = S.tris[3 * size_t(h.rec) + 1], e2 = S.tris[3 * size_t(h.rec) + 2];
-      gn = normalize(cross(mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z)));
+      gn = normalize(normalize(cross(mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z))));
       st.sn = faceforward(gn, viewDir, gn);
       gn = faceforward(gn, viewDir, gn);
       color = S.color;
+      if (gbuf)
+        gbuf_write(F, size_t(st.pixel), gn, color, __float_as_uint(v1.w), S.obj_id, S.inst_id);
     }
     LightSmp ls;
     ls.intensity = mk(0.f);
@@ -1544,7 +1646,7 @@ __global__ void __launch_bounds__(256)
       clip_ray(ray, P);  // every pass (Pathtrace_impl.cpp:559)
       const PassHit hit = intersect_all<AREA, ANYF>(sc, ray, st.rayType != Radiance, st.bounceID,
                                                     s_tf, steps, st.srng, lookups, bstack);
-      if (path_shade<NRC, AREA>(st, sc, P, N, hit)) {
+      if (path_shade<NRC, AREA>(st, sc, P, N, F, hit)) {
         float4 color = P.bg;
         if (st.firstHit) color = make_float4(st.intensity.x, st.intensity.y, st.intensity.z, 1.f);
         st.acc.x += color.x;
@@ -1566,6 +1668,7 @@ __global__ void __launch_bounds__(256)
           // one sample per pixel: the resolve pass adds the cache term
           N.sample[st.pixel] = st.acc;
           N.query_of_pixel[st.pixel] = st.qslot;
+          if (gbuf_wanted(F) && !(st.depth < 1e31f)) gbuf_miss(F, size_t(st.pixel));
           if (F.depth && st.depth <= F.depth[st.pixel]) F.depth[st.pixel] = st.depth;
           st.pixel = -1;
         } else if (++st.sampleID < P.pixel_samples) {
@@ -1574,6 +1677,7 @@ __global__ void __launch_bounds__(256)
           const float inv = 1.f / float(P.pixel_samples);
           const float4 a = make_float4(st.acc.x * inv, st.acc.y * inv, st.acc.z * inv,
                                        st.acc.w * inv);
+          if (gbuf_wanted(F) && !(st.depth < 1e31f)) gbuf_miss(F, size_t(st.pixel));
           write_sample(F, x, y, accumID, a, st.depth, true);
           st.pixel = -1;
         }

@@ -36,6 +36,7 @@ struct Grid {
   int dims[3];
   std::vector<uint32_t> start;  // dims product + 1
   std::vector<Rec> recs;        // entries, cell-major
+  std::vector<uint32_t> rec_tet;  // per entry: index of its tetrahedron in the input
   double mean_list = 0, expected_tests = 0;
   uint32_t max_list = 0;
 };
@@ -224,6 +225,7 @@ inline bool build(const std::vector<Rec> &tets, const float lo[3], const float h
   // order each list: most covered first (4^3 stratified points per cell,
   // each credited to its first containing tetrahedron by id), then by id
   g.recs.resize(total);
+  g.rec_tet.resize(total);
   std::vector<double> tests(nc, 0.0);
   parallel_for(nc, threads, [&](size_t c) {
     const uint32_t b = g.start[c], e = g.start[c + 1];
@@ -259,6 +261,7 @@ inline bool build(const std::vector<Rec> &tets, const float lo[3], const float h
     int covered = 0;
     for (uint32_t i = 0; i < k; ++i) {
       g.recs[b + i] = tets[L[order[i]]];
+      g.rec_tet[b + i] = L[order[i]];
       et += double(i + 1) * hits[order[i]];
       covered += hits[order[i]];
     }

@@ -52,7 +52,9 @@ int vrt_unstructured_create(vrt_context ctx, const float *pos, const float *vdat
 
   std::vector<tetgrid::Rec> recs;
   std::vector<float4> tets;  // {bbox min, value min}, {bbox max, value max}
+  std::vector<uint32_t> cell_of;  // kept tetrahedron -> ANARI cell index
   recs.reserve(size_t(nc));
+  cell_of.reserve(size_t(nc));
   tets.reserve(2 * size_t(nc));
   uint64_t skipped = 0;
   float blo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, bhi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
@@ -100,6 +102,7 @@ int vrt_unstructured_create(vrt_context ctx, const float *pos, const float *vdat
     // no data at the first vertex: the cell value (DeviceCopyableObjects.h:150-153)
     if (std::isnan(r.v[0][3])) r.v[1][3] = cell_value;
     recs.push_back(r);
+    cell_of.push_back(uint32_t(c));
     tets.push_back(make_float4(bmn[0], bmn[1], bmn[2], vmn));
     tets.push_back(make_float4(bmx[0], bmx[1], bmx[2], vmx));
     for (int a = 0; a < 3; ++a) {
@@ -138,24 +141,34 @@ int vrt_unstructured_create(vrt_context ctx, const float *pos, const float *vdat
   auto cleanup = [&](hipError_t e, const char *what) {
     (void)hipFree(h->start);
     (void)hipFree(h->recs);
+    (void)hipFree(h->rec_cell);
     (void)hipFree(h->tets);
     delete h;
     return hip_fail(e, what);
   };
+  // the primitive id of a list entry (UnstructuredField.cpp:146-151: elemID =
+  // cell index), read only for the g-buffer
+  std::vector<uint32_t> rec_cell(g.rec_tet.size());
+  for (size_t i = 0; i < rec_cell.size(); ++i) rec_cell[i] = cell_of[g.rec_tet[i]];
+  std::vector<uint32_t>().swap(g.rec_tet);
   const size_t start_bytes = g.start.size() * 4, rec_bytes = g.recs.size() * sizeof(tetgrid::Rec),
-               tet_bytes = tets.size() * sizeof(float4);
+               cell_bytes = rec_cell.size() * 4, tet_bytes = tets.size() * sizeof(float4);
   hipError_t e = hipMalloc(&h->start, start_bytes);
   if (e == hipSuccess) e = hipMalloc(&h->recs, std::max<size_t>(rec_bytes, 64));
+  if (e == hipSuccess) e = hipMalloc(&h->rec_cell, std::max<size_t>(cell_bytes, 4));
   if (e == hipSuccess) e = hipMalloc(&h->tets, tet_bytes);
   if (e != hipSuccess) return cleanup(e, "unstructured field allocation");
   e = hipMemcpyAsync(h->start, g.start.data(), start_bytes, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess && rec_bytes)
     e = hipMemcpyAsync(h->recs, g.recs.data(), rec_bytes, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess && cell_bytes)
+    e = hipMemcpyAsync(h->rec_cell, rec_cell.data(), cell_bytes, hipMemcpyHostToDevice,
+                       ctx->stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(h->tets, tets.data(), tet_bytes, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // host copies go out of scope
   if (e != hipSuccess) return cleanup(e, "unstructured field upload");
-  h->device_bytes = start_bytes + rec_bytes + tet_bytes;
+  h->device_bytes = start_bytes + rec_bytes + cell_bytes + tet_bytes;
   *out = h;
   return VRT_OK;
 }
@@ -186,6 +199,7 @@ int vrt_unstructured_destroy(vrt_unstructured f) {
   (void)hipSetDevice(f->device);
   (void)hipFree(f->start);
   (void)hipFree(f->recs);
+  (void)hipFree(f->rec_cell);
   (void)hipFree(f->tets);
   delete f;
   return VRT_OK;

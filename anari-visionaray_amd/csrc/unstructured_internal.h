@@ -14,6 +14,7 @@ struct vrt_unstructured_s {
   // {max xyz, max value}) for the majorant grid
   uint32_t *start = nullptr;
   float4 *recs = nullptr;
+  uint32_t *rec_cell = nullptr;  // per list entry: the ANARI cell index (primitive id)
   float4 *tets = nullptr;
   float lo[3] = {0, 0, 0}, inv_cell[3] = {1, 1, 1};
   int dims[3] = {1, 1, 1};

@@ -72,6 +72,8 @@ int volume_common(const vrt_volume &in, int lds_offset, KVolume &v) {
   v.maj_lds_offset = -1;
   for (int i = 0; i < 9; ++i) v.A[i] = in.affine_inv[i];
   v.tinv = mk(in.trans_inv[0], in.trans_inv[1], in.trans_inv[2]);
+  v.obj_id = in.object_id;
+  v.inst_id = in.instance_id;
   v.filter = in.filter;
   return VRT_OK;
 }
@@ -122,6 +124,7 @@ int make_unstructured_volume(const vrt_volume &in, int lds_offset, KVolume &v) {
   v.cell_size = f->cell_size;
   v.ug_start = f->start;
   v.ug_recs = f->recs;
+  v.ug_cell = f->rec_cell;
   v.ug_tets = f->tets;
   v.ug_num_tets = uint32_t(f->num_tets);
   v.ug_lo = mk(f->lo[0], f->lo[1], f->lo[2]);
@@ -233,6 +236,8 @@ int make_scene(const vrt_scene *s, KScene &k) {
     k.surf[i].nodes = S.bvh->nodes;
     k.surf[i].tris = S.bvh->tris;
     k.surf[i].color = mk(S.color[0], S.color[1], S.color[2]);
+    k.surf[i].obj_id = S.object_id;
+    k.surf[i].inst_id = S.instance_id;
   }
   return VRT_OK;
 }
@@ -310,6 +315,14 @@ int make_frame(const vrt_frame *f, KFrame &k) {
   k.depth = f->depth;
   k.accum64 = f->accum64;
   k.counters = f->counters;
+  k.normal = f->normal;
+  k.albedo = f->albedo;
+  k.prim_id = f->primitive_id;
+  k.obj_id = f->object_id;
+  k.inst_id = f->instance_id;
+  if (!f->depth && (f->normal || f->albedo || f->primitive_id || f->object_id || f->instance_id))
+    return fail(VRT_ERROR_INVALID_ARGUMENT,
+                "g-buffer channels need the depth channel (Frame::fillGBuffer)");
   return VRT_OK;
 }
 

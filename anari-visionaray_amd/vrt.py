@@ -46,6 +46,7 @@ class Volume(C.Structure):
         ("value_hi", C.c_float), ("unit_distance", C.c_float), ("majorants", C.c_void_p),
         ("grid_dims", i3), ("majorant_bytes", C.c_void_p), ("majorant_step", C.c_void_p),
         ("affine_inv", C.c_float * 9), ("trans_inv", f3),
+        ("object_id", C.c_uint32), ("instance_id", C.c_uint32),
     ]
 
 
@@ -57,7 +58,11 @@ class Light(C.Structure):
 
 
 class Surface(C.Structure):
-    _fields_ = [("bvh", C.c_void_p), ("color", f3)]
+    _fields_ = [("bvh", C.c_void_p), ("color", f3), ("object_id", C.c_uint32),
+                ("instance_id", C.c_uint32)]
+
+
+NO_ID = 0xFFFFFFFF  # an unset ANARI "id"
 
 
 MAX_SURFACES = 4
@@ -100,6 +105,8 @@ class Frame(C.Structure):
         ("width", C.c_int), ("height", C.c_int), ("color_type", C.c_int),
         ("color", C.c_void_p), ("accum", C.c_void_p), ("depth", C.c_void_p),
         ("accum64", C.c_void_p), ("counters", C.c_void_p),
+        ("normal", C.c_void_p), ("albedo", C.c_void_p), ("primitive_id", C.c_void_p),
+        ("object_id", C.c_void_p), ("instance_id", C.c_void_p),
     ]
 
 
@@ -520,6 +527,8 @@ class DeviceVolume:
         self._finish(ctx, v, tf_rgba, value_range, unit_distance, grid_dims, xfm)
 
     def _finish(self, ctx, v, tf_rgba, value_range, unit_distance, grid_dims, xfm):
+        v.object_id = NO_ID
+        v.instance_id = NO_ID
         tf = np.ascontiguousarray(tf_rgba, np.float32).reshape(-1, 4)
         self.tf = ctx.upload(tf)
         v.tf_rgba = self.tf.ptr
@@ -574,6 +583,8 @@ def make_scene(volumes, lights=(), surfaces=()):
     for i, (mesh, color) in enumerate(surfaces):
         s.surfaces[i].bvh = mesh.handle
         s.surfaces[i].color = f3(*color)
+        s.surfaces[i].object_id = NO_ID
+        s.surfaces[i].instance_id = NO_ID
     return s
 
 
@@ -682,7 +693,9 @@ def params(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.
 
 class DeviceFrame:
     def __init__(self, ctx, width, height, color_type=TYPE_FLOAT32_VEC4, depth=False,
-                 accum64=False, counters=True):
+                 accum64=False, counters=True, gbuffer=False):
+        """gbuffer: also the normal, albedo and primitive / object / instance
+        id channels (they need the depth channel, which it turns on)."""
         self.ctx = ctx
         self.width, self.height = width, height
         n = width * height
@@ -692,6 +705,12 @@ class DeviceFrame:
         self.depth = ctx.buffer(n * 4) if depth else None
         self.accum64 = ctx.buffer(n * 32) if accum64 else None
         self.counters = ctx.buffer(64) if counters else None
+        if gbuffer and not depth:
+            depth = True
+            self.depth = ctx.buffer(n * 4)
+        self.gbuf = {k: ctx.buffer(n * b) for k, b in
+                     (("normal", 12), ("albedo", 12), ("primitive_id", 4), ("object_id", 4),
+                      ("instance_id", 4))} if gbuffer else {}
         f = Frame()
         f.width, f.height, f.color_type = width, height, color_type
         f.color = self.color.ptr
@@ -699,6 +718,8 @@ class DeviceFrame:
         f.depth = self.depth.ptr if depth else None
         f.accum64 = self.accum64.ptr if accum64 else None
         f.counters = self.counters.ptr if counters else None
+        for k, b in self.gbuf.items():
+            setattr(f, k, b.ptr)
         self.desc = f
         self.reset()
 
@@ -720,6 +741,17 @@ class DeviceFrame:
 
     def depth_array(self):
         return self.depth.download(np.float32, (self.height, self.width))
+
+    def gbuffer_arrays(self):
+        """{normal, albedo: (h, w, 3) float32; primitive_id, object_id,
+        instance_id: (h, w) uint32}"""
+        out = {}
+        for k, b in self.gbuf.items():
+            if k in ("normal", "albedo"):
+                out[k] = b.download(np.float32, (self.height, self.width, 3))
+            else:
+                out[k] = b.download(np.uint32, (self.height, self.width))
+        return out
 
     def counter_values(self):
         return self.counters.download(np.uint64, (8,))

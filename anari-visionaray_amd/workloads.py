@@ -253,7 +253,7 @@ class GpuScene:
     """The workload resident on one GPU through the vrt C ABI."""
 
     def __init__(self, ctx, wl: Workload, color_type=None, depth=False, accum64=False,
-                 volume=None, params=None, bvh_builder="sah"):
+                 volume=None, params=None, bvh_builder="sah", gbuffer=False):
         self.wl = wl
         self.ctx = ctx
         if volume is None and wl.nanovdb is not None:
@@ -280,7 +280,8 @@ class GpuScene:
         p.update(params or {})
         self.params = _vrt.params(**p)
         self.frame = _vrt.DeviceFrame(ctx, wl.width, wl.height,
-                                      color_type or _vrt.TYPE_FLOAT32_VEC4, depth, accum64)
+                                      color_type or _vrt.TYPE_FLOAT32_VEC4, depth, accum64,
+                                      gbuffer=gbuffer)
 
     def render(self, accum_id=0):
         if self.wl.renderer == "raycast":

@@ -87,6 +87,11 @@ typedef struct vrt_volume {
   /* instance: object = affine_inv * (world + trans_inv) */
   float affine_inv[9]; /* row-major 3x3 */
   float trans_inv[3];
+  /* ids reported in the frame's objectId / instanceId channels: the
+     volume's and its instance's ANARI "id" (Volume.cpp:25, Instance "id";
+     ~0u when unset) */
+  uint32_t object_id;
+  uint32_t instance_id;
 } vrt_volume;
 
 #define VRT_LIGHT_DIRECTIONAL 0
@@ -116,6 +121,8 @@ typedef struct vrt_bvh_s *vrt_bvh;
 typedef struct vrt_surface {
   vrt_bvh bvh;
   float color[3];
+  uint32_t object_id;    /* Surface "id" (Surface.cpp:21), ~0u when unset */
+  uint32_t instance_id;  /* the instance's "id", ~0u when unset */
 } vrt_surface;
 
 #define VRT_MAX_SURFACES 4
@@ -187,6 +194,18 @@ typedef struct vrt_frame {
                          long-run reference accumulation; may be NULL) */
   unsigned long long *counters; /* optional [0] trilinear lookups,
                                    [1] paths, [2] collisions */
+  /* optional g-buffer channels (Frame::fillGBuffer,
+     DeviceCopyableObjects.h:2745-2774): the first radiance hit of the
+     pixel's closest sample -- geometric normal (float3), albedo (float3),
+     primitive / object / instance id (uint32, ~0u for none).  As in the
+     reference they are written only together with `depth`, and only where
+     the sample's depth is <= the stored depth (which the accumulation reset
+     sets to 1e31). */
+  float *normal;
+  float *albedo;
+  uint32_t *primitive_id;
+  uint32_t *object_id;
+  uint32_t *instance_id;
 } vrt_frame;
 
 /* ---------------------------------------------------------------------- */

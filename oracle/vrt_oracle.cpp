@@ -201,6 +201,7 @@ inline bool intersect_tet(R &value, V3<R> P, V3<R> a, R wa, V3<R> b, R wb, V3<R>
 struct TetField {
   struct Cell {
     uint32_t v[4];
+    uint32_t src;  // index in the ANARI cell arrays: the primitive id
     float cell_value;
     float lo[3], hi[3];
     float vmin, vmax;
@@ -244,6 +245,7 @@ struct TetField {
       }
       if (ic != 4) return false;
       Cell cell;
+      cell.src = uint32_t(c);
       cell.cell_value = v->cell_data ? v->cell_data[c] : 0.f;
       for (int a = 0; a < 3; ++a) {
         cell.lo[a] = FLT_MAX;
@@ -319,14 +321,17 @@ struct TetField {
 
   // sampleField: the first cell (in id order) containing P, found through
   // the grid; false outside every cell
-  template <class R> bool sample(V3<R> P, R &value) const {
+  template <class R> bool sample(V3<R> P, R &value, int64_t *prim = nullptr) const {
     for (int a = 0; a < 3; ++a)
       if (!(P[a] >= R(blo[a]) && P[a] <= R(bhi[a]))) return false;
     const int x = grid_coord(0, double(P.x)), y = grid_coord(1, double(P.y)),
               z = grid_coord(2, double(P.z));
     const size_t g = (size_t(z) * G + y) * G + x;
     for (uint32_t i = gstart[g]; i < gstart[g + 1]; ++i)
-      if (test_cell(gids[i], P, value)) return true;
+      if (test_cell(gids[i], P, value)) {
+        if (prim) *prim = cells[gids[i]].src;
+        return true;
+      }
     return false;
   }
 
@@ -413,6 +418,12 @@ template <class R> struct Volume {
     }
   }
 
+  // the normal transform transpose(affineInv) (Instance.cpp:96-98) applied
+  // as mat3 * vec3 = col0 n.x + col1 n.y + col2 n.z
+  V3<R> mul_normal(V3<R> n) const {
+    return {A[0] * n.x + A[3] * n.y + A[6] * n.z, A[1] * n.x + A[4] * n.y + A[7] * n.z,
+            A[2] * n.x + A[5] * n.y + A[8] * n.z};
+  }
   V3<R> mulA(V3<R> p) const {
     return {A[0] * p.x + A[1] * p.y + A[2] * p.z, A[3] * p.x + A[4] * p.y + A[5] * p.z,
             A[6] * p.x + A[7] * p.y + A[8] * p.z};
@@ -437,8 +448,11 @@ template <class R> struct Volume {
   // the TF at (i+.5)/N under the same convention)
   // sampleField (DeviceCopyableObjects.h:474-570): false where the field has
   // no value (a point outside every cell of an unstructured field)
-  bool sample(V3<R> P, R &value) const {
-    if (v->field_type == 2) return is_tets && tets.sample(P, value);
+  // `prim`: the primitive id of the sample -- the containing cell of an
+  // unstructured field, 0 for the other fields (:478-481, 556-567)
+  bool sample(V3<R> P, R &value, int64_t *prim = nullptr) const {
+    if (prim) *prim = 0;
+    if (v->field_type == 2) return is_tets && tets.sample(P, value, prim);
     value = sample_regular(P);
     return true;
   }
@@ -570,6 +584,7 @@ template <class R> struct VolHit {
   R t = std::numeric_limits<R>::max();
   V3<R> albedo;
   R Tr = R(1);
+  int64_t prim = 0;  // primitive id of the collision's sample
 };
 
 struct Counter { uint64_t lookups = 0; };
@@ -594,6 +609,7 @@ VolHit<R> delta_track(const Volume<R> &vol, Ray<R> ray, Random &rnd, Counter *cn
   bool hit = false;
   R hit_t = ray.tmax;
   V3<R> albedo;
+  int64_t hit_prim = 0;
 
   auto woodcock = [&](int leafID, R t0, R t1) -> bool {
     const R majorant = leafID >= 0 ? vol.majorant(leafID) : R(1);
@@ -604,15 +620,17 @@ VolHit<R> delta_track(const Volume<R> &vol, Ray<R> ray, Random &rnd, Counter *cn
       if (t >= t1) break;
       V3<R> P = ray.ori + ray.dir * t;
       R v;
+      int64_t prim = 0;
       if (cnt) cnt->lookups++;
       // the acceptance draw only where the field has a value (:739-753)
-      if (vol.sample(P, v)) {
+      if (vol.sample(P, v, &prim)) {
         V4<R> s = vol.classify(v);
         R u = R(rnd());
         if (s.w >= u * majorant) {
           albedo = V3<R>(s.x, s.y, s.z);
           hit = true;
           hit_t = t;
+          hit_prim = prim;
           return false;
         }
       }
@@ -634,6 +652,7 @@ VolHit<R> delta_track(const Volume<R> &vol, Ray<R> ray, Random &rnd, Counter *cn
     out.t = hit_t / dt_scale;
     out.albedo = albedo;
     out.Tr = R(0);
+    out.prim = hit_prim;
   }
   return out;
 }
@@ -805,7 +824,8 @@ template <class R> struct Mesh {
   std::vector<Node> nodes;
   std::vector<int> order;
 
-  explicit Mesh(const orc_trimesh *m) {
+  const orc_trimesh *m;
+  explicit Mesh(const orc_trimesh *mesh) : m(mesh) {
     color = from<R>(m->color);
     tris.resize(size_t(m->num_triangles));
     for (int i = 0; i < m->num_triangles; ++i) {
@@ -1079,6 +1099,14 @@ template <class R> void clip_ray(Ray<R> &ray, const orc_params *P) {
 // ---------------------------------------------------------------------------
 enum RayType { Radiance, Shadow, AO, Miss, None };
 
+// the PixelSample fields of the g-buffer (frame/common.h:15-26); normal and
+// albedo, unset there for a sample without a hit, are 0 here
+template <class R> struct GBuf {
+  R depth = R(1e31f);
+  V3<R> Ng = V3<R>(R(0)), albedo = V3<R>(R(0));
+  uint32_t prim = ~0u, obj = ~0u, inst = ~0u;
+};
+
 template <class R> struct ShadeState {
   V3<R> baseColor, shadedColor, hitPosOff, sn;
   int aoSamples = 0;
@@ -1104,7 +1132,7 @@ template <class R> inline R power_heuristic(R f, R g) {
 template <class R>
 void shade(Random &srnd, const Ray<R> &ray, RayType rayType, const Scene<R> &sc,
            const orc_params *P, const HitRec<R> &hitRec, ShadeState<R> &st,
-           unsigned bounceID = 0) {
+           unsigned bounceID = 0, GBuf<R> *gb = nullptr) {
   st.nextType = Miss;
   if (rayType == Radiance) {
     st.baseColor = V3<R>(R(0));
@@ -1151,10 +1179,20 @@ void shade(Random &srnd, const Ray<R> &ray, RayType rayType, const Scene<R> &sc,
         gn = uniform_sample_sphere(u1, u2);
       }
       sn = gn;
-      // identity normal transform for unscaled instances (getNormalTransform)
+      // getNormalTransform (renderer/common.h:766-770): transpose(affineInv),
+      // no renormalisation for volumes (:217-219)
+      const Volume<R> &V = sc.vols[hitRec.vol_inst];
+      gn = V.mul_normal(gn);
+      sn = V.mul_normal(sn);
       sn = faceforward(sn, viewDir, gn);
       gn = faceforward(gn, viewDir, gn);
       color = hitRec.vol.albedo;
+      if (gb && bounceID == 0) {  // :228-233
+        gb->depth = hitRec.vol.t;
+        gb->prim = uint32_t(hitRec.vol.prim);
+        gb->obj = V.v->object_id;
+        gb->inst = V.v->instance_id;
+      }
     } else {
       // triangle surface (Pathtrace_impl.cpp:234-274): Ng = normalize(e1 x
       // e2), no vertex normals (getNormals, common.h:233-264), matte color
@@ -1164,9 +1202,23 @@ void shade(Random &srnd, const Ray<R> &ray, RayType rayType, const Scene<R> &sc,
       eps = epsilon_from(hitPos, ray.dir, hitRec.surf_t);
       gn = normalize(cross(T.e1, T.e2));
       sn = gn;
+      // the normal transform of the (identity) instance, then normalised
+      // again (:243-247)
+      gn = normalize(gn);
+      sn = normalize(sn);
       sn = faceforward(sn, viewDir, gn);
       gn = faceforward(gn, viewDir, gn);
       color = M.color;
+      if (gb && bounceID == 0) {  // :273-278
+        gb->depth = hitRec.surf_t;
+        gb->prim = uint32_t(hitRec.surf_prim);
+        gb->obj = M.m->object_id;
+        gb->inst = M.m->instance_id;
+      }
+    }
+    if (gb && bounceID == 0) {  // :281-285
+      gb->Ng = gn;
+      gb->albedo = color;
     }
     st.sn = sn;
     st.hitPosOff = hitPos + sn * eps;
@@ -1288,7 +1340,8 @@ void shade(Random &srnd, const Ray<R> &ray, RayType rayType, const Scene<R> &sc,
 
 template <class R>
 V4<R> pathtrace_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params *P,
-                      int x, int y, int W, int H, int accumID, Counter *cnt) {
+                      int x, int y, int W, int H, int accumID, Counter *cnt,
+                      GBuf<R> *closest = nullptr) {
   // Pathtrace_impl.cpp:510-513: rng seeded by (pixelID, accumID); the
   // ScreenSample holds a copy, RR draws from the un-advanced original
   uint32_t pixelID = uint32_t(x + W * y);
@@ -1296,7 +1349,9 @@ V4<R> pathtrace_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params
   Random srnd = rng;
   V4<R> accum{0, 0, 0, 0};
   const int spp = P->pixel_samples;
+  if (closest) *closest = GBuf<R>();
   for (int sampleID = 0; sampleID < spp; ++sampleID) {
+    GBuf<R> ps;
     R xf = R(x), yf = R(y);
     R jx = R(srnd()) - R(0.5);
     R jy = R(srnd()) - R(0.5);
@@ -1314,7 +1369,7 @@ V4<R> pathtrace_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params
         clip_ray(ray, P);  // Pathtrace_impl.cpp:559
         const bool shadow = rayType == Shadow || rayType == AO;
         HitRec<R> hitRec = intersect_all(sc, ray, srnd, cnt, bounceID, shadow);
-        shade(srnd, ray, rayType, sc, P, hitRec, st, bounceID);
+        shade(srnd, ray, rayType, sc, P, hitRec, st, bounceID, closest ? &ps : nullptr);
         if (passID == 0 && bounceID == 0) {
           firstHit = hitRec.hit;
           firstLightInvisible = hitRec.lightHit && !hitRec.light_visible;
@@ -1343,6 +1398,8 @@ V4<R> pathtrace_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params
     accum.y += color.y;
     accum.z += color.z;
     accum.w += color.w;
+    // the closest sample's g-buffer, the first of equal depths (:616-618)
+    if (closest && ps.depth < closest->depth) *closest = ps;
   }
   R inv = R(1) / R(spp);
   return V4<R>{accum.x * inv, accum.y * inv, accum.z * inv, accum.w * inv};
@@ -1793,6 +1850,72 @@ void orc_camera_ray(const orc_camera *c, float x, float y, int W, int H, float *
     ori[i] = r.ori[i];
     dir[i] = r.dir[i];
   }
+}
+
+// Frame::fillGBuffer (DeviceCopyableObjects.h:2745-2774) of one pixel
+static void fill_gbuffer(const orc::GBuf<float> &g, size_t i, float *depth, float *normal,
+                         float *albedo, uint32_t *prim, uint32_t *obj, uint32_t *inst) {
+  if (g.depth > depth[i]) return;
+  depth[i] = g.depth;
+  if (normal) {
+    normal[3 * i] = g.Ng.x;
+    normal[3 * i + 1] = g.Ng.y;
+    normal[3 * i + 2] = g.Ng.z;
+  }
+  if (albedo) {
+    albedo[3 * i] = g.albedo.x;
+    albedo[3 * i + 1] = g.albedo.y;
+    albedo[3 * i + 2] = g.albedo.z;
+  }
+  if (prim) prim[i] = g.prim;
+  if (obj) obj[i] = g.obj;
+  if (inst) inst[i] = g.inst;
+}
+
+void orc_pathtrace_gbuffer(const orc_scene *scene, const orc_camera *c, const orc_params *P,
+                           int W, int H, int accumID, float *depth, float *normal, float *albedo,
+                           uint32_t *prim, uint32_t *obj, uint32_t *inst, int threads) {
+  Scene<float> sc(scene);
+  Camera<float> cam(c);
+  parallel_tiles(W, H, threads, [&](int x, int y) {
+    GBuf<float> g;
+    (void)pathtrace_pixel(sc, cam, P, x, y, W, H, accumID, nullptr, &g);
+    fill_gbuffer(g, size_t(y) * W + x, depth, normal, albedo, prim, obj, inst);
+  });
+}
+
+// Raycast_impl.cpp:18-20, 230-237: depth stays 1e31 and the primitive id
+// unset; a hit of the closest volume box sets the object and instance ids
+// (normal and albedo 0)
+void orc_raycast_gbuffer(const orc_scene *scene, const orc_camera *c, const orc_params *P,
+                         int W, int H, float *depth, float *normal, float *albedo,
+                         uint32_t *prim, uint32_t *obj, uint32_t *inst, int threads) {
+  Scene<float> sc(scene);
+  Camera<float> cam(c);
+  parallel_tiles(W, H, threads, [&](int x, int y) {
+    GBuf<float> g;
+    if (!sc.empty()) {
+      Ray<float> ray = cam.primary(float(x), float(y), float(W), float(H));
+      clip_ray(ray, P);
+      int best = -1;
+      float bestT = std::numeric_limits<float>::max();
+      for (size_t i = 0; i < sc.vols.size(); ++i) {
+        Ray<float> lr = sc.vols[i].to_local(ray);
+        BoxHit<float> bh = intersect_box(lr, sc.vols[i].blo, sc.vols[i].bhi);
+        if (!bh.hit || !(bh.tfar >= lr.tmin)) continue;
+        const float t = std::max(lr.tmin, bh.tnear);
+        if (t < bestT) {
+          bestT = t;
+          best = int(i);
+        }
+      }
+      if (best >= 0) {
+        g.obj = sc.vols[best].v->object_id;
+        g.inst = sc.vols[best].v->instance_id;
+      }
+    }
+    fill_gbuffer(g, size_t(y) * W + x, depth, normal, albedo, prim, obj, inst);
+  });
 }
 
 void orc_raycast(const orc_scene *scene, const orc_camera *c, const orc_params *P, int W,

@@ -72,6 +72,10 @@ typedef struct orc_volume {
   const uint8_t *cell_type;
   const float *cell_data;
   int64_t num_cells;
+  /* ANARI "id" of the volume and of its instance (Volume.cpp:25, the
+     instance's userID), reported in the g-buffer; ~0u when unset */
+  uint32_t object_id;
+  uint32_t instance_id;
 } orc_volume;
 
 typedef struct orc_trimesh {
@@ -80,6 +84,8 @@ typedef struct orc_trimesh {
   int num_triangles;
   float color[3]; /* matte color */
   float opacity;
+  uint32_t object_id;   /* Surface "id" (Surface.cpp:21), ~0u when unset */
+  uint32_t instance_id; /* the instance's "id" */
 } orc_trimesh;
 
 typedef struct orc_scene {
@@ -205,6 +211,23 @@ void orc_hg_sample(float g, const float d[3], int n, uint32_t seed, float *out);
 double orc_hg_4pi(double g, double cos_theta);
 void orc_camera_ray(const orc_camera *cam, float x, float y, int width,
                     int height, float *ori, float *dir);
+
+/* --- g-buffer of one frame (Frame::fillGBuffer, DeviceCopyableObjects.h:
+   2745-2774), FP32.  Path tracer: per pixel the first radiance hit of the
+   closest sample (Pathtrace_impl.cpp:226-285, 523-617); ray caster: the
+   closest volume box's ids (Raycast_impl.cpp:18-20, 230-237).  `depth` is
+   the caller's depth buffer (1e31 after an accumulation reset), read and
+   updated: a pixel's channels are written where its depth <= the stored
+   one.  normal / albedo float3, ids uint32 (~0u: none); any output but
+   depth may be NULL. --- */
+void orc_pathtrace_gbuffer(const orc_scene *scene, const orc_camera *cam,
+                           const orc_params *params, int width, int height, int accum_id,
+                           float *depth, float *normal, float *albedo, uint32_t *prim,
+                           uint32_t *obj, uint32_t *inst, int threads);
+void orc_raycast_gbuffer(const orc_scene *scene, const orc_camera *cam,
+                         const orc_params *params, int width, int height, float *depth,
+                         float *normal, float *albedo, uint32_t *prim, uint32_t *obj,
+                         uint32_t *inst, int threads);
 
 #ifdef __cplusplus
 }

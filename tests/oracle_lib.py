@@ -30,12 +30,14 @@ class OrcVolume(C.Structure):
                 ("vertex_data", C.c_void_p), ("num_vertices", C.c_int64),
                 ("index", C.c_void_p), ("num_indices", C.c_int64),
                 ("cell_index", C.c_void_p), ("cell_type", C.c_void_p),
-                ("cell_data", C.c_void_p), ("num_cells", C.c_int64)]
+                ("cell_data", C.c_void_p), ("num_cells", C.c_int64),
+                ("object_id", C.c_uint32), ("instance_id", C.c_uint32)]
 
 
 class OrcTrimesh(C.Structure):
     _fields_ = [("vertices", C.c_void_p), ("indices", C.c_void_p), ("num_triangles", C.c_int),
-                ("color", f3), ("opacity", C.c_float)]
+                ("color", f3), ("opacity", C.c_float), ("object_id", C.c_uint32),
+                ("instance_id", C.c_uint32)]
 
 
 class OrcScene(C.Structure):
@@ -84,6 +86,12 @@ def lib():
         L.orc_pathtrace_frame.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcCamera),
                                           C.POINTER(OrcParams), ip, ip, ip, ip, ip, ip, ip, ip,
                                           vp, ip, C.POINTER(C.c_uint64)]
+        L.orc_pathtrace_gbuffer.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcCamera),
+                                            C.POINTER(OrcParams), ip, ip, ip, vp, vp, vp, vp,
+                                            vp, vp, ip]
+        L.orc_raycast_gbuffer.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcCamera),
+                                          C.POINTER(OrcParams), ip, ip, vp, vp, vp, vp, vp,
+                                          vp, ip]
         L.orc_pathtrace_accumulate.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcCamera),
                                                C.POINTER(OrcParams), ip, ip, ip, ip, ip, ip,
                                                vp, ip]
@@ -180,6 +188,7 @@ class OracleVolume:
         self.tf = np.ascontiguousarray(tf, np.float32).reshape(-1, 4)
         nz, ny, nx = self.voxels.shape
         v = OrcVolume()
+        v.object_id = v.instance_id = 0xFFFFFFFF
         v.voxels = self.voxels.ctypes.data
         v.dims = (C.c_int * 3)(nx, ny, nz)
         v.origin = f3(*origin)
@@ -204,6 +213,7 @@ class OracleVolume:
         self.buf = buf
         self.tf = np.ascontiguousarray(tf, np.float32).reshape(-1, 4)
         v = OrcVolume()
+        v.object_id = v.instance_id = 0xFFFFFFFF
         v.field_type = 1
         v.nanovdb = buf.ctypes.data
         v.filter = {"linear": 0, "nearest": 1}[filter]
@@ -236,6 +246,7 @@ class OracleVolume:
         self.cd = None if cell_data is None else np.ascontiguousarray(cell_data, np.float32)
         self.tf = np.ascontiguousarray(tf, np.float32).reshape(-1, 4)
         v = OrcVolume()
+        v.object_id = v.instance_id = 0xFFFFFFFF
         v.field_type = 2
         v.vertex_position = self.pos.ctypes.data
         v.vertex_data = None if self.vd is None else self.vd.ctypes.data
@@ -307,6 +318,7 @@ class OracleMesh:
         self.v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
         self.i = None if indices is None else np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
         m = OrcTrimesh()
+        m.object_id = m.instance_id = 0xFFFFFFFF
         m.vertices = self.v.ctypes.data
         m.indices = None if self.i is None else self.i.ctypes.data
         m.num_triangles = len(self.v) // 3 if self.i is None else len(self.i)
@@ -439,6 +451,27 @@ class OracleWorkload:
                                   W, H, accum_id, 0, int(fp64), stride, y0, y1, out.ctypes.data,
                                   threads, C.byref(lookups))
         return out, lookups.value
+
+    def gbuffer(self, accum_ids=(0,), threads=0):
+        """The frame's g-buffer after the frames `accum_ids` (from an
+        accumulation reset: depth 1e31): dict of depth (h, w), normal and
+        albedo (h, w, 3), primitive_id / object_id / instance_id (h, w)
+        uint32."""
+        W, H = self.wl.width, self.wl.height
+        g = {"depth": np.full((H, W), 1e31, np.float32),
+             "normal": np.zeros((H, W, 3), np.float32), "albedo": np.zeros((H, W, 3), np.float32),
+             "primitive_id": np.zeros((H, W), np.uint32), "object_id": np.zeros((H, W), np.uint32),
+             "instance_id": np.zeros((H, W), np.uint32)}
+        ptrs = [g[k].ctypes.data for k in ("depth", "normal", "albedo", "primitive_id",
+                                           "object_id", "instance_id")]
+        for a in accum_ids:
+            if self.wl.renderer == "raycast":
+                lib().orc_raycast_gbuffer(C.byref(self.scene.s), C.byref(self.cam),
+                                          C.byref(self.params), W, H, *ptrs, threads)
+            else:
+                lib().orc_pathtrace_gbuffer(C.byref(self.scene.s), C.byref(self.cam),
+                                            C.byref(self.params), W, H, a, *ptrs, threads)
+        return g
 
     def pathtrace_accumulate(self, frames, first_accum_id=0, fp64=False, stride=1, threads=0):
         W, H = self.wl.width, self.wl.height
