@@ -13,7 +13,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -50,7 +55,8 @@ size_t type_size(ANARIDataType t) {
     case ANARI_FLOAT32_VEC4:
     case ANARI_FLOAT32_BOX2: return 16;
     case ANARI_FLOAT32_MAT4: return 64;
-    case ANARI_VOID_POINTER: return sizeof(void *);
+    case ANARI_VOID_POINTER:
+    case ANARI_FRAME_COMPLETION_CALLBACK: return sizeof(void *);
     default: return is_object_type(t) ? sizeof(void *) : 0;
   }
 }
@@ -96,6 +102,10 @@ struct Object {
   float get_f(const char *name, float def) const {
     float v;
     return get(name, ANARI_FLOAT32, &v) ? v : def;
+  }
+  uint32_t get_u(const char *name, uint32_t def) const {
+    uint32_t v;
+    return get(name, ANARI_UINT32, &v) ? v : def;
   }
   int get_i(const char *name, int def) const {
     int32_t v;
@@ -218,6 +228,11 @@ struct Frame : Object {
   uint32_t w = 0, h = 0;
   ANARIDataType color_type = ANARI_UNKNOWN, depth_type = ANARI_UNKNOWN;
   DevMem color, accum, depth;
+  // g-buffer channels (Frame.cpp:84-89, 139-162): normal, albedo (float3),
+  // primitive / object / instance id (uint32); null when not requested
+  DevMem gbuf[5];
+  std::vector<uint8_t> host_gbuf[5];
+  std::atomic<int> callbacks_pending{0};
   uint64_t built = ~0ull;
   uint64_t scene_version = ~0ull;
   int accum_id = 0, frame_id = 0;
@@ -231,6 +246,95 @@ struct Frame : Object {
   }
 };
 
+// channel names, ANARI element types and bytes per pixel of the g-buffer
+struct GbufChannel {
+  const char *name;
+  ANARIDataType type;
+  size_t bytes;
+};
+const GbufChannel kGbuf[5] = {{"channel.normal", ANARI_FLOAT32_VEC3, 12},
+                              {"channel.albedo", ANARI_FLOAT32_VEC3, 12},
+                              {"channel.primitiveId", ANARI_UINT32, 4},
+                              {"channel.objectId", ANARI_UINT32, 4},
+                              {"channel.instanceId", ANARI_UINT32, 4}};
+
+// Frame completion callbacks run on a device-owned thread once the event
+// recorded after the frame's kernels has completed.
+struct Completion {
+  hipEvent_t event;
+  ANARIFrameCompletionCallback fn;
+  const void *user;
+  Frame *frame;
+};
+
+struct CompletionThread {
+  std::mutex m;
+  std::condition_variable cv;
+  std::deque<Completion> q;
+  bool stop = false;
+  std::thread t;
+  // the frame whose callback runs now, and whether that callback released
+  // it (both only touched by the thread itself)
+  Frame *current = nullptr;
+  bool current_destroyed = false;
+  void start(ANARIDevice dev) {
+    t = std::thread([this, dev] {
+      for (;;) {
+        Completion c;
+        {
+          std::unique_lock<std::mutex> lk(m);
+          cv.wait(lk, [this] { return stop || !q.empty(); });
+          if (q.empty()) return;  // stop requested and drained
+          c = q.front();
+          q.pop_front();
+        }
+        (void)hipEventSynchronize(c.event);
+        (void)hipEventDestroy(c.event);
+        current = c.frame;
+        current_destroyed = false;
+        c.fn(c.user, dev, reinterpret_cast<ANARIFrame>(c.frame));
+        if (!current_destroyed) c.frame->callbacks_pending.fetch_sub(1);
+        current = nullptr;
+      }
+    });
+  }
+  bool on_thread() const { return t.joinable() && std::this_thread::get_id() == t.get_id(); }
+  // a frame about to be destroyed by its own callback: its other queued
+  // callbacks are dropped
+  void forget(Frame *f) {
+    if (current == f) current_destroyed = true;
+    std::lock_guard<std::mutex> lk(m);
+    for (auto it = q.begin(); it != q.end();) {
+      if (it->frame == f) {
+        (void)hipEventDestroy(it->event);
+        it = q.erase(it);
+      } else {
+        ++it;
+      }
+    }
+  }
+  void push(const Completion &c) {
+    {
+      std::lock_guard<std::mutex> lk(m);
+      q.push_back(c);
+    }
+    cv.notify_one();
+  }
+  void join() {
+    if (!t.joinable()) return;
+    if (on_thread()) {  // the device released from inside a callback
+      t.detach();
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> lk(m);
+      stop = true;
+    }
+    cv.notify_one();
+    t.join();
+  }
+};
+
 struct Device : Object {
   vrt_context ctx = nullptr;
   hipStream_t stream = nullptr;
@@ -238,6 +342,7 @@ struct Device : Object {
   const void *cb_user = nullptr;
   std::vector<Object *> commit_buffer;
   uint64_t serial = 0;
+  CompletionThread completions;
 
   void report(Object *src, ANARIStatusSeverity sev, ANARIStatusCode code, const char *fmt, ...)
       __attribute__((format(printf, 5, 6))) {
@@ -271,6 +376,7 @@ struct Device : Object {
     }
   }
   ~Device() override {
+    completions.join();
     if (ctx) vrt_context_destroy(ctx);
   }
 };
@@ -280,6 +386,15 @@ void retain_internal(Object *o) {
 }
 void destroy(Object *o) {
   Device *d = o->kind == ANARI_DEVICE ? nullptr : o->dev;
+  if (o->kind == ANARI_FRAME && d) {
+    // frame completion callbacks name the frame: released from inside one,
+    // the rest are dropped; from elsewhere, the queued ones run first
+    Frame *f = static_cast<Frame *>(o);
+    if (d->completions.on_thread())
+      d->completions.forget(f);
+    else
+      while (f->callbacks_pending.load() > 0) std::this_thread::yield();
+  }
   delete o;
   if (d) release_internal(d);  // every object holds its device
 }
@@ -622,7 +737,8 @@ struct SceneBuild {
   uint64_t version = 0;  // newest commit among the objects used
 };
 
-void add_group(Device *d, const Object *grp, const Affine &x, SceneBuild &sb) {
+// inst_id: the instance's "id" (the world's for its own objects, World.cpp:96)
+void add_group(Device *d, const Object *grp, const Affine &x, uint32_t inst_id, SceneBuild &sb) {
   sb.version = std::max(sb.version, grp->version);
   for (Object *o : objects_of(grp, "volume", ANARI_VOLUME)) {
     Volume *v = static_cast<Volume *>(o);
@@ -653,6 +769,8 @@ void add_group(Device *d, const Object *grp, const Affine &x, SceneBuild &sb) {
                            (L[0] * L[4] - L[1] * L[3]) / det};
     for (int i = 0; i < 9; ++i) vd.affine_inv[i] = float(inv[i]);
     for (int k = 0; k < 3; ++k) vd.trans_inv[k] = -x.m[4 * k + 3];
+    vd.object_id = v->get_u("id", ~0u);  // Volume.cpp:25
+    vd.instance_id = inst_id;
     sb.volumes.push_back(vd);
   }
   for (Object *o : objects_of(grp, "surface", ANARI_SURFACE)) {
@@ -684,6 +802,8 @@ void add_group(Device *d, const Object *grp, const Affine &x, SceneBuild &sb) {
       if (!m->get("color", ANARI_FLOAT32_VEC4, &c)) m->get_v3("color", c);
     }
     for (int k = 0; k < 3; ++k) s.color[k] = c[k];
+    s.object_id = o->get_u("id", ~0u);  // Surface.cpp:21
+    s.instance_id = inst_id;
     sb.surfaces.push_back(s);
   }
   for (Object *o : objects_of(grp, "light", ANARI_LIGHT)) {
@@ -729,7 +849,7 @@ bool assemble(Device *d, const Object *world, SceneBuild &sb) {
   sb.version = world->version;
   // the world's own surfaces/volumes/lights form a group under identity
   // (World.cpp:56-97)
-  add_group(d, world, Affine{}, sb);
+  add_group(d, world, Affine{}, world->get_u("id", ~0u), sb);
   for (Object *inst : objects_of(world, "instance", ANARI_INSTANCE)) {
     sb.version = std::max(sb.version, inst->version);
     const Object *g = inst->get_o("group", ANARI_GROUP);
@@ -739,7 +859,7 @@ bool assemble(Device *d, const Object *world, SceneBuild &sb) {
     if (inst->get("transform", ANARI_FLOAT32_MAT4, &m4))
       for (int r = 0; r < 3; ++r)
         for (int c = 0; c < 4; ++c) x.m[4 * r + c] = m4[4 * c + r];  // column-major
-    add_group(d, g, x, sb);
+    add_group(d, g, x, inst->get_u("id", ~0u), sb);
   }
   sb.scene.num_volumes = int(sb.volumes.size());
   std::copy(sb.volumes.begin(), sb.volumes.end(), sb.scene.volumes);
@@ -784,6 +904,25 @@ bool build_frame(Device *d, Frame *fr) {
     return false;
   }
   if (fr->depth_type != ANARI_FLOAT32) fr->depth.reset();
+  for (int k = 0; k < 5; ++k) {
+    int32_t t = ANARI_UNKNOWN;
+    fr->get(kGbuf[k].name, ANARI_DATA_TYPE, &t);
+    if (t == ANARI_UNKNOWN) {
+      fr->gbuf[k].reset();
+      continue;
+    }
+    if (t != kGbuf[k].type) {
+      d->report(fr, ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
+                "%s must be of type %d: channel ignored", kGbuf[k].name, kGbuf[k].type);
+      fr->gbuf[k].reset();
+      continue;
+    }
+    if (!fr->gbuf[k].alloc(d->ctx, np * kGbuf[k].bytes) ||
+        vrt_memset(d->ctx, fr->gbuf[k].p, 0, np * kGbuf[k].bytes) != VRT_OK) {
+      d->vrt_error(fr, "frame allocation");
+      return false;
+    }
+  }
   fr->scene_version = ~0ull;  // a new size restarts accumulation
   return true;
 }
@@ -856,6 +995,13 @@ void render(Device *d, Frame *fr) {
   f.color = fr->color.p;
   f.accum = static_cast<float *>(fr->accum.p);
   f.depth = fr->depth_type == ANARI_FLOAT32 ? static_cast<float *>(fr->depth.p) : nullptr;
+  if (f.depth) {  // written only together with depth (Frame::fillGBuffer)
+    f.normal = static_cast<float *>(fr->gbuf[0].p);
+    f.albedo = static_cast<float *>(fr->gbuf[1].p);
+    f.primitive_id = static_cast<uint32_t *>(fr->gbuf[2].p);
+    f.object_id = static_cast<uint32_t *>(fr->gbuf[3].p);
+    f.instance_id = static_cast<uint32_t *>(fr->gbuf[4].p);
+  }
   if (ver != fr->scene_version) {  // Frame.cpp:499-513
     fr->scene_version = ver;
     fr->accum_id = 0;
@@ -881,6 +1027,24 @@ void render(Device *d, Frame *fr) {
   }
   ++fr->accum_id;
   ++fr->frame_id;
+  // frameCompletionCallback (Frame.cpp:90-93, 383-384)
+  ANARIFrameCompletionCallback fn = nullptr;
+  const void *user = nullptr;
+  fr->get("frameCompletionCallback", ANARI_FRAME_COMPLETION_CALLBACK, &fn);
+  fr->get("frameCompletionCallbackUserData", ANARI_VOID_POINTER, &user);
+  if (fn) {
+    hipEvent_t ev = nullptr;
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventRecord(ev, d->stream) != hipSuccess) {
+      d->report(fr, ANARI_SEVERITY_ERROR, ANARI_STATUS_UNKNOWN_ERROR,
+                "frame completion event failed");
+      if (ev) (void)hipEventDestroy(ev);
+      return;
+    }
+    if (!d->completions.t.joinable()) d->completions.start(reinterpret_cast<ANARIDevice>(d));
+    fr->callbacks_pending.fetch_add(1);
+    d->completions.push({ev, fn, user, fr});
+  }
 }
 
 }  // namespace
@@ -1149,6 +1313,19 @@ const void *anariMapFrame(ANARIDevice dh, ANARIFrame fh, const char *channel, ui
     if (h) *h = fr->h;
     if (type) *type = ANARI_FLOAT32;
     return fr->host_depth.data();
+  }
+  for (int k = 0; k < 5; ++k) {
+    if (std::strcmp(channel, kGbuf[k].name) || !fr->gbuf[k].p) continue;
+    fr->host_gbuf[k].resize(np * kGbuf[k].bytes);
+    if (vrt_download(d->ctx, fr->host_gbuf[k].data(), fr->gbuf[k].p, fr->host_gbuf[k].size()) !=
+        VRT_OK) {
+      d->vrt_error(fr, "map g-buffer channel");
+      return nullptr;
+    }
+    if (w) *w = fr->w;
+    if (h) *h = fr->h;
+    if (type) *type = kGbuf[k].type;
+    return fr->host_gbuf[k].data();
   }
   d->report(fr, ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
             "frame has no channel '%s'", channel);

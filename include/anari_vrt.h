@@ -17,7 +17,11 @@
  *  - errors are reported through the device's status callback, never by
  *    return codes (functions that create objects return NULL on failure);
  *  - mapped frame channels are host copies (the reference maps device
- *    memory on HIP, Frame.h:83-95).
+ *    memory on HIP, Frame.h:83-95);
+ *  - a frame's "frameCompletionCallback" (with
+ *    "frameCompletionCallbackUserData") is called once the frame has
+ *    finished on the GPU, from a device-owned thread (the reference calls it
+ *    right after enqueueing the kernel on HIP, Frame.cpp:383-384).
  *
  * Supported subtypes: spatial fields "structuredRegular" (float32 data),
  * "nanovdb" (a float grid buffer), "unstructured" (tetrahedra); volume
@@ -66,6 +70,7 @@ typedef int ANARIDataType;
 #define ANARI_STRING 101
 #define ANARI_VOID_POINTER 102
 #define ANARI_BOOL 103
+#define ANARI_FRAME_COMPLETION_CALLBACK 203
 #define ANARI_DEVICE 501
 #define ANARI_OBJECT 502
 #define ANARI_ARRAY 503
@@ -122,6 +127,8 @@ typedef int ANARIWaitMask;
 #define ANARI_WAIT 1
 
 typedef void (*ANARIMemoryDeleter)(const void *userData, const void *appMemory);
+typedef void (*ANARIFrameCompletionCallback)(const void *userData, ANARIDevice device,
+                                             ANARIFrame frame);
 typedef void (*ANARIStatusCallback)(const void *userData, ANARIDevice device,
                                     ANARIObject source, ANARIDataType sourceType,
                                     ANARIStatusSeverity severity, ANARIStatusCode code,
@@ -176,7 +183,11 @@ ANARI_VRT_API int anariGetProperty(ANARIDevice, ANARIObject, const char *name,
 ANARI_VRT_API void anariRenderFrame(ANARIDevice, ANARIFrame);
 ANARI_VRT_API int anariFrameReady(ANARIDevice, ANARIFrame, ANARIWaitMask);
 ANARI_VRT_API void anariDiscardFrame(ANARIDevice, ANARIFrame);
-/* "channel.color" / "channel.depth": a host copy valid until unmapped */
+/* "channel.color", "channel.depth", "channel.normal" / "channel.albedo"
+   (FLOAT32_VEC3), "channel.primitiveId" / "channel.objectId" /
+   "channel.instanceId" (UINT32): a host copy valid until unmapped.  As in the
+   reference (Frame::fillGBuffer) the normal, albedo and id channels are
+   written only when the frame also has a depth channel. */
 ANARI_VRT_API const void *anariMapFrame(ANARIDevice, ANARIFrame, const char *channel,
                                         uint32_t *width, uint32_t *height,
                                         ANARIDataType *pixelType);

@@ -15,7 +15,9 @@ MATERIAL, RENDERER, SURFACE, SPATIAL_FIELD, VOLUME, WORLD = 513, 514, 515, 517, 
 UINT8, INT32, UINT32, UINT32_VEC2, UINT32_VEC3 = 1004, 1016, 1020, 1021, 1022
 UFIXED8_VEC4, FLOAT32, FLOAT32_VEC2, FLOAT32_VEC3, FLOAT32_VEC4 = 1039, 1068, 1069, 1070, 1071
 FLOAT32_BOX1, FLOAT32_BOX2, FLOAT32_MAT4 = 2008, 2009, 2015
+VOID_POINTER, FRAME_COMPLETION_CALLBACK = 102, 203
 NO_WAIT, WAIT = 0, 1
+FRAME_CALLBACK = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_void_p)
 
 _CB = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
                   C.c_char_p)
@@ -128,8 +130,12 @@ class Anari:
             return None, t.value
         if t.value == FLOAT32_VEC4:
             n, dt, comp = w.value * h.value * 4, np.float32, 4
+        elif t.value == FLOAT32_VEC3:
+            n, dt, comp = w.value * h.value * 3, np.float32, 3
         elif t.value == FLOAT32:
             n, dt, comp = w.value * h.value, np.float32, 1
+        elif t.value == UINT32:
+            n, dt, comp = w.value * h.value, np.uint32, 1
         else:
             n, dt, comp = w.value * h.value * 4, np.uint8, 4
         buf = (C.c_uint8 * (n * np.dtype(dt).itemsize)).from_address(p)

@@ -100,6 +100,7 @@ def _anari_scene(a, wl, xfm=None, bvh_builder=None):
         a.set(s, "material", A.MATERIAL, mat)
         a.commit(g, mat, s)
         surfs.append(s)
+        objs.setdefault("surfaces", []).append(s)
     world = a.new(A.WORLD)
     if xfm is None:
         if vols:
@@ -305,6 +306,131 @@ def test_geometry_gpu_bvh_builder(pkg, gpu_ctx):
         assert a.prop_i(g, "bvhNodes") > 0
         print("lbvh", t_gpu * 1e3, "ms", n_gpu, "nodes; sah", a.prop_f(g, "bvhBuildTime") * 1e3,
               "ms", a.prop_i(g, "bvhNodes"), "nodes")
+        errors = [m for m in a.messages if m[0] <= 1]
+        assert not errors, errors
+    finally:
+        a.close()
+
+
+GBUF = (("channel.normal", A.FLOAT32_VEC3, "normal"), ("channel.albedo", A.FLOAT32_VEC3, "albedo"),
+        ("channel.primitiveId", A.UINT32, "primitive_id"),
+        ("channel.objectId", A.UINT32, "object_id"),
+        ("channel.instanceId", A.UINT32, "instance_id"))
+
+
+def test_gbuffer_channels_match_abi(pkg, gpu_ctx):
+    """channel.depth/normal/albedo/primitiveId/objectId/instanceId with the
+    "id" parameters of surface, volume and world (the world's own objects sit
+    under an instance with the world's id, World.cpp:96): bit-identical to
+    the C ABI frame's g-buffer."""
+    wl = _same_tf(pkg, CASES["mixed surfaces"](pkg.workloads))
+    gs = pkg.workloads.GpuScene(gpu_ctx, wl, gbuffer=True)
+    gs.scene.volumes[0].object_id, gs.scene.volumes[0].instance_id = 21, 99
+    gs.scene.surfaces[0].object_id, gs.scene.surfaces[0].instance_id = 31, 99
+    gs.frame.reset()
+    for k in range(2):
+        gs.render(k)
+    gpu_ctx.sync()
+    ref = gs.frame.gbuffer_arrays()
+    ref["depth"] = gs.frame.depth_array()
+    a = A.Anari()
+    try:
+        fr, objs = _anari_scene(a, wl)
+        a.set(objs["volume"], "id", A.UINT32, 21)
+        a.set(objs["surfaces"][0], "id", A.UINT32, 31)
+        a.set(objs["world"], "id", A.UINT32, 99)
+        a.set(fr, "channel.depth", A.DATA_TYPE, A.FLOAT32)
+        for name, t, _ in GBUF:
+            a.set(fr, name, A.DATA_TYPE, t)
+        a.commit(objs["volume"], objs["surfaces"][0], objs["world"], fr)
+        for k in range(2):
+            a.render(fr)
+        depth, t = a.map(fr, "channel.depth")
+        assert t == A.FLOAT32
+        np.testing.assert_array_equal(depth, ref["depth"])
+        for name, t, key in GBUF:
+            img, tt = a.map(fr, name)
+            assert tt == t
+            np.testing.assert_array_equal(img, ref[key])
+        assert {21, 31} <= set(np.unique(ref["object_id"]).tolist())
+        assert set(np.unique(ref["instance_id"]).tolist()) <= {99, 0xFFFFFFFF}
+        errors = [m for m in a.messages if m[0] <= 1]
+        assert not errors, errors
+    finally:
+        a.close()
+
+
+def test_gbuffer_channels_need_depth(pkg, gpu_ctx):
+    """Without channel.depth the reference never fills the other channels
+    (Frame::fillGBuffer returns early): they map as zeros."""
+    wl = _same_tf(pkg, CASES["mixed surfaces"](pkg.workloads))
+    a = A.Anari()
+    try:
+        fr, _ = _anari_scene(a, wl)
+        for name, t, _ in GBUF:
+            a.set(fr, name, A.DATA_TYPE, t)
+        a.commit(fr)
+        a.render(fr)
+        for name, t, _ in GBUF:
+            img, tt = a.map(fr, name)
+            assert tt == t and not img.any()
+        assert a.map(fr, "channel.depth")[0] is None
+    finally:
+        a.close()
+
+
+def test_frame_completion_callback(pkg, gpu_ctx):
+    """frameCompletionCallback runs once per rendered frame, after the frame
+    is complete, with the frame handle and the user data; a callback may
+    release its own frame."""
+    import threading
+    import time
+    wl = _same_tf(pkg, CASES["structured raycast"](pkg.workloads))
+    a = A.Anari()
+    calls = []
+    done = threading.Event()
+
+    def on_frame(user, dev, frame):
+        calls.append((user, frame, threading.get_ident()))
+        if len(calls) == 3:
+            done.set()
+
+    cb = A.FRAME_CALLBACK(on_frame)
+    try:
+        fr, objs = _anari_scene(a, wl)
+        a.set(fr, "frameCompletionCallback", A.FRAME_COMPLETION_CALLBACK,
+              C.cast(cb, C.c_void_p).value)
+        a.set(fr, "frameCompletionCallbackUserData", A.VOID_POINTER, 1234)
+        a.commit(fr)
+        for _ in range(3):
+            a.render(fr, wait=False)
+        assert done.wait(30.0), calls
+        time.sleep(0.1)
+        assert len(calls) == 3
+        assert all(u == 1234 and f == fr for u, f, _ in calls)
+        assert all(tid != threading.get_ident() for _, _, tid in calls)
+        img, _ = a.map(fr)
+        assert np.abs(img).max() > 0
+
+        # a second frame whose callback releases it
+        released = threading.Event()
+        fr2 = a.new(A.FRAME)
+
+        def release_self(user, dev, frame):
+            a.L.anariRelease(a.dev, frame)
+            released.set()
+
+        cb2 = A.FRAME_CALLBACK(release_self)
+        a.set(fr2, "size", A.UINT32_VEC2, (wl.width, wl.height))
+        a.set(fr2, "channel.color", A.DATA_TYPE, A.FLOAT32_VEC4)
+        for k in ("world", "camera", "renderer"):
+            a.set(fr2, k, {"world": A.WORLD, "camera": A.CAMERA, "renderer": A.RENDERER}[k],
+                  objs[k])
+        a.set(fr2, "frameCompletionCallback", A.FRAME_COMPLETION_CALLBACK,
+              C.cast(cb2, C.c_void_p).value)
+        a.commit(fr2)
+        a.render(fr2, wait=False)
+        assert released.wait(30.0)
         errors = [m for m in a.messages if m[0] <= 1]
         assert not errors, errors
     finally:
