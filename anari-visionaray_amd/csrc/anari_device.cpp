@@ -977,6 +977,7 @@ void render(Device *d, Frame *fr) {
   p.pixel_samples = std::min(std::max(ren->get_i("pixelSamples", 1), 1), 256);
   p.volume_sampling_rate = ren->get_f("volumeSamplingRate", 0.5f);
   p.phase_g = ren->get_f("phaseG", 0.f);  // extension
+  p.gradient_shading = ren->get_b("gradientShading", false) ? 1 : 0;  // Renderer.cpp:50
   if (const Array *cp = array_param(ren, "clipPlane")) {
     if (cp->elem == ANARI_FLOAT32_VEC4) {
       p.num_clip_planes = int(std::min<uint64_t>(cp->count(), VRT_MAX_CLIP_PLANES));

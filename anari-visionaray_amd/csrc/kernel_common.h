@@ -127,6 +127,7 @@ struct KParams {
   int num_clip_planes;
   float4 clip[VRT_MAX_CLIP_PLANES];
   float phase_g;
+  int gradient_shading;
 };
 
 struct KFrame {

@@ -21,7 +21,8 @@ hipError_t launch_nrc_resolve(const KFrame &F, const KNrc &N, int accumID, hipSt
 // adds sum (x - r)^2 / (r^2 + eps) over n pixels' RGB to *out
 hipError_t launch_relmse(const float4 *x, const double *ref_sum, double ref_scale, int n,
                          float eps, double *out, hipStream_t s);
-int pathtrace_blocks_per_cu(const KScene &sc, size_t lds_bytes, bool nrc = false);
+int pathtrace_blocks_per_cu(const KScene &sc, size_t lds_bytes, bool nrc = false,
+                            bool grad = false);
 hipError_t launch_brickify(const float *lin, const int dims[3], float *bricks, hipStream_t s);
 hipError_t launch_value_ranges(const KVolume &v, float *ranges, hipStream_t s);
 // nanovdb: min/max (float2) of each block slot's values

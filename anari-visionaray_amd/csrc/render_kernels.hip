@@ -317,6 +317,26 @@ __device__ inline bool sample_field(const KVolume &v, f3 P, float &value) {
   return sample_field<ANYF>(v, P, value, rec);
 }
 
+// sampleGradient (DeviceCopyableObjects.h:582-597): central differences at
+// P +- delta along each axis, false unless all six samples have a value.
+// The gradient normal is its normalisation; a zero gradient (a flat region)
+// gives no normal here -- the reference's normalize(0) is NaN.
+template <bool ANYF = true>
+__device__ inline f3 gradient_normal(const KVolume &v, f3 P) {
+  const f3 delta = v.S * v.cell_size;  // cellSize * diag(voxelSpaceTransform)
+  float x0, x1, y0, y1, z0, z1;
+  const bool b0 = sample_field<ANYF>(v, P + mk(delta.x, 0.f, 0.f), x1);
+  const bool b1 = sample_field<ANYF>(v, P - mk(delta.x, 0.f, 0.f), x0);
+  const bool b2 = sample_field<ANYF>(v, P + mk(0.f, delta.y, 0.f), y1);
+  const bool b3 = sample_field<ANYF>(v, P - mk(0.f, delta.y, 0.f), y0);
+  const bool b4 = sample_field<ANYF>(v, P + mk(0.f, 0.f, delta.z), z1);
+  const bool b5 = sample_field<ANYF>(v, P - mk(0.f, 0.f, delta.z), z0);
+  if (!(b0 && b1 && b2 && b3 && b4 && b5)) return mk(0.f);
+  const f3 g = mk(x1, y1, z1) - mk(x0, y0, z0);
+  if (g.x == 0.f && g.y == 0.f && g.z == 0.f) return mk(0.f);
+  return normalize(g);
+}
+
 // postClassify (DeviceCopyableObjects.h:615-619): tex1D(linear, clamp) of the
 // TF table, read from the workgroup's LDS copy
 // the two table entries and the weight tex1D(linear, clamp) reads for val
@@ -910,6 +930,72 @@ __device__ inline void wave_add_counter(unsigned long long *ctr, uint32_t v) {
 // alpha < 0.99), and fetches past the end are discarded.
 constexpr int kRaySteps = 4;
 
+// sampleLight (renderer/common.h:1432-1466).  Directional: the stored
+// direction, pdf 1, distance FLT_MAX, no draws.  Quad (area_light<Quad>, not
+// vendored: parity unpinned): one draw picks a triangle
+// (DeviceCopyableObjects.h:2258-2270), two place a uniform point on it;
+// pdf = 1 / (A |cos theta_light|), so that with the 1/dist^2 of sampleLight
+// the NEE estimate is the area-sampling estimator.
+struct LightSmp {
+  f3 intensity, dir;
+  float pdf, dist, dist2;
+  int type;
+};
+
+template <bool AREA>
+__device__ inline LightSmp sample_light(const KLight &L, f3 hitPos, uint32_t &rs) {
+  LightSmp ls;
+  ls.type = L.type;
+  ls.intensity = L.intensity;
+  if (AREA && L.type == VRT_LIGHT_QUAD) {
+    const float r = rnd(rs);
+    const f3 d = L.e1 + L.e2;
+    const bool first = L.a1 / (L.a1 + L.a2) < r;
+    const f3 te1 = first ? L.e1 : d, te2 = first ? d : L.e2;
+    const float u1 = rnd(rs), u2 = rnd(rs);
+    const float su = sqrtf(u1);
+    const f3 v2 = L.v1 + te1, v3 = L.v1 + te2;
+    const f3 pos = L.v1 * (1.f - su) + v2 * (su * (1.f - u2)) + v3 * (su * u2);
+    ls.dir = pos - hitPos;
+    ls.dist = length(ls.dir);
+    const float cosl = fabsf(dot(L.nrm, ls.dir * (1.f / ls.dist)));
+    ls.pdf = cosl > 0.f ? 1.f / ((L.a1 + L.a2) * cosl) : 0.f;
+    ls.dist2 = ls.dist * ls.dist;
+  } else {
+    ls.dir = L.dir;
+    ls.dist = FLT_MAX;
+    ls.pdf = 1.f;
+    ls.dist2 = 1.f;
+  }
+  return ls;
+}
+
+__device__ inline float safe_rcp(float f) { return f > 0.f ? 1.f / f : 0.f; }
+
+// Gradient shading of one ray-caster sample (VolumeIntegration.h:420-459):
+// matte (albedo / pi, common.h:825-828, 1033-1048) under every light, along
+// the faceforwarded gradient normal at the voxel-space sample point P, plus
+// the ambient term.  Lights are sampled at P itself, in voxel space, as the
+// reference does.
+template <bool ANYF>
+__device__ inline f3 gradient_shade(const KScene &sc, const KVolume &v, const KParams &P, f3 Pv,
+                                    f3 viewDir, f3 albedo, uint32_t &rs) {
+  f3 gn = gradient_normal<ANYF>(v, Pv);
+  gn = faceforward(gn, viewDir, gn);
+  f3 shaded = mk(0.f);
+  if (length(gn) > 1e-10f) {
+    const float inv_pi = 0.318309886183790671538f;
+    for (int i = 0; i < sc.num_lights; ++i) {
+      const LightSmp ls = sample_light<true>(sc.light[i], Pv, rs);
+      const f3 lightDir = normalize(ls.dir);
+      const f3 lightIntensity = ls.intensity * safe_rcp(ls.dist2);
+      const float NdotL = fmaxf(0.f, dot(gn, lightDir));
+      shaded = shaded + albedo * inv_pi * lightIntensity * NdotL * safe_rcp(ls.pdf);
+    }
+  }
+  return shaded + albedo * P.ambient_color * P.ambient_radiance;
+}
+
 // Frame::fillGBuffer (DeviceCopyableObjects.h:2745-2774): the channels other
 // than depth take the values of the pixel's closest sample (first of equal
 // depths, Pathtrace_impl.cpp:616) when the frame has a depth channel and that
@@ -945,7 +1031,7 @@ __device__ inline void gbuf_miss(const KFrame &F, size_t i) {
   if (1e31f <= F.depth[i]) gbuf_write(F, i, mk(0.f), mk(0.f), ~0u, ~0u, ~0u);
 }
 
-template <bool ANYF>
+template <bool ANYF, bool SHADE>
 __global__ void __launch_bounds__(256)
     k_raycast(KScene sc, KCamera cam, KParams P, KFrame F) {
   extern __shared__ float4 s_tf[];
@@ -983,6 +1069,9 @@ __global__ void __launch_bounds__(256)
         const float tmax = fminf(lr.tmax, bh.tfar);
         const f3 ori = v.S * (lr.ori + v.T);
         const f3 dir = v.S * lr.dir;  // not renormalised (VolumeIntegration.h:395-396)
+        // gradient shading: lights sampled with the pixel's path-tracer stream
+        // (the reference's ray caster leaves its generator unseeded)
+        uint32_t rs = SHADE ? tea4(uint32_t(y * F.width + x), 0u) : 0u;
         const float dt = v.cell_size * P.sampling_rate_inv;
         const float expo = dt / v.unit_distance;
         float cr = 0.f, cg = 0.f, cb = 0.f, alpha = 0.f, trans = 1.f;
@@ -995,13 +1084,14 @@ __global__ void __launch_bounds__(256)
         float t = tmin;
         bool marching = true;
         while (marching) {
-          float val[kRaySteps];
+          float val[kRaySteps], tb[kRaySteps];
           bool in[kRaySteps], has[kRaySteps];
 #pragma unroll
           for (int b = 0; b < kRaySteps; ++b) {
             // the same t sequence as for (t = tmin; ...; t += dt)
             in[b] = t <= tmax && step + b < max_steps;
             has[b] = in[b] && sample_field<ANYF>(v, ori + dir * t, val[b]);
+            tb[b] = t;
             t += dt;
           }
 #pragma unroll
@@ -1015,9 +1105,11 @@ __global__ void __launch_bounds__(256)
             const float4 s = classify(v, s_tf, val[b]);
             const float stepT = pow_ieee(1.f - s.w, expo);
             const float k = trans * (1.f - stepT);
-            cr += k * s.x;
-            cg += k * s.y;
-            cb += k * s.z;
+            f3 c = mk(s.x, s.y, s.z);
+            if constexpr (SHADE) c = gradient_shade<ANYF>(sc, v, P, ori + dir * tb[b], -dir, c, rs);
+            cr += k * c.x;
+            cg += k * c.y;
+            cb += k * c.z;
             alpha += k;
             trans *= stepT;
           }
@@ -1262,47 +1354,6 @@ __device__ inline PassHit intersect_all(const KScene &sc, const KRay &ray, bool 
   return out;
 }
 
-// sampleLight (renderer/common.h:1432-1466).  Directional: the stored
-// direction, pdf 1, distance FLT_MAX, no draws.  Quad (area_light<Quad>, not
-// vendored: parity unpinned): one draw picks a triangle
-// (DeviceCopyableObjects.h:2258-2270), two place a uniform point on it;
-// pdf = 1 / (A |cos theta_light|), so that with the 1/dist^2 of sampleLight
-// the NEE estimate is the area-sampling estimator.
-struct LightSmp {
-  f3 intensity, dir;
-  float pdf, dist, dist2;
-  int type;
-};
-
-template <bool AREA>
-__device__ inline LightSmp sample_light(const KLight &L, f3 hitPos, uint32_t &rs) {
-  LightSmp ls;
-  ls.type = L.type;
-  ls.intensity = L.intensity;
-  if (AREA && L.type == VRT_LIGHT_QUAD) {
-    const float r = rnd(rs);
-    const f3 d = L.e1 + L.e2;
-    const bool first = L.a1 / (L.a1 + L.a2) < r;
-    const f3 te1 = first ? L.e1 : d, te2 = first ? d : L.e2;
-    const float u1 = rnd(rs), u2 = rnd(rs);
-    const float su = sqrtf(u1);
-    const f3 v2 = L.v1 + te1, v3 = L.v1 + te2;
-    const f3 pos = L.v1 * (1.f - su) + v2 * (su * (1.f - u2)) + v3 * (su * u2);
-    ls.dir = pos - hitPos;
-    ls.dist = length(ls.dir);
-    const float cosl = fabsf(dot(L.nrm, ls.dir * (1.f / ls.dist)));
-    ls.pdf = cosl > 0.f ? 1.f / ((L.a1 + L.a2) * cosl) : 0.f;
-    ls.dist2 = ls.dist * ls.dist;
-  } else {
-    ls.dir = L.dir;
-    ls.dist = FLT_MAX;
-    ls.pdf = 1.f;
-    ls.dist2 = 1.f;
-  }
-  return ls;
-}
-
-__device__ inline float safe_rcp(float f) { return f > 0.f ? 1.f / f : 0.f; }
 
 // Russian roulette, run after every pass that continues the path once
 // bounceID > 1 -- shadow and AO passes included -- drawing from the
@@ -1321,7 +1372,7 @@ __device__ inline bool roulette(PathState &st) {
 // Returns true when the sample's path ended.  With NRC, a path reaching its
 // second volume vertex either becomes a cache query and ends, or -- a
 // training path -- records the vertex and goes on.
-template <bool NRC, bool AREA>
+template <bool NRC, bool AREA, bool ANYF, bool GRAD>
 __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams &P,
                                   const KNrc &N, const KFrame &F, const PassHit &h) {
   KRay ray;
@@ -1396,20 +1447,25 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
     const bool gbuf = st.bounceID == 0 && gbuf_wanted(F) && h.t < st.depth &&
                       h.t <= F.depth[st.pixel];
     if (h.kind == kVolume) {
-      if (P.ambient_samples > 0) {
+      const KVolume &hv = sc.vol[h.id];
+      if (GRAD) {
+        // the gradient at the hit in voxel space (Pathtrace_impl.cpp:204-212)
+        const KRay lr = to_local(hv, ray);
+        gn = gradient_normal<ANYF>(hv, hv.S * ((lr.ori + lr.dir * h.t) + hv.T));
+      }
+      if (P.ambient_samples > 0 && length(gn) < 1e-3f) {
         const float u1 = rnd(st.srng), u2 = rnd(st.srng);
         gn = uniform_sample_sphere(u1, u2);
+      }
+      if (GRAD || P.ambient_samples > 0) {
         // normal transform transpose(affineInv) (Pathtrace_impl.cpp:217-219,
         // Instance.cpp:96-98), as mat3 * vec3 = col0 x + col1 y + col2 z
-        float A[9];
-        for (int k = 0; k < 9; ++k) A[k] = sc.vol[0].A[k];
-        for (int i = 1; i < sc.num_volumes; ++i)
-          if (i == h.id)
-            for (int k = 0; k < 9; ++k) A[k] = sc.vol[i].A[k];
+        const float *A = hv.A;
         gn = mk(A[0] * gn.x + A[3] * gn.y + A[6] * gn.z, A[1] * gn.x + A[4] * gn.y + A[7] * gn.z,
                 A[2] * gn.x + A[5] * gn.y + A[8] * gn.z);
       }
       st.sn = faceforward(gn, viewDir, gn);
+      gn = faceforward(gn, viewDir, gn);
       color = h.albedo;
       if (gbuf) {
         uint32_t obj = 0, inst = 0, prim = 0;
@@ -1419,7 +1475,7 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
             inst = sc.vol[i].inst_id;
             if (h.rec != kNoRec) prim = sc.vol[i].ug_cell[h.rec];
           }
-        gbuf_write(F, size_t(st.pixel), faceforward(gn, viewDir, gn), color, prim, obj, inst);
+        gbuf_write(F, size_t(st.pixel), gn, color, prim, obj, inst);
       }
     } else if (AREA) {
       // triangle: Ng = normalize(e1 x e2), no vertex normals (getNormals,
@@ -1456,14 +1512,22 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
       // isotropic phase sample: f = albedo, pdf = cosTheta = 1.  With an
       // anisotropic phase (g != 0) NEE carries 4 pi p_HG towards the light
       // and the sample is drawn from p_HG, whose f/pdf is again the albedo.
-      st.direct = h.albedo * lightIntensity * safe_rcp(ls.pdf) * safe_rcp(ls.dist2);
+      // gradient shading: matte NEE along the gradient normal (:311-328)
+      const bool matte = GRAD && length(gn) > 1e-10f;
+      if (matte) {
+        const float inv_pi = 0.318309886183790671538f;
+        const float NdotL = fmaxf(0.f, dot(st.sn, normalize(ls.dir)));
+        st.direct = h.albedo * inv_pi * lightIntensity * NdotL * safe_rcp(ls.pdf) * float(nl);
+      } else {
+        st.direct = h.albedo * lightIntensity * safe_rcp(ls.pdf) * safe_rcp(ls.dist2);
+      }
       const float u1 = rnd(st.srng), u2 = rnd(st.srng);
       if (P.phase_g == 0.f) {
         st.bs_dir = uniform_sample_sphere(u1, u2);
         st.bs_pdf = 1.f;
       } else {
         const f3 d = normalize(ray.dir);
-        if (nl > 0) st.direct = st.direct * hg_4pi(P.phase_g, dot(d, normalize(ls.dir)));
+        if (nl > 0 && !matte) st.direct = st.direct * hg_4pi(P.phase_g, dot(d, normalize(ls.dir)));
         st.bs_dir = hg_sample(P.phase_g, d, u1, u2);
         st.bs_pdf = hg_4pi(P.phase_g, dot(d, st.bs_dir));
       }
@@ -1551,7 +1615,9 @@ __device__ inline float train_hash(uint32_t pixel, uint32_t accumID) {
          (1.0f / 16777216.0f);
 }
 
-template <bool NRC, bool AREA, bool ANYF>
+// GRAD: gradient shading (the renderer's gradientShading), compiled only
+// into the variants that use it
+template <bool NRC, bool AREA, bool ANYF, bool GRAD>
 __global__ void __launch_bounds__(256)
     k_pathtrace(KScene sc, KCamera cam, KParams P, KFrame F, int accumID, uint32_t *queues,
                 uint32_t *queues_next, KNrc N) {
@@ -1646,7 +1712,7 @@ __global__ void __launch_bounds__(256)
       clip_ray(ray, P);  // every pass (Pathtrace_impl.cpp:559)
       const PassHit hit = intersect_all<AREA, ANYF>(sc, ray, st.rayType != Radiance, st.bounceID,
                                                     s_tf, steps, st.srng, lookups, bstack);
-      if (path_shade<NRC, AREA>(st, sc, P, N, F, hit)) {
+      if (path_shade<NRC, AREA, ANYF, GRAD>(st, sc, P, N, F, hit)) {
         float4 color = P.bg;
         if (st.firstHit) color = make_float4(st.intensity.x, st.intensity.y, st.intensity.z, 1.f);
         st.acc.x += color.x;
@@ -2061,10 +2127,10 @@ static bool scene_has_other_fields(const KScene &sc);
 hipError_t launch_raycast(const KScene &sc, const KCamera &cam, const KParams &P, const KFrame &F,
                           hipStream_t s) {
   const size_t lds = scene_lds_bytes(sc);
-  if (scene_has_other_fields(sc))
-    hipLaunchKernelGGL(k_raycast<true>, dim3(tiles(F)), dim3(256), lds, s, sc, cam, P, F);
-  else
-    hipLaunchKernelGGL(k_raycast<false>, dim3(tiles(F)), dim3(256), lds, s, sc, cam, P, F);
+  const bool any = scene_has_other_fields(sc), shade = P.gradient_shading != 0;
+  auto *k = any ? (shade ? k_raycast<true, true> : k_raycast<true, false>)
+                : (shade ? k_raycast<false, true> : k_raycast<false, false>);
+  hipLaunchKernelGGL(k, dim3(tiles(F)), dim3(256), lds, s, sc, cam, P, F);
   return hipGetLastError();
 }
 
@@ -2087,13 +2153,18 @@ static bool scene_has_other_fields(const KScene &sc) {
 using PathtraceKernel = void (*)(KScene, KCamera, KParams, KFrame, int, uint32_t *, uint32_t *,
                                  KNrc);
 
-static PathtraceKernel pathtrace_kernel(const KScene &sc, bool nrc) {
-  static const PathtraceKernel table[8] = {
-      k_pathtrace<false, false, false>, k_pathtrace<false, false, true>,
-      k_pathtrace<false, true, false>,  k_pathtrace<false, true, true>,
-      k_pathtrace<true, false, false>,  k_pathtrace<true, false, true>,
-      k_pathtrace<true, true, false>,   k_pathtrace<true, true, true>};
-  return table[(nrc ? 4 : 0) + (scene_has_area(sc) ? 2 : 0) + (scene_has_other_fields(sc) ? 1 : 0)];
+static PathtraceKernel pathtrace_kernel(const KScene &sc, bool nrc, bool grad) {
+  static const PathtraceKernel table[16] = {
+      k_pathtrace<false, false, false, false>, k_pathtrace<false, false, true, false>,
+      k_pathtrace<false, true, false, false>,  k_pathtrace<false, true, true, false>,
+      k_pathtrace<true, false, false, false>,  k_pathtrace<true, false, true, false>,
+      k_pathtrace<true, true, false, false>,   k_pathtrace<true, true, true, false>,
+      k_pathtrace<false, false, false, true>,  k_pathtrace<false, false, true, true>,
+      k_pathtrace<false, true, false, true>,   k_pathtrace<false, true, true, true>,
+      k_pathtrace<true, false, false, true>,   k_pathtrace<true, false, true, true>,
+      k_pathtrace<true, true, false, true>,    k_pathtrace<true, true, true, true>};
+  return table[(grad ? 8 : 0) + (nrc ? 4 : 0) + (scene_has_area(sc) ? 2 : 0) +
+               (scene_has_other_fields(sc) ? 1 : 0)];
 }
 
 size_t pathtrace_lds_size(const KScene &sc) {
@@ -2105,7 +2176,8 @@ hipError_t launch_pathtrace(const KScene &sc, const KCamera &cam, const KParams 
                             uint32_t *queues_next, int blocks, hipStream_t s, const KNrc *nrc) {
   const size_t lds = pathtrace_lds_size(sc);
   if (nrc) hipLaunchKernelGGL(k_nrc_frame_begin, dim3(1), dim3(64), 0, s, *nrc);
-  hipLaunchKernelGGL(pathtrace_kernel(sc, nrc != nullptr), dim3(blocks), dim3(256), lds, s, sc,
+  hipLaunchKernelGGL(pathtrace_kernel(sc, nrc != nullptr, P.gradient_shading != 0), dim3(blocks),
+                     dim3(256), lds, s, sc,
                      cam, P, F, accumID, queues, queues_next, nrc ? *nrc : KNrc{});
   return hipGetLastError();
 }
@@ -2117,10 +2189,10 @@ hipError_t launch_nrc_resolve(const KFrame &F, const KNrc &N, int accumID, hipSt
   return hipGetLastError();
 }
 
-int pathtrace_blocks_per_cu(const KScene &sc, size_t lds_bytes, bool nrc) {
+int pathtrace_blocks_per_cu(const KScene &sc, size_t lds_bytes, bool nrc, bool grad) {
   int n = 0;
   const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &n, pathtrace_kernel(sc, nrc), 256, lds_bytes);
+      &n, pathtrace_kernel(sc, nrc, grad), 256, lds_bytes);
   if (e != hipSuccess) return 1;
   return n > 0 ? n : 1;
 }

@@ -290,6 +290,7 @@ int make_params(const vrt_params *p, KParams &k) {
   if (!(p->phase_g > -1.f && p->phase_g < 1.f))
     return fail(VRT_ERROR_INVALID_ARGUMENT, "phase_g %g out of (-1, 1)", double(p->phase_g));
   k.phase_g = p->phase_g;
+  k.gradient_shading = p->gradient_shading ? 1 : 0;
   for (int i = 0; i < VRT_MAX_CLIP_PLANES; ++i) {
     const float *c = p->clip_planes[i];
     k.clip[i] = i < p->num_clip_planes ? make_float4(c[0], c[1], c[2], c[3])
@@ -660,7 +661,8 @@ int vrt_pathtrace(vrt_context ctx, const vrt_scene *scene, const vrt_camera *cam
   if (int r = make_frame(frame, F)) return r;
   if (!F.accum) return fail(VRT_ERROR_INVALID_ARGUMENT, "pathtrace needs an accumulation buffer");
   const size_t lds = pathtrace_lds_size(sc);
-  int blocks = std::max(1, ctx->num_cus) * pathtrace_blocks_per_cu(sc, lds);
+  int blocks = std::max(1, ctx->num_cus) * pathtrace_blocks_per_cu(sc, lds, false,
+                                                                    P.gradient_shading != 0);
   blocks = std::max(8, (blocks + 7) / 8 * 8);  // every XCD queue gets workgroups
   uint32_t *cur = ctx->queues + 8 * ctx->queue_parity;
   uint32_t *nxt = ctx->queues + 8 * (1 - ctx->queue_parity);
@@ -716,7 +718,8 @@ int vrt_pathtrace_nrc(vrt_context ctx, const vrt_scene *scene, const vrt_camera 
                    1.f / (opts->bounds_hi[2] - opts->bounds_lo[2]));
 
   const size_t lds = pathtrace_lds_size(sc);
-  int blocks = std::max(1, ctx->num_cus) * pathtrace_blocks_per_cu(sc, lds, true);
+  int blocks = std::max(1, ctx->num_cus) * pathtrace_blocks_per_cu(sc, lds, true,
+                                                                    P.gradient_shading != 0);
   blocks = std::max(8, (blocks + 7) / 8 * 8);
   uint32_t *cur = ctx->queues + 8 * ctx->queue_parity;
   uint32_t *nxt = ctx->queues + 8 * (1 - ctx->queue_parity);

@@ -96,7 +96,7 @@ class Params(C.Structure):
         ("max_bounce", C.c_int), ("pixel_samples", C.c_int),
         ("volume_sampling_rate", C.c_float), ("rng_offset", C.c_uint32),
         ("num_clip_planes", C.c_int), ("clip_planes", (C.c_float * 4) * 8),
-        ("phase_g", C.c_float),
+        ("phase_g", C.c_float), ("gradient_shading", C.c_int),
     ]
 
 
@@ -667,7 +667,8 @@ def camera(position, direction, up, fovy, aspect, region=(0, 0, 1, 1), shutter=(
 
 def params(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.2,
            occlusion_distance=1e20, ambient_samples=1, max_bounce=7, pixel_samples=1,
-           volume_sampling_rate=0.5, rng_offset=0, clip_planes=(), phase_g=0.0):
+           volume_sampling_rate=0.5, rng_offset=0, clip_planes=(), phase_g=0.0,
+           gradient_shading=False):
     """Renderer parameters with the ANARI defaults of the reference
     (Renderer.cpp:42-53, Pathtrace.cpp:19-25, Raycast.cpp:19).  `clip_planes`:
     up to 8 (nx, ny, nz, d); rays keep n.x <= d.  `phase_g`: Henyey-Greenstein
@@ -675,6 +676,7 @@ def params(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.
     p = Params()
     p.rng_offset = rng_offset
     p.phase_g = phase_g
+    p.gradient_shading = int(bool(gradient_shading))
     if len(clip_planes) > 8:
         raise ValueError("at most 8 clip planes")
     p.num_clip_planes = len(clip_planes)

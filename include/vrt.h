@@ -175,6 +175,12 @@ typedef struct vrt_params {
      values sample HG around the ray direction with the same two random
      numbers and weight volume NEE by 4 pi p_HG, which is 1 at g = 0. */
   float phase_g;
+  /* gradientShading (Renderer.cpp:50, default 0): the ray caster shades
+     each sample as matte under every light along the field's central-
+     difference gradient (VolumeIntegration.h:420-459); the path tracer takes
+     that gradient as the volume normal for ambient occlusion and its NEE as
+     matte (Pathtrace_impl.cpp:204-212, 311-328) */
+  int gradient_shading;
 } vrt_params;
 
 /* ANARI data types used for frame channels (ANARI 1.0 enum values). */

@@ -448,6 +448,25 @@ template <class R> struct Volume {
   // the TF at (i+.5)/N under the same convention)
   // sampleField (DeviceCopyableObjects.h:474-570): false where the field has
   // no value (a point outside every cell of an unstructured field)
+  // sampleGradient (DeviceCopyableObjects.h:582-597) at voxel-space P with
+  // delta = cellSize * diag(voxelSpaceTransform), normalised; no normal
+  // (0) unless all six samples have a value and the gradient is not zero
+  // (the reference's normalize(0) would be NaN)
+  V3<R> gradient_normal(V3<R> P) const {
+    const V3<R> delta = S * cellSize;
+    R x0, x1, y0, y1, z0, z1;
+    const bool b0 = sample(P + V3<R>(delta.x, R(0), R(0)), x1);
+    const bool b1 = sample(P - V3<R>(delta.x, R(0), R(0)), x0);
+    const bool b2 = sample(P + V3<R>(R(0), delta.y, R(0)), y1);
+    const bool b3 = sample(P - V3<R>(R(0), delta.y, R(0)), y0);
+    const bool b4 = sample(P + V3<R>(R(0), R(0), delta.z), z1);
+    const bool b5 = sample(P - V3<R>(R(0), R(0), delta.z), z0);
+    if (!(b0 && b1 && b2 && b3 && b4 && b5)) return V3<R>(R(0));
+    const V3<R> g = V3<R>(x1, y1, z1) - V3<R>(x0, y0, z0);
+    if (g.x == R(0) && g.y == R(0) && g.z == R(0)) return V3<R>(R(0));
+    return normalize(g);
+  }
+
   // `prim`: the primitive id of the sample -- the containing cell of an
   // unstructured field, 0 for the other fields (:478-481, 556-567)
   bool sample(V3<R> P, R &value, int64_t *prim = nullptr) const {
@@ -1174,6 +1193,11 @@ void shade(Random &srnd, const Ray<R> &ray, RayType rayType, const Scene<R> &sc,
       hitPos = ray.ori + ray.dir * hitRec.vol.t;
       eps = epsilon_from(hitPos, ray.dir, hitRec.vol.t);
       gn = V3<R>(R(0));
+      const Volume<R> &V = sc.vols[hitRec.vol_inst];
+      if (P->gradient_shading) {  // :204-212, at the hit in voxel space
+        const Ray<R> lr = V.to_local(ray);
+        gn = V.gradient_normal(V.point_to_voxel(lr.ori + lr.dir * hitRec.vol.t));
+      }
       if (P->ambient_samples > 0 && length(gn) < R(1e-3)) {
         R u1 = R(srnd()), u2 = R(srnd());
         gn = uniform_sample_sphere(u1, u2);
@@ -1181,7 +1205,6 @@ void shade(Random &srnd, const Ray<R> &ray, RayType rayType, const Scene<R> &sc,
       sn = gn;
       // getNormalTransform (renderer/common.h:766-770): transpose(affineInv),
       // no renormalisation for volumes (:217-219)
-      const Volume<R> &V = sc.vols[hitRec.vol_inst];
       gn = V.mul_normal(gn);
       sn = V.mul_normal(sn);
       sn = faceforward(sn, viewDir, gn);
@@ -1253,9 +1276,18 @@ void shade(Random &srnd, const Ray<R> &ray, RayType rayType, const Scene<R> &sc,
       st.bs_w = f * (st.bs_cosT * safe_rcp(st.bs_pdf));
     }
     if (hitRec.volumeHit) {
-      // Pathtrace_impl.cpp:330-339: no phase factor, no light count
-      st.shadedColor = hitRec.vol.albedo * lightIntensity * safe_rcp(st.ls.pdf) *
-                       safe_rcp(st.ls.dist2);
+      // gradient shading: matte NEE along the gradient normal (:311-328);
+      // else :330-339, no phase factor, no light count
+      const bool matte = P->gradient_shading && length(gn) > R(1e-10);
+      if (matte) {
+        const R inv_pi = R(0.318309886183790671538);
+        const R NdotL = std::max(R(0), dot(sn, normalize(st.ls.dir)));
+        st.shadedColor = hitRec.vol.albedo * inv_pi * lightIntensity * NdotL *
+                         safe_rcp(st.ls.pdf) * R(numLights);
+      } else {
+        st.shadedColor = hitRec.vol.albedo * lightIntensity * safe_rcp(st.ls.pdf) *
+                         safe_rcp(st.ls.dist2);
+      }
       R u1 = R(srnd()), u2 = R(srnd());
       if (P->phase_g == 0.f) {
         st.bs_dir = uniform_sample_sphere(u1, u2);
@@ -1265,7 +1297,8 @@ void shade(Random &srnd, const Ray<R> &ray, RayType rayType, const Scene<R> &sc,
         // the light, the sample is drawn from p_HG and weighs the albedo
         const R g = R(P->phase_g);
         const V3<R> d = normalize(ray.dir);
-        if (numLights > 0) st.shadedColor = st.shadedColor * hg_4pi(g, dot(d, normalize(st.ls.dir)));
+        if (numLights > 0 && !matte)
+          st.shadedColor = st.shadedColor * hg_4pi(g, dot(d, normalize(st.ls.dir)));
         st.bs_dir = hg_sample(g, d, u1, u2);
         st.bs_pdf = hg_4pi(g, dot(d, st.bs_dir));
       }
@@ -1408,6 +1441,30 @@ V4<R> pathtrace_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params
 // ---------------------------------------------------------------------------
 // A14/A15: raycast (Raycast_impl.cpp, VolumeIntegration.h:374-470)
 // ---------------------------------------------------------------------------
+// Gradient shading of one ray-caster sample (VolumeIntegration.h:420-459):
+// matte under every light along the faceforwarded gradient normal at the
+// voxel-space point Pv (lights sampled there, as the reference does) plus
+// the ambient term
+template <class R>
+V3<R> gradient_shade(const Scene<R> &sc, const Volume<R> &vol, const orc_params *P, V3<R> Pv,
+                     V3<R> viewDir, V3<R> albedo, Random &rs) {
+  V3<R> gn = vol.gradient_normal(Pv);
+  gn = faceforward(gn, viewDir, gn);
+  V3<R> shaded(R(0));
+  auto safe_rcp = [](R f) { return f > R(0) ? R(1) / f : R(0); };
+  if (length(gn) > R(1e-10)) {
+    const R inv_pi = R(0.318309886183790671538);
+    for (int i = 0; i < sc.s->num_lights; ++i) {
+      const LightSample<R> ls = sample_light<R>(sc.s->lights[i], sc.quads[i], Pv, rs);
+      const V3<R> lightDir = normalize(ls.dir);
+      const V3<R> lightIntensity = ls.intensity * safe_rcp(ls.dist2);
+      const R NdotL = std::max(R(0), dot(gn, lightDir));
+      shaded = shaded + albedo * inv_pi * lightIntensity * NdotL * safe_rcp(ls.pdf);
+    }
+  }
+  return shaded + albedo * from<R>(P->ambient_color) * R(P->ambient_radiance);
+}
+
 template <class R>
 V4<R> raycast_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params *P,
                     int x, int y, int W, int H) {
@@ -1443,13 +1500,18 @@ V4<R> raycast_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params *
   const R expo = dt / R(vol.v->unit_distance);
   V3<R> color(R(0));
   R alpha = R(0), transmittance = R(1);
+  // gradient shading samples lights with the pixel's path-tracer stream (the
+  // reference's ray caster leaves its generator unseeded)
+  Random rs(uint32_t(x + W * y), 0u);
   for (R t = lr.tmin; t <= lr.tmax && alpha < R(0.99f); t += dt) {
     V3<R> p = ori + dir * t;
     R v;
     if (!vol.sample(p, v)) continue;  // VolumeIntegration.h:412
     V4<R> s = vol.classify(v);
     R stepT = opacity_pow(R(1) - s.w, expo);
-    color = color + V3<R>(s.x, s.y, s.z) * (transmittance * (R(1) - stepT));
+    V3<R> c(s.x, s.y, s.z);
+    if (P->gradient_shading) c = gradient_shade(sc, vol, P, p, -dir, c, rs);
+    color = color + c * (transmittance * (R(1) - stepT));
     alpha += transmittance * (R(1) - stepT);
     transmittance *= stepT;
   }

@@ -123,6 +123,9 @@ typedef struct orc_params {
   /* Henyey-Greenstein anisotropy of the volume phase function (an
      extension; 0 = the reference's isotropic phase) */
   float phase_g;
+  /* gradientShading (Renderer.cpp:50): raycast matte shading along the
+     central-difference gradient; path-tracer volume normal and matte NEE */
+  int gradient_shading;
 } orc_params;
 
 /* --- A1: LCG<4> / Random (common.h:55-102) --- */

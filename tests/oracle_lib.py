@@ -57,7 +57,7 @@ class OrcParams(C.Structure):
                 ("max_bounce", C.c_int), ("pixel_samples", C.c_int),
                 ("volume_sampling_rate", C.c_float), ("rng_offset", C.c_uint32),
                 ("num_clip_planes", C.c_int), ("clip_planes", (C.c_float * 4) * 8),
-                ("phase_g", C.c_float)]
+                ("phase_g", C.c_float), ("gradient_shading", C.c_int)]
 
 
 _lib = None
@@ -382,9 +382,11 @@ def camera(position, direction, up, fovy, aspect, region=(0, 0, 1, 1), shutter=(
 
 def params(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.2,
            occlusion_distance=1e20, ambient_samples=1, max_bounce=7, pixel_samples=1,
-           volume_sampling_rate=0.5, rng_offset=0, clip_planes=(), phase_g=0.0):
+           volume_sampling_rate=0.5, rng_offset=0, clip_planes=(), phase_g=0.0,
+           gradient_shading=False):
     p = OrcParams()
     p.phase_g = phase_g
+    p.gradient_shading = int(bool(gradient_shading))
     p.num_clip_planes = len(clip_planes)
     for i, c in enumerate(clip_planes):
         p.clip_planes[i] = (C.c_float * 4)(*c)

@@ -136,7 +136,8 @@ def _anari_scene(a, wl, xfm=None, bvh_builder=None):
                        ("max_bounce", "maxBounce", A.INT32),
                        ("pixel_samples", "pixelSamples", A.INT32),
                        ("volume_sampling_rate", "volumeSamplingRate", A.FLOAT32),
-                       ("phase_g", "phaseG", A.FLOAT32)):
+                       ("phase_g", "phaseG", A.FLOAT32),
+                       ("gradient_shading", "gradientShading", A.BOOL)):
         if k in p:
             a.set(ren, n, kind, p[k])
     fr = a.new(A.FRAME)
@@ -169,6 +170,14 @@ CASES = {
     "structured pathtrace": lambda w: w.fbm_pathtrace(n=32, width=64, height=48, frames=1,
                                                       brick=8, phase_g=0.3),
     "structured raycast": lambda w: w.blobs_raycast(n=48, width=64, height=48),
+    "gradient-shaded raycast": lambda w: dataclasses.replace(
+        w.blobs_raycast(n=48, width=64, height=48),
+        lights=[dict(type="directional", direction=(0.5, -0.7, -0.4), color=(1, 1, 1),
+                     intensity=2.0)],
+        params=dict(volume_sampling_rate=2.0, gradient_shading=True)),
+    "gradient-shaded pathtrace": lambda w: dataclasses.replace(
+        w.fbm_pathtrace(n=32, width=64, height=48, frames=1, brick=8),
+        params=dict(max_bounce=4, ambient_samples=1, gradient_shading=True)),
     "nanovdb pathtrace": lambda w: w.cloud_nanovdb_pathtrace(n=64, width=64, height=36,
                                                             frames=1, brick=8),
     "tets pathtrace": lambda w: w.tets_pathtrace(n=8, width=48, height=32, frames=1, grid=8),
