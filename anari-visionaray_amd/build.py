@@ -31,7 +31,8 @@ HIP_FLAGS = [
 ]
 
 HIP_SOURCES = ["render_kernels.hip", "nrc_kernels.hip", "vrt_api.cpp", "nrc_api.cpp",
-               "nanovdb_field.cpp", "bvh_build.cpp", "bvh_gpu.hip", "unstructured_field.cpp"]
+               "nanovdb_field.cpp", "bvh_build.cpp", "bvh_gpu.hip", "unstructured_field.cpp",
+               "hdri_light.cpp"]
 
 
 def _run(cmd):

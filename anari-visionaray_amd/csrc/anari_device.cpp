@@ -224,6 +224,15 @@ struct Geometry : Object {
   }
 };
 
+// lights; an "hdri" light keeps its map and sampling tables on the device
+struct Light : Object {
+  uint64_t built = ~0ull;
+  vrt_hdri hdri = nullptr;
+  ~Light() override {
+    if (hdri) vrt_hdri_destroy(hdri);
+  }
+};
+
 struct Frame : Object {
   uint32_t w = 0, h = 0;
   ANARIDataType color_type = ANARI_UNKNOWN, depth_type = ANARI_UNKNOWN;
@@ -836,6 +845,37 @@ void add_group(Device *d, const Object *grp, const Affine &x, uint32_t inst_id, 
       x.vector(e2, L.edge2);
       L.type = VRT_LIGHT_QUAD;
       L.intensity = std::max(0.f, o->get_f("intensity", 1.f));
+    } else if (o->subtype == "hdri") {
+      // HDRI.cpp:102-162: "radiance" (2D float3 map), "up", "direction",
+      // "scale"; not moved by instance transforms (xfmLight)
+      Light *hl = static_cast<Light *>(o);
+      const Array *rad = array_param(o, "radiance");
+      if (!rad || rad->kind != ANARI_ARRAY2D || rad->elem != ANARI_FLOAT32_VEC3) {
+        d->report(o, ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_ARGUMENT,
+                  "missing required parameter 'radiance' on 'hdri' light");
+        continue;
+      }
+      sb.version = std::max(sb.version, rad->version);
+      const uint64_t ver = std::max(o->version, rad->version);
+      if (hl->built != ver) {
+        hl->built = ver;
+        if (hl->hdri) vrt_hdri_destroy(hl->hdri);
+        hl->hdri = nullptr;
+        if (vrt_hdri_create(d->ctx, reinterpret_cast<const float *>(rad->data()), int(rad->n[0]),
+                            int(rad->n[1]), &hl->hdri) != VRT_OK) {
+          d->vrt_error(o, "hdri light");
+          hl->hdri = nullptr;
+        }
+      }
+      if (!hl->hdri) continue;
+      L.type = VRT_LIGHT_HDRI;
+      L.hdri = hl->hdri;
+      float up[3] = {1, 0, 0}, dir[3] = {0, 0, 1};
+      o->get_v3("up", up);
+      o->get_v3("direction", dir);
+      std::memcpy(L.up, up, 12);
+      std::memcpy(L.direction, dir, 12);
+      L.scale = o->get_f("scale", 1.f);
     } else {
       d->report(o, ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_OPERATION,
                 "light subtype '%s' not supported: ignored", o->subtype.c_str());
@@ -1142,7 +1182,7 @@ ANARISurface anariNewSurface(ANARIDevice dh) VRT_NEW(s, Object, ANARI_SURFACE, "
 ANARISpatialField anariNewSpatialField(ANARIDevice dh, const char *t)
     VRT_NEW(f, Field, ANARI_SPATIAL_FIELD, t)
 ANARIVolume anariNewVolume(ANARIDevice dh, const char *t) VRT_NEW(v, Volume, ANARI_VOLUME, t)
-ANARILight anariNewLight(ANARIDevice dh, const char *t) VRT_NEW(l, Object, ANARI_LIGHT, t)
+ANARILight anariNewLight(ANARIDevice dh, const char *t) VRT_NEW(l, Light, ANARI_LIGHT, t)
 ANARIGroup anariNewGroup(ANARIDevice dh) VRT_NEW(g, Object, ANARI_GROUP, "")
 ANARIInstance anariNewInstance(ANARIDevice dh, const char *t) VRT_NEW(i, Object, ANARI_INSTANCE, t)
 ANARIWorld anariNewWorld(ANARIDevice dh) VRT_NEW(w, Object, ANARI_WORLD, "")

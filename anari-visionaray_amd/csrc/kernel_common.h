@@ -90,6 +90,16 @@ struct KLight {
   float a1, a2;  // quad: areas of its triangles {v1, e1, e1+e2}, {v1, e1+e2, e2}
 };
 
+// the scene's HDRI environment light (HDRI.cpp:102-162)
+struct KHdri {
+  const float4 *tex;   // w x h RGBA radiance, x fastest
+  const float *rows;   // normalised CDF over the rows' totals (h entries)
+  const float *cols;   // per row, the normalised CDF over its pixels (w entries each)
+  int w, h;
+  float scale;
+  float to_world[9], to_local[9];  // row-major
+};
+
 struct KSurface {
   const float4 *nodes;  // 4-wide BVH nodes, 8 float4 each
   const float4 *tris;   // {v1, prim id bits}, e1, e2 per triangle
@@ -106,6 +116,8 @@ struct KScene {
   KVolume vol[VRT_MAX_VOLUMES];
   KLight light[VRT_MAX_LIGHTS];
   KSurface surf[VRT_MAX_SURFACES];
+  int hdri_light;  // index of the HDRI light, -1 for none
+  KHdri hdri;
 };
 
 struct KCamera {

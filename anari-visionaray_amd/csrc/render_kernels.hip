@@ -930,6 +930,104 @@ __device__ inline void wave_add_counter(unsigned long long *ctr, uint32_t v) {
 // alpha < 0.99), and fetches past the end are discarded.
 constexpr int kRaySteps = 4;
 
+// ---------------------------------------------------------------------------
+// HDRI environment light (sampleCDF.h, DeviceCopyableObjects.h:2398-2418,
+// HDRI.cpp).  The mappings between directions and map coordinates keep the
+// reference's arithmetic: M_PI terms in double, rounded to float once.
+// ---------------------------------------------------------------------------
+constexpr double kPiD = 3.14159265358979323846;
+
+// upper_bound: the first entry greater than x (sampleCDF.h:14-31)
+__device__ inline uint32_t cdf_upper_bound(const float *d, uint32_t n, float x) {
+  uint32_t first = 0, count = n;
+  while (count > 0) {
+    const uint32_t step = count / 2, it = first + step;
+    if (!(x < d[it])) {
+      first = it + 1;
+      count -= step + 1;
+    } else {
+      count = step;
+    }
+  }
+  return first;
+}
+
+// sample_cdf (sampleCDF.h:40-54): the bin of x and its probability; a bin
+// past the end (a map or row summing to 0) has probability 0 here, the
+// reference leaves it unset
+__device__ inline uint32_t cdf_bin(const float *d, uint32_t n, float x, float &pdf) {
+  const uint32_t i = cdf_upper_bound(d, n, x);
+  pdf = i == 0 ? d[0] : (i < n ? d[i] - d[i - 1] : 0.f);
+  return i;
+}
+
+// sampleCDF (sampleCDF.h:88-98): the row from the row totals, then the
+// column from that row
+__device__ inline void hdri_cdf(const KHdri &H, float rx, float ry, uint32_t &x, uint32_t &y,
+                                float &pdfx, float &pdfy) {
+  y = min(cdf_bin(H.rows, uint32_t(H.h), ry, pdfy), uint32_t(H.h - 1));
+  x = cdf_bin(H.cols + size_t(y) * uint32_t(H.w), uint32_t(H.w), rx, pdfx);
+}
+
+// toPolar (sampleCDF.h:58-72)
+__device__ inline f3 hdri_to_polar(float u, float v) {
+  const float theta = float(2.0 * kPiD * double(u) + -kPiD / 2.0);
+  const float phi = float(kPiD * double(v));
+  const float sp = sinf(phi);
+  return mk(-(cosf(theta) * sp), cosf(phi), sinf(theta) * sp);
+}
+
+// toUV (sampleCDF.h:74-85)
+__device__ inline void hdri_to_uv(f3 n, float &u, float &v) {
+  const float a = atan2f(n.x, n.z);
+  u = float((double(a) + kPiD / 2.0) / (kPiD * 2.0) + kPiD * double(28.670f / 360.0f));
+  v = fminf(fmaxf(float(double(acosf(n.y)) / kPiD), .001f), .999f);
+}
+
+// intensity (DeviceCopyableObjects.h:2413-2417): the map at toUV of the
+// direction in the light's frame, bilinear with clamp addressing at texel
+// centres, times scale
+__device__ inline f3 hdri_radiance(const KHdri &H, f3 dir) {
+  float u, v;
+  hdri_to_uv(mulA(H.to_local, dir), u, v);
+  int i[2], j[2];
+  float wx, wy;
+  {
+    float c = fminf(fmaxf(u * float(H.w) - 0.5f, -1.f), float(H.w));
+    const float fl = floorf(c);
+    wx = c - fl;
+    i[0] = min(max(int(fl), 0), H.w - 1);
+    i[1] = min(max(int(fl) + 1, 0), H.w - 1);
+    c = fminf(fmaxf(v * float(H.h) - 0.5f, -1.f), float(H.h));
+    const float fy = floorf(c);
+    wy = c - fy;
+    j[0] = min(max(int(fy), 0), H.h - 1);
+    j[1] = min(max(int(fy) + 1, 0), H.h - 1);
+  }
+  const float4 t00 = H.tex[size_t(j[0]) * H.w + i[0]], t10 = H.tex[size_t(j[0]) * H.w + i[1]];
+  const float4 t01 = H.tex[size_t(j[1]) * H.w + i[0]], t11 = H.tex[size_t(j[1]) * H.w + i[1]];
+  auto lp = [](float a, float b, float t) { return a + t * (b - a); };
+  const f3 r = mk(lp(lp(t00.x, t10.x, wx), lp(t01.x, t11.x, wx), wy),
+                  lp(lp(t00.y, t10.y, wx), lp(t01.y, t11.y, wx), wy),
+                  lp(lp(t00.z, t10.z, wx), lp(t01.z, t11.z, wx), wy));
+  return r * H.scale;
+}
+
+// the light pdf of a direction that hit the map (Pathtrace_impl.cpp:160-172):
+// the map coordinates run through the CDF as if they were the random numbers
+__device__ inline float hdri_pdf(const KHdri &H, f3 raydir) {
+  const f3 d = mulA(H.to_local, raydir);
+  float u, v;
+  hdri_to_uv(d, u, v);
+  uint32_t x, y;
+  float pdfx, pdfy;
+  hdri_cdf(H, u, v, x, y, pdfx, pdfy);
+  const float sinT = sinf(acosf(fminf(fmaxf(d.y, -1.f), 1.f)));
+  if (sinT == 0.f) return 0.f;
+  const float pi = 3.14159265358979323846f;
+  return (pdfx * pdfy) * float(uint32_t(H.w) * uint32_t(H.h)) / (2.0f * pi * pi * sinT);
+}
+
 // sampleLight (renderer/common.h:1432-1466).  Directional: the stored
 // direction, pdf 1, distance FLT_MAX, no draws.  Quad (area_light<Quad>, not
 // vendored: parity unpinned): one draw picks a triangle
@@ -943,11 +1041,26 @@ struct LightSmp {
 };
 
 template <bool AREA>
-__device__ inline LightSmp sample_light(const KLight &L, f3 hitPos, uint32_t &rs) {
+__device__ inline LightSmp sample_light(const KScene &sc, const KLight &L, f3 hitPos,
+                                       uint32_t &rs) {
   LightSmp ls;
   ls.type = L.type;
   ls.intensity = L.intensity;
-  if (AREA && L.type == VRT_LIGHT_QUAD) {
+  if (AREA && L.type == VRT_LIGHT_HDRI) {
+    // asHDRI.sample: a cell by the CDFs (rx then ry, drawn left to right),
+    // the direction of its corner, pdf = pdfx pdfy w h / 4 pi; infinitely far
+    const KHdri &H = sc.hdri;
+    const float rx = rnd(rs), ry = rnd(rs);
+    uint32_t x, y;
+    float pdfx, pdfy;
+    hdri_cdf(H, rx, ry, x, y, pdfx, pdfy);
+    const float invjac = float(uint32_t(H.w) * uint32_t(H.h)) / float(4.0 * kPiD);
+    ls.dir = mulA(H.to_world, hdri_to_polar(float(x) / float(H.w), float(y) / float(H.h)));
+    ls.intensity = hdri_radiance(H, ls.dir);
+    ls.dist = FLT_MAX;
+    ls.pdf = pdfx * pdfy * invjac;
+    ls.dist2 = 1.f;
+  } else if (AREA && L.type == VRT_LIGHT_QUAD) {
     const float r = rnd(rs);
     const f3 d = L.e1 + L.e2;
     const bool first = L.a1 / (L.a1 + L.a2) < r;
@@ -972,6 +1085,39 @@ __device__ inline LightSmp sample_light(const KLight &L, f3 hitPos, uint32_t &rs
 
 __device__ inline float safe_rcp(float f) { return f > 0.f ? 1.f / f : 0.f; }
 
+// The ray caster's backdrop (Raycast_impl.cpp:158-173): the closest visible
+// light the primary ray meets -- a quad, or the environment at FLT_MAX -- as
+// float4(intensity, 1), else the background colour
+__device__ inline float4 light_backdrop(const KScene &sc, const KRay &ray, float4 bg) {
+  bool lh = false;
+  float lt = FLT_MAX;
+  int lid = -1;
+  for (int i = 0; i < sc.num_lights; ++i) {
+    const KLight &L = sc.light[i];
+    if (!L.visible) continue;
+    if (L.type == VRT_LIGHT_HDRI) {
+      if (!lh) {
+        lh = true;
+        lid = i;
+      }
+      continue;
+    }
+    if (L.type != VRT_LIGHT_QUAD) continue;
+    float t;
+    if (quad_hit(L, ray, t) && t < lt) {
+      lh = true;
+      lt = t;
+      lid = i;
+    }
+  }
+  if (!lh) return bg;
+  f3 I = mk(0.f);
+  for (int i = 0; i < sc.num_lights; ++i)
+    if (i == lid) I = sc.light[i].type == VRT_LIGHT_HDRI ? hdri_radiance(sc.hdri, ray.dir)
+                                                        : sc.light[i].intensity;
+  return make_float4(I.x, I.y, I.z, 1.f);
+}
+
 // Gradient shading of one ray-caster sample (VolumeIntegration.h:420-459):
 // matte (albedo / pi, common.h:825-828, 1033-1048) under every light, along
 // the faceforwarded gradient normal at the voxel-space sample point P, plus
@@ -986,7 +1132,7 @@ __device__ inline f3 gradient_shade(const KScene &sc, const KVolume &v, const KP
   if (length(gn) > 1e-10f) {
     const float inv_pi = 0.318309886183790671538f;
     for (int i = 0; i < sc.num_lights; ++i) {
-      const LightSmp ls = sample_light<true>(sc.light[i], Pv, rs);
+      const LightSmp ls = sample_light<true>(sc, sc.light[i], Pv, rs);
       const f3 lightDir = normalize(ls.dir);
       const f3 lightIntensity = ls.intensity * safe_rcp(ls.dist2);
       const float NdotL = fmaxf(0.f, dot(gn, lightDir));
@@ -1043,9 +1189,10 @@ __global__ void __launch_bounds__(256)
   if (active) {
     float4 result = P.bg;
     int hit_vol = -1;
-    if (sc.num_volumes > 0) {
+    if (sc.num_volumes > 0 || sc.num_surfaces > 0) {  // an empty scene shows the background
       KRay ray = primary_ray(cam, float(x), float(y), float(F.width), float(F.height));
       clip_ray(ray, P);
+      result = light_backdrop(sc, ray, P.bg);
       // closest volume box (intersectVolumeBounds, DeviceCopyableObjects.h:1841-1859;
       // a box hit counts when tfar >= tmin, :695-700)
       int best = -1;
@@ -1300,7 +1447,17 @@ __device__ inline PassHit intersect_all(const KScene &sc, const KRay &ray, bool 
   int lid = -1;
   for (int i = 0; AREA && i < sc.num_lights; ++i) {
     const KLight &L = sc.light[i];
-    if (L.type != VRT_LIGHT_QUAD || (bounceID == 0 && !L.visible)) continue;
+    if (bounceID == 0 && !L.visible) continue;
+    if (L.type == VRT_LIGHT_HDRI) {
+      // every ray meets the environment, at FLT_MAX (common.h:1574-1588)
+      if (!lh) {
+        lh = true;
+        lt = FLT_MAX;
+        lid = i;
+      }
+      continue;
+    }
+    if (L.type != VRT_LIGHT_QUAD) continue;
     float t;
     if (quad_hit(L, ray, t) && t < lt) {
       lh = true;
@@ -1424,6 +1581,17 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
       KLight L = sc.light[0];
       for (int i = 1; i < nl; ++i)
         if (i == h.id) L = sc.light[i];
+      if (L.type == VRT_LIGHT_HDRI) {
+        // the map in the ray's direction, MIS-weighted after the first
+        // bounce (Pathtrace_impl.cpp:144-182)
+        f3 shaded = hdri_radiance(sc.hdri, ray.dir);
+        if (st.bounceID > 0)
+          shaded = shaded * power_heuristic(st.bs_pdf, hdri_pdf(sc.hdri, ray.dir) / float(nl));
+        st.intensity = st.intensity + st.throughput * ((shaded * 1.f) * 1.f);
+        st.intensity = st.intensity + st.throughput * (mk(0.f) * P.ambient_color *
+                                                       P.ambient_radiance * 1.f);
+        return true;
+      }
       f3 shaded = L.intensity;
       if (st.bounceID > 0) {
         const f3 hitPos = ray.ori + ray.dir * h.t;
@@ -1504,7 +1672,7 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
       KLight L = sc.light[0];
       for (int i = 1; i < nl; ++i)
         if (i == which) L = sc.light[i];
-      ls = sample_light<AREA>(L, hitPos, st.srng);
+      ls = sample_light<AREA>(sc, L, hitPos, st.srng);
     }
     const f3 lightIntensity = ls.intensity * safe_rcp(ls.dist2);
     if (h.kind == kVolume) {
@@ -1552,8 +1720,9 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
       st.albedo = f * (cosT * safe_rcp(st.bs_pdf));
     }
     // misWeightNEE (Pathtrace_impl.cpp:370-380)
-    st.misw = nl > 0 ? (ls.type == VRT_LIGHT_QUAD ? power_heuristic(ls.pdf / float(nl), st.bs_pdf)
-                                                 : 1.f)
+    st.misw = nl > 0 ? ((ls.type == VRT_LIGHT_QUAD || ls.type == VRT_LIGHT_HDRI)
+                            ? power_heuristic(ls.pdf / float(nl), st.bs_pdf)
+                            : 1.f)
                      : 0.f;
     st.base = color;
     st.aoSamples = 0;
@@ -2138,7 +2307,7 @@ hipError_t launch_raycast(const KScene &sc, const KCamera &cam, const KParams &P
 static bool scene_has_area(const KScene &sc) {
   if (sc.num_surfaces > 0) return true;
   for (int i = 0; i < sc.num_lights; ++i)
-    if (sc.light[i].type == VRT_LIGHT_QUAD) return true;
+    if (sc.light[i].type == VRT_LIGHT_QUAD || sc.light[i].type == VRT_LIGHT_HDRI) return true;
   return false;
 }
 

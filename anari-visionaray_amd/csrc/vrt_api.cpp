@@ -16,6 +16,7 @@
 #include "nrc_internal.h"
 #include "render_kernels.h"
 #include "unstructured_internal.h"
+#include "hdri_internal.h"
 #include "vrt.h"
 
 namespace vrt {
@@ -177,6 +178,7 @@ int make_scene(const vrt_scene *s, KScene &k) {
   k = KScene{};
   k.num_volumes = s->num_volumes;
   k.num_lights = s->num_lights;
+  k.hdri_light = -1;
   int off = 0;
   for (int i = 0; i < s->num_volumes; ++i) {
     int r = make_volume(s->volumes[i], off, k.vol[i]);
@@ -223,6 +225,19 @@ int make_scene(const vrt_scene *s, KScene &k) {
       o.a1 = 0.5f * length(cross(o.e1, d));
       o.a2 = 0.5f * length(cross(d, o.e2));
       o.intensity = mk(L.color[0], L.color[1], L.color[2]) * std::max(0.f, L.intensity);
+    } else if (L.type == VRT_LIGHT_HDRI) {
+      if (!L.hdri) return fail(VRT_ERROR_INVALID_ARGUMENT, "hdri light without a map");
+      if (k.hdri_light >= 0) return fail(VRT_ERROR_UNSUPPORTED, "more than one hdri light");
+      k.hdri_light = i;
+      KHdri &H = k.hdri;
+      H.tex = L.hdri->tex;
+      H.rows = L.hdri->rows;
+      H.cols = L.hdri->cols;
+      H.w = L.hdri->w;
+      H.h = L.hdri->h;
+      H.scale = L.scale;
+      if (!hdri_frame(L.up, L.direction, H.to_world, H.to_local))
+        return fail(VRT_ERROR_INVALID_ARGUMENT, "hdri up and direction span no frame");
     } else {
       return fail(VRT_ERROR_UNSUPPORTED, "light type %d not supported by this entry point", L.type);
     }

@@ -20,6 +20,7 @@ FIELD_NANOVDB = 1
 FIELD_UNSTRUCTURED = 2
 LIGHT_DIRECTIONAL = 0
 LIGHT_QUAD = 1
+LIGHT_HDRI = 2
 MAX_VOLUMES = 4
 MAX_LIGHTS = 8
 TYPE_FLOAT32 = 1068
@@ -54,6 +55,7 @@ class Light(C.Structure):
     _fields_ = [
         ("type", C.c_int), ("visible", C.c_int), ("direction", f3), ("position", f3),
         ("edge1", f3), ("edge2", f3), ("color", f3), ("intensity", C.c_float),
+        ("hdri", C.c_void_p), ("up", f3), ("scale", C.c_float),
     ]
 
 
@@ -187,6 +189,8 @@ def lib():
         L.vrt_bvh_build_gpu.argtypes = [vp, vp, C.c_int64, vp, C.c_int64, C.POINTER(C.c_void_p)]
         L.vrt_bvh_get_info.argtypes = [vp, C.c_void_p]
         L.vrt_bvh_destroy.argtypes = [vp]
+        L.vrt_hdri_create.argtypes = [vp, vp, ip, ip, C.POINTER(C.c_void_p)]
+        L.vrt_hdri_destroy.argtypes = [vp]
         L.vrt_delta_track.argtypes = [vp, C.POINTER(Scene), ip, vp, vp, vp]
         L.vrt_raycast.argtypes = [vp, C.POINTER(Scene), C.POINTER(Camera), C.POINTER(Params),
                                   C.POINTER(Frame), ip]
@@ -640,6 +644,43 @@ def quad_light(position, edge1, edge2, intensity=1.0, color=(1, 1, 1), visible=T
     L.edge2 = f3(*edge2)
     L.color = f3(*color)
     L.intensity = float(intensity)
+    return L
+
+
+class DeviceHdri:
+    """An HDRI environment map (h, w, 3) float32 on the GPU with its
+    sampling CDFs (vrt_hdri_create)."""
+
+    def __init__(self, ctx, radiance):
+        rgb = np.ascontiguousarray(radiance, np.float32)
+        h, w = rgb.shape[:2]
+        out = C.c_void_p()
+        check(lib().vrt_hdri_create(ctx.handle, rgb.ctypes.data, w, h, C.byref(out)))
+        self.handle = out.value
+        self.width, self.height = w, h
+
+    def close(self):
+        if self.handle:
+            lib().vrt_hdri_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def hdri_light(hdri, up=(1, 0, 0), direction=(0, 0, 1), scale=1.0, visible=True):
+    """ANARI "hdri" light (HDRI.cpp:102-108 defaults)."""
+    L = Light()
+    L.type = LIGHT_HDRI
+    L.visible = int(visible)
+    L.hdri = hdri.handle
+    L.up = f3(*up)
+    L.direction = f3(*direction)
+    L.scale = float(scale)
+    L.color = f3(1, 1, 1)
     return L
 
 

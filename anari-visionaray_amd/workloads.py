@@ -242,7 +242,11 @@ def volume_bounds(wl):
 # ---------------------------------------------------------------------------
 # GPU side
 # ---------------------------------------------------------------------------
-def light_desc(L):
+def light_desc(L, hdri=None):
+    """`hdri`: the DeviceHdri of an "hdri" light (its "radiance" map)."""
+    if L.get("type") == "hdri":
+        return _vrt.hdri_light(hdri, L.get("up", (1, 0, 0)), L.get("direction", (0, 0, 1)),
+                               L.get("scale", 1.0), L.get("visible", True))
     if L.get("type", "directional") == "quad":
         return _vrt.quad_light(L["position"], L["edge1"], L["edge2"], L["intensity"], L["color"],
                                L.get("visible", True))
@@ -271,7 +275,9 @@ class GpuScene:
             volume = _vrt.DeviceVolume(ctx, wl.voxels, wl.origin, wl.spacing, wl.tf,
                                        wl.value_range, wl.unit_distance, wl.grid_dims)
         self.volume = volume
-        lights = [light_desc(L) for L in wl.lights]
+        self.hdris = [_vrt.DeviceHdri(ctx, L["radiance"]) if L.get("type") == "hdri" else None
+                      for L in wl.lights]
+        lights = [light_desc(L, h) for L, h in zip(wl.lights, self.hdris)]
         self.meshes = [(_vrt.DeviceMesh(ctx, m["vertices"], m.get("indices"), bvh_builder),
                         m["color"]) for m in (wl.meshes or [])]
         self.scene = _vrt.make_scene([volume] if volume is not None else [], lights, self.meshes)

@@ -26,7 +26,7 @@
  * Supported subtypes: spatial fields "structuredRegular" (float32 data),
  * "nanovdb" (a float grid buffer), "unstructured" (tetrahedra); volume
  * "transferFunction1D"; geometry "triangle"; material "matte"; lights
- * "directional", "quad"; camera "perspective"; renderers "raycast",
+ * "directional", "quad", "hdri"; camera "perspective"; renderers "raycast",
  * "pathtrace"; instance "transform".  Unknown subtypes yield an object that
  * is reported invalid when used, as in the reference.
  *

@@ -96,16 +96,26 @@ typedef struct vrt_volume {
 
 #define VRT_LIGHT_DIRECTIONAL 0
 #define VRT_LIGHT_QUAD 1
+#define VRT_LIGHT_HDRI 2
+
+/* An HDRI environment map on the device (vrt_hdri_create). */
+typedef struct vrt_hdri_s *vrt_hdri;
 
 typedef struct vrt_light {
   int type;
   int visible;
-  float direction[3]; /* directional: ANARI "direction" (light travel) */
+  float direction[3]; /* directional: ANARI "direction" (light travel);
+                         hdri: "direction" (HDRI.cpp:106, default (0,0,1)) */
   float position[3];  /* quad */
   float edge1[3];
   float edge2[3];
   float color[3];
   float intensity;    /* directional: irradiance; quad: intensity */
+  /* hdri (HDRI.cpp:102-162; at most one per scene): the map, "up"
+     (default (1,0,0)) and "scale" (default 1) */
+  vrt_hdri hdri;
+  float up[3];
+  float scale;
 } vrt_light;
 
 #define VRT_MAX_VOLUMES 4
@@ -307,6 +317,17 @@ VRT_API int vrt_bvh_build_gpu(vrt_context ctx, const float *vertices, int64_t nu
                               const uint32_t *indices, int64_t num_triangles, vrt_bvh *out);
 VRT_API int vrt_bvh_get_info(vrt_bvh bvh, vrt_bvh_info *info);
 VRT_API int vrt_bvh_destroy(vrt_bvh bvh);
+
+/* HDRI environment light (HDRI::finalize, HDRI.cpp:31-162): `radiance` is
+   width x height RGB float32, x fastest.  Builds the per-row and last-column
+   CDFs of the max-component luminance (makeCDF) on the host and uploads them
+   with an RGBA copy of the map.  Light samples pick a cell by the CDFs and
+   take the direction of its corner (sampleCDF.h, DeviceCopyableObjects.h:
+   2398-2418); lookups are bilinear with clamp addressing. */
+VRT_API int vrt_hdri_create(vrt_context ctx, const float *radiance, int width, int height,
+                            vrt_hdri *out);
+VRT_API int vrt_hdri_destroy(vrt_hdri hdri);
+
 
 /* Unstructured (tetrahedral) field: UnstructuredField::commitParameters /
  * finalize (UnstructuredField.cpp:17-198).  The host arrays are the field's

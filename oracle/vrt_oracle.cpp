@@ -977,15 +977,156 @@ template <class R> struct QuadLight {
   }
 };
 
+// HDRI environment light (HDRI.cpp:31-162, sampleCDF.h,
+// DeviceCopyableObjects.h:2398-2418).  The map coordinates and lookups are
+// float, with the M_PI terms of toPolar / toUV in double as written there.
+struct HdriLight {
+  int w = 0, h = 0;
+  std::vector<float> rgb, rows, cols;
+  float scale = 1.f;
+  float W[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, L[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  bool ok = false;
+
+  explicit HdriLight(const orc_light &l) {
+    if (l.type != ORC_LIGHT_HDRI || !l.hdri_rgb || l.hdri_w < 1 || l.hdri_h < 1) return;
+    w = l.hdri_w;
+    h = l.hdri_h;
+    scale = l.scale;
+    rgb.assign(l.hdri_rgb, l.hdri_rgb + size_t(w) * h * 3);
+    // makeCDF (:31-66): per row the running sum of max(r, g, b), divided by
+    // the row total unless it is 0; the totals' running sum likewise
+    cols.resize(size_t(w) * h);
+    rows.resize(h);
+    for (int y = 0; y < h; ++y) {
+      float *r = cols.data() + size_t(y) * w;
+      for (int x = 0; x < w; ++x) {
+        const float *c = rgb.data() + 3 * (size_t(y) * w + x);
+        r[x] = std::max(std::max(c[0], c[1]), c[2]);
+        if (x > 0) r[x] = r[x - 1] + r[x];
+      }
+      rows[y] = r[w - 1];
+      const float last = r[w - 1];
+      if (last != 0.f)
+        for (int x = 0; x < w; ++x) r[x] /= last;
+    }
+    for (int y = 1; y < h; ++y) rows[y] = rows[y - 1] + rows[y];
+    const float last = rows[h - 1];
+    if (last != 0.f)
+      for (int y = 0; y < h; ++y) rows[y] /= last;
+    // the frame (:150-155): col1 = -up, col0 = col1 x direction,
+    // col2 = col0 x col1, normalised; the inverse by the adjugate
+    auto nrm = [](V3<float> a) { return a * (1.f / length(a)); };
+    const V3<float> c1 = -nrm(from<float>(l.up));
+    const V3<float> c0 = nrm(cross(c1, from<float>(l.dir)));
+    const V3<float> c2 = nrm(cross(c0, c1));
+    const V3<float> cs[3] = {c0, c1, c2};
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) W[3 * r + c] = cs[c][r];
+    const float det = W[0] * (W[4] * W[8] - W[5] * W[7]) - W[1] * (W[3] * W[8] - W[5] * W[6]) +
+                      W[2] * (W[3] * W[7] - W[4] * W[6]);
+    if (!(det != 0.f) || !std::isfinite(det)) return;
+    const float inv = 1.f / det;
+    L[0] = (W[4] * W[8] - W[5] * W[7]) * inv;
+    L[1] = (W[2] * W[7] - W[1] * W[8]) * inv;
+    L[2] = (W[1] * W[5] - W[2] * W[4]) * inv;
+    L[3] = (W[5] * W[6] - W[3] * W[8]) * inv;
+    L[4] = (W[0] * W[8] - W[2] * W[6]) * inv;
+    L[5] = (W[2] * W[3] - W[0] * W[5]) * inv;
+    L[6] = (W[3] * W[7] - W[4] * W[6]) * inv;
+    L[7] = (W[1] * W[6] - W[0] * W[7]) * inv;
+    L[8] = (W[0] * W[4] - W[1] * W[3]) * inv;
+    ok = true;
+  }
+
+  // mat3 * vec3 = col0 x + col1 y + col2 z
+  static V3<float> mul(const float *M, V3<float> d) {
+    return {M[0] * d.x + M[1] * d.y + M[2] * d.z, M[3] * d.x + M[4] * d.y + M[5] * d.z,
+            M[6] * d.x + M[7] * d.y + M[8] * d.z};
+  }
+  // sample_cdf (:40-54): std::upper_bound, the bin's probability (0 past
+  // the end, which the reference leaves unset)
+  static unsigned bin(const float *d, unsigned n, float x, float &pdf) {
+    const unsigned i = unsigned(std::upper_bound(d, d + n, x) - d);
+    pdf = i == 0 ? d[0] : (i < n ? d[i] - d[i - 1] : 0.f);
+    return i;
+  }
+  // sampleCDF (:88-98)
+  void cdf(float rx, float ry, unsigned &x, unsigned &y, float &pdfx, float &pdfy) const {
+    y = std::min(bin(rows.data(), unsigned(h), ry, pdfy), unsigned(h - 1));
+    x = bin(cols.data() + size_t(y) * w, unsigned(w), rx, pdfx);
+  }
+  // toPolar (:58-72), toUV (:74-85)
+  static V3<float> to_polar(float u, float v) {
+    const float theta = float(2.0 * M_PI * double(u) + -M_PI / 2.0);
+    const float phi = float(M_PI * double(v));
+    const float sp = std::sin(phi);
+    return {-(std::cos(theta) * sp), std::cos(phi), std::sin(theta) * sp};
+  }
+  static void to_uv(V3<float> n, float &u, float &v) {
+    const float a = std::atan2(n.x, n.z);
+    u = float((double(a) + M_PI / 2.0) / (M_PI * 2.0) + M_PI * double(28.670f / 360.0f));
+    // fmin / fmax: a NaN (|n.y| rounded past 1) clamps to .001, as on the GPU
+    v = std::fmin(std::fmax(float(double(std::acos(n.y)) / M_PI), .001f), .999f);
+  }
+  // intensity (:2413-2417): bilinear, clamp, texel centres (parity
+  // unpinned: visionaray's tex2D), times scale
+  V3<float> radiance(V3<float> dir) const {
+    float u, v;
+    to_uv(mul(L, dir), u, v);
+    auto axis = [](float c, int n, int &i0, int &i1) {
+      c = std::min(std::max(c, -1.f), float(n));
+      const float fl = std::floor(c);
+      i0 = std::min(std::max(int(fl), 0), n - 1);
+      i1 = std::min(std::max(int(fl) + 1, 0), n - 1);
+      return c - fl;
+    };
+    int i0, i1, j0, j1;
+    const float wx = axis(u * float(w) - 0.5f, w, i0, i1);
+    const float wy = axis(v * float(h) - 0.5f, h, j0, j1);
+    auto t = [&](int i, int j, int k) { return rgb[3 * (size_t(j) * w + i) + k]; };
+    auto lp = [](float a, float b, float s) { return a + s * (b - a); };
+    V3<float> r;
+    for (int k = 0; k < 3; ++k)
+      r[k] = lp(lp(t(i0, j0, k), t(i1, j0, k), wx), lp(t(i0, j1, k), t(i1, j1, k), wx), wy);
+    return r * scale;
+  }
+  // asHDRI.sample: rx then ry (drawn left to right), the cell corner's
+  // direction, pdf = pdfx pdfy w h / 4 pi
+  void sample(Random &rnd, V3<float> &dir, float &pdf) const {
+    const float rx = rnd(), ry = rnd();
+    unsigned x, y;
+    float pdfx, pdfy;
+    cdf(rx, ry, x, y, pdfx, pdfy);
+    const float invjac = float(unsigned(w) * unsigned(h)) / float(4.0 * M_PI);
+    dir = mul(W, to_polar(float(x) / float(w), float(y) / float(h)));
+    pdf = pdfx * pdfy * invjac;
+  }
+  // the light pdf of a direction that met the map (Pathtrace_impl.cpp:160-172)
+  float pdf(V3<float> raydir) const {
+    const V3<float> d = mul(L, raydir);
+    float u, v;
+    to_uv(d, u, v);
+    unsigned x, y;
+    float pdfx, pdfy;
+    cdf(u, v, x, y, pdfx, pdfy);
+    const float sinT = std::sin(std::acos(std::min(std::max(d.y, -1.f), 1.f)));
+    if (sinT == 0.f) return 0.f;
+    const float pi = float(M_PI);
+    return (pdfx * pdfy) * float(unsigned(w) * unsigned(h)) / (2.0f * pi * pi * sinT);
+  }
+};
+
 template <class R> struct Scene {
   std::vector<Volume<R>> vols;
   std::vector<Mesh<R>> meshes;
   std::vector<QuadLight<R>> quads;  // per light (unused entries for other types)
+  std::vector<HdriLight> hdris;     // per light (empty for other types)
   const orc_scene *s;
   explicit Scene(const orc_scene *sc) : s(sc) {
     for (int i = 0; i < sc->num_volumes; ++i) vols.emplace_back(&sc->volumes[i]);
     for (int i = 0; i < sc->num_meshes; ++i) meshes.emplace_back(&sc->meshes[i]);
     for (int i = 0; i < sc->num_lights; ++i) quads.emplace_back(sc->lights[i]);
+    for (int i = 0; i < sc->num_lights; ++i) hdris.emplace_back(sc->lights[i]);
   }
   bool empty() const { return s->num_volumes == 0 && s->num_meshes == 0; }
 };
@@ -1001,8 +1142,22 @@ template <class R> struct LightSample {
 // color*irradiance, distance FLT_MAX, and draws no random numbers
 // (parity unpinned).
 template <class R>
-LightSample<R> sample_light(const orc_light &L, const QuadLight<R> &Q, V3<R> hitPos, Random &rnd) {
+LightSample<R> sample_light(const orc_light &L, const QuadLight<R> &Q, const HdriLight &H,
+                            V3<R> hitPos, Random &rnd) {
   LightSample<R> ls;
+  if (L.type == ORC_LIGHT_HDRI) {
+    // infinitely far, not attenuated (common.h:1448-1461)
+    V3<float> d;
+    float pdf;
+    H.sample(rnd, d, pdf);
+    ls.dir = V3<R>(R(d.x), R(d.y), R(d.z));
+    ls.dist = R(std::numeric_limits<float>::max());
+    const V3<float> I = H.radiance(d);
+    ls.intensity = V3<R>(R(I.x), R(I.y), R(I.z));
+    ls.pdf = R(pdf);
+    ls.dist2 = R(1);
+    return ls;
+  }
   if (L.type == ORC_LIGHT_QUAD) {
     // area_light<float, Quad>::sample (visionaray, not vendored: parity
     // unpinned): sample_surface(Quad) picks a triangle with one draw
@@ -1071,6 +1226,16 @@ HitRec<R> intersect_all(const Scene<R> &sc, const Ray<R> &ray, Random &rnd, Coun
   for (int i = 0; i < sc.s->num_lights; ++i) {
     const orc_light &L = sc.s->lights[i];
     if (bounceID == 0 && !L.visible) continue;
+    if (L.type == ORC_LIGHT_HDRI) {
+      // every ray meets the environment, at FLT_MAX (:1574-1588)
+      if (!hr.light_hit) {
+        hr.light_hit = true;
+        hr.light_t = std::numeric_limits<R>::max();
+        hr.light_visible = L.visible != 0;
+        hr.light_id = i;
+      }
+      continue;
+    }
     if (L.type != ORC_LIGHT_QUAD) continue;
     const TriHit<R> h = sc.quads[i].intersect(ray);
     if (h.hit && h.t < hr.light_t) {
@@ -1166,6 +1331,20 @@ void shade(Random &srnd, const Ray<R> &ray, RayType rayType, const Scene<R> &sc,
       return;
     }
     const int numLights = sc.s->num_lights;
+    if (hitRec.lightHit && sc.s->lights[hitRec.light_id].type == ORC_LIGHT_HDRI) {
+      // Pathtrace_impl.cpp:144-182: the map in the ray's direction,
+      // MIS-weighted after the first bounce
+      const HdriLight &H = sc.hdris[hitRec.light_id];
+      const V3<float> d(float(ray.dir.x), float(ray.dir.y), float(ray.dir.z));
+      const V3<float> I = H.radiance(d);
+      st.shadedColor = V3<R>(R(I.x), R(I.y), R(I.z));
+      st.misWeightNEE = R(1);
+      if (bounceID > 0)
+        st.shadedColor =
+            st.shadedColor * power_heuristic(st.bs_pdf, R(H.pdf(d)) / R(numLights));
+      st.nextType = Miss;
+      return;
+    }
     if (hitRec.lightHit) {
       // Pathtrace_impl.cpp:139-183: the quad's intensity, MIS-weighted
       // against the BSDF sample after the first bounce
@@ -1252,7 +1431,8 @@ void shade(Random &srnd, const Ray<R> &ray, RayType rayType, const Scene<R> &sc,
       int which = int(srnd() * float(numLights));
       if (which == numLights) which = 0;
       lightID = which;
-      st.ls = sample_light<R>(sc.s->lights[lightID], sc.quads[lightID], hitPos, srnd);
+      st.ls = sample_light<R>(sc.s->lights[lightID], sc.quads[lightID], sc.hdris[lightID], hitPos,
+                              srnd);
     }
     auto safe_rcp = [](R f) { return f > R(0) ? R(1) / f : R(0); };
     V3<R> lightIntensity = st.ls.intensity * safe_rcp(st.ls.dist2);
@@ -1310,7 +1490,8 @@ void shade(Random &srnd, const Ray<R> &ray, RayType rayType, const Scene<R> &sc,
     if (numLights > 0) {
       // MIS against the BSDF sample for area lights, 1 for delta lights
       // (:370-380)
-      if (sc.s->lights[lightID].type == ORC_LIGHT_QUAD)
+      if (sc.s->lights[lightID].type == ORC_LIGHT_QUAD ||
+          sc.s->lights[lightID].type == ORC_LIGHT_HDRI)
         st.misWeightNEE = power_heuristic(st.ls.pdf / R(numLights), st.bs_pdf);
       else
         st.misWeightNEE = R(1);
@@ -1455,7 +1636,7 @@ V3<R> gradient_shade(const Scene<R> &sc, const Volume<R> &vol, const orc_params 
   if (length(gn) > R(1e-10)) {
     const R inv_pi = R(0.318309886183790671538);
     for (int i = 0; i < sc.s->num_lights; ++i) {
-      const LightSample<R> ls = sample_light<R>(sc.s->lights[i], sc.quads[i], Pv, rs);
+      const LightSample<R> ls = sample_light<R>(sc.s->lights[i], sc.quads[i], sc.hdris[i], Pv, rs);
       const V3<R> lightDir = normalize(ls.dir);
       const V3<R> lightIntensity = ls.intensity * safe_rcp(ls.dist2);
       const R NdotL = std::max(R(0), dot(gn, lightDir));
@@ -1472,8 +1653,38 @@ V4<R> raycast_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params *
   if (sc.empty()) return bg;
   Ray<R> ray = cam.primary(R(x), R(y), R(W), R(H));
   clip_ray(ray, P);  // Raycast_impl.cpp:288
-  // over(float4(0), bg) = bg (no surfaces), Raycast_impl.cpp:170-173
+  // over(float4(0), backdrop) (no surfaces), Raycast_impl.cpp:158-173: the
+  // closest visible light the ray meets -- a quad, or the environment at
+  // FLT_MAX -- as float4(intensity, 1), else the background colour
   V4<R> result = bg;
+  {
+    int lid = -1;
+    R lt = std::numeric_limits<R>::max();
+    for (int i = 0; i < sc.s->num_lights; ++i) {
+      const orc_light &L = sc.s->lights[i];
+      if (!L.visible) continue;
+      if (L.type == ORC_LIGHT_HDRI) {
+        if (lid < 0) lid = i;
+        continue;
+      }
+      if (L.type != ORC_LIGHT_QUAD) continue;
+      const TriHit<R> h = sc.quads[i].intersect(ray);
+      if (h.hit && h.t < lt) {
+        lt = h.t;
+        lid = i;
+      }
+    }
+    if (lid >= 0) {
+      if (sc.s->lights[lid].type == ORC_LIGHT_HDRI) {
+        const V3<float> I = sc.hdris[lid].radiance(
+            V3<float>(float(ray.dir.x), float(ray.dir.y), float(ray.dir.z)));
+        result = V4<R>{R(I.x), R(I.y), R(I.z), R(1)};
+      } else {
+        const V3<R> I = sc.quads[lid].intensity;
+        result = V4<R>{I.x, I.y, I.z, R(1)};
+      }
+    }
+  }
   // intersectVolumeBounds: closest volume box (DeviceCopyableObjects.h:695-709)
   int best = -1;
   R bestT = std::numeric_limits<R>::max();
@@ -2138,13 +2349,51 @@ void orc_hg_sample(float g, const float d[3], int n, uint32_t seed, float *out) 
 
 double orc_hg_4pi(double g, double cos_theta) { return hg_4pi(g, cos_theta); }
 
+int orc_hdri_cdf(const orc_light *L, float *rows, float *cols) {
+  orc::HdriLight H(*L);
+  if (!H.ok) return -1;
+  std::copy(H.rows.begin(), H.rows.end(), rows);
+  std::copy(H.cols.begin(), H.cols.end(), cols);
+  return 0;
+}
+
+void orc_hdri_sample(const orc_light *L, int n, uint32_t seed, float *dirs, float *pdfs,
+                     float *rgb) {
+  orc::HdriLight H(*L);
+  for (int i = 0; i < n; ++i) {
+    orc::Random rnd(seed, uint32_t(i));
+    orc::V3<float> d;
+    float pdf;
+    H.sample(rnd, d, pdf);
+    const orc::V3<float> I = H.radiance(d);
+    for (int k = 0; k < 3; ++k) {
+      dirs[3 * i + k] = d[k];
+      rgb[3 * i + k] = I[k];
+    }
+    pdfs[i] = pdf;
+  }
+}
+
+void orc_hdri_eval(const orc_light *L, int n, const float *dirs, float *rgb, float *pdfs,
+                   float *uv) {
+  orc::HdriLight H(*L);
+  for (int i = 0; i < n; ++i) {
+    const orc::V3<float> d(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]);
+    const orc::V3<float> I = H.radiance(d);
+    for (int k = 0; k < 3; ++k) rgb[3 * i + k] = I[k];
+    pdfs[i] = H.pdf(d);
+    orc::HdriLight::to_uv(orc::HdriLight::mul(H.L, d), uv[2 * i], uv[2 * i + 1]);
+  }
+}
+
 void orc_quad_light_sample(const orc_light *L, const float ref[3], int n, uint32_t seed,
                            float *pos_out, float *pdf_out) {
   orc::QuadLight<float> Q(*L);
+  orc::HdriLight H(*L);
   const orc::V3<float> p(ref[0], ref[1], ref[2]);
   for (int i = 0; i < n; ++i) {
     orc::Random rnd(seed, uint32_t(i));
-    const orc::LightSample<float> ls = orc::sample_light<float>(*L, Q, p, rnd);
+    const orc::LightSample<float> ls = orc::sample_light<float>(*L, Q, H, p, rnd);
     const orc::V3<float> pos = p + ls.dir;
     pos_out[3 * i] = pos.x;
     pos_out[3 * i + 1] = pos.y;

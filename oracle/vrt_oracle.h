@@ -23,6 +23,7 @@ extern "C" {
 
 #define ORC_LIGHT_DIRECTIONAL 0
 #define ORC_LIGHT_QUAD 1
+#define ORC_LIGHT_HDRI 2
 
 typedef struct orc_light {
   int type;      /* ORC_LIGHT_* */
@@ -33,6 +34,12 @@ typedef struct orc_light {
   float e2[3];   /* quad: "edge2" */
   float color[3];
   float intensity; /* directional: "irradiance", quad: "intensity" */
+  /* hdri (HDRI.cpp:102-162; one per scene): RGB float map, x fastest,
+     "up", "scale"; `dir` holds its "direction" */
+  const float *hdri_rgb;
+  int hdri_w, hdri_h;
+  float up[3];
+  float scale;
 } orc_light;
 
 typedef struct orc_volume {
@@ -212,6 +219,15 @@ void orc_quad_light_sample(const orc_light *L, const float ref[3], int n, uint32
    (RNG seeded (seed, i)), and 4 pi p_HG(cos theta) --- */
 void orc_hg_sample(float g, const float d[3], int n, uint32_t seed, float *out);
 double orc_hg_4pi(double g, double cos_theta);
+/* --- HDRI light (type ORC_LIGHT_HDRI): its CDFs (h row-total entries, w*h
+   per-row entries; -1 for a degenerate light), n light samples (RNG seeded
+   (seed, i)): direction, pdf and radiance; and for given directions the
+   radiance, the hit-side light pdf and the map coordinates --- */
+int orc_hdri_cdf(const orc_light *L, float *rows, float *cols);
+void orc_hdri_sample(const orc_light *L, int n, uint32_t seed, float *dirs, float *pdfs,
+                     float *rgb);
+void orc_hdri_eval(const orc_light *L, int n, const float *dirs, float *rgb, float *pdfs,
+                   float *uv);
 void orc_camera_ray(const orc_camera *cam, float x, float y, int width,
                     int height, float *ori, float *dir);
 

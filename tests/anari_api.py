@@ -9,7 +9,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(ROOT, "anari-visionaray_amd", "_build", "libanari_vrt.so")
 
 STRING, BOOL, DATA_TYPE = 101, 103, 100
-ARRAY1D, ARRAY3D = 504, 506
+ARRAY1D, ARRAY2D, ARRAY3D = 504, 505, 506
 CAMERA, FRAME, GEOMETRY, GROUP, INSTANCE, LIGHT = 507, 508, 509, 510, 511, 512
 MATERIAL, RENDERER, SURFACE, SPATIAL_FIELD, VOLUME, WORLD = 513, 514, 515, 517, 518, 519
 UINT8, INT32, UINT32, UINT32_VEC2, UINT32_VEC3 = 1004, 1016, 1020, 1021, 1022
@@ -35,7 +35,7 @@ class Anari:
         vp, u64, ip = C.c_void_p, C.c_uint64, C.c_int
         L.anariNewVisionarayDevice.restype = vp
         L.anariNewVisionarayDevice.argtypes = [_CB, vp]
-        for fn, n in (("anariNewArray1D", 1), ("anariNewArray3D", 3)):
+        for fn, n in (("anariNewArray1D", 1), ("anariNewArray2D", 2), ("anariNewArray3D", 3)):
             getattr(L, fn).restype = vp
             getattr(L, fn).argtypes = [vp, vp, vp, vp, ip] + [u64] * n
         for fn in ("anariNewGeometry", "anariNewMaterial", "anariNewSpatialField",
@@ -85,6 +85,12 @@ class Anari:
         self._keep.append(a)
         n = a.size // _NP[elem][1] if elem in _NP else len(a)
         return self.L.anariNewArray1D(self.dev, a.ctypes.data, None, None, elem, n)
+
+    def array2d(self, image, elem):
+        a = np.ascontiguousarray(image)
+        self._keep.append(a)
+        h, w = a.shape[:2]
+        return self.L.anariNewArray2D(self.dev, a.ctypes.data, None, None, elem, w, h)
 
     def array3d(self, voxels, elem=FLOAT32):
         a = np.ascontiguousarray(voxels)

@@ -17,7 +17,9 @@ f3 = C.c_float * 3
 
 class OrcLight(C.Structure):
     _fields_ = [("type", C.c_int), ("visible", C.c_int), ("dir", f3), ("pos", f3),
-                ("e1", f3), ("e2", f3), ("color", f3), ("intensity", C.c_float)]
+                ("e1", f3), ("e2", f3), ("color", f3), ("intensity", C.c_float),
+                ("hdri_rgb", C.c_void_p), ("hdri_w", C.c_int), ("hdri_h", C.c_int),
+                ("up", f3), ("scale", C.c_float)]
 
 
 class OrcVolume(C.Structure):
@@ -107,6 +109,9 @@ def lib():
         L.orc_hg_sample.argtypes = [C.c_float, vp, ip, C.c_uint32, vp]
         L.orc_hg_4pi.restype = C.c_double
         L.orc_hg_4pi.argtypes = [C.c_double, C.c_double]
+        L.orc_hdri_cdf.argtypes = [C.POINTER(OrcLight), vp, vp]
+        L.orc_hdri_sample.argtypes = [C.POINTER(OrcLight), ip, C.c_uint32, vp, vp, vp]
+        L.orc_hdri_eval.argtypes = [C.POINTER(OrcLight), ip, vp, vp, vp, vp]
         L.orc_nvdb_get.argtypes = [vp, ip, vp, vp]
         L.orc_nvdb_sample.argtypes = [vp, ip, vp, ip, vp]
         L.orc_nvdb_world_bbox.argtypes = [vp, vp]
@@ -358,6 +363,24 @@ def light_quad(position, edge1, edge2, intensity=1.0, color=(1, 1, 1), visible=T
     return L
 
 
+_HDRI_KEEP = []
+
+
+def light_hdri(radiance, up=(1, 0, 0), direction=(0, 0, 1), scale=1.0, visible=True):
+    """An "hdri" light over the (h, w, 3) float32 map (kept alive here)."""
+    rgb = np.ascontiguousarray(radiance, np.float32)
+    _HDRI_KEEP.append(rgb)
+    L = OrcLight()
+    L.type = 2
+    L.visible = int(visible)
+    L.hdri_rgb = rgb.ctypes.data
+    L.hdri_h, L.hdri_w = rgb.shape[:2]
+    L.up = f3(*up)
+    L.dir = f3(*direction)
+    L.scale = scale
+    return L
+
+
 def light_directional(direction, intensity=1.0, color=(1, 1, 1), visible=True):
     L = OrcLight()
     L.type = 0
@@ -423,7 +446,11 @@ class OracleWorkload:
             self.vol.attach_majorants(wl.grid_dims)
         lights = []
         for L in wl.lights:
-            if L.get("type", "directional") == "quad":
+            if L.get("type") == "hdri":
+                lights.append(light_hdri(L["radiance"], L.get("up", (1, 0, 0)),
+                                         L.get("direction", (0, 0, 1)), L.get("scale", 1.0),
+                                         L.get("visible", True)))
+            elif L.get("type", "directional") == "quad":
                 lights.append(light_quad(L["position"], L["edge1"], L["edge2"], L["intensity"],
                                          L["color"], L.get("visible", True)))
             else:

@@ -71,6 +71,16 @@ def _anari_scene(a, wl, xfm=None, bvh_builder=None):
         objs.update(field=f, volume=v)
     lights = []
     for L in wl.lights:
+        if L.get("type") == "hdri":
+            l = a.new(A.LIGHT, "hdri")
+            a.set(l, "radiance", A.ARRAY2D, a.array2d(L["radiance"], A.FLOAT32_VEC3))
+            a.set(l, "up", A.FLOAT32_VEC3, L.get("up", (1, 0, 0)))
+            a.set(l, "direction", A.FLOAT32_VEC3, L.get("direction", (0, 0, 1)))
+            a.set(l, "scale", A.FLOAT32, L.get("scale", 1.0))
+            a.set(l, "visible", A.BOOL, int(L.get("visible", True)))
+            a.commit(l)
+            lights.append(l)
+            continue
         if L.get("type", "directional") == "quad":
             l = a.new(A.LIGHT, "quad")
             for k in ("position", "edge1", "edge2"):
@@ -170,6 +180,11 @@ CASES = {
     "structured pathtrace": lambda w: w.fbm_pathtrace(n=32, width=64, height=48, frames=1,
                                                       brick=8, phase_g=0.3),
     "structured raycast": lambda w: w.blobs_raycast(n=48, width=64, height=48),
+    "hdri mixed surfaces": lambda w: dataclasses.replace(
+        w.mixed_pathtrace(level=3, vol_n=24, width=48, height=32, frames=1),
+        lights=[dict(type="hdri", up=(0, 0, 1), direction=(1, 0, 0), scale=1.3,
+                     radiance=np.random.default_rng(2).uniform(0.1, 2.0, (8, 16, 3))
+                     .astype(np.float32))]),
     "gradient-shaded raycast": lambda w: dataclasses.replace(
         w.blobs_raycast(n=48, width=64, height=48),
         lights=[dict(type="directional", direction=(0.5, -0.7, -0.4), color=(1, 1, 1),
