@@ -1177,7 +1177,9 @@ __device__ inline void gbuf_miss(const KFrame &F, size_t i) {
   if (1e31f <= F.depth[i]) gbuf_write(F, i, mk(0.f), mk(0.f), ~0u, ~0u, ~0u);
 }
 
-template <bool ANYF, bool SHADE>
+// SURF: the scene has surfaces (the BVH walk and surface shading are
+// compiled in, with the traversal stack's top in LDS after the tables)
+template <bool ANYF, bool SHADE, bool SURF>
 __global__ void __launch_bounds__(256)
     k_raycast(KScene sc, KCamera cam, KParams P, KFrame F) {
   extern __shared__ float4 s_tf[];
@@ -1189,10 +1191,75 @@ __global__ void __launch_bounds__(256)
   if (active) {
     float4 result = P.bg;
     int hit_vol = -1;
+    // g-buffer of the sample (Raycast_impl.cpp:18-20, 135-143, 230-237)
+    float gb_depth = 1e31f;
+    f3 gb_n = mk(0.f), gb_alb = mk(0.f);
+    uint32_t gb_prim = ~0u, gb_obj = ~0u, gb_inst = ~0u;
+    // light samples (surface and gradient shading) from the pixel's
+    // path-tracer stream: the reference's ray caster leaves its generator
+    // unseeded
+    uint32_t rs = (SHADE || SURF) ? tea4(uint32_t(y * F.width + x), 0u) : 0u;
     if (sc.num_volumes > 0 || sc.num_surfaces > 0) {  // an empty scene shows the background
       KRay ray = primary_ray(cam, float(x), float(y), float(F.width), float(F.height));
       clip_ray(ray, P);
-      result = light_backdrop(sc, ray, P.bg);
+      float4 surf = make_float4(0.f, 0.f, 0.f, 0.f);  // surfaceColor, surfaceAlpha
+      if constexpr (SURF) {
+        // the closest surface (Raycast_impl.cpp:27-156): matte under every
+        // light along the surface normal (not faceforwarded) plus ambient;
+        // matte is opaque, so the first hit ends the surface loop and the
+        // volume march stops there
+        int2 *bstack = reinterpret_cast<int2 *>(reinterpret_cast<char *>(s_tf) +
+                                                scene_lds_bytes(sc)) + threadIdx.x;
+        float st = ray.tmax;
+        int sid = -1, prim = -1;
+        uint32_t srec = 0;
+        for (int i = 0; i < sc.num_surfaces; ++i) {
+          float t = st;
+          uint32_t rec;
+          int p;
+          if (bvh_hit(sc.surf[i], ray, false, t, p, rec, bstack)) {
+            st = t;
+            sid = i;
+            srec = rec;
+            prim = p;
+          }
+        }
+        if (sid >= 0) {
+          KSurface S = sc.surf[0];
+          for (int i = 1; i < sc.num_surfaces; ++i)
+            if (i == sid) S = sc.surf[i];
+          const float4 v1 = S.tris[3 * size_t(srec)];
+          const float4 e1 = S.tris[3 * size_t(srec) + 1], e2 = S.tris[3 * size_t(srec) + 2];
+          const f3 hitPos = ray.ori + ray.dir * st;
+          // getNormals, then normalize(nxfm * n) under the identity instance
+          const f3 sn = normalize(normalize(cross(mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z))));
+          const float inv_pi = 0.318309886183790671538f;
+          f3 shaded = mk(0.f);
+          for (int i = 0; i < sc.num_lights; ++i) {
+            const LightSmp ls = sample_light<true>(sc, sc.light[i], hitPos, rs);
+            const f3 lightDir = normalize(ls.dir);
+            const f3 lightIntensity = ls.intensity * safe_rcp(ls.dist2);
+            const float NdotL = fmaxf(0.f, dot(sn, lightDir));
+            shaded = shaded + S.color * inv_pi * lightIntensity * NdotL * safe_rcp(ls.pdf);
+          }
+          shaded = shaded + S.color * P.ambient_color * P.ambient_radiance;
+          surf = make_float4(shaded.x, shaded.y, shaded.z, 1.f);
+          ray.tmax = st;
+          gb_depth = st;
+          gb_n = sn;
+          gb_alb = S.color;
+          gb_prim = __float_as_uint(v1.w);
+          gb_obj = S.obj_id;
+          gb_inst = S.inst_id;
+          (void)prim;
+        }
+      }
+      // over(float4(surfaceColor, surfaceAlpha), backdrop) (:167-173)
+      {
+        const float4 b = light_backdrop(sc, ray, P.bg);
+        const float a = 1.f - surf.w;
+        result = make_float4(surf.x + a * b.x, surf.y + a * b.y, surf.z + a * b.z, surf.w + a * b.w);
+      }
       // closest volume box (intersectVolumeBounds, DeviceCopyableObjects.h:1841-1859;
       // a box hit counts when tfar >= tmin, :695-700)
       int best = -1;
@@ -1216,9 +1283,6 @@ __global__ void __launch_bounds__(256)
         const float tmax = fminf(lr.tmax, bh.tfar);
         const f3 ori = v.S * (lr.ori + v.T);
         const f3 dir = v.S * lr.dir;  // not renormalised (VolumeIntegration.h:395-396)
-        // gradient shading: lights sampled with the pixel's path-tracer stream
-        // (the reference's ray caster leaves its generator unseeded)
-        uint32_t rs = SHADE ? tea4(uint32_t(y * F.width + x), 0u) : 0u;
         const float dt = v.cell_size * P.sampling_rate_inv;
         const float expo = dt / v.unit_distance;
         float cr = 0.f, cg = 0.f, cb = 0.f, alpha = 0.f, trans = 1.f;
@@ -1269,19 +1333,20 @@ __global__ void __launch_bounds__(256)
       }
     }
     if (gbuf_wanted(F)) {
-      // a volume box hit sets the object and instance ids, normal and
-      // albedo 0; depth and primitive id keep their defaults
-      // (Raycast_impl.cpp:18-20, 230-237)
-      uint32_t obj = ~0u, inst = ~0u;
+      // a volume box hit sets the object and instance ids and zeroes normal
+      // and albedo; depth and primitive id stay the surface's (or the
+      // defaults)
       for (int i = 0; i < sc.num_volumes; ++i)
         if (i == hit_vol) {
-          obj = sc.vol[i].obj_id;
-          inst = sc.vol[i].inst_id;
+          gb_obj = sc.vol[i].obj_id;
+          gb_inst = sc.vol[i].inst_id;
+          gb_n = mk(0.f);
+          gb_alb = mk(0.f);
         }
       const size_t i = size_t(y) * F.width + x;
-      if (1e31f <= F.depth[i]) gbuf_write(F, i, mk(0.f), mk(0.f), ~0u, obj, inst);
+      if (gb_depth <= F.depth[i]) gbuf_write(F, i, gb_n, gb_alb, gb_prim, gb_obj, gb_inst);
     }
-    write_sample(F, x, y, 0, result, 1e31f, false);
+    write_sample(F, x, y, 0, result, gb_depth, false);
   }
   if (F.counters) wave_add_counter(&F.counters[0], lookups);
 }
@@ -2295,11 +2360,17 @@ static bool scene_has_other_fields(const KScene &sc);
 
 hipError_t launch_raycast(const KScene &sc, const KCamera &cam, const KParams &P, const KFrame &F,
                           hipStream_t s) {
-  const size_t lds = scene_lds_bytes(sc);
+  const bool surf = sc.num_surfaces > 0;
+  const size_t lds = scene_lds_bytes(sc) + (surf ? bvh_lds_stack_bytes() : 0);
   const bool any = scene_has_other_fields(sc), shade = P.gradient_shading != 0;
-  auto *k = any ? (shade ? k_raycast<true, true> : k_raycast<true, false>)
-                : (shade ? k_raycast<false, true> : k_raycast<false, false>);
-  hipLaunchKernelGGL(k, dim3(tiles(F)), dim3(256), lds, s, sc, cam, P, F);
+  using RaycastKernel = void (*)(KScene, KCamera, KParams, KFrame);
+  static const RaycastKernel table[8] = {
+      k_raycast<false, false, false>, k_raycast<true, false, false>,
+      k_raycast<false, true, false>,  k_raycast<true, true, false>,
+      k_raycast<false, false, true>,  k_raycast<true, false, true>,
+      k_raycast<false, true, true>,   k_raycast<true, true, true>};
+  hipLaunchKernelGGL(table[(surf ? 4 : 0) + (shade ? 2 : 0) + (any ? 1 : 0)], dim3(tiles(F)),
+                     dim3(256), lds, s, sc, cam, P, F);
   return hipGetLastError();
 }
 

@@ -1648,14 +1648,67 @@ V3<R> gradient_shade(const Scene<R> &sc, const Volume<R> &vol, const orc_params 
 
 template <class R>
 V4<R> raycast_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params *P,
-                    int x, int y, int W, int H) {
+                    int x, int y, int W, int H, GBuf<R> *gb = nullptr) {
   V4<R> bg{R(P->bg[0]), R(P->bg[1]), R(P->bg[2]), R(P->bg[3])};
+  if (gb) *gb = GBuf<R>();
   if (sc.empty()) return bg;
   Ray<R> ray = cam.primary(R(x), R(y), R(W), R(H));
   clip_ray(ray, P);  // Raycast_impl.cpp:288
-  // over(float4(0), backdrop) (no surfaces), Raycast_impl.cpp:158-173: the
-  // closest visible light the ray meets -- a quad, or the environment at
-  // FLT_MAX -- as float4(intensity, 1), else the background colour
+  // light samples (surface and gradient shading) come from the pixel's
+  // path-tracer stream: the reference's ray caster leaves its generator
+  // unseeded
+  Random rs(uint32_t(x + W * y), 0u);
+  auto safe_rcp = [](R f) { return f > R(0) ? R(1) / f : R(0); };
+  // the closest surface (Raycast_impl.cpp:27-156): matte under every light
+  // along the surface normal (not faceforwarded) plus the ambient term;
+  // matte is opaque, so the first hit ends the loop and bounds the volume
+  V4<R> surf{R(0), R(0), R(0), R(0)};
+  {
+    R st = std::numeric_limits<R>::max();
+    int sm = -1, sp = -1;
+    for (size_t m = 0; m < sc.meshes.size(); ++m) {
+      R t, u, v;
+      int prim = -1;
+      Ray<R> r = ray;
+      if (sm >= 0) r.tmax = st;
+      if (sc.meshes[m].intersect(r, false, t, prim, u, v)) {
+        st = t;
+        sm = int(m);
+        sp = prim;
+      }
+    }
+    if (sm >= 0) {
+      const Mesh<R> &M = sc.meshes[sm];
+      const Tri<R> &T = M.tris[sp];
+      const V3<R> hitPos = ray.ori + ray.dir * st;
+      const V3<R> sn = normalize(normalize(cross(T.e1, T.e2)));
+      const R inv_pi = R(0.318309886183790671538);
+      V3<R> shaded(R(0));
+      for (int i = 0; i < sc.s->num_lights; ++i) {
+        const LightSample<R> ls =
+            sample_light<R>(sc.s->lights[i], sc.quads[i], sc.hdris[i], hitPos, rs);
+        const V3<R> lightDir = normalize(ls.dir);
+        const V3<R> lightIntensity = ls.intensity * safe_rcp(ls.dist2);
+        const R NdotL = std::max(R(0), dot(sn, lightDir));
+        shaded = shaded + M.color * inv_pi * lightIntensity * NdotL * safe_rcp(ls.pdf);
+      }
+      shaded = shaded + M.color * from<R>(P->ambient_color) * R(P->ambient_radiance);
+      surf = V4<R>{shaded.x, shaded.y, shaded.z, R(1)};
+      ray.tmax = st;
+      if (gb) {  // :135-143
+        gb->depth = st;
+        gb->Ng = sn;
+        gb->albedo = M.color;
+        gb->prim = uint32_t(sp);
+        gb->obj = M.m->object_id;
+        gb->inst = M.m->instance_id;
+      }
+    }
+  }
+  // over(float4(surfaceColor, surfaceAlpha), backdrop), Raycast_impl.cpp:
+  // 158-173: the backdrop is the closest visible light the ray meets -- a
+  // quad, or the environment at FLT_MAX -- as float4(intensity, 1), else the
+  // background colour
   V4<R> result = bg;
   {
     int lid = -1;
@@ -1684,6 +1737,9 @@ V4<R> raycast_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params *
         result = V4<R>{I.x, I.y, I.z, R(1)};
       }
     }
+    const R a = R(1) - surf.w;
+    result = V4<R>{surf.x + a * result.x, surf.y + a * result.y, surf.z + a * result.z,
+                   surf.w + a * result.w};
   }
   // intersectVolumeBounds: closest volume box (DeviceCopyableObjects.h:695-709)
   int best = -1;
@@ -1700,6 +1756,12 @@ V4<R> raycast_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params *
   }
   if (best < 0) return result;
   const Volume<R> &vol = sc.vols[best];
+  if (gb) {  // :230-237: the volume's ids; normal and albedo 0
+    gb->obj = vol.v->object_id;
+    gb->inst = vol.v->instance_id;
+    gb->Ng = V3<R>(R(0));
+    gb->albedo = V3<R>(R(0));
+  }
   Ray<R> lr = vol.to_local(ray);
   BoxHit<R> bh = intersect_box(lr, vol.blo, vol.bhi);
   lr.tmin = std::max(lr.tmin, bh.tnear);
@@ -1711,9 +1773,6 @@ V4<R> raycast_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params *
   const R expo = dt / R(vol.v->unit_distance);
   V3<R> color(R(0));
   R alpha = R(0), transmittance = R(1);
-  // gradient shading samples lights with the pixel's path-tracer stream (the
-  // reference's ray caster leaves its generator unseeded)
-  Random rs(uint32_t(x + W * y), 0u);
   for (R t = lr.tmin; t <= lr.tmax && alpha < R(0.99f); t += dt) {
     V3<R> p = ori + dir * t;
     R v;
@@ -2157,9 +2216,9 @@ void orc_pathtrace_gbuffer(const orc_scene *scene, const orc_camera *c, const or
   });
 }
 
-// Raycast_impl.cpp:18-20, 230-237: depth stays 1e31 and the primitive id
-// unset; a hit of the closest volume box sets the object and instance ids
-// (normal and albedo 0)
+// Raycast_impl.cpp:18-20, 135-143, 230-237: the first surface's depth,
+// normal, albedo and ids (else depth 1e31, ids unset); a hit of the closest
+// volume box then sets the object and instance ids, normal and albedo 0
 void orc_raycast_gbuffer(const orc_scene *scene, const orc_camera *c, const orc_params *P,
                          int W, int H, float *depth, float *normal, float *albedo,
                          uint32_t *prim, uint32_t *obj, uint32_t *inst, int threads) {
@@ -2167,26 +2226,7 @@ void orc_raycast_gbuffer(const orc_scene *scene, const orc_camera *c, const orc_
   Camera<float> cam(c);
   parallel_tiles(W, H, threads, [&](int x, int y) {
     GBuf<float> g;
-    if (!sc.empty()) {
-      Ray<float> ray = cam.primary(float(x), float(y), float(W), float(H));
-      clip_ray(ray, P);
-      int best = -1;
-      float bestT = std::numeric_limits<float>::max();
-      for (size_t i = 0; i < sc.vols.size(); ++i) {
-        Ray<float> lr = sc.vols[i].to_local(ray);
-        BoxHit<float> bh = intersect_box(lr, sc.vols[i].blo, sc.vols[i].bhi);
-        if (!bh.hit || !(bh.tfar >= lr.tmin)) continue;
-        const float t = std::max(lr.tmin, bh.tnear);
-        if (t < bestT) {
-          bestT = t;
-          best = int(i);
-        }
-      }
-      if (best >= 0) {
-        g.obj = sc.vols[best].v->object_id;
-        g.inst = sc.vols[best].v->instance_id;
-      }
-    }
+    (void)raycast_pixel(sc, cam, P, x, y, W, H, &g);
     fill_gbuffer(g, size_t(y) * W + x, depth, normal, albedo, prim, obj, inst);
   });
 }

@@ -254,3 +254,37 @@ def test_raycast_volume_box_ids(pkg, oracle):
             assert g["instance_id"][y, x] == {5: 50, 6: 60, NO_ID: NO_ID}[want]
             counts[want] += 1
     assert min(counts.values()) > 50
+
+
+def test_raycast_surface_matte_closed_form(pkg, oracle):
+    """Ray caster over a quad facing the camera under one directional light:
+    matte color/pi * E * max(0, n.L) + color * ambient, with n the geometric
+    normal as stored (not faceforwarded, Raycast_impl.cpp:45-100); the
+    g-buffer holds the surface's depth, normal, albedo and ids."""
+    wl = dataclasses.replace(_surface_workload(pkg), renderer="raycast",
+                             params=dict(ambient_radiance=0.25, ambient_color=(1, 1, 1)))
+    L = np.array((0.3, -0.5, -1.0))
+    ow = _oracle(oracle, wl)
+    img = ow.raycast()
+    g = ow.gbuffer()
+    Ld = -L / np.linalg.norm(L)
+    col = np.array([0.25, 0.5, 0.75])
+    o, d = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    hits = 0
+    for y in range(wl.height):
+        for x in range(wl.width):
+            oracle.lib().orc_camera_ray(C.byref(ow.cam), x, y, wl.width, wl.height,
+                                        o.ctypes.data, d.ctypes.data)
+            t = -o[2] / d[2]
+            P = o + t * d
+            if max(abs(P[0]), abs(P[1])) > 0.6 - 1e-3:
+                continue
+            hits += 1
+            # e1 x e2 of both triangles points to +z: towards the camera
+            want = col / math.pi * 2.0 * max(0.0, Ld[2]) + col * 0.25
+            np.testing.assert_allclose(img[y, x, :3], want, rtol=2e-6)
+            assert img[y, x, 3] == 1.0
+            assert g["depth"][y, x] == pytest.approx(t, rel=1e-5)
+            np.testing.assert_allclose(g["normal"][y, x], [0, 0, 1], atol=1e-7)
+            assert (g["object_id"][y, x], g["instance_id"][y, x]) == (7, 3)
+    assert hits > 400

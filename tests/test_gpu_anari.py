@@ -185,6 +185,9 @@ CASES = {
         lights=[dict(type="hdri", up=(0, 0, 1), direction=(1, 0, 0), scale=1.3,
                      radiance=np.random.default_rng(2).uniform(0.1, 2.0, (8, 16, 3))
                      .astype(np.float32))]),
+    "raycast mixed surfaces": lambda w: dataclasses.replace(
+        w.mixed_pathtrace(level=3, vol_n=24, width=48, height=32, frames=1),
+        renderer="raycast", params=dict(volume_sampling_rate=1.0)),
     "gradient-shaded raycast": lambda w: dataclasses.replace(
         w.blobs_raycast(n=48, width=64, height=48),
         lights=[dict(type="directional", direction=(0.5, -0.7, -0.4), color=(1, 1, 1),

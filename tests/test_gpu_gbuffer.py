@@ -171,3 +171,13 @@ def test_gbuffer_needs_depth(pkg, gpu_ctx):
     gs.frame.desc.depth = None
     with pytest.raises(vrt.VrtError, match="depth"):
         gs.render(0)
+
+
+def test_raycast_surfaces_gbuffer(pkg, oracle, gpu_ctx):
+    """Ray caster with surfaces: the first surface's depth, normal, albedo and
+    ids, then the volume box's ids over them (Raycast_impl.cpp:135-143,
+    230-237)."""
+    wl = pkg.workloads.mixed_pathtrace(level=3, vol_n=24, width=48, height=32, frames=1)
+    wl = dataclasses.replace(wl, renderer="raycast", params=dict(volume_sampling_rate=1.0))
+    g = _check(pkg, oracle, gpu_ctx, wl, frames=2, vol_ids=(21, 22), surf_ids=[(31, 32)])
+    assert (g["depth"] < 1e30).mean() > 0.5

@@ -78,3 +78,18 @@ def test_pathtrace_gradient_shading(pkg, oracle, gpu_ctx, ambient_samples):
     assert close.mean() >= oracle.STREAM_AGREEMENT_MIN
     hit = np.linalg.norm(n_orc, axis=-1) > 0
     assert hit.mean() > 0.2
+
+
+@pytest.mark.parametrize("gradient", [False, True])
+def test_raycast_with_surfaces(pkg, oracle, gpu_ctx, gradient):
+    """Ray caster over the mixed scene: matte surfaces under a directional and
+    a quad light, the volume in front of them, the quad as backdrop."""
+    wl = pkg.workloads.mixed_pathtrace(level=3, vol_n=24, width=64, height=48, frames=1)
+    sun = dict(type="directional", direction=(0.4, -0.8, -0.3), color=(1, 1, 1), intensity=1.5)
+    wl = dataclasses.replace(wl, renderer="raycast", lights=list(wl.lights) + [sun],
+                             params=dict(volume_sampling_rate=1.0, gradient_shading=gradient))
+    g, o = _raycast_pair(pkg, oracle, gpu_ctx, wl)
+    err = np.abs(g - o).max()
+    print("raycast with surfaces max-abs", err, "gradient", gradient)
+    assert err <= 1e-3
+    assert (g[..., 3] > 0.99).mean() > 0.5  # the sphere covers most of the image
