@@ -241,6 +241,14 @@ struct Frame : Object {
   // primitive / object / instance id (uint32); null when not requested
   DevMem gbuf[5];
   std::vector<uint8_t> host_gbuf[5];
+  // temporal anti-aliasing (Frame.cpp:339-381, 515-540): motion vectors,
+  // current / history colour and albedo; the previous frame's camera
+  DevMem taa[5];
+  bool taa_on = false;
+  float taa_alpha = 0.3f;
+  size_t taa_pixels = 0;
+  bool has_prev_cam = false;
+  vrt_camera prev_cam{};
   std::atomic<int> callbacks_pending{0};
   uint64_t built = ~0ull;
   uint64_t scene_version = ~0ull;
@@ -1025,6 +1033,9 @@ void render(Device *d, Frame *fr) {
     }
   }
   const bool pathtrace = ren->subtype == "pathtrace";
+  // path tracer "taa" / "taaAlpha" (Pathtrace.cpp:24-25)
+  const bool taa = pathtrace && ren->get_b("taa", false);
+  const float taa_alpha = ren->get_f("taaAlpha", 0.3f);
   if (!pathtrace && ren->subtype != "raycast" && ren->subtype != "default")
     d->report(ren, ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_OPERATION,
               "renderer '%s' not supported: using raycast", ren->subtype.c_str());
@@ -1042,6 +1053,34 @@ void render(Device *d, Frame *fr) {
     f.primitive_id = static_cast<uint32_t *>(fr->gbuf[2].p);
     f.object_id = static_cast<uint32_t *>(fr->gbuf[3].p);
     f.instance_id = static_cast<uint32_t *>(fr->gbuf[4].p);
+  }
+  if (taa) {
+    // checkTAAReset (Frame.cpp:515-540): fresh zeroed buffers when TAA is
+    // switched on, the size changes or alpha changes
+    const size_t np = size_t(fr->w) * fr->h;
+    if (!fr->taa_on || fr->taa_pixels != np || fr->taa_alpha != taa_alpha) {
+      const size_t bytes[5] = {np * 16, np * 16, np * 16, np * 12, np * 12};
+      for (int k = 0; k < 5; ++k)
+        if (!fr->taa[k].alloc(d->ctx, bytes[k]) ||
+            vrt_memset(d->ctx, fr->taa[k].p, 0, bytes[k]) != VRT_OK) {
+          d->vrt_error(fr, "taa buffers");
+          return;
+        }
+      fr->taa_pixels = np;
+      fr->taa_alpha = taa_alpha;
+    }
+    fr->taa_on = true;
+    f.motion_vec = static_cast<float *>(fr->taa[0].p);
+    f.taa_curr = static_cast<float *>(fr->taa[1].p);
+    f.taa_prev = static_cast<float *>(fr->taa[2].p);
+    f.taa_curr_albedo = static_cast<float *>(fr->taa[3].p);
+    f.taa_prev_albedo = static_cast<float *>(fr->taa[4].p);
+    p.taa = 1;
+    p.taa_alpha = taa_alpha;
+    p.has_prev_camera = fr->has_prev_cam ? 1 : 0;
+    p.prev_camera = fr->prev_cam;
+  } else {
+    fr->taa_on = false;
   }
   if (ver != fr->scene_version) {  // Frame.cpp:499-513
     fr->scene_version = ver;
@@ -1068,6 +1107,9 @@ void render(Device *d, Frame *fr) {
   }
   ++fr->accum_id;
   ++fr->frame_id;
+  // the matrices of this frame become the previous ones (Frame.cpp:333-334)
+  fr->prev_cam = c;
+  fr->has_prev_cam = true;
   // frameCompletionCallback (Frame.cpp:90-93, 383-384)
   ANARIFrameCompletionCallback fn = nullptr;
   const void *user = nullptr;

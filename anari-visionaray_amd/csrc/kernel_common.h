@@ -140,6 +140,11 @@ struct KParams {
   float4 clip[VRT_MAX_CLIP_PLANES];
   float phase_g;
   int gradient_shading;
+  // temporal anti-aliasing: previous and current view-projection (row-major
+  // proj * view), blend weight
+  int taa;
+  float taa_alpha;
+  float vp_prev[16], vp_curr[16];
 };
 
 struct KFrame {
@@ -153,6 +158,9 @@ struct KFrame {
   float *normal;  // g-buffer (float3 per pixel), optional
   float *albedo;
   uint32_t *prim_id, *obj_id, *inst_id;
+  float4 *motion;                    // TAA: motion vector per pixel
+  float4 *taa_curr, *taa_prev;       // TAA: current sample / history colour
+  float *taa_curr_alb, *taa_prev_alb;  // TAA: albedo (float3) of each
 };
 
 // Per-frame buffers of the radiance-cache path tracer.  Paths that are not

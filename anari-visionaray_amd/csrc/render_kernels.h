@@ -18,6 +18,7 @@ hipError_t launch_pathtrace(const KScene &sc, const KCamera &cam, const KParams 
                             uint32_t *queues_next, int blocks, hipStream_t s,
                             const KNrc *nrc = nullptr);
 hipError_t launch_nrc_resolve(const KFrame &F, const KNrc &N, int accumID, hipStream_t s);
+hipError_t launch_taa_resolve(const KFrame &F, float alpha, hipStream_t s);
 // adds sum (x - r)^2 / (r^2 + eps) over n pixels' RGB to *out
 hipError_t launch_relmse(const float4 *x, const double *ref_sum, double ref_scale, int n,
                          float eps, double *out, hipStream_t s);

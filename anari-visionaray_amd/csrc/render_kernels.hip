@@ -879,6 +879,8 @@ __device__ inline float linear_to_srgb(float x) {
   return x <= 0.0031308f ? 12.92f * x : 1.055f * powf(x, 1.f / 2.4f) - 0.055f;
 }
 
+__device__ inline void tone_map(const KFrame &F, size_t idx, float4 c);
+
 __device__ inline void write_sample(const KFrame &F, int x, int y, int accumID, float4 c,
                                     float depth, bool stochastic) {
   const size_t idx = size_t(y) * F.width + x;
@@ -898,7 +900,11 @@ __device__ inline void write_sample(const KFrame &F, int x, int y, int accumID, 
                     (1.f - alpha) * prev.z + alpha * c.z, (1.f - alpha) * prev.w + alpha * c.w);
     F.accum[idx] = c;
   }
-  // toneMap (DeviceCopyableObjects.h:2720-2739), cvt_uint32 (frame/common.h:31-47)
+  tone_map(F, idx, c);
+}
+
+// toneMap (DeviceCopyableObjects.h:2720-2739), cvt_uint32 (frame/common.h:31-47)
+__device__ inline void tone_map(const KFrame &F, size_t idx, float4 c) {
   if (F.color_type == VRT_TYPE_FLOAT32_VEC4) {
     reinterpret_cast<float4 *>(F.color)[idx] = c;
   } else if (F.color_type == VRT_TYPE_UFIXED8_VEC4) {
@@ -1386,6 +1392,7 @@ struct PathState {
   int rayType;
   int bounceID;
   bool first;  // the sample's first pass
+  bool taa_take;  // TAA: this sample is the pixel's closest so far
   bool firstHit;
   f3 throughput, intensity;
   // ShadeState (Pathtrace_impl.cpp:15-35) of the current vertex
@@ -1589,6 +1596,32 @@ __device__ inline bool roulette(PathState &st) {
   return false;
 }
 
+// project (visionaray, parity unpinned: gluProject) of a world point with a
+// row-major proj * view matrix into window coordinates of a W x H viewport
+__device__ inline float2 project_window(const float *M, f3 p, int W, int H) {
+  const float x = ((M[0] * p.x + M[1] * p.y) + M[2] * p.z) + M[3];
+  const float y = ((M[4] * p.x + M[5] * p.y) + M[6] * p.z) + M[7];
+  const float w = ((M[12] * p.x + M[13] * p.y) + M[14] * p.z) + M[15];
+  return make_float2(float(W) * (x / w + 1.f) / 2.f, float(H) * (y / w + 1.f) / 2.f);
+}
+
+// TAA at a radiance vertex of the pixel's closest sample: the motion vector
+// prev - curr window position of the hit, written at every such vertex
+// (the last one stands, Pathtrace_impl.cpp:290-297), and the first hit's
+// albedo
+__device__ inline void taa_vertex(const KFrame &F, const KParams &P, const PathState &st,
+                                  f3 hitPos, f3 albedo) {
+  const float2 a = project_window(P.vp_prev, hitPos, F.width, F.height);
+  const float2 b = project_window(P.vp_curr, hitPos, F.width, F.height);
+  F.motion[st.pixel] = make_float4(a.x - b.x, a.y - b.y, 0.f, 1.f);
+  if (st.bounceID == 0) {
+    float *al = F.taa_curr_alb + 3 * size_t(st.pixel);
+    al[0] = albedo.x;
+    al[1] = albedo.y;
+    al[2] = albedo.z;
+  }
+}
+
 // One pass of the bounce loop (Pathtrace_impl.cpp:558-600) after the ray's
 // intersection `h`: shade, advance the state machine, set up the next ray.
 // Returns true when the sample's path ended.  With NRC, a path reaching its
@@ -1605,6 +1638,9 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
     st.first = false;
     st.firstHit = hit;
     if (h.kind == kVolume || h.kind == kSurface) st.sdepth = h.t;
+    // TAA keeps the closest sample's albedo and motion (the first of equal
+    // depths, Pathtrace_impl.cpp:616)
+    st.taa_take = P.taa && (h.kind == kVolume || h.kind == kSurface) && h.t < st.depth;
   }
   if constexpr (NRC) {
     const bool v2 = st.rayType == Radiance && h.kind == kVolume && st.bounceID == 1;
@@ -1710,6 +1746,7 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
           }
         gbuf_write(F, size_t(st.pixel), gn, color, prim, obj, inst);
       }
+      if (st.taa_take) taa_vertex(F, P, st, hitPos, color);
     } else if (AREA) {
       // triangle: Ng = normalize(e1 x e2), no vertex normals (getNormals,
       // common.h:233-264), normalised again after the (identity) normal
@@ -1725,6 +1762,7 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
       color = S.color;
       if (gbuf)
         gbuf_write(F, size_t(st.pixel), gn, color, __float_as_uint(v1.w), S.obj_id, S.inst_id);
+      if (st.taa_take) taa_vertex(F, P, st, hitPos, color);
     }
     LightSmp ls;
     ls.intensity = mk(0.f);
@@ -1978,13 +2016,68 @@ __global__ void __launch_bounds__(256)
           const float4 a = make_float4(st.acc.x * inv, st.acc.y * inv, st.acc.z * inv,
                                        st.acc.w * inv);
           if (gbuf_wanted(F) && !(st.depth < 1e31f)) gbuf_miss(F, size_t(st.pixel));
-          write_sample(F, x, y, accumID, a, st.depth, true);
+          if (P.taa) {
+            // TAA: record the sample for the resolve pass (fillGBuffer only,
+            // Pathtrace_impl.cpp:636-637); a pixel without hits keeps zero
+            // motion and albedo
+            const size_t i = size_t(st.pixel);
+            if (!(st.depth < 1e31f)) {
+              F.motion[i] = make_float4(0.f, 0.f, 0.f, 1.f);
+              F.taa_curr_alb[3 * i] = F.taa_curr_alb[3 * i + 1] = F.taa_curr_alb[3 * i + 2] = 0.f;
+            }
+            F.taa_curr[i] = a;
+            if (F.depth && st.depth <= F.depth[i]) F.depth[i] = st.depth;
+          } else {
+            write_sample(F, x, y, accumID, a, st.depth, true);
+          }
           st.pixel = -1;
         }
       }
     }
   }
   if (F.counters) wave_add_counter(&F.counters[0], lookups);
+}
+
+// TAA resolve (Frame.cpp:339-381, DeviceCopyableObjects.h:2657-2712): the
+// history (bilinear over the previous colours, clamp) at the motion-
+// compensated position, blended with the frame's sample by alpha -- 1 where
+// the albedo at the previous pixel differs by 1e-2 or more -- then tone
+// mapped
+__global__ void __launch_bounds__(256) k_taa_resolve(KFrame F, float alpha) {
+  const int n = F.width * F.height;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int x = i % F.width, y = i / F.width;
+  const float4 mv = F.motion[i];
+  int px = int(float(x) + mv.x), py = int(float(y) + mv.y);
+  px = min(max(px, 0), F.width - 1);
+  py = min(max(py, 0), F.height - 1);
+  const size_t pi = size_t(py) * F.width + px;
+  float a = alpha;
+  const float *pa = F.taa_prev_alb + 3 * pi, *ca = F.taa_curr_alb + 3 * size_t(i);
+  if (!(fabsf(pa[0] - ca[0]) < 1e-2f && fabsf(pa[1] - ca[1]) < 1e-2f &&
+        fabsf(pa[2] - ca[2]) < 1e-2f))
+    a = 1.f;
+  // tex2D at ((prevX + 0.5) / W, (prevY + 0.5) / H): texel coordinates prevX, prevY
+  const float tx = float(x) + mv.x, ty = float(y) + mv.y;
+  const float cx = fminf(fmaxf(tx, -1.f), float(F.width)), cy = fminf(fmaxf(ty, -1.f), float(F.height));
+  const float fx = floorf(cx), fy = floorf(cy);
+  const float wx = cx - fx, wy = cy - fy;
+  const int x0 = min(max(int(fx), 0), F.width - 1), x1 = min(max(int(fx) + 1, 0), F.width - 1);
+  const int y0 = min(max(int(fy), 0), F.height - 1), y1 = min(max(int(fy) + 1, 0), F.height - 1);
+  const float4 t00 = F.taa_prev[size_t(y0) * F.width + x0], t10 = F.taa_prev[size_t(y0) * F.width + x1];
+  const float4 t01 = F.taa_prev[size_t(y1) * F.width + x0], t11 = F.taa_prev[size_t(y1) * F.width + x1];
+  auto lp = [](float p, float q, float t) { return p + t * (q - p); };
+  const float4 h = make_float4(lp(lp(t00.x, t10.x, wx), lp(t01.x, t11.x, wx), wy),
+                               lp(lp(t00.y, t10.y, wx), lp(t01.y, t11.y, wx), wy),
+                               lp(lp(t00.z, t10.z, wx), lp(t01.z, t11.z, wx), wy),
+                               lp(lp(t00.w, t10.w, wx), lp(t01.w, t11.w, wx), wy));
+  const float4 s = F.taa_curr[i];
+  const float b = 1.f - a;
+  const float4 c = make_float4(b * h.x + a * s.x, b * h.y + a * s.y, b * h.z + a * s.z,
+                               b * h.w + a * s.w);
+  F.taa_curr[i] = c;
+  tone_map(F, size_t(i), c);
 }
 
 // Radiance-cache frame: start (counters) and resolve (cache term + frame
@@ -2420,6 +2513,18 @@ hipError_t launch_pathtrace(const KScene &sc, const KCamera &cam, const KParams 
                      dim3(256), lds, s, sc,
                      cam, P, F, accumID, queues, queues_next, nrc ? *nrc : KNrc{});
   return hipGetLastError();
+}
+
+hipError_t launch_taa_resolve(const KFrame &F, float alpha, hipStream_t s) {
+  const int n = F.width * F.height;
+  hipLaunchKernelGGL(k_taa_resolve, dim3((n + 255) / 256), dim3(256), 0, s, F, alpha);
+  hipError_t e = hipGetLastError();
+  // the history for the next frame (Frame.cpp:362-380)
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(F.taa_prev, F.taa_curr, size_t(n) * sizeof(float4), hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(F.taa_prev_alb, F.taa_curr_alb, size_t(n) * 12, hipMemcpyDeviceToDevice, s);
+  return e;
 }
 
 hipError_t launch_nrc_resolve(const KFrame &F, const KNrc &N, int accumID, hipStream_t s) {

@@ -306,6 +306,8 @@ int make_params(const vrt_params *p, KParams &k) {
     return fail(VRT_ERROR_INVALID_ARGUMENT, "phase_g %g out of (-1, 1)", double(p->phase_g));
   k.phase_g = p->phase_g;
   k.gradient_shading = p->gradient_shading ? 1 : 0;
+  k.taa = p->taa ? 1 : 0;
+  k.taa_alpha = p->taa_alpha;
   for (int i = 0; i < VRT_MAX_CLIP_PLANES; ++i) {
     const float *c = p->clip_planes[i];
     k.clip[i] = i < p->num_clip_planes ? make_float4(c[0], c[1], c[2], c[3])
@@ -331,6 +333,11 @@ int make_frame(const vrt_frame *f, KFrame &k) {
   k.depth = f->depth;
   k.accum64 = f->accum64;
   k.counters = f->counters;
+  k.motion = reinterpret_cast<float4 *>(f->motion_vec);
+  k.taa_curr = reinterpret_cast<float4 *>(f->taa_curr);
+  k.taa_prev = reinterpret_cast<float4 *>(f->taa_prev);
+  k.taa_curr_alb = f->taa_curr_albedo;
+  k.taa_prev_alb = f->taa_prev_albedo;
   k.normal = f->normal;
   k.albedo = f->albedo;
   k.prim_id = f->primitive_id;
@@ -661,6 +668,34 @@ int vrt_raycast(vrt_context ctx, const vrt_scene *scene, const vrt_camera *camer
   return VRT_OK;
 }
 
+// the camera's proj * view, row-major (pinhole_camera: perspective(fovy,
+// aspect, .001, 1000) and the view of eye / dir / up, camera/Perspective.cpp:
+// 36-40; visionaray's matrices, parity unpinned: gluPerspective, gluLookAt)
+int camera_view_proj(const vrt_camera *c, float M[16]) {
+  f3 f = mk(c->direction[0], c->direction[1], c->direction[2]);
+  f3 up = mk(c->up[0], c->up[1], c->up[2]);
+  const f3 eye = mk(c->position[0], c->position[1], c->position[2]);
+  if (!(length(f) > 0.f) || !(length(up) > 0.f))
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "camera direction/up must be non-zero");
+  f = normalize(f);
+  const f3 s = normalize(cross(f, up));
+  const f3 u = cross(s, f);
+  const float V[16] = {s.x, s.y, s.z, -dot(s, eye), u.x, u.y, u.z, -dot(u, eye),
+                       -f.x, -f.y, -f.z, dot(f, eye), 0.f, 0.f, 0.f, 1.f};
+  const float zn = .001f, zf = 1000.f;
+  const float g = 1.f / std::tan(c->fovy * 0.5f);
+  const float Pm[16] = {g / c->aspect, 0.f, 0.f, 0.f, 0.f, g, 0.f, 0.f,
+                        0.f, 0.f, (zf + zn) / (zn - zf), (2.f * zf * zn) / (zn - zf),
+                        0.f, 0.f, -1.f, 0.f};
+  for (int r = 0; r < 4; ++r)
+    for (int k = 0; k < 4; ++k) {
+      float acc = 0.f;
+      for (int j = 0; j < 4; ++j) acc += Pm[4 * r + j] * V[4 * j + k];
+      M[4 * r + k] = acc;
+    }
+  return VRT_OK;
+}
+
 int vrt_pathtrace(vrt_context ctx, const vrt_scene *scene, const vrt_camera *camera,
                   const vrt_params *params, const vrt_frame *frame, int accum_id, int frame_id) {
   (void)frame_id;
@@ -675,6 +710,16 @@ int vrt_pathtrace(vrt_context ctx, const vrt_scene *scene, const vrt_camera *cam
   if (int r = make_params(params, P)) return r;
   if (int r = make_frame(frame, F)) return r;
   if (!F.accum) return fail(VRT_ERROR_INVALID_ARGUMENT, "pathtrace needs an accumulation buffer");
+  if (P.taa) {
+    if (!F.motion || !F.taa_curr || !F.taa_prev || !F.taa_curr_alb || !F.taa_prev_alb)
+      return fail(VRT_ERROR_INVALID_ARGUMENT, "taa needs the frame's motion and taa buffers");
+    if (int r = camera_view_proj(camera, P.vp_curr)) return r;
+    if (params->has_prev_camera) {
+      if (int r = camera_view_proj(&params->prev_camera, P.vp_prev)) return r;
+    } else {
+      std::copy(P.vp_curr, P.vp_curr + 16, P.vp_prev);
+    }
+  }
   const size_t lds = pathtrace_lds_size(sc);
   int blocks = std::max(1, ctx->num_cus) * pathtrace_blocks_per_cu(sc, lds, false,
                                                                     P.gradient_shading != 0);
@@ -683,6 +728,7 @@ int vrt_pathtrace(vrt_context ctx, const vrt_scene *scene, const vrt_camera *cam
   uint32_t *nxt = ctx->queues + 8 * (1 - ctx->queue_parity);
   VRT_HIP(launch_pathtrace(sc, cam, P, F, accum_id, cur, nxt, blocks, ctx->stream));
   ctx->queue_parity ^= 1;
+  if (P.taa) VRT_HIP(launch_taa_resolve(F, P.taa_alpha, ctx->stream));
   return VRT_OK;
 }
 
@@ -706,6 +752,7 @@ int vrt_pathtrace_nrc(vrt_context ctx, const vrt_scene *scene, const vrt_camera 
   if (int r = make_scene(scene, sc)) return r;
   if (int r = make_camera(camera, cam)) return r;
   if (int r = make_params(params, P)) return r;
+  if (P.taa) return fail(VRT_ERROR_UNSUPPORTED, "taa with the radiance cache is not supported");
   if (int r = make_frame(frame, F)) return r;
   if (!F.accum) return fail(VRT_ERROR_INVALID_ARGUMENT, "pathtrace needs an accumulation buffer");
   if (P.pixel_samples != 1)

@@ -99,6 +99,8 @@ class Params(C.Structure):
         ("volume_sampling_rate", C.c_float), ("rng_offset", C.c_uint32),
         ("num_clip_planes", C.c_int), ("clip_planes", (C.c_float * 4) * 8),
         ("phase_g", C.c_float), ("gradient_shading", C.c_int),
+        ("taa", C.c_int), ("taa_alpha", C.c_float), ("has_prev_camera", C.c_int),
+        ("prev_camera", Camera),
     ]
 
 
@@ -109,6 +111,8 @@ class Frame(C.Structure):
         ("accum64", C.c_void_p), ("counters", C.c_void_p),
         ("normal", C.c_void_p), ("albedo", C.c_void_p), ("primitive_id", C.c_void_p),
         ("object_id", C.c_void_p), ("instance_id", C.c_void_p),
+        ("motion_vec", C.c_void_p), ("taa_curr", C.c_void_p), ("taa_prev", C.c_void_p),
+        ("taa_curr_albedo", C.c_void_p), ("taa_prev_albedo", C.c_void_p),
     ]
 
 
@@ -709,7 +713,7 @@ def camera(position, direction, up, fovy, aspect, region=(0, 0, 1, 1), shutter=(
 def params(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.2,
            occlusion_distance=1e20, ambient_samples=1, max_bounce=7, pixel_samples=1,
            volume_sampling_rate=0.5, rng_offset=0, clip_planes=(), phase_g=0.0,
-           gradient_shading=False):
+           gradient_shading=False, taa=False, taa_alpha=0.3, prev_camera=None):
     """Renderer parameters with the ANARI defaults of the reference
     (Renderer.cpp:42-53, Pathtrace.cpp:19-25, Raycast.cpp:19).  `clip_planes`:
     up to 8 (nx, ny, nz, d); rays keep n.x <= d.  `phase_g`: Henyey-Greenstein
@@ -718,6 +722,11 @@ def params(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.
     p.rng_offset = rng_offset
     p.phase_g = phase_g
     p.gradient_shading = int(bool(gradient_shading))
+    p.taa = int(bool(taa))
+    p.taa_alpha = taa_alpha
+    if prev_camera is not None:  # a Camera from camera()
+        p.has_prev_camera = 1
+        p.prev_camera = prev_camera
     if len(clip_planes) > 8:
         raise ValueError("at most 8 clip planes")
     p.num_clip_planes = len(clip_planes)
@@ -736,7 +745,7 @@ def params(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.
 
 class DeviceFrame:
     def __init__(self, ctx, width, height, color_type=TYPE_FLOAT32_VEC4, depth=False,
-                 accum64=False, counters=True, gbuffer=False):
+                 accum64=False, counters=True, gbuffer=False, taa=False):
         """gbuffer: also the normal, albedo and primitive / object / instance
         id channels (they need the depth channel, which it turns on)."""
         self.ctx = ctx
@@ -762,6 +771,14 @@ class DeviceFrame:
         f.accum64 = self.accum64.ptr if accum64 else None
         f.counters = self.counters.ptr if counters else None
         for k, b in self.gbuf.items():
+            setattr(f, k, b.ptr)
+        # temporal anti-aliasing: motion vectors, current / history colour and
+        # albedo, zeroed (the reference resizes them with zeros)
+        self.taa = {k: ctx.buffer(n * b) for k, b in
+                    (("motion_vec", 16), ("taa_curr", 16), ("taa_prev", 16),
+                     ("taa_curr_albedo", 12), ("taa_prev_albedo", 12))} if taa else {}
+        for k, b in self.taa.items():
+            b.zero()
             setattr(f, k, b.ptr)
         self.desc = f
         self.reset()

@@ -257,7 +257,7 @@ class GpuScene:
     """The workload resident on one GPU through the vrt C ABI."""
 
     def __init__(self, ctx, wl: Workload, color_type=None, depth=False, accum64=False,
-                 volume=None, params=None, bvh_builder="sah", gbuffer=False):
+                 volume=None, params=None, bvh_builder="sah", gbuffer=False, taa=False):
         self.wl = wl
         self.ctx = ctx
         if volume is None and wl.nanovdb is not None:
@@ -287,7 +287,7 @@ class GpuScene:
         self.params = _vrt.params(**p)
         self.frame = _vrt.DeviceFrame(ctx, wl.width, wl.height,
                                       color_type or _vrt.TYPE_FLOAT32_VEC4, depth, accum64,
-                                      gbuffer=gbuffer)
+                                      gbuffer=gbuffer, taa=taa)
 
     def render(self, accum_id=0):
         if self.wl.renderer == "raycast":

@@ -191,6 +191,17 @@ typedef struct vrt_params {
      that gradient as the volume normal for ambient occlusion and its NEE as
      matte (Pathtrace_impl.cpp:204-212, 311-328) */
   int gradient_shading;
+  /* path tracer "taa" / "taaAlpha" (Pathtrace.cpp:24-25, default 0 / 0.3):
+     the frame pass records the raw sample, its albedo and motion vector
+     (frame taa buffers below) and a resolve pass blends it with the history
+     at the motion-compensated position (Frame.cpp:339-381,
+     DeviceCopyableObjects.h:2693-2712).  Motion vectors project the hits
+     with the previous and the current camera (prev_camera; has_prev_camera
+     0 on the first frame: no motion). */
+  int taa;
+  float taa_alpha;
+  int has_prev_camera;
+  vrt_camera prev_camera;
 } vrt_params;
 
 /* ANARI data types used for frame channels (ANARI 1.0 enum values). */
@@ -222,6 +233,14 @@ typedef struct vrt_frame {
   uint32_t *primitive_id;
   uint32_t *object_id;
   uint32_t *instance_id;
+  /* temporal anti-aliasing (vrt_params.taa): per pixel the motion vector
+     (float4), the current and previous (history) colour (float4) and
+     albedo (float3); required when taa is set */
+  float *motion_vec;
+  float *taa_curr;
+  float *taa_prev;
+  float *taa_curr_albedo;
+  float *taa_prev_albedo;
 } vrt_frame;
 
 /* ---------------------------------------------------------------------- */

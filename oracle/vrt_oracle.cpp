@@ -1289,7 +1289,33 @@ template <class R> struct GBuf {
   R depth = R(1e31f);
   V3<R> Ng = V3<R>(R(0)), albedo = V3<R>(R(0));
   uint32_t prim = ~0u, obj = ~0u, inst = ~0u;
+  // TAA: motion vector of the sample's last radiance vertex, with the
+  // previous / current proj * view matrices it is measured with
+  float mvx = 0.f, mvy = 0.f;
+  const float *vp_prev = nullptr, *vp_curr = nullptr;
+  int W = 0, H = 0;
 };
+
+// project (visionaray, parity unpinned: gluProject) into window coordinates
+inline void project_window(const float *M, V3<float> p, int W, int H, float &wx, float &wy) {
+  const float x = ((M[0] * p.x + M[1] * p.y) + M[2] * p.z) + M[3];
+  const float y = ((M[4] * p.x + M[5] * p.y) + M[6] * p.z) + M[7];
+  const float w = ((M[12] * p.x + M[13] * p.y) + M[14] * p.z) + M[15];
+  wx = float(W) * (x / w + 1.f) / 2.f;
+  wy = float(H) * (y / w + 1.f) / 2.f;
+}
+
+// the motion vector of a radiance vertex (Pathtrace_impl.cpp:290-297:
+// every vertex overwrites it)
+template <class R> inline void gbuf_motion(GBuf<R> *gb, V3<R> hitPos) {
+  if (!gb || !gb->vp_curr) return;
+  const V3<float> p(float(hitPos.x), float(hitPos.y), float(hitPos.z));
+  float ax, ay, bx, by;
+  project_window(gb->vp_prev, p, gb->W, gb->H, ax, ay);
+  project_window(gb->vp_curr, p, gb->W, gb->H, bx, by);
+  gb->mvx = ax - bx;
+  gb->mvy = ay - by;
+}
 
 template <class R> struct ShadeState {
   V3<R> baseColor, shadedColor, hitPosOff, sn;
@@ -1422,6 +1448,7 @@ void shade(Random &srnd, const Ray<R> &ray, RayType rayType, const Scene<R> &sc,
       gb->Ng = gn;
       gb->albedo = color;
     }
+    gbuf_motion(gb, hitPos);
     st.sn = sn;
     st.hitPosOff = hitPos + sn * eps;
 
@@ -1563,9 +1590,16 @@ V4<R> pathtrace_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params
   Random srnd = rng;
   V4<R> accum{0, 0, 0, 0};
   const int spp = P->pixel_samples;
-  if (closest) *closest = GBuf<R>();
+  GBuf<R> proto;  // the TAA matrices, carried into every sample's record
+  if (closest) {
+    proto.vp_prev = closest->vp_prev;
+    proto.vp_curr = closest->vp_curr;
+    proto.W = closest->W;
+    proto.H = closest->H;
+    *closest = proto;
+  }
   for (int sampleID = 0; sampleID < spp; ++sampleID) {
-    GBuf<R> ps;
+    GBuf<R> ps = proto;
     R xf = R(x), yf = R(y);
     R jx = R(srnd()) - R(0.5);
     R jy = R(srnd()) - R(0.5);
@@ -2229,6 +2263,98 @@ void orc_raycast_gbuffer(const orc_scene *scene, const orc_camera *c, const orc_
     (void)raycast_pixel(sc, cam, P, x, y, W, H, &g);
     fill_gbuffer(g, size_t(y) * W + x, depth, normal, albedo, prim, obj, inst);
   });
+}
+
+// the camera's proj * view, row-major: perspective(fovy, aspect, .001, 1000)
+// and the view of eye / dir / up (camera/Perspective.cpp:36-40; visionaray's
+// pinhole matrices, parity unpinned: gluPerspective, gluLookAt)
+static void camera_view_proj(const orc_camera *c, float M[16]) {
+  auto nrm = [](V3<float> a) { return a * (1.f / length(a)); };
+  const V3<float> f = nrm(from<float>(c->dir)), up = from<float>(c->up), eye = from<float>(c->pos);
+  const V3<float> s = nrm(cross(f, up));
+  const V3<float> u = cross(s, f);
+  const float V[16] = {s.x, s.y, s.z, -dot(s, eye), u.x, u.y, u.z, -dot(u, eye),
+                       -f.x, -f.y, -f.z, dot(f, eye), 0.f, 0.f, 0.f, 1.f};
+  const float zn = .001f, zf = 1000.f;
+  const float g = 1.f / std::tan(c->fovy * 0.5f);
+  const float Pm[16] = {g / c->aspect, 0.f, 0.f, 0.f, 0.f, g, 0.f, 0.f,
+                        0.f, 0.f, (zf + zn) / (zn - zf), (2.f * zf * zn) / (zn - zf),
+                        0.f, 0.f, -1.f, 0.f};
+  for (int r = 0; r < 4; ++r)
+    for (int k = 0; k < 4; ++k) {
+      float acc = 0.f;
+      for (int j = 0; j < 4; ++j) acc += Pm[4 * r + j] * V[4 * j + k];
+      M[4 * r + k] = acc;
+    }
+}
+
+// One TAA frame (Pathtrace_impl.cpp:636-637, Frame.cpp:339-381,
+// DeviceCopyableObjects.h:2657-2712), FP32: every pixel's mean sample, the
+// closest sample's albedo and last-vertex motion vector (0 for a pixel
+// without hits), then the resolve against the history `prev_rgba` /
+// `prev_albedo` (read and replaced by this frame's, as the reference copies
+// them) into `out_rgba`.  prev_camera NULL: no motion.
+void orc_pathtrace_taa(const orc_scene *scene, const orc_camera *c, const orc_camera *prev_camera,
+                       const orc_params *P, int W, int H, int accumID, float alpha,
+                       float *prev_rgba, float *prev_albedo, float *out_rgba, float *motion,
+                       int threads) {
+  Scene<float> sc(scene);
+  Camera<float> cam(c);
+  float vp_curr[16], vp_prev[16];
+  camera_view_proj(c, vp_curr);
+  if (prev_camera) camera_view_proj(prev_camera, vp_prev);
+  else std::copy(vp_curr, vp_curr + 16, vp_prev);
+  const size_t n = size_t(W) * H;
+  std::vector<float> curr(4 * n), calb(3 * n), mv(2 * n);
+  parallel_tiles(W, H, threads, [&](int x, int y) {
+    GBuf<float> g;
+    g.vp_prev = vp_prev;
+    g.vp_curr = vp_curr;
+    g.W = W;
+    g.H = H;
+    const V4<float> s = pathtrace_pixel(sc, cam, P, x, y, W, H, accumID, nullptr, &g);
+    const size_t i = size_t(y) * W + x;
+    curr[4 * i] = s.x;
+    curr[4 * i + 1] = s.y;
+    curr[4 * i + 2] = s.z;
+    curr[4 * i + 3] = s.w;
+    const bool hit = g.depth < 1e31f;
+    mv[2 * i] = hit ? g.mvx : 0.f;
+    mv[2 * i + 1] = hit ? g.mvy : 0.f;
+    for (int k = 0; k < 3; ++k) calb[3 * i + k] = hit ? g.albedo[k] : 0.f;
+  });
+  parallel_tiles(W, H, threads, [&](int x, int y) {
+    const size_t i = size_t(y) * W + x;
+    const float mx = mv[2 * i], my = mv[2 * i + 1];
+    int px = int(float(x) + mx), py = int(float(y) + my);
+    px = std::min(std::max(px, 0), W - 1);
+    py = std::min(std::max(py, 0), H - 1);
+    const size_t pi = size_t(py) * W + px;
+    float a = alpha;
+    if (!(std::fabs(prev_albedo[3 * pi] - calb[3 * i]) < 1e-2f &&
+          std::fabs(prev_albedo[3 * pi + 1] - calb[3 * i + 1]) < 1e-2f &&
+          std::fabs(prev_albedo[3 * pi + 2] - calb[3 * i + 2]) < 1e-2f))
+      a = 1.f;
+    // tex2D(history, ((x + mx + .5) / W, (y + my + .5) / H)): bilinear, clamp
+    // (parity unpinned: visionaray's tex2D)
+    const float cx = std::fmin(std::fmax(float(x) + mx, -1.f), float(W));
+    const float cy = std::fmin(std::fmax(float(y) + my, -1.f), float(H));
+    const float fx = std::floor(cx), fy = std::floor(cy);
+    const float wx = cx - fx, wy = cy - fy;
+    const int x0 = std::min(std::max(int(fx), 0), W - 1), x1 = std::min(std::max(int(fx) + 1, 0), W - 1);
+    const int y0 = std::min(std::max(int(fy), 0), H - 1), y1 = std::min(std::max(int(fy) + 1, 0), H - 1);
+    auto lp = [](float p, float q, float t) { return p + t * (q - p); };
+    const float b = 1.f - a;
+    for (int k = 0; k < 4; ++k) {
+      const float h = lp(lp(prev_rgba[4 * (size_t(y0) * W + x0) + k], prev_rgba[4 * (size_t(y0) * W + x1) + k], wx),
+                         lp(prev_rgba[4 * (size_t(y1) * W + x0) + k], prev_rgba[4 * (size_t(y1) * W + x1) + k], wx),
+                         wy);
+      out_rgba[4 * i + k] = b * h + a * curr[4 * i + k];
+    }
+  });
+  std::copy(out_rgba, out_rgba + 4 * n, prev_rgba);
+  std::copy(calb.begin(), calb.end(), prev_albedo);
+  if (motion) std::copy(mv.begin(), mv.end(), motion);
 }
 
 void orc_raycast(const orc_scene *scene, const orc_camera *c, const orc_params *P, int W,

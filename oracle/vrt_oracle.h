@@ -243,6 +243,15 @@ void orc_pathtrace_gbuffer(const orc_scene *scene, const orc_camera *cam,
                            const orc_params *params, int width, int height, int accum_id,
                            float *depth, float *normal, float *albedo, uint32_t *prim,
                            uint32_t *obj, uint32_t *inst, int threads);
+/* --- one TAA frame (path tracer "taa", Frame.cpp:339-381), FP32: the
+   frame's samples resolved against the history prev_rgba (w*h float4) /
+   prev_albedo (w*h float3), which are then replaced by this frame's; the
+   output colours (float4) and motion vectors (float2, may be NULL).
+   prev_camera NULL: no motion. --- */
+void orc_pathtrace_taa(const orc_scene *scene, const orc_camera *cam,
+                       const orc_camera *prev_camera, const orc_params *params, int width,
+                       int height, int accum_id, float alpha, float *prev_rgba, float *prev_albedo,
+                       float *out_rgba, float *motion, int threads);
 void orc_raycast_gbuffer(const orc_scene *scene, const orc_camera *cam,
                          const orc_params *params, int width, int height, float *depth,
                          float *normal, float *albedo, uint32_t *prim, uint32_t *obj,

@@ -91,6 +91,9 @@ def lib():
         L.orc_pathtrace_gbuffer.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcCamera),
                                             C.POINTER(OrcParams), ip, ip, ip, vp, vp, vp, vp,
                                             vp, vp, ip]
+        L.orc_pathtrace_taa.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcCamera),
+                                         C.POINTER(OrcCamera), C.POINTER(OrcParams), ip, ip, ip,
+                                         C.c_float, vp, vp, vp, vp, ip]
         L.orc_raycast_gbuffer.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcCamera),
                                           C.POINTER(OrcParams), ip, ip, vp, vp, vp, vp, vp,
                                           vp, ip]
@@ -406,7 +409,9 @@ def camera(position, direction, up, fovy, aspect, region=(0, 0, 1, 1), shutter=(
 def params(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.2,
            occlusion_distance=1e20, ambient_samples=1, max_bounce=7, pixel_samples=1,
            volume_sampling_rate=0.5, rng_offset=0, clip_planes=(), phase_g=0.0,
-           gradient_shading=False):
+           gradient_shading=False, taa=False, taa_alpha=0.3):
+    # taa / taa_alpha: the oracle's TAA frames take them as arguments
+    # (OracleWorkload.pathtrace_taa)
     p = OrcParams()
     p.phase_g = phase_g
     p.gradient_shading = int(bool(gradient_shading))
@@ -501,6 +506,25 @@ class OracleWorkload:
                 lib().orc_pathtrace_gbuffer(C.byref(self.scene.s), C.byref(self.cam),
                                             C.byref(self.params), W, H, a, *ptrs, threads)
         return g
+
+    def taa_state(self):
+        """Zeroed TAA history (colour float4, albedo float3), as the reference
+        allocates it."""
+        W, H = self.wl.width, self.wl.height
+        return {"rgba": np.zeros((H, W, 4), np.float32), "albedo": np.zeros((H, W, 3), np.float32)}
+
+    def pathtrace_taa(self, accum_id, state, alpha=0.3, cam=None, prev_cam=None, threads=0):
+        """One TAA frame against `state` (updated in place): the resolved
+        image and the motion vectors (h, w, 2)."""
+        W, H = self.wl.width, self.wl.height
+        out = np.empty((H, W, 4), np.float32)
+        mv = np.empty((H, W, 2), np.float32)
+        lib().orc_pathtrace_taa(C.byref(self.scene.s), C.byref(cam or self.cam),
+                                C.byref(prev_cam) if prev_cam is not None else None,
+                                C.byref(self.params), W, H, accum_id, alpha,
+                                state["rgba"].ctypes.data, state["albedo"].ctypes.data,
+                                out.ctypes.data, mv.ctypes.data, threads)
+        return out, mv
 
     def pathtrace_accumulate(self, frames, first_accum_id=0, fp64=False, stride=1, threads=0):
         W, H = self.wl.width, self.wl.height

@@ -462,3 +462,43 @@ def test_frame_completion_callback(pkg, gpu_ctx):
         assert not errors, errors
     finally:
         a.close()
+
+
+def test_taa_frames_match_abi(pkg, gpu_ctx):
+    """Renderer "taa" / "taaAlpha" with the camera moved between frames by
+    commits: the resolved frames are those of the C ABI with the previous
+    camera passed along."""
+    vrt = pkg.vrt
+    wl = pkg.workloads.fbm_pathtrace(n=24, width=40, height=32, frames=1, brick=8)
+    wl = _same_tf(pkg, dataclasses.replace(
+        wl, params=dict(max_bounce=3, ambient_samples=1, taa=True, taa_alpha=0.25)))
+    poses = [np.array(wl.camera["position"]) + 0.015 * k * np.array([1.0, 0.4, 0.0])
+             for k in range(3)]
+    gs = pkg.workloads.GpuScene(gpu_ctx, wl, taa=True)
+    ref, prev = [], None
+    for k, p in enumerate(poses):
+        c = dict(wl.camera, position=tuple(p))
+        gs.camera = vrt.camera(**c)
+        gs.params = vrt.params(**wl.params, prev_camera=vrt.camera(**prev) if prev else None)
+        # every camera commit restarts the accumulation (accumID 0); the
+        # TAA history carries over
+        gs.frame.reset()
+        gs.render(0)
+        gpu_ctx.sync()
+        ref.append(gs.frame.color_array().copy())
+        prev = c
+    a = A.Anari()
+    try:
+        fr, objs = _anari_scene(a, wl)
+        a.set(objs["renderer"], "taa", A.BOOL, 1)
+        a.set(objs["renderer"], "taaAlpha", A.FLOAT32, 0.25)
+        a.commit(objs["renderer"])
+        for k, p in enumerate(poses):
+            a.set(objs["camera"], "position", A.FLOAT32_VEC3, p)
+            a.commit(objs["camera"])
+            a.render(fr)
+            np.testing.assert_array_equal(a.map(fr)[0], ref[k])
+        errors = [m for m in a.messages if m[0] <= 1]
+        assert not errors, errors
+    finally:
+        a.close()
