@@ -1716,9 +1716,9 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
     const bool gbuf = st.bounceID == 0 && gbuf_wanted(F) && h.t < st.depth &&
                       h.t <= F.depth[st.pixel];
     if (h.kind == kVolume) {
-      const KVolume &hv = sc.vol[h.id];
-      if (GRAD) {
+      if constexpr (GRAD) {
         // the gradient at the hit in voxel space (Pathtrace_impl.cpp:204-212)
+        const KVolume &hv = sc.vol[h.id];
         const KRay lr = to_local(hv, ray);
         gn = gradient_normal<ANYF>(hv, hv.S * ((lr.ori + lr.dir * h.t) + hv.T));
       }
@@ -1728,8 +1728,14 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
       }
       if (GRAD || P.ambient_samples > 0) {
         // normal transform transpose(affineInv) (Pathtrace_impl.cpp:217-219,
-        // Instance.cpp:96-98), as mat3 * vec3 = col0 x + col1 y + col2 z
-        const float *A = hv.A;
+        // Instance.cpp:96-98), as mat3 * vec3 = col0 x + col1 y + col2 z;
+        // the volume's matrix picked by a select loop (a divergent index into
+        // the kernel argument would copy it to scratch)
+        float A[9];
+        for (int k = 0; k < 9; ++k) A[k] = sc.vol[0].A[k];
+        for (int i = 1; i < sc.num_volumes; ++i)
+          if (i == h.id)
+            for (int k = 0; k < 9; ++k) A[k] = sc.vol[i].A[k];
         gn = mk(A[0] * gn.x + A[3] * gn.y + A[6] * gn.z, A[1] * gn.x + A[4] * gn.y + A[7] * gn.z,
                 A[2] * gn.x + A[5] * gn.y + A[8] * gn.z);
       }
@@ -1889,8 +1895,11 @@ __device__ inline float train_hash(uint32_t pixel, uint32_t accumID) {
 
 // GRAD: gradient shading (the renderer's gradientShading), compiled only
 // into the variants that use it
+// 3 waves per SIMD (<= 168 VGPRs): the variants with surfaces or other
+// field types would otherwise settle at 2 and lose a third of their latency
+// hiding; the compiler fits them without spilling
 template <bool NRC, bool AREA, bool ANYF, bool GRAD>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 3)
     k_pathtrace(KScene sc, KCamera cam, KParams P, KFrame F, int accumID, uint32_t *queues,
                 uint32_t *queues_next, KNrc N) {
   extern __shared__ float4 s_tf[];
