@@ -163,8 +163,7 @@ __global__ void __launch_bounds__(256)
 }
 
 __device__ inline void child_box(int c, const uint32_t *__restrict__ ids,
-                                 const float *__restrict__ tbox, const volatile float *nbox,
-                                 float b[6]) {
+                                 const float *__restrict__ tbox, const float *nbox, float b[6]) {
   if (c < 0) {
     const float *t = tbox + 6 * size_t(ids[-1 - c]);
     for (int k = 0; k < 6; ++k) b[k] = t[k];
@@ -173,30 +172,71 @@ __device__ inline void child_box(int c, const uint32_t *__restrict__ ids,
   }
 }
 
-// bottom-up boxes: from each leaf towards the root, the second arrival at
-// a node computes its box from both children (boxes written by other
-// workgroups are read through volatile, i.e. uncached, loads)
+// A child's key range shares a longer common prefix than its parent's (at
+// most 63 key bits plus 32 index bits), so the binary tree is at most 97
+// levels deep.
+constexpr int kMaxLevels = 128;
+
+// depth of each inner node (root 0) by walking its parent chain, and a
+// histogram of the depths (per workgroup in LDS, then one atomic per level)
 __global__ void __launch_bounds__(256)
-    k_bvh_refit(int64_t n, const uint32_t *__restrict__ ids, const float *__restrict__ tbox,
-                const int *__restrict__ left, const int *__restrict__ right,
-                const int *__restrict__ parent, const int *__restrict__ leaf_parent,
-                float *nbox, unsigned *__restrict__ visits) {
-  const int64_t k = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (k >= n) return;
-  int p = leaf_parent[k];
-  volatile float *vb = nbox;
-  while (p >= 0) {
-    __threadfence();  // release: this thread's child box before its arrival
-    if (atomicAdd(&visits[p], 1u) == 0u) return;
-    __threadfence();  // acquire: the sibling's box written before its arrival
-    float a[6], b[6];
-    child_box(left[p], ids, tbox, vb, a);
-    child_box(right[p], ids, tbox, vb, b);
-    for (int q = 0; q < 3; ++q) {
-      vb[6 * size_t(p) + q] = fminf(a[q], b[q]);
-      vb[6 * size_t(p) + 3 + q] = fmaxf(a[3 + q], b[3 + q]);
+    k_bvh_depth(int64_t ni, const int *__restrict__ parent, uint8_t *__restrict__ depth,
+                unsigned *__restrict__ level_count, unsigned *__restrict__ err) {
+  __shared__ unsigned hist[kMaxLevels];
+  for (int b = threadIdx.x; b < kMaxLevels; b += 256) hist[b] = 0;
+  __syncthreads();
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i < ni) {
+    int d = 0;
+    for (int p = parent[i]; p >= 0 && d < kMaxLevels; p = parent[p]) ++d;
+    if (d >= kMaxLevels) {
+      atomicOr(err, 1u);
+      d = kMaxLevels - 1;
     }
-    p = parent[p];
+    depth[i] = uint8_t(d);
+    atomicAdd(&hist[d], 1u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < kMaxLevels; b += 256)
+    if (hist[b]) atomicAdd(&level_count[b], hist[b]);
+}
+
+// inner nodes listed level by level: cursor[d] starts at level d's offset;
+// a workgroup reserves its run of each level with one atomic
+__global__ void __launch_bounds__(256)
+    k_bvh_bucket(int64_t ni, const uint8_t *__restrict__ depth, unsigned *__restrict__ cursor,
+                 int *__restrict__ order) {
+  __shared__ unsigned hist[kMaxLevels], base[kMaxLevels];
+  for (int b = threadIdx.x; b < kMaxLevels; b += 256) hist[b] = 0;
+  __syncthreads();
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  unsigned d = 0, local = 0;
+  if (i < ni) {
+    d = depth[i];
+    local = atomicAdd(&hist[d], 1u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < kMaxLevels; b += 256)
+    if (hist[b]) base[b] = atomicAdd(&cursor[b], hist[b]);
+  __syncthreads();
+  if (i < ni) order[base[d] + local] = int(i);
+}
+
+// boxes of one level's inner nodes from their children's (the deeper
+// levels were written by earlier launches)
+__global__ void __launch_bounds__(256)
+    k_bvh_refit_level(int count, const int *__restrict__ nodes, const uint32_t *__restrict__ ids,
+                      const float *__restrict__ tbox, const int *__restrict__ left,
+                      const int *__restrict__ right, float *nbox) {
+  const int w = blockIdx.x * 256 + threadIdx.x;
+  if (w >= count) return;
+  const int p = nodes[w];
+  float a[6], b[6];
+  child_box(left[p], ids, tbox, nbox, a);
+  child_box(right[p], ids, tbox, nbox, b);
+  for (int q = 0; q < 3; ++q) {
+    nbox[6 * size_t(p) + q] = fminf(a[q], b[q]);
+    nbox[6 * size_t(p) + 3 + q] = fmaxf(a[3 + q], b[3 + q]);
   }
 }
 
@@ -285,15 +325,29 @@ __global__ void __launch_bounds__(256)
   out[3 * k + 2] = in[3 * t + 2];
 }
 
+// the build's scratch buffers, carved from one device allocation (one
+// hipMalloc and one hipFree per build instead of one per buffer)
 struct Scratch {
-  std::vector<void *> ptrs;
-  hipError_t alloc(void **p, size_t bytes) {
-    hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 16));
-    if (e == hipSuccess) ptrs.push_back(*p);
-    return e;
+  void *base = nullptr;
+  std::vector<std::pair<void **, size_t>> parts;
+  template <class T>
+  void add(T **p, size_t bytes) {
+    parts.emplace_back(reinterpret_cast<void **>(p), (std::max<size_t>(bytes, 16) + 255) & ~size_t(255));
+  }
+  hipError_t commit() {
+    size_t total = 0;
+    for (auto &q : parts) total += q.second;
+    hipError_t e = hipMalloc(&base, total);
+    if (e != hipSuccess) return e;
+    char *c = static_cast<char *>(base);
+    for (auto &q : parts) {
+      *q.first = c;
+      c += q.second;
+    }
+    return hipSuccess;
   }
   ~Scratch() {
-    for (void *p : ptrs) (void)hipFree(p);
+    if (base) (void)hipFree(base);
   }
 };
 
@@ -317,21 +371,44 @@ int vrt_bvh_build_gpu(vrt_context ctx, const float *vertices, int64_t num_vertic
   Scratch S;
   float4 *tris_in = nullptr;
   float *tbox = nullptr, *cen = nullptr, *nbox = nullptr;
-  unsigned *bounds = nullptr, *err = nullptr, *visits = nullptr, *counters = nullptr;
+  unsigned *bounds = nullptr, *err = nullptr, *levels = nullptr, *counters = nullptr;
+  uint8_t *depth8 = nullptr;
+  int *order = nullptr;
   uint64_t *keys = nullptr, *keys_sorted = nullptr;
   uint32_t *ids = nullptr, *ids_sorted = nullptr;
   int *left = nullptr, *right = nullptr, *first = nullptr, *last = nullptr, *parent = nullptr,
       *leaf_parent = nullptr;
   int2 *work[2] = {nullptr, nullptr};
-  hipError_t e = S.alloc((void **)&tris_in, size_t(n) * 48);
-  if (e == hipSuccess) e = S.alloc((void **)&tbox, size_t(n) * 24);
-  if (e == hipSuccess) e = S.alloc((void **)&cen, size_t(n) * 12);
-  if (e == hipSuccess) e = S.alloc((void **)&bounds, 6 * 4);
-  if (e == hipSuccess) e = S.alloc((void **)&err, 4);
-  if (e == hipSuccess) e = S.alloc((void **)&keys, size_t(n) * 8);
-  if (e == hipSuccess) e = S.alloc((void **)&keys_sorted, size_t(n) * 8);
-  if (e == hipSuccess) e = S.alloc((void **)&ids, size_t(n) * 4);
-  if (e == hipSuccess) e = S.alloc((void **)&ids_sorted, size_t(n) * 4);
+  const int64_t ni = std::max<int64_t>(n - 1, 1);
+  size_t temp_bytes = 0;
+  void *temp = nullptr;
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, keys, keys_sorted, ids,
+                                                    ids_sorted, int(n), 0, 63, s);
+  if (e != hipSuccess) return hip_fail(e, "bvh sort");
+  S.add(&tris_in, size_t(n) * 48);
+  S.add(&tbox, size_t(n) * 24);
+  S.add(&cen, size_t(n) * 12);
+  S.add(&bounds, 6 * 4);
+  S.add(&err, 4);
+  S.add(&keys, size_t(n) * 8);
+  S.add(&keys_sorted, size_t(n) * 8);
+  S.add(&ids, size_t(n) * 4);
+  S.add(&ids_sorted, size_t(n) * 4);
+  S.add(&temp, temp_bytes);
+  S.add(&left, size_t(ni) * 4);
+  S.add(&right, size_t(ni) * 4);
+  S.add(&first, size_t(ni) * 4);
+  S.add(&last, size_t(ni) * 4);
+  S.add(&parent, size_t(ni) * 4);
+  S.add(&leaf_parent, size_t(n) * 4);
+  S.add(&nbox, size_t(ni) * 24);
+  S.add(&levels, 2 * kMaxLevels * 4);
+  S.add(&depth8, size_t(ni));
+  S.add(&order, size_t(ni) * 4);
+  S.add(&counters, 8);
+  S.add(&work[0], size_t(n) * 8);
+  S.add(&work[1], size_t(n) * 8);
+  e = S.commit();
   if (e != hipSuccess) return hip_fail(e, "bvh scratch");
   const unsigned init_bounds[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
   e = hipMemcpyAsync(bounds, init_bounds, sizeof(init_bounds), hipMemcpyHostToDevice, s);
@@ -348,39 +425,47 @@ int vrt_bvh_build_gpu(vrt_context ctx, const float *vertices, int64_t num_vertic
   float3 lo = make_float3(from_ord(hb[0]), from_ord(hb[1]), from_ord(hb[2]));
   float3 ext = make_float3(from_ord(hb[3]) - lo.x, from_ord(hb[4]) - lo.y, from_ord(hb[5]) - lo.z);
   hipLaunchKernelGGL(k_bvh_morton, dim3(blocks), dim3(256), 0, s, n, cen, lo, ext, keys, ids);
-  size_t temp_bytes = 0;
-  e = hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, keys, keys_sorted, ids, ids_sorted,
+  e = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys, keys_sorted, ids, ids_sorted,
                                          int(n), 0, 63, s);
-  void *temp = nullptr;
-  if (e == hipSuccess) e = S.alloc(&temp, temp_bytes);
-  if (e == hipSuccess)
-    e = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys, keys_sorted, ids, ids_sorted,
-                                           int(n), 0, 63, s);
   if (e != hipSuccess) return hip_fail(e, "bvh sort");
 
-  const int64_t ni = std::max<int64_t>(n - 1, 1);
-  e = S.alloc((void **)&left, size_t(ni) * 4);
-  if (e == hipSuccess) e = S.alloc((void **)&right, size_t(ni) * 4);
-  if (e == hipSuccess) e = S.alloc((void **)&first, size_t(ni) * 4);
-  if (e == hipSuccess) e = S.alloc((void **)&last, size_t(ni) * 4);
-  if (e == hipSuccess) e = S.alloc((void **)&parent, size_t(ni) * 4);
-  if (e == hipSuccess) e = S.alloc((void **)&leaf_parent, size_t(n) * 4);
-  if (e == hipSuccess) e = S.alloc((void **)&nbox, size_t(ni) * 24);
-  if (e == hipSuccess) e = S.alloc((void **)&visits, size_t(ni) * 4);
-  if (e == hipSuccess) e = S.alloc((void **)&counters, 8);
-  if (e == hipSuccess) e = S.alloc((void **)&work[0], size_t(n) * 8);
-  if (e == hipSuccess) e = S.alloc((void **)&work[1], size_t(n) * 8);
-  if (e != hipSuccess) return hip_fail(e, "bvh scratch");
   // parent of the root and of a lone leaf: -1
   e = hipMemsetAsync(parent, 0xFF, size_t(ni) * 4, s);
   if (e == hipSuccess) e = hipMemsetAsync(leaf_parent, 0xFF, size_t(n) * 4, s);
-  if (e == hipSuccess) e = hipMemsetAsync(visits, 0, size_t(ni) * 4, s);
+  if (e == hipSuccess) e = hipMemsetAsync(levels, 0, kMaxLevels * 4, s);
+  if (e == hipSuccess) e = hipMemsetAsync(err, 0, 4, s);
   if (e != hipSuccess) return hip_fail(e, "bvh init");
   if (n > 1) {
-    hipLaunchKernelGGL(k_bvh_hierarchy, dim3(unsigned((n - 1 + 255) / 256)), dim3(256), 0, s, n,
-                       keys_sorted, left, right, first, last, parent, leaf_parent);
-    hipLaunchKernelGGL(k_bvh_refit, dim3(blocks), dim3(256), 0, s, n, ids_sorted, tbox, left,
-                       right, parent, leaf_parent, nbox, visits);
+    const unsigned iblocks = unsigned((ni + 255) / 256);
+    hipLaunchKernelGGL(k_bvh_hierarchy, dim3(iblocks), dim3(256), 0, s, n, keys_sorted, left,
+                       right, first, last, parent, leaf_parent);
+    // bottom-up boxes, one launch per level from the deepest to the root
+    hipLaunchKernelGGL(k_bvh_depth, dim3(iblocks), dim3(256), 0, s, ni, parent, depth8, levels,
+                       err);
+    unsigned hl[kMaxLevels + 1];
+    e = hipMemcpyAsync(hl, levels, kMaxLevels * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(hl + kMaxLevels, err, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "bvh levels");
+    if (hl[kMaxLevels]) return fail(VRT_ERROR_UNSUPPORTED, "bvh deeper than its key bits allow");
+    unsigned off[kMaxLevels];
+    unsigned sum = 0;
+    for (int d = 0; d < kMaxLevels; ++d) {
+      off[d] = sum;
+      sum += hl[d];
+    }
+    if (sum != unsigned(ni)) return fail(VRT_ERROR_UNSUPPORTED, "bvh level counts do not add up");
+    e = hipMemcpyAsync(levels + kMaxLevels, off, sizeof(off), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_fail(e, "bvh levels");
+    hipLaunchKernelGGL(k_bvh_bucket, dim3(iblocks), dim3(256), 0, s, ni, depth8,
+                       levels + kMaxLevels, order);
+    for (int d = kMaxLevels - 1; d >= 0; --d)
+      if (hl[d])
+        hipLaunchKernelGGL(k_bvh_refit_level, dim3((hl[d] + 255) / 256), dim3(256), 0, s,
+                           int(hl[d]), order + off[d], ids_sorted, tbox, left, right, nbox);
+    // the host copy of the offsets must outlive the asynchronous upload
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "bvh refit");
   }
 
   auto *h = new vrt_bvh_s;
