@@ -178,6 +178,7 @@ struct KNrc {
   uint32_t *ctr;         // [0] queries [1] training claims [2] vertex-2 paths [3] previous [2]
   int q_cap, t_cap;
   int train;             // record training paths this frame
+  int qbounce;           // bounce index of the querying volume vertex (0 or 1)
   f3 pos_lo, pos_scale;  // world -> [0,1]^3 cache coordinates
 };
 

@@ -1625,8 +1625,9 @@ __device__ inline void taa_vertex(const KFrame &F, const KParams &P, const PathS
 // One pass of the bounce loop (Pathtrace_impl.cpp:558-600) after the ray's
 // intersection `h`: shade, advance the state machine, set up the next ray.
 // Returns true when the sample's path ended.  With NRC, a path reaching its
-// second volume vertex either becomes a cache query and ends, or -- a
-// training path -- records the vertex and goes on.
+// query vertex (the second volume vertex, or the first) either becomes a
+// cache query and ends, or -- a training path -- records the vertex and goes
+// on.
 template <bool NRC, bool AREA, bool ANYF, bool GRAD>
 __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams &P,
                                   const KNrc &N, const KFrame &F, const PassHit &h) {
@@ -1643,7 +1644,7 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
     st.taa_take = P.taa && (h.kind == kVolume || h.kind == kSurface) && h.t < st.depth;
   }
   if constexpr (NRC) {
-    const bool v2 = st.rayType == Radiance && h.kind == kVolume && st.bounceID == 1;
+    const bool v2 = st.rayType == Radiance && h.kind == kVolume && st.bounceID == N.qbounce;
     (void)wave_claim(&N.ctr[2], v2);  // vertex-2 paths of this frame
     const bool want_train = v2 && st.train_sel;
     const uint32_t tslot = wave_claim(&N.ctr[1], want_train);

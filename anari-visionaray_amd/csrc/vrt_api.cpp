@@ -745,6 +745,9 @@ int vrt_pathtrace_nrc(vrt_context ctx, const vrt_scene *scene, const vrt_camera 
                 opts->train_samples);
   if (opts->train && opts->train_steps < 1)
     return fail(VRT_ERROR_INVALID_ARGUMENT, "train_steps must be >= 1 when training");
+  if (opts->query_vertex < 0 || opts->query_vertex > 2)
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "query_vertex must be 1 or 2 (0: 2)");
+  const int qbounce = (opts->query_vertex == 0 ? 2 : opts->query_vertex) - 1;
   KScene sc;
   KCamera cam;
   KParams P;
@@ -757,6 +760,8 @@ int vrt_pathtrace_nrc(vrt_context ctx, const vrt_scene *scene, const vrt_camera 
   if (!F.accum) return fail(VRT_ERROR_INVALID_ARGUMENT, "pathtrace needs an accumulation buffer");
   if (P.pixel_samples != 1)
     return fail(VRT_ERROR_UNSUPPORTED, "the radiance-cache path tracer takes one sample per pixel");
+  if (qbounce == 0 && F.depth && (F.normal || F.albedo || F.prim_id || F.obj_id || F.inst_id))
+    return fail(VRT_ERROR_UNSUPPORTED, "g-buffer channels with first-vertex cache queries");
   const size_t pixels = size_t(F.width) * F.height;
   if (int r = nrc_frame_buffers(nrc, pixels, std::max(1, opts->train_samples))) return r;
   KNrc N{};
@@ -771,6 +776,7 @@ int vrt_pathtrace_nrc(vrt_context ctx, const vrt_scene *scene, const vrt_camera 
   N.q_cap = int(pixels);
   N.t_cap = opts->train ? std::max(0, opts->train_samples) : 0;
   N.train = opts->train && opts->train_samples > 0;
+  N.qbounce = qbounce;
   N.pos_lo = mk(opts->bounds_lo[0], opts->bounds_lo[1], opts->bounds_lo[2]);
   for (int a = 0; a < 3; ++a)
     if (!(opts->bounds_hi[a] > opts->bounds_lo[a]))

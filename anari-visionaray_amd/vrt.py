@@ -127,7 +127,7 @@ class NrcConfig(C.Structure):
 class NrcRender(C.Structure):
     _fields_ = [
         ("train", C.c_int), ("train_samples", C.c_int), ("train_steps", C.c_int),
-        ("bounds_lo", f3), ("bounds_hi", f3),
+        ("bounds_lo", f3), ("bounds_hi", f3), ("query_vertex", C.c_int),
     ]
 
 
@@ -936,8 +936,10 @@ class Nrc:
             self.handle = None
 
 
-def nrc_render_options(bounds_lo, bounds_hi, train=True, train_samples=65536, train_steps=4):
+def nrc_render_options(bounds_lo, bounds_hi, train=True, train_samples=65536, train_steps=4,
+                       query_vertex=2):
     o = NrcRender()
+    o.query_vertex = int(query_vertex)
     o.train = int(train)
     o.train_samples = int(train_samples)
     o.train_steps = int(train_steps)

@@ -74,7 +74,7 @@ PRECISION_NOTE = {"fp16": "fp16 weights/activations, fp32 accumulation, fp32 Ada
 
 def nrc_vs_plain(pkg, ctx, wl, volume=None, ref_frames=2048, cap=1024, warmup=4, train_steps=8,
                  target=1e-2, oracle_stride=24, oracle_spp=4096, timed_frames=32,
-                 precisions=("fp16",)):
+                 precisions=("fp16",), query_vertex=1, train_samples=32768):
     """Wall time to relMSE <= target of the plain path tracer and of the
     radiance-cache path tracer (its training warm-up included) for each cache
     precision, all measured against an FP64-accumulated plain reference of
@@ -110,7 +110,8 @@ def nrc_vs_plain(pkg, ctx, wl, volume=None, ref_frames=2048, cap=1024, warmup=4,
     # same images) without the relMSE checks and timed on HIP events
     warm = wls.GpuScene(ctx, wl, volume=volume, params={"rng_offset": 2 << 20})
     gn = wls.GpuScene(ctx, wl, volume=volume, params={"rng_offset": 3 << 20})
-    opts = warm.nrc_options(train=True, train_samples=65536, train_steps=train_steps)
+    opts = warm.nrc_options(train=True, train_samples=train_samples, train_steps=train_steps,
+                            query_vertex=query_vertex)
 
     def nrc_run(precision, frames=None):
         nrc = vrt.Nrc(ctx, precision=precision, max_batch=16384, seed=1)
@@ -147,11 +148,12 @@ def nrc_vs_plain(pkg, ctx, wl, volume=None, ref_frames=2048, cap=1024, warmup=4,
         _, _, t_nrc, _ = nrc_run(code, frames_nrc)
         nrc_ms = t_nrc / (warmup + frames_nrc)
         key = "nrc" if i == 0 else f"nrc_{prec}"
-        out[key] = {"warmup_frames": warmup, "frames": frames_nrc,
+        out[key] = {"query_vertex": query_vertex, "warmup_frames": warmup, "frames": frames_nrc,
                     "ms_per_frame": round(nrc_ms, 4),
                     "ms_to_target": round(t_nrc, 2) if nrc_reached else None, "relmse": e_nrc,
                     "queries_last_frame": counts[0],
-                    "training_samples_last_frame": min(counts[1], 65536),
+                    "training_samples_last_frame": min(counts[1], train_samples),
+                    "train_steps_per_frame": train_steps,
                     "precision": PRECISION_NOTE[prec]}
         out["speedup" if i == 0 else f"speedup_{prec}"] = (
             round(t_plain / t_nrc, 3) if plain_reached and nrc_reached else None)
@@ -272,6 +274,8 @@ def main():
     ap.add_argument("--no-nrc", action="store_true",
                     help="skip the time-to-relMSE comparison with the radiance cache")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--nrc-query-vertex", type=int, default=1, choices=(1, 2),
+                    help="volume vertex whose cache query ends a path")
     ap.add_argument("--ttr-size", type=int, default=1024,
                     help="index-space edge of the nanovdb cloud of the time-to-relMSE run")
     ap.add_argument("--no-mixed", action="store_true",
@@ -342,7 +346,8 @@ def main():
                                           phase_g=PHASE_G)
         print(f"# generated {wl3.name} in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
               flush=True)
-        nrc = nrc_vs_plain(pkg, ctx, wl3, oracle_stride=0 if args.no_cpu else 24)
+        nrc = nrc_vs_plain(pkg, ctx, wl3, oracle_stride=0 if args.no_cpu else 24,
+                           query_vertex=args.nrc_query_vertex)
         print(f"# time to relMSE 1e-2: {json.dumps(nrc)}", file=sys.stderr, flush=True)
 
     other = {"raycast_blobs": raycast_frame_time(pkg, ctx, cpu=not args.no_cpu)}
@@ -368,7 +373,7 @@ def main():
         inf = gs4.volume.field.info
         setup = time.perf_counter() - t0
         c4 = nrc_vs_plain(pkg, ctx, wl4, volume=gs4.volume, oracle_stride=0,
-                          precisions=("fp16", "bf16"))
+                          precisions=("fp16", "bf16"), query_vertex=args.nrc_query_vertex)
         c4["field"] = {"tetrahedra": int(inf.num_cells), "grid_dims": list(inf.grid_dims),
                        "mean_list": round(inf.mean_list, 2),
                        "expected_tests": round(inf.expected_tests, 3),

@@ -519,10 +519,11 @@ VRT_API int vrt_nrc_gradients(vrt_nrc nrc, int n, const float *raw, const float 
 VRT_API int vrt_nrc_train_step(vrt_nrc nrc, int n, const float *raw, const float *target,
                                float *loss);
 /* 'pathtrace' frame with the radiance cache.  Paths end at their second
- * volume vertex in a cache query (the cache supplies the radiance leaving
- * that vertex towards the path); a fraction of paths, adapted on the GPU to
- * give `train_samples` per frame, are traced to full length instead and
- * leave one training sample (vertex-2 inputs, suffix radiance / throughput).
+ * volume vertex (or their first: query_vertex 1) in a cache query (the cache
+ * supplies the radiance leaving that vertex towards the path); a fraction of
+ * paths, adapted on the GPU to give `train_samples` per frame, are traced to
+ * full length instead and leave one training sample (the query vertex's
+ * inputs, suffix radiance / throughput).
  * After the frame is resolved the cache takes `train_steps` Adam steps on
  * those samples.  Requires params->pixel_samples == 1.  The cache inputs
  * map world positions in [bounds_lo, bounds_hi] onto [0,1]^3. */
@@ -532,6 +533,8 @@ typedef struct vrt_nrc_render {
   int train_steps;    /* Adam steps (mini-batches) per frame */
   float bounds_lo[3];
   float bounds_hi[3];
+  int query_vertex;   /* volume vertex that queries the cache: 2 (0 = default) or 1;
+                         1 writes no g-buffer channels (rejected when the frame has them) */
 } vrt_nrc_render;
 
 VRT_API int vrt_pathtrace_nrc(vrt_context ctx, const vrt_scene *scene,

@@ -4,7 +4,9 @@
                         cache replaces exactly the radiance from vertex 2 on
   constant cache c   adds throughput(vertex 2) * c = albedo * c per query
   all-training frame == plain full-length path tracer (bit-exact): training
-                        paths are the plain paths
+                        paths are the plain paths (queries at vertex 2 or 1)
+  first-vertex query  a constant cache c adds exactly c per query (throughput
+                        1 before the first scattering)
   learning gain      after online training the cached estimator is far
                         closer to the converged image than the truncated
                         estimator without the cache
@@ -74,14 +76,17 @@ def test_constant_cache_adds_throughput_times_value(pkg, gpu_ctx):
         nrc.close()
 
 
-def test_all_training_frame_equals_plain(pkg, gpu_ctx):
+@pytest.mark.parametrize("query_vertex", [2, 1])
+def test_all_training_frame_equals_plain(pkg, gpu_ctx, query_vertex):
     vrt, wls = pkg.vrt, pkg.workloads
     nrc = vrt.Nrc(gpu_ctx, max_batch=65536)
     try:
         gs = wls.GpuScene(gpu_ctx, _wl(pkg))
         ref = wls.GpuScene(gpu_ctx, _wl(pkg))
-        # first frame: no previous count, every vertex-2 path trains
-        opts = gs.nrc_options(train=True, train_samples=65536, train_steps=1)
+        # first frame: no previous count, every path reaching the query
+        # vertex trains
+        opts = gs.nrc_options(train=True, train_samples=65536, train_steps=1,
+                              query_vertex=query_vertex)
         g = _frame(gs, lambda k: gs.render_nrc(nrc, opts, k), 0)
         r = _frame(ref, ref.render, 0)
         np.testing.assert_array_equal(g, r)
@@ -91,9 +96,55 @@ def test_all_training_frame_equals_plain(pkg, gpu_ctx):
         nrc.close()
 
 
-def test_online_training_gain(pkg, gpu_ctx):
+def test_first_vertex_constant_cache(pkg, gpu_ctx):
+    """Queries at the first volume vertex: the path has not scattered yet
+    (throughput 1) and gathered nothing, so a constant cache c adds exactly c
+    to every queried pixel over the zero cache's frame, which holds only the
+    background of the pixels that miss the volume."""
+    vrt, wls = pkg.vrt, pkg.workloads
+    nrc = vrt.Nrc(gpu_ctx)
+    try:
+        gs = wls.GpuScene(gpu_ctx, _wl(pkg))
+        opts = gs.nrc_options(train=False, query_vertex=1)
+        nrc.set_params(np.zeros(nrc.nparams, np.float32))
+        z = _frame(gs, lambda k: gs.render_nrc(nrc, opts, k), 0)
+        c = np.array([0.25, 0.5, 1.0], np.float32)
+        p = np.zeros(nrc.nparams, np.float32)
+        p[NW + 5 * 64:NW + 5 * 64 + 3] = c  # output biases
+        nrc.set_params(p)
+        g = _frame(gs, lambda k: gs.render_nrc(nrc, opts, k), 0)
+        d = (g - z)[..., :3]
+        queried = np.abs(d[..., 2]) > 1e-6
+        np.testing.assert_allclose(d[queried], np.broadcast_to(c, d[queried].shape), rtol=1e-6)
+        assert np.all(d[~queried] == 0)
+        q, t, v1, _ = nrc.frame_counts()
+        assert queried.sum() == q == v1 > 100 and t == 0
+        assert np.all(z[queried][..., :3] == 0)
+    finally:
+        nrc.close()
+
+
+def test_first_vertex_queries_reject_gbuffer(pkg, gpu_ctx):
+    vrt, wls = pkg.vrt, pkg.workloads
+    nrc = vrt.Nrc(gpu_ctx)
+    try:
+        gs = wls.GpuScene(gpu_ctx, _wl(pkg), gbuffer=True)
+        with pytest.raises(vrt.VrtError, match="g-buffer"):
+            gs.render_nrc(nrc, gs.nrc_options(train=False, query_vertex=1), 0)
+        gs.render_nrc(nrc, gs.nrc_options(train=False, query_vertex=2), 0)
+        with pytest.raises(vrt.VrtError, match="query_vertex"):
+            gs.render_nrc(nrc, gs.nrc_options(train=False, query_vertex=3), 0)
+    finally:
+        nrc.close()
+
+
+@pytest.mark.parametrize("query_vertex", [2, 1])
+def test_online_training_gain(pkg, gpu_ctx, query_vertex):
     """relMSE against the converged plain image: the trained cache beats the
-    same truncated estimator without a cache by a wide margin."""
+    truncated estimator without a cache (its paths end after the first
+    vertex's light sample) by a wide margin, whether it is queried at the
+    second volume vertex or at the first, where it also stands in for that
+    light sample."""
     vrt, wls = pkg.vrt, pkg.workloads
     wl = _wl(pkg, n=48, w=128, h=96)
     ref = wls.GpuScene(gpu_ctx, wl, accum64=True)
@@ -114,7 +165,8 @@ def test_online_training_gain(pkg, gpu_ctx):
     nrc = vrt.Nrc(gpu_ctx, max_batch=4096, seed=7)
     try:
         gs = wls.GpuScene(gpu_ctx, wl)
-        opts = gs.nrc_options(train=True, train_samples=4096, train_steps=2)
+        opts = gs.nrc_options(train=True, train_samples=4096, train_steps=2,
+                              query_vertex=query_vertex)
         for a in range(300):  # warm-up training, frames discarded
             gs.render_nrc(nrc, opts, 10000 + a)
         gs.frame.reset()
@@ -124,5 +176,6 @@ def test_online_training_gain(pkg, gpu_ctx):
         e_nrc = relmse(gs.frame.accum_array(), truth)
     finally:
         nrc.close()
-    print(f"relMSE truncated {e_trunc:.4f}  with trained cache {e_nrc:.4f}")
+    print(f"relMSE truncated {e_trunc:.4f}  with trained cache {e_nrc:.4f} "
+          f"(queries at vertex {query_vertex})")
     assert e_nrc < 0.3 * e_trunc
