@@ -106,8 +106,10 @@ def nrc_vs_plain(pkg, ctx, wl, volume=None, ref_frames=2048, cap=1024, warmup=4,
     plain_reached = e <= target
 
     # radiance cache: train from scratch for `warmup` frames, then accumulate
-    # until the target; the run is then repeated from scratch (same seeds,
-    # same images) without the relMSE checks and timed on HIP events
+    # until the target; the run is then repeated from scratch for that many
+    # frames (same seeds; the training paths claim their slots by atomics, so
+    # the mini-batch order and the cache can differ slightly) without the
+    # relMSE checks and timed on HIP events
     warm = wls.GpuScene(ctx, wl, volume=volume, params={"rng_offset": 2 << 20})
     gn = wls.GpuScene(ctx, wl, volume=volume, params={"rng_offset": 3 << 20})
     opts = warm.nrc_options(train=True, train_samples=train_samples, train_steps=train_steps,
