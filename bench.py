@@ -264,6 +264,60 @@ def calibrate_copy(vrt, mib, reps):
     ctx.close()
 
 
+KERNEL_WORKLOADS = ("pathtrace_fbm512", "raycast_blobs512", "raycast_blobs128",
+                    "pathtrace_cloud1024", "nrc_cloud1024", "pathtrace_tets2m", "nrc_tets2m")
+
+
+def kernel_only(pkg, ctx, name, steps, warmup, precision="fp16", query_vertex=2,
+                train_samples=65536, train_steps=4):
+    """One workload's frames alone (profiling runs): `warmup` untimed frames,
+    then `steps` frames timed on HIP events; prints one JSON line."""
+    vrt, wls = pkg.vrt, pkg.workloads
+    if name == "pathtrace_fbm512":
+        wl = wls.fbm_pathtrace(n=512, phase_g=PHASE_G)
+    elif name == "raycast_blobs512":
+        wl = wls.blobs_raycast(n=512, width=1920, height=1080)
+    elif name == "raycast_blobs128":
+        wl = wls.blobs_raycast(n=128)
+    elif name in ("pathtrace_cloud1024", "nrc_cloud1024"):
+        wl = wls.cloud_nanovdb_pathtrace(n=1024, phase_g=PHASE_G)
+    else:
+        wl = wls.tets_pathtrace()
+    gs = wls.GpuScene(ctx, wl)
+    nrc = opts = None
+    if name.startswith("nrc_"):
+        nrc = vrt.Nrc(ctx, precision={"fp16": vrt.NRC_FP16, "bf16": vrt.NRC_BF16}[precision],
+                      max_batch=16384, seed=1)
+        opts = gs.nrc_options(train=True, train_samples=train_samples, train_steps=train_steps,
+                              query_vertex=query_vertex)
+
+    def frame(a):
+        if nrc is not None:
+            gs.render_nrc(nrc, opts, a)
+        else:
+            gs.render(a)
+
+    gs.frame.reset()
+    for a in range(warmup):
+        frame(a)
+    ctx.sync()
+    gs.frame.counters.zero()
+    ctx.sync()
+    ctx.timer_start()
+    for a in range(steps):
+        frame(warmup + a)
+    ms = ctx.timer_stop() / steps
+    lookups = int(gs.frame.counter_values()[0]) / steps
+    out = {"kernel_workload": name, "workload": wl.name, "frames": steps,
+           "ms_per_frame": round(ms, 4), "lookups_per_frame": lookups,
+           "algorithmic_GBps": round(lookups * BYTES_PER_LOOKUP / (ms / 1e3) / 1e9, 1)}
+    if nrc is not None:
+        out.update(precision=precision, query_vertex=query_vertex, train_samples=train_samples,
+                   train_steps=train_steps, counts=list(nrc.frame_counts()))
+        nrc.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -287,6 +341,10 @@ def main():
                          "radiance cache in fp16 and bf16")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (counter calibration)")
+    ap.add_argument("--kernel", choices=KERNEL_WORKLOADS, default=None,
+                    help="only run this workload's frames (profiling runs)")
+    ap.add_argument("--precision", choices=("fp16", "bf16"), default="fp16",
+                    help="cache precision of the --kernel nrc_* runs")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -299,6 +357,12 @@ def main():
     vrt, wls = pkg.vrt, pkg.workloads
     if args.calibrate_copy:
         calibrate_copy(vrt, args.calibrate_copy, args.steps)
+        return
+    if args.kernel:
+        ctx = vrt.Context(0)
+        kernel_only(pkg, ctx, args.kernel, args.steps, args.warmup, args.precision,
+                    args.nrc_query_vertex)
+        ctx.close()
         return
     t0 = time.perf_counter()
     wl = wls.fbm_pathtrace(n=args.size, width=args.width, height=args.height,

@@ -23,6 +23,9 @@
 
 namespace orc {
 
+// Russian roulette on (the reference) or off (orc_set_roulette, a test switch)
+static bool g_roulette = true;
+
 // ---------------------------------------------------------------------------
 // small vector type, templated on the arithmetic precision
 // ---------------------------------------------------------------------------
@@ -1634,7 +1637,7 @@ V4<R> pathtrace_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params
           bounceID++;
         }
         if (rayType == Miss || int(bounceID) > P->max_bounce) break;
-        if (bounceID > 1) {  // Russian roulette, :594-599
+        if (bounceID > 1 && g_roulette) {  // Russian roulette, :594-599
           R prob = max_element(throughput);
           if (R(rng()) > prob) break;
           throughput = throughput * (R(1) / prob);
@@ -2514,6 +2517,19 @@ void orc_hg_sample(float g, const float d[3], int n, uint32_t seed, float *out) 
 }
 
 double orc_hg_4pi(double g, double cos_theta) { return hg_4pi(g, cos_theta); }
+
+void orc_uniform_sample_sphere(int n, uint32_t seed, float *out) {
+  for (int i = 0; i < n; ++i) {
+    Random r(seed, uint32_t(i));
+    const float u1 = r(), u2 = r();
+    const V3<float> w = uniform_sample_sphere(u1, u2);
+    out[3 * i] = w.x;
+    out[3 * i + 1] = w.y;
+    out[3 * i + 2] = w.z;
+  }
+}
+
+void orc_set_roulette(int enabled) { g_roulette = enabled != 0; }
 
 int orc_hdri_cdf(const orc_light *L, float *rows, float *cols) {
   orc::HdriLight H(*L);

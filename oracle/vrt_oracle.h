@@ -219,6 +219,13 @@ void orc_quad_light_sample(const orc_light *L, const float ref[3], int n, uint32
    (RNG seeded (seed, i)), and 4 pi p_HG(cos theta) --- */
 void orc_hg_sample(float g, const float d[3], int n, uint32_t seed, float *out);
 double orc_hg_4pi(double g, double cos_theta);
+/* --- isotropic phase sample (uniform_sample_sphere, Pathtrace_impl.cpp:335;
+   parity unpinned: PBRT definition): n directions (RNG seeded (seed, i)) --- */
+void orc_uniform_sample_sphere(int n, uint32_t seed, float *out);
+/* --- Russian roulette of the path tracer (Pathtrace_impl.cpp:594-599): 1
+   (default, the reference) or 0 (paths end only at max_bounce or a miss).
+   Process-wide; a test switch for the estimator's unbiasedness. --- */
+void orc_set_roulette(int enabled);
 /* --- HDRI light (type ORC_LIGHT_HDRI): its CDFs (h row-total entries, w*h
    per-row entries; -1 for a degenerate light), n light samples (RNG seeded
    (seed, i)): direction, pdf and radiance; and for given directions the

@@ -110,6 +110,8 @@ def lib():
         L.orc_unstructured_info.argtypes = [C.POINTER(OrcVolume), vp, vp, vp, vp]
         L.orc_sample_field_brute.argtypes = [C.POINTER(OrcVolume), ip, vp, vp]
         L.orc_hg_sample.argtypes = [C.c_float, vp, ip, C.c_uint32, vp]
+        L.orc_uniform_sample_sphere.argtypes = [ip, C.c_uint32, vp]
+        L.orc_set_roulette.argtypes = [ip]
         L.orc_hg_4pi.restype = C.c_double
         L.orc_hg_4pi.argtypes = [C.c_double, C.c_double]
         L.orc_hdri_cdf.argtypes = [C.POINTER(OrcLight), vp, vp]
