@@ -16,6 +16,9 @@ struct vrt_context_s {
   // alternately; each launch clears the set the next launch uses
   uint32_t *queues = nullptr;
   int queue_parity = 0;
+  // device address of the library's file-scope queue counters on this
+  // device (vrt_pathtrace_noalloc), looked up once at creation
+  uint32_t *static_queues = nullptr;
 };
 
 namespace vrt {

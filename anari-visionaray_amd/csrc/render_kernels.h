@@ -17,6 +17,13 @@ hipError_t launch_pathtrace(const KScene &sc, const KCamera &cam, const KParams 
                             const KFrame &F, int accumID, uint32_t *queues,
                             uint32_t *queues_next, int blocks, hipStream_t s,
                             const KNrc *nrc = nullptr);
+// the device address of the file-scope work-queue counters on the current
+// device (vrt_pathtrace_noalloc), and the path tracer launched over them:
+// the counters are cleared by a kernel first, nothing is allocated
+hipError_t static_queue_address(uint32_t **ptr);
+hipError_t launch_pathtrace_static(const KScene &sc, const KCamera &cam, const KParams &P,
+                                   const KFrame &F, int accumID, uint32_t *scratch_ptr,
+                                   int blocks, hipStream_t s);
 hipError_t launch_nrc_resolve(const KFrame &F, const KNrc &N, int accumID, hipStream_t s);
 hipError_t launch_taa_resolve(const KFrame &F, float alpha, hipStream_t s);
 // adds sum (x - r)^2 / (r^2 + eps) over n pixels' RGB to *out

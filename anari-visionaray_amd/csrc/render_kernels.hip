@@ -2525,6 +2525,38 @@ hipError_t launch_pathtrace(const KScene &sc, const KCamera &cam, const KParams 
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Work-queue counters of vrt_pathtrace_noalloc: a file-scope device array
+// (two sets of kQueues words, 64 B) in place of the context's allocation.
+// Every GPU holds its own copy until the process ends; host code reaches it
+// only through the address hipGetSymbolAddress returns on that device.
+// Each call clears the set it claims from with a kernel (no memset node), so
+// the call may be captured into a graph; calls that overlap on one GPU share
+// the array and race.
+// ---------------------------------------------------------------------------
+constexpr int kScratchWords = 2 * kQueues;
+static __device__ __align__(256) uint32_t scratch[kScratchWords];
+
+__global__ void k_clear_words(uint32_t *p, int n) {
+  if (int(threadIdx.x) < n) p[threadIdx.x] = 0u;
+}
+
+hipError_t static_queue_address(uint32_t **ptr) {
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e == hipSuccess) *ptr = static_cast<uint32_t *>(p);
+  return e;
+}
+
+hipError_t launch_pathtrace_static(const KScene &sc, const KCamera &cam, const KParams &P,
+                                   const KFrame &F, int accumID, uint32_t *scratch_ptr,
+                                   int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_clear_words, dim3(1), dim3(64), 0, s, scratch_ptr, kScratchWords);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_pathtrace(sc, cam, P, F, accumID, scratch_ptr, scratch_ptr + kQueues, blocks, s);
+}
+
 hipError_t launch_taa_resolve(const KFrame &F, float alpha, hipStream_t s) {
   const int n = F.width * F.height;
   hipLaunchKernelGGL(k_taa_resolve, dim3((n + 255) / 256), dim3(256), 0, s, F, alpha);

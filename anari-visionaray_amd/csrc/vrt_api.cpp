@@ -370,6 +370,7 @@ int vrt_context_create(int device, vrt_context *out) {
   if (e == hipSuccess) e = hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, device);
   if (e == hipSuccess) e = hipMalloc(&c->queues, 2 * 8 * sizeof(uint32_t));
   if (e == hipSuccess) e = hipMemset(c->queues, 0, 2 * 8 * sizeof(uint32_t));
+  if (e == hipSuccess) e = vrt::static_queue_address(&c->static_queues);
   if (e != hipSuccess) {
     delete c;
     return hip_fail(e, "context creation");
@@ -696,9 +697,16 @@ int camera_view_proj(const vrt_camera *c, float M[16]) {
   return VRT_OK;
 }
 
-int vrt_pathtrace(vrt_context ctx, const vrt_scene *scene, const vrt_camera *camera,
-                  const vrt_params *params, const vrt_frame *frame, int accum_id, int frame_id) {
-  (void)frame_id;
+}  // extern "C"
+
+namespace {
+
+// vrt_pathtrace and vrt_pathtrace_noalloc: the same frame, the work-queue
+// counters either the context's (alternating sets) or the library's
+// file-scope array (cleared per call)
+int pathtrace_frame(vrt_context ctx, const vrt_scene *scene, const vrt_camera *camera,
+                    const vrt_params *params, const vrt_frame *frame, int accum_id,
+                    bool static_queues) {
   if (int r = check_ctx(ctx)) return r;
   if (accum_id < 0) return fail(VRT_ERROR_INVALID_ARGUMENT, "accum_id must be >= 0");
   KScene sc;
@@ -724,12 +732,44 @@ int vrt_pathtrace(vrt_context ctx, const vrt_scene *scene, const vrt_camera *cam
   int blocks = std::max(1, ctx->num_cus) * pathtrace_blocks_per_cu(sc, lds, false,
                                                                     P.gradient_shading != 0);
   blocks = std::max(8, (blocks + 7) / 8 * 8);  // every XCD queue gets workgroups
-  uint32_t *cur = ctx->queues + 8 * ctx->queue_parity;
-  uint32_t *nxt = ctx->queues + 8 * (1 - ctx->queue_parity);
-  VRT_HIP(launch_pathtrace(sc, cam, P, F, accum_id, cur, nxt, blocks, ctx->stream));
-  ctx->queue_parity ^= 1;
+  if (static_queues) {
+    VRT_HIP(launch_pathtrace_static(sc, cam, P, F, accum_id, ctx->static_queues, blocks,
+                                    ctx->stream));
+  } else {
+    uint32_t *cur = ctx->queues + 8 * ctx->queue_parity;
+    uint32_t *nxt = ctx->queues + 8 * (1 - ctx->queue_parity);
+    VRT_HIP(launch_pathtrace(sc, cam, P, F, accum_id, cur, nxt, blocks, ctx->stream));
+    ctx->queue_parity ^= 1;
+  }
   if (P.taa) VRT_HIP(launch_taa_resolve(F, P.taa_alpha, ctx->stream));
   return VRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vrt_pathtrace(vrt_context ctx, const vrt_scene *scene, const vrt_camera *camera,
+                  const vrt_params *params, const vrt_frame *frame, int accum_id, int frame_id) {
+  (void)frame_id;
+  return pathtrace_frame(ctx, scene, camera, params, frame, accum_id, false);
+}
+
+int vrt_pathtrace_noalloc(vrt_context ctx, const vrt_scene *scene, const vrt_camera *camera,
+                          const vrt_params *params, const vrt_frame *frame, int accum_id,
+                          int frame_id) {
+  (void)frame_id;
+  if (int r = check_ctx(ctx)) return r;
+  if (!frame) return fail(VRT_ERROR_INVALID_ARGUMENT, "null frame");
+  if (int64_t(frame->width) * frame->height > VRT_NOALLOC_MAX_PIXELS)
+    return fail(VRT_ERROR_INVALID_ARGUMENT,
+                "vrt_pathtrace_noalloc: %d x %d pixels exceeds its maximum of %lld pixels "
+                "(the range of its work-queue counters)",
+                frame->width, frame->height, (long long)VRT_NOALLOC_MAX_PIXELS);
+  if (!ctx->static_queues)
+    return fail(VRT_ERROR_HIP, "vrt_pathtrace_noalloc: the library's queue counters are not "
+                               "addressable on this device");
+  return pathtrace_frame(ctx, scene, camera, params, frame, accum_id, true);
 }
 
 int vrt_pathtrace_nrc(vrt_context ctx, const vrt_scene *scene, const vrt_camera *camera,

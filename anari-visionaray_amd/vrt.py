@@ -200,6 +200,8 @@ def lib():
                                   C.POINTER(Frame), ip]
         L.vrt_pathtrace.argtypes = [vp, C.POINTER(Scene), C.POINTER(Camera), C.POINTER(Params),
                                     C.POINTER(Frame), ip, ip]
+        L.vrt_pathtrace_noalloc.argtypes = [vp, C.POINTER(Scene), C.POINTER(Camera),
+                                            C.POINTER(Params), C.POINTER(Frame), ip, ip]
         _lib = L
     return _lib
 
@@ -869,6 +871,12 @@ def raycast(ctx, scene, cam, prm, frame, frame_id=0):
 def pathtrace(ctx, scene, cam, prm, frame, accum_id, frame_id=0):
     check(lib().vrt_pathtrace(ctx.handle, C.byref(scene), C.byref(cam), C.byref(prm),
                               C.byref(frame.desc), accum_id, frame_id))
+
+
+def pathtrace_noalloc(ctx, scene, cam, prm, frame, accum_id, frame_id=0):
+    """vrt_pathtrace_noalloc: the same frame with no device allocation."""
+    check(lib().vrt_pathtrace_noalloc(ctx.handle, C.byref(scene), C.byref(cam), C.byref(prm),
+                                      C.byref(frame.desc), accum_id, frame_id))
 
 
 class Nrc:

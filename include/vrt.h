@@ -474,6 +474,21 @@ VRT_API int vrt_pathtrace(vrt_context ctx, const vrt_scene *scene,
                           const vrt_camera *camera, const vrt_params *params,
                           const vrt_frame *frame, int accum_id, int frame_id);
 
+/* 'pathtrace' renderer, one frame, allocating nothing: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops).  Same arguments, semantics and image as
+ * vrt_pathtrace; the persistent kernel's work-queue counters (16 uint32) live
+ * in a file-scope device array of the library instead of the context's
+ * allocation, cleared by a kernel at the start of every call (no memset).
+ * Each GPU holds one such array until the process ends, so calls that
+ * overlap on one GPU (several contexts or streams) race; serialise them.
+ * Images are limited to VRT_NOALLOC_MAX_PIXELS pixels (the counters' range);
+ * larger frames return VRT_ERROR_INVALID_ARGUMENT. */
+#define VRT_NOALLOC_MAX_PIXELS (1LL << 30)
+VRT_API int vrt_pathtrace_noalloc(vrt_context ctx, const vrt_scene *scene,
+                                  const vrt_camera *camera, const vrt_params *params,
+                                  const vrt_frame *frame, int accum_id, int frame_id);
+
 /* ---------------------------------------------------------------------- */
 /* Neural radiance cache (no counterpart in the reference)                 */
 /* ---------------------------------------------------------------------- */
