@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <set>
 #include <string>
 #include <vector>
@@ -77,7 +78,11 @@ struct Object {
   Device *dev = nullptr;
   ANARIDataType kind = ANARI_OBJECT;
   std::string subtype;
-  int pub = 1, internal = 0;
+  // public (application) plus internal (parameter, commit-buffer, array
+  // element) references; the object dies when the count reaches 0.  Atomic:
+  // a frame completion callback may release objects on the completion thread
+  // while the application thread retains and releases others.
+  std::atomic<int> refs{1};
   std::map<std::string, Param> staged, committed;
   uint64_t version = 0;  // device serial of the last commit
   bool pending = false;  // in the commit buffer
@@ -194,12 +199,12 @@ struct Field : Object {
   uint64_t built = ~0ull;
   bool valid = false;
   vrt_volume desc{};  // field part: type, dims, bounds, cell size, data handles
-  DevMem bricks;
+  DevMem cells;  // structuredRegular: the cell records of vrt_structured_cells
   vrt_nanovdb nvdb = nullptr;
   vrt_unstructured tets = nullptr;
   int default_grid[3] = {16, 16, 16};
   void drop() {
-    bricks.reset();
+    cells.reset();
     if (nvdb) vrt_nanovdb_destroy(nvdb);
     if (tets) vrt_unstructured_destroy(tets);
     nvdb = nullptr;
@@ -285,33 +290,47 @@ struct Completion {
 };
 
 struct CompletionThread {
-  std::mutex m;
-  std::condition_variable cv;
-  std::deque<Completion> q;
-  bool stop = false;
+  // Shared by the device and the thread: a device released from inside a
+  // callback is destroyed while that callback still runs on the thread, so
+  // the thread keeps this block alive and leaves its loop on `orphaned`.
+  struct State {
+    std::mutex m;
+    std::condition_variable cv;
+    std::deque<Completion> q;
+    bool stop = false;
+    bool orphaned = false;  // the device is gone: touch nothing after the callback
+    // the frame whose callback runs now, and whether that callback released
+    // it (both only touched by the thread itself)
+    Frame *current = nullptr;
+    bool current_destroyed = false;
+  };
+  std::shared_ptr<State> s = std::make_shared<State>();
   std::thread t;
-  // the frame whose callback runs now, and whether that callback released
-  // it (both only touched by the thread itself)
-  Frame *current = nullptr;
-  bool current_destroyed = false;
   void start(ANARIDevice dev) {
-    t = std::thread([this, dev] {
+    std::shared_ptr<State> st = s;
+    t = std::thread([st, dev] {
       for (;;) {
         Completion c;
         {
-          std::unique_lock<std::mutex> lk(m);
-          cv.wait(lk, [this] { return stop || !q.empty(); });
-          if (q.empty()) return;  // stop requested and drained
-          c = q.front();
-          q.pop_front();
+          std::unique_lock<std::mutex> lk(st->m);
+          st->cv.wait(lk, [&st] { return st->stop || !st->q.empty(); });
+          if (st->orphaned || st->q.empty()) return;  // stop requested and drained
+          c = st->q.front();
+          st->q.pop_front();
         }
         (void)hipEventSynchronize(c.event);
         (void)hipEventDestroy(c.event);
-        current = c.frame;
-        current_destroyed = false;
+        st->current = c.frame;
+        st->current_destroyed = false;
         c.fn(c.user, dev, reinterpret_cast<ANARIFrame>(c.frame));
-        if (!current_destroyed) c.frame->callbacks_pending.fetch_sub(1);
-        current = nullptr;
+        {
+          std::lock_guard<std::mutex> lk(st->m);
+          // a device destroyed by this callback outlives none of its frames,
+          // so c.frame is gone too: leave without touching either
+          if (st->orphaned) return;
+        }
+        if (!st->current_destroyed) c.frame->callbacks_pending.fetch_sub(1);
+        st->current = nullptr;
       }
     });
   }
@@ -319,12 +338,12 @@ struct CompletionThread {
   // a frame about to be destroyed by its own callback: its other queued
   // callbacks are dropped
   void forget(Frame *f) {
-    if (current == f) current_destroyed = true;
-    std::lock_guard<std::mutex> lk(m);
-    for (auto it = q.begin(); it != q.end();) {
+    if (s->current == f) s->current_destroyed = true;
+    std::lock_guard<std::mutex> lk(s->m);
+    for (auto it = s->q.begin(); it != s->q.end();) {
       if (it->frame == f) {
         (void)hipEventDestroy(it->event);
-        it = q.erase(it);
+        it = s->q.erase(it);
       } else {
         ++it;
       }
@@ -332,22 +351,29 @@ struct CompletionThread {
   }
   void push(const Completion &c) {
     {
-      std::lock_guard<std::mutex> lk(m);
-      q.push_back(c);
+      std::lock_guard<std::mutex> lk(s->m);
+      s->q.push_back(c);
     }
-    cv.notify_one();
+    s->cv.notify_one();
   }
   void join() {
     if (!t.joinable()) return;
-    if (on_thread()) {  // the device released from inside a callback
+    if (on_thread()) {
+      // the device released from inside a callback: the thread ends when
+      // that callback returns; queued completions (none can remain: every
+      // frame holds its device) are dropped
+      std::lock_guard<std::mutex> lk(s->m);
+      s->stop = s->orphaned = true;
+      for (const Completion &c : s->q) (void)hipEventDestroy(c.event);
+      s->q.clear();
       t.detach();
       return;
     }
     {
-      std::lock_guard<std::mutex> lk(m);
-      stop = true;
+      std::lock_guard<std::mutex> lk(s->m);
+      s->stop = true;
     }
-    cv.notify_one();
+    s->cv.notify_one();
     t.join();
   }
 };
@@ -399,7 +425,7 @@ struct Device : Object {
 };
 
 void retain_internal(Object *o) {
-  if (o) ++o->internal;
+  if (o) o->refs.fetch_add(1);
 }
 void destroy(Object *o) {
   Device *d = o->kind == ANARI_DEVICE ? nullptr : o->dev;
@@ -417,7 +443,7 @@ void destroy(Object *o) {
 }
 void release_internal(Object *o) {
   if (!o) return;
-  if (--o->internal <= 0 && o->pub <= 0) destroy(o);
+  if (o->refs.fetch_sub(1) == 1) destroy(o);
 }
 
 Device *as_device(ANARIDevice d) { return reinterpret_cast<Device *>(d); }
@@ -476,10 +502,11 @@ bool build_field(Device *d, Field *f) {
     const int dims[3] = {int(a->n[0]), int(a->n[1]), int(a->n[2])};
     const size_t bytes = a->count() * 4;
     DevMem lin;
-    if (!lin.alloc(d->ctx, bytes) || !f->bricks.alloc(d->ctx, vrt_structured_brick_bytes(dims)) ||
+    const size_t cell_bytes = vrt_structured_cell_bytes(dims);
+    if (cell_bytes == 0 || !lin.alloc(d->ctx, bytes) || !f->cells.alloc(d->ctx, cell_bytes) ||
         vrt_upload(d->ctx, lin.p, a->data(), bytes) != VRT_OK ||
-        vrt_structured_brickify(d->ctx, static_cast<const float *>(lin.p), dims,
-                                static_cast<float *>(f->bricks.p)) != VRT_OK ||
+        vrt_structured_cells(d->ctx, static_cast<const float *>(lin.p), dims,
+                             static_cast<float *>(f->cells.p)) != VRT_OK ||
         vrt_synchronize(d->ctx) != VRT_OK) {
       d->vrt_error(f, "structuredRegular upload");
       return false;
@@ -493,7 +520,7 @@ bool build_field(Device *d, Field *f) {
       v.bounds_hi[k] = origin[k] + (float(dims[k]) - 1.f) * spacing[k];
     }
     v.cell_size = std::min(spacing[0], std::min(spacing[1], spacing[2]));
-    v.bricks = static_cast<const float *>(f->bricks.p);
+    v.cells = static_cast<const float *>(f->cells.p);
     for (int k = 0; k < 3; ++k) f->default_grid[k] = 16;  // StructuredRegularField.cpp:100
   } else if (f->subtype == "nanovdb") {
     const Array *a = array_param(f, "data");
@@ -1294,11 +1321,11 @@ void anariCommitParameters(ANARIDevice dh, ANARIObject oh) {
 void anariRelease(ANARIDevice, ANARIObject oh) {
   Object *o = as_obj(oh);
   if (!o) return;
-  if (--o->pub <= 0 && o->internal <= 0) destroy(o);
+  if (o->refs.fetch_sub(1) == 1) destroy(o);
 }
 
 void anariRetain(ANARIDevice, ANARIObject oh) {
-  if (Object *o = as_obj(oh)) ++o->pub;
+  if (Object *o = as_obj(oh)) o->refs.fetch_add(1);
 }
 
 int anariGetProperty(ANARIDevice dh, ANARIObject oh, const char *name, ANARIDataType t,

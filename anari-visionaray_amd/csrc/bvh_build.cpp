@@ -254,8 +254,35 @@ int vrt_bvh_build(vrt_context ctx, const float *vertices, int64_t num_vertices,
     nodes4.insert(nodes4.end(), rec, rec + 8);
   }
 
+  // the traversal stacks, at a node, every inner child it hit but the one it
+  // descends into: the most entries along any root-to-leaf path, bottom-up
+  // (children follow their parent in the node array)
+  const size_t nn = nodes4.size() / 8;
+  std::vector<uint32_t> need(nn, 0u);
+  for (size_t i = nn; i-- > 0;) {
+    const float4 ch = nodes4[8 * i + 6], cn = nodes4[8 * i + 7];
+    const float chv[4] = {ch.x, ch.y, ch.z, ch.w}, cnv[4] = {cn.x, cn.y, cn.z, cn.w};
+    uint32_t inner = 0, below = 0;
+    for (int k = 0; k < 4; ++k) {
+      int32_t c;
+      uint32_t cnt;
+      std::memcpy(&c, &chv[k], 4);
+      std::memcpy(&cnt, &cnv[k], 4);
+      if (c < 0 || cnt != 0u) continue;
+      ++inner;
+      below = std::max(below, need[size_t(c)]);
+    }
+    need[i] = (inner > 0 ? inner - 1 : 0) + below;
+  }
+  const uint32_t stack_bound = nn ? need[0] : 0u;
+  if (stack_bound > VRT_BVH_MAX_STACK)
+    return fail(VRT_ERROR_UNSUPPORTED,
+                "bvh: a traversal of this tree can stack %u entries, the kernels hold %d "
+                "(binary depth %u)", stack_bound, VRT_BVH_MAX_STACK, B.depth);
+
   auto *h = new vrt_bvh_s;
   h->device = ctx->device;
+  h->stack_bound = stack_bound;
   h->num_nodes = uint32_t(nodes4.size() / 8);
   h->num_tris = uint32_t(n);
   h->depth = B.depth;
@@ -291,6 +318,7 @@ int vrt_bvh_get_info(vrt_bvh b, vrt_bvh_info *info) {
   }
   info->build_ms = b->build_ms;
   info->device_bytes = uint64_t(b->num_nodes) * 128 + uint64_t(b->num_tris) * 48;
+  info->stack_bound = b->stack_bound;
   return VRT_OK;
 }
 

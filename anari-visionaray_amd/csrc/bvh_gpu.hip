@@ -508,6 +508,15 @@ int vrt_bvh_build_gpu(vrt_context ctx, const float *vertices, int64_t num_vertic
   if (e != hipSuccess) return cleanup(e, "bvh build");
   h->num_nodes = nodes_used;
   h->depth = depth;
+  // a node stacks at most 3 of its children
+  h->stack_bound = depth > 1 ? 3u * (depth - 1u) : 0u;
+  if (h->stack_bound > VRT_BVH_MAX_STACK) {
+    const unsigned sb = h->stack_bound;
+    (void)cleanup(hipSuccess, "bvh build");
+    return fail(VRT_ERROR_UNSUPPORTED,
+                "bvh: a traversal of this tree can stack %u entries, the kernels hold %d "
+                "(4-wide depth %u)", sb, VRT_BVH_MAX_STACK, depth);
+  }
   for (int a = 0; a < 3; ++a) {
     h->lo[a] = rb[a];
     h->hi[a] = rb[3 + a];

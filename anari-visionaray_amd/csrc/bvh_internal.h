@@ -12,6 +12,7 @@ struct vrt_bvh_s {
   float4 *nodes = nullptr;
   float4 *tris = nullptr;
   uint32_t num_nodes = 0, num_tris = 0, depth = 0;
+  uint32_t stack_bound = 0;  // most traversal-stack entries (vrt_bvh_info)
   float lo[3], hi[3];
   double build_ms = 0.0;
 };

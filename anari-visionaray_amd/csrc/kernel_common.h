@@ -36,16 +36,17 @@ __host__ __device__ inline float comp(f3 a, int i) { return i == 0 ? a.x : (i ==
 struct KVolume {
   int field_type;
   int dims[3];
-  uint32_t bstride_y;  // bricks per z-slab row  * 512 (floats)
-  uint32_t bstride_z;  // bricks per z slab * 512
   f3 blo, bhi;         // object-space bounds
   f3 S, T;             // voxel space = S * (p + T)
   float cell_size;
-  const float *bricks;
+  const float4 *cells; // structured: 2 float4 (the cell's 8 corner voxels) per voxel cell
+  const float *leaves; // nanovdb: 512 values per leaf
   const float4 *tf;    // global copy of the table
   int tf_size;
   int tf_lds_offset;   // offset of this table in the block's LDS copy
   float vlo, vhi;
+  float vrange_inv;    // 1 / (vhi - vlo), used when that range is a power of two
+  int vrange_pow2;
   float unit_distance;
   const float *majorants;
   int gdims[3];
@@ -58,7 +59,7 @@ struct KVolume {
   f3 glo, ghi;         // majorant grid bounds in voxel space
   // nanovdb: top table over 128^3 regions (>= 0 block index, < 0: -1 -
   // index into nv_consts), blocks of 16^3 slot entries (>= 0 leaf index
-  // into `bricks`, else a constant), top origin (in 128-voxel units) / dims,
+  // into `leaves`, else a constant), top origin (in 128-voxel units) / dims,
   // world -> index map, filter
   const int32_t *nv_top;
   const int32_t *nv_blocks;

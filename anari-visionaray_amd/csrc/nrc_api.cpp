@@ -75,9 +75,8 @@ int nrc_train_frame(vrt_nrc n, int steps) {
   for (int b = 0; b < steps; ++b) {
     VRT_HIP(nrc_launch_gradients(n->d, n->t_cap, n->t_raw, n->t_target, n->max_workgroups,
                                  n->ctx->stream, &n->ctr[1], b, steps));
-    n->step++;
     VRT_HIP(nrc_launch_adam(n->d, n->t_cap, n->cfg.learning_rate, n->cfg.beta1, n->cfg.beta2,
-                            n->cfg.epsilon, n->step, n->ctx->stream, &n->ctr[1], b, steps));
+                            n->cfg.epsilon, n->ctx->stream, &n->ctr[1], b, steps));
   }
   return VRT_OK;
 }
@@ -182,9 +181,8 @@ int vrt_nrc_gradients(vrt_nrc n, int count, const float *raw, const float *targe
 int vrt_nrc_train_step(vrt_nrc n, int count, const float *raw, const float *target, float *loss) {
   if (int r = vrt_nrc_gradients(n, count, raw, target, nullptr, nullptr)) return r;
   if (count == 0) return VRT_OK;
-  n->step++;
   VRT_HIP(nrc_launch_adam(n->d, count, n->cfg.learning_rate, n->cfg.beta1, n->cfg.beta2,
-                          n->cfg.epsilon, n->step, n->ctx->stream));
+                          n->cfg.epsilon, n->ctx->stream));
   if (loss) {
     VRT_HIP(hipMemcpyAsync(loss, n->d.stats, sizeof(float), hipMemcpyDeviceToHost, n->ctx->stream));
     VRT_HIP(hipStreamSynchronize(n->ctx->stream));

@@ -11,7 +11,6 @@ struct vrt_nrc_s {
   vrt_nrc_config cfg{};
   vrt::NrcDevice d{};
   int max_workgroups = 0;
-  int step = 0;
   // per-frame buffers of vrt_pathtrace_nrc, grown on demand
   size_t pixels = 0;
   int t_cap = 0;

@@ -39,7 +39,7 @@ struct NrcDevice {
   float *bias;         // kNrcBiases fp32 (added to the fp32 accumulators)
   float *grad;         // kNrcParams gradient sums of the last step
   float *partials;     // max_workgroups x kNrcParams
-  float *stats;        // [0] loss sum of the last step, [1] samples
+  float *stats;        // [0] loss sum of the last step, [1] samples, [2] Adam steps taken
   int bf16;            // 0: fp16, 1: bf16
 };
 
@@ -55,8 +55,10 @@ hipError_t nrc_launch_gradients(const NrcDevice &d, int n, const float *raw,
                                 const float *target, int max_workgroups, hipStream_t s,
                                 const uint32_t *n_dev = nullptr, int batch = 0,
                                 int nbatches = 1);
+// one Adam step on the mean gradient of mini-batch `batch`; an empty
+// mini-batch changes nothing, not even the step count of the bias correction
 hipError_t nrc_launch_adam(const NrcDevice &d, int n, float lr, float beta1, float beta2,
-                           float eps, int step, hipStream_t s, const uint32_t *n_dev = nullptr,
+                           float eps, hipStream_t s, const uint32_t *n_dev = nullptr,
                            int batch = 0, int nbatches = 1);
 hipError_t nrc_launch_refresh(const NrcDevice &d, hipStream_t s);
 

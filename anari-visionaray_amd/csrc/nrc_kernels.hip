@@ -438,14 +438,28 @@ __device__ inline void emit16(const NrcDevice &d, int p, float v) {
   }
 }
 
+// The step count of Adam's bias correction, kept on the device in
+// d.stats[2]: it advances only for a mini-batch that holds samples (a
+// frame's sample count is known only on the device), before the update
+// that uses it
+__global__ void k_nrc_adam_count(NrcDevice d, int n, const uint32_t *n_dev, int batch,
+                                 int nbatches) {
+  int lo, cnt;
+  batch_range(n, n_dev, batch, nbatches, lo, cnt);
+  if (cnt > 0 && threadIdx.x == 0) d.stats[2] += 1.f;
+}
+
 // Adam on the fp32 master parameters (Kingma & Ba 2015, bias-corrected)
 __global__ void k_nrc_adam(NrcDevice d, int n, const uint32_t *n_dev, int batch, int nbatches,
-                           float lr, float b1, float b2, float eps, float c1, float c2) {
+                           float lr, float b1, float b2, float eps) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= kNrcParams) return;
   int lo, cnt;
   batch_range(n, n_dev, batch, nbatches, lo, cnt);
   if (cnt <= 0) return;  // empty mini-batch: no step
+  const float step = d.stats[2];
+  const float c1 = 1.f / (1.f - powf(b1, step));
+  const float c2 = 1.f / (1.f - powf(b2, step));
   const float gr = d.grad[p] * (1.f / float(cnt));
   const float m = b1 * d.adam_m[p] + (1.f - b1) * gr;
   const float v = b2 * d.adam_v[p] + (1.f - b2) * gr * gr;
@@ -488,6 +502,7 @@ __global__ void k_nrc_init(NrcDevice d, uint64_t seed) {
   d.adam_m[p] = 0.f;
   d.adam_v[p] = 0.f;
   emit16(d, p, w);
+  if (p < 4) d.stats[p] = 0.f;  // loss, samples, Adam step count
 }
 
 // one-wave MFMA lane-layout probe
@@ -548,12 +563,10 @@ hipError_t nrc_launch_gradients(const NrcDevice &d, int n, const float *raw, con
 }
 
 hipError_t nrc_launch_adam(const NrcDevice &d, int n, float lr, float b1, float b2, float eps,
-                           int step, hipStream_t s, const uint32_t *n_dev, int batch,
-                           int nbatches) {
-  const float c1 = 1.f / (1.f - powf(b1, float(step)));
-  const float c2 = 1.f / (1.f - powf(b2, float(step)));
+                           hipStream_t s, const uint32_t *n_dev, int batch, int nbatches) {
+  hipLaunchKernelGGL(k_nrc_adam_count, dim3(1), dim3(64), 0, s, d, n, n_dev, batch, nbatches);
   hipLaunchKernelGGL(k_nrc_adam, dim3((kNrcParams + 255) / 256), dim3(256), 0, s, d, n, n_dev,
-                     batch, nbatches, lr, b1, b2, eps, c1, c2);
+                     batch, nbatches, lr, b1, b2, eps);
   return hipGetLastError();
 }
 

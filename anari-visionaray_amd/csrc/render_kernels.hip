@@ -109,86 +109,143 @@ __device__ inline KRay to_local(const KVolume &v, const KRay &r) {
 }
 
 // ---------------------------------------------------------------------------
-// Field fetch: trilinear over 8^3 bricks, texel-centre convention (p*N-0.5),
-// clamp addressing.  The 8 taps are formed from per-axis partial offsets.
+// Field fetch: trilinear, texel-centre convention (p*N-0.5), clamp
+// addressing, on the cell records of vrt_structured_cells: the cell index
+// and weight per axis, then one 32-byte record holds all 8 taps.
 // ---------------------------------------------------------------------------
-__device__ inline void axis_taps(float p, int n, uint32_t stride_brick, int lshift,
-                                 uint32_t &o0, uint32_t &o1, float &w) {
+// Clamping c to [0, n-1] gives the clamp-addressed taps exactly: below 0
+// both taps are voxel 0 (the weight 0 keeps it), above n-1 both are voxel
+// n-1 (the record repeats it); NaN clamps to 0 as the tap clamp did.
+__device__ inline int axis_cell(float p, int n, float &w) {
   float c = p * float(n) - 0.5f;
-  c = fminf(fmaxf(c, -1.f), float(n));
-  float fl = floorf(c);
+  c = fminf(fmaxf(c, 0.f), float(n - 1));
+  const float fl = floorf(c);
   w = c - fl;
-  int i = int(fl);
-  int i0 = min(max(i, 0), n - 1);
-  int i1 = min(max(i + 1, 0), n - 1);
-  o0 = uint32_t(i0 >> 3) * stride_brick + (uint32_t(i0 & 7) << lshift);
-  o1 = uint32_t(i1 >> 3) * stride_brick + (uint32_t(i1 & 7) << lshift);
+  return int(fl);
 }
 
 __device__ inline float lerpf(float a, float b, float t) { return a + t * (b - a); }
 
-// x^y from IEEE basic operations only (no library transcendental), so the
-// opacity correction powf(1-alpha, dt/unitDistance) of
-// VolumeIntegration.h:416-417 rounds identically on every IEEE-754 machine
-// that evaluates it without contraction: x = m*2^e with m in [sqrt.5, sqrt2),
-// ln m = 2s(1 + s^2/3 + s^4/5 + s^6/7 + s^8/9) with s = (m-1)/(m+1),
-// 2^z = 2^n * exp(f ln2) with a degree-7 Taylor polynomial, |f| <= 1/2.
-// Error <= 1.5 (2 + |y log2 x|) ulp for x in [0, 2^127), y finite.
-__device__ inline float pow_ieee(float x, float y) {
+// x^y from IEEE basic operations (fused multiply-add, ldexp, rint) and a
+// table, so the opacity correction powf(1-alpha, dt/unitDistance) of
+// VolumeIntegration.h:416-417 rounds identically on every machine that
+// evaluates it this way.  log2: x = 2^e z with z in [0.69921875,
+// 1.3984375) (float bits offset by 0x3f330000), z's subinterval i (the next
+// 6 mantissa bits) gives {1/c_i as float, -log2 of that float}, r = z/c_i - 1
+// = fma(z, 1/c_i, -1) with |r| <= 2^-7, log2(1 + r) from its Taylor series
+// to r^4; c_i is the subinterval's midpoint, or 1 exactly for the one holding
+// 1.0, so x near 1 keeps its digits.  2^(y log2 x) = 2^n 2^f with n =
+// rint, |f| <= 1/2, 2^f from its Taylor series to f^7, scaled by ldexp.
+// Error <= 1.5 (2 + |y log2 x|) ulp for normal x, y finite; results below
+// 2^-126 are returned as 0.
+__constant__ float2 kPowLog2[64] = {
+{1.4222222566604614f, -0.5081469416618347f},
+    {1.4065934419631958f, -0.4922053813934326f},
+    {1.39130437374115f, -0.47643807530403137f},
+    {1.376344084739685f, -0.46084117889404297f},
+    {1.3617020845413208f, -0.4454111158847809f},
+    {1.3473684787750244f, -0.43014445900917053f},
+    {1.3333333730697632f, -0.4150375425815582f},
+    {1.3195875883102417f, -0.4000871181488037f},
+    {1.3061224222183228f, -0.38529011607170105f},
+    {1.2929292917251587f, -0.37064337730407715f},
+    {1.2799999713897705f, -0.3561437726020813f},
+    {1.2673267126083374f, -0.34178850054740906f},
+    {1.2549020051956177f, -0.3275747001171112f},
+    {1.2427184581756592f, -0.31349948048591614f},
+    {1.2307692766189575f, -0.2995603382587433f},
+    {1.2190476655960083f, -0.285754531621933f},
+    {1.2075471878051758f, -0.27207955718040466f},
+    {1.1962616443634033f, -0.25853297114372253f},
+    {1.185185194015503f, -0.24511250853538513f},
+    {1.174311876296997f, -0.23181560635566711f},
+    {1.163636326789856f, -0.21864023804664612f},
+    {1.1531531810760498f, -0.20558416843414307f},
+    {1.1428571939468384f, -0.1926451474428177f},
+    {1.1327433586120605f, -0.17982102930545807f},
+    {1.1228070259094238f, -0.16710999608039856f},
+    {1.1130434274673462f, -0.15450988709926605f},
+    {1.1034482717514038f, -0.14201900362968445f},
+    {1.0940171480178833f, -0.12963534891605377f},
+    {1.0847457647323608f, -0.11735695600509644f},
+    {1.0756303071975708f, -0.10518231242895126f},
+    {1.0666667222976685f, -0.09310948103666306f},
+    {1.0578511953353882f, -0.08113670349121094f},
+    {1.049180269241333f, -0.06926257908344269f},
+    {1.0406503677368164f, -0.057485442608594894f},
+    {1.0322580337524414f, -0.04580364748835564f},
+    {1.0240000486373901f, -0.034215785562992096f},
+    {1.0158730745315552f, -0.022720159962773323f},
+    {1.0078740119934082f, -0.011315307579934597f},
+    {1.0f, -0.0f},
+    {0.9846153855323792f, 0.0223678108304739f},
+    {0.9696969985961914f, 0.044394075870513916f},
+    {0.9552238583564758f, 0.0660892203450203f},
+    {0.9411764740943909f, 0.08746283501386642f},
+    {0.9275362491607666f, 0.10852442681789398f},
+    {0.9142857193946838f, 0.12928301095962524f},
+    {0.9014084339141846f, 0.14974714815616608f},
+    {0.8888888955116272f, 0.16992498934268951f},
+    {0.8767123222351074f, 0.18982456624507904f},
+    {0.8648648858070374f, 0.20945332944393158f},
+    {0.8533333539962769f, 0.22881865501403809f},
+    {0.8421052694320679f, 0.2479275017976761f},
+    {0.8311688303947449f, 0.2667865455150604f},
+    {0.8205128312110901f, 0.28540220856666565f},
+    {0.8101266026496887f, 0.3037807047367096f},
+    {0.800000011920929f, 0.32192808389663696f},
+    {0.790123462677002f, 0.33984997868537903f},
+    {0.7804877758026123f, 0.35755205154418945f},
+    {0.7710843086242676f, 0.3750394880771637f},
+    {0.761904776096344f, 0.39231738448143005f},
+    {0.7529411911964417f, 0.4093908965587616f},
+    {0.7441860437393188f, 0.42626476287841797f},
+    {0.7356321811676025f, 0.4429435133934021f},
+    {0.7272727489471436f, 0.45943158864974976f},
+    {0.7191011309623718f, 0.4757334291934967f}};
+
+__device__ inline float pow_ieee(float x, float y, const float2 *tab) {
   if (y == 0.f || x == 1.f) return 1.f;
   if (!(x > 0.f)) return y > 0.f ? 0.f : __builtin_inff();
   const uint32_t bits = __float_as_uint(x);
-  int e = int((bits >> 23) & 255u);
-  if (e == 0) return y > 0.f ? 0.f : __builtin_inff();  // subnormal base
-  if (e == 255) return x;                               // inf / nan
-  e -= 127;
-  float m = __uint_as_float((bits & 0x007FFFFFu) | 0x3F800000u);
-  if (m > 1.41421356f) {
-    m = m * 0.5f;
-    e += 1;
-  }
-  const float s = (m - 1.f) / (m + 1.f);
-  const float s2 = s * s;
-  float p = 0.111111111f;
-  p = p * s2 + 0.142857143f;
-  p = p * s2 + 0.2f;
-  p = p * s2 + 0.333333333f;
-  p = p * s2 + 1.f;
-  const float l2 = float(e) + (2.f * s * p) * 1.44269504f;
-  const float z = y * l2;
-  if (z < -150.f) return 0.f;
-  if (z > 128.f) return __builtin_inff();
-  const float n = floorf(z + 0.5f);
-  const float t = (z - n) * 0.693147181f;
-  float q = 1.98412698e-4f;
-  q = q * t + 1.38888889e-3f;
-  q = q * t + 8.33333333e-3f;
-  q = q * t + 4.16666667e-2f;
-  q = q * t + 0.166666667f;
-  q = q * t + 0.5f;
-  q = q * t + 1.f;
-  q = q * t + 1.f;
-  int ni = int(n);
-  if (ni < -126) {  // subnormal result: scale in two steps
-    q = q * 5.87747175e-39f;  // 2^-127
-    ni += 127;
-  }
-  return q * __uint_as_float(uint32_t(ni + 127) << 23);
+  if (bits < 0x00800000u) return y > 0.f ? 0.f : __builtin_inff();  // subnormal base
+  if (bits >= 0x7f800000u) return x;                                 // inf
+  const uint32_t tmp = bits - 0x3f330000u;
+  const int e = int(tmp) >> 23;
+  const float2 t = tab[(tmp >> 17) & 63u];
+  const float z = __uint_as_float(bits - (tmp & 0xff800000u));
+  const float r = __builtin_fmaf(z, t.x, -1.f);
+  float q = __builtin_fmaf(r, -0.36067376f, 0.48089835f);  // -1/(4 ln2), 1/(3 ln2)
+  q = __builtin_fmaf(r, q, -0.72134752f);                   // -1/(2 ln2)
+  q = __builtin_fmaf(r, q, 1.44269504f);                    // 1/ln2
+  const float l2 = __builtin_fmaf(r, q, float(e) + t.y);
+  const float zz = y * l2;
+  if (zz < -126.f) return 0.f;
+  if (zz > 128.f) return __builtin_inff();
+  const float n = __builtin_rintf(zz);
+  const float f = zz - n;
+  float p = 1.52527338e-5f;                  // ln2^7 / 7!
+  p = __builtin_fmaf(p, f, 1.54035304e-4f);  // ln2^6 / 6!
+  p = __builtin_fmaf(p, f, 1.33335581e-3f);
+  p = __builtin_fmaf(p, f, 9.61812911e-3f);
+  p = __builtin_fmaf(p, f, 5.55041087e-2f);
+  p = __builtin_fmaf(p, f, 0.240226507f);
+  p = __builtin_fmaf(p, f, 0.693147181f);
+  p = __builtin_fmaf(p, f, 1.f);
+  return __builtin_ldexpf(p, int(n));
 }
 
 __device__ inline float sample_structured(const KVolume &v, f3 P) {
-  uint32_t x0, x1, y0, y1, z0, z1;
   float wx, wy, wz;
-  axis_taps(P.x, v.dims[0], 512u, 0, x0, x1, wx);
-  axis_taps(P.y, v.dims[1], v.bstride_y, 3, y0, y1, wy);
-  axis_taps(P.z, v.dims[2], v.bstride_z, 6, z0, z1, wz);
-  const float *b = v.bricks;
-  const float v000 = b[x0 + y0 + z0], v100 = b[x1 + y0 + z0];
-  const float v010 = b[x0 + y1 + z0], v110 = b[x1 + y1 + z0];
-  const float v001 = b[x0 + y0 + z1], v101 = b[x1 + y0 + z1];
-  const float v011 = b[x0 + y1 + z1], v111 = b[x1 + y1 + z1];
-  float c00 = lerpf(v000, v100, wx), c10 = lerpf(v010, v110, wx);
-  float c01 = lerpf(v001, v101, wx), c11 = lerpf(v011, v111, wx);
+  const int i = axis_cell(P.x, v.dims[0], wx);
+  const int j = axis_cell(P.y, v.dims[1], wy);
+  const int k = axis_cell(P.z, v.dims[2], wz);
+  const uint32_t cell = (uint32_t(k) * uint32_t(v.dims[1]) + uint32_t(j)) * uint32_t(v.dims[0]) +
+                        uint32_t(i);
+  const float4 *r = v.cells + 2 * size_t(cell);
+  const float4 z0 = r[0], z1 = r[1];  // (v000, v100, v010, v110), (v001, v101, v011, v111)
+  float c00 = lerpf(z0.x, z0.y, wx), c10 = lerpf(z0.z, z0.w, wx);
+  float c01 = lerpf(z1.x, z1.y, wx), c11 = lerpf(z1.z, z1.w, wx);
   return lerpf(lerpf(c00, c10, wy), lerpf(c01, c11, wy), wz);
 }
 
@@ -212,7 +269,7 @@ __device__ inline int nvdb_slot(const KVolume &v, int sx, int sy, int sz) {
 
 __device__ inline float nvdb_value(const KVolume &v, int i, int j, int k) {
   const int e = nvdb_slot(v, i >> 3, j >> 3, k >> 3);
-  return e >= 0 ? v.bricks[size_t(e) * 512u + uint32_t(((i & 7) << 6) | ((j & 7) << 3) | (k & 7))]
+  return e >= 0 ? v.leaves[size_t(e) * 512u + uint32_t(((i & 7) << 6) | ((j & 7) << 3) | (k & 7))]
                 : v.nv_consts[-1 - e];
 }
 
@@ -341,7 +398,8 @@ __device__ inline f3 gradient_normal(const KVolume &v, f3 P) {
 // TF table, read from the workgroup's LDS copy
 // the two table entries and the weight tex1D(linear, clamp) reads for val
 __device__ inline void tf_taps(const KVolume &v, float val, int &i0, int &i1, float &w) {
-  float u = (val - v.vlo) / (v.vhi - v.vlo);
+  // a power-of-two range divides exactly as its reciprocal multiplies
+  float u = v.vrange_pow2 ? (val - v.vlo) * v.vrange_inv : (val - v.vlo) / (v.vhi - v.vlo);
   const int n = v.tf_size;
   float c = u * float(n) - 0.5f;
   c = fminf(fmaxf(c, -1.f), float(n));
@@ -456,7 +514,7 @@ __device__ inline bool tri_test(const KRay &r, f3 v1, f3 e1, f3 e2, float &t, fl
   return true;
 }
 
-constexpr int kBvhStack = 64;     // entries per lane
+constexpr int kBvhStack = VRT_BVH_MAX_STACK;  // entries per lane (builds check the bound)
 constexpr int kBvhLdsStack = 16;  // of which the first (top of a shallow stack) live in LDS
 constexpr int kBvhLanes = 256;    // lanes of the workgroups that traverse (LDS stride)
 
@@ -1189,7 +1247,9 @@ template <bool ANYF, bool SHADE, bool SURF>
 __global__ void __launch_bounds__(256)
     k_raycast(KScene sc, KCamera cam, KParams P, KFrame F) {
   extern __shared__ float4 s_tf[];
-  stage_tf(sc, s_tf);
+  __shared__ float2 s_pow[64];  // pow_ieee's log2 table
+  if (threadIdx.x < 64) s_pow[threadIdx.x] = kPowLog2[threadIdx.x];
+  stage_tf(sc, s_tf);  // (its barrier also publishes s_pow)
   int x, y;
   tile_pixel(F.width, x, y);
   const bool active = x < F.width && y < F.height;
@@ -1313,14 +1373,12 @@ __global__ void __launch_bounds__(256)
           }
 #pragma unroll
           for (int b = 0; b < kRaySteps; ++b) {
-            if (!in[b] || !(alpha < 0.99f)) {
-              marching = false;
-              break;
-            }
+            marching = marching && in[b] && alpha < 0.99f;
+            if (!marching) continue;
             ++lookups;
             if (!has[b]) continue;  // VolumeIntegration.h:412
             const float4 s = classify(v, s_tf, val[b]);
-            const float stepT = pow_ieee(1.f - s.w, expo);
+            const float stepT = pow_ieee(1.f - s.w, expo, s_pow);
             const float k = trans * (1.f - stepT);
             f3 c = mk(s.x, s.y, s.z);
             if constexpr (SHADE) c = gradient_shade<ANYF>(sc, v, P, ori + dir * tb[b], -dir, c, rs);
@@ -2152,20 +2210,20 @@ __global__ void k_nrc_resolve(KFrame F, KNrc N, int accumID) {
 // ---------------------------------------------------------------------------
 // Field preparation
 // ---------------------------------------------------------------------------
-__global__ void k_brickify(const float *__restrict__ lin, int nx, int ny, int nz, int nbx,
-                           int nby, int nbz, float *__restrict__ bricks) {
-  // one thread per brick-ordered output element: coalesced writes
-  const size_t total = size_t(nbx) * nby * nbz * 512;
-  for (size_t o = size_t(blockIdx.x) * blockDim.x + threadIdx.x; o < total;
-       o += size_t(gridDim.x) * blockDim.x) {
-    const size_t b = o >> 9;
-    const int l = int(o & 511);
-    const int lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
-    const int bx = int(b % nbx), by = int((b / nbx) % nby), bz = int(b / (size_t(nbx) * nby));
-    // pad with edge values (clamp addressing)
-    const int x = min(bx * 8 + lx, nx - 1), y = min(by * 8 + ly, ny - 1),
-              z = min(bz * 8 + lz, nz - 1);
-    bricks[o] = lin[(size_t(z) * ny + y) * nx + x];
+__global__ void k_cells(const float *__restrict__ lin, int nx, int ny, int nz,
+                        float4 *__restrict__ cells) {
+  // one thread per cell record: neighbouring threads read neighbouring
+  // voxels and write neighbouring 32-byte records
+  const size_t total = size_t(nx) * ny * nz;
+  for (size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x; c < total;
+       c += size_t(gridDim.x) * blockDim.x) {
+    const int x = int(c % nx), y = int((c / nx) % ny), z = int(c / (size_t(nx) * ny));
+    // the +1 neighbours, clamped to the last voxel (clamp addressing)
+    const int x1 = min(x + 1, nx - 1), y1 = min(y + 1, ny - 1), z1 = min(z + 1, nz - 1);
+    const float *r00 = lin + (size_t(z) * ny + y) * nx, *r10 = lin + (size_t(z) * ny + y1) * nx;
+    const float *r01 = lin + (size_t(z1) * ny + y) * nx, *r11 = lin + (size_t(z1) * ny + y1) * nx;
+    cells[2 * c] = make_float4(r00[x], r00[x1], r10[x], r10[x1]);
+    cells[2 * c + 1] = make_float4(r01[x], r01[x1], r11[x], r11[x1]);
   }
 }
 
@@ -2190,10 +2248,9 @@ __global__ void __launch_bounds__(256)
   float vmin = FLT_MAX, vmax = -FLT_MAX;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const int x = lo[0] + i % ex, y = lo[1] + (i / ex) % ey, z = lo[2] + i / (ex * ey);
-    const uint32_t o = uint32_t(x >> 3) * 512u + uint32_t(x & 7) +
-                       uint32_t(y >> 3) * v.bstride_y + (uint32_t(y & 7) << 3) +
-                       uint32_t(z >> 3) * v.bstride_z + (uint32_t(z & 7) << 6);
-    const float val = v.bricks[o];
+    // voxel (x, y, z) is the first corner of its own cell's record
+    const size_t cell = (size_t(z) * v.dims[1] + y) * v.dims[0] + x;
+    const float val = v.cells[2 * cell].x;
     vmin = fminf(vmin, val);
     vmax = fmaxf(vmax, val);
   }
@@ -2584,12 +2641,11 @@ int pathtrace_blocks_per_cu(const KScene &sc, size_t lds_bytes, bool nrc, bool g
   return n > 0 ? n : 1;
 }
 
-hipError_t launch_brickify(const float *lin, const int dims[3], float *bricks, hipStream_t s) {
-  const int nbx = (dims[0] + 7) / 8, nby = (dims[1] + 7) / 8, nbz = (dims[2] + 7) / 8;
-  const size_t total = size_t(nbx) * nby * nbz * 512;
+hipError_t launch_cells(const float *lin, const int dims[3], float *cells, hipStream_t s) {
+  const size_t total = size_t(dims[0]) * dims[1] * dims[2];
   const size_t blocks = std::min<size_t>((total + 255) / 256, 65536);
-  hipLaunchKernelGGL(k_brickify, dim3(unsigned(blocks)), dim3(256), 0, s, lin, dims[0], dims[1],
-                     dims[2], nbx, nby, nbz, bricks);
+  hipLaunchKernelGGL(k_cells, dim3(unsigned(blocks)), dim3(256), 0, s, lin, dims[0], dims[1],
+                     dims[2], reinterpret_cast<float4 *>(cells));
   return hipGetLastError();
 }
 

@@ -50,6 +50,9 @@ namespace {
 
 using namespace vrt;
 
+// structured fields: cells are addressed with 32-bit indices (float4 pairs)
+constexpr size_t kMaxCells = size_t(1) << 31;
+
 // shared by both field types: transfer function, majorants, instance
 int volume_common(const vrt_volume &in, int lds_offset, KVolume &v) {
   if (!in.tf_rgba || in.tf_size < 1)
@@ -65,6 +68,14 @@ int volume_common(const vrt_volume &in, int lds_offset, KVolume &v) {
   v.tf_lds_offset = lds_offset;
   v.vlo = in.value_lo;
   v.vhi = in.value_hi;
+  {
+    // (val - lo) / (hi - lo) equals (val - lo) * (1 / (hi - lo)) bit for bit
+    // when hi - lo is a power of two whose reciprocal is a normal float
+    const float range = in.value_hi - in.value_lo;
+    int e = 0;
+    v.vrange_pow2 = range > 0.f && std::frexp(range, &e) == 0.5f && e > -120 && e < 120;
+    v.vrange_inv = v.vrange_pow2 ? 1.f / range : 0.f;
+  }
   v.unit_distance = in.unit_distance;
   v.majorants = in.majorants;
   for (int a = 0; a < 3; ++a) v.gdims[a] = in.grid_dims[0] > 0 ? in.grid_dims[a] : 0;
@@ -94,7 +105,7 @@ int make_nanovdb_volume(const vrt_volume &in, int lds_offset, KVolume &v) {
   v.glo = v.blo;
   v.ghi = v.bhi;
   v.cell_size = 1.f;
-  v.bricks = g->leaves;
+  v.leaves = g->leaves;
   v.nv_top = g->top;
   v.nv_blocks = g->blocks;
   v.nv_consts = g->consts;
@@ -143,16 +154,12 @@ int make_volume(const vrt_volume &in, int lds_offset, KVolume &v) {
   for (int a = 0; a < 3; ++a)
     if (in.dims[a] < 1 || in.dims[a] > 4096)
       return fail(VRT_ERROR_INVALID_ARGUMENT, "volume dims[%d]=%d out of range", a, in.dims[a]);
-  if (!in.bricks) return fail(VRT_ERROR_INVALID_ARGUMENT, "volume has no voxel bricks");
-  const size_t nb = size_t((in.dims[0] + 7) / 8) * ((in.dims[1] + 7) / 8) * ((in.dims[2] + 7) / 8);
-  if (nb * 512 > 0xFFFFFFFFull)
-    return fail(VRT_ERROR_UNSUPPORTED, "volume exceeds 32-bit brick addressing");
+  if (!in.cells) return fail(VRT_ERROR_INVALID_ARGUMENT, "volume has no cell records");
+  if (size_t(in.dims[0]) * in.dims[1] * in.dims[2] > kMaxCells)
+    return fail(VRT_ERROR_UNSUPPORTED, "volume exceeds 2^31 cells (32-bit cell addressing)");
   v = KVolume{};
   v.field_type = in.field_type;
   for (int a = 0; a < 3; ++a) v.dims[a] = in.dims[a];
-  const uint32_t nbx = uint32_t((in.dims[0] + 7) / 8), nby = uint32_t((in.dims[1] + 7) / 8);
-  v.bstride_y = nbx * 512u;
-  v.bstride_z = nbx * nby * 512u;
   v.blo = mk(in.bounds_lo[0], in.bounds_lo[1], in.bounds_lo[2]);
   v.bhi = mk(in.bounds_hi[0], in.bounds_hi[1], in.bounds_hi[2]);
   // voxelSpaceTransform = (scaling(1/bounds.size()), -bounds.min)
@@ -160,7 +167,7 @@ int make_volume(const vrt_volume &in, int lds_offset, KVolume &v) {
   v.S = mk(1.f / (v.bhi.x - v.blo.x), 1.f / (v.bhi.y - v.blo.y), 1.f / (v.bhi.z - v.blo.z));
   v.T = -v.blo;
   v.cell_size = in.cell_size;
-  v.bricks = in.bricks;
+  v.cells = reinterpret_cast<const float4 *>(in.cells);
   v.glo = mk(0.f);
   v.ghi = mk(1.f);
   return volume_common(in, lds_offset, v);
@@ -473,18 +480,21 @@ int vrt_timer_stop(vrt_context ctx, float *ms) {
   return VRT_OK;
 }
 
-size_t vrt_structured_brick_bytes(const int dims[3]) {
+size_t vrt_structured_cell_bytes(const int dims[3]) {
   if (!dims || dims[0] < 1 || dims[1] < 1 || dims[2] < 1) return 0;
-  return size_t((dims[0] + 7) / 8) * ((dims[1] + 7) / 8) * ((dims[2] + 7) / 8) * 512 * sizeof(float);
+  const size_t n = size_t(dims[0]) * dims[1] * dims[2];
+  return n > kMaxCells ? 0 : n * 8 * sizeof(float);
 }
 
-int vrt_structured_brickify(vrt_context ctx, const float *lin, const int dims[3], float *bricks) {
+int vrt_structured_cells(vrt_context ctx, const float *lin, const int dims[3], float *cells) {
   if (int r = check_ctx(ctx)) return r;
-  if (!lin || !bricks || !dims) return fail(VRT_ERROR_INVALID_ARGUMENT, "null argument");
+  if (!lin || !cells || !dims) return fail(VRT_ERROR_INVALID_ARGUMENT, "null argument");
   for (int a = 0; a < 3; ++a)
     if (dims[a] < 1 || dims[a] > 4096)
       return fail(VRT_ERROR_INVALID_ARGUMENT, "dims[%d]=%d out of range", a, dims[a]);
-  VRT_HIP(launch_brickify(lin, dims, bricks, ctx->stream));
+  if (size_t(dims[0]) * dims[1] * dims[2] > kMaxCells)
+    return fail(VRT_ERROR_UNSUPPORTED, "field exceeds 2^31 cells (32-bit cell addressing)");
+  VRT_HIP(launch_cells(lin, dims, cells, ctx->stream));
   return VRT_OK;
 }
 

@@ -41,7 +41,7 @@ i3 = C.c_int * 3
 class Volume(C.Structure):
     _fields_ = [
         ("field_type", C.c_int), ("dims", i3), ("bounds_lo", f3), ("bounds_hi", f3),
-        ("cell_size", C.c_float), ("bricks", C.c_void_p), ("nanovdb", C.c_void_p),
+        ("cell_size", C.c_float), ("cells", C.c_void_p), ("nanovdb", C.c_void_p),
         ("filter", C.c_int), ("unstructured", C.c_void_p),
         ("tf_rgba", C.c_void_p), ("tf_size", C.c_int), ("value_lo", C.c_float),
         ("value_hi", C.c_float), ("unit_distance", C.c_float), ("majorants", C.c_void_p),
@@ -81,7 +81,7 @@ class Scene(C.Structure):
 class BvhInfo(C.Structure):
     _fields_ = [("num_nodes", C.c_uint32), ("num_triangles", C.c_uint32), ("depth", C.c_uint32),
                 ("bounds_lo", f3), ("bounds_hi", f3), ("build_ms", C.c_double),
-                ("device_bytes", C.c_uint64)]
+                ("device_bytes", C.c_uint64), ("stack_bound", C.c_uint32)]
 
 
 class Camera(C.Structure):
@@ -159,9 +159,9 @@ def lib():
         L.vrt_device_copy.argtypes = [vp, vp, vp, sz]
         L.vrt_timer_start.argtypes = [vp]
         L.vrt_timer_stop.argtypes = [vp, C.POINTER(C.c_float)]
-        L.vrt_structured_brick_bytes.argtypes = [C.POINTER(C.c_int)]
-        L.vrt_structured_brick_bytes.restype = sz
-        L.vrt_structured_brickify.argtypes = [vp, vp, C.POINTER(C.c_int), vp]
+        L.vrt_structured_cell_bytes.argtypes = [C.POINTER(C.c_int)]
+        L.vrt_structured_cell_bytes.restype = sz
+        L.vrt_structured_cells.argtypes = [vp, vp, C.POINTER(C.c_int), vp]
         L.vrt_build_value_ranges.argtypes = [vp, C.POINTER(Volume), vp]
         L.vrt_compute_majorants.argtypes = [vp, C.POINTER(Volume), vp, vp, vp, vp]
         L.vrt_frame_reset.argtypes = [vp, C.POINTER(Frame)]
@@ -520,9 +520,11 @@ class DeviceVolume:
         nz, ny, nx = voxels.shape
         self.dims = (nx, ny, nz)
         lin = ctx.upload(voxels)
-        nbytes = lib().vrt_structured_brick_bytes(_dims(self.dims))
-        self.bricks = ctx.buffer(nbytes)
-        check(lib().vrt_structured_brickify(ctx.handle, lin.ptr, _dims(self.dims), self.bricks.ptr))
+        nbytes = lib().vrt_structured_cell_bytes(_dims(self.dims))
+        if nbytes == 0:
+            raise VrtError(f"structured field {self.dims} exceeds 2^31 cells")
+        self.cells = ctx.buffer(nbytes)
+        check(lib().vrt_structured_cells(ctx.handle, lin.ptr, _dims(self.dims), self.cells.ptr))
         ctx.sync()
         lin.free()
         v = Volume()
@@ -533,7 +535,7 @@ class DeviceVolume:
         v.bounds_lo = f3(*lo)
         v.bounds_hi = f3(*hi)
         v.cell_size = float(np.min(np.array(spacing, np.float32)))
-        v.bricks = self.bricks.ptr
+        v.cells = self.cells.ptr
         self._finish(ctx, v, tf_rgba, value_range, unit_distance, grid_dims, xfm)
 
     def _finish(self, ctx, v, tf_rgba, value_range, unit_distance, grid_dims, xfm):

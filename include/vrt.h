@@ -48,8 +48,6 @@ typedef enum vrt_status {
 #define VRT_FILTER_LINEAR 0
 #define VRT_FILTER_NEAREST 1
 
-#define VRT_BRICK_LOG2 3 /* 8^3-voxel bricks, 2 KiB of fp32 each */
-
 /* A transferFunction1D volume over one spatial field, placed by one
  * instance.  Mirrors dco::Volume + dco::SpatialField + dco::GridAccel +
  * dco::TransferFunction1D (DeviceCopyableObjects.h:331-638) and the
@@ -60,11 +58,11 @@ typedef struct vrt_volume {
   float bounds_lo[3];  /* object-space bounds: origin ..            */
   float bounds_hi[3];  /*   origin + (dims-1)*spacing               */
   float cell_size;     /* min(spacing) (StructuredRegularField.cpp:40) */
-  /* structuredRegular: brick-ordered voxels, see vrt_structured_brickify */
-  const float *bricks;
+  /* structuredRegular: the cell records of vrt_structured_cells */
+  const float *cells;
   /* nanovdb: the converted grid (vrt_nanovdb_create, replacing the device
      copy of the grid buffer, NanoVDBField.cpp:29-64); bounds and cell size
-     then come from the grid and dims / bricks are unused */
+     then come from the grid and dims / cells are unused */
   struct vrt_nanovdb_s *nanovdb;
   int filter;          /* VRT_FILTER_*: nanovdb "filter" (NanoVDBField.cpp:27) */
   /* unstructured: the tetrahedral field (vrt_unstructured_create); bounds
@@ -273,15 +271,23 @@ VRT_API int vrt_timer_stop(vrt_context ctx, float *ms);
 /* Spatial field + majorant grid                                           */
 /* ---------------------------------------------------------------------- */
 
-/* Bytes needed for the brick-ordered copy of a dims[0]*dims[1]*dims[2]
- * field (each axis padded to whole 8-voxel bricks). */
-VRT_API size_t vrt_structured_brick_bytes(const int dims[3]);
+/* The renderers' layout of a structuredRegular field: one 32-byte record
+ * per voxel cell (i, j, k), i < dims[0], j < dims[1], k < dims[2], in
+ * x-fastest cell order, holding the 8 voxels a trilinear sample in the cell
+ * reads: v(i+dx, j+dy, k+dz) for dz, dy, dx in {0, 1} (dx fastest), indices
+ * clamped to the last voxel (clamp addressing).  A lookup is then one
+ * aligned 32-byte read (two 16-byte loads) at one address, instead of 8
+ * scattered taps over up to 4 cache lines; the price is 8x the voxel
+ * memory (a 512^3 fp32 field takes 4 GiB), which HBM has room for.
+ * Bytes needed for a dims[0]*dims[1]*dims[2] field; 0 for bad dims or a
+ * field of more than 2^31 cells. */
+VRT_API size_t vrt_structured_cell_bytes(const int dims[3]);
 
-/* x-fastest linear fp32 voxels (device) -> brick-ordered copy (device).
+/* x-fastest linear fp32 voxels (device) -> cell records (device).
  * Replaces the hipArray texture upload of StructuredRegularField::finalize
  * (StructuredRegularField.cpp:42-76). */
-VRT_API int vrt_structured_brickify(vrt_context ctx, const float *linear,
-                                    const int dims[3], float *bricks);
+VRT_API int vrt_structured_cells(vrt_context ctx, const float *linear, const int dims[3],
+                                 float *cells);
 
 /* NanoVDB float grid (NanoVDBField.cpp:25-65).  The host buffer of one
  * float grid in the NanoVDB 32.x format (the "data" array of a "nanovdb"
@@ -317,11 +323,20 @@ VRT_API int vrt_nanovdb_destroy(vrt_nanovdb grid);
  * triangle).  Built on the host by binned SAH, collapsed to 4-wide nodes and
  * uploaded; replaces the reference's per-geometry BVH build and upload
  * (DeviceBVH.h:153-255, 320; Triangle.cpp). */
+/* Entries of the kernels' per-ray traversal stack.  A built tree whose
+ * traversal could stack more (stack_bound below) is rejected with
+ * VRT_ERROR_UNSUPPORTED rather than traversed with dropped subtrees. */
+#define VRT_BVH_MAX_STACK 64
+
 typedef struct vrt_bvh_info {
   uint32_t num_nodes, num_triangles, depth;
   float bounds_lo[3], bounds_hi[3];
   double build_ms;
   uint64_t device_bytes;
+  /* most entries the traversal stack can hold for this tree (host build:
+     the exact maximum over root-to-leaf paths of the inner children each
+     node stacks; GPU build: 3 per 4-wide level below the root) */
+  uint32_t stack_bound;
 } vrt_bvh_info;
 
 VRT_API int vrt_bvh_build(vrt_context ctx, const float *vertices, int64_t num_vertices,
@@ -428,7 +443,7 @@ VRT_API int vrt_compute_majorants(vrt_context ctx, const vrt_volume *vol,
 
 /* Field values at n voxel-space points (device float3 array -> device
  * float array): sampleField (DeviceCopyableObjects.h:474-570) -- trilinear
- * on the brick layout, the NanoVDB sampler, or the containing tetrahedron's
+ * on the cell records, the NanoVDB sampler, or the containing tetrahedron's
  * value; a point outside every tetrahedron (sampleField returns false) gives
  * NaN. */
 VRT_API int vrt_sample_field(vrt_context ctx, const vrt_volume *vol, int n,

@@ -107,59 +107,127 @@ inline BoxHit<R> intersect_box(const Ray<R> &ray, V3<R> lo, V3<R> hi) {
   return h;
 }
 
-// powf of VolumeIntegration.h:416-417.  Parity mode (float) evaluates it
-// from IEEE basic operations only, so that CPU and GPU round identically:
-// x = m*2^e, m in [sqrt.5, sqrt2), ln m = 2s(1 + s^2/3 + s^4/5 + s^6/7 +
-// s^8/9), s = (m-1)/(m+1); 2^z = 2^n exp(f ln2), degree-7 Taylor, |f| <= 1/2.
-// Error <= 1.5 (2 + |y log2 x|) ulp against std::pow, pinned in tests.  Oracle mode
-// (double) calls std::pow.
+// x^y from IEEE basic operations (fused multiply-add, ldexp, rint) and a
+// table: the opacity correction powf(1 - alpha, dt/unitDistance)
+// (VolumeIntegration.h:416-417) evaluated so that it rounds identically on
+// every machine that follows these steps.  log2: x = 2^e z with z in
+// [0.69921875, 1.3984375) (float bits offset by 0x3f330000); z's subinterval
+// i (the next 6 mantissa bits) holds c_i (its midpoint, or exactly 1 for the
+// subinterval holding 1.0); the table keeps 1/c_i rounded to float and
+// -log2 of that float, r = fma(z, 1/c_i, -1), |r| <= 2^-7, and log2(1 + r)
+// is its Taylor series to r^4.  2^(y log2 x) = ldexp(2^f, n), n = rint, f the
+// rest, 2^f from its Taylor series to f^7.  Results below 2^-126 are 0.
+// Error <= 1.5 (2 + |y log2 x|) ulp against std::pow, pinned in tests, as is
+// the table.  This is parity mode (float); oracle mode (double) calls
+// std::pow.
+static const float kPowLog2[64][2] = {
+{1.4222222566604614f, -0.5081469416618347f},
+    {1.4065934419631958f, -0.4922053813934326f},
+    {1.39130437374115f, -0.47643807530403137f},
+    {1.376344084739685f, -0.46084117889404297f},
+    {1.3617020845413208f, -0.4454111158847809f},
+    {1.3473684787750244f, -0.43014445900917053f},
+    {1.3333333730697632f, -0.4150375425815582f},
+    {1.3195875883102417f, -0.4000871181488037f},
+    {1.3061224222183228f, -0.38529011607170105f},
+    {1.2929292917251587f, -0.37064337730407715f},
+    {1.2799999713897705f, -0.3561437726020813f},
+    {1.2673267126083374f, -0.34178850054740906f},
+    {1.2549020051956177f, -0.3275747001171112f},
+    {1.2427184581756592f, -0.31349948048591614f},
+    {1.2307692766189575f, -0.2995603382587433f},
+    {1.2190476655960083f, -0.285754531621933f},
+    {1.2075471878051758f, -0.27207955718040466f},
+    {1.1962616443634033f, -0.25853297114372253f},
+    {1.185185194015503f, -0.24511250853538513f},
+    {1.174311876296997f, -0.23181560635566711f},
+    {1.163636326789856f, -0.21864023804664612f},
+    {1.1531531810760498f, -0.20558416843414307f},
+    {1.1428571939468384f, -0.1926451474428177f},
+    {1.1327433586120605f, -0.17982102930545807f},
+    {1.1228070259094238f, -0.16710999608039856f},
+    {1.1130434274673462f, -0.15450988709926605f},
+    {1.1034482717514038f, -0.14201900362968445f},
+    {1.0940171480178833f, -0.12963534891605377f},
+    {1.0847457647323608f, -0.11735695600509644f},
+    {1.0756303071975708f, -0.10518231242895126f},
+    {1.0666667222976685f, -0.09310948103666306f},
+    {1.0578511953353882f, -0.08113670349121094f},
+    {1.049180269241333f, -0.06926257908344269f},
+    {1.0406503677368164f, -0.057485442608594894f},
+    {1.0322580337524414f, -0.04580364748835564f},
+    {1.0240000486373901f, -0.034215785562992096f},
+    {1.0158730745315552f, -0.022720159962773323f},
+    {1.0078740119934082f, -0.011315307579934597f},
+    {1.0f, -0.0f},
+    {0.9846153855323792f, 0.0223678108304739f},
+    {0.9696969985961914f, 0.044394075870513916f},
+    {0.9552238583564758f, 0.0660892203450203f},
+    {0.9411764740943909f, 0.08746283501386642f},
+    {0.9275362491607666f, 0.10852442681789398f},
+    {0.9142857193946838f, 0.12928301095962524f},
+    {0.9014084339141846f, 0.14974714815616608f},
+    {0.8888888955116272f, 0.16992498934268951f},
+    {0.8767123222351074f, 0.18982456624507904f},
+    {0.8648648858070374f, 0.20945332944393158f},
+    {0.8533333539962769f, 0.22881865501403809f},
+    {0.8421052694320679f, 0.2479275017976761f},
+    {0.8311688303947449f, 0.2667865455150604f},
+    {0.8205128312110901f, 0.28540220856666565f},
+    {0.8101266026496887f, 0.3037807047367096f},
+    {0.800000011920929f, 0.32192808389663696f},
+    {0.790123462677002f, 0.33984997868537903f},
+    {0.7804877758026123f, 0.35755205154418945f},
+    {0.7710843086242676f, 0.3750394880771637f},
+    {0.761904776096344f, 0.39231738448143005f},
+    {0.7529411911964417f, 0.4093908965587616f},
+    {0.7441860437393188f, 0.42626476287841797f},
+    {0.7356321811676025f, 0.4429435133934021f},
+    {0.7272727489471436f, 0.45943158864974976f},
+    {0.7191011309623718f, 0.4757334291934967f}};
+
 inline float pow_ieee(float x, float y) {
   if (y == 0.f || x == 1.f) return 1.f;
   if (!(x > 0.f)) return y > 0.f ? 0.f : std::numeric_limits<float>::infinity();
   uint32_t bits;
   std::memcpy(&bits, &x, 4);
-  int e = int((bits >> 23) & 255u);
-  if (e == 0) return y > 0.f ? 0.f : std::numeric_limits<float>::infinity();
-  if (e == 255) return x;
-  e -= 127;
-  uint32_t mb = (bits & 0x007FFFFFu) | 0x3F800000u;
-  float m;
-  std::memcpy(&m, &mb, 4);
-  if (m > 1.41421356f) {
-    m = m * 0.5f;
-    e += 1;
-  }
-  const float s = (m - 1.f) / (m + 1.f);
-  const float s2 = s * s;
-  float p = 0.111111111f;
-  p = p * s2 + 0.142857143f;
-  p = p * s2 + 0.2f;
-  p = p * s2 + 0.333333333f;
-  p = p * s2 + 1.f;
-  const float l2 = float(e) + (2.f * s * p) * 1.44269504f;
-  const float z = y * l2;
-  if (z < -150.f) return 0.f;
-  if (z > 128.f) return std::numeric_limits<float>::infinity();
-  const float n = std::floor(z + 0.5f);
-  const float t = (z - n) * 0.693147181f;
-  float q = 1.98412698e-4f;
-  q = q * t + 1.38888889e-3f;
-  q = q * t + 8.33333333e-3f;
-  q = q * t + 4.16666667e-2f;
-  q = q * t + 0.166666667f;
-  q = q * t + 0.5f;
-  q = q * t + 1.f;
-  q = q * t + 1.f;
-  int ni = int(n);
-  if (ni < -126) {
-    q = q * 5.87747175e-39f;
-    ni += 127;
-  }
-  uint32_t sb = uint32_t(ni + 127) << 23;
-  float sc;
-  std::memcpy(&sc, &sb, 4);
-  return q * sc;
+  if (bits < 0x00800000u) return y > 0.f ? 0.f : std::numeric_limits<float>::infinity();
+  if (bits >= 0x7f800000u) return x;
+  const uint32_t tmp = bits - 0x3f330000u;
+  const int e = int32_t(tmp) >> 23;
+  const float *t = kPowLog2[(tmp >> 17) & 63u];
+  const uint32_t zb = bits - (tmp & 0xff800000u);
+  float z;
+  std::memcpy(&z, &zb, 4);
+  const float r = std::fma(z, t[0], -1.f);
+  float q = std::fma(r, -0.36067376f, 0.48089835f);  // -1/(4 ln2), 1/(3 ln2)
+  q = std::fma(r, q, -0.72134752f);                   // -1/(2 ln2)
+  q = std::fma(r, q, 1.44269504f);                    // 1/ln2
+  const float l2 = std::fma(r, q, float(e) + t[1]);
+  const float zz = y * l2;
+  if (zz < -126.f) return 0.f;
+  if (zz > 128.f) return std::numeric_limits<float>::infinity();
+  const float n = std::rint(zz);
+  const float f = zz - n;
+  float p = 1.52527338e-5f;            // ln2^7 / 7!
+  p = std::fma(p, f, 1.54035304e-4f);  // ln2^6 / 6!
+  p = std::fma(p, f, 1.33335581e-3f);
+  p = std::fma(p, f, 9.61812911e-3f);
+  p = std::fma(p, f, 5.55041087e-2f);
+  p = std::fma(p, f, 0.240226507f);
+  p = std::fma(p, f, 0.693147181f);
+  p = std::fma(p, f, 1.f);
+  return std::ldexp(p, int(n));
 }
+
+// the table, for its pin (tests)
+extern "C" void orc_pow_log2_table(float *out) {
+  for (int i = 0; i < 64; ++i) {
+    out[2 * i] = kPowLog2[i][0];
+    out[2 * i + 1] = kPowLog2[i][1];
+  }
+}
+
 inline float opacity_pow(float x, float y) { return pow_ieee(x, y); }
 inline double opacity_pow(double x, double y) { return std::pow(x, y); }
 

@@ -202,8 +202,10 @@ void orc_pathtrace_accumulate(const orc_scene *scene, const orc_camera *cam,
 uint32_t orc_cvt_uint32(const float rgba[4]);
 uint32_t orc_cvt_uint32_srgb(const float rgba[4]);
 
-/* powf of the opacity correction from IEEE basic operations */
+/* powf of the opacity correction from IEEE basic operations, and its
+   64-entry log2 table {1/c_i, -log2(1/c_i)} */
 float orc_pow_ieee(float x, float y);
+void orc_pow_log2_table(float *out);
 
 /* camera ray for pixel coordinate (x, y) (parity unpinned: ANARI spec) */
 /* --- A5/A6: closest (any=0) or any (any=1) hit of n rays {ori, dir, tmin,

@@ -111,6 +111,7 @@ def lib():
         L.orc_sample_field_brute.argtypes = [C.POINTER(OrcVolume), ip, vp, vp]
         L.orc_hg_sample.argtypes = [C.c_float, vp, ip, C.c_uint32, vp]
         L.orc_uniform_sample_sphere.argtypes = [ip, C.c_uint32, vp]
+        L.orc_pow_log2_table.argtypes = [vp]
         L.orc_set_roulette.argtypes = [ip]
         L.orc_hg_4pi.restype = C.c_double
         L.orc_hg_4pi.argtypes = [C.c_double, C.c_double]

@@ -464,6 +464,71 @@ def test_frame_completion_callback(pkg, gpu_ctx):
         a.close()
 
 
+def test_device_released_inside_callback(pkg, gpu_ctx):
+    """The last references -- the frame's, then the device's -- released from
+    inside the frame's completion callback: the device is destroyed on the
+    completion thread while that callback runs, and the thread then ends
+    without touching the freed device; devices created afterwards work."""
+    import threading
+    import time
+    a = A.Anari()
+    made = []
+
+    def keep(h):
+        made.append(h)
+        return h
+
+    f = keep(a.new(A.SPATIAL_FIELD, "structuredRegular"))
+    a.set(f, "data", A.ARRAY3D, keep(a.array3d(np.full((8, 8, 8), 0.5, np.float32))))
+    a.set(f, "origin", A.FLOAT32_VEC3, (-0.5, -0.5, -0.5))
+    a.set(f, "spacing", A.FLOAT32_VEC3, (1 / 7, 1 / 7, 1 / 7))
+    v = keep(a.new(A.VOLUME, "transferFunction1D"))
+    a.set(v, "value", A.SPATIAL_FIELD, f)
+    a.set(v, "color", A.ARRAY1D, keep(a.array1d(np.ones((2, 3), np.float32), A.FLOAT32_VEC3)))
+    a.set(v, "opacity", A.ARRAY1D, keep(a.array1d(np.array([0, 1], np.float32), A.FLOAT32)))
+    a.commit(f, v)
+    world = keep(a.new(A.WORLD))
+    a.set(world, "volume", A.ARRAY1D, keep(a.objects([v], A.VOLUME)))
+    cam = keep(a.new(A.CAMERA, "perspective"))
+    a.set(cam, "position", A.FLOAT32_VEC3, (0, 0, 3))
+    a.set(cam, "direction", A.FLOAT32_VEC3, (0, 0, -1))
+    a.set(cam, "up", A.FLOAT32_VEC3, (0, 1, 0))
+    ren = keep(a.new(A.RENDERER, "raycast"))
+    a.commit(world, cam, ren)
+    fr = a.new(A.FRAME)
+    a.set(fr, "size", A.UINT32_VEC2, (32, 32))
+    a.set(fr, "channel.color", A.DATA_TYPE, A.FLOAT32_VEC4)
+    a.set(fr, "world", A.WORLD, world)
+    a.set(fr, "camera", A.CAMERA, cam)
+    a.set(fr, "renderer", A.RENDERER, ren)
+    done = threading.Event()
+
+    def last_callback(user, dev, frame):
+        a.L.anariRelease(dev, frame)
+        a.L.anariRelease(dev, dev)  # the device's last reference
+        done.set()
+
+    cb = A.FRAME_CALLBACK(last_callback)
+    a.set(fr, "frameCompletionCallback", A.FRAME_COMPLETION_CALLBACK,
+          C.cast(cb, C.c_void_p).value)
+    a.commit(fr)
+    a.release(*made)  # everything but the frame and the device
+    a.render(fr, wait=False)
+    assert done.wait(30.0)
+    a.dev = None  # released by the callback
+    time.sleep(0.2)  # the completion thread leaves its loop
+    errors = [m for m in a.messages if m[0] <= 1]
+    assert not errors, errors
+    b = A.Anari()
+    try:
+        fr2, _ = _anari_scene(b, _same_tf(pkg, CASES["structured raycast"](pkg.workloads)))
+        b.render(fr2)
+        img, _ = b.map(fr2)
+        assert np.abs(img).max() > 0
+    finally:
+        b.close()
+
+
 def test_taa_frames_match_abi(pkg, gpu_ctx):
     """Renderer "taa" / "taaAlpha" with the camera moved between frames by
     commits: the resolved frames are those of the C ABI with the previous

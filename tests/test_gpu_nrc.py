@@ -98,6 +98,42 @@ def test_adam_step_closed_form(pkg, gpu_ctx):
         nrc.close()
 
 
+def test_adam_step_count_skips_empty_batches(pkg, gpu_ctx):
+    """Mini-batches without samples leave Adam untouched, its bias-correction
+    step count included: after a cache frame whose camera sees no volume
+    (16 empty mini-batches), one training step still follows the first-step
+    closed form."""
+    import dataclasses
+    vrt, wls = pkg.vrt, pkg.workloads
+    lr, b1, b2, eps = 1e-2, 0.9, 0.99, 1e-8
+    nrc = vrt.Nrc(gpu_ctx, learning_rate=lr, beta1=b1, beta2=b2, epsilon=eps, max_batch=1024)
+    try:
+        wl = wls.fbm_pathtrace(n=16, width=16, height=8, frames=1, brick=8)
+        away = tuple(-x for x in wl.camera["direction"])
+        wl = dataclasses.replace(wl, camera=dict(wl.camera, direction=away))
+        gs = wls.GpuScene(gpu_ctx, wl)
+        opts = gs.nrc_options(train=True, train_samples=256, train_steps=16, query_vertex=1)
+        gs.render_nrc(nrc, opts, 0)
+        assert nrc.frame_counts()[1] == 0  # no training sample this frame
+        params = random_params(35)
+        nrc.set_params(params)
+        raw = random_raw(1024, 36)
+        target = np.random.default_rng(37).uniform(0, 1, (1024, 3)).astype(np.float32)
+        g, _ = nrc.gradients(raw, target)
+        dr, dt = gpu_ctx.upload(raw), gpu_ctx.upload(target)
+        nrc.train_step(dr, dt, 1024)
+        new = nrc.params()
+        gm = g.astype(np.float64) / 1024
+        m = (1 - b1) * gm
+        v = (1 - b2) * gm * gm
+        exp = params - lr * (m / (1 - b1)) / (np.sqrt(v / (1 - b2)) + eps)
+        np.testing.assert_allclose(new, exp, rtol=2e-5, atol=2e-7)
+        dr.free()
+        dt.free()
+    finally:
+        nrc.close()
+
+
 def test_training_reduces_loss(pkg, gpu_ctx):
     vrt = pkg.vrt
     nrc = vrt.Nrc(gpu_ctx, max_batch=16384, seed=3)

@@ -51,6 +51,63 @@ def test_bvh_hits_bit_exact(pkg, oracle, gpu_ctx, kind, builder):
     assert (po >= 0).mean() > 0.2
 
 
+def _clustered_mesh(rng, clusters, per_cluster=24, ratio=0.5):
+    """Triangle clusters along the x axis at x = ratio^i: binned SAH peels a few
+    clusters off per level, so the tree is deep and its nodes have several
+    inner children, which a ray along the axis hits one after another."""
+    tris = []
+    for i in range(clusters):
+        x = ratio ** i
+        s = 0.15 * x
+        c = np.array([x, 0.0, 0.0]) + rng.uniform(-s, s, (per_cluster, 1, 3))
+        tris.append(c + rng.normal(0.0, 0.3 * s, (per_cluster, 3, 3)))
+    return np.concatenate(tris).astype(np.float32).reshape(-1, 3)
+
+
+@pytest.mark.parametrize("builder", ["sah", "lbvh"])
+@pytest.mark.parametrize("clusters", [30, 60, 100])
+def test_bvh_deep_clustered_tree(pkg, oracle, gpu_ctx, builder, clusters):
+    """A deep tree either builds with a traversal-stack bound the kernels hold
+    and then hits bit-exactly as brute force (rays along the cluster axis
+    stack every sibling), or is rejected with an error -- never traversed
+    with dropped subtrees."""
+    vrt = pkg.vrt
+    rng = np.random.default_rng(clusters)
+    v = _clustered_mesh(rng, clusters)
+    try:
+        mesh = vrt.DeviceMesh(gpu_ctx, v, None, builder)
+    except vrt.VrtError as e:
+        assert "can stack" in str(e), e
+        print(builder, clusters, "rejected:", e)
+        return
+    info = mesh.info
+    print(builder, clusters, "depth", info.depth, "stack bound", info.stack_bound)
+    assert info.stack_bound <= 64
+    scene = vrt.make_scene([], [], [(mesh, (0.7, 0.7, 0.7))])
+    n = 4000
+    r = np.zeros((n, 8), np.float32)
+    r[:, 7] = 1e30
+    q = n // 4
+    # from beyond the largest cluster towards the origin
+    r[:q, 0] = rng.uniform(1.6, 2.0, q)
+    r[:q, 1:3] = rng.normal(0.0, 0.02, (q, 2))
+    d = np.stack([-np.ones(q), rng.normal(0, 0.01, q), rng.normal(0, 0.01, q)], 1)
+    r[:q, 3:6] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    # from the small end along the axis: the nearest children are the small
+    # clusters, so the walk descends the deep side and stacks the larger
+    # clusters at every level
+    r[q:, 0] = -0.5
+    r[q:, 3] = 1.0
+    r[2 * q:, 1:3] = rng.normal(0.0, 1e-4, (n - 2 * q, 2))
+    om = oracle.OracleMesh(v)
+    t, p = vrt.intersect_surfaces(gpu_ctx, scene, r, False)
+    to, po = oracle.mesh_intersect(om, r, any_hit=False, brute=True)
+    np.testing.assert_array_equal(p, po)
+    np.testing.assert_array_equal(t, to)
+    assert (po >= 0).mean() > 0.05
+    mesh.close()
+
+
 def _mixed(pkg, **kw):
     return pkg.workloads.mixed_pathtrace(level=4, vol_n=32, width=64, height=40, frames=1, **kw)
 

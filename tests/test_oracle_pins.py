@@ -297,6 +297,30 @@ def test_pow_ieee_error_bound(oracle):
     assert np.all(ulp[ok] <= 1.5 * (2 + z[ok]))
     assert L.orc_pow_ieee(0.0, 0.5) == 0.0 and L.orc_pow_ieee(1.0, 3.0) == 1.0
     assert L.orc_pow_ieee(0.25, 0.5) == 0.5
+    # x near 1 (the opacity correction of small opacities) keeps its digits
+    xs = (1.0 - np.geomspace(1e-7, 1e-2, 2000)).astype(np.float32)
+    got = np.array([L.orc_pow_ieee(float(x), 0.5) for x in xs], np.float64)
+    ref = np.power(xs.astype(np.float64), 0.5)
+    assert np.all(np.abs(got - ref) / np.spacing(ref.astype(np.float32)) <= 3.0)
+
+
+def test_pow_ieee_table(oracle):
+    """The log2 table of pow_ieee: subinterval i of z in [0.69921875,
+    1.3984375) spans float bits 0x3f330000 + [i, i+1) * 2^17; entry i is
+    {float(1/c_i), float(-log2(float(1/c_i)))} with c_i its midpoint, or 1
+    for the subinterval holding 1.0."""
+    L = oracle.lib()
+    tab = np.empty(128, np.float32)
+    L.orc_pow_log2_table(tab.ctypes.data)
+    tab = tab.reshape(64, 2)
+    bits = 0x3f330000 + (np.arange(65, dtype=np.uint32) << 17)
+    edges = bits.view(np.float32).astype(np.float64)
+    for i in range(64):
+        lo, hi = edges[i], edges[i + 1]
+        c = 1.0 if lo <= 1.0 < hi else (lo + hi) / 2
+        invc = np.float32(1.0 / c)
+        assert tab[i, 0] == invc, i
+        assert tab[i, 1] == np.float32(-np.log2(np.float64(invc))), i
 
 
 # ---------------------------------------------------------------------------
