@@ -199,12 +199,12 @@ struct Field : Object {
   uint64_t built = ~0ull;
   bool valid = false;
   vrt_volume desc{};  // field part: type, dims, bounds, cell size, data handles
-  DevMem cells;  // structuredRegular: the cell records of vrt_structured_cells
+  DevMem bricks;  // structuredRegular: the brick records of vrt_structured_bricks
   vrt_nanovdb nvdb = nullptr;
   vrt_unstructured tets = nullptr;
   int default_grid[3] = {16, 16, 16};
   void drop() {
-    cells.reset();
+    bricks.reset();
     if (nvdb) vrt_nanovdb_destroy(nvdb);
     if (tets) vrt_unstructured_destroy(tets);
     nvdb = nullptr;
@@ -502,11 +502,11 @@ bool build_field(Device *d, Field *f) {
     const int dims[3] = {int(a->n[0]), int(a->n[1]), int(a->n[2])};
     const size_t bytes = a->count() * 4;
     DevMem lin;
-    const size_t cell_bytes = vrt_structured_cell_bytes(dims);
-    if (cell_bytes == 0 || !lin.alloc(d->ctx, bytes) || !f->cells.alloc(d->ctx, cell_bytes) ||
+    const size_t brick_bytes = vrt_structured_brick_bytes(dims);
+    if (brick_bytes == 0 || !lin.alloc(d->ctx, bytes) || !f->bricks.alloc(d->ctx, brick_bytes) ||
         vrt_upload(d->ctx, lin.p, a->data(), bytes) != VRT_OK ||
-        vrt_structured_cells(d->ctx, static_cast<const float *>(lin.p), dims,
-                             static_cast<float *>(f->cells.p)) != VRT_OK ||
+        vrt_structured_bricks(d->ctx, static_cast<const float *>(lin.p), dims,
+                              static_cast<float *>(f->bricks.p)) != VRT_OK ||
         vrt_synchronize(d->ctx) != VRT_OK) {
       d->vrt_error(f, "structuredRegular upload");
       return false;
@@ -520,7 +520,7 @@ bool build_field(Device *d, Field *f) {
       v.bounds_hi[k] = origin[k] + (float(dims[k]) - 1.f) * spacing[k];
     }
     v.cell_size = std::min(spacing[0], std::min(spacing[1], spacing[2]));
-    v.cells = static_cast<const float *>(f->cells.p);
+    v.bricks = static_cast<const float *>(f->bricks.p);
     for (int k = 0; k < 3; ++k) f->default_grid[k] = 16;  // StructuredRegularField.cpp:100
   } else if (f->subtype == "nanovdb") {
     const Array *a = array_param(f, "data");
@@ -1053,6 +1053,8 @@ void render(Device *d, Frame *fr) {
   p.volume_sampling_rate = ren->get_f("volumeSamplingRate", 0.5f);
   p.phase_g = ren->get_f("phaseG", 0.f);  // extension
   p.gradient_shading = ren->get_b("gradientShading", false) ? 1 : 0;  // Renderer.cpp:50
+  // extension: shadow / AO transmittance by ratio tracking
+  p.ratio_tracking = ren->get_b("ratioTracking", false) ? 1 : 0;
   if (const Array *cp = array_param(ren, "clipPlane")) {
     if (cp->elem == ANARI_FLOAT32_VEC4) {
       p.num_clip_planes = int(std::min<uint64_t>(cp->count(), VRT_MAX_CLIP_PLANES));

@@ -7,8 +7,9 @@
 // Octrees, and k-d Trees", HPG 2012): 63-bit Morton codes of the triangle
 // box centres over their bounds, a radix sort (hipCUB), one thread per inner
 // node finding its key range and split from common-prefix lengths (ties
-// between equal codes broken by index), boxes refitted bottom-up (the
-// second child to arrive at a node computes its box), then a top-down
+// between equal codes broken by index), boxes refitted bottom-up one launch
+// per tree level (nodes bucketed by depth, deepest level first, so kernel
+// boundaries order the box writes), then a top-down
 // collapse into the 4-wide node layout of bvh_build.cpp, one launch per
 // level: a node opens its largest child while it has fewer than 4, and any
 // subtree of at most 4 triangles -- a contiguous run in Morton order --

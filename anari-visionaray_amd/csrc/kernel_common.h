@@ -39,7 +39,8 @@ struct KVolume {
   f3 blo, bhi;         // object-space bounds
   f3 S, T;             // voxel space = S * (p + T)
   float cell_size;
-  const float4 *cells; // structured: 2 float4 (the cell's 8 corner voxels) per voxel cell
+  const float *bricks; // structured: brick records of 9^3 voxels (VRT_BRICK_FLOATS each)
+  uint32_t nbx, nby;   // structured: bricks per x row, per y column
   const float *leaves; // nanovdb: 512 values per leaf
   const float4 *tf;    // global copy of the table
   int tf_size;
@@ -141,6 +142,7 @@ struct KParams {
   float4 clip[VRT_MAX_CLIP_PLANES];
   float phase_g;
   int gradient_shading;
+  int ratio_tracking;  // shadow / AO transmittance by ratio tracking
   // temporal anti-aliasing: previous and current view-projection (row-major
   // proj * view), blend weight
   int taa;

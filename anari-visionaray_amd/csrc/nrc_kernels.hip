@@ -1,14 +1,19 @@
 // Neural radiance cache on gfx950 matrix cores.
 //
 // Every layer is evaluated transposed, Z^T (out x M) = W (out x in) X^T
-// (in x M), with v_mfma_f32_16x16x16 (fp16 or bf16 operands, fp32
-// accumulation).  Lane l of a wave then holds, for a 16x16 block,
-//   A operand: W[l%16][4*(l/16) + j]        (j = 0..3)
-//   B operand: X^T[4*(l/16) + j][l%16]
-//   D result : Z^T[4*(l/16) + j][l%16]
-// so the accumulator of output block b of one layer, after bias + ReLU and
-// rounding to 16 bits, *is* the B operand of input block b of the next
-// layer: activations stay in registers from the encoding to the output.
+// (in x M), with gfx950's v_mfma_f32_16x16x32 (fp16 or bf16 operands, fp32
+// accumulation: twice the work of the 16x16x16 form per instruction).  Lane
+// l of a wave holds, for one 16 x 32 x 16 step,
+//   A operand: W[l%16][8*(l/16) + j]        (j = 0..7)
+//   B operand: X^T[8*(l/16) + j][l%16]
+//   D result : Z^T[4*(l/16) + j][l%16]      (j = 0..3)
+// A lane's results of output blocks 2b and 2b+1 (rows 4g..4g+3 of each,
+// g = l/16), after bias + ReLU and rounding to 16 bits, *are* its B operand
+// of input block b of the next layer, provided that layer reads its inputs
+// in the order kperm below: activations stay in registers from the encoding
+// to the output.  The 16-bit weight copies are stored with their input
+// columns in that order (layers 1-5), so each lane reads its 8 weights with
+// one 16-byte load.
 //
 // Training keeps the same per-wave forward (16 samples per wave, 64 per
 // workgroup), stores each layer's input X^T and the gradient dZ^T in LDS
@@ -26,6 +31,8 @@ namespace vrt {
 
 typedef _Float16 h4v __attribute__((ext_vector_type(4)));
 typedef short s4v __attribute__((ext_vector_type(4)));
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+typedef __bf16 b8v __attribute__((ext_vector_type(8)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
@@ -56,6 +63,21 @@ __device__ inline float unpack(const Frag &f, int j, int bf16) {
   return from16(uint16_t((j & 1) ? (w >> 16) : (w & 0xFFFFu)), bf16);
 }
 
+struct Frag8 {  // 8 x 16-bit operand of one lane (16x16x32 MFMA)
+  uint4 v;
+};
+__device__ inline Frag8 cat(const Frag &lo, const Frag &hi) {
+  Frag8 f;
+  f.v = make_uint4(lo.v.x, lo.v.y, hi.v.x, hi.v.y);
+  return f;
+}
+__device__ inline Frag8 load8(const uint16_t *p) {
+  Frag8 f;
+  f.v = *reinterpret_cast<const uint4 *>(p);
+  return f;
+}
+
+// 16x16x16: the output layer's backward pass (16 outputs)
 template <int BF16>
 __device__ inline f4v mfma(const Frag &a, const Frag &b, f4v c) {
   if constexpr (BF16) {
@@ -66,6 +88,27 @@ __device__ inline f4v mfma(const Frag &a, const Frag &b, f4v c) {
                                                  __builtin_bit_cast(h4v, b.v), c, 0, 0, 0);
   }
 }
+// 16x16x32: everything else
+template <int BF16>
+__device__ inline f4v mfma(const Frag8 &a, const Frag8 &b, f4v c) {
+  if constexpr (BF16) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8v, a.v),
+                                                   __builtin_bit_cast(b8v, b.v), c, 0, 0, 0);
+  } else {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8v, a.v),
+                                                  __builtin_bit_cast(h8v, b.v), c, 0, 0, 0);
+  }
+}
+
+// Order in which a layer fed by a previous layer's results reads its 64
+// inputs: in each 32-input block b, position 8g + j holds input
+// 32b + 4g + j (j < 4) or 32b + 16 + 4g + (j - 4) -- the rows lane group g
+// holds of output blocks 2b and 2b+1.  kperm maps an input index to its
+// position.
+__host__ __device__ constexpr int kperm(int c) {
+  return (c & ~31) + (((c & 15) >> 2) << 3) + (((c >> 4) & 1) << 2) + (c & 3);
+}
+__host__ __device__ constexpr int weight_col(int l, int c) { return l == 0 ? c : kperm(c); }
 
 // ---------------------------------------------------------------------------
 // Input encoding: feature f < 120 is sin/cos(2^k pi x_s) with s = f / 24,
@@ -95,12 +138,13 @@ __device__ inline void load_raw(const float *raw, int n, int q, float r[6]) {
 
 // ---------------------------------------------------------------------------
 // Forward pass of QB 16-query blocks per wave.  `hin` holds the layer-0 B
-// operands (8 k-blocks); the 3 outputs of query block qb are returned in
-// out[qb] (rows 0..3 of lanes 0..15).  Weights come through `wrow(l, row,
-// col)` -> pointer to 4 consecutive 16-bit weights.
+// operands (4 k-blocks of 32 features); the 3 outputs of query block qb are
+// returned in out[qb] (rows 0..3 of lanes 0..15).  Weights come through
+// `wrow(l, row, col)` -> pointer to 8 consecutive 16-bit weights of the
+// stored (kperm-ordered) row.
 // ---------------------------------------------------------------------------
 template <int BF16, int QB, class WPtr, class Store>
-__device__ inline void forward(const WPtr &wrow, const float *bias, Frag (&hin)[8][QB],
+__device__ inline void forward(const WPtr &wrow, const float *bias, Frag8 (&hin)[4][QB],
                                f4v (&out)[QB], const Store &store_act) {
   const int lane = threadIdx.x & 63;
   const int r16 = lane & 15, g = lane >> 4;
@@ -114,11 +158,10 @@ __device__ inline void forward(const WPtr &wrow, const float *bias, Frag (&hin)[
       for (int qb = 0; qb < QB; ++qb)
         for (int j = 0; j < 4; ++j) acc[ob][qb][j] = bias[nrc_boff(0) + ob * 16 + 4 * g + j];
 #pragma unroll
-    for (int kb = 0; kb < 8; ++kb)
+    for (int kb = 0; kb < 4; ++kb)
 #pragma unroll
       for (int ob = 0; ob < 4; ++ob) {
-        Frag a;
-        a.v = *reinterpret_cast<const uint2 *>(wrow(0, ob * 16 + r16, kb * 16 + 4 * g));
+        const Frag8 a = load8(wrow(0, ob * 16 + r16, kb * 32 + 8 * g));
 #pragma unroll
         for (int qb = 0; qb < QB; ++qb) acc[ob][qb] = mfma<BF16>(a, hin[kb][qb], acc[ob][qb]);
       }
@@ -131,7 +174,7 @@ __device__ inline void forward(const WPtr &wrow, const float *bias, Frag (&hin)[
         store_act(1, ob, qb, h[ob][qb]);
       }
   }
-  // hidden layers 1..4: 64 -> 64
+  // hidden layers 1..4: 64 -> 64; input block kb is output blocks 2kb, 2kb+1
 #pragma unroll 1
   for (int l = 1; l <= 4; ++l) {
     f4v acc[4][QB];
@@ -141,13 +184,13 @@ __device__ inline void forward(const WPtr &wrow, const float *bias, Frag (&hin)[
       for (int qb = 0; qb < QB; ++qb)
         for (int j = 0; j < 4; ++j) acc[ob][qb][j] = bias[nrc_boff(l) + ob * 16 + 4 * g + j];
 #pragma unroll
-    for (int kb = 0; kb < 4; ++kb)
+    for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
       for (int ob = 0; ob < 4; ++ob) {
-        Frag a;
-        a.v = *reinterpret_cast<const uint2 *>(wrow(l, ob * 16 + r16, kb * 16 + 4 * g));
+        const Frag8 a = load8(wrow(l, ob * 16 + r16, kb * 32 + 8 * g));
 #pragma unroll
-        for (int qb = 0; qb < QB; ++qb) acc[ob][qb] = mfma<BF16>(a, h[kb][qb], acc[ob][qb]);
+        for (int qb = 0; qb < QB; ++qb)
+          acc[ob][qb] = mfma<BF16>(a, cat(h[2 * kb][qb], h[2 * kb + 1][qb]), acc[ob][qb]);
       }
 #pragma unroll
     for (int ob = 0; ob < 4; ++ob)
@@ -163,24 +206,24 @@ __device__ inline void forward(const WPtr &wrow, const float *bias, Frag (&hin)[
   for (int qb = 0; qb < QB; ++qb)
     for (int j = 0; j < 4; ++j) out[qb][j] = bias[nrc_boff(5) + 4 * g + j];
 #pragma unroll
-  for (int kb = 0; kb < 4; ++kb) {
-    Frag a;
-    a.v = *reinterpret_cast<const uint2 *>(wrow(5, r16, kb * 16 + 4 * g));
+  for (int kb = 0; kb < 2; ++kb) {
+    const Frag8 a = load8(wrow(5, r16, kb * 32 + 8 * g));
 #pragma unroll
-    for (int qb = 0; qb < QB; ++qb) out[qb] = mfma<BF16>(a, h[kb][qb], out[qb]);
+    for (int qb = 0; qb < QB; ++qb)
+      out[qb] = mfma<BF16>(a, cat(h[2 * kb][qb], h[2 * kb + 1][qb]), out[qb]);
   }
 }
 
 // ---------------------------------------------------------------------------
 // Inference: persistent workgroups, weights staged in LDS with rows padded by
-// 4 halves (row stride 8 B off a multiple of 128 B: the 16 rows one
-// ds_read_b64 touches spread over all banks); 64 queries per wave.
+// 8 halves (row stride 16 B off a multiple of 128 B: the 16 rows one
+// ds_read_b128 phase touches spread over all banks); 64 queries per wave.
 // ---------------------------------------------------------------------------
-__host__ __device__ constexpr int lds_stride(int l) { return nrc_in(l) + 4; }
+__host__ __device__ constexpr int lds_stride(int l) { return nrc_in(l) + 8; }
 __host__ __device__ constexpr int lds_woff(int l) {
   return l == 0 ? 0 : 64 * lds_stride(0) + (l - 1) * 64 * lds_stride(1);
 }
-constexpr int kLdsWeights = 64 * (128 + 4) + 4 * 64 * (64 + 4) + 16 * (64 + 4);
+constexpr int kLdsWeights = 64 * (128 + 8) + 4 * 64 * (64 + 8) + 16 * (64 + 8);
 
 // Sample range [lo, lo + cnt) of mini-batch `batch` of `nbatches` over the
 // first min(n, *n_dev) samples (n_dev: a count produced on the GPU, or null).
@@ -199,7 +242,7 @@ __global__ void __launch_bounds__(256) k_nrc_infer(NrcDevice d, int n, const uin
     int lo;
     batch_range(n, n_dev, 0, 1, lo, n);
   }
-  __shared__ uint16_t sw[kLdsWeights];
+  __shared__ __align__(16) uint16_t sw[kLdsWeights];
   __shared__ float sb[kNrcBiases];
   for (int l = 0; l < kNrcLayers; ++l) {
     const int rows = nrc_out(l), cols = nrc_in(l);
@@ -218,15 +261,18 @@ __global__ void __launch_bounds__(256) k_nrc_infer(NrcDevice d, int n, const uin
   constexpr int QB = 4;
   const int tiles = (n + 63) / 64;  // 64 queries (QB blocks of 16) per wave-tile
   for (int t = blockIdx.x * 4 + wave; t < tiles; t += gridDim.x * 4) {
-    Frag hin[8][QB];
+    Frag8 hin[4][QB];
 #pragma unroll
     for (int qb = 0; qb < QB; ++qb) {
       float r[6];
       load_raw(raw, n, t * 64 + qb * 16 + r16, r);
 #pragma unroll
-      for (int kb = 0; kb < 8; ++kb) {
-        const int f = kb * 16 + 4 * g;
-        hin[kb][qb] = pack4(encode(r, f), encode(r, f + 1), encode(r, f + 2), encode(r, f + 3), BF16);
+      for (int kb = 0; kb < 4; ++kb) {
+        const int f = kb * 32 + 8 * g;
+        hin[kb][qb] = cat(pack4(encode(r, f), encode(r, f + 1), encode(r, f + 2), encode(r, f + 3),
+                                BF16),
+                          pack4(encode(r, f + 4), encode(r, f + 5), encode(r, f + 6),
+                                encode(r, f + 7), BF16));
       }
     }
     f4v o[QB];
@@ -250,7 +296,7 @@ __global__ void __launch_bounds__(256) k_nrc_infer(NrcDevice d, int n, const uin
 //   L = sum_q sum_c (p - t)^2 / (p^2 + 0.01),  dL/dp = 2 (p - t) / (p^2 + 0.01)
 // with p^2 in the denominator treated as a constant.
 // ---------------------------------------------------------------------------
-constexpr int kTS = 64 + 4;  // LDS row stride of [feature][sample] tiles (halves)
+constexpr int kTS = 64 + 8;  // LDS row stride of [feature][sample] tiles (halves, 16-B rows)
 
 template <int BF16>
 __global__ void __launch_bounds__(256) k_nrc_grad(NrcDevice d, int n, const uint32_t *n_dev,
@@ -263,9 +309,9 @@ __global__ void __launch_bounds__(256) k_nrc_grad(NrcDevice d, int n, const uint
     raw += 6 * size_t(lo);
     target += 3 * size_t(lo);
   }
-  __shared__ uint16_t sx0[128 * kTS];      // layer-0 input X^T [feature][sample]
-  __shared__ uint16_t sx[5][64 * kTS];     // inputs of layers 1..5
-  __shared__ uint16_t sdz[64 * kTS];       // dZ^T of the current layer [out][sample]
+  __shared__ __align__(16) uint16_t sx0[128 * kTS];   // layer-0 input X^T [feature][sample]
+  __shared__ __align__(16) uint16_t sx[5][64 * kTS];  // inputs of layers 1..5
+  __shared__ __align__(16) uint16_t sdz[64 * kTS];    // dZ^T of the current layer [out][sample]
   __shared__ float sdb[kNrcBiases];        // bias-gradient sums of this workgroup
   __shared__ float sloss[4];
 
@@ -301,14 +347,14 @@ __global__ void __launch_bounds__(256) k_nrc_grad(NrcDevice d, int n, const uint
     const bool valid = q < n;
     float r[6];
     load_raw(raw, n, q, r);
-    Frag hin[8][1];
+    Frag8 hin[4][1];
 #pragma unroll
-    for (int kb = 0; kb < 8; ++kb) {
-      const int f = kb * 16 + 4 * g;
-      float e[4];
-      for (int j = 0; j < 4; ++j) e[j] = valid ? encode(r, f + j) : 0.f;
-      hin[kb][0] = pack4(e[0], e[1], e[2], e[3], BF16);
-      for (int j = 0; j < 4; ++j) sx0[(f + j) * kTS + col] = to16(e[j], BF16);
+    for (int kb = 0; kb < 4; ++kb) {
+      const int f = kb * 32 + 8 * g;
+      float e[8];
+      for (int j = 0; j < 8; ++j) e[j] = valid ? encode(r, f + j) : 0.f;
+      hin[kb][0] = cat(pack4(e[0], e[1], e[2], e[3], BF16), pack4(e[4], e[5], e[6], e[7], BF16));
+      for (int j = 0; j < 8; ++j) sx0[(f + j) * kTS + col] = to16(e[j], BF16);
     }
     f4v out[1];
     forward<BF16, 1>(wrow, d.bias, hin, out, store_act);
@@ -353,17 +399,16 @@ __global__ void __launch_bounds__(256) k_nrc_grad(NrcDevice d, int n, const uint
         }
       }
       __syncthreads();
-      // dW tiles (ob, ib) over the chunk's 64 samples
+      // dW tiles (ob, ib) over the chunk's 64 samples (k = sample)
       const uint16_t *xin = l == 0 ? sx0 : sx[l - 1];
       const int ntiles = nob * nib;
       for (int t = wave, k = 0; t < ntiles; t += 4, ++k) {
         const int ob = t / nib, ib = t % nib;
         f4v acc = l == 0 ? acc0[k] : (l == 5 ? acc5 : acc14[l - 1][k]);
 #pragma unroll
-        for (int kb = 0; kb < 4; ++kb) {
-          Frag a, b;
-          a.v = *reinterpret_cast<const uint2 *>(&sdz[(ob * 16 + r16) * kTS + kb * 16 + 4 * g]);
-          b.v = *reinterpret_cast<const uint2 *>(&xin[(ib * 16 + r16) * kTS + kb * 16 + 4 * g]);
+        for (int kb = 0; kb < 2; ++kb) {
+          const Frag8 a = load8(&sdz[(ob * 16 + r16) * kTS + kb * 32 + 8 * g]);
+          const Frag8 b = load8(&xin[(ib * 16 + r16) * kTS + kb * 32 + 8 * g]);
           acc = mfma<BF16>(a, b, acc);
         }
         if (l == 0)
@@ -374,14 +419,19 @@ __global__ void __launch_bounds__(256) k_nrc_grad(NrcDevice d, int n, const uint
           acc14[l - 1][k] = acc;
       }
       if (l > 0) {
-        // dX^T = W^T dZ^T, then the ReLU mask of layer l-1's output (= X of l)
+        // dX^T = W^T dZ^T (k = output of layer l), then the ReLU mask of
+        // layer l-1's output (= X of l).  64 outputs: two 32-blocks read in
+        // kperm order; the output layer's 16: one 16x16x16 step
         for (int ib = 0; ib < nib; ++ib) {
           f4v acc = f4v{0.f, 0.f, 0.f, 0.f};
-          for (int kb = 0; kb < nob; ++kb) {
+          const uint16_t *wt = &d.wt[nrc_woff(l) + (ib * 16 + r16) * nrc_out(l)];
+          if (nob == 1) {
             Frag a;
-            a.v = *reinterpret_cast<const uint2 *>(
-                &d.wt[nrc_woff(l) + (ib * 16 + r16) * nrc_out(l) + kb * 16 + 4 * g]);
-            acc = mfma<BF16>(a, dzf[kb], acc);
+            a.v = *reinterpret_cast<const uint2 *>(wt + 4 * g);
+            acc = mfma<BF16>(a, dzf[0], acc);
+          } else {
+            for (int kb = 0; kb < 2; ++kb)
+              acc = mfma<BF16>(load8(wt + kb * 32 + 8 * g), cat(dzf[2 * kb], dzf[2 * kb + 1]), acc);
           }
           for (int j = 0; j < 4; ++j) {
             const float x = from16(xin[(ib * 16 + 4 * g + j) * kTS + col], BF16);
@@ -431,8 +481,10 @@ __device__ inline void emit16(const NrcDevice &d, int p, float v) {
     const int local = p - nrc_woff(l);
     const int o = local / nrc_in(l), i = local % nrc_in(l);
     const uint16_t b = to16(v, d.bf16);
-    d.w[p] = b;
-    d.wt[nrc_woff(l) + i * nrc_out(l) + o] = b;
+    // W [out][in], inputs of layers 1-5 in kperm order; W^T [in][out],
+    // outputs in kperm order where they are 64 (layers 1-4)
+    d.w[nrc_woff(l) + o * nrc_in(l) + weight_col(l, i)] = b;
+    d.wt[nrc_woff(l) + i * nrc_out(l) + (nrc_out(l) == 64 ? kperm(o) : o)] = b;
   } else {
     d.bias[p - kNrcWeights] = v;
   }
@@ -505,14 +557,18 @@ __global__ void k_nrc_init(NrcDevice d, uint64_t seed) {
   if (p < 4) d.stats[p] = 0.f;  // loss, samples, Adam step count
 }
 
-// one-wave MFMA lane-layout probe
+// one-wave MFMA lane-layout probe of the 16x16x32 instruction: C (16x16)
+// = A (16x32) B (32x16) with the operand layout the kernels assume
 template <int BF16>
 __global__ void k_mfma_probe(const float *a, const float *b, float *c) {
   const int l = threadIdx.x & 63, r16 = l & 15, g = l >> 4;
-  Frag fa = pack4(a[r16 * 16 + 4 * g], a[r16 * 16 + 4 * g + 1], a[r16 * 16 + 4 * g + 2],
-                  a[r16 * 16 + 4 * g + 3], BF16);
-  Frag fb = pack4(b[(4 * g) * 16 + r16], b[(4 * g + 1) * 16 + r16], b[(4 * g + 2) * 16 + r16],
-                  b[(4 * g + 3) * 16 + r16], BF16);
+  const float *ar = a + r16 * 32 + 8 * g;
+  const Frag8 fa = cat(pack4(ar[0], ar[1], ar[2], ar[3], BF16),
+                       pack4(ar[4], ar[5], ar[6], ar[7], BF16));
+  float bv[8];
+  for (int j = 0; j < 8; ++j) bv[j] = b[(8 * g + j) * 16 + r16];
+  const Frag8 fb = cat(pack4(bv[0], bv[1], bv[2], bv[3], BF16),
+                       pack4(bv[4], bv[5], bv[6], bv[7], BF16));
   f4v acc = f4v{0.f, 0.f, 0.f, 0.f};
   acc = mfma<BF16>(fa, fb, acc);
   for (int j = 0; j < 4; ++j) c[(4 * g + j) * 16 + r16] = acc[j];

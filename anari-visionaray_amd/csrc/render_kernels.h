@@ -31,7 +31,7 @@ hipError_t launch_relmse(const float4 *x, const double *ref_sum, double ref_scal
                          float eps, double *out, hipStream_t s);
 int pathtrace_blocks_per_cu(const KScene &sc, size_t lds_bytes, bool nrc = false,
                             bool grad = false);
-hipError_t launch_cells(const float *lin, const int dims[3], float *cells, hipStream_t s);
+hipError_t launch_bricks(const float *lin, const int dims[3], float *bricks, hipStream_t s);
 hipError_t launch_value_ranges(const KVolume &v, float *ranges, hipStream_t s);
 // nanovdb: min/max (float2) of each block slot's values
 hipError_t launch_nanovdb_slot_ranges(const int32_t *slots, size_t nslots, const float *leaves,

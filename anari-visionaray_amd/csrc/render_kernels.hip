@@ -110,8 +110,9 @@ __device__ inline KRay to_local(const KVolume &v, const KRay &r) {
 
 // ---------------------------------------------------------------------------
 // Field fetch: trilinear, texel-centre convention (p*N-0.5), clamp
-// addressing, on the cell records of vrt_structured_cells: the cell index
-// and weight per axis, then one 32-byte record holds all 8 taps.
+// addressing, on the brick records of vrt_structured_bricks: the cell index
+// and weight per axis give one brick record holding all 8 taps at fixed
+// offsets from one address.
 // ---------------------------------------------------------------------------
 // Clamping c to [0, n-1] gives the clamp-addressed taps exactly: below 0
 // both taps are voxel 0 (the weight 0 keeps it), above n-1 both are voxel
@@ -240,12 +241,13 @@ __device__ inline float sample_structured(const KVolume &v, f3 P) {
   const int i = axis_cell(P.x, v.dims[0], wx);
   const int j = axis_cell(P.y, v.dims[1], wy);
   const int k = axis_cell(P.z, v.dims[2], wz);
-  const uint32_t cell = (uint32_t(k) * uint32_t(v.dims[1]) + uint32_t(j)) * uint32_t(v.dims[0]) +
-                        uint32_t(i);
-  const float4 *r = v.cells + 2 * size_t(cell);
-  const float4 z0 = r[0], z1 = r[1];  // (v000, v100, v010, v110), (v001, v101, v011, v111)
-  float c00 = lerpf(z0.x, z0.y, wx), c10 = lerpf(z0.z, z0.w, wx);
-  float c01 = lerpf(z1.x, z1.y, wx), c11 = lerpf(z1.z, z1.w, wx);
+  const uint32_t brick = (uint32_t(k >> 3) * v.nby + uint32_t(j >> 3)) * v.nbx + uint32_t(i >> 3);
+  const float *b = v.bricks + size_t(brick) * VRT_BRICK_FLOATS +
+                   uint32_t(((k & 7) * 9 + (j & 7)) * 9 + (i & 7));
+  const float v000 = b[0], v100 = b[1], v010 = b[9], v110 = b[10];
+  const float v001 = b[81], v101 = b[82], v011 = b[90], v111 = b[91];
+  float c00 = lerpf(v000, v100, wx), c10 = lerpf(v010, v110, wx);
+  float c01 = lerpf(v001, v101, wx), c11 = lerpf(v011, v111, wx);
   return lerpf(lerpf(c00, c10, wy), lerpf(c01, c11, wy), wz);
 }
 
@@ -691,6 +693,7 @@ struct Trav {
   float maj, scale;
   int cells_left;   // < 0: no majorant grid, one range with majorant 1
   bool in_cell;
+  float tr;         // ratio tracking: the transmittance estimate so far
 };
 
 // false: the ray misses the volume's box
@@ -710,6 +713,7 @@ __device__ inline bool trav_begin(const KVolume &v, KRay ray, Trav &T) {
   T.invUD = 1.f / (v.unit_distance * T.dt_scale);
   T.t0 = 0.f;
   T.in_cell = false;
+  T.tr = 1.f;
   if (v.gdims[0] == 0) {
     T.tm = tmax;
     T.cells_left = -1;
@@ -751,8 +755,10 @@ __device__ inline bool trav_begin(const KVolume &v, KRay ray, Trav &T) {
 // One step of the walk.  Returns true when it ended: on a collision (`hit`,
 // voxel-space `t_hit`, classified sample `smp`) or on leaving the grid.
 // `mb`/`mstep`: the volume's majorant bytes in LDS and their step (mb null:
-// the float majorants in global memory).
-template <bool ANYF = true>
+// the float majorants in global memory).  RATIO: ratio tracking -- a
+// tentative collision multiplies T.tr by 1 - alpha/majorant and the walk goes
+// on (no acceptance draw), so it ends only on leaving the grid.
+template <bool ANYF = true, bool RATIO = false>
 __device__ inline bool trav_step(const KVolume &v, const float4 *lds, const uint8_t *mb,
                                  float mstep, Trav &T, uint32_t &rs, uint32_t &lookups,
                                  bool &hit, float &t_hit, float4 &smp, uint32_t &rec_hit) {
@@ -798,8 +804,10 @@ __device__ inline bool trav_step(const KVolume &v, const float4 *lds, const uint
       // no acceptance draw where the field has no value
       // (DeviceCopyableObjects.h:739-753)
       if (sample_field<ANYF>(v, T.ori + T.dir * T.t, val, rec)) {
-        // colour only for an accepted collision
-        if (classify_alpha(v, lds, val) >= rnd(rs) * T.maj) {
+        if (RATIO) {
+          T.tr = T.tr * (1.f - classify_alpha(v, lds, val) / T.maj);
+        } else if (classify_alpha(v, lds, val) >= rnd(rs) * T.maj) {
+          // colour only for an accepted collision
           hit = true;
           t_hit = T.t;
           smp = classify(v, lds, val);
@@ -843,6 +851,23 @@ __device__ inline VolHit make_hit(bool hit, float t_hit, float4 smp, float dt_sc
   out.albedo = mk(smp.x, smp.y, smp.z);
   out.extinction = smp.w;
   return out;
+}
+
+// Ratio tracking of a shadow / AO ray through one volume (an extension: the
+// same expectation as the binary transmittance of delta tracking): the
+// estimate Tr in [0, 1]
+template <bool ANYF = true>
+__device__ inline float ratio_track(const KVolume &v, const float4 *lds, const uint8_t *mb,
+                                    float mstep, KRay ray, uint32_t &rs, uint32_t &lookups) {
+  Trav T;
+  bool hit = false;
+  float t_hit = 0.f;
+  float4 smp;
+  uint32_t rec;
+  if (!trav_begin(v, ray, T)) return 1.f;
+  while (!trav_step<ANYF, true>(v, lds, mb, mstep, T, rs, lookups, hit, t_hit, smp, rec)) {
+  }
+  return T.tr;
 }
 
 // intersect(Ray, Volume) (DeviceCopyableObjects.h:680-796) in one call
@@ -1543,6 +1568,7 @@ enum HitKind { kNone = 0, kVolume = 1, kSurface = 2, kLight = 3 };
 struct PassHit {
   int kind;
   bool surf_hit;     // any surface hit in [tmin, tmax], whichever kind wins
+  float trans;       // ratio tracking: the volumes' transmittance estimate (else 1)
   float t;
   f3 albedo;         // volume: classified color
   float extinction;  // volume: classified opacity
@@ -1556,7 +1582,8 @@ struct PassHit {
 template <bool AREA, bool ANYF>
 __device__ inline PassHit intersect_all(const KScene &sc, const KRay &ray, bool shadow,
                                         int bounceID, const float4 *s_tf, const float *steps,
-                                        uint32_t &srng, uint32_t &lookups, int2 *bstack) {
+                                        uint32_t &srng, uint32_t &lookups, int2 *bstack,
+                                        bool ratio = false) {
   bool sh = false;
   float st = ray.tmax;
   int sid = -1;
@@ -1599,7 +1626,13 @@ __device__ inline PassHit intersect_all(const KScene &sc, const KRay &ray, bool 
   vh.hit = false;
   vh.t = FLT_MAX;
   int vid = -1;
+  float trans = 1.f;
   for (int i = 0; i < sc.num_volumes; ++i) {
+    if (ratio) {  // shadow / AO ray: the product of the volumes' estimates
+      trans = trans * ratio_track<ANYF>(sc.vol[i], s_tf, lds_majorants(sc, s_tf, i), steps[i],
+                                        to_local(sc.vol[i], ray), srng, lookups);
+      continue;
+    }
     VolHit h = delta_track<ANYF>(sc.vol[i], s_tf, lds_majorants(sc, s_tf, i), steps[i],
                                  to_local(sc.vol[i], ray), srng, lookups);
     if (h.hit && h.t < vh.t) {
@@ -1610,6 +1643,7 @@ __device__ inline PassHit intersect_all(const KScene &sc, const KRay &ray, bool 
   PassHit out;
   out.kind = kNone;
   out.surf_hit = sh;
+  out.trans = trans;
   out.t = FLT_MAX;
   out.id = -1;
   out.rec = 0;
@@ -1906,13 +1940,15 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
   float vis_ao = 1.f;
   if (rayType == Shadow) {
     // visibility = surfV * volV * Tr (Pathtrace_impl.cpp:439-444): blocked
-    // by any surface hit or by a volume collision that wins the pass
-    st.direct = st.direct * ((h.surf_hit || h.kind == kVolume) ? 0.f : 1.f);
+    // by any surface hit or by a volume collision that wins the pass; with
+    // ratio tracking Tr is the volumes' transmittance estimate
+    st.direct = st.direct * ((h.surf_hit || h.kind == kVolume) ? 0.f : h.trans);
   } else {  // AO
     st.aoSamples++;
     const float weight = fmaxf(0.f, dot(ray.dir, st.sn));
     st.aoWeights += weight;
-    if (weight > 0.f && hit) st.aoCount += weight;
+    // occluded by any hit; with ratio tracking the volumes occlude by 1 - Tr
+    if (weight > 0.f) st.aoCount += weight * (hit ? 1.f : 1.f - h.trans);
     if (st.aoSamples >= P.ambient_samples)
       vis_ao = st.aoWeights > 0.f ? 1.f - st.aoCount / st.aoWeights : 0.f;
   }
@@ -1991,7 +2027,7 @@ __global__ void __launch_bounds__(256, 3)
 
   PathState st;
   st.pixel = -1;
-  uint32_t lookups = 0;
+  uint32_t lookups = 0, hit_samples = 0;
   int q = int(blockIdx.x) & (kQueues - 1);  // wave-uniform queue
   int qtries = 0;
   uint32_t pool_base = 0, pool_left = 0;    // wave-uniform local pool
@@ -2050,11 +2086,14 @@ __global__ void __launch_bounds__(256, 3)
       ray.tmin = 0.f;
       ray.tmax = st.tmax;
       clip_ray(ray, P);  // every pass (Pathtrace_impl.cpp:559)
-      const PassHit hit = intersect_all<AREA, ANYF>(sc, ray, st.rayType != Radiance, st.bounceID,
-                                                    s_tf, steps, st.srng, lookups, bstack);
+      const bool shadow = st.rayType != Radiance;
+      const PassHit hit = intersect_all<AREA, ANYF>(sc, ray, shadow, st.bounceID, s_tf, steps,
+                                                    st.srng, lookups, bstack,
+                                                    shadow && P.ratio_tracking != 0);
       if (path_shade<NRC, AREA, ANYF, GRAD>(st, sc, P, N, F, hit)) {
         float4 color = P.bg;
         if (st.firstHit) color = make_float4(st.intensity.x, st.intensity.y, st.intensity.z, 1.f);
+        hit_samples += st.firstHit ? 1u : 0u;
         st.acc.x += color.x;
         st.acc.y += color.y;
         st.acc.z += color.z;
@@ -2103,7 +2142,10 @@ __global__ void __launch_bounds__(256, 3)
       }
     }
   }
-  if (F.counters) wave_add_counter(&F.counters[0], lookups);
+  if (F.counters) {
+    wave_add_counter(&F.counters[0], lookups);
+    wave_add_counter(&F.counters[1], hit_samples);
+  }
 }
 
 // TAA resolve (Frame.cpp:339-381, DeviceCopyableObjects.h:2657-2712): the
@@ -2210,20 +2252,25 @@ __global__ void k_nrc_resolve(KFrame F, KNrc N, int accumID) {
 // ---------------------------------------------------------------------------
 // Field preparation
 // ---------------------------------------------------------------------------
-__global__ void k_cells(const float *__restrict__ lin, int nx, int ny, int nz,
-                        float4 *__restrict__ cells) {
-  // one thread per cell record: neighbouring threads read neighbouring
-  // voxels and write neighbouring 32-byte records
-  const size_t total = size_t(nx) * ny * nz;
-  for (size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x; c < total;
-       c += size_t(gridDim.x) * blockDim.x) {
-    const int x = int(c % nx), y = int((c / nx) % ny), z = int(c / (size_t(nx) * ny));
-    // the +1 neighbours, clamped to the last voxel (clamp addressing)
-    const int x1 = min(x + 1, nx - 1), y1 = min(y + 1, ny - 1), z1 = min(z + 1, nz - 1);
-    const float *r00 = lin + (size_t(z) * ny + y) * nx, *r10 = lin + (size_t(z) * ny + y1) * nx;
-    const float *r01 = lin + (size_t(z1) * ny + y) * nx, *r11 = lin + (size_t(z1) * ny + y1) * nx;
-    cells[2 * c] = make_float4(r00[x], r00[x1], r10[x], r10[x1]);
-    cells[2 * c + 1] = make_float4(r01[x], r01[x1], r11[x], r11[x1]);
+__global__ void k_bricks(const float *__restrict__ lin, int nx, int ny, int nz, int nbx, int nby,
+                         int nbz, float *__restrict__ bricks) {
+  // one thread per stored value: coalesced writes
+  const size_t total = size_t(nbx) * nby * nbz * VRT_BRICK_FLOATS;
+  for (size_t o = size_t(blockIdx.x) * blockDim.x + threadIdx.x; o < total;
+       o += size_t(gridDim.x) * blockDim.x) {
+    const size_t b = o / VRT_BRICK_FLOATS;
+    const int l = int(o - b * VRT_BRICK_FLOATS);
+    if (l >= 729) {  // record padding
+      bricks[o] = 0.f;
+      continue;
+    }
+    const int lx = l % 9, ly = (l / 9) % 9, lz = l / 81;
+    const int bx = int(b % nbx), by = int((b / nbx) % nby), bz = int(b / (size_t(nbx) * nby));
+    // the 9th voxel per axis overlaps the next brick; past the last voxel
+    // the edge repeats (clamp addressing)
+    const int x = min(bx * 8 + lx, nx - 1), y = min(by * 8 + ly, ny - 1),
+              z = min(bz * 8 + lz, nz - 1);
+    bricks[o] = lin[(size_t(z) * ny + y) * nx + x];
   }
 }
 
@@ -2248,9 +2295,9 @@ __global__ void __launch_bounds__(256)
   float vmin = FLT_MAX, vmax = -FLT_MAX;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const int x = lo[0] + i % ex, y = lo[1] + (i / ex) % ey, z = lo[2] + i / (ex * ey);
-    // voxel (x, y, z) is the first corner of its own cell's record
-    const size_t cell = (size_t(z) * v.dims[1] + y) * v.dims[0] + x;
-    const float val = v.cells[2 * cell].x;
+    const size_t brick = (size_t(z >> 3) * v.nby + size_t(y >> 3)) * v.nbx + size_t(x >> 3);
+    const float val =
+        v.bricks[brick * VRT_BRICK_FLOATS + size_t(((z & 7) * 9 + (y & 7)) * 9 + (x & 7))];
     vmin = fminf(vmin, val);
     vmax = fmaxf(vmax, val);
   }
@@ -2641,11 +2688,12 @@ int pathtrace_blocks_per_cu(const KScene &sc, size_t lds_bytes, bool nrc, bool g
   return n > 0 ? n : 1;
 }
 
-hipError_t launch_cells(const float *lin, const int dims[3], float *cells, hipStream_t s) {
-  const size_t total = size_t(dims[0]) * dims[1] * dims[2];
+hipError_t launch_bricks(const float *lin, const int dims[3], float *bricks, hipStream_t s) {
+  const int nbx = (dims[0] + 7) / 8, nby = (dims[1] + 7) / 8, nbz = (dims[2] + 7) / 8;
+  const size_t total = size_t(nbx) * nby * nbz * VRT_BRICK_FLOATS;
   const size_t blocks = std::min<size_t>((total + 255) / 256, 65536);
-  hipLaunchKernelGGL(k_cells, dim3(unsigned(blocks)), dim3(256), 0, s, lin, dims[0], dims[1],
-                     dims[2], reinterpret_cast<float4 *>(cells));
+  hipLaunchKernelGGL(k_bricks, dim3(unsigned(blocks)), dim3(256), 0, s, lin, dims[0], dims[1],
+                     dims[2], nbx, nby, nbz, bricks);
   return hipGetLastError();
 }
 

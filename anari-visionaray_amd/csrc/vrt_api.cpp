@@ -50,9 +50,6 @@ namespace {
 
 using namespace vrt;
 
-// structured fields: cells are addressed with 32-bit indices (float4 pairs)
-constexpr size_t kMaxCells = size_t(1) << 31;
-
 // shared by both field types: transfer function, majorants, instance
 int volume_common(const vrt_volume &in, int lds_offset, KVolume &v) {
   if (!in.tf_rgba || in.tf_size < 1)
@@ -154,9 +151,7 @@ int make_volume(const vrt_volume &in, int lds_offset, KVolume &v) {
   for (int a = 0; a < 3; ++a)
     if (in.dims[a] < 1 || in.dims[a] > 4096)
       return fail(VRT_ERROR_INVALID_ARGUMENT, "volume dims[%d]=%d out of range", a, in.dims[a]);
-  if (!in.cells) return fail(VRT_ERROR_INVALID_ARGUMENT, "volume has no cell records");
-  if (size_t(in.dims[0]) * in.dims[1] * in.dims[2] > kMaxCells)
-    return fail(VRT_ERROR_UNSUPPORTED, "volume exceeds 2^31 cells (32-bit cell addressing)");
+  if (!in.bricks) return fail(VRT_ERROR_INVALID_ARGUMENT, "volume has no brick records");
   v = KVolume{};
   v.field_type = in.field_type;
   for (int a = 0; a < 3; ++a) v.dims[a] = in.dims[a];
@@ -167,7 +162,9 @@ int make_volume(const vrt_volume &in, int lds_offset, KVolume &v) {
   v.S = mk(1.f / (v.bhi.x - v.blo.x), 1.f / (v.bhi.y - v.blo.y), 1.f / (v.bhi.z - v.blo.z));
   v.T = -v.blo;
   v.cell_size = in.cell_size;
-  v.cells = reinterpret_cast<const float4 *>(in.cells);
+  v.bricks = in.bricks;
+  v.nbx = uint32_t((in.dims[0] + 7) / 8);
+  v.nby = uint32_t((in.dims[1] + 7) / 8);
   v.glo = mk(0.f);
   v.ghi = mk(1.f);
   return volume_common(in, lds_offset, v);
@@ -313,6 +310,7 @@ int make_params(const vrt_params *p, KParams &k) {
     return fail(VRT_ERROR_INVALID_ARGUMENT, "phase_g %g out of (-1, 1)", double(p->phase_g));
   k.phase_g = p->phase_g;
   k.gradient_shading = p->gradient_shading ? 1 : 0;
+  k.ratio_tracking = p->ratio_tracking ? 1 : 0;
   k.taa = p->taa ? 1 : 0;
   k.taa_alpha = p->taa_alpha;
   for (int i = 0; i < VRT_MAX_CLIP_PLANES; ++i) {
@@ -480,21 +478,19 @@ int vrt_timer_stop(vrt_context ctx, float *ms) {
   return VRT_OK;
 }
 
-size_t vrt_structured_cell_bytes(const int dims[3]) {
+size_t vrt_structured_brick_bytes(const int dims[3]) {
   if (!dims || dims[0] < 1 || dims[1] < 1 || dims[2] < 1) return 0;
-  const size_t n = size_t(dims[0]) * dims[1] * dims[2];
-  return n > kMaxCells ? 0 : n * 8 * sizeof(float);
+  return size_t((dims[0] + 7) / 8) * ((dims[1] + 7) / 8) * ((dims[2] + 7) / 8) *
+         VRT_BRICK_FLOATS * sizeof(float);
 }
 
-int vrt_structured_cells(vrt_context ctx, const float *lin, const int dims[3], float *cells) {
+int vrt_structured_bricks(vrt_context ctx, const float *lin, const int dims[3], float *bricks) {
   if (int r = check_ctx(ctx)) return r;
-  if (!lin || !cells || !dims) return fail(VRT_ERROR_INVALID_ARGUMENT, "null argument");
+  if (!lin || !bricks || !dims) return fail(VRT_ERROR_INVALID_ARGUMENT, "null argument");
   for (int a = 0; a < 3; ++a)
     if (dims[a] < 1 || dims[a] > 4096)
       return fail(VRT_ERROR_INVALID_ARGUMENT, "dims[%d]=%d out of range", a, dims[a]);
-  if (size_t(dims[0]) * dims[1] * dims[2] > kMaxCells)
-    return fail(VRT_ERROR_UNSUPPORTED, "field exceeds 2^31 cells (32-bit cell addressing)");
-  VRT_HIP(launch_cells(lin, dims, cells, ctx->stream));
+  VRT_HIP(launch_bricks(lin, dims, bricks, ctx->stream));
   return VRT_OK;
 }
 

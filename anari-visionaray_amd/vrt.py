@@ -41,7 +41,7 @@ i3 = C.c_int * 3
 class Volume(C.Structure):
     _fields_ = [
         ("field_type", C.c_int), ("dims", i3), ("bounds_lo", f3), ("bounds_hi", f3),
-        ("cell_size", C.c_float), ("cells", C.c_void_p), ("nanovdb", C.c_void_p),
+        ("cell_size", C.c_float), ("bricks", C.c_void_p), ("nanovdb", C.c_void_p),
         ("filter", C.c_int), ("unstructured", C.c_void_p),
         ("tf_rgba", C.c_void_p), ("tf_size", C.c_int), ("value_lo", C.c_float),
         ("value_hi", C.c_float), ("unit_distance", C.c_float), ("majorants", C.c_void_p),
@@ -100,7 +100,7 @@ class Params(C.Structure):
         ("num_clip_planes", C.c_int), ("clip_planes", (C.c_float * 4) * 8),
         ("phase_g", C.c_float), ("gradient_shading", C.c_int),
         ("taa", C.c_int), ("taa_alpha", C.c_float), ("has_prev_camera", C.c_int),
-        ("prev_camera", Camera),
+        ("prev_camera", Camera), ("ratio_tracking", C.c_int),
     ]
 
 
@@ -159,9 +159,9 @@ def lib():
         L.vrt_device_copy.argtypes = [vp, vp, vp, sz]
         L.vrt_timer_start.argtypes = [vp]
         L.vrt_timer_stop.argtypes = [vp, C.POINTER(C.c_float)]
-        L.vrt_structured_cell_bytes.argtypes = [C.POINTER(C.c_int)]
-        L.vrt_structured_cell_bytes.restype = sz
-        L.vrt_structured_cells.argtypes = [vp, vp, C.POINTER(C.c_int), vp]
+        L.vrt_structured_brick_bytes.argtypes = [C.POINTER(C.c_int)]
+        L.vrt_structured_brick_bytes.restype = sz
+        L.vrt_structured_bricks.argtypes = [vp, vp, C.POINTER(C.c_int), vp]
         L.vrt_build_value_ranges.argtypes = [vp, C.POINTER(Volume), vp]
         L.vrt_compute_majorants.argtypes = [vp, C.POINTER(Volume), vp, vp, vp, vp]
         L.vrt_frame_reset.argtypes = [vp, C.POINTER(Frame)]
@@ -520,11 +520,9 @@ class DeviceVolume:
         nz, ny, nx = voxels.shape
         self.dims = (nx, ny, nz)
         lin = ctx.upload(voxels)
-        nbytes = lib().vrt_structured_cell_bytes(_dims(self.dims))
-        if nbytes == 0:
-            raise VrtError(f"structured field {self.dims} exceeds 2^31 cells")
-        self.cells = ctx.buffer(nbytes)
-        check(lib().vrt_structured_cells(ctx.handle, lin.ptr, _dims(self.dims), self.cells.ptr))
+        nbytes = lib().vrt_structured_brick_bytes(_dims(self.dims))
+        self.bricks = ctx.buffer(nbytes)
+        check(lib().vrt_structured_bricks(ctx.handle, lin.ptr, _dims(self.dims), self.bricks.ptr))
         ctx.sync()
         lin.free()
         v = Volume()
@@ -535,7 +533,7 @@ class DeviceVolume:
         v.bounds_lo = f3(*lo)
         v.bounds_hi = f3(*hi)
         v.cell_size = float(np.min(np.array(spacing, np.float32)))
-        v.cells = self.cells.ptr
+        v.bricks = self.bricks.ptr
         self._finish(ctx, v, tf_rgba, value_range, unit_distance, grid_dims, xfm)
 
     def _finish(self, ctx, v, tf_rgba, value_range, unit_distance, grid_dims, xfm):
@@ -717,12 +715,16 @@ def camera(position, direction, up, fovy, aspect, region=(0, 0, 1, 1), shutter=(
 def params(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.2,
            occlusion_distance=1e20, ambient_samples=1, max_bounce=7, pixel_samples=1,
            volume_sampling_rate=0.5, rng_offset=0, clip_planes=(), phase_g=0.0,
-           gradient_shading=False, taa=False, taa_alpha=0.3, prev_camera=None):
+           gradient_shading=False, taa=False, taa_alpha=0.3, prev_camera=None,
+           ratio_tracking=False):
     """Renderer parameters with the ANARI defaults of the reference
     (Renderer.cpp:42-53, Pathtrace.cpp:19-25, Raycast.cpp:19).  `clip_planes`:
     up to 8 (nx, ny, nz, d); rays keep n.x <= d.  `phase_g`: Henyey-Greenstein
-    anisotropy of the volume phase (0: the reference's isotropic phase)."""
+    anisotropy of the volume phase (0: the reference's isotropic phase).
+    `ratio_tracking`: shadow and AO transmittance by ratio tracking (the
+    reference: binary delta tracking)."""
     p = Params()
+    p.ratio_tracking = int(bool(ratio_tracking))
     p.rng_offset = rng_offset
     p.phase_g = phase_g
     p.gradient_shading = int(bool(gradient_shading))
@@ -973,6 +975,7 @@ def pathtrace_nrc(ctx, scene, cam, prm, frame, nrc, opts, accum_id, frame_id=0):
 
 
 def mfma_probe(ctx, a, b, precision=NRC_FP16):
+    """C (16x16) = A (16x32) B (32x16) through one 16x16x32 MFMA."""
     da = ctx.upload(np.ascontiguousarray(a, np.float32))
     db = ctx.upload(np.ascontiguousarray(b, np.float32))
     dc = ctx.buffer(16 * 16 * 4)

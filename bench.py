@@ -72,28 +72,40 @@ PRECISION_NOTE = {"fp16": "fp16 weights/activations, fp32 accumulation, fp32 Ada
                   "bf16": "bf16 weights/activations, fp32 accumulation, fp32 Adam"}
 
 
-def nrc_vs_plain(pkg, ctx, wl, volume=None, ref_frames=2048, cap=1024, warmup=4, train_steps=8,
-                 target=1e-2, oracle_stride=24, oracle_spp=4096, timed_frames=32,
-                 precisions=("fp16",), query_vertex=1, train_samples=32768):
-    """Wall time to relMSE <= target of the plain path tracer and of the
-    radiance-cache path tracer (its training warm-up included) for each cache
-    precision, all measured against an FP64-accumulated plain reference of
-    ref_frames spp; frame times on HIP events; the reference's own relMSE
-    against the CPU FP64 oracle is measured on every oracle_stride-th pixel."""
-    vrt, wls = pkg.vrt, pkg.workloads
-    ref = wls.GpuScene(ctx, wl, accum64=True, volume=volume)
-    volume = ref.volume
-    for a in range(4):
-        ref.render(a)
-    ctx.sync()
-    ctx.timer_start()
-    for a in range(timed_frames):
-        ref.render(4 + a)
-    plain_ms = ctx.timer_stop() / timed_frames
+def reference_scene(pkg, ctx, wl, volume=None, ref_frames=2048):
+    """The plain GPU path tracer accumulated over ref_frames spp in FP64: the
+    reference every time-to-relMSE figure is measured against."""
+    ref = pkg.workloads.GpuScene(ctx, wl, accum64=True, volume=volume)
     ref.frame.reset()
     for a in range(ref_frames):
         ref.render(a)
     ctx.sync()
+    return ref
+
+
+def nrc_vs_plain(pkg, ctx, wl, volume=None, ref_frames=2048, cap=1024, warmup=4, train_steps=8,
+                 target=1e-2, oracle_stride=24, oracle_spp=4096, timed_frames=32,
+                 precisions=("fp16",), query_vertex=2, train_samples=65536, ref=None):
+    """Wall time to relMSE <= target of the plain path tracer and of the
+    radiance-cache path tracer (its training warm-up included) for each cache
+    precision, all measured against an FP64-accumulated plain reference of
+    ref_frames spp (`ref`, or made here); frame times on HIP events, the
+    plain frames on a frame without the FP64 buffer, as the cache frames are;
+    the reference's own relMSE against the CPU FP64 oracle is measured on
+    every oracle_stride-th pixel."""
+    vrt, wls = pkg.vrt, pkg.workloads
+    if ref is None:
+        ref = reference_scene(pkg, ctx, wl, volume, ref_frames)
+    volume = ref.volume
+    timing = wls.GpuScene(ctx, wl, volume=volume)
+    timing.frame.reset()
+    for a in range(4):
+        timing.render(a)
+    ctx.sync()
+    ctx.timer_start()
+    for a in range(timed_frames):
+        timing.render(4 + a)
+    plain_ms = ctx.timer_stop() / timed_frames
 
     # plain: pixel streams independent of the reference's (rng_offset)
     gs = wls.GpuScene(ctx, wl, volume=volume, params={"rng_offset": 1 << 20})
@@ -150,7 +162,8 @@ def nrc_vs_plain(pkg, ctx, wl, volume=None, ref_frames=2048, cap=1024, warmup=4,
         _, _, t_nrc, _ = nrc_run(code, frames_nrc)
         nrc_ms = t_nrc / (warmup + frames_nrc)
         key = "nrc" if i == 0 else f"nrc_{prec}"
-        out[key] = {"query_vertex": query_vertex, "warmup_frames": warmup, "frames": frames_nrc,
+        out[key] = {"query_vertex": query_vertex, "train_samples_per_frame": train_samples,
+                    "warmup_frames": warmup, "frames": frames_nrc,
                     "ms_per_frame": round(nrc_ms, 4),
                     "ms_to_target": round(t_nrc, 2) if nrc_reached else None, "relmse": e_nrc,
                     "queries_last_frame": counts[0],
@@ -318,6 +331,29 @@ def kernel_only(pkg, ctx, name, steps, warmup, precision="fp16", query_vertex=2,
     print(json.dumps(out), flush=True)
 
 
+# what bench.py accepts of a committed traffic profile: it must have been
+# taken on a kernel of about the same time and lookups per launch
+TRAFFIC_TOLERANCE = 0.10
+
+
+def load_traffic(ms_per_launch, lookups_per_launch):
+    """HBM bytes per launch from the committed rocprofv3 summary of this
+    workload, or (None, reason) when it is missing or was taken on a kernel
+    whose time or lookup count differs from this run's by more than 10 %."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None, "no rocprofv3 traffic summary for this workload"
+    with open(TRAFFIC_FILE) as f:
+        t = json.load(f)
+    pms, plk = t.get("ms_per_launch"), t.get("lookups_per_launch")
+    if not pms or not plk:
+        return None, "the traffic summary records no kernel time / lookups to check it against"
+    if (abs(pms - ms_per_launch) > TRAFFIC_TOLERANCE * ms_per_launch or
+            abs(plk - lookups_per_launch) > TRAFFIC_TOLERANCE * lookups_per_launch):
+        return None, (f"stale: profiled at {pms:.4f} ms and {plk:.4g} lookups per launch, "
+                      f"this run {ms_per_launch:.4f} ms and {lookups_per_launch:.4g}")
+    return t.get("hbm_bytes_per_launch"), f"rocprofv3 FETCH_SIZE + WRITE_SIZE, {t.get('note', '')}"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -328,17 +364,17 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-nrc", action="store_true",
-                    help="skip the time-to-relMSE comparison with the radiance cache")
+                    help="skip the time-to-relMSE comparisons with the radiance cache")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--nrc-query-vertex", type=int, default=1, choices=(1, 2),
-                    help="volume vertex whose cache query ends a path")
+    ap.add_argument("--nrc-query-vertex", type=int, default=2, choices=(1, 2),
+                    help="volume vertex whose cache query ends a path (2: the configuration)")
     ap.add_argument("--ttr-size", type=int, default=1024,
                     help="index-space edge of the nanovdb cloud of the time-to-relMSE run")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the frame time of the mixed triangle + volume scene")
-    ap.add_argument("--c4", action="store_true",
-                    help="also run time-to-relMSE on the 2M-tetrahedron 4K scene, plain vs "
-                         "radiance cache in fp16 and bf16")
+    ap.add_argument("--no-c4", action="store_true",
+                    help="skip time-to-relMSE on the 2M-tetrahedron 4K scene (plain vs radiance "
+                         "cache in fp16 and bf16)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (counter calibration)")
     ap.add_argument("--kernel", choices=KERNEL_WORKLOADS, default=None,
@@ -386,35 +422,43 @@ def main():
     ms = ctx.timer_stop()
     ctx.sync()
     wall = time.perf_counter() - t_wall
-    lookups = int(gs.frame.counter_values()[0])
+    cv = gs.frame.counter_values()
+    lookups, hit_samples = int(cv[0]), int(cv[1])
 
     samples = wl.width * wl.height * args.steps
     value = samples / (ms / 1e3) / 1e6
     ms_per_step = ms / args.steps
-    bytes_per_launch = lookups / args.steps * BYTES_PER_LOOKUP
+    lookups_per_launch = lookups / args.steps
+    bytes_per_launch = lookups_per_launch * BYTES_PER_LOOKUP
     achieved = bytes_per_launch / (ms_per_step / 1e3) / 1e9
-    traffic = None
-    if os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    traffic, traffic_note = load_traffic(ms_per_step, lookups_per_launch)
     print(f"# {args.steps} frames: {ms:.1f} ms on HIP events, {wall:.2f} s wall; "
-          f"{lookups / args.steps:.3e} lookups/frame", file=sys.stderr, flush=True)
+          f"{lookups_per_launch:.3e} lookups/frame, {hit_samples / args.steps:.4g} "
+          f"volume-hitting samples/frame", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(wl, args.cpu_seconds)
-    nrc = None
+    nrc = nrc_v1 = None
     if not args.no_nrc:
         # radiance cache vs plain on the nanovdb cloud: 1024^3 index space,
-        # ~5% active, 8^3 leaves
+        # ~5% active, 8^3 leaves; paths end in the cache at their second
+        # volume vertex, 65,536 training samples per frame
         t0 = time.perf_counter()
         wl3 = wls.cloud_nanovdb_pathtrace(n=args.ttr_size, width=args.width, height=args.height,
                                           phase_g=PHASE_G)
         print(f"# generated {wl3.name} in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
               flush=True)
-        nrc = nrc_vs_plain(pkg, ctx, wl3, oracle_stride=0 if args.no_cpu else 24,
-                           query_vertex=args.nrc_query_vertex)
+        ref3 = reference_scene(pkg, ctx, wl3)
+        nrc = nrc_vs_plain(pkg, ctx, wl3, ref=ref3, oracle_stride=0 if args.no_cpu else 24,
+                           query_vertex=args.nrc_query_vertex, train_samples=65536)
         print(f"# time to relMSE 1e-2: {json.dumps(nrc)}", file=sys.stderr, flush=True)
+        # the same with queries at the first volume vertex (32,768 samples)
+        nrc_v1 = nrc_vs_plain(pkg, ctx, wl3, ref=ref3, oracle_stride=0, query_vertex=1,
+                              train_samples=32768)
+        print(f"# time to relMSE 1e-2, first-vertex queries: {json.dumps(nrc_v1)}",
+              file=sys.stderr, flush=True)
+        del ref3
 
     other = {"raycast_blobs": raycast_frame_time(pkg, ctx, cpu=not args.no_cpu)}
     print(f"# raycast: {json.dumps(other['raycast_blobs'])}", file=sys.stderr, flush=True)
@@ -432,14 +476,17 @@ def main():
                                                                args.height, builder="lbvh")
         print(f"# mixed scene, GPU-built BVH: "
               f"{json.dumps(other['mixed_surfaces_volume_lbvh'])}", file=sys.stderr, flush=True)
-    if args.c4:
+    if not args.no_nrc and not args.no_c4:
+        # unstructured-plus-cache: 2,058,000 tetrahedra, 3840x2160, plain vs
+        # the cache in fp16 and bf16, queries at the second volume vertex
         t0 = time.perf_counter()
         wl4 = wls.tets_pathtrace()
         gs4 = wls.GpuScene(ctx, wl4)
         inf = gs4.volume.field.info
         setup = time.perf_counter() - t0
-        c4 = nrc_vs_plain(pkg, ctx, wl4, volume=gs4.volume, oracle_stride=0,
-                          precisions=("fp16", "bf16"), query_vertex=args.nrc_query_vertex)
+        c4 = nrc_vs_plain(pkg, ctx, wl4, volume=gs4.volume, oracle_stride=0, ref_frames=1024,
+                          precisions=("fp16", "bf16"), query_vertex=args.nrc_query_vertex,
+                          train_samples=65536)
         c4["field"] = {"tetrahedra": int(inf.num_cells), "grid_dims": list(inf.grid_dims),
                        "mean_list": round(inf.mean_list, 2),
                        "expected_tests": round(inf.expected_tests, 3),
@@ -464,12 +511,15 @@ def main():
         "config": {
             "workload": f"pathtrace fBm {args.size}^3 f32, density x50, 32^3 majorants, "
                         f"albedo 0.8, Henyey-Greenstein g={PHASE_G}, directional + ambient, "
-                        f"<=8 bounces, {args.width}x{args.height}, 1 spp/frame",
+                        f"<=8 bounces, {args.width}x{args.height}, 1 spp/frame; camera at "
+                        f"(1.5, 1, 2) looking at the volume centre, fovy 40 deg: "
+                        f"{hit_samples / samples:.1%} of the samples hit the volume",
             "renderer": "pathtrace",
             "image": [args.width, args.height],
             "volume": [args.size] * 3,
             "spp_per_step": 1,
         },
+        "volume_samples_per_s_M": round(hit_samples / (ms / 1e3) / 1e6, 3),
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 2),
@@ -477,12 +527,18 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "traffic_note": traffic_note,
             "kernel": "k_pathtrace",
-            "lookups_per_launch": lookups / args.steps,
+            "lookups_per_launch": lookups_per_launch,
             "bytes_per_lookup": BYTES_PER_LOOKUP,
+            "lookups_per_volume_sample": round(lookups / max(1, hit_samples), 3),
+            "limiter": "the HBM roofline is the one the volume fetch is measured against; the "
+                       "kernel runs far below it, bound by dependent fetch latency and VALU "
+                       "issue of divergent paths (see traffic: HBM bytes actually moved)",
         },
         "cpu_baseline": cpu,
         "time_to_relmse": nrc,
+        "time_to_relmse_vertex1": nrc_v1,
         "other_configs": other or None,
     }
     ctx.close()

@@ -58,11 +58,11 @@ typedef struct vrt_volume {
   float bounds_lo[3];  /* object-space bounds: origin ..            */
   float bounds_hi[3];  /*   origin + (dims-1)*spacing               */
   float cell_size;     /* min(spacing) (StructuredRegularField.cpp:40) */
-  /* structuredRegular: the cell records of vrt_structured_cells */
-  const float *cells;
+  /* structuredRegular: the brick records of vrt_structured_bricks */
+  const float *bricks;
   /* nanovdb: the converted grid (vrt_nanovdb_create, replacing the device
      copy of the grid buffer, NanoVDBField.cpp:29-64); bounds and cell size
-     then come from the grid and dims / cells are unused */
+     then come from the grid and dims / bricks are unused */
   struct vrt_nanovdb_s *nanovdb;
   int filter;          /* VRT_FILTER_*: nanovdb "filter" (NanoVDBField.cpp:27) */
   /* unstructured: the tetrahedral field (vrt_unstructured_create); bounds
@@ -200,6 +200,14 @@ typedef struct vrt_params {
   float taa_alpha;
   int has_prev_camera;
   vrt_camera prev_camera;
+  /* extension (default 0): 1 = shadow and ambient-occlusion rays estimate the
+     volumes' transmittance by ratio tracking -- the same free-flight steps
+     and majorant cells as delta tracking, each tentative collision
+     multiplying the estimate by 1 - sigma/majorant -- instead of the
+     reference's binary visibility (DeviceCopyableObjects.h:722-757,
+     Pathtrace_impl.cpp:441-444, 461-466).  Same expectation, lower
+     variance; the pixels' random streams then differ from the reference's. */
+  int ratio_tracking;
 } vrt_params;
 
 /* ANARI data types used for frame channels (ANARI 1.0 enum values). */
@@ -217,8 +225,10 @@ typedef struct vrt_frame {
   float *depth;       /* optional float depth channel (may be NULL) */
   double *accum64;    /* optional float4-as-double running sum (FP64
                          long-run reference accumulation; may be NULL) */
-  unsigned long long *counters; /* optional [0] trilinear lookups,
-                                   [1] paths, [2] collisions */
+  unsigned long long *counters; /* optional: [0] field lookups (trilinear
+                                   taps of one sample, or one point
+                                   location), [1] path-tracer samples whose
+                                   camera ray hit the scene, [2] unused */
   /* optional g-buffer channels (Frame::fillGBuffer,
      DeviceCopyableObjects.h:2745-2774): the first radiance hit of the
      pixel's closest sample -- geometric normal (float3), albedo (float3),
@@ -271,23 +281,23 @@ VRT_API int vrt_timer_stop(vrt_context ctx, float *ms);
 /* Spatial field + majorant grid                                           */
 /* ---------------------------------------------------------------------- */
 
-/* The renderers' layout of a structuredRegular field: one 32-byte record
- * per voxel cell (i, j, k), i < dims[0], j < dims[1], k < dims[2], in
- * x-fastest cell order, holding the 8 voxels a trilinear sample in the cell
- * reads: v(i+dx, j+dy, k+dz) for dz, dy, dx in {0, 1} (dx fastest), indices
- * clamped to the last voxel (clamp addressing).  A lookup is then one
- * aligned 32-byte read (two 16-byte loads) at one address, instead of 8
- * scattered taps over up to 4 cache lines; the price is 8x the voxel
- * memory (a 512^3 fp32 field takes 4 GiB), which HBM has room for.
- * Bytes needed for a dims[0]*dims[1]*dims[2] field; 0 for bad dims or a
- * field of more than 2^31 cells. */
-VRT_API size_t vrt_structured_cell_bytes(const int dims[3]);
+/* The renderers' layout of a structuredRegular field: bricks of 8^3
+ * voxel cells, brick (bx, by, bz) x-fastest, each a record of
+ * VRT_BRICK_FLOATS fp32 values holding the 9^3 voxels 8bx .. 8bx+8 (and so
+ * on per axis; x fastest, then y, then z; indices past the last voxel
+ * repeat it, which is clamp addressing) and 7 pad values (records are
+ * 128-byte multiples).  Every trilinear footprint then lies in one record:
+ * a lookup forms one address and reads its 8 taps at fixed offsets (0, 1,
+ * 9, 10, 81, 82, 90, 91 values), for 1.42x the voxel memory.
+ * Bytes needed for a dims[0]*dims[1]*dims[2] field (0 for bad dims). */
+#define VRT_BRICK_FLOATS 736
+VRT_API size_t vrt_structured_brick_bytes(const int dims[3]);
 
-/* x-fastest linear fp32 voxels (device) -> cell records (device).
+/* x-fastest linear fp32 voxels (device) -> brick records (device).
  * Replaces the hipArray texture upload of StructuredRegularField::finalize
  * (StructuredRegularField.cpp:42-76). */
-VRT_API int vrt_structured_cells(vrt_context ctx, const float *linear, const int dims[3],
-                                 float *cells);
+VRT_API int vrt_structured_bricks(vrt_context ctx, const float *linear, const int dims[3],
+                                  float *bricks);
 
 /* NanoVDB float grid (NanoVDBField.cpp:25-65).  The host buffer of one
  * float grid in the NanoVDB 32.x format (the "data" array of a "nanovdb"
@@ -443,7 +453,7 @@ VRT_API int vrt_compute_majorants(vrt_context ctx, const vrt_volume *vol,
 
 /* Field values at n voxel-space points (device float3 array -> device
  * float array): sampleField (DeviceCopyableObjects.h:474-570) -- trilinear
- * on the cell records, the NanoVDB sampler, or the containing tetrahedron's
+ * on the brick records, the NanoVDB sampler, or the containing tetrahedron's
  * value; a point outside every tetrahedron (sampleField returns false) gives
  * NaN. */
 VRT_API int vrt_sample_field(vrt_context ctx, const vrt_volume *vol, int n,
@@ -575,8 +585,9 @@ VRT_API int vrt_pathtrace_nrc(vrt_context ctx, const vrt_scene *scene,
  * [2] paths reaching vertex 2, [3] the previous frame's [2] (synchronises) */
 VRT_API int vrt_nrc_frame_counts(vrt_nrc nrc, uint32_t counts[4]);
 
-/* matrix-core lane-layout check: c (16x16) = a (16x16) b (16x16), device
- * row-major fp32 in/out, operands rounded to fp16 or bf16 */
+/* matrix-core lane-layout check of the 16x16x32 instruction the cache
+ * kernels use: c (16x16) = a (16x32) b (32x16), device row-major fp32
+ * in/out, operands rounded to fp16 or bf16 */
 VRT_API int vrt_mfma_probe(vrt_context ctx, const float *a, const float *b, float *c,
                            int precision);
 

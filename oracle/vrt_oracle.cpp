@@ -667,7 +667,7 @@ void dda3(Ray<R> ray, const int dims[3], V3<R> lo, V3<R> hi, F &&func) {
 
 // ---------------------------------------------------------------------------
 // A7: intersect(Ray, Volume) -- box clip, voxel space, DDA + Woodcock
-// (DeviceCopyableObjects.h:680-796)
+// (DeviceCopyableObjects.h:680-796); with `ratio`, ratio tracking
 // ---------------------------------------------------------------------------
 template <class R> struct VolHit {
   bool hit = false;
@@ -679,8 +679,17 @@ template <class R> struct VolHit {
 
 struct Counter { uint64_t lookups = 0; };
 
+// Delta tracking (ratio = false): the first accepted collision, Tr = 0 at
+// it.  Ratio tracking (ratio = true, an extension for shadow and AO rays:
+// Novak et al. 2014, "Residual ratio tracking"): the same free-flight steps
+// over the same majorant cells, but every tentative collision multiplies the
+// transmittance estimate Tr by 1 - sigma/majorant instead of drawing an
+// acceptance, and the walk runs to the end of the segment; E[Tr] is the
+// transmittance, as the reference's binary Tr estimates it
+// (DeviceCopyableObjects.h:722-757, Pathtrace_impl.cpp:441-444).
 template <class R>
-VolHit<R> delta_track(const Volume<R> &vol, Ray<R> ray, Random &rnd, Counter *cnt) {
+VolHit<R> delta_track(const Volume<R> &vol, Ray<R> ray, Random &rnd, Counter *cnt,
+                      bool ratio = false) {
   VolHit<R> out;
   BoxHit<R> bh = intersect_box(ray, vol.blo, vol.bhi);
   if (!bh.hit) return out;
@@ -700,6 +709,7 @@ VolHit<R> delta_track(const Volume<R> &vol, Ray<R> ray, Random &rnd, Counter *cn
   R hit_t = ray.tmax;
   V3<R> albedo;
   int64_t hit_prim = 0;
+  R tr = R(1);
 
   auto woodcock = [&](int leafID, R t0, R t1) -> bool {
     const R majorant = leafID >= 0 ? vol.majorant(leafID) : R(1);
@@ -715,6 +725,10 @@ VolHit<R> delta_track(const Volume<R> &vol, Ray<R> ray, Random &rnd, Counter *cn
       // the acceptance draw only where the field has a value (:739-753)
       if (vol.sample(P, v, &prim)) {
         V4<R> s = vol.classify(v);
+        if (ratio) {
+          tr = tr * (R(1) - s.w / majorant);
+          continue;
+        }
         R u = R(rnd());
         if (s.w >= u * majorant) {
           albedo = V3<R>(s.x, s.y, s.z);
@@ -743,6 +757,8 @@ VolHit<R> delta_track(const Volume<R> &vol, Ray<R> ray, Random &rnd, Counter *cn
     out.albedo = albedo;
     out.Tr = R(0);
     out.prim = hit_prim;
+  } else {
+    out.Tr = tr;
   }
   return out;
 }
@@ -1275,7 +1291,7 @@ template <class R> struct HitRec {
 
 template <class R>
 HitRec<R> intersect_all(const Scene<R> &sc, const Ray<R> &ray, Random &rnd, Counter *cnt,
-                        unsigned bounceID = 0, bool shadow = false) {
+                        unsigned bounceID = 0, bool shadow = false, bool ratio = false) {
   HitRec<R> hr;
   // intersectSurfaces<1>: closest hit, any hit for shadow / AO rays
   for (size_t m = 0; m < sc.meshes.size(); ++m) {
@@ -1316,11 +1332,14 @@ HitRec<R> intersect_all(const Scene<R> &sc, const Ray<R> &ray, Random &rnd, Coun
       hr.light_id = i;
     }
   }
-  // sampleFreeFlightDistanceAllVolumes -> intersectVolumes, closest wins
+  // sampleFreeFlightDistanceAllVolumes -> intersectVolumes, closest wins;
+  // ratio tracking (shadow / AO rays) multiplies the volumes' estimates
   for (size_t i = 0; i < sc.vols.size(); ++i) {
     Ray<R> lr = sc.vols[i].to_local(ray);
-    VolHit<R> vh = delta_track(sc.vols[i], lr, rnd, cnt);
-    if (vh.hit && vh.t < hr.vol.t) {
+    VolHit<R> vh = delta_track(sc.vols[i], lr, rnd, cnt, ratio);
+    if (ratio) {
+      hr.vol.Tr = hr.vol.Tr * vh.Tr;
+    } else if (vh.hit && vh.t < hr.vol.t) {
       hr.vol = vh;
       hr.vol_inst = int(i);
     }
@@ -1640,7 +1659,9 @@ void shade(Random &srnd, const Ray<R> &ray, RayType rayType, const Scene<R> &sc,
     st.aoSamples++;
     R weight = std::max(R(0), dot(ray.dir, st.sn));
     st.aoWeights += weight;
-    if (weight > R(0) && hitRec.hit) st.aoCount += weight;
+    // occluded by any hit; with ratio tracking a volume occludes by 1 - Tr
+    // (delta tracking: Tr is 0 at a volume collision, else 1)
+    if (weight > R(0)) st.aoCount += weight * (hitRec.hit ? R(1) : R(1) - hitRec.vol.Tr);
     if (st.aoSamples < P->ambient_samples) { prepAO(); return; }
     R aoV = R(0);
     if (st.aoWeights > R(0)) aoV = R(1) - st.aoCount / st.aoWeights;
@@ -1687,7 +1708,8 @@ V4<R> pathtrace_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params
       for (unsigned passID = 0, bounceID = 0;; ++passID) {
         clip_ray(ray, P);  // Pathtrace_impl.cpp:559
         const bool shadow = rayType == Shadow || rayType == AO;
-        HitRec<R> hitRec = intersect_all(sc, ray, srnd, cnt, bounceID, shadow);
+        HitRec<R> hitRec = intersect_all(sc, ray, srnd, cnt, bounceID, shadow,
+                                         shadow && P->ratio_tracking != 0);
         shade(srnd, ray, rayType, sc, P, hitRec, st, bounceID, closest ? &ps : nullptr);
         if (passID == 0 && bounceID == 0) {
           firstHit = hitRec.hit;
@@ -2278,6 +2300,24 @@ int orc_delta_track(const orc_volume *v, const float ori[3], const float dir[3],
   }
   *state = r.state;
   return hit;
+}
+
+float orc_ratio_track(const orc_volume *v, const float ori[3], const float dir[3], float tmin,
+                      float tmax, uint32_t *state, int fp64) {
+  Random r;
+  r.state = *state;
+  float tr;
+  if (fp64) {
+    Volume<double> vol(v);
+    Ray<double> ray{from<double>(ori), from<double>(dir), tmin, tmax};
+    tr = float(delta_track(vol, vol.to_local(ray), r, nullptr, true).Tr);
+  } else {
+    Volume<float> vol(v);
+    Ray<float> ray{from<float>(ori), from<float>(dir), tmin, tmax};
+    tr = delta_track(vol, vol.to_local(ray), r, nullptr, true).Tr;
+  }
+  *state = r.state;
+  return tr;
 }
 
 void orc_camera_ray(const orc_camera *c, float x, float y, int W, int H, float *ori, float *dir) {

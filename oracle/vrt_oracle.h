@@ -133,6 +133,9 @@ typedef struct orc_params {
   /* gradientShading (Renderer.cpp:50): raycast matte shading along the
      central-difference gradient; path-tracer volume normal and matte NEE */
   int gradient_shading;
+  /* extension: 1 = shadow and AO rays estimate the volumes' transmittance
+     by ratio tracking instead of the reference's binary delta tracking */
+  int ratio_tracking;
 } orc_params;
 
 /* --- A1: LCG<4> / Random (common.h:55-102) --- */
@@ -177,6 +180,10 @@ int orc_dda_cells(const float ori[3], const float dir[3], float tmin,
 int orc_delta_track(const orc_volume *vol, const float ori[3],
                     const float dir[3], float tmin, float tmax,
                     uint32_t *rng_state, float *t_out, int fp64);
+/* ratio tracking of one ray (an extension for shadow / AO rays): the
+   transmittance estimate Tr in [0, 1] */
+float orc_ratio_track(const orc_volume *v, const float ori[3], const float dir[3], float tmin,
+                      float tmax, uint32_t *state, int fp64);
 
 /* --- A14/A15: raycast renderer (Raycast_impl.cpp:245-322) --- */
 void orc_raycast(const orc_scene *scene, const orc_camera *cam,

@@ -59,7 +59,8 @@ class OrcParams(C.Structure):
                 ("max_bounce", C.c_int), ("pixel_samples", C.c_int),
                 ("volume_sampling_rate", C.c_float), ("rng_offset", C.c_uint32),
                 ("num_clip_planes", C.c_int), ("clip_planes", (C.c_float * 4) * 8),
-                ("phase_g", C.c_float), ("gradient_shading", C.c_int)]
+                ("phase_g", C.c_float), ("gradient_shading", C.c_int),
+                ("ratio_tracking", C.c_int)]
 
 
 _lib = None
@@ -83,6 +84,9 @@ def lib():
         L.orc_dda_cells.argtypes = [vp, vp, C.c_float, C.c_float, vp, vp, vp, ip, vp, vp, vp]
         L.orc_delta_track.argtypes = [C.POINTER(OrcVolume), vp, vp, C.c_float, C.c_float,
                                       C.POINTER(C.c_uint32), C.POINTER(C.c_float), ip]
+        L.orc_ratio_track.restype = C.c_float
+        L.orc_ratio_track.argtypes = [C.POINTER(OrcVolume), vp, vp, C.c_float, C.c_float,
+                                      C.POINTER(C.c_uint32), ip]
         L.orc_raycast.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcCamera),
                                   C.POINTER(OrcParams), ip, ip, vp, ip]
         L.orc_pathtrace_frame.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcCamera),
@@ -412,10 +416,11 @@ def camera(position, direction, up, fovy, aspect, region=(0, 0, 1, 1), shutter=(
 def params(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.2,
            occlusion_distance=1e20, ambient_samples=1, max_bounce=7, pixel_samples=1,
            volume_sampling_rate=0.5, rng_offset=0, clip_planes=(), phase_g=0.0,
-           gradient_shading=False, taa=False, taa_alpha=0.3):
+           gradient_shading=False, taa=False, taa_alpha=0.3, ratio_tracking=False):
     # taa / taa_alpha: the oracle's TAA frames take them as arguments
     # (OracleWorkload.pathtrace_taa)
     p = OrcParams()
+    p.ratio_tracking = int(bool(ratio_tracking))
     p.phase_g = phase_g
     p.gradient_shading = int(bool(gradient_shading))
     p.num_clip_planes = len(clip_planes)

@@ -28,16 +28,15 @@ def test_vrt_exports_every_declared_symbol(pkg):
     assert not missing, missing
 
 
-def test_cell_record_bytes(pkg):
-    """One 32-byte record (8 fp32 corner voxels) per voxel cell; none for an
-    empty or over-large field."""
+def test_brick_record_bytes(pkg):
+    """One record of 9^3 voxels padded to 736 fp32 (2944 B, 23 x 128 B) per
+    brick of 8^3 voxel cells, each axis rounded up to whole bricks; none for
+    an empty field."""
     L = pkg.vrt.lib()
     dims = (C.c_int * 3)(9, 8, 17)
-    assert L.vrt_structured_cell_bytes(dims) == 9 * 8 * 17 * 32
+    assert L.vrt_structured_brick_bytes(dims) == 2 * 1 * 3 * 736 * 4
     dims = (C.c_int * 3)(0, 8, 8)
-    assert L.vrt_structured_cell_bytes(dims) == 0
-    dims = (C.c_int * 3)(2048, 1024, 1025)  # 2^31 + 2^21 cells
-    assert L.vrt_structured_cell_bytes(dims) == 0
+    assert L.vrt_structured_brick_bytes(dims) == 0
 
 
 def test_null_context_rejected_without_gpu(pkg):

@@ -147,7 +147,8 @@ def _anari_scene(a, wl, xfm=None, bvh_builder=None):
                        ("pixel_samples", "pixelSamples", A.INT32),
                        ("volume_sampling_rate", "volumeSamplingRate", A.FLOAT32),
                        ("phase_g", "phaseG", A.FLOAT32),
-                       ("gradient_shading", "gradientShading", A.BOOL)):
+                       ("gradient_shading", "gradientShading", A.BOOL),
+                       ("ratio_tracking", "ratioTracking", A.BOOL)):
         if k in p:
             a.set(ren, n, kind, p[k])
     fr = a.new(A.FRAME)
@@ -199,6 +200,9 @@ CASES = {
     "nanovdb pathtrace": lambda w: w.cloud_nanovdb_pathtrace(n=64, width=64, height=36,
                                                             frames=1, brick=8),
     "tets pathtrace": lambda w: w.tets_pathtrace(n=8, width=48, height=32, frames=1, grid=8),
+    "ratio-tracked pathtrace": lambda w: dataclasses.replace(
+        w.fbm_pathtrace(n=32, width=64, height=48, frames=1, brick=8),
+        params=dict(max_bounce=4, ambient_samples=1, ratio_tracking=True)),
     "mixed surfaces": lambda w: w.mixed_pathtrace(level=3, vol_n=24, width=48, height=32,
                                                   frames=1),
 }

@@ -18,9 +18,12 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("precision", [0, 1])
 def test_mfma_lane_layout(pkg, gpu_ctx, precision):
+    """The operand layout the cache kernels assume for v_mfma_f32_16x16x32:
+    small integers (exact in fp16 and bf16), so C must equal A B exactly; a
+    K-slot mix-up between lane groups would scramble the products."""
     rng = np.random.default_rng(precision)
-    a = rng.integers(-4, 5, (16, 16)).astype(np.float32)
-    b = rng.integers(-4, 5, (16, 16)).astype(np.float32)
+    a = rng.integers(-4, 5, (16, 32)).astype(np.float32)
+    b = rng.integers(-4, 5, (32, 16)).astype(np.float32)
     c = pkg.vrt.mfma_probe(gpu_ctx, a, b, precision)
     np.testing.assert_array_equal(c, a @ b)
 

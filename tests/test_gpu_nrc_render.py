@@ -10,6 +10,10 @@
   learning gain      after online training the cached estimator is far
                         closer to the converged image than the truncated
                         estimator without the cache
+
+The zero-cache, constant-cache and learning-gain checks run on a structured
+fBm field with the fp16 cache and on a tetrahedral field (the unstructured-
+plus-cache configuration) with the bf16 cache.
 """
 import numpy as np
 import pytest
@@ -21,9 +25,21 @@ pytestmark = pytest.mark.gpu
 NW = 8192 + 4 * 4096 + 1024
 
 
-def _wl(pkg, n=32, w=96, h=64, max_bounce=8):
+def _wl(pkg, n=32, w=96, h=64, max_bounce=8, field="fbm"):
+    if field == "tets":
+        # jittered lattice of 6 n^3 tetrahedra, the 2M-tet scene's generator
+        return pkg.workloads.tets_pathtrace(n=max(4, n // 3), width=w, height=h, frames=1,
+                                            max_bounce=max_bounce, grid=16)
     return pkg.workloads.fbm_pathtrace(n=n, width=w, height=h, frames=1, brick=8,
                                        max_bounce=max_bounce)
+
+
+FIELDS = [("fbm", "fp16"), ("tets", "bf16")]
+
+
+def _nrc(pkg, ctx, precision, **kw):
+    vrt = pkg.vrt
+    return vrt.Nrc(ctx, precision={"fp16": vrt.NRC_FP16, "bf16": vrt.NRC_BF16}[precision], **kw)
 
 
 def _frame(gs, render, accum_id):
@@ -33,13 +49,14 @@ def _frame(gs, render, accum_id):
     return gs.frame.color_array()
 
 
-def test_zero_cache_equals_vertex1_truncation(pkg, gpu_ctx):
+@pytest.mark.parametrize("field,precision", FIELDS)
+def test_zero_cache_equals_vertex1_truncation(pkg, gpu_ctx, field, precision):
     vrt, wls = pkg.vrt, pkg.workloads
-    nrc = vrt.Nrc(gpu_ctx)
+    nrc = _nrc(pkg, gpu_ctx, precision)
     try:
         nrc.set_params(np.zeros(nrc.nparams, np.float32))
-        gs = wls.GpuScene(gpu_ctx, _wl(pkg))
-        ref = wls.GpuScene(gpu_ctx, _wl(pkg, max_bounce=0))
+        gs = wls.GpuScene(gpu_ctx, _wl(pkg, field=field))
+        ref = wls.GpuScene(gpu_ctx, _wl(pkg, max_bounce=0, field=field))
         opts = gs.nrc_options(train=False)
         for a in range(3):
             g = _frame(gs, lambda k: gs.render_nrc(nrc, opts, k), a)
@@ -51,16 +68,17 @@ def test_zero_cache_equals_vertex1_truncation(pkg, gpu_ctx):
         nrc.close()
 
 
-def test_constant_cache_adds_throughput_times_value(pkg, gpu_ctx):
+@pytest.mark.parametrize("field,precision", FIELDS)
+def test_constant_cache_adds_throughput_times_value(pkg, gpu_ctx, field, precision):
     vrt, wls = pkg.vrt, pkg.workloads
-    nrc = vrt.Nrc(gpu_ctx)
+    nrc = _nrc(pkg, gpu_ctx, precision)
     try:
         c = np.array([0.25, 0.5, 1.0], np.float32)
         p = np.zeros(nrc.nparams, np.float32)
-        p[NW + 5 * 64:NW + 5 * 64 + 3] = c  # output biases
+        p[NW + 5 * 64:NW + 5 * 64 + 3] = c  # output biases (exact in fp32: biases stay fp32)
         nrc.set_params(p)
-        gs = wls.GpuScene(gpu_ctx, _wl(pkg))
-        ref = wls.GpuScene(gpu_ctx, _wl(pkg, max_bounce=0))
+        gs = wls.GpuScene(gpu_ctx, _wl(pkg, field=field))
+        ref = wls.GpuScene(gpu_ctx, _wl(pkg, max_bounce=0, field=field))
         opts = gs.nrc_options(train=False)
         # accum_id 0 after a reset: the frame holds the sample itself
         g = _frame(gs, lambda k: gs.render_nrc(nrc, opts, k), 0)
@@ -138,15 +156,16 @@ def test_first_vertex_queries_reject_gbuffer(pkg, gpu_ctx):
         nrc.close()
 
 
-@pytest.mark.parametrize("query_vertex", [2, 1])
-def test_online_training_gain(pkg, gpu_ctx, query_vertex):
+@pytest.mark.parametrize("field,precision,query_vertex",
+                         [("fbm", "fp16", 2), ("fbm", "fp16", 1), ("tets", "bf16", 2)])
+def test_online_training_gain(pkg, gpu_ctx, field, precision, query_vertex):
     """relMSE against the converged plain image: the trained cache beats the
     truncated estimator without a cache (its paths end after the first
     vertex's light sample) by a wide margin, whether it is queried at the
     second volume vertex or at the first, where it also stands in for that
     light sample."""
     vrt, wls = pkg.vrt, pkg.workloads
-    wl = _wl(pkg, n=48, w=128, h=96)
+    wl = _wl(pkg, n=48, w=128, h=96, field=field)
     ref = wls.GpuScene(gpu_ctx, wl, accum64=True)
     ref.frame.reset()
     nref = 2048
@@ -155,16 +174,17 @@ def test_online_training_gain(pkg, gpu_ctx, query_vertex):
     gpu_ctx.sync()
     truth = ref.frame.accum64_array() / nref
 
-    trunc = wls.GpuScene(gpu_ctx, _wl(pkg, n=48, w=128, h=96, max_bounce=0))
+    trunc = wls.GpuScene(gpu_ctx, _wl(pkg, n=48, w=128, h=96, max_bounce=0, field=field),
+                         volume=ref.volume if field == "tets" else None)
     trunc.frame.reset()
     for a in range(256):
         trunc.render(a)
     gpu_ctx.sync()
     e_trunc = relmse(trunc.frame.accum_array(), truth)
 
-    nrc = vrt.Nrc(gpu_ctx, max_batch=4096, seed=7)
+    nrc = _nrc(pkg, gpu_ctx, precision, max_batch=4096, seed=7)
     try:
-        gs = wls.GpuScene(gpu_ctx, wl)
+        gs = wls.GpuScene(gpu_ctx, wl, volume=ref.volume)
         opts = gs.nrc_options(train=True, train_samples=4096, train_steps=2,
                               query_vertex=query_vertex)
         for a in range(300):  # warm-up training, frames discarded
@@ -176,6 +196,6 @@ def test_online_training_gain(pkg, gpu_ctx, query_vertex):
         e_nrc = relmse(gs.frame.accum_array(), truth)
     finally:
         nrc.close()
-    print(f"relMSE truncated {e_trunc:.4f}  with trained cache {e_nrc:.4f} "
-          f"(queries at vertex {query_vertex})")
+    print(f"{field} {precision}: relMSE truncated {e_trunc:.4f}  with trained cache "
+          f"{e_nrc:.4f} (queries at vertex {query_vertex})")
     assert e_nrc < 0.3 * e_trunc

@@ -271,3 +271,37 @@ def test_clip_planes_parity(pkg, oracle, gpu_ctx):
         agree = oracle.stream_agreement(g, o)
         print("clip planes stream agreement", accum_id, agree)
         assert agree >= oracle.STREAM_AGREEMENT_MIN, agree
+
+
+@pytest.mark.parametrize("field", ["fbm", "tets"])
+def test_pathtrace_ratio_tracking_parity(pkg, oracle, gpu_ctx, field):
+    """Ratio-tracked shadow and AO rays (the ratio_tracking extension): the GPU
+    frames stream-aligned with the oracle's, and the 32-frame mean within
+    relMSE 1e-3 of the oracle's FP32 and FP64 means."""
+    import dataclasses
+    if field == "tets":
+        wl = pkg.workloads.tets_pathtrace(n=8, width=64, height=48, frames=1, grid=16)
+    else:
+        wl = _pt_workload(pkg)
+    wl = dataclasses.replace(wl, params=dict(wl.params, ratio_tracking=True))
+    gs = pkg.workloads.GpuScene(gpu_ctx, wl)
+    ow = oracle.OracleWorkload(wl)
+    agree = []
+    for accum_id in range(3):
+        gs.frame.reset()
+        gs.render(accum_id)
+        gpu_ctx.sync()
+        o, _ = ow.pathtrace_frame(accum_id)
+        o = (np.float32(1) / np.float32(accum_id + 1)) * o
+        agree.append(oracle.stream_agreement(gs.frame.color_array(), o))
+    frames = 32
+    gs.frame.reset()
+    for a in range(frames):
+        gs.render(a)
+    gpu_ctx.sync()
+    g = gs.frame.accum_array()
+    e32 = relmse(g, ow.pathtrace_accumulate(frames))
+    e64 = relmse(g, ow.pathtrace_accumulate(frames, fp64=True))
+    print(field, "ratio tracking: stream agreement", agree, "relMSE fp32", e32, "fp64", e64)
+    assert min(agree) >= oracle.STREAM_AGREEMENT_MIN
+    assert e32 <= 1e-3 and e64 <= 1e-3
