@@ -178,10 +178,15 @@ struct KNrc {
   float *q_out;          // q_cap x 3 cache predictions
   float *t_raw;          // t_cap x 6
   float *t_target;       // t_cap x 3
+  // self-training (textra > 0): per training sample, the query that ended
+  // its path (-1: none) and that vertex's throughput over the query vertex's
+  float *t_ratio;        // t_cap x 3
+  int *t_query;          // t_cap
   uint32_t *ctr;         // [0] queries [1] training claims [2] vertex-2 paths [3] previous [2]
   int q_cap, t_cap;
   int train;             // record training paths this frame
   int qbounce;           // bounce index of the querying volume vertex (0 or 1)
+  int textra;            // training paths end in a query this many volume vertices later (0: never)
   f3 pos_lo, pos_scale;  // world -> [0,1]^3 cache coordinates
 };
 

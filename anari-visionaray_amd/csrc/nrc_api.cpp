@@ -15,11 +15,13 @@ using namespace vrt;
 
 void release_frame(vrt_nrc n) {
   for (void *p : {(void *)n->sample, (void *)n->query_of_pixel, (void *)n->q_raw, (void *)n->q_T,
-                  (void *)n->q_out, (void *)n->t_raw, (void *)n->t_target, (void *)n->ctr})
+                  (void *)n->q_out, (void *)n->t_raw, (void *)n->t_target, (void *)n->t_ratio,
+                  (void *)n->t_query, (void *)n->ctr})
     if (p) (void)hipFree(p);
   n->sample = nullptr;
   n->query_of_pixel = nullptr;
-  n->q_raw = n->q_T = n->q_out = n->t_raw = n->t_target = nullptr;
+  n->q_raw = n->q_T = n->q_out = n->t_raw = n->t_target = n->t_ratio = nullptr;
+  n->t_query = nullptr;
   n->ctr = nullptr;
   n->pixels = 0;
   n->t_cap = 0;
@@ -59,6 +61,8 @@ int nrc_frame_buffers(vrt_nrc n, size_t pixels, int t_cap) {
   A((void **)&n->q_out, sizeof(float) * 3 * P);
   A((void **)&n->t_raw, sizeof(float) * 6 * size_t(T));
   A((void **)&n->t_target, sizeof(float) * 3 * size_t(T));
+  A((void **)&n->t_ratio, sizeof(float) * 3 * size_t(T));
+  A((void **)&n->t_query, sizeof(int) * size_t(T));
   A((void **)&n->ctr, sizeof(uint32_t) * 4);
   if (e == hipSuccess) e = hipMemsetAsync(n->ctr, 0, sizeof(uint32_t) * 4, n->ctx->stream);
   if (e != hipSuccess) {

@@ -17,7 +17,8 @@ struct vrt_nrc_s {
   float4 *sample = nullptr;
   int *query_of_pixel = nullptr;
   float *q_raw = nullptr, *q_T = nullptr, *q_out = nullptr;
-  float *t_raw = nullptr, *t_target = nullptr;
+  float *t_raw = nullptr, *t_target = nullptr, *t_ratio = nullptr;
+  int *t_query = nullptr;
   uint32_t *ctr = nullptr;  // see vrt::KNrc::ctr
 };
 

@@ -24,6 +24,8 @@ hipError_t static_queue_address(uint32_t **ptr);
 hipError_t launch_pathtrace_static(const KScene &sc, const KCamera &cam, const KParams &P,
                                    const KFrame &F, int accumID, uint32_t *scratch_ptr,
                                    int blocks, hipStream_t s);
+// self-training: add the terminal queries' predictions to the targets
+hipError_t launch_nrc_train_targets(const KNrc &N, hipStream_t s);
 hipError_t launch_nrc_resolve(const KFrame &F, const KNrc &N, int accumID, hipStream_t s);
 hipError_t launch_taa_resolve(const KFrame &F, float alpha, hipStream_t s);
 // adds sum (x - r)^2 / (r^2 + eps) over n pixels' RGB to *out

@@ -1743,6 +1743,30 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
     const bool is_train = want_train && tslot < uint32_t(N.t_cap);
     const bool want_query = v2 && !is_train;
     const uint32_t qslot = wave_claim(&N.ctr[0], want_query);
+    // self-training: a training path whose vertex textra bounces after its
+    // query vertex is a volume vertex ends there in a cache query of its own;
+    // the query's prediction completes both the pixel (through the resolve)
+    // and the training target
+    const bool term = st.tslot >= 0 && N.textra > 0 && st.rayType == Radiance &&
+                      h.kind == kVolume && st.bounceID == N.qbounce + N.textra;
+    const uint32_t tq = wave_claim(&N.ctr[0], term);
+    if (term) {
+      if (tq < uint32_t(N.q_cap)) {
+        float raw[6];
+        cache_inputs(N, ray.ori + ray.dir * h.t, ray.dir, h.extinction, raw);
+        for (int i = 0; i < 6; ++i) N.q_raw[6 * size_t(tq) + i] = raw[i];
+        N.q_T[3 * size_t(tq)] = st.throughput.x;
+        N.q_T[3 * size_t(tq) + 1] = st.throughput.y;
+        N.q_T[3 * size_t(tq) + 2] = st.throughput.z;
+        st.qslot = int(tq);
+        N.t_query[st.tslot] = int(tq);
+        float *r = N.t_ratio + 3 * size_t(st.tslot);
+        r[0] = st.T2.x > 0.f ? st.throughput.x / st.T2.x : 0.f;
+        r[1] = st.T2.y > 0.f ? st.throughput.y / st.T2.y : 0.f;
+        r[2] = st.T2.z > 0.f ? st.throughput.z / st.T2.z : 0.f;
+      }
+      return true;
+    }
     if (v2) {
       float raw[6];
       cache_inputs(N, ray.ori + ray.dir * h.t, ray.dir, h.extinction, raw);
@@ -1751,6 +1775,7 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
         st.tslot = int(tslot);
         st.I2 = st.intensity;
         st.T2 = st.throughput;
+        if (N.textra > 0) N.t_query[tslot] = -1;
       } else {
         if (qslot < uint32_t(N.q_cap)) {
           for (int i = 0; i < 6; ++i) N.q_raw[6 * size_t(qslot) + i] = raw[i];
@@ -2233,6 +2258,24 @@ __global__ void k_nrc_frame_begin(KNrc N) {
   }
 }
 
+// Self-training targets: a training sample whose path ended in a cache
+// query adds that query's prediction, scaled by the path's throughput there
+// over its throughput at the query vertex, to its target (the suffix radiance
+// the path gathered on the way)
+__global__ void k_nrc_train_targets(KNrc N) {
+  const int n = min(N.t_cap, int(N.ctr[1]));
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int q = N.t_query[i];
+    if (q < 0) continue;
+    const float *o = N.q_out + 3 * size_t(q);
+    const float *r = N.t_ratio + 3 * size_t(i);
+    float *t = N.t_target + 3 * size_t(i);
+    t[0] += r[0] * fmaxf(o[0], 0.f);
+    t[1] += r[1] * fmaxf(o[1], 0.f);
+    t[2] += r[2] * fmaxf(o[2], 0.f);
+  }
+}
+
 __global__ void k_nrc_resolve(KFrame F, KNrc N, int accumID) {
   const int n = F.width * F.height;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -2671,6 +2714,13 @@ hipError_t launch_taa_resolve(const KFrame &F, float alpha, hipStream_t s) {
   if (e == hipSuccess)
     e = hipMemcpyAsync(F.taa_prev_alb, F.taa_curr_alb, size_t(n) * 12, hipMemcpyDeviceToDevice, s);
   return e;
+}
+
+hipError_t launch_nrc_train_targets(const KNrc &N, hipStream_t s) {
+  if (N.t_cap <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_nrc_train_targets, dim3(std::min((N.t_cap + 255) / 256, 4096)), dim3(256),
+                     0, s, N);
+  return hipGetLastError();
 }
 
 hipError_t launch_nrc_resolve(const KFrame &F, const KNrc &N, int accumID, hipStream_t s) {

@@ -793,6 +793,9 @@ int vrt_pathtrace_nrc(vrt_context ctx, const vrt_scene *scene, const vrt_camera 
     return fail(VRT_ERROR_INVALID_ARGUMENT, "train_steps must be >= 1 when training");
   if (opts->query_vertex < 0 || opts->query_vertex > 2)
     return fail(VRT_ERROR_INVALID_ARGUMENT, "query_vertex must be 1 or 2 (0: 2)");
+  if (opts->train_extra_vertices < 0 || opts->train_extra_vertices > 64)
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "train_extra_vertices %d out of [0, 64]",
+                opts->train_extra_vertices);
   const int qbounce = (opts->query_vertex == 0 ? 2 : opts->query_vertex) - 1;
   KScene sc;
   KCamera cam;
@@ -818,11 +821,14 @@ int vrt_pathtrace_nrc(vrt_context ctx, const vrt_scene *scene, const vrt_camera 
   N.q_out = nrc->q_out;
   N.t_raw = nrc->t_raw;
   N.t_target = nrc->t_target;
+  N.t_ratio = nrc->t_ratio;
+  N.t_query = nrc->t_query;
   N.ctr = nrc->ctr;
   N.q_cap = int(pixels);
   N.t_cap = opts->train ? std::max(0, opts->train_samples) : 0;
   N.train = opts->train && opts->train_samples > 0;
   N.qbounce = qbounce;
+  N.textra = opts->train_extra_vertices;
   N.pos_lo = mk(opts->bounds_lo[0], opts->bounds_lo[1], opts->bounds_lo[2]);
   for (int a = 0; a < 3; ++a)
     if (!(opts->bounds_hi[a] > opts->bounds_lo[a]))
@@ -844,6 +850,7 @@ int vrt_pathtrace_nrc(vrt_context ctx, const vrt_scene *scene, const vrt_camera 
   VRT_HIP(nrc_launch_infer(nrc->d, N.q_cap, N.q_raw, N.q_out, ctx->num_cus, ctx->stream,
                            &N.ctr[0]));
   VRT_HIP(launch_nrc_resolve(F, N, accum_id, ctx->stream));
+  if (N.train && N.textra > 0) VRT_HIP(launch_nrc_train_targets(N, ctx->stream));
   if (N.train) {
     const int saved = nrc->t_cap;
     nrc->t_cap = N.t_cap;

@@ -128,6 +128,7 @@ class NrcRender(C.Structure):
     _fields_ = [
         ("train", C.c_int), ("train_samples", C.c_int), ("train_steps", C.c_int),
         ("bounds_lo", f3), ("bounds_hi", f3), ("query_vertex", C.c_int),
+        ("train_extra_vertices", C.c_int),
     ]
 
 
@@ -949,9 +950,10 @@ class Nrc:
 
 
 def nrc_render_options(bounds_lo, bounds_hi, train=True, train_samples=65536, train_steps=4,
-                       query_vertex=2):
+                       query_vertex=2, train_extra_vertices=0):
     o = NrcRender()
     o.query_vertex = int(query_vertex)
+    o.train_extra_vertices = int(train_extra_vertices)
     o.train = int(train)
     o.train_samples = int(train_samples)
     o.train_steps = int(train_steps)

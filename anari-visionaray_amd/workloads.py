@@ -295,10 +295,12 @@ class GpuScene:
         else:
             _vrt.pathtrace(self.ctx, self.scene, self.camera, self.params, self.frame, accum_id)
 
-    def nrc_options(self, train=True, train_samples=65536, train_steps=4, query_vertex=2):
+    def nrc_options(self, train=True, train_samples=65536, train_steps=4, query_vertex=2,
+                    train_extra_vertices=0):
         """Cache options whose [0,1]^3 box is the volume's bounds."""
         lo, hi = volume_bounds(self.wl)
-        return _vrt.nrc_render_options(lo, hi, train, train_samples, train_steps, query_vertex)
+        return _vrt.nrc_render_options(lo, hi, train, train_samples, train_steps, query_vertex,
+                                       train_extra_vertices)
 
     def render_nrc(self, nrc, opts, accum_id=0):
         _vrt.pathtrace_nrc(self.ctx, self.scene, self.camera, self.params, self.frame, nrc, opts,

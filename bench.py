@@ -85,7 +85,8 @@ def reference_scene(pkg, ctx, wl, volume=None, ref_frames=2048):
 
 def nrc_vs_plain(pkg, ctx, wl, volume=None, ref_frames=2048, cap=1024, warmup=4, train_steps=8,
                  target=1e-2, oracle_stride=24, oracle_spp=4096, timed_frames=32,
-                 precisions=("fp16",), query_vertex=2, train_samples=65536, ref=None):
+                 precisions=("fp16",), query_vertex=2, train_samples=65536, ref=None,
+                 train_extra=0):
     """Wall time to relMSE <= target of the plain path tracer and of the
     radiance-cache path tracer (its training warm-up included) for each cache
     precision, all measured against an FP64-accumulated plain reference of
@@ -125,7 +126,7 @@ def nrc_vs_plain(pkg, ctx, wl, volume=None, ref_frames=2048, cap=1024, warmup=4,
     warm = wls.GpuScene(ctx, wl, volume=volume, params={"rng_offset": 2 << 20})
     gn = wls.GpuScene(ctx, wl, volume=volume, params={"rng_offset": 3 << 20})
     opts = warm.nrc_options(train=True, train_samples=train_samples, train_steps=train_steps,
-                            query_vertex=query_vertex)
+                            query_vertex=query_vertex, train_extra_vertices=train_extra)
 
     def nrc_run(precision, frames=None):
         nrc = vrt.Nrc(ctx, precision=precision, max_batch=16384, seed=1)
@@ -163,7 +164,7 @@ def nrc_vs_plain(pkg, ctx, wl, volume=None, ref_frames=2048, cap=1024, warmup=4,
         nrc_ms = t_nrc / (warmup + frames_nrc)
         key = "nrc" if i == 0 else f"nrc_{prec}"
         out[key] = {"query_vertex": query_vertex, "train_samples_per_frame": train_samples,
-                    "warmup_frames": warmup, "frames": frames_nrc,
+                    "train_extra_vertices": train_extra, "warmup_frames": warmup, "frames": frames_nrc,
                     "ms_per_frame": round(nrc_ms, 4),
                     "ms_to_target": round(t_nrc, 2) if nrc_reached else None, "relmse": e_nrc,
                     "queries_last_frame": counts[0],
@@ -282,7 +283,7 @@ KERNEL_WORKLOADS = ("pathtrace_fbm512", "raycast_blobs512", "raycast_blobs128",
 
 
 def kernel_only(pkg, ctx, name, steps, warmup, precision="fp16", query_vertex=2,
-                train_samples=65536, train_steps=4):
+                train_samples=65536, train_steps=8, train_extra=0):
     """One workload's frames alone (profiling runs): `warmup` untimed frames,
     then `steps` frames timed on HIP events; prints one JSON line."""
     vrt, wls = pkg.vrt, pkg.workloads
@@ -302,7 +303,7 @@ def kernel_only(pkg, ctx, name, steps, warmup, precision="fp16", query_vertex=2,
         nrc = vrt.Nrc(ctx, precision={"fp16": vrt.NRC_FP16, "bf16": vrt.NRC_BF16}[precision],
                       max_batch=16384, seed=1)
         opts = gs.nrc_options(train=True, train_samples=train_samples, train_steps=train_steps,
-                              query_vertex=query_vertex)
+                              query_vertex=query_vertex, train_extra_vertices=train_extra)
 
     def frame(a):
         if nrc is not None:
@@ -326,7 +327,8 @@ def kernel_only(pkg, ctx, name, steps, warmup, precision="fp16", query_vertex=2,
            "algorithmic_GBps": round(lookups * BYTES_PER_LOOKUP / (ms / 1e3) / 1e9, 1)}
     if nrc is not None:
         out.update(precision=precision, query_vertex=query_vertex, train_samples=train_samples,
-                   train_steps=train_steps, counts=list(nrc.frame_counts()))
+                   train_steps=train_steps, train_extra_vertices=train_extra,
+                   counts=list(nrc.frame_counts()))
         nrc.close()
     print(json.dumps(out), flush=True)
 
@@ -368,6 +370,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--nrc-query-vertex", type=int, default=2, choices=(1, 2),
                     help="volume vertex whose cache query ends a path (2: the configuration)")
+    ap.add_argument("--nrc-train-extra", type=int, default=0,
+                    help="self-training: training paths end in a cache query this many bounces "
+                         "after the query vertex (0: traced to full length)")
     ap.add_argument("--ttr-size", type=int, default=1024,
                     help="index-space edge of the nanovdb cloud of the time-to-relMSE run")
     ap.add_argument("--no-mixed", action="store_true",
@@ -397,7 +402,7 @@ def main():
     if args.kernel:
         ctx = vrt.Context(0)
         kernel_only(pkg, ctx, args.kernel, args.steps, args.warmup, args.precision,
-                    args.nrc_query_vertex)
+                    args.nrc_query_vertex, train_extra=args.nrc_train_extra)
         ctx.close()
         return
     t0 = time.perf_counter()
@@ -451,11 +456,12 @@ def main():
               flush=True)
         ref3 = reference_scene(pkg, ctx, wl3)
         nrc = nrc_vs_plain(pkg, ctx, wl3, ref=ref3, oracle_stride=0 if args.no_cpu else 24,
-                           query_vertex=args.nrc_query_vertex, train_samples=65536)
+                           query_vertex=args.nrc_query_vertex, train_samples=65536,
+                           train_extra=args.nrc_train_extra)
         print(f"# time to relMSE 1e-2: {json.dumps(nrc)}", file=sys.stderr, flush=True)
         # the same with queries at the first volume vertex (32,768 samples)
         nrc_v1 = nrc_vs_plain(pkg, ctx, wl3, ref=ref3, oracle_stride=0, query_vertex=1,
-                              train_samples=32768)
+                              train_samples=32768, train_extra=args.nrc_train_extra)
         print(f"# time to relMSE 1e-2, first-vertex queries: {json.dumps(nrc_v1)}",
               file=sys.stderr, flush=True)
         del ref3
@@ -486,7 +492,7 @@ def main():
         setup = time.perf_counter() - t0
         c4 = nrc_vs_plain(pkg, ctx, wl4, volume=gs4.volume, oracle_stride=0, ref_frames=1024,
                           precisions=("fp16", "bf16"), query_vertex=args.nrc_query_vertex,
-                          train_samples=65536)
+                          train_samples=65536, train_extra=args.nrc_train_extra)
         c4["field"] = {"tetrahedra": int(inf.num_cells), "grid_dims": list(inf.grid_dims),
                        "mean_list": round(inf.mean_list, 2),
                        "expected_tests": round(inf.expected_tests, 3),

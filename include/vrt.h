@@ -561,9 +561,10 @@ VRT_API int vrt_nrc_train_step(vrt_nrc nrc, int n, const float *raw, const float
 /* 'pathtrace' frame with the radiance cache.  Paths end at their second
  * volume vertex (or their first: query_vertex 1) in a cache query (the cache
  * supplies the radiance leaving that vertex towards the path); a fraction of
- * paths, adapted on the GPU to give `train_samples` per frame, are traced to
- * full length instead and leave one training sample (the query vertex's
- * inputs, suffix radiance / throughput).
+ * paths, adapted on the GPU to give `train_samples` per frame, are traced on
+ * instead (to full length, or train_extra_vertices further bounces and a
+ * cache query of their own) and leave one training sample (the query
+ * vertex's inputs, suffix radiance / throughput).
  * After the frame is resolved the cache takes `train_steps` Adam steps on
  * those samples.  Requires params->pixel_samples == 1.  The cache inputs
  * map world positions in [bounds_lo, bounds_hi] onto [0,1]^3. */
@@ -575,6 +576,13 @@ typedef struct vrt_nrc_render {
   float bounds_hi[3];
   int query_vertex;   /* volume vertex that queries the cache: 2 (0 = default) or 1;
                          1 writes no g-buffer channels (rejected when the frame has them) */
+  /* self-training (Mueller et al. 2021): 0 (default) traces training paths
+     to full length; k in [1, 64] ends a training path whose vertex k bounces
+     after its query vertex is a volume vertex in a cache query there, whose
+     prediction (times the throughput gathered since the query vertex)
+     completes both the pixel and the training target; paths meeting a
+     surface or a light there go on to full length */
+  int train_extra_vertices;
 } vrt_nrc_render;
 
 VRT_API int vrt_pathtrace_nrc(vrt_context ctx, const vrt_scene *scene,

@@ -7,6 +7,9 @@
                         paths are the plain paths (queries at vertex 2 or 1)
   first-vertex query  a constant cache c adds exactly c per query (throughput
                         1 before the first scattering)
+  self-training      an all-training frame whose paths end in their own query
+                        k bounces later equals, with a zero cache, the plain
+                        path tracer cut at max_bounce k (bit-exact)
   learning gain      after online training the cached estimator is far
                         closer to the converged image than the truncated
                         estimator without the cache
@@ -156,14 +159,74 @@ def test_first_vertex_queries_reject_gbuffer(pkg, gpu_ctx):
         nrc.close()
 
 
-@pytest.mark.parametrize("field,precision,query_vertex",
-                         [("fbm", "fp16", 2), ("fbm", "fp16", 1), ("tets", "bf16", 2)])
-def test_online_training_gain(pkg, gpu_ctx, field, precision, query_vertex):
+@pytest.mark.parametrize("textra", [1, 2])
+def test_self_training_zero_cache_equals_truncation(pkg, gpu_ctx, textra):
+    """First frame, every path reaching vertex 2 trains; with self-training a
+    training path whose vertex textra bounces later is a volume vertex ends
+    there in a cache query.  A zero cache adds nothing, so the frame is the
+    plain path tracer's with max_bounce textra (the zero cache at vertex 2
+    itself gives max_bounce 0).  A constant cache c then adds throughput * c
+    at each terminal query: the same multiple of c in every channel (the
+    medium is grey)."""
+    vrt, wls = pkg.vrt, pkg.workloads
+    nrc = vrt.Nrc(gpu_ctx, max_batch=65536)
+    try:
+        nrc.set_params(np.zeros(nrc.nparams, np.float32))
+        gs = wls.GpuScene(gpu_ctx, _wl(pkg))
+        ref = wls.GpuScene(gpu_ctx, _wl(pkg, max_bounce=textra))
+        opts = gs.nrc_options(train=True, train_samples=65536, train_steps=1,
+                              train_extra_vertices=textra)
+        g = _frame(gs, lambda k: gs.render_nrc(nrc, opts, k), 0)
+        r = _frame(ref, ref.render, 0)
+        np.testing.assert_array_equal(g, r)
+        q, t, v2, _ = nrc.frame_counts()
+        assert t == v2 > 100 and 0 < q < t
+
+        c = np.array([0.25, 0.5, 1.0], np.float32)
+        p = np.zeros(nrc.nparams, np.float32)
+        p[NW + 5 * 64:NW + 5 * 64 + 3] = c  # output biases
+        nrc2 = vrt.Nrc(gpu_ctx, max_batch=65536)  # fresh counters: all paths train again
+        try:
+            nrc2.set_params(p)
+            g2 = _frame(gs, lambda k: gs.render_nrc(nrc2, opts, k), 0)
+            q2 = nrc2.frame_counts()[0]
+        finally:
+            nrc2.close()
+        d = (g2 - r)[..., :3]
+        queried = np.abs(d[..., 2]) > 1e-6
+        assert queried.sum() == q2 == q
+        k = d[queried] / c
+        np.testing.assert_allclose(k[:, 0], k[:, 2], rtol=1e-5)
+        np.testing.assert_allclose(k[:, 1], k[:, 2], rtol=1e-5)
+        assert np.all(np.abs(d[~queried]) <= 1e-6)
+    finally:
+        nrc.close()
+
+
+@pytest.mark.parametrize("textra", [0, 64])
+def test_self_training_option_bounds(pkg, gpu_ctx, textra):
+    vrt, wls = pkg.vrt, pkg.workloads
+    nrc = vrt.Nrc(gpu_ctx)
+    try:
+        gs = wls.GpuScene(gpu_ctx, _wl(pkg))
+        gs.render_nrc(nrc, gs.nrc_options(train_samples=1024, train_extra_vertices=textra), 0)
+        for bad in (-1, 65):
+            with pytest.raises(vrt.VrtError, match="train_extra_vertices"):
+                gs.render_nrc(nrc, gs.nrc_options(train_extra_vertices=bad), 0)
+    finally:
+        nrc.close()
+
+
+@pytest.mark.parametrize("field,precision,query_vertex,textra",
+                         [("fbm", "fp16", 2, 0), ("fbm", "fp16", 1, 0), ("tets", "bf16", 2, 0),
+                          ("fbm", "fp16", 2, 2), ("tets", "bf16", 2, 1)])
+def test_online_training_gain(pkg, gpu_ctx, field, precision, query_vertex, textra):
     """relMSE against the converged plain image: the trained cache beats the
     truncated estimator without a cache (its paths end after the first
     vertex's light sample) by a wide margin, whether it is queried at the
     second volume vertex or at the first, where it also stands in for that
-    light sample."""
+    light sample, and whether its training paths run to full length or end
+    in a query of their own (self-training, textra > 0)."""
     vrt, wls = pkg.vrt, pkg.workloads
     wl = _wl(pkg, n=48, w=128, h=96, field=field)
     ref = wls.GpuScene(gpu_ctx, wl, accum64=True)
@@ -186,7 +249,7 @@ def test_online_training_gain(pkg, gpu_ctx, field, precision, query_vertex):
     try:
         gs = wls.GpuScene(gpu_ctx, wl, volume=ref.volume)
         opts = gs.nrc_options(train=True, train_samples=4096, train_steps=2,
-                              query_vertex=query_vertex)
+                              query_vertex=query_vertex, train_extra_vertices=textra)
         for a in range(300):  # warm-up training, frames discarded
             gs.render_nrc(nrc, opts, 10000 + a)
         gs.frame.reset()
@@ -197,5 +260,5 @@ def test_online_training_gain(pkg, gpu_ctx, field, precision, query_vertex):
     finally:
         nrc.close()
     print(f"{field} {precision}: relMSE truncated {e_trunc:.4f}  with trained cache "
-          f"{e_nrc:.4f} (queries at vertex {query_vertex})")
+          f"{e_nrc:.4f} (queries at vertex {query_vertex}, self-training {textra})")
     assert e_nrc < 0.3 * e_trunc
