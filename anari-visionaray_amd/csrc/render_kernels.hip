@@ -293,10 +293,20 @@ __device__ inline float sample_nanovdb(const KVolume &v, f3 P) {
   const float fx = floorf(idx[0]), fy = floorf(idx[1]), fz = floorf(idx[2]);
   const float wx = idx[0] - fx, wy = idx[1] - fy, wz = idx[2] - fz;
   const int i = int(fx), j = int(fy), k = int(fz);
-  const float v00 = lerpf(nvdb_value(v, i, j, k), nvdb_value(v, i, j, k + 1), wz);
-  const float v01 = lerpf(nvdb_value(v, i, j + 1, k), nvdb_value(v, i, j + 1, k + 1), wz);
-  const float v10 = lerpf(nvdb_value(v, i + 1, j, k), nvdb_value(v, i + 1, j, k + 1), wz);
-  const float v11 = lerpf(nvdb_value(v, i + 1, j + 1, k), nvdb_value(v, i + 1, j + 1, k + 1), wz);
+  // the accessor's leaf cache: the slot of the (i, j, k) corner serves every
+  // tap in the same 8^3 slot; a tap across a slot boundary looks its own up
+  const int e0 = nvdb_slot(v, i >> 3, j >> 3, k >> 3);
+  auto tap = [&](int a, int b, int c) {
+    int e = e0;
+    if (((a >> 3) != (i >> 3)) | ((b >> 3) != (j >> 3)) | ((c >> 3) != (k >> 3)))
+      e = nvdb_slot(v, a >> 3, b >> 3, c >> 3);
+    return e >= 0 ? v.leaves[size_t(e) * 512u + uint32_t(((a & 7) << 6) | ((b & 7) << 3) | (c & 7))]
+                  : v.nv_consts[-1 - e];
+  };
+  const float v00 = lerpf(tap(i, j, k), tap(i, j, k + 1), wz);
+  const float v01 = lerpf(tap(i, j + 1, k), tap(i, j + 1, k + 1), wz);
+  const float v10 = lerpf(tap(i + 1, j, k), tap(i + 1, j, k + 1), wz);
+  const float v11 = lerpf(tap(i + 1, j + 1, k), tap(i + 1, j + 1, k + 1), wz);
   return lerpf(lerpf(v00, v01, wy), lerpf(v10, v11, wy), wx);
 }
 
