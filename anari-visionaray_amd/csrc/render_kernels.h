@@ -6,7 +6,7 @@
 namespace vrt {
 
 hipError_t launch_raycast(const KScene &sc, const KCamera &cam, const KParams &P, const KFrame &F,
-                          hipStream_t s);
+                          uint32_t *queues, uint32_t *queues_next, int num_cus, hipStream_t s);
 // persistent path tracer: `blocks` workgroups (a multiple of 8) drain the
 // per-XCD pixel queues; `queues` must be zero, `queues_next` is zeroed for
 // the following launch

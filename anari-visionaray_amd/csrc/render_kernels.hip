@@ -50,24 +50,11 @@ __device__ inline float rnd(uint32_t &s) {
 }
 
 // ---------------------------------------------------------------------------
-// XCD-aware tile mapping
+// Work queues: the persistent kernels cut the image into 8 parts, one per
+// XCD (the hardware places workgroup b on XCD b % 8), each drained from an
+// atomic counter
 // ---------------------------------------------------------------------------
-__device__ inline int xcd_remap(int b, int n) {
-  // hardware dispatches workgroup b to XCD b % 8; hand XCD k the k-th
-  // contiguous share of the logical tile sequence
-  const int q = n >> 3, r = n & 7;
-  const int xcd = b & 7, idx = b >> 3;
-  return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
-}
-
-__device__ inline void tile_pixel(int W, int &px, int &py) {
-  const int ntx = (W + 15) >> 4;
-  const int t = xcd_remap(int(blockIdx.x), int(gridDim.x));
-  const int tx = t % ntx, ty = t / ntx;
-  const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
-  px = (tx << 4) + ((wave & 1) << 3) + (lane & 7);
-  py = (ty << 4) + ((wave >> 1) << 3) + (lane >> 3);
-}
+constexpr int kQueues = 8;
 
 // ---------------------------------------------------------------------------
 // Rays / boxes
@@ -1276,19 +1263,14 @@ __device__ inline void gbuf_miss(const KFrame &F, size_t i) {
   if (1e31f <= F.depth[i]) gbuf_write(F, i, mk(0.f), mk(0.f), ~0u, ~0u, ~0u);
 }
 
-// SURF: the scene has surfaces (the BVH walk and surface shading are
-// compiled in, with the traversal stack's top in LDS after the tables)
+// One pixel of the ray caster (Raycast_impl.cpp).  SURF: the scene has
+// surfaces (the BVH walk and surface shading are compiled in, with the
+// traversal stack's top in LDS after the tables)
 template <bool ANYF, bool SHADE, bool SURF>
-__global__ void __launch_bounds__(256)
-    k_raycast(KScene sc, KCamera cam, KParams P, KFrame F) {
-  extern __shared__ float4 s_tf[];
-  __shared__ float2 s_pow[64];  // pow_ieee's log2 table
-  if (threadIdx.x < 64) s_pow[threadIdx.x] = kPowLog2[threadIdx.x];
-  stage_tf(sc, s_tf);  // (its barrier also publishes s_pow)
-  int x, y;
-  tile_pixel(F.width, x, y);
+__device__ inline void raycast_pixel(const KScene &sc, const KCamera &cam, const KParams &P,
+                                     const KFrame &F, int x, int y, float4 *s_tf,
+                                     const float2 *s_pow, uint32_t &lookups) {
   const bool active = x < F.width && y < F.height;
-  uint32_t lookups = 0;
   if (active) {
     float4 result = P.bg;
     int hit_vol = -1;
@@ -1447,6 +1429,46 @@ __global__ void __launch_bounds__(256)
     }
     write_sample(F, x, y, 0, result, gb_depth, false);
   }
+}
+
+// The ray caster.  Persistent: the grid fills the chip once and every wave
+// claims 8x8-pixel tiles from a work queue until the image is done, so waves
+// of short rays (background, early saturation) hand their SIMD to the next
+// tile at once instead of waiting for workgroup dispatch, and the frame has
+// no tail of half-empty CUs.  The row-major tile sequence is cut into 8 contiguous
+// bands; band q is queue q, drained first by the workgroups the hardware
+// placed on XCD q (blockIdx % 8), which then help the other bands.
+template <bool ANYF, bool SHADE, bool SURF>
+__global__ void __launch_bounds__(256)
+    k_raycast(KScene sc, KCamera cam, KParams P, KFrame F, uint32_t *queues,
+              uint32_t *queues_next) {
+  extern __shared__ float4 s_tf[];
+  __shared__ float2 s_pow[64];  // pow_ieee's log2 table
+  if (threadIdx.x < 64) s_pow[threadIdx.x] = kPowLog2[threadIdx.x];
+  stage_tf(sc, s_tf);  // (its barrier also publishes s_pow)
+  // clear the counters the next launch will use (the current ones were
+  // cleared by the previous launch)
+  if (blockIdx.x == 0 && threadIdx.x < kQueues) queues_next[threadIdx.x] = 0u;
+  const int lane = int(threadIdx.x) & 63;
+  const int ntx = (F.width + 7) >> 3;
+  const uint32_t ntiles = uint32_t(ntx) * uint32_t((F.height + 7) >> 3);
+  uint32_t lookups = 0;
+  int q = int(blockIdx.x) & (kQueues - 1);
+  for (int tries = 0; tries < kQueues;) {
+    const uint32_t lo = ntiles * uint32_t(q) / kQueues;
+    const uint32_t hi = ntiles * uint32_t(q + 1) / kQueues;
+    uint32_t k = 0;
+    if (lane == 0) k = atomicAdd(&queues[q], 1u);
+    k = __shfl(k, 0, 64);
+    if (lo + k >= hi) {  // band drained: help the next one
+      q = (q + 1) & (kQueues - 1);
+      ++tries;
+      continue;
+    }
+    const int tile = int(lo + k);
+    const int x = (tile % ntx) * 8 + (lane & 7), y = (tile / ntx) * 8 + (lane >> 3);
+    raycast_pixel<ANYF, SHADE, SURF>(sc, cam, P, F, x, y, s_tf, s_pow, lookups);
+  }
   if (F.counters) wave_add_counter(&F.counters[0], lookups);
 }
 
@@ -1532,7 +1554,6 @@ __device__ inline void cache_inputs(const KNrc &N, f3 pos, f3 dir, float density
 }
 
 constexpr int kChunkLog2 = 5;  // 32x32-pixel chunks
-constexpr int kQueues = 8;
 
 __device__ inline void chunk_pixel(int q, uint32_t local, int nchx, int &x, int &y) {
   const uint32_t j = local >> (2 * kChunkLog2);
@@ -2613,24 +2634,31 @@ __global__ void k_frame_reset(KFrame F) {
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-static inline int tiles(const KFrame &F) { return ((F.width + 15) / 16) * ((F.height + 15) / 16); }
 
 
 static bool scene_has_other_fields(const KScene &sc);
 
 hipError_t launch_raycast(const KScene &sc, const KCamera &cam, const KParams &P, const KFrame &F,
-                          hipStream_t s) {
+                          uint32_t *queues, uint32_t *queues_next, int num_cus, hipStream_t s) {
   const bool surf = sc.num_surfaces > 0;
   const size_t lds = scene_lds_bytes(sc) + (surf ? bvh_lds_stack_bytes() : 0);
   const bool any = scene_has_other_fields(sc), shade = P.gradient_shading != 0;
-  using RaycastKernel = void (*)(KScene, KCamera, KParams, KFrame);
+  using RaycastKernel = void (*)(KScene, KCamera, KParams, KFrame, uint32_t *, uint32_t *);
   static const RaycastKernel table[8] = {
       k_raycast<false, false, false>, k_raycast<true, false, false>,
       k_raycast<false, true, false>,  k_raycast<true, true, false>,
       k_raycast<false, false, true>,  k_raycast<true, false, true>,
       k_raycast<false, true, true>,   k_raycast<true, true, true>};
-  hipLaunchKernelGGL(table[(surf ? 4 : 0) + (shade ? 2 : 0) + (any ? 1 : 0)], dim3(tiles(F)),
-                     dim3(256), lds, s, sc, cam, P, F);
+  const RaycastKernel k = table[(surf ? 4 : 0) + (shade ? 2 : 0) + (any ? 1 : 0)];
+  // one resident wave per SIMD slot, no more workgroups than tiles need
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, lds) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  const int ntiles = ((F.width + 7) / 8) * ((F.height + 7) / 8);
+  int blocks = std::min(std::max(1, num_cus) * per_cu, (ntiles + 3) / 4);
+  blocks = std::max(8, (blocks + 7) / 8 * 8);  // every XCD's queue gets workgroups
+  hipLaunchKernelGGL(k, dim3(blocks), dim3(256), lds, s, sc, cam, P, F, queues, queues_next);
   return hipGetLastError();
 }
 

@@ -671,7 +671,10 @@ int vrt_raycast(vrt_context ctx, const vrt_scene *scene, const vrt_camera *camer
   // the fixed-step marcher reads no majorants
   sc.maj_lds_bytes = 0;
   for (int i = 0; i < sc.num_volumes; ++i) sc.vol[i].maj_lds_offset = -1;
-  VRT_HIP(launch_raycast(sc, cam, P, F, ctx->stream));
+  uint32_t *cur = ctx->queues + 8 * ctx->queue_parity;
+  uint32_t *nxt = ctx->queues + 8 * (1 - ctx->queue_parity);
+  VRT_HIP(launch_raycast(sc, cam, P, F, cur, nxt, ctx->num_cus, ctx->stream));
+  ctx->queue_parity ^= 1;
   return VRT_OK;
 }
 

@@ -278,8 +278,63 @@ def calibrate_copy(vrt, mib, reps):
     ctx.close()
 
 
+# radiance-cache network, unpadded multiply-adds per sample: 121x64 + 4 x 64x64
+# + 64x3 forward; the backward adds W^T dZ through layers 5..1 (64x3 + 4 x
+# 64x64) and dW of every layer (the forward count again)
+NRC_FWD_MACS = 121 * 64 + 4 * 64 * 64 + 64 * 3
+NRC_BWD_MACS = (64 * 3 + 4 * 64 * 64) + NRC_FWD_MACS
+MFMA_PEAK_TFLOPS = {"fp16": 2500.0, "bf16": 2500.0}  # MI355X dense
+
+
+def mfma_rates(pkg, ctx, infer_n=1 << 21, train_n=65536, reps=20):
+    """The cache's MFMA kernels alone, on device-resident random inputs:
+    inference over infer_n samples (a 1080p frame's worth of queries) and the
+    gradient pass (forward + backward + reduction) over train_n samples,
+    timed on HIP events; algorithmic TFLOP/s from the unpadded multiply-adds
+    and the fraction of the MI355X dense fp16/bf16 peak."""
+    import numpy as np
+    vrt = pkg.vrt
+    rng = np.random.default_rng(5)
+    raw = ctx.upload(rng.random((infer_n, 6), dtype=np.float32))
+    out = ctx.buffer(infer_n * 12)
+    tgt = ctx.upload(rng.random((train_n, 3), dtype=np.float32))
+    lib = vrt.lib()
+    res = {}
+    for prec in ("fp16", "bf16"):
+        code = {"fp16": vrt.NRC_FP16, "bf16": vrt.NRC_BF16}[prec]
+        nrc = vrt.Nrc(ctx, precision=code, max_batch=train_n, seed=3)
+        for _ in range(3):
+            vrt.check(lib.vrt_nrc_infer(nrc.handle, infer_n, raw.ptr, out.ptr))
+            vrt.check(lib.vrt_nrc_gradients(nrc.handle, train_n, raw.ptr, tgt.ptr, None, None))
+        ctx.sync()
+        ctx.timer_start()
+        for _ in range(reps):
+            vrt.check(lib.vrt_nrc_infer(nrc.handle, infer_n, raw.ptr, out.ptr))
+        ms_inf = ctx.timer_stop() / reps
+        ctx.timer_start()
+        for _ in range(reps):
+            vrt.check(lib.vrt_nrc_gradients(nrc.handle, train_n, raw.ptr, tgt.ptr, None, None))
+        ms_grad = ctx.timer_stop() / reps
+        nrc.close()
+        t_inf = infer_n * NRC_FWD_MACS * 2 / (ms_inf / 1e3) / 1e12
+        t_grad = train_n * (NRC_FWD_MACS + NRC_BWD_MACS) * 2 / (ms_grad / 1e3) / 1e12
+        res[prec] = {
+            "infer": {"samples": infer_n, "ms": round(ms_inf, 4), "TFLOPs": round(t_inf, 1),
+                      "frac_of_peak": round(t_inf / MFMA_PEAK_TFLOPS[prec], 4)},
+            "train_gradients": {"samples": train_n, "ms": round(ms_grad, 4),
+                                "TFLOPs": round(t_grad, 1),
+                                "frac_of_peak": round(t_grad / MFMA_PEAK_TFLOPS[prec], 4)}}
+    for b in (raw, out, tgt):
+        b.free()
+    res["peak_TFLOPs"] = MFMA_PEAK_TFLOPS["fp16"]
+    res["flops_note"] = (f"unpadded: {2 * NRC_FWD_MACS} FLOP/sample forward, "
+                         f"{2 * (NRC_FWD_MACS + NRC_BWD_MACS)} FLOP/sample forward+backward")
+    return res
+
+
 KERNEL_WORKLOADS = ("pathtrace_fbm512", "raycast_blobs512", "raycast_blobs128",
-                    "pathtrace_cloud1024", "nrc_cloud1024", "pathtrace_tets2m", "nrc_tets2m")
+                    "pathtrace_cloud1024", "nrc_cloud1024", "pathtrace_tets2m", "nrc_tets2m",
+                    "nrc_mfma")
 
 
 def kernel_only(pkg, ctx, name, steps, warmup, precision="fp16", query_vertex=2,
@@ -287,6 +342,10 @@ def kernel_only(pkg, ctx, name, steps, warmup, precision="fp16", query_vertex=2,
     """One workload's frames alone (profiling runs): `warmup` untimed frames,
     then `steps` frames timed on HIP events; prints one JSON line."""
     vrt, wls = pkg.vrt, pkg.workloads
+    if name == "nrc_mfma":
+        print(json.dumps({"kernel_workload": name, **mfma_rates(pkg, ctx, reps=steps)}),
+              flush=True)
+        return
     if name == "pathtrace_fbm512":
         wl = wls.fbm_pathtrace(n=512, phase_g=PHASE_G)
     elif name == "raycast_blobs512":
@@ -501,6 +560,9 @@ def main():
         other["tets_time_to_relmse"] = c4
         print(f"# tets: {json.dumps(c4)}", file=sys.stderr, flush=True)
 
+    mfma = mfma_rates(pkg, ctx)
+    print(f"# cache MFMA kernels: {json.dumps(mfma)}", file=sys.stderr, flush=True)
+
     out = {
         "metric": "Msamples/s at 1080p; time-to-relMSE 1e-2, NRC vs plain pathtrace",
         "value": round(value, 3),
@@ -542,6 +604,8 @@ def main():
                        "kernel runs far below it, bound by dependent fetch latency and VALU "
                        "issue of divergent paths (see traffic: HBM bytes actually moved)",
         },
+        "mfma": {"bound": "mfma", "unit": "TFLOP/s", "kernels": "k_nrc_infer, k_nrc_grad",
+                 **mfma},
         "cpu_baseline": cpu,
         "time_to_relmse": nrc,
         "time_to_relmse_vertex1": nrc_v1,
