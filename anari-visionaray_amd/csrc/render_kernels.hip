@@ -1358,8 +1358,11 @@ __device__ inline void raycast_pixel(const KScene &sc, const KCamera &cam, const
         }
       }
       hit_vol = best;
-      if (best >= 0) {
-        const KVolume &v = sc.vol[best];
+      // the march over volume `best`, addressed through a wave-uniform index
+      // so that its fields are scalar loads
+      for (int vi = 0; vi < sc.num_volumes; ++vi) {
+        if (vi != best) continue;
+        const KVolume &v = sc.vol[__builtin_amdgcn_readfirstlane(vi)];
         KRay lr = to_local(v, ray);
         BoxHit bh = intersect_box(lr, v.blo, v.bhi);
         const float tmin = fmaxf(lr.tmin, bh.tnear);
@@ -1384,7 +1387,14 @@ __device__ inline void raycast_pixel(const KScene &sc, const KCamera &cam, const
           for (int b = 0; b < kRaySteps; ++b) {
             // the same t sequence as for (t = tmin; ...; t += dt)
             in[b] = t <= tmax && step + b < max_steps;
-            has[b] = in[b] && sample_field<ANYF>(v, ori + dir * t, val[b]);
+            if constexpr (ANYF) {
+              has[b] = in[b] && sample_field<ANYF>(v, ori + dir * t, val[b]);
+            } else {
+              // structured fields: every position reads clamped, in-bounds
+              // taps, so the batch's fetches go out together, unconditionally
+              val[b] = sample_structured(v, ori + dir * t);
+              has[b] = in[b];
+            }
             tb[b] = t;
             t += dt;
           }
