@@ -386,6 +386,11 @@ struct Device : Object {
   std::vector<Object *> commit_buffer;
   uint64_t serial = 0;
   CompletionThread completions;
+  // the instance BVH of the last world rendered, for the scene version it
+  // was built at
+  vrt_tls tls = nullptr;
+  const Object *tls_world = nullptr;
+  uint64_t tls_version = 0;
 
   void report(Object *src, ANARIStatusSeverity sev, ANARIStatusCode code, const char *fmt, ...)
       __attribute__((format(printf, 5, 6))) {
@@ -420,6 +425,7 @@ struct Device : Object {
   }
   ~Device() override {
     completions.join();
+    if (tls) vrt_tls_destroy(tls);
     if (ctx) vrt_context_destroy(ctx);
   }
 };
@@ -778,6 +784,7 @@ struct SceneBuild {
   std::vector<vrt_volume> volumes;
   std::vector<vrt_light> lights;
   std::vector<vrt_surface> surfaces;
+  std::vector<vrt_instance> instances;  // surfaces under transforms, and beyond the surface slots
   uint64_t version = 0;  // newest commit among the objects used
 };
 
@@ -824,20 +831,8 @@ void add_group(Device *d, const Object *grp, const Affine &x, uint32_t inst_id, 
     if (!g) continue;
     sb.version = std::max(sb.version, g->version);
     if (m) sb.version = std::max(sb.version, m->version);
-    if (!x.identity()) {
-      d->report(o, ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_OPERATION,
-                "surfaces under a non-identity instance transform are not supported: ignored");
-      continue;
-    }
     vrt_bvh bvh = build_geometry(d, g);
     if (!bvh) continue;
-    if (sb.surfaces.size() >= VRT_MAX_SURFACES) {
-      d->report(o, ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_OPERATION,
-                "more than %d surfaces: ignored", VRT_MAX_SURFACES);
-      continue;
-    }
-    vrt_surface s{};
-    s.bvh = bvh;
     float c[4] = {1, 1, 1, 1};  // Matte.h:17
     if (m) {
       if (m->subtype != "matte")
@@ -845,10 +840,24 @@ void add_group(Device *d, const Object *grp, const Affine &x, uint32_t inst_id, 
                   "material '%s' not supported: rendered as matte", m->subtype.c_str());
       if (!m->get("color", ANARI_FLOAT32_VEC4, &c)) m->get_v3("color", c);
     }
-    for (int k = 0; k < 3; ++k) s.color[k] = c[k];
-    s.object_id = o->get_u("id", ~0u);  // Surface.cpp:21
-    s.instance_id = inst_id;
-    sb.surfaces.push_back(s);
+    if (x.identity() && sb.surfaces.size() < VRT_MAX_SURFACES) {
+      vrt_surface s{};
+      s.bvh = bvh;
+      for (int k = 0; k < 3; ++k) s.color[k] = c[k];
+      s.object_id = o->get_u("id", ~0u);  // Surface.cpp:21
+      s.instance_id = inst_id;
+      sb.surfaces.push_back(s);
+    } else {
+      // a surface under a transform (or beyond the surface slots): an
+      // instance of the world's instance BVH (VisionarayScene.cpp:69-117)
+      vrt_instance in{};
+      in.bvh = bvh;
+      std::memcpy(in.transform, x.m, sizeof(in.transform));
+      for (int k = 0; k < 3; ++k) in.color[k] = c[k];
+      in.object_id = o->get_u("id", ~0u);
+      in.instance_id = inst_id;
+      sb.instances.push_back(in);
+    }
   }
   for (Object *o : objects_of(grp, "light", ANARI_LIGHT)) {
     sb.version = std::max(sb.version, o->version);
@@ -942,6 +951,21 @@ bool assemble(Device *d, const Object *world, SceneBuild &sb) {
   std::copy(sb.lights.begin(), sb.lights.end(), sb.scene.lights);
   sb.scene.num_surfaces = int(sb.surfaces.size());
   std::copy(sb.surfaces.begin(), sb.surfaces.end(), sb.scene.surfaces);
+  sb.scene.tls = nullptr;
+  if (!sb.instances.empty()) {
+    if (!d->tls || d->tls_world != world || d->tls_version != sb.version) {
+      if (d->tls) vrt_tls_destroy(d->tls);
+      d->tls = nullptr;
+      d->tls_world = world;
+      d->tls_version = sb.version;
+      if (vrt_tls_build(d->ctx, sb.instances.data(), int(sb.instances.size()), &d->tls) !=
+          VRT_OK) {
+        d->vrt_error(const_cast<Object *>(world), "instance BVH");
+        d->tls = nullptr;
+      }
+    }
+    sb.scene.tls = d->tls;
+  }
   return true;
 }
 

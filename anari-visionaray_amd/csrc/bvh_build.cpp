@@ -1,4 +1,5 @@
-// vrt_bvh: triangle BVH for the surface hit path, replacing the reference's
+// vrt_bvh: triangle BVH for the surface hit path (and vrt_tls, the instance
+// BVH over meshes placed by transforms), replacing the reference's
 // CPU-built visionaray BVH uploaded per geometry (DeviceBVH.h:153-255, 320;
 // the reference's device traverses a binary BVH with a per-thread stack).
 //
@@ -142,6 +143,90 @@ struct Builder {
   }
 };
 
+// Collapses the binary tree into 4-wide nodes, breadth first, by
+// repeatedly opening the child of largest surface area.  A leaf child's
+// entry is the position of its first item in leaf order; `leaf_items`
+// receives the items (indices into the builder's input) in that order.
+void collapse4(const Builder &B, std::vector<float4> &nodes4, std::vector<int> &leaf_items) {
+  std::vector<int> queue = {0};
+  for (size_t q = 0; q < queue.size(); ++q) {
+    const Node2 &root = B.nodes[queue[q]];
+    int kids[4];
+    int nk = 0;
+    if (root.count) {
+      kids[nk++] = queue[q];  // a leaf root (tiny inputs): one leaf child
+    } else {
+      kids[nk++] = root.left;
+      kids[nk++] = root.right;
+      while (nk < 4) {
+        int best = -1;
+        float ba = -1.f;
+        for (int k = 0; k < nk; ++k) {
+          const Node2 &c = B.nodes[kids[k]];
+          if (!c.count && c.box.area() > ba) {
+            ba = c.box.area();
+            best = k;
+          }
+        }
+        if (best < 0) break;
+        const Node2 &c = B.nodes[kids[best]];
+        kids[best] = c.left;
+        kids[nk++] = c.right;
+      }
+    }
+    float lanes[8][4];
+    for (int k = 0; k < 4; ++k) {
+      lanes[0][k] = lanes[2][k] = lanes[4][k] = 1.f;   // empty slot: inverted box
+      lanes[1][k] = lanes[3][k] = lanes[5][k] = -1.f;
+      int32_t e = -1;
+      uint32_t cnt = 0;
+      if (k < nk) {
+        const Node2 &c = B.nodes[kids[k]];
+        for (int a = 0; a < 3; ++a) {
+          lanes[2 * a][k] = c.box.lo[a];
+          lanes[2 * a + 1][k] = c.box.hi[a];
+        }
+        if (c.count) {
+          e = int32_t(leaf_items.size());
+          cnt = uint32_t(c.count);
+          for (int i = c.first; i < c.first + c.count; ++i) leaf_items.push_back(B.order[i]);
+        } else {
+          e = int32_t(queue.size());
+          queue.push_back(kids[k]);
+        }
+      }
+      std::memcpy(&lanes[6][k], &e, 4);
+      std::memcpy(&lanes[7][k], &cnt, 4);
+    }
+    for (int r = 0; r < 8; ++r)
+      nodes4.push_back(make_float4(lanes[r][0], lanes[r][1], lanes[r][2], lanes[r][3]));
+  }
+}
+
+// The traversal stacks, at a node, every inner child it hit but the one it
+// descends into: the most entries along any root-to-leaf path, bottom-up
+// (children follow their parent in the node array)
+uint32_t stack_need(const std::vector<float4> &nodes4) {
+  const size_t nn = nodes4.size() / 8;
+  std::vector<uint32_t> need(nn, 0u);
+  for (size_t i = nn; i-- > 0;) {
+    const float4 ch = nodes4[8 * i + 6], cn = nodes4[8 * i + 7];
+    const float chv[4] = {ch.x, ch.y, ch.z, ch.w}, cnv[4] = {cn.x, cn.y, cn.z, cn.w};
+    uint32_t inner = 0, below = 0;
+    for (int k = 0; k < 4; ++k) {
+      int32_t c;
+      uint32_t cnt;
+      std::memcpy(&c, &chv[k], 4);
+      std::memcpy(&cnt, &cnv[k], 4);
+      if (c < 0 || cnt != 0u) continue;
+      ++inner;
+      below = std::max(below, need[size_t(c)]);
+    }
+    need[i] = (inner > 0 ? inner - 1 : 0) + below;
+  }
+  return nn ? need[0] : 0u;
+}
+
 }  // namespace
 
 extern "C" {
@@ -184,97 +269,21 @@ int vrt_bvh_build(vrt_context ctx, const float *vertices, int64_t num_vertices,
   B.nodes.reserve(2 * n / 4 + 16);
   B.build(0, int(n), 0);
 
-  // collapse into 4-wide nodes, breadth first
-  struct Slot { int node2; };
   std::vector<float4> nodes4;
+  std::vector<int> leaf_items;
+  collapse4(B, nodes4, leaf_items);
   std::vector<float4> tris_out;
   tris_out.reserve(3 * n);
-  std::vector<int> queue = {0};
-  std::vector<int> node4_of;  // queue position -> node4 index (same order)
-  for (size_t q = 0; q < queue.size(); ++q) {
-    const Node2 &root = B.nodes[queue[q]];
-    int kids[4];
-    int nk = 0;
-    if (root.count) {
-      kids[nk++] = queue[q];  // a leaf root (tiny meshes): one leaf child
-    } else {
-      kids[nk++] = root.left;
-      kids[nk++] = root.right;
-      while (nk < 4) {
-        int best = -1;
-        float ba = -1.f;
-        for (int k = 0; k < nk; ++k) {
-          const Node2 &c = B.nodes[kids[k]];
-          if (!c.count && c.box.area() > ba) {
-            ba = c.box.area();
-            best = k;
-          }
-        }
-        if (best < 0) break;
-        const Node2 &c = B.nodes[kids[best]];
-        kids[best] = c.left;
-        kids[nk++] = c.right;
-      }
-    }
-    float4 rec[8];
-    float lanes[8][4];
-    for (int k = 0; k < 4; ++k) {
-      lanes[0][k] = lanes[2][k] = lanes[4][k] = 1.f;   // empty slot: inverted box
-      lanes[1][k] = lanes[3][k] = lanes[5][k] = -1.f;
-      int32_t e = -1;
-      uint32_t cnt = 0;
-      if (k < nk) {
-        const Node2 &c = B.nodes[kids[k]];
-        for (int a = 0; a < 3; ++a) {
-          lanes[2 * a][k] = c.box.lo[a];
-          lanes[2 * a + 1][k] = c.box.hi[a];
-        }
-        if (c.count) {
-          e = int32_t(tris_out.size() / 3);
-          cnt = uint32_t(c.count);
-          for (int i = c.first; i < c.first + c.count; ++i) {
-            const int t = B.order[i];
-            const float *p = &tri[9 * size_t(t)];
-            uint32_t pid = uint32_t(t);
-            float pidf;
-            std::memcpy(&pidf, &pid, 4);
-            tris_out.push_back(make_float4(p[0], p[1], p[2], pidf));
-            tris_out.push_back(make_float4(p[3], p[4], p[5], 0.f));
-            tris_out.push_back(make_float4(p[6], p[7], p[8], 0.f));
-          }
-        } else {
-          e = int32_t(queue.size());
-          queue.push_back(kids[k]);
-        }
-      }
-      std::memcpy(&lanes[6][k], &e, 4);
-      std::memcpy(&lanes[7][k], &cnt, 4);
-    }
-    for (int r = 0; r < 8; ++r) rec[r] = make_float4(lanes[r][0], lanes[r][1], lanes[r][2], lanes[r][3]);
-    nodes4.insert(nodes4.end(), rec, rec + 8);
+  for (const int t : leaf_items) {
+    const float *p = &tri[9 * size_t(t)];
+    uint32_t pid = uint32_t(t);
+    float pidf;
+    std::memcpy(&pidf, &pid, 4);
+    tris_out.push_back(make_float4(p[0], p[1], p[2], pidf));
+    tris_out.push_back(make_float4(p[3], p[4], p[5], 0.f));
+    tris_out.push_back(make_float4(p[6], p[7], p[8], 0.f));
   }
-
-  // the traversal stacks, at a node, every inner child it hit but the one it
-  // descends into: the most entries along any root-to-leaf path, bottom-up
-  // (children follow their parent in the node array)
-  const size_t nn = nodes4.size() / 8;
-  std::vector<uint32_t> need(nn, 0u);
-  for (size_t i = nn; i-- > 0;) {
-    const float4 ch = nodes4[8 * i + 6], cn = nodes4[8 * i + 7];
-    const float chv[4] = {ch.x, ch.y, ch.z, ch.w}, cnv[4] = {cn.x, cn.y, cn.z, cn.w};
-    uint32_t inner = 0, below = 0;
-    for (int k = 0; k < 4; ++k) {
-      int32_t c;
-      uint32_t cnt;
-      std::memcpy(&c, &chv[k], 4);
-      std::memcpy(&cnt, &cnv[k], 4);
-      if (c < 0 || cnt != 0u) continue;
-      ++inner;
-      below = std::max(below, need[size_t(c)]);
-    }
-    need[i] = (inner > 0 ? inner - 1 : 0) + below;
-  }
-  const uint32_t stack_bound = nn ? need[0] : 0u;
+  const uint32_t stack_bound = stack_need(nodes4);
   if (stack_bound > VRT_BVH_MAX_STACK)
     return fail(VRT_ERROR_UNSUPPORTED,
                 "bvh: a traversal of this tree can stack %u entries, the kernels hold %d "
@@ -328,6 +337,143 @@ int vrt_bvh_destroy(vrt_bvh b) {
   (void)hipFree(b->nodes);
   (void)hipFree(b->tris);
   delete b;
+  return VRT_OK;
+}
+
+int vrt_instance_transforms(const float xf[12], float affine_inv[9], float trans_inv[3],
+                            float normal_xfm[9]) {
+  if (!xf || !affine_inv || !trans_inv || !normal_xfm)
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "null transform argument");
+  double L[9];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) L[3 * r + c] = double(xf[4 * r + c]);
+  for (int i = 0; i < 12; ++i)
+    if (!std::isfinite(xf[i])) return fail(VRT_ERROR_INVALID_ARGUMENT, "non-finite instance transform");
+  // inverse(top_left(xfm)) by cofactors (Instance.cpp:94)
+  const double det = L[0] * (L[4] * L[8] - L[5] * L[7]) - L[1] * (L[3] * L[8] - L[5] * L[6]) +
+                     L[2] * (L[3] * L[7] - L[4] * L[6]);
+  if (!(det != 0.0) || !std::isfinite(det))
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "singular instance transform");
+  const double inv[9] = {(L[4] * L[8] - L[5] * L[7]) / det, (L[2] * L[7] - L[1] * L[8]) / det,
+                         (L[1] * L[5] - L[2] * L[4]) / det, (L[5] * L[6] - L[3] * L[8]) / det,
+                         (L[0] * L[8] - L[2] * L[6]) / det, (L[2] * L[3] - L[0] * L[5]) / det,
+                         (L[3] * L[7] - L[4] * L[6]) / det, (L[1] * L[6] - L[0] * L[7]) / det,
+                         (L[0] * L[4] - L[1] * L[3]) / det};
+  for (int i = 0; i < 9; ++i) {
+    affine_inv[i] = float(inv[i]);
+    if (!std::isfinite(affine_inv[i]))
+      return fail(VRT_ERROR_INVALID_ARGUMENT, "instance transform not invertible in float");
+  }
+  for (int k = 0; k < 3; ++k) trans_inv[k] = -xf[4 * k + 3];  // Instance.cpp:95
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) normal_xfm[3 * r + c] = affine_inv[3 * c + r];  // :96
+  return VRT_OK;
+}
+
+int vrt_tls_build(vrt_context ctx, const vrt_instance *ins, int n, vrt_tls *out) {
+  if (int r = check_ctx(ctx)) return r;
+  if (!out || !ins || n < 1) return fail(VRT_ERROR_INVALID_ARGUMENT, "bad tls arguments");
+  if (n > (1 << 24)) return fail(VRT_ERROR_UNSUPPORTED, "too many instances (%d)", n);
+  *out = nullptr;
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<Box> boxes(static_cast<size_t>(n));
+  std::vector<float> centre(3 * size_t(n));
+  std::vector<KInst> recs(static_cast<size_t>(n));
+  for (int i = 0; i < n; ++i) {
+    const vrt_instance &I = ins[i];
+    if (!I.bvh) return fail(VRT_ERROR_INVALID_ARGUMENT, "instance %d has no bvh", i);
+    if (I.bvh->device != ctx->device)
+      return fail(VRT_ERROR_INVALID_ARGUMENT, "instance %d: bvh on another device", i);
+    KInst &k = recs[size_t(i)];
+    std::memset(&k, 0, sizeof(k));
+    k.nodes = I.bvh->nodes;
+    k.tris = I.bvh->tris;
+    if (vrt_instance_transforms(I.transform, k.A, k.tinv, k.N) != VRT_OK)
+      return fail(VRT_ERROR_INVALID_ARGUMENT, "instance %d: singular or non-finite transform", i);
+    for (int a = 0; a < 3; ++a) k.color[a] = I.color[a];
+    k.obj_id = I.object_id;
+    k.inst_id = I.instance_id;
+    k.index = uint32_t(i);
+    // world box: the mesh box's corners under [L | t] in double, rounded outwards
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int c = 0; c < 8; ++c) {
+      const double p[3] = {(c & 1) ? I.bvh->hi[0] : I.bvh->lo[0],
+                           (c & 2) ? I.bvh->hi[1] : I.bvh->lo[1],
+                           (c & 4) ? I.bvh->hi[2] : I.bvh->lo[2]};
+      for (int r = 0; r < 3; ++r) {
+        const double w = double(I.transform[4 * r]) * p[0] + double(I.transform[4 * r + 1]) * p[1] +
+                         double(I.transform[4 * r + 2]) * p[2] + double(I.transform[4 * r + 3]);
+        lo[r] = std::min(lo[r], w);
+        hi[r] = std::max(hi[r], w);
+      }
+    }
+    for (int a = 0; a < 3; ++a) {
+      boxes[size_t(i)].lo[a] = std::nextafter(float(lo[a]), -INFINITY);
+      boxes[size_t(i)].hi[a] = std::nextafter(float(hi[a]), INFINITY);
+      centre[3 * size_t(i) + a] = 0.5f * (boxes[size_t(i)].lo[a] + boxes[size_t(i)].hi[a]);
+    }
+  }
+  Builder B(boxes, centre);
+  B.order.resize(size_t(n));
+  for (int i = 0; i < n; ++i) B.order[size_t(i)] = i;
+  B.build(0, n, 0);
+  std::vector<float4> nodes4;
+  std::vector<int> leaf_items;
+  collapse4(B, nodes4, leaf_items);
+  const uint32_t stack_bound = stack_need(nodes4);
+  if (stack_bound > VRT_BVH_MAX_STACK)
+    return fail(VRT_ERROR_UNSUPPORTED, "tls: a walk of this tree can stack %u entries, the "
+                "kernels hold %d", stack_bound, VRT_BVH_MAX_STACK);
+  std::vector<KInst> leaf_recs;
+  leaf_recs.reserve(size_t(n));
+  for (const int it : leaf_items) leaf_recs.push_back(recs[size_t(it)]);
+
+  auto *h = new vrt_tls_s;
+  h->device = ctx->device;
+  h->num_nodes = uint32_t(nodes4.size() / 8);
+  h->num_inst = uint32_t(n);
+  h->stack_bound = stack_bound;
+  for (int a = 0; a < 3; ++a) {
+    h->lo[a] = B.nodes[0].box.lo[a];
+    h->hi[a] = B.nodes[0].box.hi[a];
+  }
+  hipError_t e = hipMalloc(&h->nodes, nodes4.size() * sizeof(float4));
+  if (e == hipSuccess) e = hipMalloc(&h->inst, leaf_recs.size() * sizeof(KInst));
+  if (e == hipSuccess)
+    e = hipMemcpy(h->nodes, nodes4.data(), nodes4.size() * sizeof(float4), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(h->inst, leaf_recs.data(), leaf_recs.size() * sizeof(KInst),
+                  hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(h->nodes);
+    (void)hipFree(h->inst);
+    delete h;
+    return hip_fail(e, "tls upload");
+  }
+  h->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  *out = h;
+  return VRT_OK;
+}
+
+int vrt_tls_get_info(vrt_tls t, vrt_tls_info *info) {
+  if (!t || !info) return fail(VRT_ERROR_INVALID_ARGUMENT, "null argument");
+  info->num_instances = t->num_inst;
+  info->num_nodes = t->num_nodes;
+  info->stack_bound = t->stack_bound;
+  for (int a = 0; a < 3; ++a) {
+    info->bounds_lo[a] = t->lo[a];
+    info->bounds_hi[a] = t->hi[a];
+  }
+  info->build_ms = t->build_ms;
+  return VRT_OK;
+}
+
+int vrt_tls_destroy(vrt_tls t) {
+  if (!t) return VRT_OK;
+  (void)hipSetDevice(t->device);
+  (void)hipFree(t->nodes);
+  (void)hipFree(t->inst);
+  delete t;
   return VRT_OK;
 }
 

@@ -109,6 +109,28 @@ struct KSurface {
   uint32_t obj_id, inst_id;
 };
 
+// An instanced mesh (vrt_instance) on the device
+struct KInst {
+  const float4 *nodes;  // the mesh's BVH (KSurface layout)
+  const float4 *tris;
+  float A[9];           // affineInv, row-major: object = A (world + tinv)
+  float tinv[3];
+  float N[9];           // normal transform transpose(A), row-major
+  float color[3];       // matte
+  uint32_t obj_id, inst_id;
+  uint32_t index;       // position in the array given to vrt_tls_build
+  uint32_t pad;
+};
+
+// The instance-level BVH: 4-wide nodes as KSurface's, whose leaves index
+// `inst` (instances in leaf order)
+struct KTls {
+  const float4 *nodes;
+  const KInst *inst;
+  int num_inst;         // 0: no TLS
+  int pad;
+};
+
 struct KScene {
   int num_volumes;
   int num_lights;
@@ -118,6 +140,7 @@ struct KScene {
   KVolume vol[VRT_MAX_VOLUMES];
   KLight light[VRT_MAX_LIGHTS];
   KSurface surf[VRT_MAX_SURFACES];
+  KTls tls;
   int hdri_light;  // index of the HDRI light, -1 for none
   KHdri hdri;
 };

@@ -57,7 +57,7 @@ hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 hipError_t launch_sample_field(const KVolume &v, int n, const float *pts, float *out,
                                hipStream_t s);
 hipError_t launch_intersect_surfaces(const KScene &sc, int n, const float *rays, int any,
-                                     float *t_out, int *prim_out, hipStream_t s);
+                                     float *t_out, int *prim_out, int *surf_out, hipStream_t s);
 hipError_t launch_delta_track(const KScene &sc, int n, const float *rays, uint32_t *rng,
                               float *tout, hipStream_t s);
 

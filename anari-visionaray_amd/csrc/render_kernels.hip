@@ -639,6 +639,151 @@ __device__ inline bool bvh_hit(const KSurface &S, const KRay &r, bool any, float
   }
 }
 
+// intersect(Ray, TLS) over instanced meshes (DeviceCopyableObjects.h:
+// 1772-1824): the world ray walks the instance BVH (its stack, the TLS's
+// bound <= kBvhStack, in private memory; the LDS stack top `lst` belongs to
+// the mesh walks) and, per instance whose box it meets, meets the mesh in
+// object space, ori' = A (ori + tinv), dir' = A dir, so t is a world
+// distance.  A hit at t <= tbest counts when nothing was hit before
+// (`found_before`: a surface hit at tbest keeps ties), else when strictly
+// closer, or equally close on an instance of lower index.  Returns whether
+// an instance was hit; `slot` is its position in T.inst.
+__device__ inline bool tls_hit(const KTls &T, const KRay &r, bool any, bool found_before,
+                               float &tbest, int &slot, int &prim, uint32_t &rec, int2 *lst) {
+  f3 inv;
+  {
+    const float dx = fabsf(r.dir.x) < 1e-20f ? copysignf(1e-20f, r.dir.x) : r.dir.x;
+    const float dy = fabsf(r.dir.y) < 1e-20f ? copysignf(1e-20f, r.dir.y) : r.dir.y;
+    const float dz = fabsf(r.dir.z) < 1e-20f ? copysignf(1e-20f, r.dir.z) : r.dir.z;
+    inv = mk(1.f / dx, 1.f / dy, 1.f / dz);
+  }
+  int2 stk[kBvhStack];
+  int sp = 0;
+  int node = 0;
+  bool found = false;
+  uint32_t best_index = 0xFFFFFFFFu;
+  const float inf = __builtin_inff();
+  while (true) {
+    const float4 *n = T.nodes + size_t(node) * 8u;
+    const float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5];
+    const float4 chf = n[6], cnf = n[7];
+    const float lo_x[4] = {lx.x, lx.y, lx.z, lx.w}, hi_x[4] = {hx.x, hx.y, hx.z, hx.w};
+    const float lo_y[4] = {ly.x, ly.y, ly.z, ly.w}, hi_y[4] = {hy.x, hy.y, hy.z, hy.w};
+    const float lo_z[4] = {lz.x, lz.y, lz.z, lz.w}, hi_z[4] = {hz.x, hz.y, hz.z, hz.w};
+    const int child[4] = {__float_as_int(chf.x), __float_as_int(chf.y), __float_as_int(chf.z),
+                          __float_as_int(chf.w)};
+    const uint32_t count[4] = {__float_as_uint(cnf.x), __float_as_uint(cnf.y),
+                               __float_as_uint(cnf.z), __float_as_uint(cnf.w)};
+    float ct[4];
+    int cn[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      ct[c] = inf;
+      cn[c] = child[c];
+      if (child[c] < 0) continue;
+      const float tx0 = (lo_x[c] - r.ori.x) * inv.x, tx1 = (hi_x[c] - r.ori.x) * inv.x;
+      const float ty0 = (lo_y[c] - r.ori.y) * inv.y, ty1 = (hi_y[c] - r.ori.y) * inv.y;
+      const float tz0 = (lo_z[c] - r.ori.z) * inv.z, tz1 = (hi_z[c] - r.ori.z) * inv.z;
+      const float tn = fmaxf(r.tmin, fmaxf(fminf(tx0, tx1), fmaxf(fminf(ty0, ty1), fminf(tz0, tz1))));
+      const float tf = fminf(tbest, fminf(fmaxf(tx0, tx1), fminf(fmaxf(ty0, ty1), fmaxf(tz0, tz1))));
+      if (!(tn <= tf * 1.0000002f + 1e-30f)) continue;
+      if (count[c] == 0u) {
+        ct[c] = tn;
+        continue;
+      }
+      for (uint32_t k = 0; k < count[c]; ++k) {
+        const KInst &I = T.inst[size_t(child[c]) + k];
+        KRay lr;
+        lr.ori = mulA(I.A, r.ori + mk(I.tinv[0], I.tinv[1], I.tinv[2]));
+        lr.dir = mulA(I.A, r.dir);
+        lr.tmin = r.tmin;
+        lr.tmax = r.tmax;
+        KSurface S;
+        S.nodes = I.nodes;
+        S.tris = I.tris;
+        float t = tbest;
+        int p = -1;
+        uint32_t rc = 0;
+        if (!bvh_hit(S, lr, any, t, p, rc, lst)) continue;
+        if ((found_before || found) && !(t < tbest || (found && t == tbest && I.index < best_index)))
+          continue;
+        found = true;
+        tbest = t;
+        slot = child[c] + int(k);
+        prim = p;
+        rec = rc;
+        best_index = I.index;
+        if (any) return true;
+      }
+    }
+    cswap(ct[0], cn[0], ct[1], cn[1]);
+    cswap(ct[2], cn[2], ct[3], cn[3]);
+    cswap(ct[0], cn[0], ct[2], cn[2]);
+    cswap(ct[1], cn[1], ct[3], cn[3]);
+    cswap(ct[1], cn[1], ct[2], cn[2]);
+#pragma unroll
+    for (int c = 3; c >= 1; --c) {
+      if (!(ct[c] < inf) || sp >= kBvhStack) continue;
+      stk[sp++] = make_int2(cn[c], __float_as_int(ct[c]));
+    }
+    const float lim = tbest * 1.0000002f + 1e-30f;
+    if (ct[0] < inf && ct[0] <= lim) {
+      node = cn[0];
+      continue;
+    }
+    node = -1;
+    while (sp > 0) {
+      const int2 e = stk[--sp];
+      if (__int_as_float(e.y) <= tbest * 1.0000002f + 1e-30f) {
+        node = e.x;
+        break;
+      }
+    }
+    if (node < 0) return found;
+  }
+}
+
+// The matte triangle hit by a pass: surface `id` (< num_surfaces) or
+// instance id - num_surfaces of the TLS, with the instance's normal
+// transform (identity: nxfm null)
+struct SurfRef {
+  const float4 *tris;
+  f3 color;
+  uint32_t obj_id, inst_id;
+  const float *nxfm;
+};
+
+__device__ inline SurfRef surface_ref(const KScene &sc, int id) {
+  SurfRef R;
+  if (id < sc.num_surfaces) {
+    KSurface S = sc.surf[0];
+    for (int i = 1; i < sc.num_surfaces; ++i)
+      if (i == id) S = sc.surf[i];
+    R.tris = S.tris;
+    R.color = S.color;
+    R.obj_id = S.obj_id;
+    R.inst_id = S.inst_id;
+    R.nxfm = nullptr;
+  } else {
+    const KInst &I = sc.tls.inst[id - sc.num_surfaces];
+    R.tris = I.tris;
+    R.color = mk(I.color[0], I.color[1], I.color[2]);
+    R.obj_id = I.obj_id;
+    R.inst_id = I.inst_id;
+    R.nxfm = I.N;
+  }
+  return R;
+}
+
+// the geometric normal of triangle record `rec` in world space:
+// normalize(nxfm * normalize(e1 x e2)) (getNormals, common.h:233-264, then
+// Pathtrace_impl.cpp:243-247; identity: normalised twice)
+__device__ inline f3 surface_normal(const SurfRef &R, uint32_t rec) {
+  const float4 e1 = R.tris[3 * size_t(rec) + 1], e2 = R.tris[3 * size_t(rec) + 2];
+  const f3 n = normalize(cross(mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z)));
+  return normalize(R.nxfm ? mulA(R.nxfm, n) : n);
+}
+
 // intersect(Ray, Quad) (DeviceCopyableObjects.h:2209-2230): the closer of
 // the two triangles, t > 0, no [tmin, tmax] test (intersectLights)
 __device__ inline bool quad_hit(const KLight &L, const KRay &r, float &t) {
@@ -1266,7 +1411,7 @@ __device__ inline void gbuf_miss(const KFrame &F, size_t i) {
 // One pixel of the ray caster (Raycast_impl.cpp).  SURF: the scene has
 // surfaces (the BVH walk and surface shading are compiled in, with the
 // traversal stack's top in LDS after the tables)
-template <bool ANYF, bool SHADE, bool SURF>
+template <bool ANYF, bool SHADE, bool SURF, bool INST>
 __device__ inline void raycast_pixel(const KScene &sc, const KCamera &cam, const KParams &P,
                                      const KFrame &F, int x, int y, float4 *s_tf,
                                      const float2 *s_pow, uint32_t &lookups) {
@@ -1282,7 +1427,7 @@ __device__ inline void raycast_pixel(const KScene &sc, const KCamera &cam, const
     // path-tracer stream: the reference's ray caster leaves its generator
     // unseeded
     uint32_t rs = (SHADE || SURF) ? tea4(uint32_t(y * F.width + x), 0u) : 0u;
-    if (sc.num_volumes > 0 || sc.num_surfaces > 0) {  // an empty scene shows the background
+    if (sc.num_volumes > 0 || sc.num_surfaces > 0 || sc.tls.num_inst > 0) {  // an empty scene shows the background
       KRay ray = primary_ray(cam, float(x), float(y), float(F.width), float(F.height));
       clip_ray(ray, P);
       float4 surf = make_float4(0.f, 0.f, 0.f, 0.f);  // surfaceColor, surfaceAlpha
@@ -1307,15 +1452,23 @@ __device__ inline void raycast_pixel(const KScene &sc, const KCamera &cam, const
             prim = p;
           }
         }
+        if (INST && sc.tls.num_inst > 0) {
+          float t = st;
+          int slot = -1, p = -1;
+          uint32_t rec = 0;
+          if (tls_hit(sc.tls, ray, false, sid >= 0, t, slot, p, rec, bstack)) {
+            st = t;
+            sid = sc.num_surfaces + slot;
+            srec = rec;
+            prim = p;
+          }
+        }
         if (sid >= 0) {
-          KSurface S = sc.surf[0];
-          for (int i = 1; i < sc.num_surfaces; ++i)
-            if (i == sid) S = sc.surf[i];
+          const SurfRef S = surface_ref(sc, sid);
           const float4 v1 = S.tris[3 * size_t(srec)];
-          const float4 e1 = S.tris[3 * size_t(srec) + 1], e2 = S.tris[3 * size_t(srec) + 2];
           const f3 hitPos = ray.ori + ray.dir * st;
-          // getNormals, then normalize(nxfm * n) under the identity instance
-          const f3 sn = normalize(normalize(cross(mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z))));
+          // getNormals, then normalize(nxfm * n) under the instance
+          const f3 sn = surface_normal(S, srec);
           const float inv_pi = 0.318309886183790671538f;
           f3 shaded = mk(0.f);
           for (int i = 0; i < sc.num_lights; ++i) {
@@ -1448,7 +1601,7 @@ __device__ inline void raycast_pixel(const KScene &sc, const KCamera &cam, const
 // no tail of half-empty CUs.  The row-major tile sequence is cut into 8 contiguous
 // bands; band q is queue q, drained first by the workgroups the hardware
 // placed on XCD q (blockIdx % 8), which then help the other bands.
-template <bool ANYF, bool SHADE, bool SURF>
+template <bool ANYF, bool SHADE, bool SURF, bool INST>
 __global__ void __launch_bounds__(256)
     k_raycast(KScene sc, KCamera cam, KParams P, KFrame F, uint32_t *queues,
               uint32_t *queues_next) {
@@ -1477,7 +1630,7 @@ __global__ void __launch_bounds__(256)
     }
     const int tile = int(lo + k);
     const int x = (tile % ntx) * 8 + (lane & 7), y = (tile / ntx) * 8 + (lane >> 3);
-    raycast_pixel<ANYF, SHADE, SURF>(sc, cam, P, F, x, y, s_tf, s_pow, lookups);
+    raycast_pixel<ANYF, SHADE, SURF, INST>(sc, cam, P, F, x, y, s_tf, s_pow, lookups);
   }
   if (F.counters) wave_add_counter(&F.counters[0], lookups);
 }
@@ -1620,7 +1773,7 @@ struct PassHit {
 // AREA: the scene has surfaces or quad lights (else the kernel is built
 // without the BVH and quad code, which would only add registers); ANYF: it
 // has fields other than structuredRegular
-template <bool AREA, bool ANYF>
+template <bool AREA, bool ANYF, bool INST>
 __device__ inline PassHit intersect_all(const KScene &sc, const KRay &ray, bool shadow,
                                         int bounceID, const float4 *s_tf, const float *steps,
                                         uint32_t &srng, uint32_t &lookups, int2 *bstack,
@@ -1637,6 +1790,17 @@ __device__ inline PassHit intersect_all(const KScene &sc, const KRay &ray, bool 
       sh = true;
       st = t;
       sid = i;
+      srec = rec;
+    }
+  }
+  if (AREA && INST && sc.tls.num_inst > 0 && !(shadow && sh)) {
+    float t = st;
+    int slot = -1, prim = -1;
+    uint32_t rec = 0;
+    if (tls_hit(sc.tls, ray, shadow, sh, t, slot, prim, rec, bstack)) {
+      sh = true;
+      st = t;
+      sid = sc.num_surfaces + slot;
       srec = rec;
     }
   }
@@ -1914,19 +2078,16 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
       if (st.taa_take) taa_vertex(F, P, st, hitPos, color);
     } else if (AREA) {
       // triangle: Ng = normalize(e1 x e2), no vertex normals (getNormals,
-      // common.h:233-264), normalised again after the (identity) normal
+      // common.h:233-264), normalised again after the instance's normal
       // transform (Pathtrace_impl.cpp:243-247); matte color
-      KSurface S = sc.surf[0];
-      for (int i = 1; i < sc.num_surfaces; ++i)
-        if (i == h.id) S = sc.surf[i];
-      const float4 v1 = S.tris[3 * size_t(h.rec)];
-      const float4 e1 = S.tris[3 * size_t(h.rec) + 1], e2 = S.tris[3 * size_t(h.rec) + 2];
-      gn = normalize(normalize(cross(mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z))));
+      const SurfRef R = surface_ref(sc, h.id);
+      const float4 v1 = R.tris[3 * size_t(h.rec)];
+      gn = surface_normal(R, h.rec);
       st.sn = faceforward(gn, viewDir, gn);
       gn = faceforward(gn, viewDir, gn);
-      color = S.color;
+      color = R.color;
       if (gbuf)
-        gbuf_write(F, size_t(st.pixel), gn, color, __float_as_uint(v1.w), S.obj_id, S.inst_id);
+        gbuf_write(F, size_t(st.pixel), gn, color, __float_as_uint(v1.w), R.obj_id, R.inst_id);
       if (st.taa_take) taa_vertex(F, P, st, hitPos, color);
     }
     LightSmp ls;
@@ -2059,7 +2220,7 @@ __device__ inline float train_hash(uint32_t pixel, uint32_t accumID) {
 // 3 waves per SIMD (<= 168 VGPRs): the variants with surfaces or other
 // field types would otherwise settle at 2 and lose a third of their latency
 // hiding; the compiler fits them without spilling
-template <bool NRC, bool AREA, bool ANYF, bool GRAD>
+template <bool NRC, bool AREA, bool ANYF, bool GRAD, bool INST>
 __global__ void __launch_bounds__(256, 3)
     k_pathtrace(KScene sc, KCamera cam, KParams P, KFrame F, int accumID, uint32_t *queues,
                 uint32_t *queues_next, KNrc N) {
@@ -2153,7 +2314,7 @@ __global__ void __launch_bounds__(256, 3)
       ray.tmax = st.tmax;
       clip_ray(ray, P);  // every pass (Pathtrace_impl.cpp:559)
       const bool shadow = st.rayType != Radiance;
-      const PassHit hit = intersect_all<AREA, ANYF>(sc, ray, shadow, st.bounceID, s_tf, steps,
+      const PassHit hit = intersect_all<AREA, ANYF, INST>(sc, ray, shadow, st.bounceID, s_tf, steps,
                                                     st.srng, lookups, bstack,
                                                     shadow && P.ratio_tracking != 0);
       if (path_shade<NRC, AREA, ANYF, GRAD>(st, sc, P, N, F, hit)) {
@@ -2599,7 +2760,8 @@ __global__ void __launch_bounds__(256)
 // miss) and the triangle's index in its mesh (-1)
 __global__ void __launch_bounds__(256)
     k_intersect_surfaces(KScene sc, int n, const float *__restrict__ rays, int any,
-                         float *__restrict__ t_out, int *__restrict__ prim_out) {
+                         float *__restrict__ t_out, int *__restrict__ prim_out,
+                         int *__restrict__ surf_out) {
   extern __shared__ int2 s_bstack[];
   const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
   if (i >= n) return;
@@ -2611,7 +2773,7 @@ __global__ void __launch_bounds__(256)
   ray.tmax = r[7];
   bool hit = false;
   float best = ray.tmax;
-  int prim = -1;
+  int prim = -1, surf = -1;
   for (int s = 0; s < sc.num_surfaces; ++s) {
     float t = best;
     int p = -1;
@@ -2620,10 +2782,23 @@ __global__ void __launch_bounds__(256)
       hit = true;
       best = t;
       prim = p;
+      surf = s;
+    }
+  }
+  if (sc.tls.num_inst > 0 && !(any && hit)) {
+    float t = best;
+    int slot = -1, p = -1;
+    uint32_t rec;
+    if (tls_hit(sc.tls, ray, any != 0, hit, t, slot, p, rec, s_bstack + threadIdx.x)) {
+      hit = true;
+      best = t;
+      prim = p;
+      surf = sc.num_surfaces + int(sc.tls.inst[slot].index);
     }
   }
   t_out[i] = hit ? best : __builtin_inff();
   prim_out[i] = hit ? prim : -1;
+  if (surf_out) surf_out[i] = hit ? surf : -1;
 }
 
 __global__ void k_frame_reset(KFrame F) {
@@ -2650,16 +2825,21 @@ static bool scene_has_other_fields(const KScene &sc);
 
 hipError_t launch_raycast(const KScene &sc, const KCamera &cam, const KParams &P, const KFrame &F,
                           uint32_t *queues, uint32_t *queues_next, int num_cus, hipStream_t s) {
-  const bool surf = sc.num_surfaces > 0;
+  const bool surf = sc.num_surfaces > 0 || sc.tls.num_inst > 0;
   const size_t lds = scene_lds_bytes(sc) + (surf ? bvh_lds_stack_bytes() : 0);
   const bool any = scene_has_other_fields(sc), shade = P.gradient_shading != 0;
   using RaycastKernel = void (*)(KScene, KCamera, KParams, KFrame, uint32_t *, uint32_t *);
-  static const RaycastKernel table[8] = {
-      k_raycast<false, false, false>, k_raycast<true, false, false>,
-      k_raycast<false, true, false>,  k_raycast<true, true, false>,
-      k_raycast<false, false, true>,  k_raycast<true, false, true>,
-      k_raycast<false, true, true>,   k_raycast<true, true, true>};
-  const RaycastKernel k = table[(surf ? 4 : 0) + (shade ? 2 : 0) + (any ? 1 : 0)];
+  static const RaycastKernel table[12] = {
+      k_raycast<false, false, false, false>, k_raycast<true, false, false, false>,
+      k_raycast<false, true, false, false>,  k_raycast<true, true, false, false>,
+      k_raycast<false, false, true, false>,  k_raycast<true, false, true, false>,
+      k_raycast<false, true, true, false>,   k_raycast<true, true, true, false>,
+      k_raycast<false, false, true, true>,   k_raycast<true, false, true, true>,
+      k_raycast<false, true, true, true>,    k_raycast<true, true, true, true>};
+  // instanced meshes take the variants with the instance walk compiled in
+  const bool inst = sc.tls.num_inst > 0;
+  const RaycastKernel k =
+      table[(inst ? 8 : surf ? 4 : 0) + (shade ? 2 : 0) + (any ? 1 : 0)];
   // one resident wave per SIMD slot, no more workgroups than tiles need
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, lds) != hipSuccess ||
@@ -2674,7 +2854,7 @@ hipError_t launch_raycast(const KScene &sc, const KCamera &cam, const KParams &P
 
 // scenes with surfaces or quad lights take the kernel with the BVH code
 static bool scene_has_area(const KScene &sc) {
-  if (sc.num_surfaces > 0) return true;
+  if (sc.num_surfaces > 0 || sc.tls.num_inst > 0) return true;
   for (int i = 0; i < sc.num_lights; ++i)
     if (sc.light[i].type == VRT_LIGHT_QUAD || sc.light[i].type == VRT_LIGHT_HDRI) return true;
   return false;
@@ -2693,14 +2873,22 @@ using PathtraceKernel = void (*)(KScene, KCamera, KParams, KFrame, int, uint32_t
 
 static PathtraceKernel pathtrace_kernel(const KScene &sc, bool nrc, bool grad) {
   static const PathtraceKernel table[16] = {
-      k_pathtrace<false, false, false, false>, k_pathtrace<false, false, true, false>,
-      k_pathtrace<false, true, false, false>,  k_pathtrace<false, true, true, false>,
-      k_pathtrace<true, false, false, false>,  k_pathtrace<true, false, true, false>,
-      k_pathtrace<true, true, false, false>,   k_pathtrace<true, true, true, false>,
-      k_pathtrace<false, false, false, true>,  k_pathtrace<false, false, true, true>,
-      k_pathtrace<false, true, false, true>,   k_pathtrace<false, true, true, true>,
-      k_pathtrace<true, false, false, true>,   k_pathtrace<true, false, true, true>,
-      k_pathtrace<true, true, false, true>,    k_pathtrace<true, true, true, true>};
+      k_pathtrace<false, false, false, false, false>, k_pathtrace<false, false, true, false, false>,
+      k_pathtrace<false, true, false, false, false>,  k_pathtrace<false, true, true, false, false>,
+      k_pathtrace<true, false, false, false, false>,  k_pathtrace<true, false, true, false, false>,
+      k_pathtrace<true, true, false, false, false>,   k_pathtrace<true, true, true, false, false>,
+      k_pathtrace<false, false, false, true, false>,  k_pathtrace<false, false, true, true, false>,
+      k_pathtrace<false, true, false, true, false>,   k_pathtrace<false, true, true, true, false>,
+      k_pathtrace<true, false, false, true, false>,   k_pathtrace<true, false, true, true, false>,
+      k_pathtrace<true, true, false, true, false>,    k_pathtrace<true, true, true, true, false>};
+  // instanced meshes: the surface variants with the instance walk compiled in
+  static const PathtraceKernel inst_table[8] = {
+      k_pathtrace<false, true, false, false, true>, k_pathtrace<false, true, true, false, true>,
+      k_pathtrace<true, true, false, false, true>,  k_pathtrace<true, true, true, false, true>,
+      k_pathtrace<false, true, false, true, true>,  k_pathtrace<false, true, true, true, true>,
+      k_pathtrace<true, true, false, true, true>,   k_pathtrace<true, true, true, true, true>};
+  if (sc.tls.num_inst > 0)
+    return inst_table[(grad ? 4 : 0) + (nrc ? 2 : 0) + (scene_has_other_fields(sc) ? 1 : 0)];
   return table[(grad ? 8 : 0) + (nrc ? 4 : 0) + (scene_has_area(sc) ? 2 : 0) +
                (scene_has_other_fields(sc) ? 1 : 0)];
 }
@@ -2847,10 +3035,10 @@ hipError_t launch_sample_field(const KVolume &v, int n, const float *pts, float 
 }
 
 hipError_t launch_intersect_surfaces(const KScene &sc, int n, const float *rays, int any,
-                                     float *t_out, int *prim_out, hipStream_t s) {
+                                     float *t_out, int *prim_out, int *surf_out, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_intersect_surfaces, dim3((n + 255) / 256), dim3(kBvhLanes),
-                     bvh_lds_stack_bytes(), s, sc, n, rays, any, t_out, prim_out);
+                     bvh_lds_stack_bytes(), s, sc, n, rays, any, t_out, prim_out, surf_out);
   return hipGetLastError();
 }
 

@@ -258,6 +258,11 @@ int make_scene(const vrt_scene *s, KScene &k) {
     k.surf[i].obj_id = S.object_id;
     k.surf[i].inst_id = S.instance_id;
   }
+  if (s->tls) {
+    k.tls.nodes = s->tls->nodes;
+    k.tls.inst = s->tls->inst;
+    k.tls.num_inst = int(s->tls->num_inst);
+  }
   return VRT_OK;
 }
 
@@ -604,14 +609,14 @@ int vrt_sample_field(vrt_context ctx, const vrt_volume *vol, int n, const float 
 }
 
 int vrt_intersect_surfaces(vrt_context ctx, const vrt_scene *scene, int n, const float *rays,
-                           int any, float *t_out, int *prim_out) {
+                           int any, float *t_out, int *prim_out, int *surf_out) {
   if (int r = check_ctx(ctx)) return r;
   if (n < 0 || (n > 0 && (!rays || !t_out || !prim_out)))
     return fail(VRT_ERROR_INVALID_ARGUMENT, "bad intersect_surfaces arguments");
   KScene sc;
   if (int r = make_scene(scene, sc)) return r;
   if (n == 0) return VRT_OK;
-  VRT_HIP(launch_intersect_surfaces(sc, n, rays, any, t_out, prim_out, ctx->stream));
+  VRT_HIP(launch_intersect_surfaces(sc, n, rays, any, t_out, prim_out, surf_out, ctx->stream));
   return VRT_OK;
 }
 

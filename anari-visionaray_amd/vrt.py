@@ -70,11 +70,17 @@ NO_ID = 0xFFFFFFFF  # an unset ANARI "id"
 MAX_SURFACES = 4
 
 
+class Instance(C.Structure):
+    _fields_ = [("bvh", C.c_void_p), ("transform", C.c_float * 12), ("color", f3),
+                ("object_id", C.c_uint32), ("instance_id", C.c_uint32)]
+
+
 class Scene(C.Structure):
     _fields_ = [
         ("num_volumes", C.c_int), ("volumes", Volume * MAX_VOLUMES),
         ("num_lights", C.c_int), ("lights", Light * MAX_LIGHTS),
         ("num_surfaces", C.c_int), ("surfaces", Surface * MAX_SURFACES),
+        ("tls", C.c_void_p),
     ]
 
 
@@ -82,6 +88,12 @@ class BvhInfo(C.Structure):
     _fields_ = [("num_nodes", C.c_uint32), ("num_triangles", C.c_uint32), ("depth", C.c_uint32),
                 ("bounds_lo", f3), ("bounds_hi", f3), ("build_ms", C.c_double),
                 ("device_bytes", C.c_uint64), ("stack_bound", C.c_uint32)]
+
+
+class TlsInfo(C.Structure):
+    _fields_ = [("num_instances", C.c_uint32), ("num_nodes", C.c_uint32),
+                ("stack_bound", C.c_uint32), ("bounds_lo", f3), ("bounds_hi", f3),
+                ("build_ms", C.c_double)]
 
 
 class Camera(C.Structure):
@@ -189,7 +201,11 @@ def lib():
                                               C.c_int64, C.POINTER(C.c_void_p)]
         L.vrt_unstructured_get_info.argtypes = [vp, C.c_void_p]
         L.vrt_unstructured_destroy.argtypes = [vp]
-        L.vrt_intersect_surfaces.argtypes = [vp, C.POINTER(Scene), ip, vp, ip, vp, vp]
+        L.vrt_intersect_surfaces.argtypes = [vp, C.POINTER(Scene), ip, vp, ip, vp, vp, vp]
+        L.vrt_instance_transforms.argtypes = [vp, vp, vp, vp]
+        L.vrt_tls_build.argtypes = [vp, vp, ip, C.POINTER(C.c_void_p)]
+        L.vrt_tls_get_info.argtypes = [vp, C.c_void_p]
+        L.vrt_tls_destroy.argtypes = [vp]
         L.vrt_bvh_build.argtypes = [vp, vp, C.c_int64, vp, C.c_int64, C.POINTER(C.c_void_p)]
         L.vrt_bvh_build_gpu.argtypes = [vp, vp, C.c_int64, vp, C.c_int64, C.POINTER(C.c_void_p)]
         L.vrt_bvh_get_info.argtypes = [vp, C.c_void_p]
@@ -581,9 +597,11 @@ def affine_inverse(xfm):
     return A, (-m[:, 3]).astype(np.float32)
 
 
-def make_scene(volumes, lights=(), surfaces=()):
-    """`surfaces`: (DeviceMesh, matte color) pairs."""
+def make_scene(volumes, lights=(), surfaces=(), tls=None):
+    """`surfaces`: (DeviceMesh, matte color) pairs; `tls`: a DeviceTls of
+    instanced meshes."""
     s = Scene()
+    s.tls = tls.handle if tls is not None else None
     s.num_volumes = len(volumes)
     for i, v in enumerate(volumes):
         s.volumes[i] = v.desc
@@ -633,6 +651,48 @@ class DeviceMesh:
     def close(self):
         if self.handle:
             lib().vrt_bvh_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def instance_transforms(transform):
+    """(affine_inv, trans_inv, normal_xfm) of a row-major 3x4 [L | t] as the
+    library derives them (vrt_instance_transforms)."""
+    x = np.ascontiguousarray(transform, np.float32).reshape(12)
+    a, t, n = np.empty(9, np.float32), np.empty(3, np.float32), np.empty(9, np.float32)
+    check(lib().vrt_instance_transforms(x.ctypes.data, a.ctypes.data, t.ctypes.data,
+                                        n.ctypes.data))
+    return a.reshape(3, 3), t, n.reshape(3, 3)
+
+
+class DeviceTls:
+    """Instance-level BVH over meshes placed by transforms: `instances` are
+    (DeviceMesh, row-major 3x4 transform, matte color[, object id, instance
+    id]) tuples; the meshes must outlive it."""
+
+    def __init__(self, ctx, instances):
+        arr = (Instance * len(instances))()
+        for i, inst in enumerate(instances):
+            mesh, xf, color = inst[:3]
+            arr[i].bvh = mesh.handle
+            arr[i].transform = (C.c_float * 12)(*np.asarray(xf, np.float32).reshape(12))
+            arr[i].color = f3(*color)
+            arr[i].object_id = inst[3] if len(inst) > 3 else NO_ID
+            arr[i].instance_id = inst[4] if len(inst) > 4 else NO_ID
+        h = C.c_void_p()
+        check(lib().vrt_tls_build(ctx.handle, arr, len(instances), C.byref(h)))
+        self.handle = h.value
+        self.info = TlsInfo()
+        check(lib().vrt_tls_get_info(self.handle, C.byref(self.info)))
+
+    def close(self):
+        if self.handle:
+            lib().vrt_tls_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
@@ -837,19 +897,22 @@ def sample_field(ctx, volume, points):
     return out
 
 
-def intersect_surfaces(ctx, scene, rays, any_hit=False):
-    """Surface hits of rays (n, 8): {ori, dir, tmin, tmax} -> (t, prim)."""
+def intersect_surfaces(ctx, scene, rays, any_hit=False, which=False):
+    """Surface hits of rays (n, 8): {ori, dir, tmin, tmax} -> (t, prim), and
+    with which=True also the surface hit (surface i, or num_surfaces + j for
+    instance j of the scene's TLS; -1 on a miss)."""
     r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
     n = len(r)
     dr = ctx.upload(r)
-    dt, dp = ctx.buffer(max(4, 4 * n)), ctx.buffer(max(4, 4 * n))
+    dt, dp, ds = ctx.buffer(max(4, 4 * n)), ctx.buffer(max(4, 4 * n)), ctx.buffer(max(4, 4 * n))
     check(lib().vrt_intersect_surfaces(ctx.handle, C.byref(scene), n, dr.ptr, int(any_hit),
-                                       dt.ptr, dp.ptr))
+                                       dt.ptr, dp.ptr, ds.ptr if which else None))
     t = dt.download(np.float32, (n,))
     p = dp.download(np.int32, (n,))
-    for b in (dr, dt, dp):
+    sf = ds.download(np.int32, (n,)) if which else None
+    for b in (dr, dt, dp, ds):
         b.free()
-    return t, p
+    return (t, p, sf) if which else (t, p)
 
 
 def delta_track(ctx, scene, rays, rng_states):

@@ -50,6 +50,8 @@ class Workload:
     meshes: list = None          # dicts: vertices (n,3), indices (m,3) or None, color
     tets: dict = None            # unstructured field: positions (V,3), index (T,4),
                                  # vertex_data (V,) [, cell_index, cell_type, cell_data]
+    instances: list = None       # instanced meshes: dicts as `meshes` plus transform
+                                 # (row-major 3x4 [L | t]) [, object_id, instance_id]
 
     @property
     def dims(self):
@@ -172,6 +174,58 @@ def mixed_pathtrace(level=8, vol_n=256, width=1920, height=1080, frames=128, max
         meshes=[dict(vertices=v, indices=i, color=(0.7, 0.7, 0.7))])
 
 
+def instance_transform(axis, angle, scale, translate):
+    """Row-major 3x4 [L | t] with L = R(axis, angle) diag(scale)."""
+    a = np.asarray(axis, np.float64)
+    a = a / np.linalg.norm(a)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    R = np.eye(3) + math.sin(angle) * K + (1 - math.cos(angle)) * (K @ K)
+    L = R @ np.diag(np.asarray(scale, np.float64))
+    return np.concatenate([L, np.asarray(translate, np.float64).reshape(3, 1)], 1).astype(np.float32)
+
+
+def instanced_meshes(count=6, level=3, seed=4, spread=0.55):
+    """`count` instances of one noise-displaced icosphere (20 * 4^level
+    triangles): random axes and angles, non-uniform scales 0.08-0.2, placed
+    on a ring of radius `spread`; instance ids 100 + i, object id 7."""
+    v, i = _vrt.gen_icosphere(level=level, radius=1.0, amp=0.1, freq=3.0, seed=seed)
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        ang = 2 * math.pi * k / count
+        t = (spread * math.cos(ang), rng.uniform(-0.15, 0.15), spread * math.sin(ang))
+        xf = instance_transform(rng.normal(size=3), rng.uniform(0, 2 * math.pi),
+                                rng.uniform(0.08, 0.2, 3), t)
+        c = tuple(float(x) for x in rng.uniform(0.3, 0.9, 3))
+        out.append(dict(vertices=v, indices=i, color=c, transform=xf, object_id=7,
+                        instance_id=100 + k))
+    return out
+
+
+def instanced_pathtrace(count=6, level=3, vol_n=48, width=320, height=240, frames=1,
+                        max_bounce=4, renderer="pathtrace"):
+    """Instanced meshes around a Gaussian-blob volume under a directional
+    light and ambient light: rotated, non-uniformly scaled instances of one
+    icosphere (instanced_meshes), camera above the ring."""
+    vox = _vrt.gen_blobs((vol_n, vol_n, vol_n), count=8, seed=3)
+    sp = 0.6 / (vol_n - 1)
+    pos = (0.9, 0.8, 1.1)
+    g = max(1, (vol_n + 15) // 16)
+    return Workload(
+        name=f"{renderer} {count} instanced icospheres + blobs {vol_n}^3 {width}x{height}",
+        voxels=vox, origin=(-0.3,) * 3, spacing=(sp,) * 3, tf=ramp_tf(256),
+        value_range=(0.0, 1.0), unit_distance=0.05, grid_dims=(g, g, g),
+        camera=dict(position=pos, direction=look_at(pos), up=(0, 1, 0),
+                    fovy=math.radians(45.0), aspect=width / height),
+        lights=[dict(type="directional", direction=(-0.3, -1.0, -0.4), color=(1, 1, 1),
+                     intensity=2.0)],
+        params=dict(background=(0.1, 0.1, 0.1, 1), ambient_color=(1, 1, 1),
+                    ambient_radiance=0.3, occlusion_distance=1e20, ambient_samples=1,
+                    max_bounce=max_bounce, pixel_samples=1, volume_sampling_rate=1.0),
+        width=width, height=height, renderer=renderer, frames=frames,
+        instances=instanced_meshes(count, level))
+
+
 def _orbit_camera(seed, radius, height, width, aspect_h, fovy_deg=40.0):
     """Camera on a circle of `radius` around the y axis at `height`, at a
     seeded angle, looking at the origin."""
@@ -280,7 +334,16 @@ class GpuScene:
         lights = [light_desc(L, h) for L, h in zip(wl.lights, self.hdris)]
         self.meshes = [(_vrt.DeviceMesh(ctx, m["vertices"], m.get("indices"), bvh_builder),
                         m["color"]) for m in (wl.meshes or [])]
-        self.scene = _vrt.make_scene([volume] if volume is not None else [], lights, self.meshes)
+        self.inst_meshes = [_vrt.DeviceMesh(ctx, m["vertices"], m.get("indices"), bvh_builder)
+                            for m in (wl.instances or [])]
+        self.tls = None
+        if wl.instances:
+            self.tls = _vrt.DeviceTls(ctx, [
+                (mesh, m["transform"], m["color"], m.get("object_id", _vrt.NO_ID),
+                 m.get("instance_id", _vrt.NO_ID))
+                for mesh, m in zip(self.inst_meshes, wl.instances)])
+        self.scene = _vrt.make_scene([volume] if volume is not None else [], lights, self.meshes,
+                                     self.tls)
         self.camera = _vrt.camera(**wl.camera)
         p = dict(wl.params)
         p.update(params or {})

@@ -135,6 +135,30 @@ typedef struct vrt_surface {
 
 #define VRT_MAX_SURFACES 4
 
+/* A triangle mesh placed by an instance transform (dco::Instance,
+ * DeviceCopyableObjects.h:1659-1680, with the matte surface above):
+ * world = L * object + t for the row-major 3x4 `transform` [L | t].  Rays
+ * meet it in object space, ori' = affineInv (ori + transInv), dir' =
+ * affineInv dir (intersect(Ray, Instance), :1772-1814), so hit distances
+ * stay world distances; normals go back by transpose(affineInv)
+ * (Instance.cpp:88-97, Pathtrace_impl.cpp:243-247). */
+typedef struct vrt_instance {
+  vrt_bvh bvh;
+  float transform[12];
+  float color[3];
+  uint32_t object_id;    /* Surface "id" (Surface.cpp:21), ~0u when unset */
+  uint32_t instance_id;  /* the instance's "id", ~0u when unset */
+} vrt_instance;
+
+/* Instance-level BVH over any number of instances (the world's TLS over
+ * its instances, scene/VisionarayScene.cpp:69-117, World.cpp:188-244):
+ * a 4-wide BVH over the instances' world boxes, built on the host and
+ * uploaded with the instance records (vrt_tls_build).  A scene's surface
+ * hit is the closest over its surfaces and its TLS's instances; a surface
+ * keeps ties with an instance, and among instances the lower index in the
+ * array given to vrt_tls_build wins ties. */
+typedef struct vrt_tls_s *vrt_tls;
+
 typedef struct vrt_scene {
   int num_volumes;
   vrt_volume volumes[VRT_MAX_VOLUMES];
@@ -142,6 +166,7 @@ typedef struct vrt_scene {
   vrt_light lights[VRT_MAX_LIGHTS];
   int num_surfaces;
   vrt_surface surfaces[VRT_MAX_SURFACES];
+  vrt_tls tls;  /* instanced meshes (NULL: none) */
 } vrt_scene;
 
 /* ANARI "perspective" camera (camera/Perspective.cpp:14-44,
@@ -362,6 +387,30 @@ VRT_API int vrt_bvh_build_gpu(vrt_context ctx, const float *vertices, int64_t nu
 VRT_API int vrt_bvh_get_info(vrt_bvh bvh, vrt_bvh_info *info);
 VRT_API int vrt_bvh_destroy(vrt_bvh bvh);
 
+/* The instance's ray and normal transforms (Instance::finalize,
+ * Instance.cpp:88-97): affine_inv = inverse(L) (row-major; evaluated in
+ * double by cofactors, rounded to float), trans_inv = -t, normal_xfm =
+ * transpose(affine_inv).  VRT_ERROR_INVALID_ARGUMENT for a singular or
+ * non-finite L.  vrt_tls_build uses exactly these values. */
+VRT_API int vrt_instance_transforms(const float transform[12], float affine_inv[9],
+                                    float trans_inv[3], float normal_xfm[9]);
+
+typedef struct vrt_tls_info {
+  uint32_t num_instances, num_nodes;
+  /* most entries the instance walk can stack (<= VRT_BVH_MAX_STACK) */
+  uint32_t stack_bound;
+  float bounds_lo[3], bounds_hi[3];  /* world box of all instances */
+  double build_ms;
+} vrt_tls_info;
+
+/* Builds the TLS of n >= 1 instances (host binned SAH over their world
+ * boxes, the corners of each mesh box under its transform, rounded
+ * outwards).  The meshes' BVHs must outlive the TLS.  Replaces the world's
+ * instance BVH build and upload (VisionarayScene.cpp:69-117). */
+VRT_API int vrt_tls_build(vrt_context ctx, const vrt_instance *instances, int n, vrt_tls *out);
+VRT_API int vrt_tls_get_info(vrt_tls tls, vrt_tls_info *info);
+VRT_API int vrt_tls_destroy(vrt_tls tls);
+
 /* HDRI environment light (HDRI::finalize, HDRI.cpp:31-162): `radiance` is
    width x height RGB float32, x fastest.  Builds the per-row and last-column
    CDFs of the max-component luminance (makeCDF) on the host and uploads them
@@ -465,11 +514,14 @@ VRT_API int vrt_sample_field(vrt_context ctx, const vrt_volume *vol, int n,
  * The intersect(Ray, Volume) collision sampler of
  * DeviceCopyableObjects.h:680-796. */
 /* Closest (any = 0) or any (any = 1) surface hit of n rays {ori xyz, dir
- * xyz, tmin, tmax} (device float[8] each): t (+inf on a miss) and the
- * triangle's index in its mesh (-1).  The surface part of intersectAll
- * (renderer/common.h:1606-1620), exposed for tests and tools. */
+ * xyz, tmin, tmax} (device float[8] each): t (+inf on a miss), the
+ * triangle's index in its mesh (-1) and, when surf_out is not NULL, what
+ * was hit: surface i as i, instance j of the scene's TLS as num_surfaces + j
+ * (-1 on a miss).  The surface part of intersectAll (renderer/common.h:
+ * 1606-1620), exposed for tests and tools. */
 VRT_API int vrt_intersect_surfaces(vrt_context ctx, const vrt_scene *scene, int n,
-                                   const float *rays, int any, float *t_out, int *prim_out);
+                                   const float *rays, int any, float *t_out, int *prim_out,
+                                   int *surf_out);
 
 VRT_API int vrt_delta_track(vrt_context ctx, const vrt_scene *scene, int n,
                             const float *rays, uint32_t *rng, float *t_out);

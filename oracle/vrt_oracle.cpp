@@ -1203,19 +1203,101 @@ struct HdriLight {
   }
 };
 
+// An instance's transforms (orc_instance), row-major 3x3 products formed as
+// (m0 x + m1 y) + m2 z
+template <class R> struct InstXf {
+  R A[9], tinv[3], N[9];
+  explicit InstXf(const orc_instance &I) {
+    for (int i = 0; i < 9; ++i) A[i] = R(I.affine_inv[i]);
+    for (int i = 0; i < 3; ++i) tinv[i] = R(I.trans_inv[i]);
+    for (int i = 0; i < 9; ++i) N[i] = R(I.normal_xfm[i]);
+  }
+  static V3<R> mul(const R *M, V3<R> p) {
+    return {M[0] * p.x + M[1] * p.y + M[2] * p.z, M[3] * p.x + M[4] * p.y + M[5] * p.z,
+            M[6] * p.x + M[7] * p.y + M[8] * p.z};
+  }
+  // intersect(Ray, Instance) (DeviceCopyableObjects.h:1796-1798)
+  Ray<R> to_object(const Ray<R> &r) const {
+    Ray<R> l = r;
+    l.ori = mul(A, r.ori + V3<R>(tinv[0], tinv[1], tinv[2]));
+    l.dir = mul(A, r.dir);
+    return l;
+  }
+};
+
 template <class R> struct Scene {
   std::vector<Volume<R>> vols;
   std::vector<Mesh<R>> meshes;
+  std::vector<Mesh<R>> imeshes;     // per instance
+  std::vector<InstXf<R>> ixf;       // per instance
   std::vector<QuadLight<R>> quads;  // per light (unused entries for other types)
   std::vector<HdriLight> hdris;     // per light (empty for other types)
   const orc_scene *s;
   explicit Scene(const orc_scene *sc) : s(sc) {
     for (int i = 0; i < sc->num_volumes; ++i) vols.emplace_back(&sc->volumes[i]);
     for (int i = 0; i < sc->num_meshes; ++i) meshes.emplace_back(&sc->meshes[i]);
+    for (int i = 0; i < sc->num_instances; ++i) {
+      imeshes.emplace_back(&sc->instances[i].mesh);
+      ixf.emplace_back(sc->instances[i]);
+    }
     for (int i = 0; i < sc->num_lights; ++i) quads.emplace_back(sc->lights[i]);
     for (int i = 0; i < sc->num_lights; ++i) hdris.emplace_back(sc->lights[i]);
   }
-  bool empty() const { return s->num_volumes == 0 && s->num_meshes == 0; }
+  bool empty() const {
+    return s->num_volumes == 0 && s->num_meshes == 0 && s->num_instances == 0;
+  }
+  // surface `id`: mesh id, or instance id - meshes.size()
+  const Mesh<R> &surface(int id) const {
+    return id < int(meshes.size()) ? meshes[size_t(id)] : imeshes[size_t(id) - meshes.size()];
+  }
+  // its triangle's geometric normal in world space: normalize(e1 x e2), then
+  // normalize(normal_xfm * n) (getNormals, common.h:233-264;
+  // Pathtrace_impl.cpp:243-247; identity meshes: normalised twice)
+  V3<R> surface_normal(int id, int prim) const {
+    const Tri<R> &T = surface(id).tris[size_t(prim)];
+    const V3<R> n = normalize(cross(T.e1, T.e2));
+    if (id < int(meshes.size())) return normalize(n);
+    return normalize(InstXf<R>::mul(ixf[size_t(id) - meshes.size()].N, n));
+  }
+  // intersectSurfaces: the closest (any: a) hit over the meshes, a later
+  // mesh taking ties, then the instances, which must be strictly closer
+  // (the lower instance index keeps ties)
+  bool surfaces_hit(const Ray<R> &ray, bool any, R &t_out, int &id, int &prim, R &u,
+                    R &v) const {
+    bool hit = false;
+    R best = ray.tmax;
+    for (size_t m = 0; m < meshes.size(); ++m) {
+      Ray<R> r = ray;
+      if (hit) r.tmax = best;
+      R t, uu, vv;
+      int p = -1;
+      if (meshes[m].intersect(r, any, t, p, uu, vv)) {
+        hit = true;
+        best = t;
+        id = int(m);
+        prim = p;
+        u = uu;
+        v = vv;
+      }
+    }
+    for (size_t j = 0; j < imeshes.size(); ++j) {
+      if (any && hit) break;
+      Ray<R> r = ixf[j].to_object(ray);
+      r.tmax = hit ? best : ray.tmax;
+      R t, uu, vv;
+      int p = -1;
+      if (!imeshes[j].intersect(r, any, t, p, uu, vv)) continue;
+      if (hit && !(t < best)) continue;
+      hit = true;
+      best = t;
+      id = int(meshes.size() + j);
+      prim = p;
+      u = uu;
+      v = vv;
+    }
+    t_out = best;
+    return hit;
+  }
 };
 
 template <class R> struct LightSample {
@@ -1294,15 +1376,13 @@ HitRec<R> intersect_all(const Scene<R> &sc, const Ray<R> &ray, Random &rnd, Coun
                         unsigned bounceID = 0, bool shadow = false, bool ratio = false) {
   HitRec<R> hr;
   // intersectSurfaces<1>: closest hit, any hit for shadow / AO rays
-  for (size_t m = 0; m < sc.meshes.size(); ++m) {
+  {
     R t, u, v;
-    int prim = -1;
-    Ray<R> r = ray;
-    if (hr.surf_hit) r.tmax = hr.surf_t;
-    if (sc.meshes[m].intersect(r, shadow, t, prim, u, v)) {
+    int id = -1, prim = -1;
+    if (sc.surfaces_hit(ray, shadow, t, id, prim, u, v)) {
       hr.surf_hit = true;
       hr.surf_t = t;
-      hr.surf_mesh = int(m);
+      hr.surf_mesh = id;
       hr.surf_prim = prim;
       hr.surf_u = u;
       hr.surf_v = v;
@@ -1514,16 +1594,12 @@ void shade(Random &srnd, const Ray<R> &ray, RayType rayType, const Scene<R> &sc,
     } else {
       // triangle surface (Pathtrace_impl.cpp:234-274): Ng = normalize(e1 x
       // e2), no vertex normals (getNormals, common.h:233-264), matte color
-      const Mesh<R> &M = sc.meshes[hitRec.surf_mesh];
-      const Tri<R> &T = M.tris[hitRec.surf_prim];
+      const Mesh<R> &M = sc.surface(hitRec.surf_mesh);
       hitPos = ray.ori + ray.dir * hitRec.surf_t;
       eps = epsilon_from(hitPos, ray.dir, hitRec.surf_t);
-      gn = normalize(cross(T.e1, T.e2));
+      // the instance's normal transform, then normalised again (:243-247)
+      gn = sc.surface_normal(hitRec.surf_mesh, hitRec.surf_prim);
       sn = gn;
-      // the normal transform of the (identity) instance, then normalised
-      // again (:243-247)
-      gn = normalize(gn);
-      sn = normalize(sn);
       sn = faceforward(sn, viewDir, gn);
       gn = faceforward(gn, viewDir, gn);
       color = M.color;
@@ -1793,22 +1869,17 @@ V4<R> raycast_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params *
   {
     R st = std::numeric_limits<R>::max();
     int sm = -1, sp = -1;
-    for (size_t m = 0; m < sc.meshes.size(); ++m) {
+    {
       R t, u, v;
-      int prim = -1;
-      Ray<R> r = ray;
-      if (sm >= 0) r.tmax = st;
-      if (sc.meshes[m].intersect(r, false, t, prim, u, v)) {
+      if (sc.surfaces_hit(ray, false, t, sm, sp, u, v))
         st = t;
-        sm = int(m);
-        sp = prim;
-      }
+      else
+        sm = -1;
     }
     if (sm >= 0) {
-      const Mesh<R> &M = sc.meshes[sm];
-      const Tri<R> &T = M.tris[sp];
+      const Mesh<R> &M = sc.surface(sm);
       const V3<R> hitPos = ray.ori + ray.dir * st;
-      const V3<R> sn = normalize(normalize(cross(T.e1, T.e2)));
+      const V3<R> sn = sc.surface_normal(sm, sp);
       const R inv_pi = R(0.318309886183790671538);
       V3<R> shaded(R(0));
       for (int i = 0; i < sc.s->num_lights; ++i) {
@@ -2576,6 +2647,51 @@ float orc_pow_ieee(float x, float y) { return orc::pow_ieee(x, y); }
 }  // extern "C"
 
 extern "C" {
+
+int orc_instance_transforms(const float xf[12], float A[9], float tinv[3], float N[9]) {
+  for (int i = 0; i < 12; ++i)
+    if (!std::isfinite(xf[i])) return -1;
+  double L[9];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) L[3 * r + c] = double(xf[4 * r + c]);
+  // the adjugate over the determinant, cofactor by cofactor
+  const double c00 = L[4] * L[8] - L[5] * L[7], c01 = L[3] * L[8] - L[5] * L[6];
+  const double c02 = L[3] * L[7] - L[4] * L[6];
+  const double det = L[0] * c00 - L[1] * c01 + L[2] * c02;
+  if (!(det != 0.0) || !std::isfinite(det)) return -1;
+  const double inv[9] = {c00 / det, (L[2] * L[7] - L[1] * L[8]) / det,
+                         (L[1] * L[5] - L[2] * L[4]) / det, (L[5] * L[6] - L[3] * L[8]) / det,
+                         (L[0] * L[8] - L[2] * L[6]) / det, (L[2] * L[3] - L[0] * L[5]) / det,
+                         c02 / det, (L[1] * L[6] - L[0] * L[7]) / det,
+                         (L[0] * L[4] - L[1] * L[3]) / det};
+  for (int i = 0; i < 9; ++i) {
+    A[i] = float(inv[i]);
+    if (!std::isfinite(A[i])) return -1;
+  }
+  for (int k = 0; k < 3; ++k) tinv[k] = -xf[4 * k + 3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) N[3 * r + c] = A[3 * c + r];
+  return 0;
+}
+
+void orc_scene_intersect_surfaces(const orc_scene *scene, int n, const float *rays, int any,
+                                  float *t_out, int *prim_out, int *surf_out) {
+  const orc::Scene<float> sc(scene);
+  for (int i = 0; i < n; ++i) {
+    const float *r = rays + 8 * size_t(i);
+    orc::Ray<float> ray;
+    ray.ori = orc::V3<float>(r[0], r[1], r[2]);
+    ray.dir = orc::V3<float>(r[3], r[4], r[5]);
+    ray.tmin = r[6];
+    ray.tmax = r[7];
+    float t, u, v;
+    int id = -1, prim = -1;
+    const bool hit = sc.surfaces_hit(ray, any != 0, t, id, prim, u, v);
+    t_out[i] = hit ? t : std::numeric_limits<float>::infinity();
+    prim_out[i] = hit ? prim : -1;
+    surf_out[i] = hit ? id : -1;
+  }
+}
 
 void orc_mesh_intersect(const orc_trimesh *m, int n, const float *rays, int any, int brute,
                         float *t_out, int *prim_out) {

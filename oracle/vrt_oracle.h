@@ -95,6 +95,29 @@ typedef struct orc_trimesh {
   uint32_t instance_id; /* the instance's "id" */
 } orc_trimesh;
 
+/* A triangle mesh placed by an instance (dco::Instance,
+   DeviceCopyableObjects.h:1659-1680; intersect(Ray, Instance), :1772-1814):
+   rays meet it in object space, ori' = affine_inv (ori + trans_inv),
+   dir' = affine_inv dir, and its normals go back by normal_xfm =
+   transpose(affine_inv) (Instance.cpp:94-97, Pathtrace_impl.cpp:243-247).
+   The matrices (row-major) are given, so the hits depend on traversal and
+   arithmetic alone.  A scene's surface hit is the closest over its meshes
+   (identity) and its instances: a mesh keeps ties with an instance, and
+   among instances the lower index wins. */
+/* affine_inv = inverse(L) of a row-major 3x4 [L | t] (cofactors in
+   double, rounded to float), trans_inv = -t, normal_xfm =
+   transpose(affine_inv) (Instance.cpp:94-97); returns 0, or -1 for a
+   singular or non-finite L */
+int orc_instance_transforms(const float xfm[12], float affine_inv[9], float trans_inv[3],
+                            float normal_xfm[9]);
+
+typedef struct orc_instance {
+  orc_trimesh mesh;
+  float affine_inv[9];
+  float trans_inv[3];
+  float normal_xfm[9];
+} orc_instance;
+
 typedef struct orc_scene {
   int num_volumes;
   const orc_volume *volumes;
@@ -102,6 +125,8 @@ typedef struct orc_scene {
   const orc_light *lights;
   int num_meshes;
   const orc_trimesh *meshes;
+  int num_instances;
+  const orc_instance *instances;
 } orc_scene;
 
 typedef struct orc_camera {
@@ -220,6 +245,12 @@ void orc_pow_log2_table(float *out);
    and prim = -1 on a miss --- */
 void orc_mesh_intersect(const orc_trimesh *m, int n, const float *rays, int any, int brute,
                         float *t_out, int *prim_out);
+/* Closest (any = 0) or any (any = 1) surface hit of n rays {ori xyz, dir
+   xyz, tmin, tmax} in a scene's meshes and instances (float arithmetic):
+   t (+inf on a miss), the triangle index in its mesh (-1) and what was hit
+   (mesh i as i, instance j as num_meshes + j; -1). */
+void orc_scene_intersect_surfaces(const orc_scene *sc, int n, const float *rays, int any,
+                                  float *t_out, int *prim_out, int *surf_out);
 /* --- A17: n quad-light samples seen from ref (RNG seeded (seed, i)):
    positions and pdfs --- */
 void orc_quad_light_sample(const orc_light *L, const float ref[3], int n, uint32_t seed,

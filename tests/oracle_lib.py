@@ -42,10 +42,16 @@ class OrcTrimesh(C.Structure):
                 ("instance_id", C.c_uint32)]
 
 
+class OrcInstance(C.Structure):
+    _fields_ = [("mesh", OrcTrimesh), ("affine_inv", C.c_float * 9), ("trans_inv", f3),
+                ("normal_xfm", C.c_float * 9)]
+
+
 class OrcScene(C.Structure):
     _fields_ = [("num_volumes", C.c_int), ("volumes", C.POINTER(OrcVolume)),
                 ("num_lights", C.c_int), ("lights", C.POINTER(OrcLight)),
-                ("num_meshes", C.c_int), ("meshes", C.POINTER(OrcTrimesh))]
+                ("num_meshes", C.c_int), ("meshes", C.POINTER(OrcTrimesh)),
+                ("num_instances", C.c_int), ("instances", C.POINTER(OrcInstance))]
 
 
 class OrcCamera(C.Structure):
@@ -110,6 +116,8 @@ def lib():
         L.orc_cvt_uint32_srgb.argtypes = [vp]
         L.orc_camera_ray.argtypes = [C.POINTER(OrcCamera), C.c_float, C.c_float, ip, ip, vp, vp]
         L.orc_mesh_intersect.argtypes = [C.POINTER(OrcTrimesh), ip, vp, ip, ip, vp, vp]
+        L.orc_scene_intersect_surfaces.argtypes = [C.POINTER(OrcScene), ip, vp, ip, vp, vp, vp]
+        L.orc_instance_transforms.argtypes = [vp, vp, vp, vp]
         L.orc_quad_light_sample.argtypes = [C.POINTER(OrcLight), vp, ip, C.c_uint32, vp, vp]
         L.orc_unstructured_info.argtypes = [C.POINTER(OrcVolume), vp, vp, vp, vp]
         L.orc_sample_field_brute.argtypes = [C.POINTER(OrcVolume), ip, vp, vp]
@@ -342,12 +350,42 @@ class OracleMesh:
         self.s = m
 
 
+def instance_transforms(transform):
+    """The oracle's (affine_inv, trans_inv, normal_xfm) of a row-major 3x4
+    [L | t]; None for a singular L."""
+    x = np.ascontiguousarray(transform, np.float32).reshape(12)
+    a, t, n = np.empty(9, np.float32), np.empty(3, np.float32), np.empty(9, np.float32)
+    if lib().orc_instance_transforms(x.ctypes.data, a.ctypes.data, t.ctypes.data,
+                                     n.ctypes.data) != 0:
+        return None
+    return a.reshape(3, 3), t, n.reshape(3, 3)
+
+
+class OracleInstance:
+    """An OracleMesh placed by a row-major 3x4 transform (its inverse and
+    normal transform from instance_transforms)."""
+
+    def __init__(self, mesh, transform, object_id=0xFFFFFFFF, instance_id=0xFFFFFFFF):
+        self.mesh = mesh
+        a, t, n = instance_transforms(transform)
+        s = OrcInstance()
+        s.mesh = mesh.s
+        s.mesh.object_id = object_id
+        s.mesh.instance_id = instance_id
+        s.affine_inv = (C.c_float * 9)(*a.ravel())
+        s.trans_inv = f3(*t)
+        s.normal_xfm = (C.c_float * 9)(*n.ravel())
+        self.s = s
+
+
 class OracleScene:
-    def __init__(self, volumes, lights=(), meshes=()):
+    def __init__(self, volumes, lights=(), meshes=(), instances=()):
         self.volumes = list(volumes)
         self.meshes = list(meshes)
+        self.instances = list(instances)
         self.varr = (OrcVolume * max(1, len(self.volumes)))(*[v.s for v in self.volumes])
         self.marr = (OrcTrimesh * max(1, len(self.meshes)))(*[m.s for m in self.meshes])
+        self.iarr = (OrcInstance * max(1, len(self.instances)))(*[i.s for i in self.instances])
         self.larr = (OrcLight * max(1, len(lights)))()
         for i, L in enumerate(lights):
             self.larr[i] = L
@@ -358,6 +396,8 @@ class OracleScene:
         s.lights = C.cast(self.larr, C.POINTER(OrcLight))
         s.num_meshes = len(self.meshes)
         s.meshes = C.cast(self.marr, C.POINTER(OrcTrimesh))
+        s.num_instances = len(self.instances)
+        s.instances = C.cast(self.iarr, C.POINTER(OrcInstance))
         self.s = s
 
 
@@ -471,7 +511,13 @@ class OracleWorkload:
                                                 L.get("visible", True)))
         self.meshes = [OracleMesh(m["vertices"], m.get("indices"), m["color"])
                        for m in (wl.meshes or [])]
-        self.scene = OracleScene([self.vol] if self.vol is not None else [], lights, self.meshes)
+        self.instances = [
+            OracleInstance(OracleMesh(m["vertices"], m.get("indices"), m["color"]),
+                           m["transform"], m.get("object_id", 0xFFFFFFFF),
+                           m.get("instance_id", 0xFFFFFFFF))
+            for m in (getattr(wl, "instances", None) or [])]
+        self.scene = OracleScene([self.vol] if self.vol is not None else [], lights, self.meshes,
+                                 self.instances)
         self.cam = camera(**wl.camera)
         self.params = params(**wl.params)
 
@@ -589,6 +635,18 @@ def nrc_encode(raw):
     out = np.empty(128, np.float64)
     L.orc_nrc_encode(r.ctypes.data, out.ctypes.data)
     return out
+
+
+def scene_intersect_surfaces(scene, rays, any_hit=False):
+    """(t, prim, surface) of rays (n, 8) over an OracleScene's meshes and
+    instances (float arithmetic)."""
+    r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    t = np.empty(len(r), np.float32)
+    p = np.empty(len(r), np.int32)
+    sf = np.empty(len(r), np.int32)
+    lib().orc_scene_intersect_surfaces(C.byref(scene.s), len(r), r.ctypes.data, int(any_hit),
+                                       t.ctypes.data, p.ctypes.data, sf.ctypes.data)
+    return t, p, sf
 
 
 def mesh_intersect(mesh, rays, any_hit=False, brute=False):

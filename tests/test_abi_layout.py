@@ -15,7 +15,8 @@ PAIRS = [("Volume", "vrt_volume"), ("Light", "vrt_light"), ("Surface", "vrt_surf
          ("Scene", "vrt_scene"), ("BvhInfo", "vrt_bvh_info"), ("Camera", "vrt_camera"),
          ("Params", "vrt_params"), ("Frame", "vrt_frame"), ("NrcConfig", "vrt_nrc_config"),
          ("NrcRender", "vrt_nrc_render"), ("NanoVDBInfo", "vrt_nanovdb_info"),
-         ("UnstructuredInfo", "vrt_unstructured_info")]
+         ("UnstructuredInfo", "vrt_unstructured_info"), ("Instance", "vrt_instance"),
+         ("TlsInfo", "vrt_tls_info")]
 
 
 @pytest.fixture(scope="module")

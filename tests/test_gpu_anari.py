@@ -111,7 +111,34 @@ def _anari_scene(a, wl, xfm=None, bvh_builder=None):
         a.commit(g, mat, s)
         surfs.append(s)
         objs.setdefault("surfaces", []).append(s)
+    insts = []
+    for m in wl.instances or []:
+        # a surface in a group placed by an instance transform
+        g = a.new(A.GEOMETRY, "triangle")
+        a.set(g, "vertex.position", A.ARRAY1D, a.array1d(m["vertices"], A.FLOAT32_VEC3))
+        if m.get("indices") is not None:
+            a.set(g, "primitive.index", A.ARRAY1D, a.array1d(m["indices"], A.UINT32_VEC3))
+        mat = a.new(A.MATERIAL, "matte")
+        a.set(mat, "color", A.FLOAT32_VEC3, m["color"])
+        sf = a.new(A.SURFACE)
+        a.set(sf, "geometry", A.GEOMETRY, g)
+        a.set(sf, "material", A.MATERIAL, mat)
+        if "object_id" in m:
+            a.set(sf, "id", A.UINT32, m["object_id"])
+        grp = a.new(A.GROUP)
+        a.set(grp, "surface", A.ARRAY1D, a.objects([sf], A.SURFACE))
+        inst = a.new(A.INSTANCE, "transform")
+        a.set(inst, "group", A.GROUP, grp)
+        m4 = np.eye(4, dtype=np.float32)
+        m4[:3, :] = np.asarray(m["transform"], np.float32).reshape(3, 4)
+        a.set(inst, "transform", A.FLOAT32_MAT4, m4.T.ravel())  # column-major
+        if "instance_id" in m:
+            a.set(inst, "id", A.UINT32, m["instance_id"])
+        a.commit(g, mat, sf, grp, inst)
+        insts.append(inst)
     world = a.new(A.WORLD)
+    if insts:
+        a.set(world, "instance", A.ARRAY1D, a.objects(insts, A.INSTANCE))
     if xfm is None:
         if vols:
             a.set(world, "volume", A.ARRAY1D, a.objects(vols, A.VOLUME))
@@ -205,6 +232,9 @@ CASES = {
         params=dict(max_bounce=4, ambient_samples=1, ratio_tracking=True)),
     "mixed surfaces": lambda w: w.mixed_pathtrace(level=3, vol_n=24, width=48, height=32,
                                                   frames=1),
+    "instanced surfaces pathtrace": lambda w: w.instanced_pathtrace(width=64, height=48),
+    "instanced surfaces raycast": lambda w: w.instanced_pathtrace(width=64, height=48,
+                                                                  renderer="raycast"),
 }
 
 
