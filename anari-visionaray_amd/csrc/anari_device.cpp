@@ -797,8 +797,9 @@ void add_group(Device *d, const Object *grp, const Affine &x, uint32_t inst_id, 
     if (Object *f = v->get_o("value", ANARI_SPATIAL_FIELD)) sb.version = std::max(sb.version, f->version);
     if (!build_volume(d, v)) continue;
     if (sb.volumes.size() >= VRT_MAX_VOLUMES) {
-      d->report(v, ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_OPERATION,
-                "more than %d volumes: ignored", VRT_MAX_VOLUMES);
+      d->report(v, ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_OPERATION,
+                "more than %d volumes in a world: this volume is not rendered",
+                VRT_MAX_VOLUMES);
       continue;
     }
     vrt_volume vd = v->desc;
@@ -862,8 +863,8 @@ void add_group(Device *d, const Object *grp, const Affine &x, uint32_t inst_id, 
   for (Object *o : objects_of(grp, "light", ANARI_LIGHT)) {
     sb.version = std::max(sb.version, o->version);
     if (sb.lights.size() >= VRT_MAX_LIGHTS) {
-      d->report(o, ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_OPERATION,
-                "more than %d lights: ignored", VRT_MAX_LIGHTS);
+      d->report(o, ANARI_SEVERITY_ERROR, ANARI_STATUS_INVALID_OPERATION,
+                "more than %d lights in a world: this light is not rendered", VRT_MAX_LIGHTS);
       continue;
     }
     vrt_light L{};

@@ -284,6 +284,41 @@ def test_instance_transform_matches_oracle(pkg, oracle, gpu_ctx):
     assert np.abs(img - _vrt_frames(pkg, gpu_ctx, wl, 1)[0]).max() > 0.05  # it moved
 
 
+def test_world_limits(pkg, gpu_ctx):
+    """Five surfaces: the fifth beyond the surface slots renders as an
+    identity instance (frames identical to the C ABI's scene of four
+    surfaces and that instance); a fifth volume is reported as an error,
+    not dropped silently."""
+    base = pkg.workloads.instanced_pathtrace(width=48, height=36, renderer="raycast")
+    meshes = [dict(vertices=m["vertices"] @ np.asarray(m["transform"])[:, :3].T
+                   + np.asarray(m["transform"])[:, 3], indices=m["indices"], color=m["color"])
+              for m in base.instances[:5]]
+    for m in meshes:
+        m["vertices"] = np.ascontiguousarray(m["vertices"], np.float32)
+    ident = np.concatenate([np.eye(3), np.zeros((3, 1))], 1).astype(np.float32)
+    wl = _same_tf(pkg, dataclasses.replace(base, instances=None, meshes=meshes))
+    ref_wl = dataclasses.replace(wl, meshes=meshes[:4],
+                                 instances=[dict(meshes[4], transform=ident)])
+    ref = _vrt_frames(pkg, gpu_ctx, ref_wl, 1)[0]
+    a = A.Anari()
+    try:
+        fr, objs = _anari_scene(a, wl)
+        a.render(fr)
+        img, _ = a.map(fr)
+        np.testing.assert_array_equal(img, ref)
+        assert not [m for m in a.messages if m[0] <= 1]
+        # five volumes: the fifth is an error
+        v = objs["volume"]
+        a.set(objs["world"], "volume", A.ARRAY1D, a.objects([v] * 5, A.VOLUME))
+        a.commit(objs["world"])
+        a.render(fr)
+        a.map(fr)
+        errs = [m for m in a.messages if m[0] <= 1]
+        assert errs and "more than 4 volumes" in errs[-1][-1], errs
+    finally:
+        a.close()
+
+
 def test_accumulation_restarts_on_commit(pkg, gpu_ctx):
     wl = _same_tf(pkg, CASES["structured pathtrace"](pkg.workloads))
     a = A.Anari()
