@@ -77,10 +77,9 @@ int nrc_frame_buffers(vrt_nrc n, size_t pixels, int t_cap) {
 
 int nrc_train_frame(vrt_nrc n, int steps) {
   for (int b = 0; b < steps; ++b) {
-    VRT_HIP(nrc_launch_gradients(n->d, n->t_cap, n->t_raw, n->t_target, n->max_workgroups,
-                                 n->ctx->stream, &n->ctr[1], b, steps));
-    VRT_HIP(nrc_launch_adam(n->d, n->t_cap, n->cfg.learning_rate, n->cfg.beta1, n->cfg.beta2,
-                            n->cfg.epsilon, n->ctx->stream, &n->ctr[1], b, steps));
+    VRT_HIP(nrc_launch_train_step(n->d, n->t_cap, n->t_raw, n->t_target, n->max_workgroups,
+                                  n->cfg.learning_rate, n->cfg.beta1, n->cfg.beta2,
+                                  n->cfg.epsilon, n->ctx->stream, &n->ctr[1], b, steps));
   }
   return VRT_OK;
 }
@@ -116,7 +115,7 @@ int vrt_nrc_create(vrt_context ctx, const vrt_nrc_config *cfg, vrt_nrc *out) {
   A((void **)&d.wt, sizeof(uint16_t) * kNrcWeights);
   A((void **)&d.bias, sizeof(float) * kNrcBiases);
   A((void **)&d.grad, sizeof(float) * kNrcParams);
-  A((void **)&d.partials, sizeof(float) * kNrcParams * size_t(n->max_workgroups));
+  A((void **)&d.partials, sizeof(float) * kNrcPartStride * size_t(n->max_workgroups));
   A((void **)&d.stats, sizeof(float) * 4);
   if (e == hipSuccess) e = nrc_launch_init(d, cfg->seed, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -183,10 +182,13 @@ int vrt_nrc_gradients(vrt_nrc n, int count, const float *raw, const float *targe
 }
 
 int vrt_nrc_train_step(vrt_nrc n, int count, const float *raw, const float *target, float *loss) {
-  if (int r = vrt_nrc_gradients(n, count, raw, target, nullptr, nullptr)) return r;
-  if (count == 0) return VRT_OK;
-  VRT_HIP(nrc_launch_adam(n->d, count, n->cfg.learning_rate, n->cfg.beta1, n->cfg.beta2,
-                          n->cfg.epsilon, n->ctx->stream));
+  if (int r = check_nrc(n)) return r;
+  if (count < 0 || count > n->cfg.max_batch || (count > 0 && (!raw || !target)))
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "bad training arguments (n=%d, max_batch=%d)", count,
+                n->cfg.max_batch);
+  VRT_HIP(nrc_launch_train_step(n->d, count, raw, target, n->max_workgroups,
+                                n->cfg.learning_rate, n->cfg.beta1, n->cfg.beta2,
+                                n->cfg.epsilon, n->ctx->stream));
   if (loss) {
     VRT_HIP(hipMemcpyAsync(loss, n->d.stats, sizeof(float), hipMemcpyDeviceToHost, n->ctx->stream));
     VRT_HIP(hipStreamSynchronize(n->ctx->stream));

@@ -25,6 +25,8 @@ __host__ __device__ constexpr int nrc_boff(int l) { return l * 64; }
 constexpr int kNrcWeights = 8192 + 4 * 4096 + 1024;  // 25600 padded weights
 constexpr int kNrcBiases = 5 * 64 + 16;               // 336 padded biases
 constexpr int kNrcParams = kNrcWeights + kNrcBiases;  // master parameter vector
+// one workgroup's partial sums: the parameters' gradients, then its loss
+constexpr int kNrcPartStride = (kNrcParams + 1 + 63) / 64 * 64;
 
 // Parameter-vector index of weight (l, o, i) and bias (l, o).
 __host__ __device__ inline int nrc_w(int l, int o, int i) { return nrc_woff(l) + o * nrc_in(l) + i; }
@@ -38,7 +40,7 @@ struct NrcDevice {
   uint16_t *wt;        // kNrcWeights fp16/bf16 bits, transposed per layer [in][out]
   float *bias;         // kNrcBiases fp32 (added to the fp32 accumulators)
   float *grad;         // kNrcParams gradient sums of the last step
-  float *partials;     // max_workgroups x kNrcParams
+  float *partials;     // max_workgroups x kNrcPartStride
   float *stats;        // [0] loss sum of the last step, [1] samples, [2] Adam steps taken
   int bf16;            // 0: fp16, 1: bf16
 };
@@ -55,11 +57,14 @@ hipError_t nrc_launch_gradients(const NrcDevice &d, int n, const float *raw,
                                 const float *target, int max_workgroups, hipStream_t s,
                                 const uint32_t *n_dev = nullptr, int batch = 0,
                                 int nbatches = 1);
-// one Adam step on the mean gradient of mini-batch `batch`; an empty
-// mini-batch changes nothing, not even the step count of the bias correction
-hipError_t nrc_launch_adam(const NrcDevice &d, int n, float lr, float beta1, float beta2,
-                           float eps, hipStream_t s, const uint32_t *n_dev = nullptr,
-                           int batch = 0, int nbatches = 1);
+// the same gradients, then one Adam step on the mini-batch's mean gradient;
+// an empty mini-batch changes nothing, not even the step count of the bias
+// correction
+hipError_t nrc_launch_train_step(const NrcDevice &d, int n, const float *raw,
+                                 const float *target, int max_workgroups, float lr, float beta1,
+                                 float beta2, float eps, hipStream_t s,
+                                 const uint32_t *n_dev = nullptr, int batch = 0,
+                                 int nbatches = 1);
 hipError_t nrc_launch_refresh(const NrcDevice &d, hipStream_t s);
 
 // MFMA lane-layout probe: C = A (16x16) * B (16x16) for one wave.

@@ -20,9 +20,13 @@
 // ([feature][sample], 16-bit), back-propagates dX^T = W^T dZ^T per wave in
 // registers, and forms dW = dZ^T X over the workgroup's 64 samples with one
 // MFMA chain per 16x16 weight tile (tiles dealt round-robin to the 4 waves,
-// accumulated in registers across the workgroup's sample chunks).  Partial
-// gradients per workgroup are summed in a fixed order (deterministic), then
-// Adam updates the fp32 master weights and re-emits the 16-bit copies.
+// accumulated in registers across the workgroup's sample chunks).  A
+// training step is latency-bound (one 64-sample chunk per workgroup at 8,192
+// samples), so the weights of layers 1-5 and their transposes are copied to
+// LDS with one round of loads at the start rather than fetched layer by
+// layer.  Partial gradients and losses per workgroup are summed in a fixed
+// order (deterministic, bias sums included), and the same kernel applies
+// Adam to the fp32 master weights and re-emits the 16-bit copies.
 #include "nrc_kernels.h"
 
 #include <algorithm>
@@ -116,10 +120,17 @@ __host__ __device__ constexpr int weight_col(int l, int c) { return l == 0 ? c :
 // v_sin/v_cos take revolutions: 2^k pi x = 2 pi * frac(2^(k-1) x), and the
 // scaling by a power of two and the fraction are exact.
 // ---------------------------------------------------------------------------
+// Raw input of a lane-dependent feature f < 120 (input f / 24), as
+// comparisons on f: an indexed read of the 6-element array would go through
+// scratch or movrel.
+__device__ inline float raw_at(const float r[6], int f) {
+  return f < 48 ? (f < 24 ? r[0] : r[1]) : (f < 72 ? r[2] : (f < 96 ? r[3] : r[4]));
+}
+
 __device__ inline float encode(const float *raw, int f) {
   if (f < 120) {
-    const int s = f / 24, w = f % 24, k = w >> 1;
-    float r = raw[s] * __uint_as_float(uint32_t(126 + k) << 23);  // * 2^(k-1)
+    const int w = f % 24, k = w >> 1;
+    float r = raw_at(raw, f) * __uint_as_float(uint32_t(126 + k) << 23);  // * 2^(k-1)
     r = r - floorf(r);
     return (w & 1) ? __builtin_amdgcn_cosf(r) : __builtin_amdgcn_sinf(r);
   }
@@ -140,12 +151,13 @@ __device__ inline void load_raw(const float *raw, int n, int q, float r[6]) {
 // Forward pass of QB 16-query blocks per wave.  `hin` holds the layer-0 B
 // operands (4 k-blocks of 32 features); the 3 outputs of query block qb are
 // returned in out[qb] (rows 0..3 of lanes 0..15).  Weights come through
-// `wrow(l, row, col)` -> pointer to 8 consecutive 16-bit weights of the
-// stored (kperm-ordered) row.
+// `w0frag(ob, kb)` -> this lane's A operand of layer 0's output block ob and
+// input block kb; `wrow(l, row, col)` (layers 1-5) -> pointer to 8
+// consecutive 16-bit weights of the stored (kperm-ordered) row.
 // ---------------------------------------------------------------------------
-template <int BF16, int QB, class WPtr, class Store>
-__device__ inline void forward(const WPtr &wrow, const float *bias, Frag8 (&hin)[4][QB],
-                               f4v (&out)[QB], const Store &store_act) {
+template <int BF16, int QB, class W0Frag, class WPtr, class Store>
+__device__ inline void forward(const W0Frag &w0frag, const WPtr &wrow, const float *bias,
+                               Frag8 (&hin)[4][QB], f4v (&out)[QB], const Store &store_act) {
   const int lane = threadIdx.x & 63;
   const int r16 = lane & 15, g = lane >> 4;
   Frag h[4][QB];
@@ -161,7 +173,7 @@ __device__ inline void forward(const WPtr &wrow, const float *bias, Frag8 (&hin)
     for (int kb = 0; kb < 4; ++kb)
 #pragma unroll
       for (int ob = 0; ob < 4; ++ob) {
-        const Frag8 a = load8(wrow(0, ob * 16 + r16, kb * 32 + 8 * g));
+        const Frag8 a = w0frag(ob, kb);
 #pragma unroll
         for (int qb = 0; qb < QB; ++qb) acc[ob][qb] = mfma<BF16>(a, hin[kb][qb], acc[ob][qb]);
       }
@@ -258,6 +270,7 @@ __global__ void __launch_bounds__(256) k_nrc_infer(NrcDevice d, int n, const uin
   auto no_store = [](int, int, int, const Frag &) {};
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r16 = lane & 15, g = lane >> 4;
+  auto w0frag = [&](int ob, int kb) { return load8(wrow(0, ob * 16 + r16, kb * 32 + 8 * g)); };
   constexpr int QB = 4;
   const int tiles = (n + 63) / 64;  // 64 queries (QB blocks of 16) per wave-tile
   for (int t = blockIdx.x * 4 + wave; t < tiles; t += gridDim.x * 4) {
@@ -276,7 +289,7 @@ __global__ void __launch_bounds__(256) k_nrc_infer(NrcDevice d, int n, const uin
       }
     }
     f4v o[QB];
-    forward<BF16, QB>(wrow, sb, hin, o, no_store);
+    forward<BF16, QB>(w0frag, wrow, sb, hin, o, no_store);
     if (g == 0) {
 #pragma unroll
       for (int qb = 0; qb < QB; ++qb) {
@@ -297,10 +310,34 @@ __global__ void __launch_bounds__(256) k_nrc_infer(NrcDevice d, int n, const uin
 // with p^2 in the denominator treated as a constant.
 // ---------------------------------------------------------------------------
 constexpr int kTS = 64 + 8;  // LDS row stride of [feature][sample] tiles (halves, 16-B rows)
+// The gradient kernel's LDS copies of W and W^T of layers 1-5: rows of 64
+// halves padded to 72, except W^T of layer 5 (rows of 16 outputs, padded to
+// 24).  Layer l starts at (l - 1) * 64 * 72 in either copy.
+__host__ __device__ constexpr int gw_off(int l) { return (l - 1) * 64 * kTS; }
+__host__ __device__ constexpr int gwt_stride(int l) { return l == 5 ? 24 : kTS; }
+constexpr int kGradW = 4 * 64 * kTS + 16 * kTS;
+constexpr int kGradWt = 4 * 64 * kTS + 64 * 24;
 
+// Sum over the 16 lanes of a DPP row, the same addition tree as butterfly
+// exchanges with lane distance 1, 2, 4, 8 (addition is commutative, so every
+// lane gets the bit-identical sum): quad_perm [1,0,3,2], quad_perm
+// [2,3,0,1], row_half_mirror, row_mirror -- no LDS round trip.
+__device__ inline float row_sum16(float s) {
+  s += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(s), 0xB1, 0xF, 0xF, false));
+  s += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(s), 0x4E, 0xF, 0xF, false));
+  s += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(s), 0x141, 0xF, 0xF, false));
+  s += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(s), 0x140, 0xF, 0xF, false));
+  return s;
+}
+
+//
+// count_step: the launch belongs to an Adam step, whose bias-correction step
+// count (d.stats[2]) advances here when the mini-batch holds samples (a
+// frame's sample count is known only on the device); nothing in this kernel
+// reads it, the update kernel that follows does.
 template <int BF16>
 __global__ void __launch_bounds__(256) k_nrc_grad(NrcDevice d, int n, const uint32_t *n_dev,
-                                                  int batch, int nbatches,
+                                                  int batch, int nbatches, int count_step,
                                                   const float *__restrict__ raw,
                                                   const float *__restrict__ target) {
   {
@@ -309,16 +346,78 @@ __global__ void __launch_bounds__(256) k_nrc_grad(NrcDevice d, int n, const uint
     raw += 6 * size_t(lo);
     target += 3 * size_t(lo);
   }
+  if (count_step && n > 0 && blockIdx.x == 0 && threadIdx.x == 0) d.stats[2] += 1.f;
   __shared__ __align__(16) uint16_t sx0[128 * kTS];   // layer-0 input X^T [feature][sample]
   __shared__ __align__(16) uint16_t sx[5][64 * kTS];  // inputs of layers 1..5
   __shared__ __align__(16) uint16_t sdz[64 * kTS];    // dZ^T of the current layer [out][sample]
-  __shared__ float sdb[kNrcBiases];        // bias-gradient sums of this workgroup
-  __shared__ float sloss[4];
+  __shared__ float sdb[4][kNrcBiases];     // bias-gradient sums of each wave
+  __shared__ float sloss[4];               // loss sums of each wave
+  // W and W^T of layers 1-5 and the biases: every layer's weights arrive
+  // with one round of loads at the start instead of one dependent global
+  // round trip per layer and direction (layer 0's W, 16 KB, is read from
+  // global memory by the forward pass)
+  __shared__ __align__(16) uint16_t sw[kGradW];
+  __shared__ __align__(16) uint16_t swt[kGradWt];
+  __shared__ __align__(16) float sbias[kNrcBiases];
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r16 = lane & 15, g = lane >> 4;
-  for (int i = threadIdx.x; i < kNrcBiases; i += blockDim.x) sdb[i] = 0.f;
+  const int col = wave * 16 + r16;  // this lane's sample column in a chunk
+  // the first chunk's inputs and layer 0's weights (this lane's A operands,
+  // held in registers) are requested together with the LDS copies below
+  float r[6];
+  load_raw(raw, n, blockIdx.x * 64 + col, r);
+  Frag8 w0[4][4];
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+    for (int ob = 0; ob < 4; ++ob)
+      w0[kb][ob] = load8(&d.w[nrc_woff(0) + (ob * 16 + r16) * nrc_in(0) + kb * 32 + 8 * g]);
+  // W and W^T of layers 1-4: 64 rows x 8 chunks of 8 halves each, 8
+  // unconditional chunks of either per thread; layer 5: W 16 rows x 8
+  // chunks (threads 0-127), W^T 64 rows x 2 chunks (threads 128-255); the
+  // biases as 84 float4.  Every load is issued before the first LDS store.
+  const int j5 = threadIdx.x & 127;
+  const bool tr5 = threadIdx.x >= 128;
+  const uint4 v5 = *reinterpret_cast<const uint4 *>(
+      tr5 ? &d.wt[nrc_woff(5) + (j5 >> 1) * 16 + 8 * (j5 & 1)]
+          : &d.w[nrc_woff(5) + (j5 >> 3) * 64 + 8 * (j5 & 7)]);
+  const float4 vb =
+      reinterpret_cast<const float4 *>(d.bias)[min(int(threadIdx.x), kNrcBiases / 4 - 1)];
+  uint4 vw[8], vt[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int i = threadIdx.x + 256 * k;
+    const int l = 1 + (i >> 9), r = (i >> 3) & 63, c = i & 7;
+    vw[k] = *reinterpret_cast<const uint4 *>(&d.w[nrc_woff(l) + r * 64 + 8 * c]);
+    vt[k] = *reinterpret_cast<const uint4 *>(&d.wt[nrc_woff(l) + r * 64 + 8 * c]);
+  }
+  // the encoding of chunk c's inputs r: layer 0's B operands in registers,
+  // X^T of layer 0 in LDS
+  auto encode_chunk = [&](int c, Frag8 (&hin)[4][1]) {
+    const bool valid = c * 64 + col < n;
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+      const int f = kb * 32 + 8 * g;
+      float e[8];
+      for (int j = 0; j < 8; ++j) e[j] = valid ? encode(r, f + j) : 0.f;
+      hin[kb][0] = cat(pack4(e[0], e[1], e[2], e[3], BF16), pack4(e[4], e[5], e[6], e[7], BF16));
+      for (int j = 0; j < 8; ++j) sx0[(f + j) * kTS + col] = to16(e[j], BF16);
+    }
+  };
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int i = threadIdx.x + 256 * k;
+    const int l = 1 + (i >> 9), r = (i >> 3) & 63, c = i & 7;
+    *reinterpret_cast<uint4 *>(&sw[gw_off(l) + r * kTS + 8 * c]) = vw[k];
+    *reinterpret_cast<uint4 *>(&swt[gw_off(l) + r * kTS + 8 * c]) = vt[k];
+  }
+  *reinterpret_cast<uint4 *>(tr5 ? &swt[gw_off(5) + (j5 >> 1) * 24 + 8 * (j5 & 1)]
+                                 : &sw[gw_off(5) + (j5 >> 3) * kTS + 8 * (j5 & 7)]) = v5;
+  if (threadIdx.x < kNrcBiases / 4) reinterpret_cast<float4 *>(sbias)[threadIdx.x] = vb;
+  for (int i = threadIdx.x; i < 4 * kNrcBiases; i += blockDim.x) (&sdb[0][0])[i] = 0.f;
   if (threadIdx.x < 4) sloss[threadIdx.x] = 0.f;
+  __syncthreads();
 
   // dW accumulators: tiles of each layer dealt round-robin to the waves
   // layer 0: 4x8 tiles (8 per wave), layers 1-4: 4x4 (4 per wave), layer 5:
@@ -329,10 +428,10 @@ __global__ void __launch_bounds__(256) k_nrc_grad(NrcDevice d, int n, const uint
     for (int i = 0; i < 4; ++i) acc14[l][i] = f4v{0.f, 0.f, 0.f, 0.f};
   acc5 = f4v{0.f, 0.f, 0.f, 0.f};
 
+  auto w0frag = [&](int ob, int kb) { return w0[kb][ob]; };
   auto wrow = [&](int l, int row, int col) -> const uint16_t * {
-    return &d.w[nrc_woff(l) + row * nrc_in(l) + col];
+    return &sw[gw_off(l) + row * kTS + col];
   };
-  const int col = wave * 16 + r16;  // this lane's sample column in the chunk
   auto store_act = [&](int l, int ob, int, const Frag &h) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -345,24 +444,16 @@ __global__ void __launch_bounds__(256) k_nrc_grad(NrcDevice d, int n, const uint
   for (int c = blockIdx.x; c < chunks; c += gridDim.x) {
     const int q = c * 64 + col;
     const bool valid = q < n;
-    float r[6];
-    load_raw(raw, n, q, r);
+    if (c != int(blockIdx.x)) load_raw(raw, n, q, r);
     Frag8 hin[4][1];
-#pragma unroll
-    for (int kb = 0; kb < 4; ++kb) {
-      const int f = kb * 32 + 8 * g;
-      float e[8];
-      for (int j = 0; j < 8; ++j) e[j] = valid ? encode(r, f + j) : 0.f;
-      hin[kb][0] = cat(pack4(e[0], e[1], e[2], e[3], BF16), pack4(e[4], e[5], e[6], e[7], BF16));
-      for (int j = 0; j < 8; ++j) sx0[(f + j) * kTS + col] = to16(e[j], BF16);
-    }
+    encode_chunk(c, hin);
     f4v out[1];
-    forward<BF16, 1>(wrow, d.bias, hin, out, store_act);
+    forward<BF16, 1>(w0frag, wrow, sbias, hin, out, store_act);
 
     // output gradient: rows 0..2 of lanes 0..15 hold the prediction
     float gz[4] = {0.f, 0.f, 0.f, 0.f};
+    float ls = 0.f;
     if (g == 0 && valid) {
-      float ls = 0.f;
       for (int ch = 0; ch < 3; ++ch) {
         const float p = out[0][ch], t = target[3 * size_t(q) + ch];
         const float den = p * p + 0.01f;
@@ -370,8 +461,9 @@ __global__ void __launch_bounds__(256) k_nrc_grad(NrcDevice d, int n, const uint
         ls += diff * diff / den;
         gz[ch] = 2.f * diff / den;
       }
-      atomicAdd(&sloss[wave], ls);
     }
+    ls = row_sum16(ls);
+    if (lane == 0) sloss[wave] += ls;
     // ---- backward ------------------------------------------------------------
     // dZ^T of the current layer for this wave's 16 samples, D layout,
     // [ob][j] -> row ob*16 + 4g + j
@@ -390,21 +482,23 @@ __global__ void __launch_bounds__(256) k_nrc_grad(NrcDevice d, int n, const uint
         dzf[ob] = pack4(dz[ob][0], dz[ob][1], dz[ob][2], dz[ob][3], BF16);
         for (int j = 0; j < 4; ++j) {
           sdz[(ob * 16 + 4 * g + j) * kTS + col] = to16(dz[ob][j], BF16);
-          float s = dz[ob][j];
-          s += __shfl_xor(s, 1, 64);
-          s += __shfl_xor(s, 2, 64);
-          s += __shfl_xor(s, 4, 64);
-          s += __shfl_xor(s, 8, 64);
-          if (r16 == 0) atomicAdd(&sdb[nrc_boff(l) + ob * 16 + 4 * g + j], s);
+          // one lane per entry of this wave's sums: the additions happen in
+          // program order, and an atomic add returns nothing to wait for
+          const float s = row_sum16(dz[ob][j]);
+          if (r16 == 0) atomicAdd(&sdb[wave][nrc_boff(l) + ob * 16 + 4 * g + j], s);
         }
       }
       __syncthreads();
       // dW tiles (ob, ib) over the chunk's 64 samples (k = sample)
       const uint16_t *xin = l == 0 ? sx0 : sx[l - 1];
-      const int ntiles = nob * nib;
-      for (int t = wave, k = 0; t < ntiles; t += 4, ++k) {
+      // nob * nib is a multiple of 4: every wave owns the same number of
+      // tiles, and with l and k known at compile time the accumulators stay
+      // in registers
+#pragma unroll
+      for (int k = 0; k < nob * nib / 4; ++k) {
+        const int t = wave + 4 * k;
         const int ob = t / nib, ib = t % nib;
-        f4v acc = l == 0 ? acc0[k] : (l == 5 ? acc5 : acc14[l - 1][k]);
+        f4v acc = l == 0 ? acc0[k & 7] : (l == 5 ? acc5 : acc14[(l - 1) & 3][k & 3]);
 #pragma unroll
         for (int kb = 0; kb < 2; ++kb) {
           const Frag8 a = load8(&sdz[(ob * 16 + r16) * kTS + kb * 32 + 8 * g]);
@@ -412,11 +506,11 @@ __global__ void __launch_bounds__(256) k_nrc_grad(NrcDevice d, int n, const uint
           acc = mfma<BF16>(a, b, acc);
         }
         if (l == 0)
-          acc0[k] = acc;
+          acc0[k & 7] = acc;
         else if (l == 5)
           acc5 = acc;
         else
-          acc14[l - 1][k] = acc;
+          acc14[(l - 1) & 3][k & 3] = acc;
       }
       if (l > 0) {
         // dX^T = W^T dZ^T (k = output of layer l), then the ReLU mask of
@@ -424,7 +518,7 @@ __global__ void __launch_bounds__(256) k_nrc_grad(NrcDevice d, int n, const uint
         // kperm order; the output layer's 16: one 16x16x16 step
         for (int ib = 0; ib < nib; ++ib) {
           f4v acc = f4v{0.f, 0.f, 0.f, 0.f};
-          const uint16_t *wt = &d.wt[nrc_woff(l) + (ib * 16 + r16) * nrc_out(l)];
+          const uint16_t *wt = &swt[gw_off(l) + (ib * 16 + r16) * gwt_stride(l)];
           if (nob == 1) {
             Frag a;
             a.v = *reinterpret_cast<const uint2 *>(wt + 4 * g);
@@ -443,35 +537,24 @@ __global__ void __launch_bounds__(256) k_nrc_grad(NrcDevice d, int n, const uint
     }
   }
 
-  // ---- write this workgroup's partial gradient ---------------------------------
-  float *part = d.partials + size_t(blockIdx.x) * kNrcParams;
+  // ---- write this workgroup's partial gradient and loss -----------------------
+  float *part = d.partials + size_t(blockIdx.x) * kNrcPartStride;
   auto put = [&](int l, int t, const f4v &acc) {
     const int nib = nrc_in(l) / 16;
     const int ob = t / nib, ib = t % nib;
     for (int j = 0; j < 4; ++j) part[nrc_w(l, ob * 16 + 4 * g + j, ib * 16 + r16)] = acc[j];
   };
-  for (int t = wave, k = 0; t < 32; t += 4, ++k) put(0, t, acc0[k]);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) put(0, wave + 4 * k, acc0[k]);
+#pragma unroll
   for (int l = 1; l <= 4; ++l)
-    for (int t = wave, k = 0; t < 16; t += 4, ++k) put(l, t, acc14[l - 1][k]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) put(l, wave + 4 * k, acc14[l - 1][k]);
   put(5, wave, acc5);
   __syncthreads();
-  for (int i = threadIdx.x; i < kNrcBiases; i += blockDim.x) part[kNrcWeights + i] = sdb[i];
-  if (threadIdx.x == 0) atomicAdd(&d.stats[0], sloss[0] + sloss[1] + sloss[2] + sloss[3]);
-}
-
-// fixed-order sum of the workgroup partials: a workgroup owns 64 parameters;
-// its 4 waves sum interleaved quarters of the partials (64 consecutive floats
-// per load), then combine in LDS in a fixed order
-__global__ void __launch_bounds__(256) k_nrc_reduce(NrcDevice d, int parts) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int p = blockIdx.x * 64 + lane;
-  float s = 0.f;
-  if (p < kNrcParams)
-    for (int i = wave; i < parts; i += 4) s += d.partials[size_t(i) * kNrcParams + p];
-  __shared__ float sh[4][64];
-  sh[wave][lane] = s;
-  __syncthreads();
-  if (wave == 0 && p < kNrcParams) d.grad[p] = (sh[0][lane] + sh[1][lane]) + (sh[2][lane] + sh[3][lane]);
+  for (int i = threadIdx.x; i < kNrcBiases; i += blockDim.x)
+    part[kNrcWeights + i] = (sdb[0][i] + sdb[1][i]) + (sdb[2][i] + sdb[3][i]);
+  if (threadIdx.x == 0) part[kNrcParams] = (sloss[0] + sloss[1]) + (sloss[2] + sloss[3]);
 }
 
 __device__ inline void emit16(const NrcDevice &d, int p, float v) {
@@ -490,29 +573,52 @@ __device__ inline void emit16(const NrcDevice &d, int p, float v) {
   }
 }
 
-// The step count of Adam's bias correction, kept on the device in
-// d.stats[2]: it advances only for a mini-batch that holds samples (a
-// frame's sample count is known only on the device), before the update
-// that uses it
-__global__ void k_nrc_adam_count(NrcDevice d, int n, const uint32_t *n_dev, int batch,
-                                 int nbatches) {
+// Fixed-order sum of the workgroup partials: a workgroup owns 64 entries
+// (parameters, and at kNrcParams the loss); its 4 waves sum interleaved
+// quarters of the partials (64 consecutive floats per load), then combine in
+// LDS in a fixed order.  With ADAM the sum is the gradient of an Adam step
+// (Kingma & Ba 2015, bias-corrected) on the fp32 master parameters, which
+// then re-emits the 16-bit copies; the step count was advanced by the
+// gradient launch, and an empty mini-batch changes nothing.
+template <int ADAM>
+__global__ void __launch_bounds__(256) k_nrc_reduce(NrcDevice d, int parts, int n,
+                                                    const uint32_t *n_dev, int batch,
+                                                    int nbatches, float lr, float b1, float b2,
+                                                    float eps) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int p = blockIdx.x * 64 + lane;
+  float s = 0.f;
+  if (p <= kNrcParams) {
+    // batches of 16 loads in flight, summed in partial order
+    const float *src = d.partials + p;
+    int i = wave;
+    for (; i + 4 * 15 < parts; i += 4 * 16) {
+      float v[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) v[k] = src[size_t(i + 4 * k) * kNrcPartStride];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) s += v[k];
+    }
+    for (; i < parts; i += 4) s += src[size_t(i) * kNrcPartStride];
+  }
+  __shared__ float sh[4][64];
+  sh[wave][lane] = s;
+  __syncthreads();
+  if (wave != 0 || p > kNrcParams) return;
+  const float sum = (sh[0][lane] + sh[1][lane]) + (sh[2][lane] + sh[3][lane]);
+  if (p == kNrcParams) {
+    d.stats[0] = sum;
+    return;
+  }
+  d.grad[p] = sum;
+  if (!ADAM) return;
   int lo, cnt;
   batch_range(n, n_dev, batch, nbatches, lo, cnt);
-  if (cnt > 0 && threadIdx.x == 0) d.stats[2] += 1.f;
-}
-
-// Adam on the fp32 master parameters (Kingma & Ba 2015, bias-corrected)
-__global__ void k_nrc_adam(NrcDevice d, int n, const uint32_t *n_dev, int batch, int nbatches,
-                           float lr, float b1, float b2, float eps) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= kNrcParams) return;
-  int lo, cnt;
-  batch_range(n, n_dev, batch, nbatches, lo, cnt);
-  if (cnt <= 0) return;  // empty mini-batch: no step
+  if (cnt <= 0) return;
   const float step = d.stats[2];
   const float c1 = 1.f / (1.f - powf(b1, step));
   const float c2 = 1.f / (1.f - powf(b2, step));
-  const float gr = d.grad[p] * (1.f / float(cnt));
+  const float gr = sum * (1.f / float(cnt));
   const float m = b1 * d.adam_m[p] + (1.f - b1) * gr;
   const float v = b2 * d.adam_v[p] + (1.f - b2) * gr * gr;
   d.adam_m[p] = m;
@@ -597,33 +703,42 @@ hipError_t nrc_launch_infer(const NrcDevice &d, int n, const float *raw, float *
   return hipGetLastError();
 }
 
-hipError_t nrc_launch_gradients(const NrcDevice &d, int n, const float *raw, const float *target,
-                                int max_workgroups, hipStream_t s, const uint32_t *n_dev,
-                                int batch, int nbatches) {
-  hipError_t e = hipMemsetAsync(d.stats, 0, 2 * sizeof(float), s);
-  if (e != hipSuccess) return e;
-  // workgroup count from the largest possible mini-batch; every workgroup
-  // writes its partial (zeros when it got no chunk)
+// The gradient pass of mini-batch `batch`, then the fixed-order reduction
+// (ADAM: fused with the Adam step).  The workgroup count comes from the
+// largest possible mini-batch; every workgroup writes its partial (zeros
+// when it got no chunk).
+template <int ADAM>
+static hipError_t launch_grad_reduce(const NrcDevice &d, int n, const float *raw,
+                                     const float *target, int max_workgroups, float lr, float b1,
+                                     float b2, float eps, hipStream_t s, const uint32_t *n_dev,
+                                     int batch, int nbatches) {
   const int chunks = (n / std::max(1, nbatches) + 1 + 63) / 64;
   const int parts = std::max(1, std::min(chunks, max_workgroups));
   if (d.bf16)
     hipLaunchKernelGGL(k_nrc_grad<1>, dim3(parts), dim3(256), 0, s, d, n, n_dev, batch, nbatches,
-                       raw, target);
+                       ADAM, raw, target);
   else
     hipLaunchKernelGGL(k_nrc_grad<0>, dim3(parts), dim3(256), 0, s, d, n, n_dev, batch, nbatches,
-                       raw, target);
-  e = hipGetLastError();
+                       ADAM, raw, target);
+  hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_nrc_reduce, dim3((kNrcParams + 63) / 64), dim3(256), 0, s, d, parts);
+  hipLaunchKernelGGL(k_nrc_reduce<ADAM>, dim3((kNrcParams + 1 + 63) / 64), dim3(256), 0, s, d,
+                     parts, n, n_dev, batch, nbatches, lr, b1, b2, eps);
   return hipGetLastError();
 }
 
-hipError_t nrc_launch_adam(const NrcDevice &d, int n, float lr, float b1, float b2, float eps,
-                           hipStream_t s, const uint32_t *n_dev, int batch, int nbatches) {
-  hipLaunchKernelGGL(k_nrc_adam_count, dim3(1), dim3(64), 0, s, d, n, n_dev, batch, nbatches);
-  hipLaunchKernelGGL(k_nrc_adam, dim3((kNrcParams + 255) / 256), dim3(256), 0, s, d, n, n_dev,
-                     batch, nbatches, lr, b1, b2, eps);
-  return hipGetLastError();
+hipError_t nrc_launch_gradients(const NrcDevice &d, int n, const float *raw, const float *target,
+                                int max_workgroups, hipStream_t s, const uint32_t *n_dev,
+                                int batch, int nbatches) {
+  return launch_grad_reduce<0>(d, n, raw, target, max_workgroups, 0.f, 0.f, 0.f, 0.f, s, n_dev,
+                               batch, nbatches);
+}
+
+hipError_t nrc_launch_train_step(const NrcDevice &d, int n, const float *raw, const float *target,
+                                 int max_workgroups, float lr, float b1, float b2, float eps,
+                                 hipStream_t s, const uint32_t *n_dev, int batch, int nbatches) {
+  return launch_grad_reduce<1>(d, n, raw, target, max_workgroups, lr, b1, b2, eps, s, n_dev,
+                               batch, nbatches);
 }
 
 hipError_t nrc_launch_mfma_probe(const float *a, const float *b, float *c, int bf16,
