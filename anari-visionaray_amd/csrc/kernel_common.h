@@ -107,6 +107,10 @@ struct KSurface {
   const float4 *tris;   // {v1, prim id bits}, e1, e2 per triangle
   f3 color;             // matte
   uint32_t obj_id, inst_id;
+  int num_nodes = 0;
+  // nodes [0, lds_nodes) -- breadth-first, so the top levels -- are read
+  // from the workgroup's LDS copy at float4 lds_first of the staged region
+  int lds_nodes = 0, lds_first = 0;
 };
 
 // An instanced mesh (vrt_instance) on the device

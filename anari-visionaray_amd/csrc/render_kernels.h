@@ -49,9 +49,13 @@ hipError_t launch_majorants(const KVolume &v, const float *ranges, float *maj, u
                             float *step, hipStream_t s);
 // LDS bytes a render kernel needs for the scene's tables and majorant grids
 size_t scene_lds_size(const KScene &sc);
-// LDS bytes of the path tracer: the above plus the BVH traversal stacks when
-// the scene has surfaces or quad lights
+// LDS bytes of the path tracer: the above plus the BVH traversal stacks and
+// the staged BVH nodes when the scene has surfaces or quad lights
 size_t pathtrace_lds_size(const KScene &sc);
+// sets the BVH nodes each workgroup stages in LDS (sc.surf[i].lds_nodes):
+// as many top nodes as fit beside `base` bytes per workgroup without
+// lowering the `per_cu` resident workgroups
+void plan_staged_nodes(KScene &sc, size_t base, int per_cu);
 hipError_t launch_frame_reset(const KFrame &F, hipStream_t s);
 hipError_t launch_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 hipError_t launch_sample_field(const KVolume &v, int n, const float *pts, float *out,

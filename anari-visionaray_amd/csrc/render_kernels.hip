@@ -522,6 +522,33 @@ __host__ __device__ constexpr size_t bvh_lds_stack_bytes() {
   return size_t(kBvhLdsStack) * kBvhLanes * sizeof(int2);
 }
 
+// BVH nodes staged in LDS: every ray enters at the root, so the top levels
+// of each mesh's tree (the first nodes of its breadth-first array) are read
+// by all lanes; a workgroup copies them once and reads them from LDS.
+constexpr int kMaxStagedNodes = 256;  // 32 KB per workgroup at most
+constexpr size_t kLdsBytesPerCu = 160 * 1024;
+__host__ __device__ inline size_t staged_nodes_bytes(const KScene &sc) {
+  size_t n = 0;
+  for (int i = 0; i < sc.num_surfaces; ++i) n += size_t(sc.surf[i].lds_nodes);
+  return n * 8 * sizeof(float4);
+}
+// the staged nodes follow the tables and the traversal stacks
+__host__ __device__ inline size_t staged_nodes_offset(const KScene &sc) {
+  return scene_lds_bytes(sc) + bvh_lds_stack_bytes();
+}
+__device__ inline const float4 *staged_nodes(const KScene &sc, const float4 *lds) {
+  return reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(lds) +
+                                          staged_nodes_offset(sc));
+}
+// copy of the staged nodes (the caller's barrier publishes it)
+__device__ inline void stage_nodes(const KScene &sc, float4 *lds) {
+  float4 *dst = const_cast<float4 *>(staged_nodes(sc, lds));
+  for (int si = 0; si < sc.num_surfaces; ++si) {
+    const KSurface &S = sc.surf[si];
+    for (int i = threadIdx.x; i < 8 * S.lds_nodes; i += blockDim.x) dst[S.lds_first + i] = S.nodes[i];
+  }
+}
+
 // compare-exchange of (node, entry distance) pairs by distance
 __device__ inline void cswap(float &ta, int &na, float &tb, int &nb) {
   if (tb < ta) {
@@ -542,9 +569,11 @@ __device__ inline void cswap(float &ta, int &na, float &tb, int &nb) {
 // child box that was hit and stacks the others (node, entry distance); the
 // stack's first kBvhLdsStack entries live in LDS (`lst`: this lane's slot,
 // entries kBvhLanes apart, so the lanes of a wave hit distinct banks), the
-// rest in private memory, which only deep traversals reach.
+// rest in private memory, which only deep traversals reach.  Nodes below
+// S.lds_nodes are read from the workgroup's staged copy (`lnodes`).
 __device__ inline bool bvh_hit(const KSurface &S, const KRay &r, bool any, float &tbest,
-                               int &prim, uint32_t &rec, int2 *lst) {
+                               int &prim, uint32_t &rec, int2 *lst,
+                               const float4 *lnodes = nullptr) {
   f3 inv;
   {
     const float dx = fabsf(r.dir.x) < 1e-20f ? copysignf(1e-20f, r.dir.x) : r.dir.x;
@@ -558,9 +587,18 @@ __device__ inline bool bvh_hit(const KSurface &S, const KRay &r, bool any, float
   bool found = false;
   const float inf = __builtin_inff();
   while (true) {
-    const float4 *n = S.nodes + size_t(node) * 8u;
-    const float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5];
-    const float4 chf = n[6], cnf = n[7];
+    float4 nd[8];
+    if (node < S.lds_nodes) {
+      const float4 *n = lnodes + S.lds_first + 8 * node;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) nd[k] = n[k];
+    } else {
+      const float4 *n = S.nodes + size_t(node) * 8u;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) nd[k] = n[k];
+    }
+    const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
+    const float4 chf = nd[6], cnf = nd[7];
     const float lo_x[4] = {lx.x, lx.y, lx.z, lx.w}, hi_x[4] = {hx.x, hx.y, hx.z, hx.w};
     const float lo_y[4] = {ly.x, ly.y, ly.z, ly.w}, hi_y[4] = {hy.x, hy.y, hy.z, hy.w};
     const float lo_z[4] = {lz.x, lz.y, lz.z, lz.w}, hi_z[4] = {hz.x, hz.y, hz.z, hz.w};
@@ -1445,7 +1483,7 @@ __device__ inline void raycast_pixel(const KScene &sc, const KCamera &cam, const
           float t = st;
           uint32_t rec;
           int p;
-          if (bvh_hit(sc.surf[i], ray, false, t, p, rec, bstack)) {
+          if (bvh_hit(sc.surf[i], ray, false, t, p, rec, bstack, staged_nodes(sc, s_tf))) {
             st = t;
             sid = i;
             srec = rec;
@@ -1608,7 +1646,8 @@ __global__ void __launch_bounds__(256)
   extern __shared__ float4 s_tf[];
   __shared__ float2 s_pow[64];  // pow_ieee's log2 table
   if (threadIdx.x < 64) s_pow[threadIdx.x] = kPowLog2[threadIdx.x];
-  stage_tf(sc, s_tf);  // (its barrier also publishes s_pow)
+  if (SURF) stage_nodes(sc, s_tf);
+  stage_tf(sc, s_tf);  // (its barrier also publishes s_pow and the nodes)
   // clear the counters the next launch will use (the current ones were
   // cleared by the previous launch)
   if (blockIdx.x == 0 && threadIdx.x < kQueues) queues_next[threadIdx.x] = 0u;
@@ -1777,7 +1816,7 @@ template <bool AREA, bool ANYF, bool INST>
 __device__ inline PassHit intersect_all(const KScene &sc, const KRay &ray, bool shadow,
                                         int bounceID, const float4 *s_tf, const float *steps,
                                         uint32_t &srng, uint32_t &lookups, int2 *bstack,
-                                        bool ratio = false) {
+                                        const float4 *lnodes, bool ratio = false) {
   bool sh = false;
   float st = ray.tmax;
   int sid = -1;
@@ -1786,7 +1825,7 @@ __device__ inline PassHit intersect_all(const KScene &sc, const KRay &ray, bool 
     float t = st;
     int prim = -1;
     uint32_t rec = 0;
-    if (bvh_hit(sc.surf[i], ray, shadow, t, prim, rec, bstack)) {
+    if (bvh_hit(sc.surf[i], ray, shadow, t, prim, rec, bstack, lnodes)) {
       sh = true;
       st = t;
       sid = i;
@@ -2225,7 +2264,8 @@ __global__ void __launch_bounds__(256, 3)
     k_pathtrace(KScene sc, KCamera cam, KParams P, KFrame F, int accumID, uint32_t *queues,
                 uint32_t *queues_next, KNrc N) {
   extern __shared__ float4 s_tf[];
-  stage_tf(sc, s_tf);
+  if (AREA) stage_nodes(sc, s_tf);
+  stage_tf(sc, s_tf);  // (its barrier also publishes the nodes)
   // clear the counters the next launch will use (the current ones were
   // cleared by the previous launch)
   if (blockIdx.x == 0 && threadIdx.x < kQueues) queues_next[threadIdx.x] = 0u;
@@ -2251,6 +2291,7 @@ __global__ void __launch_bounds__(256, 3)
                                                  scene_lds_bytes(sc)) +
                             threadIdx.x
                       : nullptr;
+  const float4 *lnodes = AREA ? staged_nodes(sc, s_tf) : nullptr;
 
   PathState st;
   st.pixel = -1;
@@ -2315,7 +2356,7 @@ __global__ void __launch_bounds__(256, 3)
       clip_ray(ray, P);  // every pass (Pathtrace_impl.cpp:559)
       const bool shadow = st.rayType != Radiance;
       const PassHit hit = intersect_all<AREA, ANYF, INST>(sc, ray, shadow, st.bounceID, s_tf, steps,
-                                                    st.srng, lookups, bstack,
+                                                    st.srng, lookups, bstack, lnodes,
                                                     shadow && P.ratio_tracking != 0);
       if (path_shade<NRC, AREA, ANYF, GRAD>(st, sc, P, N, F, hit)) {
         float4 color = P.bg;
@@ -2762,7 +2803,11 @@ __global__ void __launch_bounds__(256)
     k_intersect_surfaces(KScene sc, int n, const float *__restrict__ rays, int any,
                          float *__restrict__ t_out, int *__restrict__ prim_out,
                          int *__restrict__ surf_out) {
-  extern __shared__ int2 s_bstack[];
+  extern __shared__ float4 s_tf[];  // [tables, unused here][stacks][staged nodes]
+  stage_nodes(sc, s_tf);
+  __syncthreads();
+  int2 *s_bstack = reinterpret_cast<int2 *>(reinterpret_cast<char *>(s_tf) + scene_lds_bytes(sc));
+  const float4 *lnodes = staged_nodes(sc, s_tf);
   const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
   if (i >= n) return;
   const float *r = rays + 8 * size_t(i);
@@ -2778,7 +2823,7 @@ __global__ void __launch_bounds__(256)
     float t = best;
     int p = -1;
     uint32_t rec;
-    if (bvh_hit(sc.surf[s], ray, any != 0, t, p, rec, s_bstack + threadIdx.x)) {
+    if (bvh_hit(sc.surf[s], ray, any != 0, t, p, rec, s_bstack + threadIdx.x, lnodes)) {
       hit = true;
       best = t;
       prim = p;
@@ -2826,7 +2871,7 @@ static bool scene_has_other_fields(const KScene &sc);
 hipError_t launch_raycast(const KScene &sc, const KCamera &cam, const KParams &P, const KFrame &F,
                           uint32_t *queues, uint32_t *queues_next, int num_cus, hipStream_t s) {
   const bool surf = sc.num_surfaces > 0 || sc.tls.num_inst > 0;
-  const size_t lds = scene_lds_bytes(sc) + (surf ? bvh_lds_stack_bytes() : 0);
+  const size_t lds0 = scene_lds_bytes(sc) + (surf ? bvh_lds_stack_bytes() : 0);
   const bool any = scene_has_other_fields(sc), shade = P.gradient_shading != 0;
   using RaycastKernel = void (*)(KScene, KCamera, KParams, KFrame, uint32_t *, uint32_t *);
   static const RaycastKernel table[12] = {
@@ -2842,13 +2887,16 @@ hipError_t launch_raycast(const KScene &sc, const KCamera &cam, const KParams &P
       table[(inst ? 8 : surf ? 4 : 0) + (shade ? 2 : 0) + (any ? 1 : 0)];
   // one resident wave per SIMD slot, no more workgroups than tiles need
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, lds) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, lds0) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
+  KScene scs = sc;
+  if (surf) plan_staged_nodes(scs, lds0, per_cu);
+  const size_t lds = lds0 + staged_nodes_bytes(scs);
   const int ntiles = ((F.width + 7) / 8) * ((F.height + 7) / 8);
   int blocks = std::min(std::max(1, num_cus) * per_cu, (ntiles + 3) / 4);
   blocks = std::max(8, (blocks + 7) / 8 * 8);  // every XCD's queue gets workgroups
-  hipLaunchKernelGGL(k, dim3(blocks), dim3(256), lds, s, sc, cam, P, F, queues, queues_next);
+  hipLaunchKernelGGL(k, dim3(blocks), dim3(256), lds, s, scs, cam, P, F, queues, queues_next);
   return hipGetLastError();
 }
 
@@ -2894,7 +2942,27 @@ static PathtraceKernel pathtrace_kernel(const KScene &sc, bool nrc, bool grad) {
 }
 
 size_t pathtrace_lds_size(const KScene &sc) {
-  return scene_lds_bytes(sc) + (scene_has_area(sc) ? bvh_lds_stack_bytes() : 0);
+  return scene_lds_bytes(sc) +
+         (scene_has_area(sc) ? bvh_lds_stack_bytes() + staged_nodes_bytes(sc) : 0);
+}
+
+// The top BVH nodes each workgroup stages: the first nodes of surface 0's
+// breadth-first array, then surface 1's, ..., as many as fit beside `base`
+// LDS bytes per workgroup without lowering the kernel's `per_cu` resident
+// workgroups (1 KB kept for allocation granularity), at most kMaxStagedNodes.
+void plan_staged_nodes(KScene &sc, size_t base, int per_cu) {
+  for (int i = 0; i < sc.num_surfaces; ++i) sc.surf[i].lds_nodes = sc.surf[i].lds_first = 0;
+  const size_t per_wg = kLdsBytesPerCu / size_t(std::max(1, per_cu));
+  if (per_wg <= base + 1024) return;
+  size_t room = std::min((per_wg - base - 1024) / (8 * sizeof(float4)), size_t(kMaxStagedNodes));
+  int first = 0;
+  for (int i = 0; i < sc.num_surfaces && room > 0; ++i) {
+    const int k = int(std::min(room, size_t(std::max(0, sc.surf[i].num_nodes))));
+    sc.surf[i].lds_nodes = k;
+    sc.surf[i].lds_first = first;
+    first += 8 * k;
+    room -= size_t(k);
+  }
 }
 
 hipError_t launch_pathtrace(const KScene &sc, const KCamera &cam, const KParams &P,
@@ -3037,8 +3105,17 @@ hipError_t launch_sample_field(const KVolume &v, int n, const float *pts, float 
 hipError_t launch_intersect_surfaces(const KScene &sc, int n, const float *rays, int any,
                                      float *t_out, int *prim_out, int *surf_out, hipStream_t s) {
   if (n <= 0) return hipSuccess;
+  const size_t lds0 = staged_nodes_offset(sc);
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_intersect_surfaces, kBvhLanes,
+                                                   lds0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  KScene scs = sc;
+  plan_staged_nodes(scs, lds0, per_cu);
   hipLaunchKernelGGL(k_intersect_surfaces, dim3((n + 255) / 256), dim3(kBvhLanes),
-                     bvh_lds_stack_bytes(), s, sc, n, rays, any, t_out, prim_out, surf_out);
+                     lds0 + staged_nodes_bytes(scs), s, scs, n, rays, any, t_out, prim_out,
+                     surf_out);
   return hipGetLastError();
 }
 

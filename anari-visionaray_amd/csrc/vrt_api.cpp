@@ -254,6 +254,7 @@ int make_scene(const vrt_scene *s, KScene &k) {
     if (!S.bvh) return fail(VRT_ERROR_INVALID_ARGUMENT, "surface %d has no bvh", i);
     k.surf[i].nodes = S.bvh->nodes;
     k.surf[i].tris = S.bvh->tris;
+    k.surf[i].num_nodes = int(S.bvh->num_nodes);
     k.surf[i].color = mk(S.color[0], S.color[1], S.color[2]);
     k.surf[i].obj_id = S.object_id;
     k.surf[i].inst_id = S.instance_id;
@@ -743,8 +744,9 @@ int pathtrace_frame(vrt_context ctx, const vrt_scene *scene, const vrt_camera *c
     }
   }
   const size_t lds = pathtrace_lds_size(sc);
-  int blocks = std::max(1, ctx->num_cus) * pathtrace_blocks_per_cu(sc, lds, false,
-                                                                    P.gradient_shading != 0);
+  const int per_cu = pathtrace_blocks_per_cu(sc, lds, false, P.gradient_shading != 0);
+  plan_staged_nodes(sc, lds, per_cu);  // the top BVH nodes that fit beside it
+  int blocks = std::max(1, ctx->num_cus) * per_cu;
   blocks = std::max(8, (blocks + 7) / 8 * 8);  // every XCD queue gets workgroups
   if (static_queues) {
     VRT_HIP(launch_pathtrace_static(sc, cam, P, F, accum_id, ctx->static_queues, blocks,
@@ -846,8 +848,9 @@ int vrt_pathtrace_nrc(vrt_context ctx, const vrt_scene *scene, const vrt_camera 
                    1.f / (opts->bounds_hi[2] - opts->bounds_lo[2]));
 
   const size_t lds = pathtrace_lds_size(sc);
-  int blocks = std::max(1, ctx->num_cus) * pathtrace_blocks_per_cu(sc, lds, true,
-                                                                    P.gradient_shading != 0);
+  const int per_cu = pathtrace_blocks_per_cu(sc, lds, true, P.gradient_shading != 0);
+  plan_staged_nodes(sc, lds, per_cu);
+  int blocks = std::max(1, ctx->num_cus) * per_cu;
   blocks = std::max(8, (blocks + 7) / 8 * 8);
   uint32_t *cur = ctx->queues + 8 * ctx->queue_parity;
   uint32_t *nxt = ctx->queues + 8 * (1 - ctx->queue_parity);

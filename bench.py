@@ -286,6 +286,54 @@ NRC_BWD_MACS = (64 * 3 + 4 * 64 * 64) + NRC_FWD_MACS
 MFMA_PEAK_TFLOPS = {"fp16": 2500.0, "bf16": 2500.0}  # MI355X dense
 
 
+def traversal_rates(pkg, ctx, n=1 << 21, reps=5, builder="sah"):
+    """BVH traversal alone on the mixed scene's 1,310,720-triangle icosphere:
+    n incoherent rays from points inside the sphere (uniform in a ball of
+    radius 0.3, uniform directions), closest hit; and n rays from the same
+    points towards the quad light's area, any hit and closest hit, through
+    vrt_intersect_surfaces."""
+    import ctypes as C
+    import numpy as np
+    vrt = pkg.vrt
+    v, i = vrt.gen_icosphere(level=8, radius=0.5, amp=0.08, freq=3.0, seed=2)
+    mesh = vrt.DeviceMesh(ctx, v, i, builder=builder)
+    scene = vrt.make_scene([], [], [(mesh, (0.7, 0.7, 0.7))])
+    rng = np.random.default_rng(1)
+    o = rng.normal(size=(n, 3))
+    o *= 0.3 * rng.uniform(0, 1, (n, 1)) ** (1 / 3) / np.linalg.norm(o, axis=1, keepdims=True)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3], rays[:, 3:6], rays[:, 6], rays[:, 7] = o, d, 1e-4, 1e30
+    tgt = np.array([0.0, 0.36, 0.0]) + rng.uniform(-0.08, 0.08, (n, 3)) * np.array([1, 0, 1])
+    sd = tgt - o
+    dist = np.linalg.norm(sd, axis=1)
+    srays = rays.copy()
+    srays[:, 3:6], srays[:, 7] = sd / dist[:, None], dist * 0.999
+    dr, ds = ctx.upload(rays), ctx.upload(srays)
+    dt, dp = ctx.buffer(4 * n), ctx.buffer(4 * n)
+    lib = vrt.lib()
+    out = {"triangles": len(i), "bvh_builder": builder, "rays": n,
+           "bvh_nodes": int(mesh.info.num_nodes)}
+    for name, buf, anyh in (("closest", dr, 0), ("light_any", ds, 1), ("light_closest", ds, 0)):
+        def run():
+            vrt.check(lib.vrt_intersect_surfaces(ctx.handle, C.byref(scene), n, buf.ptr, anyh,
+                                                 dt.ptr, dp.ptr, None))
+        run()
+        ctx.sync()
+        ctx.timer_start()
+        for _ in range(reps):
+            run()
+        ms = ctx.timer_stop() / reps
+        hit = float((dp.download(np.int32, (n,)) >= 0).mean())
+        out[name] = {"ms": round(ms, 4), "Mrays_per_s": round(n / ms / 1e3, 1),
+                     "hit_fraction": round(hit, 4)}
+    for b in (dr, ds, dt, dp):
+        b.free()
+    mesh.close()
+    return out
+
+
 def mfma_rates(pkg, ctx, infer_n=1 << 21, train_n=65536, reps=20):
     """The cache's MFMA kernels alone, on device-resident random inputs:
     inference over infer_n samples (a 1080p frame's worth of queries) and the
@@ -334,7 +382,7 @@ def mfma_rates(pkg, ctx, infer_n=1 << 21, train_n=65536, reps=20):
 
 KERNEL_WORKLOADS = ("pathtrace_fbm512", "raycast_blobs512", "raycast_blobs128",
                     "pathtrace_cloud1024", "nrc_cloud1024", "pathtrace_tets2m", "nrc_tets2m",
-                    "nrc_mfma")
+                    "nrc_mfma", "pathtrace_mixed", "bvh_mixed")
 
 
 def kernel_only(pkg, ctx, name, steps, warmup, precision="fp16", query_vertex=2,
@@ -346,6 +394,10 @@ def kernel_only(pkg, ctx, name, steps, warmup, precision="fp16", query_vertex=2,
         print(json.dumps({"kernel_workload": name, **mfma_rates(pkg, ctx, reps=steps)}),
               flush=True)
         return
+    if name == "bvh_mixed":
+        print(json.dumps({"kernel_workload": name, **traversal_rates(pkg, ctx, reps=steps)}),
+              flush=True)
+        return
     if name == "pathtrace_fbm512":
         wl = wls.fbm_pathtrace(n=512, phase_g=PHASE_G)
     elif name == "raycast_blobs512":
@@ -354,6 +406,8 @@ def kernel_only(pkg, ctx, name, steps, warmup, precision="fp16", query_vertex=2,
         wl = wls.blobs_raycast(n=128)
     elif name in ("pathtrace_cloud1024", "nrc_cloud1024"):
         wl = wls.cloud_nanovdb_pathtrace(n=1024, phase_g=PHASE_G)
+    elif name == "pathtrace_mixed":
+        wl = wls.mixed_pathtrace(level=8, vol_n=256)
     else:
         wl = wls.tets_pathtrace()
     gs = wls.GpuScene(ctx, wl)
