@@ -170,6 +170,12 @@ struct KParams {
   float phase_g;
   int gradient_shading;
   int ratio_tracking;  // shadow / AO transmittance by ratio tracking
+  // scheduling only (images do not change): a lane that continues its
+  // bounce with shadow / AO rays traces them in the next passes while the
+  // other lanes of its wave wait, and idle lanes are refilled between
+  // bounces; 0: every lane traces one ray per pass, idle lanes refilled
+  // before each pass
+  int chain_passes;
   // temporal anti-aliasing: previous and current view-projection (row-major
   // proj * view), blend weight
   int taa;

@@ -2301,9 +2301,14 @@ __global__ void __launch_bounds__(256, 3)
   uint32_t pool_base = 0, pool_left = 0;    // wave-uniform local pool
   int px = 0, py = 0;
 
+  // chain: this lane's bounce continues with its shadow / AO rays; while any
+  // lane of the wave has such a ray (P.chain_passes), the pass is theirs and
+  // nothing is refilled
+  bool chain = false;
   while (true) {
+    const bool chained = __ballot(chain) != 0ull;
     // ---- refill idle lanes ----------------------------------------------------
-    unsigned long long idle = __ballot(st.pixel < 0);
+    unsigned long long idle = chained ? 0ull : __ballot(st.pixel < 0);
     while (idle && qtries < kQueues) {
       if (pool_left == 0) {
         // pixels of queue q: the chunks c = q, q+8, ... times 1024 pixels
@@ -2339,15 +2344,16 @@ __global__ void __launch_bounds__(256, 3)
       idle = __ballot(st.pixel < 0);
       if (take == nidle) break;  // every idle lane was offered a pixel
     }
-    if (__ballot(st.pixel >= 0) == 0ull) {
+    if (!chained && __ballot(st.pixel >= 0) == 0ull) {
       // no lane holds a pixel: either every queue is drained, or the last
       // pool only covered padding pixels outside the image -- claim again
       if (qtries >= kQueues) break;
       continue;
     }
 
-    // ---- one pass for every active lane ---------------------------------------
-    if (st.pixel >= 0) {
+    // ---- one pass for every active lane (or every chained one) -----------------
+    if (st.pixel >= 0 && (!chained || chain)) {
+      chain = false;
       KRay ray;
       ray.ori = st.ori;
       ray.dir = st.dir;
@@ -2407,6 +2413,8 @@ __global__ void __launch_bounds__(256, 3)
           }
           st.pixel = -1;
         }
+      } else {
+        chain = P.chain_passes && st.rayType != Radiance;
       }
     }
   }
@@ -2939,6 +2947,17 @@ static PathtraceKernel pathtrace_kernel(const KScene &sc, bool nrc, bool grad) {
     return inst_table[(grad ? 4 : 0) + (nrc ? 2 : 0) + (scene_has_other_fields(sc) ? 1 : 0)];
   return table[(grad ? 8 : 0) + (nrc ? 4 : 0) + (scene_has_area(sc) ? 2 : 0) +
                (scene_has_other_fields(sc) ? 1 : 0)];
+}
+
+// Pass scheduling of the path tracer: the passes of a bounce run back to back
+// (KParams::chain_passes) except on tetrahedral fields.  Measured frame
+// times, chained vs one ray per pass: fBm 512^3 1.215 vs 1.270 ms, mixed
+// surfaces + volume 18.4 vs 19.4 ms, NanoVDB cloud 3.48 vs 3.53 ms,
+// 2.06 M tetrahedra at 3840x2160 22.3 vs 20.3 ms.
+bool pathtrace_chain_passes(const KScene &sc) {
+  for (int i = 0; i < sc.num_volumes; ++i)
+    if (sc.vol[i].field_type == VRT_FIELD_UNSTRUCTURED) return false;
+  return true;
 }
 
 size_t pathtrace_lds_size(const KScene &sc) {
