@@ -107,10 +107,14 @@ struct KSurface {
   const float4 *tris;   // {v1, prim id bits}, e1, e2 per triangle
   f3 color;             // matte
   uint32_t obj_id, inst_id;
-  int num_nodes = 0;
-  // nodes [0, lds_nodes) -- breadth-first, so the top levels -- are read
-  // from the workgroup's LDS copy at float4 lds_first of the staged region
-  int lds_nodes = 0, lds_first = 0;
+};
+
+// A surface's BVH nodes staged in LDS: nodes [0, lds_nodes) -- breadth-
+// first, so the top levels -- are read from the workgroup's copy at float4
+// lds_first of the staged region (kept apart from KSurface, which shading
+// copies per lane)
+struct KStaged {
+  int num_nodes, lds_nodes, lds_first, pad;
 };
 
 // An instanced mesh (vrt_instance) on the device
@@ -144,6 +148,7 @@ struct KScene {
   KVolume vol[VRT_MAX_VOLUMES];
   KLight light[VRT_MAX_LIGHTS];
   KSurface surf[VRT_MAX_SURFACES];
+  KStaged staged[VRT_MAX_SURFACES];
   KTls tls;
   int hdri_light;  // index of the HDRI light, -1 for none
   KHdri hdri;

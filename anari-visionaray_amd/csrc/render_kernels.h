@@ -54,7 +54,7 @@ size_t scene_lds_size(const KScene &sc);
 size_t pathtrace_lds_size(const KScene &sc);
 // the path tracer's pass scheduling for the scene (KParams::chain_passes)
 bool pathtrace_chain_passes(const KScene &sc);
-// sets the BVH nodes each workgroup stages in LDS (sc.surf[i].lds_nodes):
+// sets the BVH nodes each workgroup stages in LDS (sc.staged[i].lds_nodes):
 // as many top nodes as fit beside `base` bytes per workgroup without
 // lowering the `per_cu` resident workgroups
 void plan_staged_nodes(KScene &sc, size_t base, int per_cu);

@@ -490,6 +490,21 @@ __device__ inline void clip_ray(KRay &r, const KParams &P) {
   }
 }
 
+// Per-lane picks from the scene's volumes, lights or surfaces (x =
+// sc.vol[id].field for a lane-dependent id) are written as a loop over
+// every entry whose loads do not depend on the comparison: x = (i == id) ?
+// sc.vol[i].field : x, with i made opaque to the compiler.  Otherwise the
+// compiler folds the loop back into sc.vol[id] -- a divergent index into the
+// kernel argument, which copies the whole scene to scratch memory.
+__device__ inline int uidx(int i) { return __builtin_amdgcn_readfirstlane(i); }
+// the same index once a comparison with the lane's id has selected it: an
+// empty asm hides the equality, so the compiler cannot address the entry
+// through the lane's id instead
+__device__ inline int opaque_uidx(int i) {
+  asm volatile("" : "+s"(i));
+  return i;
+}
+
 // ---------------------------------------------------------------------------
 // Triangles and the 4-wide BVH (bvh_build.cpp), quad lights
 // ---------------------------------------------------------------------------
@@ -529,7 +544,7 @@ constexpr int kMaxStagedNodes = 256;  // 32 KB per workgroup at most
 constexpr size_t kLdsBytesPerCu = 160 * 1024;
 __host__ __device__ inline size_t staged_nodes_bytes(const KScene &sc) {
   size_t n = 0;
-  for (int i = 0; i < sc.num_surfaces; ++i) n += size_t(sc.surf[i].lds_nodes);
+  for (int i = 0; i < sc.num_surfaces; ++i) n += size_t(sc.staged[i].lds_nodes);
   return n * 8 * sizeof(float4);
 }
 // the staged nodes follow the tables and the traversal stacks
@@ -544,8 +559,10 @@ __device__ inline const float4 *staged_nodes(const KScene &sc, const float4 *lds
 __device__ inline void stage_nodes(const KScene &sc, float4 *lds) {
   float4 *dst = const_cast<float4 *>(staged_nodes(sc, lds));
   for (int si = 0; si < sc.num_surfaces; ++si) {
-    const KSurface &S = sc.surf[si];
-    for (int i = threadIdx.x; i < 8 * S.lds_nodes; i += blockDim.x) dst[S.lds_first + i] = S.nodes[i];
+    const float4 *src = sc.surf[si].nodes;
+    const int first = sc.staged[si].lds_first;
+    for (int i = threadIdx.x; i < 8 * sc.staged[si].lds_nodes; i += blockDim.x)
+      dst[first + i] = src[i];
   }
 }
 
@@ -570,10 +587,11 @@ __device__ inline void cswap(float &ta, int &na, float &tb, int &nb) {
 // stack's first kBvhLdsStack entries live in LDS (`lst`: this lane's slot,
 // entries kBvhLanes apart, so the lanes of a wave hit distinct banks), the
 // rest in private memory, which only deep traversals reach.  Nodes below
-// S.lds_nodes are read from the workgroup's staged copy (`lnodes`).
+// lds_nodes are read from the workgroup's staged copy of this tree
+// (`lnodes`).
 __device__ inline bool bvh_hit(const KSurface &S, const KRay &r, bool any, float &tbest,
                                int &prim, uint32_t &rec, int2 *lst,
-                               const float4 *lnodes = nullptr) {
+                               const float4 *lnodes = nullptr, int lds_nodes = 0) {
   f3 inv;
   {
     const float dx = fabsf(r.dir.x) < 1e-20f ? copysignf(1e-20f, r.dir.x) : r.dir.x;
@@ -588,8 +606,8 @@ __device__ inline bool bvh_hit(const KSurface &S, const KRay &r, bool any, float
   const float inf = __builtin_inff();
   while (true) {
     float4 nd[8];
-    if (node < S.lds_nodes) {
-      const float4 *n = lnodes + S.lds_first + 8 * node;
+    if (node < lds_nodes) {
+      const float4 *n = lnodes + 8 * node;
 #pragma unroll
       for (int k = 0; k < 8; ++k) nd[k] = n[k];
     } else {
@@ -795,8 +813,15 @@ __device__ inline SurfRef surface_ref(const KScene &sc, int id) {
   SurfRef R;
   if (id < sc.num_surfaces) {
     KSurface S = sc.surf[0];
-    for (int i = 1; i < sc.num_surfaces; ++i)
-      if (i == id) S = sc.surf[i];
+    for (int j = 1; j < sc.num_surfaces; ++j) {
+      const int i = uidx(j);
+      const KSurface C = sc.surf[i];
+      const bool m = i == id;
+      S.tris = m ? C.tris : S.tris;
+      S.color = m ? C.color : S.color;
+      S.obj_id = m ? C.obj_id : S.obj_id;
+      S.inst_id = m ? C.inst_id : S.inst_id;
+    }
     R.tris = S.tris;
     R.color = S.color;
     R.obj_id = S.obj_id;
@@ -1381,9 +1406,11 @@ __device__ inline float4 light_backdrop(const KScene &sc, const KRay &ray, float
   }
   if (!lh) return bg;
   f3 I = mk(0.f);
-  for (int i = 0; i < sc.num_lights; ++i)
+  for (int j = 0; j < sc.num_lights; ++j) {
+    const int i = uidx(j);
     if (i == lid) I = sc.light[i].type == VRT_LIGHT_HDRI ? hdri_radiance(sc.hdri, ray.dir)
                                                         : sc.light[i].intensity;
+  }
   return make_float4(I.x, I.y, I.z, 1.f);
 }
 
@@ -1483,7 +1510,8 @@ __device__ inline void raycast_pixel(const KScene &sc, const KCamera &cam, const
           float t = st;
           uint32_t rec;
           int p;
-          if (bvh_hit(sc.surf[i], ray, false, t, p, rec, bstack, staged_nodes(sc, s_tf))) {
+          if (bvh_hit(sc.surf[i], ray, false, t, p, rec, bstack,
+                      staged_nodes(sc, s_tf) + sc.staged[i].lds_first, sc.staged[i].lds_nodes)) {
             st = t;
             sid = i;
             srec = rec;
@@ -1618,8 +1646,8 @@ __device__ inline void raycast_pixel(const KScene &sc, const KCamera &cam, const
       // a volume box hit sets the object and instance ids and zeroes normal
       // and albedo; depth and primitive id stay the surface's (or the
       // defaults)
-      for (int i = 0; i < sc.num_volumes; ++i)
-        if (i == hit_vol) {
+      for (int j = 0; j < sc.num_volumes; ++j)
+        if (const int i = uidx(j); i == hit_vol) {
           gb_obj = sc.vol[i].obj_id;
           gb_inst = sc.vol[i].inst_id;
           gb_n = mk(0.f);
@@ -1825,7 +1853,8 @@ __device__ inline PassHit intersect_all(const KScene &sc, const KRay &ray, bool 
     float t = st;
     int prim = -1;
     uint32_t rec = 0;
-    if (bvh_hit(sc.surf[i], ray, shadow, t, prim, rec, bstack, lnodes)) {
+    if (bvh_hit(sc.surf[i], ray, shadow, t, prim, rec, bstack, lnodes + sc.staged[i].lds_first,
+                sc.staged[i].lds_nodes)) {
       sh = true;
       st = t;
       sid = i;
@@ -2042,8 +2071,8 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
       // the quad's intensity, MIS-weighted against the BSDF sample after the
       // first bounce (Pathtrace_impl.cpp:139-183); the path ends
       KLight L = sc.light[0];
-      for (int i = 1; i < nl; ++i)
-        if (i == h.id) L = sc.light[i];
+      for (int j = 1; j < nl; ++j)
+        if (const int i = uidx(j); i == h.id) L = sc.light[i];
       if (L.type == VRT_LIGHT_HDRI) {
         // the map in the ray's direction, MIS-weighted after the first
         // bounce (Pathtrace_impl.cpp:144-182)
@@ -2079,10 +2108,14 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
                       h.t <= F.depth[st.pixel];
     if (h.kind == kVolume) {
       if constexpr (GRAD) {
-        // the gradient at the hit in voxel space (Pathtrace_impl.cpp:204-212)
-        const KVolume &hv = sc.vol[h.id];
-        const KRay lr = to_local(hv, ray);
-        gn = gradient_normal<ANYF>(hv, hv.S * ((lr.ori + lr.dir * h.t) + hv.T));
+        // the gradient at the hit in voxel space (Pathtrace_impl.cpp:204-212),
+        // the lanes of each volume in turn
+        for (int j = 0; j < sc.num_volumes; ++j) {
+          if (uidx(j) != h.id) continue;
+          const KVolume &hv = sc.vol[opaque_uidx(uidx(j))];
+          const KRay lr = to_local(hv, ray);
+          gn = gradient_normal<ANYF>(hv, hv.S * ((lr.ori + lr.dir * h.t) + hv.T));
+        }
       }
       if (P.ambient_samples > 0 && length(gn) < 1e-3f) {
         const float u1 = rnd(st.srng), u2 = rnd(st.srng);
@@ -2095,9 +2128,11 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
         // the kernel argument would copy it to scratch)
         float A[9];
         for (int k = 0; k < 9; ++k) A[k] = sc.vol[0].A[k];
-        for (int i = 1; i < sc.num_volumes; ++i)
-          if (i == h.id)
-            for (int k = 0; k < 9; ++k) A[k] = sc.vol[i].A[k];
+        for (int j = 1; j < sc.num_volumes; ++j) {
+          const int i = uidx(j);
+          const bool m = i == h.id;
+          for (int k = 0; k < 9; ++k) A[k] = m ? sc.vol[i].A[k] : A[k];
+        }
         gn = mk(A[0] * gn.x + A[3] * gn.y + A[6] * gn.z, A[1] * gn.x + A[4] * gn.y + A[7] * gn.z,
                 A[2] * gn.x + A[5] * gn.y + A[8] * gn.z);
       }
@@ -2106,8 +2141,8 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
       color = h.albedo;
       if (gbuf) {
         uint32_t obj = 0, inst = 0, prim = 0;
-        for (int i = 0; i < sc.num_volumes; ++i)
-          if (i == h.id) {
+        for (int j = 0; j < sc.num_volumes; ++j)
+          if (const int i = uidx(j); i == h.id) {  // (ids only: cheap even if folded)
             obj = sc.vol[i].obj_id;
             inst = sc.vol[i].inst_id;
             if (h.rec != kNoRec) prim = sc.vol[i].ug_cell[h.rec];
@@ -2138,8 +2173,8 @@ __device__ inline bool path_shade(PathState &st, const KScene &sc, const KParams
       int which = int(rnd(st.srng) * float(nl));
       if (which == nl) which = 0;
       KLight L = sc.light[0];
-      for (int i = 1; i < nl; ++i)
-        if (i == which) L = sc.light[i];
+      for (int j = 1; j < nl; ++j)
+        if (const int i = uidx(j); i == which) L = sc.light[i];
       ls = sample_light<AREA>(sc, L, hitPos, st.srng);
     }
     const f3 lightIntensity = ls.intensity * safe_rcp(ls.dist2);
@@ -2831,7 +2866,8 @@ __global__ void __launch_bounds__(256)
     float t = best;
     int p = -1;
     uint32_t rec;
-    if (bvh_hit(sc.surf[s], ray, any != 0, t, p, rec, s_bstack + threadIdx.x, lnodes)) {
+    if (bvh_hit(sc.surf[s], ray, any != 0, t, p, rec, s_bstack + threadIdx.x,
+                lnodes + sc.staged[s].lds_first, sc.staged[s].lds_nodes)) {
       hit = true;
       best = t;
       prim = p;
@@ -2970,15 +3006,15 @@ size_t pathtrace_lds_size(const KScene &sc) {
 // LDS bytes per workgroup without lowering the kernel's `per_cu` resident
 // workgroups (1 KB kept for allocation granularity), at most kMaxStagedNodes.
 void plan_staged_nodes(KScene &sc, size_t base, int per_cu) {
-  for (int i = 0; i < sc.num_surfaces; ++i) sc.surf[i].lds_nodes = sc.surf[i].lds_first = 0;
+  for (int i = 0; i < sc.num_surfaces; ++i) sc.staged[i].lds_nodes = sc.staged[i].lds_first = 0;
   const size_t per_wg = kLdsBytesPerCu / size_t(std::max(1, per_cu));
   if (per_wg <= base + 1024) return;
   size_t room = std::min((per_wg - base - 1024) / (8 * sizeof(float4)), size_t(kMaxStagedNodes));
   int first = 0;
   for (int i = 0; i < sc.num_surfaces && room > 0; ++i) {
-    const int k = int(std::min(room, size_t(std::max(0, sc.surf[i].num_nodes))));
-    sc.surf[i].lds_nodes = k;
-    sc.surf[i].lds_first = first;
+    const int k = int(std::min(room, size_t(std::max(0, sc.staged[i].num_nodes))));
+    sc.staged[i].lds_nodes = k;
+    sc.staged[i].lds_first = first;
     first += 8 * k;
     room -= size_t(k);
   }

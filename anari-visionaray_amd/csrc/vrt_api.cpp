@@ -254,7 +254,7 @@ int make_scene(const vrt_scene *s, KScene &k) {
     if (!S.bvh) return fail(VRT_ERROR_INVALID_ARGUMENT, "surface %d has no bvh", i);
     k.surf[i].nodes = S.bvh->nodes;
     k.surf[i].tris = S.bvh->tris;
-    k.surf[i].num_nodes = int(S.bvh->num_nodes);
+    k.staged[i].num_nodes = int(S.bvh->num_nodes);
     k.surf[i].color = mk(S.color[0], S.color[1], S.color[2]);
     k.surf[i].obj_id = S.object_id;
     k.surf[i].inst_id = S.instance_id;
