@@ -127,14 +127,33 @@ __device__ inline float raw_at(const float r[6], int f) {
   return f < 48 ? (f < 24 ? r[0] : r[1]) : (f < 72 ? r[2] : (f < 96 ? r[3] : r[4]));
 }
 
-__device__ inline float encode(const float *raw, int f) {
-  if (f < 120) {
-    const int w = f % 24, k = w >> 1;
-    float r = raw_at(raw, f) * __uint_as_float(uint32_t(126 + k) << 23);  // * 2^(k-1)
+// Features f0 .. f0 + 7 (f0 a multiple of 8, so one input's frequencies k0 ..
+// k0 + 3, or the density and padding): sin/cos of the first frequency from
+// the hardware, the next three by the double-angle identities sin 2a =
+// 2 sin a cos a, cos 2a = 1 - 2 sin^2 a.  Three doublings scale the
+// first pair's rounding error by at most 8 (a few 1e-7), far below the 16-bit
+// rounding of the features.
+__device__ inline void encode8(const float *raw, int f0, float e[8]) {
+  if (f0 < 120) {
+    const int k0 = (f0 % 24) >> 1;
+    float r = raw_at(raw, f0) * __uint_as_float(uint32_t(126 + k0) << 23);  // * 2^(k0-1)
     r = r - floorf(r);
-    return (w & 1) ? __builtin_amdgcn_cosf(r) : __builtin_amdgcn_sinf(r);
+    float sn = __builtin_amdgcn_sinf(r), cs = __builtin_amdgcn_cosf(r);
+    e[0] = sn;
+    e[1] = cs;
+#pragma unroll
+    for (int i = 1; i < 4; ++i) {
+      const float s2 = 2.f * sn * cs;
+      cs = 1.f - 2.f * sn * sn;
+      sn = s2;
+      e[2 * i] = sn;
+      e[2 * i + 1] = cs;
+    }
+  } else {
+    e[0] = f0 == 120 ? raw[5] : 0.f;
+#pragma unroll
+    for (int j = 1; j < 8; ++j) e[j] = 0.f;
   }
-  return f == 120 ? raw[5] : 0.f;
 }
 
 __device__ inline void load_raw(const float *raw, int n, int q, float r[6]) {
@@ -229,13 +248,18 @@ __device__ inline void forward(const W0Frag &w0frag, const WPtr &wrow, const flo
 // ---------------------------------------------------------------------------
 // Inference: persistent workgroups, weights staged in LDS with rows padded by
 // 8 halves (row stride 16 B off a multiple of 128 B: the 16 rows one
-// ds_read_b128 phase touches spread over all banks); 64 queries per wave.
+// ds_read_b128 phase touches spread over all banks); 16 * kInferQB queries
+// per wave.
 // ---------------------------------------------------------------------------
 __host__ __device__ constexpr int lds_stride(int l) { return nrc_in(l) + 8; }
 __host__ __device__ constexpr int lds_woff(int l) {
   return l == 0 ? 0 : 64 * lds_stride(0) + (l - 1) * 64 * lds_stride(1);
 }
 constexpr int kLdsWeights = 64 * (128 + 8) + 4 * 64 * (64 + 8) + 16 * (64 + 8);
+// 16-query blocks per wave in inference: 3 keeps the kernel at two waves per
+// SIMD (163 VGPRs + 84 AGPRs; 4 blocks need 167 + 112 and fall to one wave),
+// which the 64 KB of staged weights allow per CU anyway
+constexpr int kInferQB = 3;
 
 // Sample range [lo, lo + cnt) of mini-batch `batch` of `nbatches` over the
 // first min(n, *n_dev) samples (n_dev: a count produced on the GPU, or null).
@@ -271,21 +295,21 @@ __global__ void __launch_bounds__(256) k_nrc_infer(NrcDevice d, int n, const uin
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r16 = lane & 15, g = lane >> 4;
   auto w0frag = [&](int ob, int kb) { return load8(wrow(0, ob * 16 + r16, kb * 32 + 8 * g)); };
-  constexpr int QB = 4;
-  const int tiles = (n + 63) / 64;  // 64 queries (QB blocks of 16) per wave-tile
+  constexpr int QB = kInferQB;
+  constexpr int TQ = 16 * QB;  // queries per wave-tile
+  const int tiles = (n + TQ - 1) / TQ;
   for (int t = blockIdx.x * 4 + wave; t < tiles; t += gridDim.x * 4) {
     Frag8 hin[4][QB];
 #pragma unroll
     for (int qb = 0; qb < QB; ++qb) {
       float r[6];
-      load_raw(raw, n, t * 64 + qb * 16 + r16, r);
+      load_raw(raw, n, t * TQ + qb * 16 + r16, r);
 #pragma unroll
       for (int kb = 0; kb < 4; ++kb) {
         const int f = kb * 32 + 8 * g;
-        hin[kb][qb] = cat(pack4(encode(r, f), encode(r, f + 1), encode(r, f + 2), encode(r, f + 3),
-                                BF16),
-                          pack4(encode(r, f + 4), encode(r, f + 5), encode(r, f + 6),
-                                encode(r, f + 7), BF16));
+        float e[8];
+        encode8(r, f, e);
+        hin[kb][qb] = cat(pack4(e[0], e[1], e[2], e[3], BF16), pack4(e[4], e[5], e[6], e[7], BF16));
       }
     }
     f4v o[QB];
@@ -293,7 +317,7 @@ __global__ void __launch_bounds__(256) k_nrc_infer(NrcDevice d, int n, const uin
     if (g == 0) {
 #pragma unroll
       for (int qb = 0; qb < QB; ++qb) {
-        const int q = t * 64 + qb * 16 + r16;
+        const int q = t * TQ + qb * 16 + r16;
         if (q < n) {
           out[3 * size_t(q)] = o[qb][0];
           out[3 * size_t(q) + 1] = o[qb][1];
@@ -400,7 +424,8 @@ __global__ void __launch_bounds__(256) k_nrc_grad(NrcDevice d, int n, const uint
     for (int kb = 0; kb < 4; ++kb) {
       const int f = kb * 32 + 8 * g;
       float e[8];
-      for (int j = 0; j < 8; ++j) e[j] = valid ? encode(r, f + j) : 0.f;
+      encode8(r, f, e);
+      for (int j = 0; j < 8; ++j) e[j] = valid ? e[j] : 0.f;
       hin[kb][0] = cat(pack4(e[0], e[1], e[2], e[3], BF16), pack4(e[4], e[5], e[6], e[7], BF16));
       for (int j = 0; j < 8; ++j) sx0[(f + j) * kTS + col] = to16(e[j], BF16);
     }
@@ -694,7 +719,7 @@ hipError_t nrc_launch_refresh(const NrcDevice &d, hipStream_t s) {
 hipError_t nrc_launch_infer(const NrcDevice &d, int n, const float *raw, float *out, int num_cus,
                             hipStream_t s, const uint32_t *n_dev) {
   if (n <= 0) return hipSuccess;
-  const int tiles = (n + 63) / 64;
+  const int tiles = (n + 16 * kInferQB - 1) / (16 * kInferQB);
   const int blocks = std::max(1, std::min((tiles + 3) / 4, 2 * std::max(1, num_cus)));
   if (d.bf16)
     hipLaunchKernelGGL(k_nrc_infer<1>, dim3(blocks), dim3(256), 0, s, d, n, n_dev, raw, out);
