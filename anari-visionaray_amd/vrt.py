@@ -112,7 +112,7 @@ class Params(C.Structure):
         ("num_clip_planes", C.c_int), ("clip_planes", (C.c_float * 4) * 8),
         ("phase_g", C.c_float), ("gradient_shading", C.c_int),
         ("taa", C.c_int), ("taa_alpha", C.c_float), ("has_prev_camera", C.c_int),
-        ("prev_camera", Camera), ("ratio_tracking", C.c_int),
+        ("prev_camera", Camera), ("ratio_tracking", C.c_int), ("pass_schedule", C.c_int),
     ]
 
 
@@ -777,7 +777,7 @@ def params(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.
            occlusion_distance=1e20, ambient_samples=1, max_bounce=7, pixel_samples=1,
            volume_sampling_rate=0.5, rng_offset=0, clip_planes=(), phase_g=0.0,
            gradient_shading=False, taa=False, taa_alpha=0.3, prev_camera=None,
-           ratio_tracking=False):
+           ratio_tracking=False, pass_schedule=0):
     """Renderer parameters with the ANARI defaults of the reference
     (Renderer.cpp:42-53, Pathtrace.cpp:19-25, Raycast.cpp:19).  `clip_planes`:
     up to 8 (nx, ny, nz, d); rays keep n.x <= d.  `phase_g`: Henyey-Greenstein
@@ -786,6 +786,7 @@ def params(background=(0, 0, 0, 1), ambient_color=(1, 1, 1), ambient_radiance=0.
     reference: binary delta tracking)."""
     p = Params()
     p.ratio_tracking = int(bool(ratio_tracking))
+    p.pass_schedule = int(pass_schedule)
     p.rng_offset = rng_offset
     p.phase_g = phase_g
     p.gradient_shading = int(bool(gradient_shading))
