@@ -181,6 +181,9 @@ struct KParams {
   // bounces; 0: every lane traces one ray per pass, idle lanes refilled
   // before each pass
   int chain_passes;
+  // scheduling only: idle lanes are refilled once at least this many lanes
+  // of the wave are idle (or the wave has no active lane)
+  int refill_min;
   // temporal anti-aliasing: previous and current view-projection (row-major
   // proj * view), blend weight
   int taa;

@@ -52,8 +52,10 @@ size_t scene_lds_size(const KScene &sc);
 // LDS bytes of the path tracer: the above plus the BVH traversal stacks and
 // the staged BVH nodes when the scene has surfaces or quad lights
 size_t pathtrace_lds_size(const KScene &sc);
-// the path tracer's pass scheduling for the scene (KParams::chain_passes)
-bool pathtrace_chain_passes(const KScene &sc);
+// the path tracer's pass schedule for the scene (vrt_params.pass_schedule
+// 1-3), and its KParams fields
+int pathtrace_schedule(const KScene &sc);
+void set_pass_schedule(KParams &P, int schedule);
 // sets the BVH nodes each workgroup stages in LDS (sc.staged[i].lds_nodes):
 // as many top nodes as fit beside `base` bytes per workgroup without
 // lowering the `per_cu` resident workgroups

@@ -2344,6 +2344,8 @@ __global__ void __launch_bounds__(256, 3)
     const bool chained = __ballot(chain) != 0ull;
     // ---- refill idle lanes ----------------------------------------------------
     unsigned long long idle = chained ? 0ull : __ballot(st.pixel < 0);
+    if (P.refill_min > 1 && __popcll(idle) < P.refill_min && __ballot(st.pixel >= 0) != 0ull)
+      idle = 0ull;  // refill later, for more lanes at once
     while (idle && qtries < kQueues) {
       if (pool_left == 0) {
         // pixels of queue q: the chunks c = q, q+8, ... times 1024 pixels
@@ -2985,15 +2987,27 @@ static PathtraceKernel pathtrace_kernel(const KScene &sc, bool nrc, bool grad) {
                (scene_has_other_fields(sc) ? 1 : 0)];
 }
 
-// Pass scheduling of the path tracer: the passes of a bounce run back to back
-// (KParams::chain_passes) except on tetrahedral fields.  Measured frame
-// times, chained vs one ray per pass: fBm 512^3 1.215 vs 1.270 ms, mixed
-// surfaces + volume 18.4 vs 19.4 ms, NanoVDB cloud 3.48 vs 3.53 ms,
-// 2.06 M tetrahedra at 3840x2160 22.3 vs 20.3 ms.
-bool pathtrace_chain_passes(const KScene &sc) {
-  for (int i = 0; i < sc.num_volumes; ++i)
-    if (sc.vol[i].field_type == VRT_FIELD_UNSTRUCTURED) return false;
-  return true;
+// Pass scheduling of the path tracer (vrt_params.pass_schedule): 1 chains the
+// passes of a bounce (KParams::chain_passes), 2 traces one ray per lane and
+// pass, 3 is 2 with idle lanes refilled only 16 at a time (refill_min).
+// Measured frame times (1 / 2 / 3): fBm 512^3 1.21 / 1.27 / 1.26 ms, mixed
+// surfaces + volume 18.6 / 19.5 / 19.4 ms, NanoVDB cloud 1024^3 3.44 / 3.59 /
+// 3.36 ms (its cache frames 2.82 / 2.83 / 2.77 ms), 2.06 M tetrahedra at
+// 3840x2160 22.0 / 20.5 / 20.9 ms; the default picks per scene: 3 with a
+// NanoVDB field, 2 with a tetrahedral one (and no NanoVDB field), 1
+// otherwise.
+int pathtrace_schedule(const KScene &sc) {
+  bool nvdb = false, tets = false;
+  for (int i = 0; i < sc.num_volumes; ++i) {
+    nvdb = nvdb || sc.vol[i].field_type == VRT_FIELD_NANOVDB;
+    tets = tets || sc.vol[i].field_type == VRT_FIELD_UNSTRUCTURED;
+  }
+  return nvdb ? 3 : tets ? 2 : 1;
+}
+
+void set_pass_schedule(KParams &P, int schedule) {
+  P.chain_passes = schedule == 1;
+  P.refill_min = schedule == 3 ? 16 : 1;
 }
 
 size_t pathtrace_lds_size(const KScene &sc) {

@@ -317,9 +317,9 @@ int make_params(const vrt_params *p, KParams &k) {
   k.phase_g = p->phase_g;
   k.gradient_shading = p->gradient_shading ? 1 : 0;
   k.ratio_tracking = p->ratio_tracking ? 1 : 0;
-  if (p->pass_schedule < 0 || p->pass_schedule > 2)
-    return fail(VRT_ERROR_INVALID_ARGUMENT, "pass_schedule %d out of [0,2]", p->pass_schedule);
-  k.chain_passes = 0;  // the path tracer's entry points choose it for their scene
+  if (p->pass_schedule < 0 || p->pass_schedule > 3)
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "pass_schedule %d out of [0,3]", p->pass_schedule);
+  set_pass_schedule(k, 2);  // the path tracer's entry points choose it for their scene
   k.taa = p->taa ? 1 : 0;
   k.taa_alpha = p->taa_alpha;
   for (int i = 0; i < VRT_MAX_CLIP_PLANES; ++i) {
@@ -746,8 +746,8 @@ int pathtrace_frame(vrt_context ctx, const vrt_scene *scene, const vrt_camera *c
       std::copy(P.vp_curr, P.vp_curr + 16, P.vp_prev);
     }
   }
-  P.chain_passes = params->pass_schedule == 0 ? pathtrace_chain_passes(sc)
-                                               : params->pass_schedule == 1;
+  set_pass_schedule(P, params->pass_schedule == 0 ? pathtrace_schedule(sc)
+                                                  : params->pass_schedule);
   const size_t lds = pathtrace_lds_size(sc);
   const int per_cu = pathtrace_blocks_per_cu(sc, lds, false, P.gradient_shading != 0);
   plan_staged_nodes(sc, lds, per_cu);  // the top BVH nodes that fit beside it
@@ -852,8 +852,8 @@ int vrt_pathtrace_nrc(vrt_context ctx, const vrt_scene *scene, const vrt_camera 
                    1.f / (opts->bounds_hi[1] - opts->bounds_lo[1]),
                    1.f / (opts->bounds_hi[2] - opts->bounds_lo[2]));
 
-  P.chain_passes = params->pass_schedule == 0 ? pathtrace_chain_passes(sc)
-                                               : params->pass_schedule == 1;
+  set_pass_schedule(P, params->pass_schedule == 0 ? pathtrace_schedule(sc)
+                                                  : params->pass_schedule);
   const size_t lds = pathtrace_lds_size(sc);
   const int per_cu = pathtrace_blocks_per_cu(sc, lds, true, P.gradient_shading != 0);
   plan_staged_nodes(sc, lds, per_cu);

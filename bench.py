@@ -386,9 +386,11 @@ KERNEL_WORKLOADS = ("pathtrace_fbm512", "raycast_blobs512", "raycast_blobs128",
 
 
 def kernel_only(pkg, ctx, name, steps, warmup, precision="fp16", query_vertex=2,
-                train_samples=65536, train_steps=8, train_extra=0):
+                train_samples=65536, train_steps=8, train_extra=0, pass_schedule=0):
     """One workload's frames alone (profiling runs): `warmup` untimed frames,
-    then `steps` frames timed on HIP events; prints one JSON line."""
+    then `steps` frames timed on HIP events; prints one JSON line.
+    `pass_schedule`: the path tracer's vrt_params.pass_schedule (0: per
+    scene)."""
     vrt, wls = pkg.vrt, pkg.workloads
     if name == "nrc_mfma":
         print(json.dumps({"kernel_workload": name, **mfma_rates(pkg, ctx, reps=steps)}),
@@ -410,7 +412,7 @@ def kernel_only(pkg, ctx, name, steps, warmup, precision="fp16", query_vertex=2,
         wl = wls.mixed_pathtrace(level=8, vol_n=256)
     else:
         wl = wls.tets_pathtrace()
-    gs = wls.GpuScene(ctx, wl)
+    gs = wls.GpuScene(ctx, wl, params={"pass_schedule": pass_schedule})
     nrc = opts = None
     if name.startswith("nrc_"):
         nrc = vrt.Nrc(ctx, precision={"fp16": vrt.NRC_FP16, "bf16": vrt.NRC_BF16}[precision],
@@ -435,7 +437,8 @@ def kernel_only(pkg, ctx, name, steps, warmup, precision="fp16", query_vertex=2,
         frame(warmup + a)
     ms = ctx.timer_stop() / steps
     lookups = int(gs.frame.counter_values()[0]) / steps
-    out = {"kernel_workload": name, "workload": wl.name, "frames": steps,
+    out = {"kernel_workload": name, "workload": wl.name, "pass_schedule": pass_schedule,
+           "frames": steps,
            "ms_per_frame": round(ms, 4), "lookups_per_frame": lookups,
            "algorithmic_GBps": round(lookups * BYTES_PER_LOOKUP / (ms / 1e3) / 1e9, 1)}
     if nrc is not None:
@@ -481,6 +484,8 @@ def main():
     ap.add_argument("--no-nrc", action="store_true",
                     help="skip the time-to-relMSE comparisons with the radiance cache")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--pass-schedule", type=int, default=0, choices=(0, 1, 2, 3),
+                    help="path-tracer pass schedule of the --kernel runs (0: per scene)")
     ap.add_argument("--nrc-query-vertex", type=int, default=2, choices=(1, 2),
                     help="volume vertex whose cache query ends a path (2: the configuration)")
     ap.add_argument("--nrc-train-extra", type=int, default=0,
@@ -515,7 +520,8 @@ def main():
     if args.kernel:
         ctx = vrt.Context(0)
         kernel_only(pkg, ctx, args.kernel, args.steps, args.warmup, args.precision,
-                    args.nrc_query_vertex, train_extra=args.nrc_train_extra)
+                    args.nrc_query_vertex, train_extra=args.nrc_train_extra,
+                    pass_schedule=args.pass_schedule)
         ctx.close()
         return
     t0 = time.perf_counter()

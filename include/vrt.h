@@ -233,12 +233,13 @@ typedef struct vrt_params {
      Pathtrace_impl.cpp:441-444, 461-466).  Same expectation, lower
      variance; the pixels' random streams then differ from the reference's. */
   int ratio_tracking;
-  /* extension, path-tracer scheduling only (images are identical either
-     way): 1 = a lane's shadow and ambient-occlusion passes follow its
-     radiance pass back to back while the rest of its wave waits, idle lanes
-     are refilled between bounces; 2 = one ray per lane and pass, idle lanes
-     refilled before every pass; 0 (default) = 1, except 2 on tetrahedral
-     fields, where it measured faster. */
+  /* extension, path-tracer scheduling only (images are identical for
+     every value): 1 = a lane's shadow and ambient-occlusion passes follow
+     its radiance pass back to back while the rest of its wave waits, idle
+     lanes are refilled between bounces; 2 = one ray per lane and pass, idle
+     lanes refilled before every pass; 3 = 2 with idle lanes refilled only
+     16 at a time; 0 (default) = chosen per scene from measurements: 3 with
+     a NanoVDB field, 2 with a tetrahedral field, 1 otherwise. */
   int pass_schedule;
 } vrt_params;
 
