@@ -1042,7 +1042,7 @@ void render(Device *d, Frame *fr) {
   assemble(d, world, sb);
   const uint64_t ver = std::max({sb.version, cam->version, ren->version, fr->version});
 
-  // camera: perspective (camera/Camera.cpp:32-40, Perspective.cpp:14-31)
+  // camera: perspective (camera/Camera.cpp:32-40, Perspective.cpp:14-44)
   vrt_camera c{};
   float pos[3] = {0, 0, 0}, dir[3] = {0, 0, 1}, up[3] = {0, 1, 0};
   cam->get_v3("position", pos);
@@ -1058,6 +1058,8 @@ void render(Device *d, Frame *fr) {
   cam->get("shutter", ANARI_FLOAT32_BOX1, &shutter);
   std::memcpy(c.image_region, region, 16);
   std::memcpy(c.shutter, shutter, 8);
+  c.aperture_radius = cam->get_f("apertureRadius", 0.f);
+  c.focus_distance = cam->get_f("focusDistance", 1.f);
   if (cam->subtype != "perspective")
     d->report(cam, ANARI_SEVERITY_WARNING, ANARI_STATUS_INVALID_OPERATION,
               "camera '%s' not supported: rendered as perspective", cam->subtype.c_str());

@@ -158,6 +158,10 @@ struct KCamera {
   f3 pos, U, V, W;
   float region[4];
   float shutter[2];
+  // thin lens: lens radius / |U| and / |V| (the lens plane is spanned by U
+  // and V; 0: pinhole), distance of the plane in focus along W
+  float lens_u, lens_v;
+  float focus_distance;
 };
 
 struct KParams {

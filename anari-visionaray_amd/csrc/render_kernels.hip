@@ -1111,11 +1111,12 @@ __device__ inline f3 uniform_sample_sphere(float u1, float u2) {
   sincos_turns(u2, s, c);  // phi = 2 pi u2
   return mk(r * c, r * s, z);
 }
-__device__ inline f3 cosine_sample_hemisphere(float u1, float u2) {
+// Shirley and Chiu's concentric map of [0,1)^2 onto the unit disk
+__device__ inline void concentric_disk(float u1, float u2, float &dx, float &dy) {
   float sx = 2.f * u1 - 1.f, sy = 2.f * u2 - 1.f;
-  float dx = 0.f, dy = 0.f;
+  dx = 0.f, dy = 0.f;
   if (!(sx == 0.f && sy == 0.f)) {
-    // Shirley-Chiu: theta = pi/4 (sy/sx) or pi/2 - pi/4 (sx/sy), in turns
+    // theta = pi/4 (sy/sx) or pi/2 - pi/4 (sx/sy), in turns
     float r, turns;
     if (fabsf(sx) > fabsf(sy)) {
       r = sx;
@@ -1129,7 +1130,26 @@ __device__ inline f3 cosine_sample_hemisphere(float u1, float u2) {
     dx = r * c;
     dy = r * s;
   }
+}
+__device__ inline f3 cosine_sample_hemisphere(float u1, float u2) {
+  float dx, dy;
+  concentric_disk(u1, u2, dx, dy);
   return mk(dx, dy, sqrtf(fmaxf(0.f, 1.f - dx * dx - dy * dy)));
+}
+
+// Thin lens (the perspective camera's apertureRadius / focusDistance,
+// camera/Perspective.cpp:42-43): the pinhole ray's point on the plane in
+// focus, focus_distance along the view direction, seen from the lens point
+// off = lens radius * concentric_disk(u1, u2) in the plane of U and V
+// (Pharr, Jakob and Humphreys, PBRT 3rd ed., 6.2.3); the new direction
+// dir * ft - off is that point minus the new origin ori + off
+__device__ inline void thin_lens(f3 &ori, f3 &dir, const KCamera &c, float u1, float u2) {
+  float dx, dy;
+  concentric_disk(u1, u2, dx, dy);
+  const float ft = c.focus_distance / dot(dir, c.W);
+  const f3 off = c.U * (c.lens_u * dx) + c.V * (c.lens_v * dy);
+  dir = normalize(dir * ft - off);
+  ori = ori + off;
 }
 __device__ inline f3 faceforward(f3 n, f3 i, f3 nref) { return dot(i, nref) < 0.f ? -n : n; }
 
@@ -1490,10 +1510,16 @@ __device__ inline void raycast_pixel(const KScene &sc, const KCamera &cam, const
     uint32_t gb_prim = ~0u, gb_obj = ~0u, gb_inst = ~0u;
     // light samples (surface and gradient shading) from the pixel's
     // path-tracer stream: the reference's ray caster leaves its generator
-    // unseeded
-    uint32_t rs = (SHADE || SURF) ? tea4(uint32_t(y * F.width + x), 0u) : 0u;
+    // unseeded; a thin lens's sample, if any, comes first
+    uint32_t rs = (SHADE || SURF || cam.lens_u > 0.f) ? tea4(uint32_t(y * F.width + x), 0u)
+                                                           : 0u;
     if (sc.num_volumes > 0 || sc.num_surfaces > 0 || sc.tls.num_inst > 0) {  // an empty scene shows the background
       KRay ray = primary_ray(cam, float(x), float(y), float(F.width), float(F.height));
+      if (cam.lens_u > 0.f) {
+        const float u1 = rnd(rs);
+        const float u2 = rnd(rs);
+        thin_lens(ray.ori, ray.dir, cam, u1, u2);
+      }
       clip_ray(ray, P);
       float4 surf = make_float4(0.f, 0.f, 0.f, 0.f);  // surfaceColor, surfaceAlpha
       if constexpr (SURF) {
@@ -1795,7 +1821,10 @@ __device__ inline void chunk_pixel(int q, uint32_t local, int nchx, int &x, int 
 }
 
 // Start sample `sampleID` of the lane's pixel: jitter, camera ray, shutter
-// draw (Pathtrace_impl.cpp:526-535)
+// draw (Pathtrace_impl.cpp:526-535).  LENS = false compiles the thin lens out
+// (the radiance-cache variants, whose entry point refuses a lens camera: the
+// lens code there costs spills in the pass loop)
+template <bool LENS>
 __device__ inline void begin_sample(PathState &st, const KCamera &cam, const KParams &P, int x,
                                     int y, int W, int H) {
   float xf = float(x), yf = float(y);
@@ -1807,6 +1836,11 @@ __device__ inline void begin_sample(PathState &st, const KCamera &cam, const KPa
   st.ori = r.ori;
   st.dir = r.dir;
   st.tmax = r.tmax;
+  if (LENS && cam.lens_u > 0.f) {  // the lens sample precedes the shutter draw
+    const float u1 = rnd(st.srng);
+    const float u2 = rnd(st.srng);
+    thin_lens(st.ori, st.dir, cam, u1, u2);
+  }
   (void)rnd(st.srng);  // shutter time (DeviceCopyableObjects.h:2597)
   st.sdepth = 1e31f;
   st.rayType = Radiance;
@@ -2374,7 +2408,7 @@ __global__ void __launch_bounds__(256, 3)
         st.depth = 1e31f;
         st.train_sel =
             NRC && train_hash(uint32_t(st.pixel), uint32_t(accumID) + P.rng_offset) < p_train;
-        begin_sample(st, cam, P, px, py, W, H);
+        begin_sample<!NRC>(st, cam, P, px, py, W, H);
       }
       pool_base += take;
       pool_left -= take;
@@ -2428,7 +2462,7 @@ __global__ void __launch_bounds__(256, 3)
           if (F.depth && st.depth <= F.depth[st.pixel]) F.depth[st.pixel] = st.depth;
           st.pixel = -1;
         } else if (++st.sampleID < P.pixel_samples) {
-          begin_sample(st, cam, P, x, y, W, H);
+          begin_sample<!NRC>(st, cam, P, x, y, W, H);
         } else {
           const float inv = 1.f / float(P.pixel_samples);
           const float4 a = make_float4(st.acc.x * inv, st.acc.y * inv, st.acc.z * inv,

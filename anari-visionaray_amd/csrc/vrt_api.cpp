@@ -287,6 +287,13 @@ int make_camera(const vrt_camera *c, KCamera &k) {
   for (int i = 0; i < 4; ++i) k.region[i] = c->image_region[i];
   k.shutter[0] = c->shutter[0];
   k.shutter[1] = c->shutter[1];
+  if (!(c->aperture_radius >= 0.f) || !std::isfinite(c->aperture_radius))
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "camera aperture_radius must be finite and >= 0");
+  if (c->aperture_radius > 0.f && !(c->focus_distance > 0.f && std::isfinite(c->focus_distance)))
+    return fail(VRT_ERROR_INVALID_ARGUMENT, "camera focus_distance must be finite and > 0");
+  k.lens_u = c->aperture_radius > 0.f ? c->aperture_radius / length(k.U) : 0.f;
+  k.lens_v = c->aperture_radius > 0.f ? c->aperture_radius / length(k.V) : 0.f;
+  k.focus_distance = c->focus_distance;
   return VRT_OK;
 }
 
@@ -820,6 +827,9 @@ int vrt_pathtrace_nrc(vrt_context ctx, const vrt_scene *scene, const vrt_camera 
   if (int r = make_camera(camera, cam)) return r;
   if (int r = make_params(params, P)) return r;
   if (P.taa) return fail(VRT_ERROR_UNSUPPORTED, "taa with the radiance cache is not supported");
+  if (cam.lens_u > 0.f)
+    return fail(VRT_ERROR_UNSUPPORTED,
+                "a thin-lens camera (aperture_radius > 0) with the radiance cache is not supported");
   if (int r = make_frame(frame, F)) return r;
   if (!F.accum) return fail(VRT_ERROR_INVALID_ARGUMENT, "pathtrace needs an accumulation buffer");
   if (P.pixel_samples != 1)

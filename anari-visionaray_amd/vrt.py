@@ -100,6 +100,7 @@ class Camera(C.Structure):
     _fields_ = [
         ("position", f3), ("direction", f3), ("up", f3), ("fovy", C.c_float),
         ("aspect", C.c_float), ("image_region", f4), ("shutter", C.c_float * 2),
+        ("aperture_radius", C.c_float), ("focus_distance", C.c_float),
     ]
 
 
@@ -761,7 +762,8 @@ def directional_light(direction, irradiance=1.0, color=(1, 1, 1), visible=True):
     return L
 
 
-def camera(position, direction, up, fovy, aspect, region=(0, 0, 1, 1), shutter=(0.5, 0.5)):
+def camera(position, direction, up, fovy, aspect, region=(0, 0, 1, 1), shutter=(0.5, 0.5),
+           aperture_radius=0.0, focus_distance=1.0):
     c = Camera()
     c.position = f3(*position)
     c.direction = f3(*direction)
@@ -770,6 +772,8 @@ def camera(position, direction, up, fovy, aspect, region=(0, 0, 1, 1), shutter=(
     c.aspect = float(aspect)
     c.image_region = f4(*region)
     c.shutter = (C.c_float * 2)(*shutter)
+    c.aperture_radius = float(aperture_radius)
+    c.focus_distance = float(focus_distance)
     return c
 
 

@@ -179,6 +179,12 @@ typedef struct vrt_camera {
   float aspect;
   float image_region[4]; /* lo.x lo.y hi.x hi.y */
   float shutter[2];
+  /* thin lens (camera/Perspective.cpp:21-22, 42-43: apertureRadius,
+   * focusDistance): the lens radius (0: pinhole, no random numbers drawn)
+   * and the distance of the plane in focus along the view direction (> 0
+   * when the radius is) */
+  float aperture_radius;
+  float focus_distance;
 } vrt_camera;
 
 /* RendererState subset (renderer/common.h:55-91) with the ANARI renderer
@@ -626,7 +632,8 @@ VRT_API int vrt_nrc_train_step(vrt_nrc nrc, int n, const float *raw, const float
  * cache query of their own) and leave one training sample (the query
  * vertex's inputs, suffix radiance / throughput).
  * After the frame is resolved the cache takes `train_steps` Adam steps on
- * those samples.  Requires params->pixel_samples == 1.  The cache inputs
+ * those samples.  Requires params->pixel_samples == 1 and a pinhole camera
+ * (aperture_radius 0; VRT_ERROR_UNSUPPORTED otherwise).  The cache inputs
  * map world positions in [bounds_lo, bounds_hi] onto [0,1]^3. */
 typedef struct vrt_nrc_render {
   int train;          /* 1: record training paths and train after the frame */

@@ -764,13 +764,17 @@ VolHit<R> delta_track(const Volume<R> &vol, Ray<R> ray, Random &rnd, Counter *cn
 }
 
 // ---------------------------------------------------------------------------
-// Camera (parity unpinned: visionaray thin_lens_camera is not vendored).
+// Camera (visionaray's thin_lens_camera is not vendored; the pinhole ray is
+// pinned against the textbook look-at / perspective projection and the thin
+// lens against its geometry, in the tests).
 // ANARI perspective camera: position, direction, up, fovy, aspect,
 // imageRegion; pixel (x, y) samples screen point ((x+.5)/W, (y+.5)/H), row 0
 // at the bottom of the image.  Lens radius 0 consumes no random numbers.
 // ---------------------------------------------------------------------------
 template <class R> struct Camera {
   V3<R> pos, U, V, W;
+  // thin lens: radius / |U| and / |V| (the lens plane is spanned by U, V)
+  R lens_radius, lens_u, lens_v, focus_distance;
   R region[4];
   R shutter[2];
   explicit Camera(const orc_camera *c) {
@@ -783,6 +787,10 @@ template <class R> struct Camera {
     U = right * (th * R(c->aspect));
     V = upv * th;
     W = d;
+    lens_radius = R(c->aperture_radius);
+    lens_u = lens_radius > R(0) ? lens_radius / length(U) : R(0);
+    lens_v = lens_radius > R(0) ? lens_radius / length(V) : R(0);
+    focus_distance = R(c->focus_distance);
     for (int i = 0; i < 4; ++i) region[i] = R(c->region[i]);
     shutter[0] = R(c->shutter[0]);
     shutter[1] = R(c->shutter[1]);
@@ -832,10 +840,10 @@ template <class R> inline V3<R> hg_sample(R g, V3<R> d, R u1, R u2) {
   const V3<R> t2(b, sign + d.y * d.y * a, -d.y);
   return t1 * (sinT * std::cos(phi)) + t2 * (sinT * std::sin(phi)) + d * cosT;
 }
-template <class R> inline V3<R> cosine_sample_hemisphere(R u1, R u2) {
-  // Shirley-Chiu concentric disk, lifted to the hemisphere
+// Shirley-Chiu concentric map of [0,1)^2 onto the unit disk
+template <class R> inline void concentric_disk(R u1, R u2, R &dx, R &dy) {
   R sx = R(2) * u1 - R(1), sy = R(2) * u2 - R(1);
-  R dx = 0, dy = 0;
+  dx = 0, dy = 0;
   if (!(sx == R(0) && sy == R(0))) {
     R r, theta;
     if (std::fabs(sx) > std::fabs(sy)) {
@@ -848,8 +856,26 @@ template <class R> inline V3<R> cosine_sample_hemisphere(R u1, R u2) {
     dx = r * std::cos(theta);
     dy = r * std::sin(theta);
   }
+}
+template <class R> inline V3<R> cosine_sample_hemisphere(R u1, R u2) {
+  // the concentric disk, lifted to the hemisphere
+  R dx, dy;
+  concentric_disk(u1, u2, dx, dy);
   R z = std::sqrt(std::max(R(0), R(1) - dx * dx - dy * dy));
   return V3<R>(dx, dy, z);
+}
+// Thin lens (camera/Perspective.cpp:42-43, apertureRadius / focusDistance;
+// PBRT 3rd ed. 6.2.3): the pinhole ray's point on the plane in focus,
+// focus_distance along the view direction, seen from the lens point
+// off = lens radius * concentric_disk(u1, u2) in the plane of U and V; the
+// new direction dir * ft - off is that point minus the new origin ori + off
+template <class R> inline void thin_lens(const Camera<R> &c, Ray<R> &r, R u1, R u2) {
+  R dx, dy;
+  concentric_disk(u1, u2, dx, dy);
+  const R ft = c.focus_distance / dot(r.dir, c.W);
+  const V3<R> off = c.U * (c.lens_u * dx) + c.V * (c.lens_v * dy);
+  r.dir = normalize(r.dir * ft - off);
+  r.ori = r.ori + off;
 }
 // ONB as in the matte sampler, renderer/common.h:1384-1386
 template <class R> inline void make_onb(V3<R> w, V3<R> &u, V3<R> &v) {
@@ -1774,6 +1800,10 @@ V4<R> pathtrace_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params
     xf += jx;
     yf += jy;
     Ray<R> ray = cam.primary(xf, yf, R(W), R(H));
+    if (cam.lens_radius > R(0)) {  // the lens sample precedes the shutter draw
+      const R u1 = R(srnd()), u2 = R(srnd());
+      thin_lens(cam, ray, u1, u2);
+    }
     (void)srnd();  // shutter time lerp, DeviceCopyableObjects.h:2597
     V4<R> color{R(P->bg[0]), R(P->bg[1]), R(P->bg[2]), R(P->bg[3])};
     if (!sc.empty()) {
@@ -1855,12 +1885,16 @@ V4<R> raycast_pixel(const Scene<R> &sc, const Camera<R> &cam, const orc_params *
   V4<R> bg{R(P->bg[0]), R(P->bg[1]), R(P->bg[2]), R(P->bg[3])};
   if (gb) *gb = GBuf<R>();
   if (sc.empty()) return bg;
-  Ray<R> ray = cam.primary(R(x), R(y), R(W), R(H));
-  clip_ray(ray, P);  // Raycast_impl.cpp:288
   // light samples (surface and gradient shading) come from the pixel's
   // path-tracer stream: the reference's ray caster leaves its generator
-  // unseeded
+  // unseeded; a thin lens's sample, if any, comes first
   Random rs(uint32_t(x + W * y), 0u);
+  Ray<R> ray = cam.primary(R(x), R(y), R(W), R(H));
+  if (cam.lens_radius > R(0)) {
+    const R u1 = R(rs()), u2 = R(rs());
+    thin_lens(cam, ray, u1, u2);
+  }
+  clip_ray(ray, P);  // Raycast_impl.cpp:288
   auto safe_rcp = [](R f) { return f > R(0) ? R(1) / f : R(0); };
   // the closest surface (Raycast_impl.cpp:27-156): matte under every light
   // along the surface normal (not faceforwarded) plus the ambient term;
@@ -2394,6 +2428,17 @@ float orc_ratio_track(const orc_volume *v, const float ori[3], const float dir[3
 void orc_camera_ray(const orc_camera *c, float x, float y, int W, int H, float *ori, float *dir) {
   Camera<float> cam(c);
   Ray<float> r = cam.primary(x, y, float(W), float(H));
+  for (int i = 0; i < 3; ++i) {
+    ori[i] = r.ori[i];
+    dir[i] = r.dir[i];
+  }
+}
+
+void orc_camera_lens_ray(const orc_camera *c, float x, float y, int W, int H, float u1, float u2,
+                         double *ori, double *dir) {
+  Camera<double> cam(c);
+  Ray<double> r = cam.primary(double(x), double(y), double(W), double(H));
+  thin_lens(cam, r, double(u1), double(u2));
   for (int i = 0; i < 3; ++i) {
     ori[i] = r.ori[i];
     dir[i] = r.dir[i];

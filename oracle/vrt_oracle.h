@@ -135,6 +135,8 @@ typedef struct orc_camera {
   float aspect;
   float region[4]; /* imageRegion: lo.x lo.y hi.x hi.y */
   float shutter[2];
+  float aperture_radius; /* thin lens; 0: pinhole */
+  float focus_distance;  /* plane in focus, along the view direction */
 } orc_camera;
 
 typedef struct orc_params {
@@ -239,7 +241,8 @@ uint32_t orc_cvt_uint32_srgb(const float rgba[4]);
 float orc_pow_ieee(float x, float y);
 void orc_pow_log2_table(float *out);
 
-/* camera ray for pixel coordinate (x, y) (parity unpinned: ANARI spec) */
+/* camera ray for pixel coordinate (x, y) (the pinhole of the ANARI perspective
+   camera; pinned against the look-at / perspective projection in the tests) */
 /* --- A5/A6: closest (any=0) or any (any=1) hit of n rays {ori, dir, tmin,
    tmax} against a mesh, through the oracle's BVH or by brute force; t = inf
    and prim = -1 on a miss --- */
@@ -277,6 +280,10 @@ void orc_hdri_eval(const orc_light *L, int n, const float *dirs, float *rgb, flo
                    float *uv);
 void orc_camera_ray(const orc_camera *cam, float x, float y, int width,
                     int height, float *ori, float *dir);
+/* the pinhole ray of (x, y) moved onto the thin lens by the lens sample
+   (u1, u2), in FP64 */
+void orc_camera_lens_ray(const orc_camera *cam, float x, float y, int width, int height,
+                         float u1, float u2, double *ori, double *dir);
 
 /* --- g-buffer of one frame (Frame::fillGBuffer, DeviceCopyableObjects.h:
    2745-2774), FP32.  Path tracer: per pixel the first radiance hit of the

@@ -56,7 +56,8 @@ class OrcScene(C.Structure):
 
 class OrcCamera(C.Structure):
     _fields_ = [("pos", f3), ("dir", f3), ("up", f3), ("fovy", C.c_float),
-                ("aspect", C.c_float), ("region", C.c_float * 4), ("shutter", C.c_float * 2)]
+                ("aspect", C.c_float), ("region", C.c_float * 4), ("shutter", C.c_float * 2),
+                ("aperture_radius", C.c_float), ("focus_distance", C.c_float)]
 
 
 class OrcParams(C.Structure):
@@ -115,6 +116,8 @@ def lib():
         L.orc_cvt_uint32_srgb.restype = C.c_uint32
         L.orc_cvt_uint32_srgb.argtypes = [vp]
         L.orc_camera_ray.argtypes = [C.POINTER(OrcCamera), C.c_float, C.c_float, ip, ip, vp, vp]
+        L.orc_camera_lens_ray.argtypes = [C.POINTER(OrcCamera), C.c_float, C.c_float, ip, ip,
+                                          C.c_float, C.c_float, vp, vp]
         L.orc_mesh_intersect.argtypes = [C.POINTER(OrcTrimesh), ip, vp, ip, ip, vp, vp]
         L.orc_scene_intersect_surfaces.argtypes = [C.POINTER(OrcScene), ip, vp, ip, vp, vp, vp]
         L.orc_instance_transforms.argtypes = [vp, vp, vp, vp]
@@ -441,7 +444,8 @@ def light_directional(direction, intensity=1.0, color=(1, 1, 1), visible=True):
     return L
 
 
-def camera(position, direction, up, fovy, aspect, region=(0, 0, 1, 1), shutter=(0.5, 0.5)):
+def camera(position, direction, up, fovy, aspect, region=(0, 0, 1, 1), shutter=(0.5, 0.5),
+           aperture_radius=0.0, focus_distance=1.0):
     c = OrcCamera()
     c.pos = f3(*position)
     c.dir = f3(*direction)
@@ -450,6 +454,8 @@ def camera(position, direction, up, fovy, aspect, region=(0, 0, 1, 1), shutter=(
     c.aspect = aspect
     c.region = (C.c_float * 4)(*region)
     c.shutter = (C.c_float * 2)(*shutter)
+    c.aperture_radius = aperture_radius
+    c.focus_distance = focus_distance
     return c
 
 

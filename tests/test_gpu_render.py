@@ -273,6 +273,54 @@ def test_clip_planes_parity(pkg, oracle, gpu_ctx):
         assert agree >= oracle.STREAM_AGREEMENT_MIN, agree
 
 
+def test_thin_lens_parity(pkg, oracle, gpu_ctx):
+    """A thin-lens camera (apertureRadius / focusDistance): raycast within
+    1e-3 of the oracle and unlike the pinhole image, path tracer
+    stream-aligned with the oracle (the lens sample drawn before the shutter
+    draw)."""
+    import dataclasses
+    wl = pkg.workloads.blobs_raycast(n=64, width=128, height=96)
+    lens = {"aperture_radius": 0.08, "focus_distance": 1.6}
+    wl_lens = dataclasses.replace(wl, camera={**wl.camera, **lens})
+    g, o = _raycast_pair(pkg, oracle, gpu_ctx, wl_lens)
+    assert np.max(np.abs(g - o)) <= 1e-3
+    pin, _ = _raycast_pair(pkg, oracle, gpu_ctx, wl)
+    assert np.max(np.abs(pin - g)) > 0.05  # the lens blurs the volume out of focus
+
+    wl = _pt_workload(pkg)
+    wl = dataclasses.replace(wl, camera={**wl.camera, **lens})
+    gs = pkg.workloads.GpuScene(gpu_ctx, wl)
+    ow = oracle.OracleWorkload(wl)
+    for accum_id in range(2):
+        gs.frame.reset()
+        gs.render(accum_id)
+        gpu_ctx.sync()
+        g = gs.frame.color_array()
+        o = (np.float32(1) / np.float32(accum_id + 1)) * ow.pathtrace_frame(accum_id)[0]
+        agree = oracle.stream_agreement(g, o)
+        print("thin lens stream agreement", accum_id, agree)
+        assert agree >= oracle.STREAM_AGREEMENT_MIN, agree
+
+
+def test_camera_lens_refused(pkg, gpu_ctx):
+    """Invalid lens parameters, and a lens with the radiance cache (whose
+    kernels compile the lens out), are errors."""
+    import dataclasses
+    wl = _pt_workload(pkg, n=16, w=16, h=16)
+    for bad in ({"aperture_radius": -0.1}, {"aperture_radius": 0.1, "focus_distance": 0.0}):
+        gs = pkg.workloads.GpuScene(gpu_ctx, dataclasses.replace(wl, camera={**wl.camera, **bad}))
+        with pytest.raises(pkg.vrt.VrtError, match="aperture_radius|focus_distance"):
+            gs.render(0)
+    gs = pkg.workloads.GpuScene(gpu_ctx, dataclasses.replace(
+        wl, camera={**wl.camera, "aperture_radius": 0.05, "focus_distance": 2.0}))
+    nrc = pkg.vrt.Nrc(gpu_ctx, max_batch=16384)
+    try:
+        with pytest.raises(pkg.vrt.VrtError, match="thin-lens"):
+            gs.render_nrc(nrc, gs.nrc_options(train=False), 0)
+    finally:
+        nrc.close()
+
+
 @pytest.mark.parametrize("field", ["fbm", "tets"])
 def test_pathtrace_ratio_tracking_parity(pkg, oracle, gpu_ctx, field):
     """Ratio-tracked shadow and AO rays (the ratio_tracking extension): the GPU
