@@ -163,6 +163,9 @@ def _anari_scene(a, wl, xfm=None, bvh_builder=None):
     a.set(cam, "up", A.FLOAT32_VEC3, c["up"])
     a.set(cam, "fovy", A.FLOAT32, c["fovy"])
     a.set(cam, "aspect", A.FLOAT32, c["aspect"])
+    if "aperture_radius" in c:
+        a.set(cam, "apertureRadius", A.FLOAT32, c["aperture_radius"])
+        a.set(cam, "focusDistance", A.FLOAT32, c["focus_distance"])
     ren = a.new(A.RENDERER, wl.renderer)
     p = wl.params
     a.set(ren, "background", A.FLOAT32_VEC4, p.get("background", (0, 0, 0, 1)))
@@ -204,6 +207,11 @@ def _vrt_frames(pkg, ctx, wl, frames):
     return out
 
 
+def _lens(wl):
+    return dataclasses.replace(wl, camera={**wl.camera, "aperture_radius": 0.06,
+                                           "focus_distance": 1.8})
+
+
 CASES = {
     "structured pathtrace": lambda w: w.fbm_pathtrace(n=32, width=64, height=48, frames=1,
                                                       brick=8, phase_g=0.3),
@@ -235,6 +243,9 @@ CASES = {
     "instanced surfaces pathtrace": lambda w: w.instanced_pathtrace(width=64, height=48),
     "instanced surfaces raycast": lambda w: w.instanced_pathtrace(width=64, height=48,
                                                                   renderer="raycast"),
+    "thin-lens pathtrace": lambda w: _lens(w.fbm_pathtrace(n=32, width=64, height=48, frames=1,
+                                                           brick=8)),
+    "thin-lens raycast": lambda w: _lens(w.blobs_raycast(n=48, width=64, height=48)),
 }
 
 
