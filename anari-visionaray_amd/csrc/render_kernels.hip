@@ -198,6 +198,13 @@ __device__ inline float pow_ieee(float x, float y, const float2 *tab) {
   const uint32_t bits = __float_as_uint(x);
   if (bits < 0x00800000u) return y > 0.f ? 0.f : __builtin_inff();  // subnormal base
   if (bits >= 0x7f800000u) return x;                                 // inf
+  // the exponents of the usual sampling rates (2 and 0.5 samples per unit
+  // distance) take one correctly rounded operation
+  if (y == 0.5f) return sqrtf(x);
+  if (y == 2.f) {
+    const float s = x * x;
+    return s < 1.17549435e-38f ? 0.f : s;
+  }
   const uint32_t tmp = bits - 0x3f330000u;
   const int e = int(tmp) >> 23;
   const float2 t = tab[(tmp >> 17) & 63u];

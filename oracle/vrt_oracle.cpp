@@ -193,6 +193,13 @@ inline float pow_ieee(float x, float y) {
   std::memcpy(&bits, &x, 4);
   if (bits < 0x00800000u) return y > 0.f ? 0.f : std::numeric_limits<float>::infinity();
   if (bits >= 0x7f800000u) return x;
+  // exponents 0.5 and 2 (the usual sampling rates): one correctly rounded
+  // operation each
+  if (y == 0.5f) return std::sqrt(x);
+  if (y == 2.f) {
+    const float s = x * x;
+    return s < std::numeric_limits<float>::min() ? 0.f : s;
+  }
   const uint32_t tmp = bits - 0x3f330000u;
   const int e = int32_t(tmp) >> 23;
   const float *t = kPowLog2[(tmp >> 17) & 63u];

@@ -297,6 +297,12 @@ def test_pow_ieee_error_bound(oracle):
     assert np.all(ulp[ok] <= 1.5 * (2 + z[ok]))
     assert L.orc_pow_ieee(0.0, 0.5) == 0.0 and L.orc_pow_ieee(1.0, 3.0) == 1.0
     assert L.orc_pow_ieee(0.25, 0.5) == 0.5
+    # exponents 0.5 and 2 are the correctly rounded square root and square
+    xs = rng.random(5000).astype(np.float32) + np.float32(1e-6)
+    half = np.array([L.orc_pow_ieee(float(x), 0.5) for x in xs], np.float32)
+    np.testing.assert_array_equal(half, np.sqrt(xs.astype(np.float64)).astype(np.float32))
+    two = np.array([L.orc_pow_ieee(float(x), 2.0) for x in xs], np.float32)
+    np.testing.assert_array_equal(two, (xs.astype(np.float64) ** 2).astype(np.float32))
     # x near 1 (the opacity correction of small opacities) keeps its digits
     xs = (1.0 - np.geomspace(1e-7, 1e-2, 2000)).astype(np.float32)
     got = np.array([L.orc_pow_ieee(float(x), 0.5) for x in xs], np.float64)
