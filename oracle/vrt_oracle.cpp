@@ -571,7 +571,8 @@ template <class R> struct Volume {
       i0[a] = std::min(std::max(i, 0), n[a] - 1);
       i1[a] = std::min(std::max(i + 1, 0), n[a] - 1);
     }
-    auto lerp = [](R a, R b, R t) { return a + t * (b - a); };
+    // a + t (b - a) with one rounding for the product and sum (fused)
+    auto lerp = [](R a, R b, R t) { return std::fma(t, b - a, a); };
     R c00 = lerp(voxel(i0[0], i0[1], i0[2]), voxel(i1[0], i0[1], i0[2]), w[0]);
     R c10 = lerp(voxel(i0[0], i1[1], i0[2]), voxel(i1[0], i1[1], i0[2]), w[0]);
     R c01 = lerp(voxel(i0[0], i0[1], i1[2]), voxel(i1[0], i0[1], i1[2]), w[0]);
